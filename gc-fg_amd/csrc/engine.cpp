@@ -1,0 +1,346 @@
+// Host side of the solver engine: arena allocation, instance table, step
+// loop. Replaces the host loop of GraphCut.cu:385-534 (which synchronised
+// with the device every 10 iterations and after every GlobalRelabel and
+// SpreadLabels launch) by chunks of device-scheduled steps and one pinned
+// poll per chunk.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "engine.hpp"
+
+namespace gcfg {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+const char *last_error_cstr() { return g_last_error.c_str(); }
+
+int hip_fail(hipError_t e, const char *what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return GCFG_ERR_HIP;
+}
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+Engine::~Engine() {
+    if (stream_) hipStreamSynchronize(stream_);
+    if (graph_exec_) hipGraphExecDestroy(graph_exec_);
+    if (graph_) hipGraphDestroy(graph_);
+    for (void *p : allocs_) hipFree(p);
+    if (pin_desc_) hipHostFree(pin_desc_);
+    if (pin_slots_) hipHostFree(pin_slots_);
+    if (pin_ctl_) hipHostFree(pin_ctl_);
+    if (stream_) hipStreamDestroy(stream_);
+}
+
+int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
+    opt = o;
+    if (opt.sweeps_per_gr <= 0 || opt.inner_iters <= 0 || opt.steps_per_poll <= 0) {
+        set_error("options: sweeps_per_gr, inner_iters and steps_per_poll must be positive");
+        return GCFG_ERR_INVALID;
+    }
+    cap_tiles = std::max(cap_tiles, 1);
+    cap_slots = std::max(cap_slots, 1);
+    cap_tiles_ = cap_tiles;
+    cap_slots_ = cap_slots;
+    GCFG_HIP(hipGetDevice(&dev_));
+    GCFG_HIP(hipDeviceGetAttribute(&cu_count_, hipDeviceAttributeMultiprocessorCount, dev_));
+    GCFG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    auto dalloc = [&](void **p, size_t bytes) -> int {
+        hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+            return GCFG_ERR_NOMEM;
+        }
+        allocs_.push_back(*p);
+        return GCFG_OK;
+    };
+    const size_t nodes = (size_t)cap_tiles * TN;
+    int rc = GCFG_OK;
+    rc |= dalloc((void **)&kp_.caps, nodes * sizeof(int4));
+    rc |= dalloc((void **)&kp_.exc, nodes * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.dist, nodes * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.mask, nodes);
+    rc |= dalloc((void **)&kp_.tiles, (size_t)cap_tiles * sizeof(TileInfo));
+    rc |= dalloc((void **)&kp_.tile_cnt, (size_t)cap_tiles * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.act_flag, (size_t)cap_tiles * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.dirty, (size_t)cap_tiles * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.queue_pr, (size_t)cap_tiles * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.queue_gr, (size_t)cap_tiles * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.desc, (size_t)cap_slots * sizeof(InstDesc));
+    rc |= dalloc((void **)&kp_.state, (size_t)cap_slots * sizeof(InstState));
+    rc |= dalloc((void **)&kp_.ctl, sizeof(Ctl));
+    rc |= dalloc((void **)&d_slots_, (size_t)(2 * cap_slots + 1) * sizeof(int32_t));
+    if (rc != GCFG_OK) return GCFG_ERR_NOMEM;
+    GCFG_HIP(hipHostMalloc((void **)&pin_desc_, (size_t)cap_slots * sizeof(InstDesc), hipHostMallocDefault));
+    GCFG_HIP(hipHostMalloc((void **)&pin_slots_, (size_t)(2 * cap_slots + 1) * sizeof(int32_t), hipHostMallocDefault));
+    GCFG_HIP(hipHostMalloc((void **)&pin_ctl_, sizeof(Ctl), hipHostMallocDefault));
+    GCFG_HIP(hipMemsetAsync(kp_.state, 0, (size_t)cap_slots * sizeof(InstState), stream_));
+    GCFG_HIP(hipMemsetAsync(kp_.desc, 0, (size_t)cap_slots * sizeof(InstDesc), stream_));
+    GCFG_HIP(hipMemsetAsync(kp_.ctl, 0, sizeof(Ctl), stream_));
+    GCFG_HIP(hipMemsetAsync(kp_.dirty, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
+    GCFG_HIP(hipMemsetAsync(kp_.act_flag, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    kp_.n_slots = cap_slots;
+    kp_.inner_iters = opt.inner_iters;
+    grid_pr_ = std::max(cu_count_, 1) * 4;
+    grid_gr_ = std::max(cu_count_, 1) * 8;
+    h_desc_.assign(cap_slots, InstDesc{});
+    slot_used_.assign(cap_slots, 0);
+    free_ranges_.clear();
+    free_ranges_.push_back({0, cap_tiles});
+    std::memset(pin_ctl_, 0, sizeof(Ctl));
+    return GCFG_OK;
+}
+
+size_t Engine::tiles_free() const {
+    size_t f = 0;
+    for (auto &r : free_ranges_) f += r.second;
+    return f;
+}
+
+int Engine::alloc_tiles(int n) {
+    for (size_t i = 0; i < free_ranges_.size(); ++i) {
+        if (free_ranges_[i].second >= n) {
+            const int start = free_ranges_[i].first;
+            free_ranges_[i].first += n;
+            free_ranges_[i].second -= n;
+            if (free_ranges_[i].second == 0) free_ranges_.erase(free_ranges_.begin() + i);
+            return start;
+        }
+    }
+    return -1;
+}
+
+void Engine::free_tiles(int start, int n) {
+    free_ranges_.push_back({start, n});
+    std::sort(free_ranges_.begin(), free_ranges_.end());
+    std::vector<std::pair<int, int>> merged;
+    for (auto &r : free_ranges_) {
+        if (!merged.empty() && merged.back().first + merged.back().second == r.first)
+            merged.back().second += r.second;
+        else
+            merged.push_back(r);
+    }
+    free_ranges_.swap(merged);
+}
+
+int Engine::add_instance(int W, int H, const int32_t *pos, const int32_t *neg, const int32_t *up,
+                         const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
+                         int32_t *label_out) {
+    if (W <= 0 || H <= 0) {
+        set_error("instance shape must be positive");
+        return GCFG_ERR_INVALID;
+    }
+    int slot = -1;
+    for (int i = 0; i < cap_slots_; ++i)
+        if (!slot_used_[i]) {
+            slot = i;
+            break;
+        }
+    if (slot < 0) {
+        set_error("no free instance slot");
+        return GCFG_ERR_BUDGET;
+    }
+    const int tx = (W + TW - 1) / TW, ty = (H + TH - 1) / TH;
+    const int start = alloc_tiles(tx * ty);
+    if (start < 0) {
+        set_error("arena has no free tile range of the requested size");
+        return GCFG_ERR_BUDGET;
+    }
+    slot_used_[slot] = 1;
+    InstDesc &d = h_desc_[slot];
+    d = InstDesc{};
+    d.W = W;
+    d.H = H;
+    d.tiles_x = tx;
+    d.tiles_y = ty;
+    d.tile0 = start;
+    d.ntiles = tx * ty;
+    d.n_interior = (W > 2 && H > 2) ? (W - 2) * (H - 2) : 0;
+    set_inputs(slot, pos, neg, up, down, left, right, penalty, label_out);
+    return slot;
+}
+
+int Engine::set_inputs(int slot, const int32_t *pos, const int32_t *neg, const int32_t *up, const int32_t *down,
+                       const int32_t *left, const int32_t *right, int32_t penalty, int32_t *label_out) {
+    InstDesc &d = h_desc_[slot];
+    d.pos = pos;
+    d.neg = neg;
+    d.up = up;
+    d.down = up ? down : nullptr;
+    d.left = up ? left : nullptr;
+    d.right = up ? right : nullptr;
+    d.penalty = penalty;
+    d.label_out = label_out;
+    return GCFG_OK;
+}
+
+void Engine::remove_instance(int slot) {
+    if (slot < 0 || slot >= cap_slots_ || !slot_used_[slot]) return;
+    free_tiles(h_desc_[slot].tile0, h_desc_[slot].ntiles);
+    slot_used_[slot] = 0;
+    h_desc_[slot] = InstDesc{};
+}
+
+int Engine::build(const std::vector<int> &slots) {
+    if (slots.empty()) return GCFG_OK;
+    for (int s : slots) {
+        const InstDesc &d = h_desc_[s];
+        if (!d.pos || !d.neg) {
+            set_error("data terms not set");
+            return GCFG_ERR_STATE;
+        }
+        if (!d.label_out) {
+            set_error("label buffer not set");
+            return GCFG_ERR_STATE;
+        }
+    }
+    // The pinned staging buffers are reused; wait for earlier copies out of them.
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(pin_desc_, h_desc_.data(), (size_t)cap_slots_ * sizeof(InstDesc));
+    const int n = (int)slots.size();
+    int total = 0;
+    for (int i = 0; i < n; ++i) {
+        pin_slots_[i] = slots[i];
+        pin_slots_[n + i] = total;
+        total += h_desc_[slots[i]].ntiles;
+    }
+    GCFG_HIP(hipMemcpyAsync(kp_.desc, pin_desc_, (size_t)cap_slots_ * sizeof(InstDesc), hipMemcpyHostToDevice,
+                            stream_));
+    GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)(2 * n) * sizeof(int32_t), hipMemcpyHostToDevice,
+                            stream_));
+    launch_arm(kp_, d_slots_, n, 0, opt.sweeps_per_gr, stream_);
+    launch_build(kp_, d_slots_, n, total, stream_);
+    GCFG_HIP(hipGetLastError());
+    return GCFG_OK;
+}
+
+int Engine::rearm(const std::vector<int> &slots) {
+    if (slots.empty()) return GCFG_OK;
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    const int n = (int)slots.size();
+    for (int i = 0; i < n; ++i) pin_slots_[i] = slots[i];
+    GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+    launch_arm(kp_, d_slots_, n, 1, opt.sweeps_per_gr, stream_);
+    GCFG_HIP(hipGetLastError());
+    return GCFG_OK;
+}
+
+int Engine::edit(int slot, const std::vector<int32_t> &strokes, int32_t delta) {
+    const int n = (int)strokes.size() / 5;
+    if (n <= 0) return GCFG_OK;
+    if ((int)strokes.size() > strokes_cap_) {
+        GCFG_HIP(hipStreamSynchronize(stream_));
+        if (d_strokes_) {
+            hipFree(d_strokes_);
+            allocs_.erase(std::remove(allocs_.begin(), allocs_.end(), (void *)d_strokes_), allocs_.end());
+        }
+        strokes_cap_ = std::max<int>((int)strokes.size(), 5 * 64);
+        GCFG_HIP(hipMalloc((void **)&d_strokes_, (size_t)strokes_cap_ * sizeof(int32_t)));
+        allocs_.push_back(d_strokes_);
+    }
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(pin_slots_, strokes.data(), std::min<size_t>(strokes.size(), (size_t)(2 * cap_slots_ + 1)) *
+                                                 sizeof(int32_t));
+    if (strokes.size() <= (size_t)(2 * cap_slots_ + 1)) {
+        GCFG_HIP(hipMemcpyAsync(d_strokes_, pin_slots_, strokes.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                                stream_));
+    } else {
+        GCFG_HIP(hipMemcpy(d_strokes_, strokes.data(), strokes.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    launch_edit(kp_, slot, d_strokes_, n, delta, stream_);
+    GCFG_HIP(hipGetLastError());
+    return GCFG_OK;
+}
+
+int Engine::capture_graph() {
+    if (graph_exec_ && graph_steps_ == opt.steps_per_poll) return GCFG_OK;
+    if (graph_exec_) {
+        hipGraphExecDestroy(graph_exec_);
+        graph_exec_ = nullptr;
+    }
+    if (graph_) {
+        hipGraphDestroy(graph_);
+        graph_ = nullptr;
+    }
+    GCFG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < opt.steps_per_poll; ++i) launch_step(kp_, grid_pr_, grid_gr_, stream_);
+    GCFG_HIP(hipStreamEndCapture(stream_, &graph_));
+    GCFG_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+    graph_steps_ = opt.steps_per_poll;
+    return GCFG_OK;
+}
+
+int Engine::launch_steps(int n) {
+    if (opt.use_graph && n == opt.steps_per_poll) {
+        int rc = capture_graph();
+        if (rc != GCFG_OK) return rc;
+        GCFG_HIP(hipGraphLaunch(graph_exec_, stream_));
+    } else {
+        for (int i = 0; i < n; ++i) launch_step(kp_, grid_pr_, grid_gr_, stream_);
+    }
+    GCFG_HIP(hipGetLastError());
+    steps_run += n;
+    return GCFG_OK;
+}
+
+int Engine::run(int64_t max_steps) {
+    const int64_t start = steps_run;
+    for (;;) {
+        int rc = launch_steps(opt.steps_per_poll);
+        if (rc != GCFG_OK) return rc;
+        GCFG_HIP(hipMemcpyAsync(pin_ctl_, kp_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+        GCFG_HIP(hipStreamSynchronize(stream_));
+        if (pin_ctl_->n_live <= 0) return GCFG_OK;
+        if (max_steps > 0 && steps_run - start > max_steps) {
+            set_error("solver did not finish within max_steps");
+            return GCFG_ERR_NOT_CONVERGED;
+        }
+    }
+}
+
+int Engine::pump(int steps) { return launch_steps(steps); }
+
+int Engine::live_count(int32_t *out) {
+    GCFG_HIP(hipMemcpyAsync(pin_ctl_, kp_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    *out = pin_ctl_->n_live;
+    return GCFG_OK;
+}
+
+int Engine::fetch_state(int slot, InstState *out) {
+    GCFG_HIP(hipMemcpyAsync(out, kp_.state + slot, sizeof(InstState), hipMemcpyDeviceToHost, stream_));
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    return GCFG_OK;
+}
+
+int Engine::read_state(int slot, int32_t *h_caps, int32_t *h_exc) {
+    const InstDesc &d = h_desc_[slot];
+    const size_t n = (size_t)d.W * d.H;
+    int32_t *dc = nullptr, *de = nullptr;
+    GCFG_HIP(hipMalloc((void **)&dc, n * 4 * sizeof(int32_t)));
+    GCFG_HIP(hipMalloc((void **)&de, n * sizeof(int32_t)));
+    launch_read_state(kp_, slot, dc, de, stream_);
+    hipError_t e1 = hipMemcpyAsync(h_caps, dc, n * 4 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
+    hipError_t e2 = hipMemcpyAsync(h_exc, de, n * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
+    hipError_t e3 = hipStreamSynchronize(stream_);
+    hipFree(dc);
+    hipFree(de);
+    GCFG_HIP(e1);
+    GCFG_HIP(e2);
+    GCFG_HIP(e3);
+    return GCFG_OK;
+}
+
+int Engine::sync() {
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    return GCFG_OK;
+}
+
+}  // namespace gcfg

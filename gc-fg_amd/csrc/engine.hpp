@@ -1,0 +1,177 @@
+// Engine shared by the plain (GC_*), batched and serving paths.
+//
+// Device layout ("arena"): every instance (one image's graph) owns a
+// contiguous range of TILES; a tile is TW x TH nodes stored contiguously
+// (tile-major), row-major inside the tile. Per node: residual capacities
+// int4 {left, right, up, down} (16 B), excess int32 (4 B), distance label
+// int32 (4 B) and a residual mask byte (1 B, global-relabel only).
+// Excess < 0 means remaining sink capacity (t-links are pre-saturated,
+// GraphCutKernels.cu:196-197). Pixels outside the interior are inert nodes
+// with zero capacities (GraphCutKernels.cu:122-146).
+//
+// Control: a scheduler kernel advances one state machine per instance
+// (START -> global relabel -> push/relabel colour passes -> ... -> FIN ->
+// DONE) and emits compacted tile queues; persistent push/relabel and
+// global-relabel kernels consume the queues. A "step" is the three launches
+// sched -> pr -> gr; the host replays chunks of steps (optionally as a
+// hipGraph) and polls a pinned live-instance counter.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "gcfg.h"
+
+namespace gcfg {
+
+constexpr int TW = 64;                  // tile width (one wavefront row)
+constexpr int TH = 16;                  // tile height
+constexpr int TN = TW * TH;             // nodes per tile
+constexpr int NT = 256;                 // threads per workgroup (4 wavefronts)
+constexpr int RPT = TN / NT;            // rows per thread (vertical strip)
+constexpr int DINF = 0x3fffffff;        // distance of nodes that cannot reach the sink
+constexpr int LW = TW + 2;              // LDS row pitch with halo
+constexpr int LH = TH + 2;
+
+enum Phase : int32_t { PH_IDLE = 0, PH_START = 1, PH_GR = 2, PH_PR = 3, PH_FIN = 4, PH_DONE = 5 };
+enum GrOp : int32_t { OP_GR_INIT = 1, OP_GR_RELAX = 2, OP_FIN = 3 };
+
+struct TileInfo {
+    int32_t inst;
+    int32_t tx, ty;
+    int32_t nbr[4];  // left, right, up, down tile ids (-1: none)
+    int32_t pad;
+};
+
+struct InstDesc {
+    int32_t W, H, tiles_x, tiles_y;
+    int32_t tile0, ntiles, n_interior, penalty;
+    const int32_t *pos, *neg, *up, *down, *left, *right;
+    int32_t *label_out;  // device W*H
+};
+
+struct InstState {
+    int32_t phase;
+    int32_t pr_iter;
+    int32_t gr_rounds;
+    int32_t steps;
+    int32_t active_nodes;  // nodes with e>0 and finite distance (valid after a converged GR)
+    int32_t gr_dirty;      // tiles flagged for the next GR sweep
+    int32_t act_tiles[2];  // tiles with active nodes, per checkerboard colour
+    int64_t sum_pos;       // interior sum of pos (current t-links)
+    int64_t sum_exc;       // interior sum of max(e, 0) at FIN
+    int64_t flow;
+    int64_t pr_tiles, gr_tiles, pr_iters;
+    int32_t err;
+    int32_t sweeps;        // colour-pass pairs between global relabels
+};
+
+struct Ctl {
+    int32_t qlen_pr, head_pr, qlen_gr, head_gr;
+    int32_t gr_ticket, n_live, steps, pad;
+};
+
+struct KParams {
+    // arena
+    int4 *caps;
+    int32_t *exc, *dist, *tile_cnt, *act_flag, *dirty;
+    uint8_t *mask;
+    TileInfo *tiles;
+    InstDesc *desc;
+    InstState *state;
+    Ctl *ctl;
+    int32_t *queue_pr, *queue_gr;
+    int32_t n_slots;
+    int32_t inner_iters;
+};
+
+// Kernel launchers (solver.hip).
+void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int sweeps, hipStream_t s);
+void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_tiles, hipStream_t s);
+void launch_step(const KParams &p, int grid_pr, int grid_gr, hipStream_t s);
+void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,
+                 hipStream_t s);
+void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, hipStream_t s);
+
+void set_error(const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define GCFG_HIP(call)                                          \
+    do {                                                        \
+        hipError_t _e = (call);                                 \
+        if (_e != hipSuccess) return ::gcfg::hip_fail(_e, #call); \
+    } while (0)
+
+// One engine = one arena on one device + one stream.
+class Engine {
+   public:
+    Engine() = default;
+    ~Engine();
+    Engine(const Engine &) = delete;
+    Engine &operator=(const Engine &) = delete;
+
+    // Capacity in tiles and instance slots.
+    int init(int cap_tiles, int cap_slots, const gcfg_options &opt);
+    // Reserve a tile range and a slot for a W x H instance; returns slot or < 0.
+    int add_instance(int W, int H, const int32_t *pos, const int32_t *neg, const int32_t *up,
+                     const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
+                     int32_t *label_out);
+    void remove_instance(int slot);
+    int set_inputs(int slot, const int32_t *pos, const int32_t *neg, const int32_t *up, const int32_t *down,
+                   const int32_t *left, const int32_t *right, int32_t penalty, int32_t *label_out);
+    // Upload descriptors, re-initialise state and build the residual graphs.
+    int build(const std::vector<int> &slots);
+    // Re-arm instances from their current residual graph (warm start).
+    int rearm(const std::vector<int> &slots);
+    // Run steps until every armed instance is done.
+    int run(int64_t max_steps);
+    // Launch up to `steps` steps without waiting; returns live count seen at
+    // the last completed poll via live_out (non-blocking).
+    int pump(int steps);
+    int live_count(int32_t *out);
+    int edit(int slot, const std::vector<int32_t> &strokes, int32_t delta);
+    int read_state(int slot, int32_t *h_caps, int32_t *h_exc);
+    int fetch_state(int slot, InstState *out);
+    int sync();
+
+    hipStream_t stream() const { return stream_; }
+    const InstDesc &desc(int slot) const { return h_desc_[slot]; }
+    size_t tiles_free() const;
+    int tiles_for(int W, int H) const { return ((W + TW - 1) / TW) * ((H + TH - 1) / TH); }
+    static size_t bytes_per_tile() { return (size_t)TN * (16 + 4 + 4 + 1) + 64; }
+    gcfg_options opt;
+    int steps_run = 0;
+
+   private:
+    int alloc_tiles(int n);
+    void free_tiles(int start, int n);
+    int launch_steps(int n);
+    int capture_graph();
+
+    int dev_ = 0;
+    int cu_count_ = 0;
+    hipStream_t stream_ = nullptr;
+    KParams kp_{};
+    int cap_tiles_ = 0, cap_slots_ = 0;
+    int grid_pr_ = 0, grid_gr_ = 0;
+    std::vector<InstDesc> h_desc_;
+    std::vector<char> slot_used_;
+    std::vector<std::pair<int, int>> free_ranges_;  // (start, count)
+    InstDesc *pin_desc_ = nullptr;
+    int32_t *pin_slots_ = nullptr;
+    int32_t *d_slots_ = nullptr;
+    Ctl *pin_ctl_ = nullptr;
+    int32_t *d_strokes_ = nullptr;
+    int strokes_cap_ = 0;
+    hipGraphExec_t graph_exec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    int graph_steps_ = 0;
+    std::vector<void *> allocs_;
+};
+
+double now_ms();
+
+}  // namespace gcfg

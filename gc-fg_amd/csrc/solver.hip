@@ -1,0 +1,726 @@
+// CDNA4 kernels of the exact push-relabel min-cut solver.
+//
+// Replaces GraphCutKernels.cu (InitGraph*/Relabel/WavePush/UpdateActivity/
+// InitGlobalRelabel/GlobalRelabel/InitLabels/SpreadLabels/EnergyLevel,
+// :112-898) with a race-free design:
+//  * k_build: residual graph from caller inputs, border rule of
+//    GraphCutKernels.cu:122-146 / :179-203, t-links pre-saturated
+//    (excess = pos - neg, :197), interior sum of pos in int64.
+//  * k_pr: one 64x16 tile per workgroup, state in registers + LDS, up to
+//    inner_iters synchronous iterations of {push along admissible arcs,
+//    receive, relabel}; tiles run in a 2-colour checkerboard so the halo
+//    heights a tile reads are stable for the whole pass (every pass is a
+//    legal sequential push-relabel schedule, so the labelling stays valid).
+//    Flow into halo nodes is applied with int32 atomics at the end.
+//  * k_gr: exact BFS distance to the sink-connected nodes (excess < 0) by
+//    tile-local relaxation to a fixed point in LDS, tiles re-queued through
+//    dirty flags until no boundary changes (replaces the approximate,
+//    host-polled GlobalRelabel of GraphCutKernels.cu:757-875), plus FIN:
+//    labels (interior and unreachable, the SPREAD_ZEROS set of :228-351)
+//    and the int64 stuck-excess sum for flow = sum pos - sum max(e, 0).
+//  * k_sched: per-instance state machine and ballot-compacted tile queues.
+#include "engine.hpp"
+
+namespace gcfg {
+
+constexpr int GR_MAX_ITERS = 4 * (TW + TH);
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int lo = __shfl_xor((int)(uint32_t)(uint64_t)v, o, 64);
+        const int hi = __shfl_xor((int)(uint32_t)((uint64_t)v >> 32), o, 64);
+        v += (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    }
+    return v;
+}
+
+// Block (NT threads) sum; result valid in every thread. Uses red[NT/64].
+__device__ __forceinline__ int64_t block_sum64(int64_t v, int64_t *red) {
+    v = wave_sum64(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    int64_t s = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) s += red[w];
+    return s;
+}
+
+__device__ __forceinline__ void atomic_add64(int64_t *p, int64_t v) {
+    atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v);
+}
+
+__device__ __forceinline__ void set_active(const KParams &p, int t, int inst, int colour, int v) {
+    const int old = atomicExch(&p.act_flag[t], v);
+    if (old != v) atomicAdd(&p.state[inst].act_tiles[colour], v ? 1 : -1);
+}
+
+__device__ __forceinline__ void mark_dirty(const KParams &p, int t, int inst) {
+    if (atomicExch(&p.dirty[t], 1) == 0) atomicAdd(&p.state[inst].gr_dirty, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Arm: reset the per-instance state and mark it live.
+__global__ void k_arm(KParams p, const int32_t *slots, int n, int warm, int sweeps) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = slots[i];
+    InstState &st = p.state[s];
+    const int64_t sp = warm ? st.sum_pos : 0;
+    InstState z{};
+    z.sum_pos = sp;
+    z.phase = PH_START;
+    z.sweeps = sweeps;
+    st = z;
+    atomicAdd(&p.ctl->n_live, 1);
+}
+
+// ---------------------------------------------------------------------------
+// Build (GC_SetGraph): one tile per loop trip over the listed instances.
+__global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, const int32_t *prefix, int n,
+                                              int total) {
+    __shared__ int64_t red[NT / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int b = blockIdx.x; b < total; b += gridDim.x) {
+        int lo = 0, hi = n - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
+        }
+        const int s = slots[lo];
+        const InstDesc D = p.desc[s];
+        const int lt = b - prefix[lo];
+        const int t = D.tile0 + lt;
+        const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
+        if (threadIdx.x == 0) {
+            TileInfo ti;
+            ti.inst = s;
+            ti.tx = tx;
+            ti.ty = ty;
+            ti.nbr[0] = tx > 0 ? t - 1 : -1;
+            ti.nbr[1] = tx < D.tiles_x - 1 ? t + 1 : -1;
+            ti.nbr[2] = ty > 0 ? t - D.tiles_x : -1;
+            ti.nbr[3] = ty < D.tiles_y - 1 ? t + D.tiles_x : -1;
+            ti.pad = 0;
+            p.tiles[t] = ti;
+            p.act_flag[t] = 0;
+            p.dirty[t] = 0;
+            p.tile_cnt[t] = 0;
+        }
+        int64_t sp = 0;
+        int bad = 0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int y = wave * RPT + k, x = lane;
+            const int X = tx * TW + x, Y = ty * TH + y;
+            const size_t g = (size_t)t * TN + y * TW + x;
+            int4 c = make_int4(0, 0, 0, 0);
+            int e = 0;
+            if (X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1) {
+                const size_t i = (size_t)Y * D.W + X;
+                const int pp = D.pos[i], qq = D.neg[i];
+                if (D.up) {
+                    c.x = D.left[i];
+                    c.y = D.right[i];
+                    c.z = D.up[i];
+                    c.w = D.down[i];
+                } else {
+                    c = make_int4(D.penalty, D.penalty, D.penalty, D.penalty);
+                }
+                if (X == 1) c.x = 0;
+                if (X == D.W - 2) c.y = 0;
+                if (Y == 1) c.z = 0;
+                if (Y == D.H - 2) c.w = 0;
+                bad |= (pp < 0) | (qq < 0) | (c.x < 0) | (c.y < 0) | (c.z < 0) | (c.w < 0);
+                e = pp - qq;
+                sp += pp;
+            }
+            p.caps[g] = c;
+            p.exc[g] = e;
+            p.dist[g] = DINF;
+            p.mask[g] = 0;
+        }
+        const int64_t tot = block_sum64(sp, red);
+        const int anybad = __syncthreads_or(bad);
+        if (threadIdx.x == 0) {
+            if (tot) atomic_add64(&p.state[s].sum_pos, tot);
+            if (anybad) atomicOr(&p.state[s].err, 1);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Scheduler: one workgroup per instance slot.
+enum SchedOp : int { SO_NONE = 0, SO_GR_INIT = 1, SO_GR_DIRTY = 2, SO_FIN = 3, SO_PR = 4 };
+
+__global__ __launch_bounds__(NT) void k_sched(KParams p) {
+    const int s = blockIdx.x;
+    __shared__ int s_op, s_colour, s_qbase;
+    __shared__ int wcount[NT / 64], wbase[NT / 64];
+    InstState &st = p.state[s];
+    if (threadIdx.x == 0) {
+        int op = SO_NONE, colour = 0;
+        int phase = st.phase;
+        if (phase != PH_IDLE && phase != PH_DONE) st.steps++;
+        if (phase == PH_START && st.err) {
+            phase = PH_DONE;
+            atomicSub(&p.ctl->n_live, 1);
+        }
+        for (int guard = 0; guard < 4; ++guard) {
+            if (phase == PH_START) {
+                st.active_nodes = 0;
+                st.act_tiles[0] = st.act_tiles[1] = 0;
+                st.gr_rounds++;
+                op = SO_GR_INIT;
+                phase = PH_GR;
+                break;
+            }
+            if (phase == PH_GR) {
+                if (st.gr_dirty > 0) {
+                    op = SO_GR_DIRTY;
+                    st.gr_dirty = 0;
+                    break;
+                }
+                if (st.active_nodes <= 0) {
+                    op = SO_FIN;
+                    phase = PH_FIN;
+                    break;
+                }
+                phase = PH_PR;
+                st.pr_iter = 0;
+                continue;
+            }
+            if (phase == PH_PR) {
+                const int a0 = st.act_tiles[0], a1 = st.act_tiles[1];
+                if (st.pr_iter >= 2 * st.sweeps || (a0 <= 0 && a1 <= 0)) {
+                    phase = PH_START;
+                    continue;
+                }
+                colour = st.pr_iter & 1;
+                if ((colour ? a1 : a0) <= 0) colour ^= 1;
+                op = SO_PR;
+                st.pr_iter++;
+                break;
+            }
+            if (phase == PH_FIN) {
+                st.flow = st.sum_pos - st.sum_exc;
+                phase = PH_DONE;
+                atomicSub(&p.ctl->n_live, 1);
+            }
+            break;
+        }
+        st.phase = phase;
+        s_op = op;
+        s_colour = colour;
+    }
+    __syncthreads();
+    const int op = s_op;
+    if (op == SO_NONE) return;
+    const int colour = s_colour;
+    const InstDesc D = p.desc[s];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int32_t *queue = op == SO_PR ? p.queue_pr : p.queue_gr;
+    int32_t *qlen = op == SO_PR ? &p.ctl->qlen_pr : &p.ctl->qlen_gr;
+    const int grop = op == SO_GR_INIT ? OP_GR_INIT : (op == SO_GR_DIRTY ? OP_GR_RELAX : OP_FIN);
+    for (int base = 0; base < D.ntiles; base += NT) {
+        const int lt = base + threadIdx.x;
+        const int t = D.tile0 + lt;
+        int pred = 0;
+        if (lt < D.ntiles) {
+            if (op == SO_GR_INIT || op == SO_FIN) {
+                pred = 1;
+            } else if (op == SO_GR_DIRTY) {
+                pred = p.dirty[t];
+                if (pred) p.dirty[t] = 0;
+            } else {
+                const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
+                pred = (((tx + ty) & 1) == colour) && p.act_flag[t];
+            }
+        }
+        const unsigned long long bal = __ballot(pred);
+        const int wpre = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wcount[wave] = __popcll(bal);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int tot = 0;
+            for (int w = 0; w < NT / 64; ++w) {
+                wbase[w] = tot;
+                tot += wcount[w];
+            }
+            s_qbase = tot ? atomicAdd(qlen, tot) : 0;
+        }
+        __syncthreads();
+        if (pred) queue[s_qbase + wbase[wave] + wpre] = op == SO_PR ? t : ((t << 3) | grop);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tile push-relabel pass.
+__global__ __launch_bounds__(NT) void k_pr(KParams p) {
+    __shared__ int Hs[2][LH][LW];
+    __shared__ int FLs[LH][LW], FRs[LH][LW], FUs[LH][LW], FDs[LH][LW];
+    __shared__ int s_item;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < LH * LW; i += NT) {
+        (&FLs[0][0])[i] = 0;
+        (&FRs[0][0])[i] = 0;
+        (&FUs[0][0])[i] = 0;
+        (&FDs[0][0])[i] = 0;
+    }
+    const int qlen = p.ctl->qlen_pr;
+    for (;;) {
+        if (tid == 0) {
+            const int q = atomicAdd(&p.ctl->head_pr, 1);
+            s_item = q < qlen ? p.queue_pr[q] : -1;
+        }
+        __syncthreads();
+        const int t = s_item;
+        if (t < 0) break;
+        const TileInfo ti = p.tiles[t];
+        const int dthr = p.desc[ti.inst].n_interior;
+        const size_t base = (size_t)t * TN;
+        const int lx = lane + 1;
+        int4 c[RPT];
+        int e[RPT], d[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int y = wave * RPT + k;
+            const size_t g = base + y * TW + lane;
+            c[k] = p.caps[g];
+            e[k] = p.exc[g];
+            d[k] = p.dist[g];
+            Hs[0][y + 1][lx] = d[k];
+            Hs[1][y + 1][lx] = d[k];
+        }
+        if (tid < TH) {
+            const int y = tid;
+            const int v = ti.nbr[0] >= 0 ? p.dist[(size_t)ti.nbr[0] * TN + y * TW + TW - 1] : DINF;
+            Hs[0][y + 1][0] = v;
+            Hs[1][y + 1][0] = v;
+        } else if (tid < 2 * TH) {
+            const int y = tid - TH;
+            const int v = ti.nbr[1] >= 0 ? p.dist[(size_t)ti.nbr[1] * TN + y * TW] : DINF;
+            Hs[0][y + 1][TW + 1] = v;
+            Hs[1][y + 1][TW + 1] = v;
+        }
+        if (wave == 1) {
+            const int v = ti.nbr[2] >= 0 ? p.dist[(size_t)ti.nbr[2] * TN + (TH - 1) * TW + lane] : DINF;
+            Hs[0][0][lx] = v;
+            Hs[1][0][lx] = v;
+        } else if (wave == 2) {
+            const int v = ti.nbr[3] >= 0 ? p.dist[(size_t)ti.nbr[3] * TN + lane] : DINF;
+            Hs[0][TH + 1][lx] = v;
+            Hs[1][TH + 1][lx] = v;
+        }
+        __syncthreads();
+
+        int accL[RPT], accR[RPT];
+        int accU = 0, accD = 0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) accL[k] = accR[k] = 0;
+        int cur = 0, it = 0;
+        while (it < p.inner_iters) {
+            ++it;
+            // Push along admissible arcs (d(u) == d(v) + 1) with the excess held
+            // at the start of the iteration, in the fixed order l, r, u, d.
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int ly = wave * RPT + k + 1;
+                int fl = 0, fr = 0, fu = 0, fd = 0;
+                int ek = e[k];
+                if (ek > 0 && d[k] < DINF) {
+                    const int hd = d[k] - 1;
+                    int4 ck = c[k];
+                    if (ck.x > 0 && Hs[cur][ly][lx - 1] == hd) {
+                        fl = min(ek, ck.x);
+                        ek -= fl;
+                        ck.x -= fl;
+                    }
+                    if (ek > 0 && ck.y > 0 && Hs[cur][ly][lx + 1] == hd) {
+                        fr = min(ek, ck.y);
+                        ek -= fr;
+                        ck.y -= fr;
+                    }
+                    if (ek > 0 && ck.z > 0 && Hs[cur][ly - 1][lx] == hd) {
+                        fu = min(ek, ck.z);
+                        ek -= fu;
+                        ck.z -= fu;
+                    }
+                    if (ek > 0 && ck.w > 0 && Hs[cur][ly + 1][lx] == hd) {
+                        fd = min(ek, ck.w);
+                        ek -= fd;
+                        ck.w -= fd;
+                    }
+                    c[k] = ck;
+                    e[k] = ek;
+                }
+                FLs[ly][lx] = fl;
+                FRs[ly][lx] = fr;
+                FUs[ly][lx] = fu;
+                FDs[ly][lx] = fd;
+                if (lane == 0) accL[k] += fl;
+                if (lane == TW - 1) accR[k] += fr;
+                if (ly == 1) accU += fu;
+                if (ly == TH) accD += fd;
+            }
+            __syncthreads();
+            // Receive, then relabel nodes left with excess and no admissible arc:
+            // d = 1 + min{d(v) : r(u,v) > 0} over the heights of this iteration.
+            int act = 0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int ly = wave * RPT + k + 1;
+                const int inL = FRs[ly][lx - 1], inR = FLs[ly][lx + 1];
+                const int inU = FDs[ly - 1][lx], inD = FUs[ly + 1][lx];
+                e[k] += inL + inR + inU + inD;
+                c[k].x += inL;
+                c[k].y += inR;
+                c[k].z += inU;
+                c[k].w += inD;
+                if (e[k] > 0 && d[k] < DINF) {
+                    int m = DINF;
+                    if (c[k].x > 0) m = min(m, Hs[cur][ly][lx - 1]);
+                    if (c[k].y > 0) m = min(m, Hs[cur][ly][lx + 1]);
+                    if (c[k].z > 0) m = min(m, Hs[cur][ly - 1][lx]);
+                    if (c[k].w > 0) m = min(m, Hs[cur][ly + 1][lx]);
+                    int nd = m >= DINF ? DINF : m + 1;
+                    if (nd >= dthr) nd = DINF;
+                    if (nd > d[k]) d[k] = nd;
+                    act |= d[k] < DINF;
+                }
+                Hs[cur ^ 1][ly][lx] = d[k];
+            }
+            cur ^= 1;
+            if (!__syncthreads_or(act)) break;
+        }
+
+        int mine = 0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int y = wave * RPT + k;
+            const size_t g = base + y * TW + lane;
+            p.caps[g] = c[k];
+            p.exc[g] = e[k];
+            p.dist[g] = d[k];
+            mine |= e[k] > 0 && d[k] < DINF;
+        }
+        const int colour = (ti.tx + ti.ty) & 1;
+        // Flow pushed into halo nodes: excess and reverse residual of the
+        // neighbour tile (not running in this colour pass; int32 atomics
+        // because two same-colour tiles can feed one corner node).
+        if (lane == 0 && ti.nbr[0] >= 0) {
+            int any = 0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (accL[k]) {
+                    const size_t gn = (size_t)ti.nbr[0] * TN + (wave * RPT + k) * TW + TW - 1;
+                    atomicAdd(&p.exc[gn], accL[k]);
+                    atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 1, accL[k]);
+                    any = 1;
+                }
+            if (any) set_active(p, ti.nbr[0], ti.inst, colour ^ 1, 1);
+        }
+        if (lane == TW - 1 && ti.nbr[1] >= 0) {
+            int any = 0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (accR[k]) {
+                    const size_t gn = (size_t)ti.nbr[1] * TN + (wave * RPT + k) * TW;
+                    atomicAdd(&p.exc[gn], accR[k]);
+                    atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 0, accR[k]);
+                    any = 1;
+                }
+            if (any) set_active(p, ti.nbr[1], ti.inst, colour ^ 1, 1);
+        }
+        if (wave == 0 && accU && ti.nbr[2] >= 0) {
+            const size_t gn = (size_t)ti.nbr[2] * TN + (TH - 1) * TW + lane;
+            atomicAdd(&p.exc[gn], accU);
+            atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 3, accU);
+            set_active(p, ti.nbr[2], ti.inst, colour ^ 1, 1);
+        }
+        if (wave == NT / 64 - 1 && accD && ti.nbr[3] >= 0) {
+            const size_t gn = (size_t)ti.nbr[3] * TN + lane;
+            atomicAdd(&p.exc[gn], accD);
+            atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 2, accD);
+            set_active(p, ti.nbr[3], ti.inst, colour ^ 1, 1);
+        }
+        const int any_active = __syncthreads_or(mine);
+        if (tid == 0) {
+            set_active(p, t, ti.inst, colour, any_active);
+            InstState &st = p.state[ti.inst];
+            atomic_add64(&st.pr_tiles, 1);
+            atomic_add64(&st.pr_iters, it);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Global relabel (exact BFS by tile relaxation) and finalisation.
+__global__ __launch_bounds__(NT) void k_gr(KParams p) {
+    __shared__ int Ds[LH][LW];
+    __shared__ int s_item;
+    __shared__ int64_t red[NT / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lx = lane + 1;
+    const int qlen = p.ctl->qlen_gr;
+    for (;;) {
+        if (tid == 0) {
+            const int q = atomicAdd(&p.ctl->head_gr, 1);
+            s_item = q < qlen ? p.queue_gr[q] : -1;
+        }
+        __syncthreads();
+        const int item = s_item;
+        if (item < 0) break;
+        const int op = item & 7, t = item >> 3;
+        const TileInfo ti = p.tiles[t];
+        const InstDesc &D = p.desc[ti.inst];
+        InstState &st = p.state[ti.inst];
+        const size_t base = (size_t)t * TN;
+
+        if (op == OP_FIN) {
+            int64_t sx = 0;
+            int32_t *lab = D.label_out;
+            const int W = D.W, H = D.H;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int y = wave * RPT + k;
+                const int X = ti.tx * TW + lane, Y = ti.ty * TH + y;
+                if (X < W && Y < H) {
+                    const size_t g = base + y * TW + lane;
+                    const bool in = X > 0 && Y > 0 && X < W - 1 && Y < H - 1;
+                    lab[(size_t)Y * W + X] = (in && p.dist[g] >= DINF) ? 1 : 0;
+                    if (in) sx += max(p.exc[g], 0);
+                }
+            }
+            const int64_t tot = block_sum64(sx, red);
+            if (tid == 0 && tot) atomic_add64(&st.sum_exc, tot);
+            __syncthreads();
+            continue;
+        }
+
+        const bool init = op == OP_GR_INIT;
+        int dv[RPT], orig[RPT], m[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int y = wave * RPT + k;
+            const size_t g = base + y * TW + lane;
+            int mk, d0;
+            if (init) {
+                const int4 cc = p.caps[g];
+                const int ee = p.exc[g];
+                mk = (cc.x > 0) | ((cc.y > 0) << 1) | ((cc.z > 0) << 2) | ((cc.w > 0) << 3) | ((ee > 0) << 4);
+                p.mask[g] = (uint8_t)mk;
+                d0 = ee < 0 ? 0 : DINF;
+            } else {
+                mk = p.mask[g];
+                d0 = p.dist[g];
+            }
+            m[k] = mk;
+            dv[k] = orig[k] = d0;
+            Ds[y + 1][lx] = d0;
+        }
+        // Halo: initial distances (from the neighbour's excess) for INIT,
+        // current distances otherwise. Both are upper bounds of the BFS
+        // distance, so stale reads only delay convergence.
+        if (tid < TH) {
+            const int y = tid;
+            int v = DINF;
+            if (ti.nbr[0] >= 0) {
+                const size_t gn = (size_t)ti.nbr[0] * TN + y * TW + TW - 1;
+                v = init ? (p.exc[gn] < 0 ? 0 : DINF) : p.dist[gn];
+            }
+            Ds[y + 1][0] = v;
+        } else if (tid < 2 * TH) {
+            const int y = tid - TH;
+            int v = DINF;
+            if (ti.nbr[1] >= 0) {
+                const size_t gn = (size_t)ti.nbr[1] * TN + y * TW;
+                v = init ? (p.exc[gn] < 0 ? 0 : DINF) : p.dist[gn];
+            }
+            Ds[y + 1][TW + 1] = v;
+        }
+        if (wave == 1) {
+            int v = DINF;
+            if (ti.nbr[2] >= 0) {
+                const size_t gn = (size_t)ti.nbr[2] * TN + (TH - 1) * TW + lane;
+                v = init ? (p.exc[gn] < 0 ? 0 : DINF) : p.dist[gn];
+            }
+            Ds[0][lx] = v;
+        } else if (wave == 2) {
+            int v = DINF;
+            if (ti.nbr[3] >= 0) {
+                const size_t gn = (size_t)ti.nbr[3] * TN + lane;
+                v = init ? (p.exc[gn] < 0 ? 0 : DINF) : p.dist[gn];
+            }
+            Ds[TH + 1][lx] = v;
+        }
+        __syncthreads();
+
+        bool conv = false;
+        for (int it = 0; it < GR_MAX_ITERS; ++it) {
+            int ch = 0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int mk = m[k];
+                if ((mk & 15) && dv[k] > 0) {
+                    const int ly = wave * RPT + k + 1;
+                    int best = dv[k];
+                    if (mk & 1) best = min(best, Ds[ly][lx - 1] + 1);
+                    if (mk & 2) best = min(best, Ds[ly][lx + 1] + 1);
+                    if (mk & 4) best = min(best, Ds[ly - 1][lx] + 1);
+                    if (mk & 8) best = min(best, Ds[ly + 1][lx] + 1);
+                    if (best < dv[k]) {
+                        dv[k] = best;
+                        Ds[ly][lx] = best;
+                        ch = 1;
+                    }
+                }
+            }
+            if (!__syncthreads_or(ch)) {
+                conv = true;
+                break;
+            }
+        }
+
+        int chL = 0, chR = 0, chU = 0, chD = 0, cnt = 0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int y = wave * RPT + k;
+            const size_t g = base + y * TW + lane;
+            const int changed = dv[k] != orig[k];
+            if (changed || init) p.dist[g] = dv[k];
+            if (lane == 0) chL |= changed;
+            if (lane == TW - 1) chR |= changed;
+            if (y == 0) chU |= changed;
+            if (y == TH - 1) chD |= changed;
+            cnt += (m[k] & 16) && dv[k] < DINF;
+        }
+        chL = __syncthreads_or(chL);
+        chR = __syncthreads_or(chR);
+        chU = __syncthreads_or(chU);
+        chD = __syncthreads_or(chD);
+        const int64_t total = block_sum64(cnt, red);
+        if (tid == 0) {
+            const int inst = ti.inst;
+            if (chL && ti.nbr[0] >= 0) mark_dirty(p, ti.nbr[0], inst);
+            if (chR && ti.nbr[1] >= 0) mark_dirty(p, ti.nbr[1], inst);
+            if (chU && ti.nbr[2] >= 0) mark_dirty(p, ti.nbr[2], inst);
+            if (chD && ti.nbr[3] >= 0) mark_dirty(p, ti.nbr[3], inst);
+            if (!conv) mark_dirty(p, t, inst);
+            const int colour = (ti.tx + ti.ty) & 1;
+            const int c32 = (int)total;
+            if (init) {
+                p.tile_cnt[t] = c32;
+                if (c32) atomicAdd(&st.active_nodes, c32);
+                p.act_flag[t] = c32 > 0;
+                if (c32 > 0) atomicAdd(&st.act_tiles[colour], 1);
+            } else {
+                const int delta = c32 - p.tile_cnt[t];
+                if (delta) {
+                    atomicAdd(&st.active_nodes, delta);
+                    p.tile_cnt[t] = c32;
+                }
+                set_active(p, t, inst, colour, c32 > 0);
+            }
+            atomic_add64(&st.gr_tiles, 1);
+        }
+        __syncthreads();
+    }
+    // The last workgroup to leave resets the queues for the next step.
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        const int tk = atomicAdd(&p.ctl->gr_ticket, 1);
+        if (tk == (int)gridDim.x - 1) {
+            p.ctl->qlen_pr = 0;
+            p.ctl->head_pr = 0;
+            p.ctl->qlen_gr = 0;
+            p.ctl->head_gr = 0;
+            p.ctl->steps++;
+            __threadfence();
+            atomicExch(&p.ctl->gr_ticket, 0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Warm-start edit: strokes of (x0, y0, w, h, cls) add delta to pos (fg) or
+// neg (bg) of interior pixels, i.e. excess += delta or -= delta.
+__global__ __launch_bounds__(NT) void k_edit(KParams p, int slot, const int32_t *strokes, int n, int32_t delta) {
+    __shared__ int64_t red[NT / 64];
+    const InstDesc D = p.desc[slot];
+    for (int s = blockIdx.x; s < n; s += gridDim.x) {
+        const int x0 = strokes[5 * s], y0 = strokes[5 * s + 1], w = strokes[5 * s + 2], h = strokes[5 * s + 3];
+        const int cls = strokes[5 * s + 4];
+        int64_t dp = 0;
+        for (int i = threadIdx.x; i < w * h; i += NT) {
+            const int X = x0 + i % w, Y = y0 + i / w;
+            if (X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1) {
+                const int t = D.tile0 + (Y / TH) * D.tiles_x + X / TW;
+                const size_t g = (size_t)t * TN + (Y % TH) * TW + (X % TW);
+                if (cls == GCFG_SEED_FG) {
+                    atomicAdd(&p.exc[g], delta);
+                    dp += delta;
+                } else {
+                    atomicAdd(&p.exc[g], -delta);
+                }
+            }
+        }
+        const int64_t tot = block_sum64(dp, red);
+        if (threadIdx.x == 0 && tot) atomic_add64(&p.state[slot].sum_pos, tot);
+        __syncthreads();
+    }
+}
+
+// Residual state in image layout (certificates and debugging).
+__global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *exc_out) {
+    const InstDesc D = p.desc[slot];
+    const int64_t n = (int64_t)D.W * D.H;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int X = (int)(i % D.W), Y = (int)(i / D.W);
+        const int t = D.tile0 + (Y / TH) * D.tiles_x + X / TW;
+        const size_t g = (size_t)t * TN + (Y % TH) * TW + (X % TW);
+        const int4 c = p.caps[g];
+        caps_out[4 * i + 0] = c.x;
+        caps_out[4 * i + 1] = c.y;
+        caps_out[4 * i + 2] = c.z;
+        caps_out[4 * i + 3] = c.w;
+        exc_out[i] = p.exc[g];
+    }
+}
+
+// ---------------------------------------------------------------------------
+void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int sweeps, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_arm, dim3((n + 255) / 256), dim3(256), 0, s, p, d_slots, n, warm, sweeps);
+}
+
+void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_tiles, hipStream_t s) {
+    if (n <= 0 || total_tiles <= 0) return;
+    const int grid = total_tiles < 8192 ? total_tiles : 8192;
+    hipLaunchKernelGGL(k_build, dim3(grid), dim3(NT), 0, s, p, d_slots, d_slots + n, n, total_tiles);
+}
+
+void launch_step(const KParams &p, int grid_pr, int grid_gr, hipStream_t s) {
+    hipLaunchKernelGGL(k_sched, dim3(p.n_slots), dim3(NT), 0, s, p);
+    hipLaunchKernelGGL(k_pr, dim3(grid_pr), dim3(NT), 0, s, p);
+    hipLaunchKernelGGL(k_gr, dim3(grid_gr), dim3(NT), 0, s, p);
+}
+
+void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,
+                 hipStream_t s) {
+    if (n_strokes <= 0) return;
+    hipLaunchKernelGGL(k_edit, dim3(n_strokes < 1024 ? n_strokes : 1024), dim3(NT), 0, s, p, slot, d_strokes,
+                       n_strokes, delta);
+}
+
+void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, hipStream_t s) {
+    hipLaunchKernelGGL(k_read_state, dim3(1024), dim3(256), 0, s, p, slot, d_caps, d_exc);
+}
+
+}  // namespace gcfg

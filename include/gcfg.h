@@ -1,0 +1,172 @@
+/*
+ * gcfg.h -- C ABI of the MI355X-native binary-MRF min-cut library (libgcfg.so).
+ *
+ * Plain pointers and sizes only; every call returns an int status
+ * (GCFG_OK == 0) and gcfg_last_error() gives the message of the last failure
+ * on the calling thread. "d_" pointers are device (HBM) pointers allocated by
+ * the caller with hipMalloc on the current device; "h_" pointers are host
+ * pointers. Directional n-link arrays follow GraphCut.cu:271-289 and
+ * GraphCutKernels.cu:186-189: up[x + y*W] is the capacity from pixel (x,y) to
+ * (x,y-1), down to (x,y+1), left to (x-1,y), right to (x+1,y). Only interior
+ * pixels 0<x<W-1, 0<y<H-1 are graph nodes (GraphCutKernels.cu:122-123,
+ * :143-146). Labels: 1 <=> interior node that cannot reach the sink in the
+ * final residual graph (SPREAD_ZEROS, GraphCut.h:37), border pixels 0.
+ * Calls on one handle must not overlap; distinct handles are independent.
+ */
+#ifndef GCFG_H
+#define GCFG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    GCFG_OK = 0,
+    GCFG_ERR_INVALID = -1,       /* bad argument, shape or capacity */
+    GCFG_ERR_HIP = -2,           /* HIP runtime error */
+    GCFG_ERR_NOMEM = -3,         /* device allocation failed */
+    GCFG_ERR_STATE = -4,         /* call out of order (e.g. optimize before build) */
+    GCFG_ERR_NOT_CONVERGED = -5, /* step cap reached */
+    GCFG_ERR_BUDGET = -6         /* request larger than the serving HBM budget */
+};
+
+enum { GCFG_SEED_NONE = 0, GCFG_SEED_FG = 1, GCFG_SEED_BG = 2 };
+
+typedef struct gcfg_options {
+    int32_t sweeps_per_gr;   /* colour-pass pairs of tile push-relabel between global relabels */
+    int32_t inner_iters;     /* max in-LDS push/relabel iterations per tile pass */
+    int32_t steps_per_poll;  /* scheduler steps launched between two host polls */
+    int32_t use_graph;       /* capture a chunk of steps in a hipGraph and replay it */
+    int32_t max_steps;       /* 0 = unbounded; else GCFG_ERR_NOT_CONVERGED past this */
+    int32_t verbose;         /* 1: print the reference's "pen"/energy lines */
+} gcfg_options;
+
+typedef struct gcfg_stats {
+    int64_t flow;        /* exact max-flow value over the interior */
+    int64_t pr_tiles;    /* tile push-relabel passes executed */
+    int64_t gr_tiles;    /* tile BFS sweeps executed (global relabel) */
+    int64_t pr_iters;    /* in-LDS push/relabel iterations summed over tiles */
+    int32_t steps;       /* scheduler steps until done */
+    int32_t gr_rounds;   /* global relabels started */
+    double ms;           /* wall time of the call */
+} gcfg_stats;
+
+void gcfg_default_options(gcfg_options *opt);
+const char *gcfg_last_error(void);
+/* Name, compute-unit count and HBM size of the current device. */
+int gcfg_device_info(char *name, size_t name_len, int32_t *cu_count, size_t *hbm_bytes);
+
+/* ---- Plain drop-in graph: the GC_* functions of GraphCut.cu ---- */
+typedef struct gcfg_graph gcfg_graph;
+/* GC_Init allocation part, GraphCut.cu:187-248. */
+int gcfg_graph_create(int32_t width, int32_t height, gcfg_graph **out);
+/* GC_SetDataterms, GraphCut.cu:265-269 (borrowed device pointers, W*H each). */
+int gcfg_graph_set_dataterms(gcfg_graph *g, const int32_t *d_pos, const int32_t *d_neg);
+/* GC_SetEdges, GraphCut.cu:271-289 (borrowed device pointers, W*H each). */
+int gcfg_graph_set_edges(gcfg_graph *g, const int32_t *d_up, const int32_t *d_down, const int32_t *d_left,
+                         const int32_t *d_right);
+/* GC_SetPenalty, GraphCut.cu:296-299 (uniform n-link capacity). */
+int gcfg_graph_set_penalty(gcfg_graph *g, int32_t penalty);
+int gcfg_graph_set_options(gcfg_graph *g, const gcfg_options *opt);
+/* GC_SetGraph, GraphCut.cu:301-319: builds the residual graph on the device. */
+int gcfg_graph_build(gcfg_graph *g);
+/* GC_Optimize, GraphCut.cu:385-534: solves and writes W*H labels to host memory. */
+int gcfg_graph_optimize(gcfg_graph *g, int32_t *h_label, gcfg_stats *stats);
+/* Same solve with the W*H labels left in caller-owned device memory. */
+int gcfg_graph_optimize_device(gcfg_graph *g, int32_t *d_label, gcfg_stats *stats);
+/* Residual state after optimize, image layout: h_caps[4*(x+y*W)+k] is the
+ * residual capacity from (x,y) towards left/right/up/down (k = 0..3),
+ * h_excess[x+y*W] the excess (negative = remaining sink capacity). */
+int gcfg_graph_read_state(gcfg_graph *g, int32_t *h_caps, int32_t *h_excess);
+/* GC_End, GraphCut.cu:542-568 (borrowed inputs are not freed). */
+int gcfg_graph_destroy(gcfg_graph *g);
+
+/* ---- Colour data term: the DataTerm.h functions ---- */
+typedef struct gcfg_dataterm gcfg_dataterm;
+/* init_data_term, DataTerm.cpp:28-65. */
+int gcfg_dt_create(int32_t width, int32_t height, gcfg_dataterm **out);
+/* free_data_term, DataTerm.cpp:67-78. */
+int gcfg_dt_destroy(gcfg_dataterm *dt);
+/* setImage, DataTerm.cpp:80-86 (W*H*3 interleaved u8, host). */
+int gcfg_dt_set_image(gcfg_dataterm *dt, const uint8_t *h_rgb);
+/* Same with a caller-owned device image. */
+int gcfg_dt_set_image_device(gcfg_dataterm *dt, const uint8_t *d_rgb);
+/* setSelection, DataTerm.cpp:139-153: per-pixel seed classes (GCFG_SEED_*), host. */
+int gcfg_dt_set_seeds(gcfg_dataterm *dt, const uint8_t *h_seeds);
+/* selectOnMap, DataTerm.cpp:105-127: mark a w x h rectangle of seeds of class cls. */
+int gcfg_dt_add_seed_rect(gcfg_dataterm *dt, int32_t x0, int32_t y0, int32_t w, int32_t h, int32_t cls);
+/* reset, DataTerm.cpp:88-95: clear all seeds. */
+int gcfg_dt_reset(gcfg_dataterm *dt);
+/* n-link weight lambda (default 50). */
+int gcfg_dt_set_lambda(gcfg_dataterm *dt, int32_t lambda);
+/* getDataTermsMixture, DataTerm.cpp:129-137: histograms, beta, t-links, n-links. */
+int gcfg_dt_compute(gcfg_dataterm *dt);
+/* getPos / getNeg, DataTerm.cpp:12-26, plus the four n-link planes (device, owned by dt). */
+int gcfg_dt_get_links(gcfg_dataterm *dt, int32_t **d_pos, int32_t **d_neg, int32_t **d_up, int32_t **d_down,
+                      int32_t **d_left, int32_t **d_right);
+/* Seed histograms (class-major: fg then bg; channel; 16 bins) and seed counts. */
+int gcfg_dt_get_histograms(gcfg_dataterm *dt, int32_t *h_hist96, int32_t *h_counts2);
+/* Exact sum of squared 4-neighbour colour differences, pair count, beta. */
+int gcfg_dt_get_beta(gcfg_dataterm *dt, int64_t *sum, int64_t *pairs, double *beta);
+/* Pre-quantisation values computed on the device in f64 (W*H each, host out). */
+int gcfg_dt_unquantized(gcfg_dataterm *dt, double *h_tpos, double *h_tneg, double *h_right, double *h_down);
+
+/* ---- Batched solve: many instances of uneven size in one engine ---- */
+typedef struct gcfg_instance {
+    int32_t width, height;
+    const int32_t *d_pos, *d_neg;                   /* W*H each */
+    const int32_t *d_up, *d_down, *d_left, *d_right; /* W*H each, or all NULL for a uniform penalty */
+    int32_t penalty;
+} gcfg_instance;
+typedef struct gcfg_batch gcfg_batch;
+int gcfg_batch_create(int32_t count, const gcfg_instance *inst, const gcfg_options *opt, gcfg_batch **out);
+/* Cold solve of every instance; labels to host (h_labels[i]: W_i*H_i) and flows. */
+int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_stats *stats);
+int gcfg_batch_destroy(gcfg_batch *b);
+
+/* ---- Serving: continuous batching under an HBM budget, warm-started edits ---- */
+typedef struct gcfg_server gcfg_server;
+typedef struct gcfg_result {
+    int64_t request_id;
+    int32_t session;
+    int32_t status;      /* GCFG_OK or an error code */
+    int64_t flow;
+    double latency_ms;   /* submit -> labels in host memory */
+    double solve_ms;     /* admission -> labels in host memory */
+} gcfg_result;
+int gcfg_server_create(size_t hbm_budget_bytes, const gcfg_options *opt, gcfg_server **out);
+/* New session: image + seed map (host), data term and solve on the device.
+ * h_label (W*H int32, host) receives the labels; it must stay valid until the
+ * request's result is returned by gcfg_server_poll. */
+int gcfg_server_submit_image(gcfg_server *s, int32_t width, int32_t height, const uint8_t *h_rgb,
+                             const uint8_t *h_seeds, int32_t *h_label, int64_t *request_id, int32_t *session);
+/* Edit of a live session: n strokes of 5 ints (x0, y0, w, h, cls); a
+ * foreground stroke adds delta to pos, a background stroke adds delta to neg,
+ * on the stroke pixels. Warm-starts from the device-resident residual graph. */
+int gcfg_server_submit_edit(gcfg_server *s, int32_t session, int32_t n_strokes, const int32_t *strokes,
+                            int32_t delta, int32_t *h_label, int64_t *request_id);
+/* Advance the device and collect finished requests (non-blocking if wait == 0). */
+int gcfg_server_poll(gcfg_server *s, int32_t wait, int32_t max_results, gcfg_result *results, int32_t *n_results);
+/* Drop a session and return its memory to the budget. */
+int gcfg_server_close_session(gcfg_server *s, int32_t session);
+/* Requests admitted and waiting, and HBM bytes in use. */
+int gcfg_server_load(gcfg_server *s, int32_t *in_flight, int32_t *queued, size_t *bytes_in_use);
+int gcfg_server_destroy(gcfg_server *s);
+
+/* ---- Seeded synthetic inputs (host-only, no GPU work) ---- */
+int gcfg_gen_ellipse_image(int width, int height, uint64_t seed, uint8_t *rgb);
+int gcfg_gen_seed_map(int width, int height, uint8_t *seeds);
+int gcfg_gen_stress_tlinks(int width, int height, uint64_t seed, int32_t *pos, int32_t *neg);
+int gcfg_gen_perlin_image(int width, int height, uint64_t seed, uint8_t *rgb);
+int gcfg_gen_batch_shapes(int count, uint64_t seed, double area_min, double area_max, int32_t *widths,
+                          int32_t *heights);
+uint64_t gcfg_gen_mix_seed(uint64_t seed, uint64_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GCFG_H */

@@ -1,0 +1,156 @@
+"""CPU oracle for the binary-MRF min-cut path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg import this module, and only as the checker (or, for the
+baseline, as the timed CPU restatement of the reference schedule). The product
+library in ``gc-fg_amd/`` never imports, links or calls it.
+
+Functions (numpy in / numpy out, ctypes over ``oracle/liboracle.so``):
+
+* :func:`maxflow` -- exact int64 max-flow and the "cannot reach the sink"
+  label map on the graph of ``GraphCutKernels.cu:112-214`` (interior nodes
+  only, arcs into the border zeroed); labels as ``GraphCutKernels.cu:228-351``
+  with ``SPREAD_ZEROS`` (``GraphCut.h:37``).
+* :func:`cut_value` -- cut value of a labelling on the same graph.
+* :func:`dataterm` -- per-pixel double-precision colour data term (16-bin
+  seed histograms, ``round(64 * -ln p)`` t-links, ``round(lambda *
+  exp(-d^2 / 2 beta))`` n-links), the stage that sits behind ``DataTerm.h``.
+* :func:`refpr` -- CPU restatement of the reference push-relabel schedule
+  (``GraphCut.cu:385-534``), used as the timed CPU baseline.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_U8P = ctypes.POINTER(ctypes.c_uint8)
+_F64P = ctypes.POINTER(ctypes.c_double)
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B", "-C", _HERE], check=True, stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        lib = ctypes.CDLL(_LIB_PATH)
+        lib.oracle_maxflow_grid.restype = ctypes.c_int
+        lib.oracle_maxflow_grid.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P,
+                                            ctypes.c_int32, _I32P, _I64P]
+        lib.oracle_cut_value.restype = ctypes.c_int64
+        lib.oracle_cut_value.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P,
+                                         ctypes.c_int32, _I32P]
+        lib.oracle_dataterm.restype = ctypes.c_int
+        lib.oracle_dataterm.argtypes = [ctypes.c_int, ctypes.c_int, _U8P, _U8P, ctypes.c_int32, _I32P, _I32P,
+                                        _I64P, _I64P, _F64P, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P, _F64P,
+                                        _F64P, _F64P, _F64P]
+        lib.oracle_refpr_solve.restype = ctypes.c_int
+        lib.oracle_refpr_solve.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P,
+                                           ctypes.c_int32, ctypes.c_int, _I32P, _I64P, _I64P]
+        _lib = lib
+    return _lib
+
+
+def _ptr(a, ctype):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _edges(edges):
+    if edges is None:
+        return (None,) * 4
+    up, down, left, right = edges
+    return tuple(_i32(e).ravel() for e in (up, down, left, right))
+
+
+def maxflow(width, height, pos, neg, edges=None, penalty=0):
+    """Exact max-flow. ``edges`` = (up, down, left, right) arrays or None for a
+    uniform ``penalty``. Returns (flow:int, label:int32[H, W])."""
+    lib = _load()
+    pos, neg = _i32(pos).ravel(), _i32(neg).ravel()
+    up, down, left, right = _edges(edges)
+    label = np.zeros(width * height, np.int32)
+    flow = ctypes.c_int64(0)
+    rc = lib.oracle_maxflow_grid(width, height, _ptr(pos, ctypes.c_int32), _ptr(neg, ctypes.c_int32),
+                                 _ptr(up, ctypes.c_int32), _ptr(down, ctypes.c_int32), _ptr(left, ctypes.c_int32),
+                                 _ptr(right, ctypes.c_int32), int(penalty), _ptr(label, ctypes.c_int32),
+                                 ctypes.byref(flow))
+    if rc != 0:
+        raise RuntimeError(f"oracle_maxflow_grid failed: {rc}")
+    return int(flow.value), label.reshape(height, width)
+
+
+def cut_value(width, height, pos, neg, label, edges=None, penalty=0) -> int:
+    lib = _load()
+    pos, neg = _i32(pos).ravel(), _i32(neg).ravel()
+    up, down, left, right = _edges(edges)
+    label = _i32(label).ravel()
+    return int(lib.oracle_cut_value(width, height, _ptr(pos, ctypes.c_int32), _ptr(neg, ctypes.c_int32),
+                                    _ptr(up, ctypes.c_int32), _ptr(down, ctypes.c_int32),
+                                    _ptr(left, ctypes.c_int32), _ptr(right, ctypes.c_int32), int(penalty),
+                                    _ptr(label, ctypes.c_int32)))
+
+
+def dataterm(width, height, rgb, seeds, lam=50):
+    """Per-pixel double-precision data term. Returns a dict of numpy arrays."""
+    lib = _load()
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8).ravel()
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint8).ravel()
+    n = width * height
+    out = {
+        "hist": np.zeros(96, np.int32), "counts": np.zeros(2, np.int32),
+        "pos": np.zeros(n, np.int32), "neg": np.zeros(n, np.int32),
+        "up": np.zeros(n, np.int32), "down": np.zeros(n, np.int32),
+        "left": np.zeros(n, np.int32), "right": np.zeros(n, np.int32),
+        "tpos_f": np.zeros(n, np.float64), "tneg_f": np.zeros(n, np.float64),
+        "right_f": np.zeros(n, np.float64), "down_f": np.zeros(n, np.float64),
+    }
+    bsum, pairs, beta = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_double(0.0)
+    rc = lib.oracle_dataterm(width, height, _ptr(rgb, ctypes.c_uint8), _ptr(seeds, ctypes.c_uint8), int(lam),
+                             _ptr(out["hist"], ctypes.c_int32), _ptr(out["counts"], ctypes.c_int32),
+                             ctypes.byref(bsum), ctypes.byref(pairs), ctypes.byref(beta),
+                             *[_ptr(out[k], ctypes.c_int32) for k in ("pos", "neg", "up", "down", "left", "right")],
+                             *[_ptr(out[k], ctypes.c_double) for k in ("tpos_f", "tneg_f", "right_f", "down_f")])
+    if rc != 0:
+        raise RuntimeError(f"oracle_dataterm failed: {rc}")
+    out["hist"] = out["hist"].reshape(2, 3, 16)
+    out["beta_sum"], out["pairs"], out["beta"] = int(bsum.value), int(pairs.value), float(beta.value)
+    for k in ("pos", "neg", "up", "down", "left", "right", "tpos_f", "tneg_f", "right_f", "down_f"):
+        out[k] = out[k].reshape(height, width)
+    return out
+
+
+def refpr(width, height, pos, neg, edges=None, penalty=0, threads=0):
+    """Reference-schedule push-relabel on the CPU. Returns (flow, label, iterations, printed_energy)."""
+    lib = _load()
+    pos, neg = _i32(pos).ravel(), _i32(neg).ravel()
+    up, down, left, right = _edges(edges)
+    label = np.zeros(width * height, np.int32)
+    flow, printed = ctypes.c_int64(0), ctypes.c_int64(0)
+    it = lib.oracle_refpr_solve(width, height, _ptr(pos, ctypes.c_int32), _ptr(neg, ctypes.c_int32),
+                                _ptr(up, ctypes.c_int32), _ptr(down, ctypes.c_int32), _ptr(left, ctypes.c_int32),
+                                _ptr(right, ctypes.c_int32), int(penalty), int(threads),
+                                _ptr(label, ctypes.c_int32), ctypes.byref(flow), ctypes.byref(printed))
+    if it < 0:
+        raise RuntimeError(f"oracle_refpr_solve failed: {it}")
+    return int(flow.value), label.reshape(height, width), int(it), int(printed.value)
