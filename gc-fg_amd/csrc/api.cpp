@@ -1,0 +1,356 @@
+// C ABI: plain graph (GraphCut.cu GC_*), batched solve, options and errors.
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace gcfg {
+const char *last_error_cstr();
+}
+
+using gcfg::Engine;
+using gcfg::InstState;
+using gcfg::now_ms;
+using gcfg::set_error;
+
+struct gcfg_graph {
+    Engine eng;
+    int W = 0, H = 0, slot = -1;
+    bool built = false;
+    int32_t *d_label = nullptr;  // internal label buffer (W*H)
+    int32_t *label_target = nullptr;
+    const int32_t *pos = nullptr, *neg = nullptr, *up = nullptr, *down = nullptr, *left = nullptr,
+                  *right = nullptr;
+    int32_t penalty = 0;
+    gcfg_options opt{};
+};
+
+struct gcfg_batch {
+    Engine eng;
+    std::vector<gcfg_instance> inst;
+    std::vector<int> slots;
+    int32_t *d_labels = nullptr;  // internal label arena
+    std::vector<size_t> label_off;
+    std::vector<InstState> states;
+};
+
+extern "C" {
+
+void gcfg_default_options(gcfg_options *o) {
+    if (!o) return;
+    o->sweeps_per_gr = 8;
+    o->inner_iters = 64;
+    o->steps_per_poll = 8;
+    o->use_graph = 1;
+    o->max_steps = 200000;
+    o->verbose = 0;
+}
+
+const char *gcfg_last_error(void) { return gcfg::last_error_cstr(); }
+
+int gcfg_device_info(char *name, size_t name_len, int32_t *cu_count, size_t *hbm_bytes) {
+    int dev = 0;
+    GCFG_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    GCFG_HIP(hipGetDeviceProperties(&prop, dev));
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (cu_count) *cu_count = prop.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = prop.totalGlobalMem;
+    return GCFG_OK;
+}
+
+// ---------------------------------------------------------------------------
+int gcfg_graph_create(int32_t width, int32_t height, gcfg_graph **out) {
+    if (!out || width <= 0 || height <= 0) {
+        set_error("gcfg_graph_create: invalid shape or output pointer");
+        return GCFG_ERR_INVALID;
+    }
+    *out = nullptr;
+    gcfg_graph *g = new (std::nothrow) gcfg_graph();
+    if (!g) return GCFG_ERR_NOMEM;
+    gcfg_default_options(&g->opt);
+    g->W = width;
+    g->H = height;
+    int rc = g->eng.init(g->eng.tiles_for(width, height), 1, g->opt);
+    if (rc == GCFG_OK && hipMalloc((void **)&g->d_label, (size_t)width * height * sizeof(int32_t)) != hipSuccess) {
+        set_error("gcfg_graph_create: label buffer allocation failed");
+        rc = GCFG_ERR_NOMEM;
+    }
+    if (rc == GCFG_OK) {
+        g->slot = g->eng.add_instance(width, height, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                                      g->d_label);
+        if (g->slot < 0) rc = g->slot;
+    }
+    if (rc != GCFG_OK) {
+        if (g->d_label) hipFree(g->d_label);
+        delete g;
+        return rc;
+    }
+    g->label_target = g->d_label;
+    *out = g;
+    return GCFG_OK;
+}
+
+static int graph_push_inputs(gcfg_graph *g) {
+    return g->eng.set_inputs(g->slot, g->pos, g->neg, g->up, g->down, g->left, g->right, g->penalty,
+                             g->label_target);
+}
+
+int gcfg_graph_set_dataterms(gcfg_graph *g, const int32_t *d_pos, const int32_t *d_neg) {
+    if (!g || !d_pos || !d_neg) {
+        set_error("gcfg_graph_set_dataterms: null argument");
+        return GCFG_ERR_INVALID;
+    }
+    g->pos = d_pos;
+    g->neg = d_neg;
+    return graph_push_inputs(g);
+}
+
+int gcfg_graph_set_edges(gcfg_graph *g, const int32_t *d_up, const int32_t *d_down, const int32_t *d_left,
+                         const int32_t *d_right) {
+    if (!g || !d_up || !d_down || !d_left || !d_right) {
+        set_error("gcfg_graph_set_edges: null argument");
+        return GCFG_ERR_INVALID;
+    }
+    g->up = d_up;
+    g->down = d_down;
+    g->left = d_left;
+    g->right = d_right;
+    return graph_push_inputs(g);
+}
+
+int gcfg_graph_set_penalty(gcfg_graph *g, int32_t penalty) {
+    if (!g || penalty < 0) {
+        set_error("gcfg_graph_set_penalty: null handle or negative penalty");
+        return GCFG_ERR_INVALID;
+    }
+    g->up = g->down = g->left = g->right = nullptr;
+    g->penalty = penalty;
+    return graph_push_inputs(g);
+}
+
+int gcfg_graph_set_options(gcfg_graph *g, const gcfg_options *opt) {
+    if (!g || !opt || opt->sweeps_per_gr <= 0 || opt->inner_iters <= 0 || opt->steps_per_poll <= 0) {
+        set_error("gcfg_graph_set_options: invalid options");
+        return GCFG_ERR_INVALID;
+    }
+    g->opt = *opt;
+    g->eng.opt = *opt;
+    return GCFG_OK;
+}
+
+int gcfg_graph_build(gcfg_graph *g) {
+    if (!g) return GCFG_ERR_INVALID;
+    if (!g->pos || !g->neg) {
+        set_error("gcfg_graph_build: data terms not set");
+        return GCFG_ERR_STATE;
+    }
+    if (g->opt.verbose) std::printf("pen %d\n", g->penalty);
+    int rc = graph_push_inputs(g);
+    if (rc == GCFG_OK) rc = g->eng.build({g->slot});
+    g->built = rc == GCFG_OK;
+    return rc;
+}
+
+static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
+    const double t0 = now_ms();
+    if (!g->built) {
+        set_error("gcfg_graph_optimize: graph not built (call gcfg_graph_build)");
+        return GCFG_ERR_STATE;
+    }
+    if (d_target != g->label_target) {
+        // The label destination is part of the uploaded descriptor.
+        g->label_target = d_target;
+        int rc = graph_push_inputs(g);
+        if (rc == GCFG_OK) rc = g->eng.build({g->slot});
+        if (rc != GCFG_OK) return rc;
+    }
+    int rc = g->eng.run(g->opt.max_steps);
+    if (rc != GCFG_OK) return rc;
+    InstState st;
+    rc = g->eng.fetch_state(g->slot, &st);
+    if (rc != GCFG_OK) return rc;
+    if (st.err) {
+        set_error("negative capacity in the inputs");
+        return GCFG_ERR_INVALID;
+    }
+    if (g->opt.verbose) std::printf("Final energy level: %lld\n", (long long)st.flow);
+    if (stats) {
+        stats->flow = st.flow;
+        stats->pr_tiles = st.pr_tiles;
+        stats->gr_tiles = st.gr_tiles;
+        stats->pr_iters = st.pr_iters;
+        stats->steps = st.steps;
+        stats->gr_rounds = st.gr_rounds;
+        stats->ms = now_ms() - t0;
+    }
+    return GCFG_OK;
+}
+
+int gcfg_graph_optimize_device(gcfg_graph *g, int32_t *d_label, gcfg_stats *stats) {
+    if (!g || !d_label) {
+        set_error("gcfg_graph_optimize_device: null argument");
+        return GCFG_ERR_INVALID;
+    }
+    return graph_solve(g, d_label, stats);
+}
+
+int gcfg_graph_optimize(gcfg_graph *g, int32_t *h_label, gcfg_stats *stats) {
+    if (!g || !h_label) {
+        set_error("gcfg_graph_optimize: null argument");
+        return GCFG_ERR_INVALID;
+    }
+    const double t0 = now_ms();
+    int rc = graph_solve(g, g->d_label, stats);
+    if (rc != GCFG_OK) return rc;
+    GCFG_HIP(hipMemcpyAsync(h_label, g->d_label, (size_t)g->W * g->H * sizeof(int32_t), hipMemcpyDeviceToHost,
+                            g->eng.stream()));
+    GCFG_HIP(hipStreamSynchronize(g->eng.stream()));
+    if (stats) stats->ms = now_ms() - t0;
+    return GCFG_OK;
+}
+
+int gcfg_graph_read_state(gcfg_graph *g, int32_t *h_caps, int32_t *h_excess) {
+    if (!g || !h_caps || !h_excess) {
+        set_error("gcfg_graph_read_state: null argument");
+        return GCFG_ERR_INVALID;
+    }
+    if (!g->built) {
+        set_error("gcfg_graph_read_state: graph not built");
+        return GCFG_ERR_STATE;
+    }
+    return g->eng.read_state(g->slot, h_caps, h_excess);
+}
+
+int gcfg_graph_destroy(gcfg_graph *g) {
+    if (!g) return GCFG_OK;
+    g->eng.sync();
+    if (g->d_label) hipFree(g->d_label);
+    delete g;
+    return GCFG_OK;
+}
+
+// ---------------------------------------------------------------------------
+int gcfg_batch_create(int32_t count, const gcfg_instance *inst, const gcfg_options *opt, gcfg_batch **out) {
+    if (!out || count <= 0 || !inst) {
+        set_error("gcfg_batch_create: invalid arguments");
+        return GCFG_ERR_INVALID;
+    }
+    *out = nullptr;
+    gcfg_options o;
+    if (opt)
+        o = *opt;
+    else
+        gcfg_default_options(&o);
+    gcfg_batch *b = new (std::nothrow) gcfg_batch();
+    if (!b) return GCFG_ERR_NOMEM;
+    b->inst.assign(inst, inst + count);
+    int tiles = 0;
+    size_t labels = 0;
+    for (int i = 0; i < count; ++i) {
+        const gcfg_instance &in = inst[i];
+        if (in.width <= 0 || in.height <= 0 || !in.d_pos || !in.d_neg) {
+            set_error("gcfg_batch_create: instance " + std::to_string(i) + " has no shape or data terms");
+            delete b;
+            return GCFG_ERR_INVALID;
+        }
+        tiles += b->eng.tiles_for(in.width, in.height);
+        b->label_off.push_back(labels);
+        labels += (size_t)in.width * in.height;
+    }
+    int rc = b->eng.init(tiles, count, o);
+    if (rc == GCFG_OK && hipMalloc((void **)&b->d_labels, labels * sizeof(int32_t)) != hipSuccess) {
+        set_error("gcfg_batch_create: label arena allocation failed");
+        rc = GCFG_ERR_NOMEM;
+    }
+    for (int i = 0; rc == GCFG_OK && i < count; ++i) {
+        const gcfg_instance &in = inst[i];
+        const int s = b->eng.add_instance(in.width, in.height, in.d_pos, in.d_neg, in.d_up, in.d_down, in.d_left,
+                                          in.d_right, in.penalty, b->d_labels + b->label_off[i]);
+        if (s < 0) rc = s;
+        b->slots.push_back(s);
+    }
+    if (rc != GCFG_OK) {
+        if (b->d_labels) hipFree(b->d_labels);
+        delete b;
+        return rc;
+    }
+    b->states.resize(count);
+    *out = b;
+    return GCFG_OK;
+}
+
+static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_stats *stats) {
+    const double t0 = now_ms();
+    const int n = (int)b->slots.size();
+    for (int i = 0; i < n; ++i) {
+        const gcfg_instance &in = b->inst[i];
+        int32_t *lab = d_targets ? d_targets[i] : b->d_labels + b->label_off[i];
+        b->eng.set_inputs(b->slots[i], in.d_pos, in.d_neg, in.d_up, in.d_down, in.d_left, in.d_right, in.penalty,
+                          lab);
+    }
+    int rc = b->eng.build(b->slots);
+    if (rc == GCFG_OK) rc = b->eng.run(b->eng.opt.max_steps);
+    if (rc != GCFG_OK) return rc;
+    // Instance slots were assigned 0..n-1 in order at creation: one copy.
+    rc = b->eng.fetch_states(b->states.data(), n);
+    if (rc != GCFG_OK) return rc;
+    gcfg_stats agg{};
+    for (int i = 0; i < n; ++i) {
+        const InstState &st = b->states[i];
+        if (st.err) {
+            set_error("negative capacity in instance " + std::to_string(i));
+            return GCFG_ERR_INVALID;
+        }
+        if (flows) flows[i] = st.flow;
+        agg.flow += st.flow;
+        agg.pr_tiles += st.pr_tiles;
+        agg.gr_tiles += st.gr_tiles;
+        agg.pr_iters += st.pr_iters;
+        agg.steps = st.steps > agg.steps ? st.steps : agg.steps;
+        agg.gr_rounds += st.gr_rounds;
+    }
+    agg.ms = now_ms() - t0;
+    if (stats) *stats = agg;
+    return GCFG_OK;
+}
+
+int gcfg_batch_solve_device(gcfg_batch *b, int32_t **d_labels, int64_t *flows, gcfg_stats *stats) {
+    if (!b || !d_labels) {
+        set_error("gcfg_batch_solve_device: null argument");
+        return GCFG_ERR_INVALID;
+    }
+    return batch_run(b, d_labels, flows, stats);
+}
+
+int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_stats *stats) {
+    if (!b) return GCFG_ERR_INVALID;
+    const double t0 = now_ms();
+    int rc = batch_run(b, nullptr, flows, stats);
+    if (rc != GCFG_OK) return rc;
+    if (h_labels) {
+        for (size_t i = 0; i < b->inst.size(); ++i) {
+            if (!h_labels[i]) continue;
+            GCFG_HIP(hipMemcpyAsync(h_labels[i], b->d_labels + b->label_off[i],
+                                    (size_t)b->inst[i].width * b->inst[i].height * sizeof(int32_t),
+                                    hipMemcpyDeviceToHost, b->eng.stream()));
+        }
+        GCFG_HIP(hipStreamSynchronize(b->eng.stream()));
+    }
+    if (stats) stats->ms = now_ms() - t0;
+    return GCFG_OK;
+}
+
+int gcfg_batch_destroy(gcfg_batch *b) {
+    if (!b) return GCFG_OK;
+    b->eng.sync();
+    if (b->d_labels) hipFree(b->d_labels);
+    delete b;
+    return GCFG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+// Colour data term on the device (the stage behind DataTerm.h).
+//
+// Replaces the reference's 256^3 Gaussian-KDE colour model
+// (DataTermKernels.cu:19-150, DataTerm.cpp:28-137) with:
+//   per class k (1 = foreground, 2 = background seeds) and channel c a 16-bin
+//   histogram n_{k,c}[v >> 4] of the seed pixels,
+//   p_k(I) = prod_c (n_{k,c}[I_c >> 4] + 1) / (N_k + 16),
+//   pos = llround(64 * -ln p_bg(I)), neg = llround(64 * -ln p_fg(I)),
+//   beta = sum_{4-neighbour pairs} |I_p - I_q|^2 / #pairs (exact int64 sum),
+//   w = llround(lambda * exp(-|I_p - I_q|^2 / (2 beta)))  (lambda if beta == 0).
+// Device work: one stats pass (LDS histograms + int64 beta sum) and one links
+// pass (t-link gather from a 2 x 4096 LUT, n-link step function from lambda
+// integer thresholds). The LUT and thresholds are built on the host in double
+// from the 96 histogram counts and beta, so the quantised capacities round
+// exactly as a direct per-pixel double evaluation does.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "gcfg.h"
+
+namespace gcfg {
+
+struct DtStats {
+    int32_t hist[2][3][16];
+    int32_t counts[2];
+    unsigned long long beta_sum;
+};
+
+class DtContext {
+   public:
+    ~DtContext();
+    int init(int W, int H, hipStream_t stream, bool own_outputs);
+    // Inputs (device pointers; the image may be owned or borrowed).
+    int upload_image(const uint8_t *h_rgb);
+    void use_image(const uint8_t *d_rgb) { img_ = d_rgb; }
+    int upload_seeds(const uint8_t *h_seeds);
+    int add_seed_rect(int x0, int y0, int w, int h, int cls);
+    int clear_seeds();
+    // Full compute; outputs go to the owned planes or to the given ones.
+    int compute(int32_t *pos, int32_t *neg, int32_t *up, int32_t *down, int32_t *left, int32_t *right);
+    int compute_owned() { return compute(pos_, neg_, up_, down_, left_, right_); }
+    int unquantized(double *h_tpos, double *h_tneg, double *h_right, double *h_down);
+
+    int W = 0, H = 0;
+    int32_t lambda = 50;
+    DtStats stats{};
+    int64_t pairs = 0;
+    double beta = 0.0;
+    bool computed = false;
+    int32_t *pos_ = nullptr, *neg_ = nullptr, *up_ = nullptr, *down_ = nullptr, *left_ = nullptr,
+            *right_ = nullptr;
+    size_t device_bytes() const { return bytes_; }
+
+   private:
+    int build_tables();
+    hipStream_t stream_ = nullptr;
+    const uint8_t *img_ = nullptr;
+    uint8_t *own_img_ = nullptr, *seeds_ = nullptr;
+    DtStats *d_stats_ = nullptr, *h_stats_ = nullptr;
+    int32_t *d_lut_ = nullptr, *h_lut_ = nullptr;  // [2*4096 t-link LUT][thresholds]
+    std::vector<void *> allocs_;
+    size_t bytes_ = 0;
+};
+
+}  // namespace gcfg

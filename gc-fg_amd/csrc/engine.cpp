@@ -320,6 +320,12 @@ int Engine::fetch_state(int slot, InstState *out) {
     return GCFG_OK;
 }
 
+int Engine::fetch_states(InstState *out, int n) {
+    GCFG_HIP(hipMemcpyAsync(out, kp_.state, (size_t)n * sizeof(InstState), hipMemcpyDeviceToHost, stream_));
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    return GCFG_OK;
+}
+
 int Engine::read_state(int slot, int32_t *h_caps, int32_t *h_exc) {
     const InstDesc &d = h_desc_[slot];
     const size_t n = (size_t)d.W * d.H;

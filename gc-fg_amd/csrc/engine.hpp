@@ -135,11 +135,13 @@ class Engine {
     int edit(int slot, const std::vector<int32_t> &strokes, int32_t delta);
     int read_state(int slot, int32_t *h_caps, int32_t *h_exc);
     int fetch_state(int slot, InstState *out);
+    int fetch_states(InstState *out, int n);  // slots [0, n)
     int sync();
 
     hipStream_t stream() const { return stream_; }
     const InstDesc &desc(int slot) const { return h_desc_[slot]; }
     size_t tiles_free() const;
+    int cap_tiles() const { return cap_tiles_; }
     int tiles_for(int W, int H) const { return ((W + TW - 1) / TW) * ((H + TH - 1) / TH); }
     static size_t bytes_per_tile() { return (size_t)TN * (16 + 4 + 4 + 1) + 64; }
     gcfg_options opt;

@@ -1,0 +1,526 @@
+"""Python host binding of libgcfg.so (the MI355X binary-MRF min-cut library).
+
+Mirrors the reference's host API for this path: ``Graph`` follows the
+``GC_*`` functions of ``GraphCut.cu:187-568`` (create = GC_Init allocation,
+``set_dataterms``/``set_edges``/``set_penalty``/``build``/``optimize``/``close``),
+``DataTerm`` follows ``DataTerm.h:9-19``. ``Batch`` and ``Server`` expose the
+batched and serving forms. Device memory is plain HIP memory managed through
+``libamdhip64`` (``DeviceBuffer``); there is no CPU fallback: every solve runs
+the HIP kernels of libgcfg.so and fails loudly if the library or a GPU is
+missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgcfg.so")
+
+GCFG_OK = 0
+GCFG_ERR_INVALID = -1
+GCFG_ERR_HIP = -2
+GCFG_ERR_NOMEM = -3
+GCFG_ERR_STATE = -4
+GCFG_ERR_NOT_CONVERGED = -5
+GCFG_ERR_BUDGET = -6
+SEED_NONE, SEED_FG, SEED_BG = 0, 1, 2
+
+_I32P = ctypes.POINTER(ctypes.c_int32)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+_VP = ctypes.c_void_p
+
+
+class GcfgError(RuntimeError):
+    def __init__(self, rc: int, what: str, msg: str):
+        super().__init__(f"{what} failed ({rc}): {msg}")
+        self.rc = rc
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("sweeps_per_gr", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
+                ("steps_per_poll", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("max_steps", ctypes.c_int32), ("verbose", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("flow", ctypes.c_int64), ("pr_tiles", ctypes.c_int64), ("gr_tiles", ctypes.c_int64),
+                ("pr_iters", ctypes.c_int64), ("steps", ctypes.c_int32), ("gr_rounds", ctypes.c_int32),
+                ("ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Instance(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("d_pos", _VP), ("d_neg", _VP),
+                ("d_up", _VP), ("d_down", _VP), ("d_left", _VP), ("d_right", _VP), ("penalty", ctypes.c_int32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("request_id", ctypes.c_int64), ("session", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("flow", ctypes.c_int64), ("latency_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double)]
+
+
+_lib = None
+_hip = None
+
+
+def lib():
+    """Load libgcfg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        L.gcfg_last_error.restype = ctypes.c_char_p
+        L.gcfg_now_ms.restype = ctypes.c_double
+        L.gcfg_gen_mix_seed.restype = ctypes.c_uint64
+        L.gcfg_gen_mix_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        for name in ("gcfg_gen_ellipse_image", "gcfg_gen_perlin_image"):
+            getattr(L, name).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _VP]
+        L.gcfg_gen_seed_map.argtypes = [ctypes.c_int, ctypes.c_int, _VP]
+        L.gcfg_gen_stress_tlinks.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, _VP, _VP]
+        L.gcfg_gen_batch_shapes.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, _VP,
+                                            _VP]
+        L.gcfg_dt_unquantized.argtypes = [_VP, _VP, _VP, _VP, _VP]
+        L.gcfg_dt_get_beta.argtypes = [_VP, _I64P, _I64P, ctypes.POINTER(ctypes.c_double)]
+        L.gcfg_server_create.argtypes = [ctypes.c_size_t, _VP, _VP]
+        L.gcfg_server_submit_image.argtypes = [_VP, ctypes.c_int32, ctypes.c_int32, _VP, _VP, _VP, _I64P,
+                                               ctypes.POINTER(ctypes.c_int32)]
+        L.gcfg_server_submit_edit.argtypes = [_VP, ctypes.c_int32, ctypes.c_int32, _VP, ctypes.c_int32, _VP, _I64P]
+        L.gcfg_server_load.argtypes = [_VP, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                       ctypes.POINTER(ctypes.c_size_t)]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != GCFG_OK:
+        raise GcfgError(rc, what, lib().gcfg_last_error().decode(errors="replace"))
+
+
+def now_ms() -> float:
+    return float(lib().gcfg_now_ms())
+
+
+def default_options() -> Options:
+    o = Options()
+    lib().gcfg_default_options(ctypes.byref(o))
+    return o
+
+
+# ---------------------------------------------------------------------------
+# HIP device memory (plumbing only).
+def hip():
+    global _hip
+    if _hip is None:
+        h = ctypes.CDLL("libamdhip64.so")
+        h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        h.hipFree.argtypes = [_VP]
+        h.hipMemcpy.argtypes = [_VP, _VP, ctypes.c_size_t, ctypes.c_int]
+        h.hipMemset.argtypes = [_VP, ctypes.c_int, ctypes.c_size_t]
+        h.hipGetErrorString.restype = ctypes.c_char_p
+        h.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        _hip = h
+    return _hip
+
+
+def _hip_check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what}: {hip().hipGetErrorString(rc).decode()}")
+
+
+def gpu_available() -> bool:
+    try:
+        n = ctypes.c_int(0)
+        return hip().hipGetDeviceCount(ctypes.byref(n)) == 0 and n.value > 0
+    except OSError:
+        return False
+
+
+def device_synchronize():
+    _hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class DeviceBuffer:
+    """A hipMalloc'd buffer; ``ptr`` is the device address."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _hip_check(hip().hipMalloc(ctypes.byref(p), max(self.nbytes, 16)), "hipMalloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _hip_check(hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1), "hipMemcpy H2D")
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _hip_check(hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2), "hipMemcpy D2H")
+        return out
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Seeded generators (host-only).
+def gen_ellipse_image(width: int, height: int, seed: int) -> np.ndarray:
+    img = np.empty((height, width, 3), np.uint8)
+    _check(lib().gcfg_gen_ellipse_image(width, height, seed, img.ctypes.data), "gcfg_gen_ellipse_image")
+    return img
+
+
+def gen_perlin_image(width: int, height: int, seed: int) -> np.ndarray:
+    img = np.empty((height, width, 3), np.uint8)
+    _check(lib().gcfg_gen_perlin_image(width, height, seed, img.ctypes.data), "gcfg_gen_perlin_image")
+    return img
+
+
+def gen_seed_map(width: int, height: int) -> np.ndarray:
+    s = np.empty((height, width), np.uint8)
+    _check(lib().gcfg_gen_seed_map(width, height, s.ctypes.data), "gcfg_gen_seed_map")
+    return s
+
+
+def gen_stress_tlinks(width: int, height: int, seed: int):
+    pos = np.empty((height, width), np.int32)
+    neg = np.empty((height, width), np.int32)
+    _check(lib().gcfg_gen_stress_tlinks(width, height, seed, pos.ctypes.data, neg.ctypes.data),
+           "gcfg_gen_stress_tlinks")
+    return pos, neg
+
+
+def gen_batch_shapes(count: int, seed: int, area_min: float, area_max: float):
+    w = np.empty(count, np.int32)
+    h = np.empty(count, np.int32)
+    _check(lib().gcfg_gen_batch_shapes(count, seed, area_min, area_max, w.ctypes.data, h.ctypes.data),
+           "gcfg_gen_batch_shapes")
+    return w, h
+
+
+def mix_seed(seed: int, stream: int) -> int:
+    return int(lib().gcfg_gen_mix_seed(seed, stream))
+
+
+# ---------------------------------------------------------------------------
+class Graph:
+    """The GC_* drop-in: one W x H grid graph on the device."""
+
+    def __init__(self, width: int, height: int, options: Options | None = None):
+        self.width, self.height = int(width), int(height)
+        h = ctypes.c_void_p()
+        _check(lib().gcfg_graph_create(self.width, self.height, ctypes.byref(h)), "gcfg_graph_create")
+        self.h = h.value
+        self._keep = {}
+        if options is not None:
+            self.set_options(options)
+
+    def set_options(self, options: Options):
+        _check(lib().gcfg_graph_set_options(_VP(self.h), ctypes.byref(options)), "gcfg_graph_set_options")
+
+    def _dev(self, name, a):
+        if isinstance(a, DeviceBuffer):
+            self._keep[name] = a
+            return a.ptr
+        if isinstance(a, int):
+            return a
+        buf = DeviceBuffer.from_array(np.ascontiguousarray(a, dtype=np.int32))
+        self._keep[name] = buf
+        return buf.ptr
+
+    def set_dataterms(self, pos, neg):
+        _check(lib().gcfg_graph_set_dataterms(_VP(self.h), _VP(self._dev("pos", pos)), _VP(self._dev("neg", neg))),
+               "gcfg_graph_set_dataterms")
+
+    def set_edges(self, up, down, left, right):
+        ptrs = [_VP(self._dev(n, a)) for n, a in (("up", up), ("down", down), ("left", left), ("right", right))]
+        _check(lib().gcfg_graph_set_edges(_VP(self.h), *ptrs), "gcfg_graph_set_edges")
+
+    def set_penalty(self, penalty: int):
+        _check(lib().gcfg_graph_set_penalty(_VP(self.h), int(penalty)), "gcfg_graph_set_penalty")
+
+    def build(self):
+        _check(lib().gcfg_graph_build(_VP(self.h)), "gcfg_graph_build")
+
+    def optimize(self):
+        label = np.empty((self.height, self.width), np.int32)
+        st = Stats()
+        _check(lib().gcfg_graph_optimize(_VP(self.h), _VP(label.ctypes.data), ctypes.byref(st)), "gcfg_graph_optimize")
+        return label, st
+
+    def optimize_device(self, d_label: DeviceBuffer):
+        st = Stats()
+        _check(lib().gcfg_graph_optimize_device(_VP(self.h), _VP(d_label.ptr), ctypes.byref(st)),
+               "gcfg_graph_optimize_device")
+        return st
+
+    def read_state(self):
+        caps = np.empty((self.height, self.width, 4), np.int32)
+        exc = np.empty((self.height, self.width), np.int32)
+        _check(lib().gcfg_graph_read_state(_VP(self.h), _VP(caps.ctypes.data), _VP(exc.ctypes.data)),
+               "gcfg_graph_read_state")
+        return caps, exc
+
+    def close(self):
+        if self.h:
+            lib().gcfg_graph_destroy(_VP(self.h))
+            self.h = None
+        self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve(width, height, pos, neg, edges=None, penalty=0, options: Options | None = None):
+    """One plain drop-in solve. Returns (flow, labels[H, W], stats)."""
+    g = Graph(width, height, options)
+    try:
+        g.set_dataterms(pos, neg)
+        if edges is not None:
+            g.set_edges(*edges)
+        else:
+            g.set_penalty(penalty)
+        g.build()
+        label, st = g.optimize()
+        return int(st.flow), label, st
+    finally:
+        g.close()
+
+
+# ---------------------------------------------------------------------------
+class DataTerm:
+    """The DataTerm.h stage: seed histograms, t-links and n-links on the device."""
+
+    def __init__(self, width: int, height: int):
+        self.width, self.height = int(width), int(height)
+        h = ctypes.c_void_p()
+        _check(lib().gcfg_dt_create(self.width, self.height, ctypes.byref(h)), "gcfg_dt_create")
+        self.h = h.value
+
+    def set_image(self, rgb: np.ndarray):
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+        assert rgb.size == 3 * self.width * self.height
+        self._img = rgb
+        _check(lib().gcfg_dt_set_image(_VP(self.h), _VP(rgb.ctypes.data)), "gcfg_dt_set_image")
+
+    def set_seeds(self, seeds: np.ndarray):
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint8)
+        self._seeds = seeds
+        _check(lib().gcfg_dt_set_seeds(_VP(self.h), _VP(seeds.ctypes.data)), "gcfg_dt_set_seeds")
+
+    def add_seed_rect(self, x0, y0, w, h, cls):
+        _check(lib().gcfg_dt_add_seed_rect(_VP(self.h), x0, y0, w, h, cls), "gcfg_dt_add_seed_rect")
+
+    def reset(self):
+        _check(lib().gcfg_dt_reset(_VP(self.h)), "gcfg_dt_reset")
+
+    def set_lambda(self, lam: int):
+        _check(lib().gcfg_dt_set_lambda(_VP(self.h), int(lam)), "gcfg_dt_set_lambda")
+
+    def compute(self):
+        _check(lib().gcfg_dt_compute(_VP(self.h)), "gcfg_dt_compute")
+
+    def link_ptrs(self):
+        ps = [ctypes.c_void_p() for _ in range(6)]
+        _check(lib().gcfg_dt_get_links(_VP(self.h), *[ctypes.byref(p) for p in ps]), "gcfg_dt_get_links")
+        return [p.value for p in ps]
+
+    def links(self):
+        """Download pos, neg, up, down, left, right as int32 [H, W] arrays."""
+        n = self.width * self.height
+        out = []
+        for p in self.link_ptrs():
+            a = np.empty(n, np.int32)
+            _hip_check(hip().hipMemcpy(a.ctypes.data, p, a.nbytes, 2), "hipMemcpy D2H")
+            out.append(a.reshape(self.height, self.width))
+        return out
+
+    def histograms(self):
+        h = np.empty(96, np.int32)
+        c = np.empty(2, np.int32)
+        _check(lib().gcfg_dt_get_histograms(_VP(self.h), _VP(h.ctypes.data), _VP(c.ctypes.data)),
+               "gcfg_dt_get_histograms")
+        return h.reshape(2, 3, 16), c
+
+    def beta(self):
+        s, p, b = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        _check(lib().gcfg_dt_get_beta(_VP(self.h), ctypes.byref(s), ctypes.byref(p), ctypes.byref(b)),
+               "gcfg_dt_get_beta")
+        return int(s.value), int(p.value), float(b.value)
+
+    def unquantized(self):
+        n = self.width * self.height
+        outs = [np.empty(n, np.float64) for _ in range(4)]
+        _check(lib().gcfg_dt_unquantized(_VP(self.h), *[_VP(o.ctypes.data) for o in outs]), "gcfg_dt_unquantized")
+        return [o.reshape(self.height, self.width) for o in outs]
+
+    def close(self):
+        if self.h:
+            lib().gcfg_dt_destroy(_VP(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+@dataclass
+class BatchItem:
+    width: int
+    height: int
+    pos: np.ndarray
+    neg: np.ndarray
+    edges: tuple | None = None
+    penalty: int = 0
+
+
+class Batch:
+    """Many instances of uneven size solved together on one engine."""
+
+    def __init__(self, items: list[BatchItem], options: Options | None = None):
+        self.items = items
+        self._bufs = []
+        arr = (Instance * len(items))()
+        for i, it in enumerate(items):
+            d = {k: DeviceBuffer.from_array(np.ascontiguousarray(v, dtype=np.int32))
+                 for k, v in (("pos", it.pos), ("neg", it.neg))}
+            if it.edges is not None:
+                for k, v in zip(("up", "down", "left", "right"), it.edges):
+                    d[k] = DeviceBuffer.from_array(np.ascontiguousarray(v, dtype=np.int32))
+            self._bufs.append(d)
+            arr[i].width, arr[i].height, arr[i].penalty = it.width, it.height, it.penalty
+            arr[i].d_pos, arr[i].d_neg = d["pos"].ptr, d["neg"].ptr
+            if it.edges is not None:
+                arr[i].d_up, arr[i].d_down, arr[i].d_left, arr[i].d_right = (d[k].ptr for k in
+                                                                               ("up", "down", "left", "right"))
+        h = ctypes.c_void_p()
+        _check(lib().gcfg_batch_create(len(items), arr, ctypes.byref(options) if options is not None else None,
+                                       ctypes.byref(h)), "gcfg_batch_create")
+        self.h = h.value
+        self.labels = [DeviceBuffer(4 * it.width * it.height) for it in items]
+
+    def solve(self, labels_to_host: bool = True):
+        n = len(self.items)
+        flows = np.zeros(n, np.int64)
+        st = Stats()
+        if labels_to_host:
+            host = [np.empty((it.height, it.width), np.int32) for it in self.items]
+            ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in host])
+            _check(lib().gcfg_batch_solve(_VP(self.h), ptrs, _VP(flows.ctypes.data), ctypes.byref(st)),
+                   "gcfg_batch_solve")
+            return flows, host, st
+        ptrs = (ctypes.c_void_p * n)(*[b.ptr for b in self.labels])
+        _check(lib().gcfg_batch_solve_device(_VP(self.h), ptrs, _VP(flows.ctypes.data), ctypes.byref(st)),
+               "gcfg_batch_solve_device")
+        return flows, None, st
+
+    def device_labels(self, i: int) -> np.ndarray:
+        it = self.items[i]
+        return self.labels[i].download(np.int32, it.width * it.height).reshape(it.height, it.width)
+
+    def close(self):
+        if self.h:
+            lib().gcfg_batch_destroy(_VP(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+class Server:
+    """Continuous-batching segmentation server with device-resident sessions."""
+
+    def __init__(self, hbm_budget_bytes: int, options: Options | None = None):
+        h = ctypes.c_void_p()
+        _check(lib().gcfg_server_create(int(hbm_budget_bytes), ctypes.byref(options) if options is not None else None,
+                                        ctypes.byref(h)), "gcfg_server_create")
+        self.h = h.value
+        self._pending = {}
+
+    def submit_image(self, rgb: np.ndarray, seeds: np.ndarray):
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint8)
+        H, W = seeds.shape
+        label = np.empty((H, W), np.int32)
+        rid, sess = ctypes.c_int64(), ctypes.c_int32()
+        _check(lib().gcfg_server_submit_image(_VP(self.h), W, H, _VP(rgb.ctypes.data), _VP(seeds.ctypes.data),
+                                              _VP(label.ctypes.data), ctypes.byref(rid), ctypes.byref(sess)),
+               "gcfg_server_submit_image")
+        self._pending[rid.value] = (rgb, seeds, label)
+        return rid.value, sess.value, label
+
+    def submit_edit(self, session: int, strokes, delta: int = 1 << 16, shape=None):
+        st = np.ascontiguousarray(np.asarray(strokes, dtype=np.int32).reshape(-1, 5))
+        if shape is None:
+            raise ValueError("shape (H, W) of the session image is required for the label buffer")
+        label = np.empty(shape, np.int32)
+        rid = ctypes.c_int64()
+        _check(lib().gcfg_server_submit_edit(_VP(self.h), int(session), st.shape[0], _VP(st.ctypes.data), int(delta),
+                                             _VP(label.ctypes.data), ctypes.byref(rid)), "gcfg_server_submit_edit")
+        self._pending[rid.value] = (st, label)
+        return rid.value, label
+
+    def poll(self, wait: bool = False, max_results: int = 256):
+        res = (Result * max_results)()
+        n = ctypes.c_int32()
+        _check(lib().gcfg_server_poll(_VP(self.h), int(wait), max_results, res, ctypes.byref(n)), "gcfg_server_poll")
+        out = []
+        for i in range(n.value):
+            r = res[i]
+            self._pending.pop(r.request_id, None)
+            out.append({k: getattr(r, k) for k, _ in Result._fields_})
+        return out
+
+    def load(self):
+        a, q, b = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_size_t()
+        _check(lib().gcfg_server_load(_VP(self.h), ctypes.byref(a), ctypes.byref(q), ctypes.byref(b)),
+               "gcfg_server_load")
+        return a.value, q.value, b.value
+
+    def close_session(self, session: int):
+        _check(lib().gcfg_server_close_session(_VP(self.h), int(session)), "gcfg_server_close_session")
+
+    def close(self):
+        if self.h:
+            lib().gcfg_server_destroy(_VP(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -125,6 +125,8 @@ typedef struct gcfg_batch gcfg_batch;
 int gcfg_batch_create(int32_t count, const gcfg_instance *inst, const gcfg_options *opt, gcfg_batch **out);
 /* Cold solve of every instance; labels to host (h_labels[i]: W_i*H_i) and flows. */
 int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_stats *stats);
+/* Cold solve with labels written to caller-owned device buffers (d_labels[i]: W_i*H_i). */
+int gcfg_batch_solve_device(gcfg_batch *b, int32_t **d_labels, int64_t *flows, gcfg_stats *stats);
 int gcfg_batch_destroy(gcfg_batch *b);
 
 /* ---- Serving: continuous batching under an HBM budget, warm-started edits ---- */
@@ -136,7 +138,10 @@ typedef struct gcfg_result {
     int64_t flow;
     double latency_ms;   /* submit -> labels in host memory */
     double solve_ms;     /* admission -> labels in host memory */
+    double done_ms;      /* completion time on the gcfg_now_ms() clock */
 } gcfg_result;
+/* Monotonic clock (ms) shared with gcfg_result.done_ms. */
+double gcfg_now_ms(void);
 int gcfg_server_create(size_t hbm_budget_bytes, const gcfg_options *opt, gcfg_server **out);
 /* New session: image + seed map (host), data term and solve on the device.
  * h_label (W*H int32, host) receives the labels; it must stay valid until the

@@ -1,0 +1,168 @@
+"""Plain drop-in solver (GC_* path) on the GPU vs the exact oracle.
+
+Bar: bit-exact flow value and bit-exact label map (the "cannot reach the
+sink" set, unique over all maximum flows) on identical capacities.
+"""
+import numpy as np
+import pytest
+
+from conftest import debug_kat, glibc_rand_instance, random_instance
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(g, ora, W, H, pos, neg, edges=None, penalty=0, options=None):
+    flow, label, st = g.solve(W, H, pos, neg, edges=edges, penalty=penalty, options=options)
+    rflow, rlabel = ora.maxflow(W, H, pos, neg, edges=edges, penalty=penalty)
+    assert flow == rflow, (W, H, flow, rflow)
+    assert np.array_equal(label, rlabel), (W, H, int((label != rlabel).sum()))
+    assert ora.cut_value(W, H, pos, neg, label, edges=edges, penalty=penalty) == flow
+    return st
+
+
+def test_debug_kat(g, ora):
+    W, H, pos, neg = debug_kat()
+    flow, label, _ = g.solve(W, H, pos, neg, penalty=50)
+    assert flow == 7
+    assert label.sum() == 0
+
+
+@pytest.mark.parametrize("shape", [(3, 3), (5, 4), (17, 9), (63, 15), (64, 16), (65, 17), (130, 33), (200, 70),
+                                   (31, 120)])
+def test_random_small(g, ora, shape):
+    W, H = shape
+    rng = np.random.default_rng(W * 1000 + H)
+    for trial in range(3):
+        pos, neg, E = random_instance(rng, W, H)
+        _check(g, ora, W, H, pos, neg, edges=E)
+
+
+def test_degenerate_shapes(g, ora):
+    for W, H in ((1, 1), (2, 7), (9, 2), (3, 1)):
+        pos = np.full((H, W), 5, np.int32)
+        neg = np.zeros((H, W), np.int32)
+        flow, label, _ = g.solve(W, H, pos, neg, penalty=3)
+        assert flow == 0 and label.sum() == 0
+
+
+def test_penalty_mode(g, ora):
+    rng = np.random.default_rng(3)
+    W, H = 150, 90
+    pos, neg, _ = random_instance(rng, W, H, tmax=300, edges=False)
+    _check(g, ora, W, H, pos, neg, penalty=50)
+
+
+def test_debug_random_400(g, ora):
+    """Debug.cu:60-74 random instance (glibc rand, seed 1), penalty 50."""
+    pos, neg = glibc_rand_instance(1)
+    _check(g, ora, 400, 400, pos, neg, penalty=50)
+
+
+def test_heavy_tailed(g, ora):
+    W, H = 300, 200
+    pos, neg = g.gen_stress_tlinks(W, H, 11)
+    img = g.gen_perlin_image(W, H, 12)
+    dt = g.DataTerm(W, H)
+    dt.set_image(img)
+    dt.set_seeds(np.zeros((H, W), np.uint8))
+    dt.compute()
+    _, _, up, down, left, right = dt.links()
+    _check(g, ora, W, H, pos, neg, edges=(up, down, left, right))
+
+
+def test_ellipse_c0(g, ora):
+    """320x240 ellipse image through the device data term, then the plain solver."""
+    W, H = 320, 240
+    img = g.gen_ellipse_image(W, H, 1)
+    seeds = g.gen_seed_map(W, H)
+    dt = g.DataTerm(W, H)
+    dt.set_image(img)
+    dt.set_seeds(seeds)
+    dt.compute()
+    pos, neg, up, down, left, right = dt.links()
+    st = _check(g, ora, W, H, pos, neg, edges=(up, down, left, right))
+    assert st.steps > 0
+
+
+@pytest.mark.parametrize("opts", [dict(sweeps_per_gr=1, inner_iters=1), dict(sweeps_per_gr=2, inner_iters=8),
+                                  dict(sweeps_per_gr=32, inner_iters=256, steps_per_poll=3, use_graph=0)])
+def test_options_do_not_change_result(g, ora, opts):
+    rng = np.random.default_rng(99)
+    W, H = 140, 75
+    pos, neg, E = random_instance(rng, W, H, tmax=200, emax=60)
+    o = g.default_options()
+    for k, v in opts.items():
+        setattr(o, k, v)
+    _check(g, ora, W, H, pos, neg, edges=E, options=o)
+
+
+def test_negative_capacity_rejected(g):
+    W, H = 20, 20
+    pos = np.zeros((H, W), np.int32)
+    neg = np.zeros((H, W), np.int32)
+    pos[5, 5] = -3
+    with pytest.raises(g.GcfgError):
+        g.solve(W, H, pos, neg, penalty=1)
+
+
+def test_optimize_before_build_is_state_error(g):
+    gr = g.Graph(10, 10)
+    with pytest.raises(g.GcfgError) as e:
+        gr.optimize()
+    assert e.value.rc == g.GCFG_ERR_STATE
+    gr.close()
+
+
+def test_residual_certificate(g, ora):
+    """The returned residual state is a feasible preflow whose value equals the
+    cut value of the returned labels, and the labels are exactly the nodes that
+    cannot reach a node with remaining sink capacity."""
+    rng = np.random.default_rng(5)
+    W, H = 190, 130
+    pos, neg, E = random_instance(rng, W, H, tmax=100, emax=40)
+    gr = g.Graph(W, H)
+    gr.set_dataterms(pos, neg)
+    gr.set_edges(*E)
+    gr.build()
+    label, st = gr.optimize()
+    caps, exc = gr.read_state()
+    gr.close()
+    _verify_certificate(W, H, pos, neg, E, caps, exc, label, st.flow, ora)
+
+
+def _verify_certificate(W, H, pos, neg, E, caps, exc, label, flow, ora):
+    up, down, left, right = E
+    inter = np.zeros((H, W), bool)
+    inter[1:-1, 1:-1] = True
+    # original capacities with the border rule
+    c = np.zeros((H, W, 4), np.int64)
+    c[..., 0], c[..., 1], c[..., 2], c[..., 3] = left, right, up, down
+    c[~inter] = 0
+    c[:, 1, 0] = 0
+    c[:, W - 2, 1] = 0
+    c[1, :, 2] = 0
+    c[H - 2, :, 3] = 0
+    r = caps.astype(np.int64)
+    assert (r >= 0).all()
+    # antisymmetric flow: r(u->v) + r(v->u) == c(u->v) + c(v->u)
+    assert np.array_equal(r[:, 1:, 0] + r[:, :-1, 1], c[:, 1:, 0] + c[:, :-1, 1])
+    assert np.array_equal(r[1:, :, 2] + r[:-1, :, 3], c[1:, :, 2] + c[:-1, :, 3])
+    # conservation: e = pos - neg - net outflow
+    net = (c - r).sum(axis=2)
+    e_expect = np.where(inter, pos.astype(np.int64) - neg - net, 0)
+    assert np.array_equal(exc.astype(np.int64), e_expect)
+    assert flow == int(pos[inter].sum()) - int(np.maximum(exc[inter], 0).sum())
+    assert ora.cut_value(W, H, pos, neg, label, edges=E) == flow
+    # labels == cannot reach a node with e < 0 through residual arcs
+    reach = inter & (exc < 0)
+    while True:
+        nxt = reach.copy()
+        nxt[:, 1:] |= (r[:, 1:, 0] > 0) & reach[:, :-1]
+        nxt[:, :-1] |= (r[:, :-1, 1] > 0) & reach[:, 1:]
+        nxt[1:, :] |= (r[1:, :, 2] > 0) & reach[:-1, :]
+        nxt[:-1, :] |= (r[:-1, :, 3] > 0) & reach[1:, :]
+        nxt &= inter
+        if np.array_equal(nxt, reach):
+            break
+        reach = nxt
+    assert np.array_equal(label.astype(bool), inter & ~reach)

@@ -1,0 +1,158 @@
+"""Pins the CPU oracle's max-flow / label semantics (CPU only).
+
+Checks chosen so that plausible mistakes fail: the reference's own known
+answer (Debug.cu:24-58), exhaustive enumeration of every cut on tiny grids
+(min value AND the union of all minimum source sets, which is the unique
+"cannot reach the sink" set), an independent library max-flow (networkx),
+the reference border rule, and a transposition / direction check.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import debug_kat, glibc_rand_instance, random_instance
+
+
+def brute_force(W, H, pos, neg, E):
+    """Enumerate all labellings of the interior; return (min cut, union of optimal source sets)."""
+    up, down, left, right = E
+    nodes = [(x, y) for y in range(1, H - 1) for x in range(1, W - 1)]
+    idx = {p: i for i, p in enumerate(nodes)}
+    n = len(nodes)
+    labs = np.array(list(itertools.product([0, 1], repeat=n)), dtype=np.int64)  # 2^n x n
+    cost = np.zeros(len(labs), np.int64)
+    for i, (x, y) in enumerate(nodes):
+        s = labs[:, i]
+        cost += np.where(s == 1, neg[y, x], pos[y, x])
+        for (dx, dy, cap) in ((-1, 0, left), (1, 0, right), (0, -1, up), (0, 1, down)):
+            q = (x + dx, y + dy)
+            if q in idx:
+                cost += (s == 1) * (labs[:, idx[q]] == 0) * int(cap[y, x])
+    best = cost.min()
+    union = labs[cost == best].max(axis=0)
+    lab = np.zeros((H, W), np.int32)
+    for i, (x, y) in enumerate(nodes):
+        lab[y, x] = union[i]
+    return int(best), lab
+
+
+def test_debug_cu_known_answer(ora):
+    W, H, pos, neg = debug_kat()
+    flow, label = ora.maxflow(W, H, pos, neg, penalty=50)
+    assert flow == 7
+    assert label.sum() == 0
+
+
+def test_reference_printout_emulation(ora):
+    """The reference prints 4 for the Debug.cu instance (limited EnergyLevel bug, W % 32 != 0)."""
+    W, H, pos, neg = debug_kat()
+    flow, label, iters, printed = ora.refpr(W, H, pos, neg, penalty=50)
+    assert flow == 7 and label.sum() == 0
+    assert printed == 4
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_brute_force_tiny(ora, seed):
+    rng = np.random.default_rng(seed)
+    W, H = [(4, 4), (5, 4), (4, 6), (6, 5), (3, 7), (5, 5)][seed % 6]
+    pos, neg, E = random_instance(rng, W, H, tmax=12, emax=8)
+    # make ties likely so the union-of-optima semantics is exercised
+    if seed % 2:
+        pos[::2] = neg[::2]
+    best, blab = brute_force(W, H, pos, neg, E)
+    flow, label = ora.maxflow(W, H, pos, neg, edges=E)
+    assert flow == best
+    assert np.array_equal(label, blab)
+
+
+def test_against_networkx(ora):
+    nx = pytest.importorskip("networkx")
+    rng = np.random.default_rng(7)
+    for trial in range(20):
+        W, H = int(rng.integers(3, 14)), int(rng.integers(3, 14))
+        pos, neg, E = random_instance(rng, W, H, tmax=60, emax=30)
+        up, down, left, right = E
+        G = nx.DiGraph()
+        G.add_node("s")
+        G.add_node("t")
+        for y in range(1, H - 1):
+            for x in range(1, W - 1):
+                G.add_edge("s", (x, y), capacity=int(pos[y, x]))
+                G.add_edge((x, y), "t", capacity=int(neg[y, x]))
+                for dx, dy, cap in ((-1, 0, left), (1, 0, right), (0, -1, up), (0, 1, down)):
+                    if 1 <= x + dx <= W - 2 and 1 <= y + dy <= H - 2:
+                        G.add_edge((x, y), (x + dx, y + dy), capacity=int(cap[y, x]))
+        fv, (S, T) = nx.minimum_cut(G, "s", "t")
+        flow, label = ora.maxflow(W, H, pos, neg, edges=E)
+        assert flow == fv
+        assert ora.cut_value(W, H, pos, neg, label, edges=E) == fv
+
+
+def test_border_is_not_part_of_the_graph(ora):
+    """Huge t-links on border pixels change nothing (GraphCutKernels.cu:122-146)."""
+    rng = np.random.default_rng(1)
+    W, H = 9, 7
+    pos, neg, E = random_instance(rng, W, H)
+    f0, l0 = ora.maxflow(W, H, pos, neg, edges=E)
+    pos2, neg2 = pos.copy(), neg.copy()
+    pos2[0, :] = 10 ** 6
+    neg2[:, -1] = 10 ** 6
+    E2 = tuple(e.copy() for e in E)
+    E2[0][1, :] = 10 ** 6  # up from row 1 points into the border
+    E2[1][H - 2, :] = 10 ** 6  # down from row H-2 points into the border
+    f1, l1 = ora.maxflow(W, H, pos2, neg2, edges=E2)
+    assert f0 == f1 and np.array_equal(l0, l1)
+    assert l1[0].sum() == 0 and l1[:, -1].sum() == 0
+
+
+def test_directions_are_not_transposed(ora):
+    """A single strong arc in one direction: only the correct orientation carries flow."""
+    W, H = 5, 3  # interior (1,1) (2,1) (3,1)
+    pos = np.zeros((H, W), np.int32)
+    neg = np.zeros((H, W), np.int32)
+    pos[1, 1] = 10
+    neg[1, 2] = 10
+    z = np.zeros((H, W), np.int32)
+    right = z.copy()
+    right[1, 1] = 7  # (1,1) -> (2,1)
+    f, lab = ora.maxflow(W, H, pos, neg, edges=(z, z, z, right))
+    assert f == 7
+    left = z.copy()
+    left[1, 1] = 7  # (1,1) -> (0,1) is the border: no flow
+    f2, _ = ora.maxflow(W, H, pos, neg, edges=(z, z, left, z))
+    assert f2 == 0
+    left2 = z.copy()
+    left2[1, 2] = 7  # (2,1) -> (1,1): wrong direction for s->(1,1)->(2,1)->t
+    f3, _ = ora.maxflow(W, H, pos, neg, edges=(z, z, left2, z))
+    assert f3 == 0
+
+
+def test_degenerate_and_invalid(ora):
+    for W, H in ((1, 1), (2, 5), (5, 2)):
+        f, lab = ora.maxflow(W, H, np.ones((H, W)), np.zeros((H, W)), penalty=1)
+        assert f == 0 and lab.sum() == 0
+    pos = np.zeros((5, 5), np.int32)
+    pos[2, 2] = -1
+    with pytest.raises(RuntimeError):
+        ora.maxflow(5, 5, pos, np.zeros((5, 5)), penalty=1)
+
+
+def test_reference_restatement_matches_oracle(ora):
+    rng = np.random.default_rng(11)
+    for trial in range(6):
+        W, H = int(rng.integers(3, 70)), int(rng.integers(3, 40))
+        pos, neg, E = random_instance(rng, W, H, tmax=200, emax=60)
+        f, l = ora.maxflow(W, H, pos, neg, edges=E)
+        fr, lr, it, _ = ora.refpr(W, H, pos, neg, edges=E)
+        assert fr == f and np.array_equal(lr, l)
+
+
+def test_debug_cu_random_instance(ora):
+    """Debug.cu:60-74 (glibc rand, seed 1, 400x400, penalty 50): oracle and the
+    reference-schedule restatement agree, and the cut certifies the flow."""
+    pos, neg = glibc_rand_instance(1)
+    f, l = ora.maxflow(400, 400, pos, neg, penalty=50)
+    assert ora.cut_value(400, 400, pos, neg, l, penalty=50) == f
+    fr, lr, it, _ = ora.refpr(400, 400, pos, neg, penalty=50, threads=4)
+    assert fr == f and np.array_equal(lr, l)
