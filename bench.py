@@ -1,0 +1,333 @@
+#!/usr/bin/env python3
+"""Benchmark: exact min-cut throughput on one MI355X.
+
+Workload (the batch the headline metric is quoted on): 256 instances from the
+ellipse-image generator, areas log-uniform on [128^2, 1024^2], aspect ratios
+log-uniform on [1/2, 2], 16-bin colour data term with lambda = 50 computed on
+the device (untimed setup; timed separately as `dataterm`).
+
+A step is one cold solve of the whole batch: inputs resident in HBM ->
+residual graphs built -> max-flow -> labels in HBM and int64 flows on the host.
+`value` is batched Mpx/s (pixels = W*H of every instance, border included).
+Reported beside it:
+  * plain: the GC_* drop-in (GC_SetGraph + GC_Optimize, labels left on the
+    device) looped over the same instances one at a time, and the ratio;
+  * serve: continuous-batching server on a Poisson stream (sizes
+    log-uniform on [256^2, 2048^2], 30% seed-stroke edits warm-started on
+    device-resident residual graphs): p50/p99 latency, Mpx/s;
+  * roofline of the dominant kernel from an event-timed solve of the same batch;
+  * cpu_baseline: the CPU restatement of the reference push-relabel schedule
+    on a bounded sample of the same instances, with its results checked
+    against the GPU's.
+Every run checks that batched, plain and served flows agree bit-exactly.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "gc-fg_amd")]
+
+import numpy as np  # noqa: E402
+
+import gcfg  # noqa: E402
+
+METRIC = "min-cut Mpixels/s (bit-exact) batched+served vs plain on 1 GPU; p99 latency"
+HBM_PEAK_GBS = 8000.0
+PR_BYTES_PER_NODE = 48   # caps 16 + excess 4 + distance 4, read and written, per tile pass
+GR_BYTES_PER_NODE = 12   # distance read + write, residual mask read, per BFS tile sweep
+COMPULSORY_BYTES_PER_PX = 28  # 6 int32 inputs read + int32 label written
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class Inst:
+    def __init__(self, W, H, seed):
+        self.W, self.H, self.seed = int(W), int(H), seed
+        self.bufs = None
+        self.d_label = None
+
+
+def make_batch(n, seed):
+    Ws, Hs = gcfg.gen_batch_shapes(n, seed, 128.0 ** 2, 1024.0 ** 2)
+    insts = []
+    dt_ms = 0.0
+    for i in range(n):
+        it = Inst(Ws[i], Hs[i], gcfg.mix_seed(seed, i))
+        img = gcfg.gen_ellipse_image(it.W, it.H, it.seed)
+        seeds = gcfg.gen_seed_map(it.W, it.H)
+        dt = gcfg.DataTerm(it.W, it.H)
+        dt.set_image(img)
+        dt.set_seeds(seeds)
+        t0 = time.perf_counter()
+        dt.compute()
+        dt_ms += (time.perf_counter() - t0) * 1e3
+        nbytes = 4 * it.W * it.H
+        it.bufs = []
+        for p in dt.link_ptrs():
+            b = gcfg.DeviceBuffer(nbytes)
+            b.copy_from_device(p, nbytes)
+            it.bufs.append(b)
+        dt.close()
+        it.d_label = gcfg.DeviceBuffer(nbytes)
+        insts.append(it)
+    return insts, dt_ms
+
+
+def batch_items(insts):
+    return [gcfg.BatchItem(it.W, it.H, it.bufs[0], it.bufs[1], edges=tuple(it.bufs[2:6])) for it in insts]
+
+
+def host_links(it):
+    n = it.W * it.H
+    return [b.download(np.int32, n).reshape(it.H, it.W) for b in it.bufs]
+
+
+def run_plain(insts, graphs):
+    flows = []
+    t0 = time.perf_counter()
+    for it, g in zip(insts, graphs):
+        g.build()
+        st = g.optimize_device(it.d_label)
+        flows.append(st.flow)
+    gcfg.device_synchronize()
+    return (time.perf_counter() - t0) * 1e3, np.array(flows, np.int64)
+
+
+def serve(args, opts):
+    """Poisson stream against the continuous-batching server."""
+    rng = np.random.default_rng(args.seed + 7)
+    n = args.serve_requests
+    amin, amax = 256.0 ** 2, 2048.0 ** 2
+    reqs = []
+    sessions = []  # indices of image requests
+    for i in range(n):
+        if sessions and rng.random() < 0.3:
+            j = int(rng.choice(sessions))
+            W, H = reqs[j]["W"], reqs[j]["H"]
+            k = int(rng.integers(1, 6))
+            strokes = []
+            for _ in range(k):
+                x0 = int(rng.integers(0, max(1, W - 16)))
+                y0 = int(rng.integers(0, max(1, H - 16)))
+                strokes.append([x0, y0, 16, 16, int(rng.integers(1, 3))])
+            reqs.append(dict(kind="edit", target=j, W=W, H=H, strokes=np.array(strokes, np.int32)))
+        else:
+            a = math.exp(math.log(amin) + rng.random() * (math.log(amax) - math.log(amin)))
+            r = math.exp(math.log(0.5) + rng.random() * math.log(4.0))
+            W, H = int(round(math.sqrt(a * r))), int(round(math.sqrt(a / r)))
+            img = gcfg.gen_ellipse_image(W, H, gcfg.mix_seed(args.seed + 7, i))
+            reqs.append(dict(kind="image", W=W, H=H, img=img, seeds=gcfg.gen_seed_map(W, H)))
+            sessions.append(i)
+    gaps = rng.exponential(1000.0 / args.qps, n)
+    arrival = np.cumsum(gaps) - gaps[0]
+    srv = gcfg.Server(args.serve_budget_gb << 30, opts)
+    sess_of = {}
+    rid_of = {}
+    lat = {}
+    flows = {}
+    t0 = gcfg.now_ms()
+    i = 0
+    done = 0
+    while done < n:
+        now = gcfg.now_ms() - t0
+        while i < n and arrival[i] <= now:
+            q = reqs[i]
+            if q["kind"] == "image":
+                rid, sess, _ = srv.submit_image(q["img"], q["seeds"])
+                sess_of[i] = sess
+            else:
+                rid, _ = srv.submit_edit(sess_of[q["target"]], q["strokes"], 1 << 16, shape=(q["H"], q["W"]))
+            rid_of[rid] = i
+            i += 1
+        for r in srv.poll(wait=False):
+            k = rid_of[r["request_id"]]
+            if r["status"] != 0:
+                raise RuntimeError(f"serving request {k} failed with status {r['status']}")
+            lat[k] = r["done_ms"] - (t0 + arrival[k])
+            flows[k] = r["flow"]
+            done += 1
+        inflight, queued, _ = srv.load()
+        if inflight == 0 and queued == 0 and i < n:
+            wait = arrival[i] - (gcfg.now_ms() - t0)
+            if wait > 0:
+                time.sleep(wait / 1e3)
+    wall = gcfg.now_ms() - t0
+    srv.close()
+    px = sum(q["W"] * q["H"] for q in reqs)
+    L = np.array([lat[k] for k in range(n)])
+    return dict(requests=n, qps=args.qps, edits=sum(q["kind"] == "edit" for q in reqs),
+                p50_ms=round(float(np.percentile(L, 50)), 3), p99_ms=round(float(np.percentile(L, 99)), 3),
+                mean_ms=round(float(L.mean()), 3), mpx_s=round(px / (wall * 1e3), 2), wall_ms=round(wall, 1),
+                hbm_budget_gb=args.serve_budget_gb), reqs, flows
+
+
+def cpu_baseline(insts, flows_gpu, labels_gpu_fn, budget_s):
+    """Reference-schedule restatement on the host cores over a bounded sample."""
+    import oracle
+
+    threads = min(16, os.cpu_count() or 1)
+    order = sorted(range(len(insts)), key=lambda k: insts[k].W * insts[k].H)
+    px = 0
+    secs = 0.0
+    used = []
+    match = True
+    for k in order[:: max(1, len(order) // 64)]:
+        it = insts[k]
+        pos, neg, up, down, left, right = host_links(it)
+        t0 = time.perf_counter()
+        f, lab, iters, _ = oracle.refpr(it.W, it.H, pos, neg, edges=(up, down, left, right), threads=threads)
+        secs += time.perf_counter() - t0
+        px += it.W * it.H
+        used.append(k)
+        match &= (f == int(flows_gpu[k])) and bool(np.array_equal(lab, labels_gpu_fn(k)))
+        if secs > budget_s:
+            break
+    return dict(value=round(px / secs / 1e6, 3), unit="Mpx/s", cores=threads, kind="port",
+                sample=f"{len(used)} of the {len(insts)} batch instances ({px / 1e6:.1f} Mpx), "
+                       f"reference push-relabel schedule restated in C/OpenMP",
+                seconds=round(secs, 2), matches_gpu=bool(match))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--qps", type=float, default=40.0)
+    ap.add_argument("--serve-requests", type=int, default=120)
+    ap.add_argument("--serve-budget-gb", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--sweeps", type=int, default=0)
+    ap.add_argument("--inner", type=int, default=0)
+    ap.add_argument("--poll", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is not sharded across GPUs",
+              file=sys.stderr)
+        sys.exit(2)
+    if not gcfg.gpu_available():
+        raise SystemExit("no GPU visible: bench.py needs an MI355X")
+
+    opts = gcfg.default_options()
+    if args.sweeps:
+        opts.sweeps_per_gr = args.sweeps
+    if args.inner:
+        opts.inner_iters = args.inner
+    if args.poll:
+        opts.steps_per_poll = args.poll
+
+    log(f"generating {args.instances} instances + device data terms")
+    insts, dt_ms = make_batch(args.instances, args.seed)
+    total_px = sum(it.W * it.H for it in insts)
+    log(f"batch: {total_px / 1e6:.2f} Mpx, data term {dt_ms:.1f} ms")
+
+    batch = gcfg.Batch(batch_items(insts), opts)
+    for _ in range(args.warmup):
+        flows_b, _, _ = batch.solve(labels_to_host=False)
+    gcfg.device_synchronize()
+    times = []
+    st = None
+    for s in range(args.steps):
+        gcfg.device_synchronize()
+        t0 = time.perf_counter()
+        flows_b, _, st = batch.solve(labels_to_host=False)
+        gcfg.device_synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+        log(f"batched step {s}: {times[-1]:.2f} ms, {total_px / times[-1] / 1e3:.1f} Mpx/s, steps {st.steps}")
+    ms = float(np.mean(times))
+    value = total_px / ms / 1e3
+
+    # Event-timed solve of the same batch for the roofline.
+    popts = gcfg.Options.from_buffer_copy(opts)
+    popts.profile = 1
+    pbatch = gcfg.Batch(batch_items(insts), popts)
+    flows_p, _, pst = pbatch.solve(labels_to_host=False)
+    pr_bytes = pst.pr_tiles * 1024 * PR_BYTES_PER_NODE
+    gr_bytes = pst.gr_tiles * 1024 * GR_BYTES_PER_NODE
+    kern = {"k_pr": (pst.pr_kernel_ms, pr_bytes), "k_gr": (pst.gr_kernel_ms, gr_bytes)}
+    dom = max(kern, key=lambda k: kern[k][0])
+    dms, dbytes = kern[dom]
+    achieved = dbytes / (dms * 1e-3) / 1e9 if dms > 0 else 0.0
+    solve_bytes = COMPULSORY_BYTES_PER_PX * total_px + pr_bytes + gr_bytes
+    roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    avg_launch_us=round(dms * 1e3 / max(pst.launches, 1), 2),
+                    bytes_per_launch=int(dbytes / max(pst.launches, 1)), launches=int(pst.launches),
+                    pr_kernel_ms=round(pst.pr_kernel_ms, 3), gr_kernel_ms=round(pst.gr_kernel_ms, 3),
+                    sched_kernel_ms=round(pst.sched_kernel_ms, 3),
+                    solve_alg_frac=round(solve_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    compulsory_frac=round(COMPULSORY_BYTES_PER_PX * total_px / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                    pr_tiles=int(pst.pr_tiles), gr_tiles=int(pst.gr_tiles))
+    assert np.array_equal(flows_p, flows_b), "profiled solve differs"
+    pbatch.close()
+
+    extra = {}
+    if not args.no_plain:
+        graphs = []
+        for it in insts:
+            g = gcfg.Graph(it.W, it.H, opts)
+            g.set_dataterms(it.bufs[0], it.bufs[1])
+            g.set_edges(*it.bufs[2:6])
+            g.build()
+            g.optimize_device(it.d_label)
+            graphs.append(g)
+        plain_ms, flows_plain = run_plain(insts, graphs)
+        log(f"plain pass: {plain_ms:.1f} ms, {total_px / plain_ms / 1e3:.2f} Mpx/s")
+        assert np.array_equal(flows_plain, flows_b), "plain and batched flows differ"
+        for k in range(0, len(insts), max(1, len(insts) // 8)):
+            it = insts[k]
+            a = it.d_label.download(np.int32, it.W * it.H)
+            b = batch.device_labels(k).ravel()
+            assert np.array_equal(a, b), f"plain and batched labels differ on instance {k}"
+        extra["plain_mpx_s"] = round(total_px / plain_ms / 1e3, 3)
+        extra["plain_ms"] = round(plain_ms, 2)
+        extra["speedup_vs_plain"] = round(value / (total_px / plain_ms / 1e3), 2)
+        for g in graphs:
+            g.close()
+
+    if not args.no_serve:
+        log(f"serving {args.serve_requests} requests at {args.qps} qps")
+        sv, reqs, sflows = serve(args, opts)
+        extra["serve"] = sv
+        log(f"serve: {sv}")
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (reference schedule restated on the host)")
+        cpu = cpu_baseline(insts, flows_b, lambda k: batch.device_labels(k), args.cpu_seconds)
+        log(f"cpu: {cpu}")
+    batch.close()
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Mpx/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": f"batch of {args.instances} ellipse images, area log-uniform 128^2..1024^2, "
+                               f"aspect 1:2..2:1, 16-bin colour data term, lambda 50, cold solves",
+                   "instances": args.instances, "total_mpx": round(total_px / 1e6, 3), "seed": args.seed,
+                   "sweeps_per_gr": opts.sweeps_per_gr, "inner_iters": opts.inner_iters,
+                   "steps_per_poll": opts.steps_per_poll},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "dataterm_mpx_s": round(total_px / dt_ms / 1e3, 2),
+        "scheduler_steps": int(st.steps) if st is not None else None,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

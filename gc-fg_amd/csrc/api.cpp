@@ -46,6 +46,7 @@ void gcfg_default_options(gcfg_options *o) {
     o->use_graph = 1;
     o->max_steps = 200000;
     o->verbose = 0;
+    o->profile = 0;
 }
 
 const char *gcfg_last_error(void) { return gcfg::last_error_cstr(); }
@@ -166,9 +167,11 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
         // The label destination is part of the uploaded descriptor.
         g->label_target = d_target;
         int rc = graph_push_inputs(g);
-        if (rc == GCFG_OK) rc = g->eng.build({g->slot});
+        if (rc == GCFG_OK) rc = g->eng.upload_desc();
         if (rc != GCFG_OK) return rc;
     }
+    g->eng.reset_profile();
+    const int64_t steps0 = g->eng.steps_run;
     int rc = g->eng.run(g->opt.max_steps);
     if (rc != GCFG_OK) return rc;
     InstState st;
@@ -187,6 +190,10 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
         stats->steps = st.steps;
         stats->gr_rounds = st.gr_rounds;
         stats->ms = now_ms() - t0;
+        stats->pr_kernel_ms = g->eng.prof_pr_ms;
+        stats->gr_kernel_ms = g->eng.prof_gr_ms;
+        stats->sched_kernel_ms = g->eng.prof_sched_ms;
+        stats->launches = g->eng.steps_run - steps0;
     }
     return GCFG_OK;
 }
@@ -293,6 +300,8 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
         b->eng.set_inputs(b->slots[i], in.d_pos, in.d_neg, in.d_up, in.d_down, in.d_left, in.d_right, in.penalty,
                           lab);
     }
+    b->eng.reset_profile();
+    const int64_t steps0 = b->eng.steps_run;
     int rc = b->eng.build(b->slots);
     if (rc == GCFG_OK) rc = b->eng.run(b->eng.opt.max_steps);
     if (rc != GCFG_OK) return rc;
@@ -315,6 +324,10 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
         agg.gr_rounds += st.gr_rounds;
     }
     agg.ms = now_ms() - t0;
+    agg.pr_kernel_ms = b->eng.prof_pr_ms;
+    agg.gr_kernel_ms = b->eng.prof_gr_ms;
+    agg.sched_kernel_ms = b->eng.prof_sched_ms;
+    agg.launches = b->eng.steps_run - steps0;
     if (stats) *stats = agg;
     return GCFG_OK;
 }

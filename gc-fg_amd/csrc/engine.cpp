@@ -30,6 +30,7 @@ Engine::~Engine() {
     if (stream_) hipStreamSynchronize(stream_);
     if (graph_exec_) hipGraphExecDestroy(graph_exec_);
     if (graph_) hipGraphDestroy(graph_);
+    for (hipEvent_t e : events_) hipEventDestroy(e);
     for (void *p : allocs_) hipFree(p);
     if (pin_desc_) hipHostFree(pin_desc_);
     if (pin_slots_) hipHostFree(pin_slots_);
@@ -221,6 +222,14 @@ int Engine::build(const std::vector<int> &slots) {
     return GCFG_OK;
 }
 
+int Engine::upload_desc() {
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(pin_desc_, h_desc_.data(), (size_t)cap_slots_ * sizeof(InstDesc));
+    GCFG_HIP(hipMemcpyAsync(kp_.desc, pin_desc_, (size_t)cap_slots_ * sizeof(InstDesc), hipMemcpyHostToDevice,
+                            stream_));
+    return GCFG_OK;
+}
+
 int Engine::rearm(const std::vector<int> &slots) {
     if (slots.empty()) return GCFG_OK;
     GCFG_HIP(hipStreamSynchronize(stream_));
@@ -278,6 +287,28 @@ int Engine::capture_graph() {
 }
 
 int Engine::launch_steps(int n) {
+    if (opt.profile) {
+        if ((int)events_.size() < 4 * n) {
+            for (int i = (int)events_.size(); i < 4 * n; ++i) {
+                hipEvent_t e;
+                GCFG_HIP(hipEventCreate(&e));
+                events_.push_back(e);
+            }
+        }
+        for (int i = 0; i < n; ++i) {
+            GCFG_HIP(hipEventRecord(events_[4 * i], stream_));
+            launch_sched(kp_, stream_);
+            GCFG_HIP(hipEventRecord(events_[4 * i + 1], stream_));
+            launch_pr(kp_, grid_pr_, stream_);
+            GCFG_HIP(hipEventRecord(events_[4 * i + 2], stream_));
+            launch_gr(kp_, grid_gr_, stream_);
+            GCFG_HIP(hipEventRecord(events_[4 * i + 3], stream_));
+        }
+        events_pending_ = n;
+        GCFG_HIP(hipGetLastError());
+        steps_run += n;
+        return GCFG_OK;
+    }
     if (opt.use_graph && n == opt.steps_per_poll) {
         int rc = capture_graph();
         if (rc != GCFG_OK) return rc;
@@ -297,6 +328,10 @@ int Engine::run(int64_t max_steps) {
         if (rc != GCFG_OK) return rc;
         GCFG_HIP(hipMemcpyAsync(pin_ctl_, kp_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
         GCFG_HIP(hipStreamSynchronize(stream_));
+        if (events_pending_) {
+            rc = collect_profile();
+            if (rc != GCFG_OK) return rc;
+        }
         if (pin_ctl_->n_live <= 0) return GCFG_OK;
         if (max_steps > 0 && steps_run - start > max_steps) {
             set_error("solver did not finish within max_steps");
@@ -305,7 +340,29 @@ int Engine::run(int64_t max_steps) {
     }
 }
 
-int Engine::pump(int steps) { return launch_steps(steps); }
+int Engine::collect_profile() {
+    for (int i = 0; i < events_pending_; ++i) {
+        float a = 0, b = 0, c = 0;
+        GCFG_HIP(hipEventElapsedTime(&a, events_[4 * i], events_[4 * i + 1]));
+        GCFG_HIP(hipEventElapsedTime(&b, events_[4 * i + 1], events_[4 * i + 2]));
+        GCFG_HIP(hipEventElapsedTime(&c, events_[4 * i + 2], events_[4 * i + 3]));
+        prof_sched_ms += a;
+        prof_pr_ms += b;
+        prof_gr_ms += c;
+        prof_steps++;
+    }
+    events_pending_ = 0;
+    return GCFG_OK;
+}
+
+int Engine::pump(int steps) {
+    if (events_pending_) {
+        GCFG_HIP(hipStreamSynchronize(stream_));
+        int rc = collect_profile();
+        if (rc != GCFG_OK) return rc;
+    }
+    return launch_steps(steps);
+}
 
 int Engine::live_count(int32_t *out) {
     GCFG_HIP(hipMemcpyAsync(pin_ctl_, kp_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));

@@ -71,7 +71,7 @@ struct InstState {
 
 struct Ctl {
     int32_t qlen_pr, head_pr, qlen_gr, head_gr;
-    int32_t gr_ticket, n_live, steps, pad;
+    int32_t ticket_pr, ticket_gr, n_live, pad;
 };
 
 struct KParams {
@@ -92,6 +92,9 @@ struct KParams {
 void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int sweeps, hipStream_t s);
 void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_tiles, hipStream_t s);
 void launch_step(const KParams &p, int grid_pr, int grid_gr, hipStream_t s);
+void launch_sched(const KParams &p, hipStream_t s);
+void launch_pr(const KParams &p, int grid, hipStream_t s);
+void launch_gr(const KParams &p, int grid, hipStream_t s);
 void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,
                  hipStream_t s);
 void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, hipStream_t s);
@@ -124,6 +127,8 @@ class Engine {
                    const int32_t *left, const int32_t *right, int32_t penalty, int32_t *label_out);
     // Upload descriptors, re-initialise state and build the residual graphs.
     int build(const std::vector<int> &slots);
+    // Upload the descriptor table only (e.g. a new label destination).
+    int upload_desc();
     // Re-arm instances from their current residual graph (warm start).
     int rearm(const std::vector<int> &slots);
     // Run steps until every armed instance is done.
@@ -146,11 +151,16 @@ class Engine {
     static size_t bytes_per_tile() { return (size_t)TN * (16 + 4 + 4 + 1) + 64; }
     gcfg_options opt;
     int steps_run = 0;
+    // Event timing of the launches since the last reset (profile mode).
+    double prof_sched_ms = 0, prof_pr_ms = 0, prof_gr_ms = 0;
+    int64_t prof_steps = 0;
+    void reset_profile() { prof_sched_ms = prof_pr_ms = prof_gr_ms = 0; prof_steps = 0; }
 
    private:
     int alloc_tiles(int n);
     void free_tiles(int start, int n);
     int launch_steps(int n);
+    int collect_profile();
     int capture_graph();
 
     int dev_ = 0;
@@ -172,6 +182,8 @@ class Engine {
     hipGraph_t graph_ = nullptr;
     int graph_steps_ = 0;
     std::vector<void *> allocs_;
+    std::vector<hipEvent_t> events_;  // 4 per profiled step
+    int events_pending_ = 0;
 };
 
 double now_ms();

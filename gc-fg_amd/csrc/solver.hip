@@ -69,12 +69,13 @@ __global__ void k_arm(KParams p, const int32_t *slots, int n, int warm, int swee
     const int s = slots[i];
     InstState &st = p.state[s];
     const int64_t sp = warm ? st.sum_pos : 0;
+    const int was_live = st.phase != PH_IDLE && st.phase != PH_DONE;
     InstState z{};
     z.sum_pos = sp;
     z.phase = PH_START;
     z.sweeps = sweeps;
     st = z;
-    atomicAdd(&p.ctl->n_live, 1);
+    if (!was_live) atomicAdd(&p.ctl->n_live, 1);  // re-arming a live instance keeps one count
 }
 
 // ---------------------------------------------------------------------------
@@ -265,13 +266,15 @@ __global__ __launch_bounds__(NT) void k_pr(KParams p) {
     __shared__ int FLs[LH][LW], FRs[LH][LW], FUs[LH][LW], FDs[LH][LW];
     __shared__ int s_item;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // Workgroups past the queue length have nothing to take: leave at once.
+    const int qlen = p.ctl->qlen_pr;
+    if ((int)blockIdx.x >= qlen) return;
     for (int i = tid; i < LH * LW; i += NT) {
         (&FLs[0][0])[i] = 0;
         (&FRs[0][0])[i] = 0;
         (&FUs[0][0])[i] = 0;
         (&FDs[0][0])[i] = 0;
     }
-    const int qlen = p.ctl->qlen_pr;
     for (;;) {
         if (tid == 0) {
             const int q = atomicAdd(&p.ctl->head_pr, 1);
@@ -457,6 +460,17 @@ __global__ __launch_bounds__(NT) void k_pr(KParams p) {
         }
         __syncthreads();
     }
+    // The last participating workgroup resets this queue for the next step.
+    if (tid == 0) {
+        __threadfence();
+        const int participants = min((int)gridDim.x, qlen);
+        if (atomicAdd(&p.ctl->ticket_pr, 1) == participants - 1) {
+            p.ctl->qlen_pr = 0;
+            p.ctl->head_pr = 0;
+            __threadfence();
+            atomicExch(&p.ctl->ticket_pr, 0);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -468,6 +482,7 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lx = lane + 1;
     const int qlen = p.ctl->qlen_gr;
+    if ((int)blockIdx.x >= qlen) return;
     for (;;) {
         if (tid == 0) {
             const int q = atomicAdd(&p.ctl->head_gr, 1);
@@ -631,19 +646,15 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p) {
         }
         __syncthreads();
     }
-    // The last workgroup to leave resets the queues for the next step.
-    __syncthreads();
+    // The last participating workgroup resets this queue for the next step.
     if (tid == 0) {
         __threadfence();
-        const int tk = atomicAdd(&p.ctl->gr_ticket, 1);
-        if (tk == (int)gridDim.x - 1) {
-            p.ctl->qlen_pr = 0;
-            p.ctl->head_pr = 0;
+        const int participants = min((int)gridDim.x, qlen);
+        if (atomicAdd(&p.ctl->ticket_gr, 1) == participants - 1) {
             p.ctl->qlen_gr = 0;
             p.ctl->head_gr = 0;
-            p.ctl->steps++;
             __threadfence();
-            atomicExch(&p.ctl->gr_ticket, 0);
+            atomicExch(&p.ctl->ticket_gr, 0);
         }
     }
 }
@@ -706,10 +717,14 @@ void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_til
     hipLaunchKernelGGL(k_build, dim3(grid), dim3(NT), 0, s, p, d_slots, d_slots + n, n, total_tiles);
 }
 
+void launch_sched(const KParams &p, hipStream_t s) { hipLaunchKernelGGL(k_sched, dim3(p.n_slots), dim3(NT), 0, s, p); }
+void launch_pr(const KParams &p, int grid, hipStream_t s) { hipLaunchKernelGGL(k_pr, dim3(grid), dim3(NT), 0, s, p); }
+void launch_gr(const KParams &p, int grid, hipStream_t s) { hipLaunchKernelGGL(k_gr, dim3(grid), dim3(NT), 0, s, p); }
+
 void launch_step(const KParams &p, int grid_pr, int grid_gr, hipStream_t s) {
-    hipLaunchKernelGGL(k_sched, dim3(p.n_slots), dim3(NT), 0, s, p);
-    hipLaunchKernelGGL(k_pr, dim3(grid_pr), dim3(NT), 0, s, p);
-    hipLaunchKernelGGL(k_gr, dim3(grid_gr), dim3(NT), 0, s, p);
+    launch_sched(p, s);
+    launch_pr(p, grid_pr, s);
+    launch_gr(p, grid_gr, s);
 }
 
 void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,

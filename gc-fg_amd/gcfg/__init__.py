@@ -43,13 +43,14 @@ class GcfgError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("sweeps_per_gr", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
                 ("steps_per_poll", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("max_steps", ctypes.c_int32), ("verbose", ctypes.c_int32)]
+                ("max_steps", ctypes.c_int32), ("verbose", ctypes.c_int32), ("profile", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("flow", ctypes.c_int64), ("pr_tiles", ctypes.c_int64), ("gr_tiles", ctypes.c_int64),
                 ("pr_iters", ctypes.c_int64), ("steps", ctypes.c_int32), ("gr_rounds", ctypes.c_int32),
-                ("ms", ctypes.c_double)]
+                ("ms", ctypes.c_double), ("pr_kernel_ms", ctypes.c_double), ("gr_kernel_ms", ctypes.c_double),
+                ("sched_kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -173,6 +174,10 @@ class DeviceBuffer:
         assert out.nbytes <= self.nbytes
         _hip_check(hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2), "hipMemcpy D2H")
         return out
+
+    def copy_from_device(self, src_ptr: int, nbytes: int):
+        assert nbytes <= self.nbytes
+        _hip_check(hip().hipMemcpy(self.ptr, src_ptr, nbytes, 3), "hipMemcpy D2D")
 
     def free(self):
         if self.ptr:
@@ -411,12 +416,17 @@ class Batch:
         self.items = items
         self._bufs = []
         arr = (Instance * len(items))()
+        def dev(v):
+            # DeviceBuffer (borrowed, kept alive here) or host array (uploaded)
+            if isinstance(v, DeviceBuffer):
+                return v
+            return DeviceBuffer.from_array(np.ascontiguousarray(v, dtype=np.int32))
+
         for i, it in enumerate(items):
-            d = {k: DeviceBuffer.from_array(np.ascontiguousarray(v, dtype=np.int32))
-                 for k, v in (("pos", it.pos), ("neg", it.neg))}
+            d = {"pos": dev(it.pos), "neg": dev(it.neg)}
             if it.edges is not None:
                 for k, v in zip(("up", "down", "left", "right"), it.edges):
-                    d[k] = DeviceBuffer.from_array(np.ascontiguousarray(v, dtype=np.int32))
+                    d[k] = dev(v)
             self._bufs.append(d)
             arr[i].width, arr[i].height, arr[i].penalty = it.width, it.height, it.penalty
             arr[i].d_pos, arr[i].d_neg = d["pos"].ptr, d["neg"].ptr

@@ -42,6 +42,7 @@ typedef struct gcfg_options {
     int32_t use_graph;       /* capture a chunk of steps in a hipGraph and replay it */
     int32_t max_steps;       /* 0 = unbounded; else GCFG_ERR_NOT_CONVERGED past this */
     int32_t verbose;         /* 1: print the reference's "pen"/energy lines */
+    int32_t profile;         /* 1: time every push/relabel and global-relabel launch with HIP events */
 } gcfg_options;
 
 typedef struct gcfg_stats {
@@ -52,6 +53,9 @@ typedef struct gcfg_stats {
     int32_t steps;       /* scheduler steps until done */
     int32_t gr_rounds;   /* global relabels started */
     double ms;           /* wall time of the call */
+    /* filled when gcfg_options.profile is set (HIP events around each launch) */
+    double pr_kernel_ms, gr_kernel_ms, sched_kernel_ms;
+    int64_t launches;    /* steps launched (each = sched + pr + gr) */
 } gcfg_stats;
 
 void gcfg_default_options(gcfg_options *opt);

@@ -1,0 +1,66 @@
+"""Batched solve: many instances of uneven size in one engine; each result
+equals the oracle and the plain drop-in on the same instance."""
+import numpy as np
+import pytest
+
+from conftest import random_instance
+
+pytestmark = pytest.mark.gpu
+
+
+def _ellipse_links(g, W, H, seed):
+    img = g.gen_ellipse_image(W, H, seed)
+    dt = g.DataTerm(W, H)
+    dt.set_image(img)
+    dt.set_seeds(g.gen_seed_map(W, H))
+    dt.compute()
+    links = dt.links()
+    dt.close()
+    return links
+
+
+def test_mixed_batch_matches_oracle(g, ora):
+    rng = np.random.default_rng(12)
+    items, refs = [], []
+    shapes = [(70, 40), (3, 3), (1, 9), (257, 130), (64, 16), (129, 17), (50, 300), (400, 200)]
+    for i, (W, H) in enumerate(shapes):
+        if i % 2:
+            pos, neg, up, down, left, right = _ellipse_links(g, W, H, i + 1) if W >= 40 and H >= 40 else \
+                (lambda p, n, E: (p, n) + E)(*random_instance(rng, W, H))
+            E = (up, down, left, right)
+            items.append(g.BatchItem(W, H, pos, neg, edges=E))
+            refs.append(ora.maxflow(W, H, pos, neg, edges=E))
+        else:
+            pos, neg, _ = random_instance(rng, W, H, tmax=200, edges=False)
+            items.append(g.BatchItem(W, H, pos, neg, penalty=37))
+            refs.append(ora.maxflow(W, H, pos, neg, penalty=37))
+    b = g.Batch(items)
+    for rep in range(2):  # the same batch object solves again from scratch
+        flows, labels, st = b.solve(labels_to_host=True)
+        for k, (rf, rl) in enumerate(refs):
+            assert flows[k] == rf, (rep, k)
+            assert np.array_equal(labels[k], rl), (rep, k)
+    flows_d, _, _ = b.solve(labels_to_host=False)
+    assert np.array_equal(flows_d, flows)
+    for k, (rf, rl) in enumerate(refs):
+        assert np.array_equal(b.device_labels(k), rl)
+    b.close()
+
+
+def test_batch_of_ellipse_images_equals_plain(g, ora):
+    Ws, Hs = g.gen_batch_shapes(24, 5, 128.0 ** 2, 512.0 ** 2)
+    items, plain = [], []
+    for i in range(len(Ws)):
+        W, H = int(Ws[i]), int(Hs[i])
+        pos, neg, up, down, left, right = _ellipse_links(g, W, H, g.mix_seed(5, i))
+        items.append(g.BatchItem(W, H, pos, neg, edges=(up, down, left, right)))
+        flow, label, _ = g.solve(W, H, pos, neg, edges=(up, down, left, right))
+        plain.append((flow, label))
+        if i < 4:
+            rf, rl = ora.maxflow(W, H, pos, neg, edges=(up, down, left, right))
+            assert rf == flow and np.array_equal(rl, label)
+    b = g.Batch(items)
+    flows, labels, _ = b.solve()
+    for k, (f, l) in enumerate(plain):
+        assert flows[k] == f and np.array_equal(labels[k], l)
+    b.close()

@@ -1,0 +1,123 @@
+"""Serving: each result is the same alone and in the mix (bit-exact), warm-
+started edits equal a cold solve of the edited instance, the HBM budget is
+enforced, and edits wait for their session."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _req(g, W, H, seed):
+    return g.gen_ellipse_image(W, H, seed), g.gen_seed_map(W, H)
+
+
+def _drain(srv, n):
+    out = {}
+    while len(out) < n:
+        for r in srv.poll(wait=True):
+            out[r["request_id"]] = r
+    return out
+
+
+def _cold(g, ora, img, seeds, W, H, dpos=None, dneg=None):
+    ref = ora.dataterm(W, H, img, seeds, 50)
+    pos = ref["pos"] + (0 if dpos is None else dpos)
+    neg = ref["neg"] + (0 if dneg is None else dneg)
+    E = (ref["up"], ref["down"], ref["left"], ref["right"])
+    return ora.maxflow(W, H, pos, neg, edges=E)
+
+
+def test_alone_equals_mix_and_oracle(g, ora):
+    shapes = [(300, 200), (128, 256), (511, 300), (256, 256), (90, 400)]
+    reqs = [(W, H) + _req(g, W, H, 100 + i) for i, (W, H) in enumerate(shapes)]
+    alone = []
+    for W, H, img, seeds in reqs:
+        srv = g.Server(1 << 30)
+        rid, sess, label = srv.submit_image(img, seeds)
+        r = _drain(srv, 1)[rid]
+        assert r["status"] == 0
+        alone.append((r["flow"], label.copy()))
+        srv.close()
+    srv = g.Server(1 << 30)
+    ids = []
+    for W, H, img, seeds in reqs:
+        rid, sess, label = srv.submit_image(img, seeds)
+        ids.append((rid, label))
+    res = _drain(srv, len(reqs))
+    for k, (rid, label) in enumerate(ids):
+        assert res[rid]["status"] == 0
+        assert res[rid]["flow"] == alone[k][0]
+        assert np.array_equal(label, alone[k][1])
+        W, H, img, seeds = reqs[k]
+        rf, rl = _cold(g, ora, img, seeds, W, H)
+        assert rf == alone[k][0] and np.array_equal(rl, alone[k][1])
+    srv.close()
+
+
+def test_warm_edit_equals_cold(g, ora):
+    W, H = 360, 240
+    img, seeds = _req(g, W, H, 7)
+    srv = g.Server(1 << 30)
+    rid, sess, label0 = srv.submit_image(img, seeds)
+    _drain(srv, 1)
+    rng = np.random.default_rng(3)
+    dpos = np.zeros((H, W), np.int64)
+    dneg = np.zeros((H, W), np.int64)
+    for round_ in range(3):
+        strokes = []
+        for _ in range(int(rng.integers(1, 6))):
+            x0, y0 = int(rng.integers(0, W - 16)), int(rng.integers(0, H - 16))
+            cls = int(rng.integers(1, 3))
+            strokes.append([x0, y0, 16, 16, cls])
+            (dpos if cls == 1 else dneg)[y0:y0 + 16, x0:x0 + 16] += 1 << 16
+        rid, label = srv.submit_edit(sess, strokes, 1 << 16, shape=(H, W))
+        r = _drain(srv, 1)[rid]
+        assert r["status"] == 0
+        rf, rl = _cold(g, ora, img, seeds, W, H, dpos, dneg)
+        assert r["flow"] == rf, round_
+        assert np.array_equal(label, rl), round_
+    srv.close()
+
+
+def test_edits_queue_behind_their_image_and_mix(g, ora):
+    W, H = 200, 150
+    img, seeds = _req(g, W, H, 9)
+    img2, seeds2 = _req(g, 256, 128, 10)
+    srv = g.Server(1 << 30)
+    rid0, sess, _ = srv.submit_image(img, seeds)
+    strokes = [[50, 50, 16, 16, 2], [120, 70, 16, 16, 1]]
+    rid1, label1 = srv.submit_edit(sess, strokes, 1 << 16, shape=(H, W))
+    rid2, _, _ = srv.submit_image(img2, seeds2)
+    rid3, label3 = srv.submit_edit(sess, [[10, 10, 16, 16, 1]], 1 << 16, shape=(H, W))
+    res = _drain(srv, 4)
+    assert all(r["status"] == 0 for r in res.values())
+    dpos = np.zeros((H, W), np.int64)
+    dneg = np.zeros((H, W), np.int64)
+    dneg[50:66, 50:66] += 1 << 16
+    dpos[70:86, 120:136] += 1 << 16
+    rf, rl = _cold(g, ora, img, seeds, W, H, dpos, dneg)
+    assert res[rid1]["flow"] == rf and np.array_equal(label1, rl)
+    dpos[10:26, 10:26] += 1 << 16
+    rf, rl = _cold(g, ora, img, seeds, W, H, dpos, dneg)
+    assert res[rid3]["flow"] == rf and np.array_equal(label3, rl)
+    srv.close()
+
+
+def test_budget_refusal_and_eviction(g):
+    W, H = 512, 512
+    img, seeds = _req(g, W, H, 11)
+    srv = g.Server(4 << 20)  # too small for one 512^2 request
+    rid, sess, _ = srv.submit_image(img, seeds)
+    r = _drain(srv, 1)[rid]
+    assert r["status"] == g.GCFG_ERR_BUDGET
+    srv.close()
+    srv = g.Server(40 << 20)  # room for about one session at a time: idle ones get evicted
+    ids = [srv.submit_image(*_req(g, W, H, 20 + i))[:2] for i in range(3)]
+    res = _drain(srv, 3)
+    assert all(res[rid]["status"] == 0 for rid, _ in ids)
+    _, _, used = srv.load()
+    assert used <= 40 << 20
+    rid, _ = srv.submit_edit(ids[0][1], [[5, 5, 16, 16, 1]], 1 << 16, shape=(H, W))
+    r = _drain(srv, 1)[rid]
+    assert r["status"] == g.GCFG_ERR_STATE  # the first session was evicted
+    srv.close()

@@ -166,3 +166,37 @@ def _verify_certificate(W, H, pos, neg, E, caps, exc, label, flow, ora):
             break
         reach = nxt
     assert np.array_equal(label.astype(bool), inter & ~reach)
+
+
+def test_rebuild_and_label_target_change(g, ora):
+    """Building twice, then solving into a caller device buffer (a new label
+    destination) gives the same exact result and terminates."""
+    rng = np.random.default_rng(21)
+    W, H = 120, 66
+    pos, neg, E = random_instance(rng, W, H, tmax=90, emax=30)
+    rflow, rlabel = ora.maxflow(W, H, pos, neg, edges=E)
+    gr = g.Graph(W, H)
+    gr.set_dataterms(pos, neg)
+    gr.set_edges(*E)
+    gr.build()
+    gr.build()
+    d_label = g.DeviceBuffer(4 * W * H)
+    st = gr.optimize_device(d_label)
+    assert st.flow == rflow
+    assert np.array_equal(d_label.download(np.int32, W * H).reshape(H, W), rlabel)
+    gr.build()
+    label, st2 = gr.optimize()
+    assert st2.flow == rflow and np.array_equal(label, rlabel)
+    gr.close()
+
+
+def test_profile_mode_reports_kernel_time(g, ora):
+    rng = np.random.default_rng(4)
+    W, H = 200, 100
+    pos, neg, E = random_instance(rng, W, H, tmax=90, emax=30)
+    o = g.default_options()
+    o.profile = 1
+    flow, label, st = g.solve(W, H, pos, neg, edges=E, options=o)
+    rflow, rlabel = ora.maxflow(W, H, pos, neg, edges=E)
+    assert flow == rflow and np.array_equal(label, rlabel)
+    assert st.pr_kernel_ms > 0 and st.gr_kernel_ms > 0 and st.launches >= st.steps
