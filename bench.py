@@ -196,6 +196,69 @@ def cpu_baseline(insts, flows_gpu, labels_gpu_fn, budget_s):
                 seconds=round(secs, 2), matches_gpu=bool(match))
 
 
+def single_instance(kind, seed):
+    if kind == "c1":
+        W = H = 1024
+        img = gcfg.gen_ellipse_image(W, H, seed)
+        seeds = gcfg.gen_seed_map(W, H)
+    else:
+        W = H = 4096
+        img = gcfg.gen_perlin_image(W, H, seed)
+        seeds = np.zeros((H, W), np.uint8)
+    dt = gcfg.DataTerm(W, H)
+    dt.set_image(img)
+    dt.set_seeds(seeds)
+    dt.compute()
+    bufs = []
+    for p in dt.link_ptrs():
+        b = gcfg.DeviceBuffer(4 * W * H)
+        b.copy_from_device(p, 4 * W * H)
+        bufs.append(b)
+    dt.close()
+    if kind == "c2":
+        pos, neg = gcfg.gen_stress_tlinks(W, H, seed + 1)
+        bufs[0].upload(pos)
+        bufs[1].upload(neg)
+    return W, H, bufs
+
+
+def single(args, opts):
+    """One plain GC_* drop-in solve at C1 or C2, with labels left on the device."""
+    W, H, bufs = single_instance(args.single, args.seed)
+    g = gcfg.Graph(W, H, opts)
+    g.set_dataterms(bufs[0], bufs[1])
+    g.set_edges(*bufs[2:6])
+    d_label = gcfg.DeviceBuffer(4 * W * H)
+    g.build()
+    g.optimize_device(d_label)
+    times, st = [], None
+    for s in range(args.warmup + args.steps):
+        gcfg.device_synchronize()
+        t0 = time.perf_counter()
+        g.build()
+        st = g.optimize_device(d_label)
+        gcfg.device_synchronize()
+        if s >= args.warmup:
+            times.append((time.perf_counter() - t0) * 1e3)
+        log(f"{args.single} solve {s}: {(time.perf_counter() - t0) * 1e3:.2f} ms flow {st.flow} steps {st.steps} "
+            f"gr_rounds {st.gr_rounds} pr_tiles {st.pr_tiles} gr_tiles {st.gr_tiles}")
+    ms = float(np.mean(times))
+    out = dict(workload=args.single, W=W, H=H, ms=round(ms, 3), mpx_s=round(W * H / ms / 1e3, 2), flow=int(st.flow),
+               steps=int(st.steps), gr_rounds=int(st.gr_rounds), pr_tiles=int(st.pr_tiles),
+               gr_tiles=int(st.gr_tiles), sweeps_per_gr=opts.sweeps_per_gr, inner_iters=opts.inner_iters)
+    if not args.no_cpu:
+        import oracle
+        n = W * H
+        host = [b.download(np.int32, n).reshape(H, W) for b in bufs]
+        t0 = time.perf_counter()
+        rf, rl = oracle.maxflow(W, H, host[0], host[1], edges=tuple(host[2:6]))
+        out["oracle_s"] = round(time.perf_counter() - t0, 2)
+        lab = d_label.download(np.int32, n).reshape(H, W)
+        out["matches_oracle"] = bool(rf == st.flow and np.array_equal(rl, lab))
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +276,9 @@ def main():
     ap.add_argument("--sweeps", type=int, default=0)
     ap.add_argument("--inner", type=int, default=0)
     ap.add_argument("--poll", type=int, default=0)
+    ap.add_argument("--single", choices=["c1", "c2"], default=None,
+                    help="time one plain GC_* solve instead: c1 = 1024^2 ellipse image, "
+                         "c2 = 4096^2 heavy-tailed t-links on Perlin n-links")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path is not sharded across GPUs",
@@ -222,6 +288,13 @@ def main():
         raise SystemExit("no GPU visible: bench.py needs an MI355X")
 
     opts = gcfg.default_options()
+    if args.single:
+        if args.sweeps:
+            opts.sweeps_per_gr = args.sweeps
+        if args.inner:
+            opts.inner_iters = args.inner
+        single(args, opts)
+        return
     if args.sweeps:
         opts.sweeps_per_gr = args.sweeps
     if args.inner:

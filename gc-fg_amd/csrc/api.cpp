@@ -40,8 +40,8 @@ extern "C" {
 
 void gcfg_default_options(gcfg_options *o) {
     if (!o) return;
-    o->sweeps_per_gr = 8;
-    o->inner_iters = 64;
+    o->sweeps_per_gr = 1;
+    o->inner_iters = 4;
     o->steps_per_poll = 8;
     o->use_graph = 1;
     o->max_steps = 200000;
