@@ -47,6 +47,7 @@ void gcfg_default_options(gcfg_options *o) {
     o->max_steps = 200000;
     o->verbose = 0;
     o->profile = 0;
+    o->kernels = 1;
 }
 
 const char *gcfg_last_error(void) { return gcfg::last_error_cstr(); }
@@ -177,6 +178,9 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
     InstState st;
     rc = g->eng.fetch_state(g->slot, &st);
     if (rc != GCFG_OK) return rc;
+    gcfg::Ctl ctl;
+    rc = g->eng.fetch_ctl(&ctl);
+    if (rc != GCFG_OK) return rc;
     if (st.err) {
         set_error("negative capacity in the inputs");
         return GCFG_ERR_INVALID;
@@ -184,9 +188,10 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
     if (g->opt.verbose) std::printf("Final energy level: %lld\n", (long long)st.flow);
     if (stats) {
         stats->flow = st.flow;
-        stats->pr_tiles = st.pr_tiles;
-        stats->gr_tiles = st.gr_tiles;
-        stats->pr_iters = st.pr_iters;
+        stats->pr_tiles = ctl.pr_tiles;
+        stats->gr_tiles = ctl.gr_tiles;
+        stats->pr_iters = ctl.pr_iters;
+        stats->gr_iters = ctl.gr_iters;
         stats->steps = st.steps;
         stats->gr_rounds = st.gr_rounds;
         stats->ms = now_ms() - t0;
@@ -308,7 +313,14 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
     // Instance slots were assigned 0..n-1 in order at creation: one copy.
     rc = b->eng.fetch_states(b->states.data(), n);
     if (rc != GCFG_OK) return rc;
+    gcfg::Ctl ctl;
+    rc = b->eng.fetch_ctl(&ctl);
+    if (rc != GCFG_OK) return rc;
     gcfg_stats agg{};
+    agg.pr_tiles = ctl.pr_tiles;
+    agg.gr_tiles = ctl.gr_tiles;
+    agg.pr_iters = ctl.pr_iters;
+    agg.gr_iters = ctl.gr_iters;
     for (int i = 0; i < n; ++i) {
         const InstState &st = b->states[i];
         if (st.err) {
@@ -317,9 +329,6 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
         }
         if (flows) flows[i] = st.flow;
         agg.flow += st.flow;
-        agg.pr_tiles += st.pr_tiles;
-        agg.gr_tiles += st.gr_tiles;
-        agg.pr_iters += st.pr_iters;
         agg.steps = st.steps > agg.steps ? st.steps : agg.steps;
         agg.gr_rounds += st.gr_rounds;
     }

@@ -68,7 +68,6 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
     rc |= dalloc((void **)&kp_.dist, nodes * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.mask, nodes);
     rc |= dalloc((void **)&kp_.tiles, (size_t)cap_tiles * sizeof(TileInfo));
-    rc |= dalloc((void **)&kp_.tile_cnt, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.act_flag, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.dirty, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.queue_pr, (size_t)cap_tiles * sizeof(int32_t));
@@ -88,15 +87,37 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
     GCFG_HIP(hipMemsetAsync(kp_.act_flag, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
     GCFG_HIP(hipStreamSynchronize(stream_));
     kp_.n_slots = cap_slots;
-    kp_.inner_iters = opt.inner_iters;
-    grid_pr_ = std::max(cu_count_, 1) * 4;
-    grid_gr_ = std::max(cu_count_, 1) * 8;
+    apply_options();
     h_desc_.assign(cap_slots, InstDesc{});
     slot_used_.assign(cap_slots, 0);
     free_ranges_.clear();
     free_ranges_.push_back({0, cap_tiles});
     std::memset(pin_ctl_, 0, sizeof(Ctl));
     return GCFG_OK;
+}
+
+// Kernel parameters and grid sizes follow the current options; a captured
+// step graph with stale parameters is dropped.
+void Engine::apply_options() {
+    const int cu = std::max(cu_count_, 1);
+    const int variant = opt.kernels ? 1 : 0;
+    const int gpr = variant ? cu * 16 : cu * 4;
+    const int ggr = variant ? cu * 16 : cu * 8;
+    if (kp_.inner_iters != opt.inner_iters || kp_.variant != variant || grid_pr_ != gpr || grid_gr_ != ggr) {
+        kp_.inner_iters = opt.inner_iters;
+        kp_.variant = variant;
+        grid_pr_ = gpr;
+        grid_gr_ = ggr;
+        if (graph_exec_) {
+            hipGraphExecDestroy(graph_exec_);
+            graph_exec_ = nullptr;
+        }
+        if (graph_) {
+            hipGraphDestroy(graph_);
+            graph_ = nullptr;
+        }
+        graph_steps_ = 0;
+    }
 }
 
 size_t Engine::tiles_free() const {
@@ -216,6 +237,8 @@ int Engine::build(const std::vector<int> &slots) {
                             stream_));
     GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)(2 * n) * sizeof(int32_t), hipMemcpyHostToDevice,
                             stream_));
+    apply_options();
+    launch_reset_stats(kp_, stream_);
     launch_arm(kp_, d_slots_, n, 0, opt.sweeps_per_gr, stream_);
     launch_build(kp_, d_slots_, n, total, stream_);
     GCFG_HIP(hipGetLastError());
@@ -236,6 +259,8 @@ int Engine::rearm(const std::vector<int> &slots) {
     const int n = (int)slots.size();
     for (int i = 0; i < n; ++i) pin_slots_[i] = slots[i];
     GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+    apply_options();
+    launch_reset_stats(kp_, stream_);
     launch_arm(kp_, d_slots_, n, 1, opt.sweeps_per_gr, stream_);
     GCFG_HIP(hipGetLastError());
     return GCFG_OK;
@@ -287,6 +312,7 @@ int Engine::capture_graph() {
 }
 
 int Engine::launch_steps(int n) {
+    apply_options();
     if (opt.profile) {
         if ((int)events_.size() < 4 * n) {
             for (int i = (int)events_.size(); i < 4 * n; ++i) {
@@ -380,6 +406,13 @@ int Engine::fetch_state(int slot, InstState *out) {
 int Engine::fetch_states(InstState *out, int n) {
     GCFG_HIP(hipMemcpyAsync(out, kp_.state, (size_t)n * sizeof(InstState), hipMemcpyDeviceToHost, stream_));
     GCFG_HIP(hipStreamSynchronize(stream_));
+    return GCFG_OK;
+}
+
+int Engine::fetch_ctl(Ctl *out) {
+    GCFG_HIP(hipMemcpyAsync(pin_ctl_, kp_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    *out = *pin_ctl_;
     return GCFG_OK;
 }
 

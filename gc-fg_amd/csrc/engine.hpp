@@ -58,13 +58,9 @@ struct InstState {
     int32_t pr_iter;
     int32_t gr_rounds;
     int32_t steps;
-    int32_t active_nodes;  // nodes with e>0 and finite distance (valid after a converged GR)
-    int32_t gr_dirty;      // tiles flagged for the next GR sweep
-    int32_t act_tiles[2];  // tiles with active nodes, per checkerboard colour
     int64_t sum_pos;       // interior sum of pos (current t-links)
     int64_t sum_exc;       // interior sum of max(e, 0) at FIN
     int64_t flow;
-    int64_t pr_tiles, gr_tiles, pr_iters;
     int32_t err;
     int32_t sweeps;        // colour-pass pairs between global relabels
 };
@@ -72,12 +68,14 @@ struct InstState {
 struct Ctl {
     int32_t qlen_pr, head_pr, qlen_gr, head_gr;
     int32_t ticket_pr, ticket_gr, n_live, pad;
+    // work counters since the last reset (all instances of the engine)
+    int64_t pr_tiles, gr_tiles, pr_iters, gr_iters;
 };
 
 struct KParams {
     // arena
     int4 *caps;
-    int32_t *exc, *dist, *tile_cnt, *act_flag, *dirty;
+    int32_t *exc, *dist, *act_flag, *dirty;
     uint8_t *mask;
     TileInfo *tiles;
     InstDesc *desc;
@@ -86,6 +84,7 @@ struct KParams {
     int32_t *queue_pr, *queue_gr;
     int32_t n_slots;
     int32_t inner_iters;
+    int32_t variant;  // 0: workgroup-per-tile kernels (LDS), 1: wavefront-per-tile kernels (registers)
 };
 
 // Kernel launchers (solver.hip).
@@ -95,6 +94,7 @@ void launch_step(const KParams &p, int grid_pr, int grid_gr, hipStream_t s);
 void launch_sched(const KParams &p, hipStream_t s);
 void launch_pr(const KParams &p, int grid, hipStream_t s);
 void launch_gr(const KParams &p, int grid, hipStream_t s);
+void launch_reset_stats(const KParams &p, hipStream_t s);
 void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,
                  hipStream_t s);
 void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, hipStream_t s);
@@ -141,6 +141,7 @@ class Engine {
     int read_state(int slot, int32_t *h_caps, int32_t *h_exc);
     int fetch_state(int slot, InstState *out);
     int fetch_states(InstState *out, int n);  // slots [0, n)
+    int fetch_ctl(Ctl *out);
     int sync();
 
     hipStream_t stream() const { return stream_; }
@@ -161,6 +162,7 @@ class Engine {
     void free_tiles(int start, int n);
     int launch_steps(int n);
     int collect_profile();
+    void apply_options();
     int capture_graph();
 
     int dev_ = 0;

@@ -52,18 +52,14 @@ __device__ __forceinline__ void atomic_add64(int64_t *p, int64_t v) {
     atomicAdd(reinterpret_cast<unsigned long long *>(p), (unsigned long long)v);
 }
 
-__device__ __forceinline__ void set_active(const KParams &p, int t, int inst, int colour, int v) {
-    const int old = atomicExch(&p.act_flag[t], v);
-    if (old != v) atomicAdd(&p.state[inst].act_tiles[colour], v ? 1 : -1);
-}
-
-__device__ __forceinline__ void mark_dirty(const KParams &p, int t, int inst) {
-    if (atomicExch(&p.dirty[t], 1) == 0) atomicAdd(&p.state[inst].gr_dirty, 1);
-}
+// Tile flags are plain stores: the scheduler scans them between launches, so
+// concurrent writers of the same value need no atomics.
+__device__ __forceinline__ void set_active(const KParams &p, int t, int v) { p.act_flag[t] = v; }
+__device__ __forceinline__ void mark_dirty(const KParams &p, int t) { p.dirty[t] = 1; }
 
 // ---------------------------------------------------------------------------
 // Arm: reset the per-instance state and mark it live.
-__global__ void k_arm(KParams p, const int32_t *slots, int n, int warm, int sweeps) {
+__global__ void k_arm(KParams p, const int32_t *slots, int n, int warm, int sweeps, int start_phase) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int s = slots[i];
@@ -72,7 +68,8 @@ __global__ void k_arm(KParams p, const int32_t *slots, int n, int warm, int swee
     const int was_live = st.phase != PH_IDLE && st.phase != PH_DONE;
     InstState z{};
     z.sum_pos = sp;
-    z.phase = PH_START;
+    z.phase = start_phase;
+    z.gr_rounds = start_phase == PH_GR ? 1 : 0;
     z.sweeps = sweeps;
     st = z;
     if (!was_live) atomicAdd(&p.ctl->n_live, 1);  // re-arming a live instance keeps one count
@@ -108,7 +105,6 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
             p.tiles[t] = ti;
             p.act_flag[t] = 0;
             p.dirty[t] = 0;
-            p.tile_cnt[t] = 0;
         }
         int64_t sp = 0;
         int bad = 0;
@@ -154,86 +150,27 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
 }
 
 // ---------------------------------------------------------------------------
-// Scheduler: one workgroup per instance slot.
-enum SchedOp : int { SO_NONE = 0, SO_GR_INIT = 1, SO_GR_DIRTY = 2, SO_FIN = 3, SO_PR = 4 };
+// Scheduler: one workgroup per instance slot. Decisions are block-uniform
+// (taken from block-wide scans of the tile flags); thread 0 writes the state.
+enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2 };
 
-__global__ __launch_bounds__(NT) void k_sched(KParams p) {
-    const int s = blockIdx.x;
-    __shared__ int s_op, s_colour, s_qbase;
-    __shared__ int wcount[NT / 64], wbase[NT / 64];
-    InstState &st = p.state[s];
-    if (threadIdx.x == 0) {
-        int op = SO_NONE, colour = 0;
-        int phase = st.phase;
-        if (phase != PH_IDLE && phase != PH_DONE) st.steps++;
-        if (phase == PH_START && st.err) {
-            phase = PH_DONE;
-            atomicSub(&p.ctl->n_live, 1);
-        }
-        for (int guard = 0; guard < 4; ++guard) {
-            if (phase == PH_START) {
-                st.active_nodes = 0;
-                st.act_tiles[0] = st.act_tiles[1] = 0;
-                st.gr_rounds++;
-                op = SO_GR_INIT;
-                phase = PH_GR;
-                break;
-            }
-            if (phase == PH_GR) {
-                if (st.gr_dirty > 0) {
-                    op = SO_GR_DIRTY;
-                    st.gr_dirty = 0;
-                    break;
-                }
-                if (st.active_nodes <= 0) {
-                    op = SO_FIN;
-                    phase = PH_FIN;
-                    break;
-                }
-                phase = PH_PR;
-                st.pr_iter = 0;
-                continue;
-            }
-            if (phase == PH_PR) {
-                const int a0 = st.act_tiles[0], a1 = st.act_tiles[1];
-                if (st.pr_iter >= 2 * st.sweeps || (a0 <= 0 && a1 <= 0)) {
-                    phase = PH_START;
-                    continue;
-                }
-                colour = st.pr_iter & 1;
-                if ((colour ? a1 : a0) <= 0) colour ^= 1;
-                op = SO_PR;
-                st.pr_iter++;
-                break;
-            }
-            if (phase == PH_FIN) {
-                st.flow = st.sum_pos - st.sum_exc;
-                phase = PH_DONE;
-                atomicSub(&p.ctl->n_live, 1);
-            }
-            break;
-        }
-        st.phase = phase;
-        s_op = op;
-        s_colour = colour;
-    }
-    __syncthreads();
-    const int op = s_op;
-    if (op == SO_NONE) return;
-    const int colour = s_colour;
-    const InstDesc D = p.desc[s];
+// Append the instance's tiles selected by `kind` to a queue (ballot
+// compaction, one atomic per 256 tiles); returns the number appended.
+__device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int colour, bool to_pr, int grop) {
+    __shared__ int wcount[NT / 64], wbase[NT / 64], s_qbase, s_total;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int32_t *queue = op == SO_PR ? p.queue_pr : p.queue_gr;
-    int32_t *qlen = op == SO_PR ? &p.ctl->qlen_pr : &p.ctl->qlen_gr;
-    const int grop = op == SO_GR_INIT ? OP_GR_INIT : (op == SO_GR_DIRTY ? OP_GR_RELAX : OP_FIN);
+    int32_t *queue = to_pr ? p.queue_pr : p.queue_gr;
+    int32_t *qlen = to_pr ? &p.ctl->qlen_pr : &p.ctl->qlen_gr;
+    __syncthreads();  // earlier readers of s_total are done
+    if (threadIdx.x == 0) s_total = 0;
     for (int base = 0; base < D.ntiles; base += NT) {
         const int lt = base + threadIdx.x;
         const int t = D.tile0 + lt;
         int pred = 0;
         if (lt < D.ntiles) {
-            if (op == SO_GR_INIT || op == SO_FIN) {
+            if (kind == EM_ALL) {
                 pred = 1;
-            } else if (op == SO_GR_DIRTY) {
+            } else if (kind == EM_DIRTY) {
                 pred = p.dirty[t];
                 if (pred) p.dirty[t] = 0;
             } else {
@@ -252,11 +189,82 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p) {
                 tot += wcount[w];
             }
             s_qbase = tot ? atomicAdd(qlen, tot) : 0;
+            s_total += tot;
         }
         __syncthreads();
-        if (pred) queue[s_qbase + wbase[wave] + wpre] = op == SO_PR ? t : ((t << 3) | grop);
+        if (pred) queue[s_qbase + wbase[wave] + wpre] = to_pr ? t : ((t << 3) | grop);
         __syncthreads();
     }
+    return s_total;
+}
+
+__device__ bool any_active_tile(const KParams &p, const InstDesc &D) {
+    int any = 0;
+    for (int lt = threadIdx.x; lt < D.ntiles; lt += NT) any |= p.act_flag[D.tile0 + lt];
+    return __syncthreads_or(any) != 0;
+}
+
+__device__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st) {
+    if (threadIdx.x == 0) {
+        st.gr_rounds++;
+        st.phase = PH_GR;
+    }
+    emit_tiles(p, D, EM_ALL, 0, false, OP_GR_INIT);
+}
+
+__global__ __launch_bounds__(NT) void k_sched(KParams p) {
+    const int s = blockIdx.x;
+    InstState &st = p.state[s];
+    const int phase = st.phase;
+    if (phase == PH_IDLE || phase == PH_DONE) return;
+    const int err = st.err, sweeps = st.sweeps;
+    int pr_iter = st.pr_iter;
+    __syncthreads();  // every thread has read the state before thread 0 writes it
+    if (threadIdx.x == 0) st.steps++;
+    if ((phase == PH_START && err) || phase == PH_FIN) {
+        if (threadIdx.x == 0) {
+            if (phase == PH_FIN) st.flow = st.sum_pos - st.sum_exc;
+            st.phase = PH_DONE;
+            atomicSub(&p.ctl->n_live, 1);
+        }
+        return;
+    }
+    const InstDesc D = p.desc[s];
+    if (phase == PH_START) {
+        start_global_relabel(p, D, st);
+        return;
+    }
+    if (phase == PH_GR) {
+        if (emit_tiles(p, D, EM_DIRTY, 0, false, OP_GR_RELAX) > 0) return;
+        // The BFS has converged: finish, or push from the active tiles.
+        if (!any_active_tile(p, D)) {
+            if (threadIdx.x == 0) st.phase = PH_FIN;
+            emit_tiles(p, D, EM_ALL, 0, false, OP_FIN);
+            return;
+        }
+        if (threadIdx.x == 0) st.phase = PH_PR;
+        pr_iter = 0;
+    }
+    // PH_PR: alternate checkerboard colours, then relabel globally again.
+    if (pr_iter >= 2 * sweeps) {
+        start_global_relabel(p, D, st);
+        return;
+    }
+    const int colour = pr_iter & 1;
+    int n = emit_tiles(p, D, EM_COLOUR, colour, true, 0);
+    if (n == 0) n = emit_tiles(p, D, EM_COLOUR, colour ^ 1, true, 0);
+    if (n == 0) {
+        start_global_relabel(p, D, st);
+        return;
+    }
+    if (threadIdx.x == 0) st.pr_iter = pr_iter + 1;
+}
+
+__global__ void k_reset_stats(KParams p) {
+    p.ctl->pr_tiles = 0;
+    p.ctl->gr_tiles = 0;
+    p.ctl->pr_iters = 0;
+    p.ctl->gr_iters = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -411,7 +419,6 @@ __global__ __launch_bounds__(NT) void k_pr(KParams p) {
             p.dist[g] = d[k];
             mine |= e[k] > 0 && d[k] < DINF;
         }
-        const int colour = (ti.tx + ti.ty) & 1;
         // Flow pushed into halo nodes: excess and reverse residual of the
         // neighbour tile (not running in this colour pass; int32 atomics
         // because two same-colour tiles can feed one corner node).
@@ -425,7 +432,7 @@ __global__ __launch_bounds__(NT) void k_pr(KParams p) {
                     atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 1, accL[k]);
                     any = 1;
                 }
-            if (any) set_active(p, ti.nbr[0], ti.inst, colour ^ 1, 1);
+            if (any) set_active(p, ti.nbr[0], 1);
         }
         if (lane == TW - 1 && ti.nbr[1] >= 0) {
             int any = 0;
@@ -437,26 +444,25 @@ __global__ __launch_bounds__(NT) void k_pr(KParams p) {
                     atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 0, accR[k]);
                     any = 1;
                 }
-            if (any) set_active(p, ti.nbr[1], ti.inst, colour ^ 1, 1);
+            if (any) set_active(p, ti.nbr[1], 1);
         }
         if (wave == 0 && accU && ti.nbr[2] >= 0) {
             const size_t gn = (size_t)ti.nbr[2] * TN + (TH - 1) * TW + lane;
             atomicAdd(&p.exc[gn], accU);
             atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 3, accU);
-            set_active(p, ti.nbr[2], ti.inst, colour ^ 1, 1);
+            set_active(p, ti.nbr[2], 1);
         }
         if (wave == NT / 64 - 1 && accD && ti.nbr[3] >= 0) {
             const size_t gn = (size_t)ti.nbr[3] * TN + lane;
             atomicAdd(&p.exc[gn], accD);
             atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 2, accD);
-            set_active(p, ti.nbr[3], ti.inst, colour ^ 1, 1);
+            set_active(p, ti.nbr[3], 1);
         }
         const int any_active = __syncthreads_or(mine);
         if (tid == 0) {
-            set_active(p, t, ti.inst, colour, any_active);
-            InstState &st = p.state[ti.inst];
-            atomic_add64(&st.pr_tiles, 1);
-            atomic_add64(&st.pr_iters, it);
+            set_active(p, t, any_active);
+            atomic_add64(&p.ctl->pr_tiles, 1);
+            atomic_add64(&p.ctl->pr_iters, it);
         }
         __syncthreads();
     }
@@ -621,28 +627,13 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p) {
         chD = __syncthreads_or(chD);
         const int64_t total = block_sum64(cnt, red);
         if (tid == 0) {
-            const int inst = ti.inst;
-            if (chL && ti.nbr[0] >= 0) mark_dirty(p, ti.nbr[0], inst);
-            if (chR && ti.nbr[1] >= 0) mark_dirty(p, ti.nbr[1], inst);
-            if (chU && ti.nbr[2] >= 0) mark_dirty(p, ti.nbr[2], inst);
-            if (chD && ti.nbr[3] >= 0) mark_dirty(p, ti.nbr[3], inst);
-            if (!conv) mark_dirty(p, t, inst);
-            const int colour = (ti.tx + ti.ty) & 1;
-            const int c32 = (int)total;
-            if (init) {
-                p.tile_cnt[t] = c32;
-                if (c32) atomicAdd(&st.active_nodes, c32);
-                p.act_flag[t] = c32 > 0;
-                if (c32 > 0) atomicAdd(&st.act_tiles[colour], 1);
-            } else {
-                const int delta = c32 - p.tile_cnt[t];
-                if (delta) {
-                    atomicAdd(&st.active_nodes, delta);
-                    p.tile_cnt[t] = c32;
-                }
-                set_active(p, t, inst, colour, c32 > 0);
-            }
-            atomic_add64(&st.gr_tiles, 1);
+            if (chL && ti.nbr[0] >= 0) mark_dirty(p, ti.nbr[0]);
+            if (chR && ti.nbr[1] >= 0) mark_dirty(p, ti.nbr[1]);
+            if (chU && ti.nbr[2] >= 0) mark_dirty(p, ti.nbr[2]);
+            if (chD && ti.nbr[3] >= 0) mark_dirty(p, ti.nbr[3]);
+            if (!conv) mark_dirty(p, t);
+            set_active(p, t, total > 0);
+            atomic_add64(&p.ctl->gr_tiles, 1);
         }
         __syncthreads();
     }
@@ -655,6 +646,507 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p) {
             p.ctl->head_gr = 0;
             __threadfence();
             atomicExch(&p.ctl->ticket_gr, 0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront-per-tile kernels. Lane x of a 64-lane workgroup owns column x of
+// the tile: rows 0..15 live in registers, so vertical neighbours are register
+// reads and horizontal neighbours are one cross-lane move (DPP wave shift for
+// distance-1, ds_bpermute shuffles for the scan). No LDS arrays, no workgroup
+// barriers, no per-tile atomics; flags are plain stores read by the scheduler.
+constexpr int GRW_MAX_ROUNDS = 48;
+
+// lane x <- lane x-1 (lane 0 gets 0) / lane x <- lane x+1 (lane 63 gets 0).
+__device__ __forceinline__ int shr1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false); }
+__device__ __forceinline__ int shl1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false); }
+
+__device__ __forceinline__ int64_t wave_sum64_w(int64_t v) { return wave_sum64(v); }
+
+// Halo distances: rows of the tiles above/below (one per lane) and the
+// columns of the tiles left/right (lanes 0..15 hold the left column, lanes
+// 16..31 the right column, indexed by row).
+template <bool FROM_EXCESS>
+__device__ __forceinline__ int halo_value(const KParams &p, int nbr, size_t off) {
+    if (nbr < 0) return DINF;
+    const size_t g = (size_t)nbr * TN + off;
+    if (FROM_EXCESS) return p.exc[g] < 0 ? 0 : DINF;
+    return p.dist[g];
+}
+
+template <bool FROM_EXCESS>
+__device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, int lane, int &hU, int &hD, int &hs) {
+    hU = halo_value<FROM_EXCESS>(p, ti.nbr[2], (size_t)(TH - 1) * TW + lane);
+    hD = halo_value<FROM_EXCESS>(p, ti.nbr[3], (size_t)lane);
+    hs = DINF;
+    if (lane < TH) hs = halo_value<FROM_EXCESS>(p, ti.nbr[0], (size_t)lane * TW + TW - 1);
+    else if (lane < 2 * TH) hs = halo_value<FROM_EXCESS>(p, ti.nbr[1], (size_t)(lane - TH) * TW);
+}
+
+__device__ __forceinline__ int dmin_sat(int a, int b) { return min(min(a, b), DINF); }
+
+// Relax d(u) = min(d(u), 1 + d(v)) over residual arcs u->v (mask bits: 1 left,
+// 2 right, 4 up, 8 down) until no value changes. Per round: a downward and an
+// upward sweep of each lane's column in registers, then for each row an exact
+// propagation along the row in both directions by a (min, +) composition scan
+// over the lanes -- lane x holds f_x(v) = min(A, v + B) (B = 1 if the arc to
+// the neighbour exists, else DINF) and composing with the predecessors'
+// prefix gives the row minimum including the halo value. A row is scanned
+// only if a one-step check finds a value that can still decrease (otherwise
+// the row is already at its fixed point). Returns true when converged; chg
+// collects the rows whose value changed in this lane.
+__device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], const int (&m)[TH], int hU, int hD, int hs, int lane,
+                                                uint32_t &chg, int &rounds) {
+    rounds = 0;
+    while (rounds < GRW_MAX_ROUNDS) {
+        ++rounds;
+        uint32_t rch = 0;
+#pragma unroll
+        for (int y = 0; y < TH; ++y) {
+            if (m[y] & 4) {
+                const int cand = (y ? d[y - 1] : hU) + 1;
+                if (cand < d[y]) {
+                    d[y] = cand;
+                    rch |= 1u << y;
+                }
+            }
+        }
+#pragma unroll
+        for (int y = TH - 1; y >= 0; --y) {
+            if (m[y] & 8) {
+                const int cand = (y < TH - 1 ? d[y + 1] : hD) + 1;
+                if (cand < d[y]) {
+                    d[y] = cand;
+                    rch |= 1u << y;
+                }
+            }
+        }
+#pragma unroll
+        for (int y = 0; y < TH; ++y) {
+            int hl = shr1(d[y]), hr = shl1(d[y]);
+            const int hsl = __builtin_amdgcn_readlane(hs, y), hsr = __builtin_amdgcn_readlane(hs, TH + y);
+            if (lane == 0) hl = hsl;
+            if (lane == 63) hr = hsr;
+            const bool can = ((m[y] & 1) && hl + 1 < d[y]) || ((m[y] & 2) && hr + 1 < d[y]);
+            if (!__any(can)) continue;
+            // left to right: arcs x -> x-1, halo = left column
+            int A = d[y], B = (m[y] & 1) ? 1 : DINF;
+#pragma unroll
+            for (int k = 1; k < 64; k <<= 1) {
+                const int pA = __shfl_up(A, k, 64), pB = __shfl_up(B, k, 64);
+                if (lane >= k) {
+                    A = min(A, pA + B);
+                    B = dmin_sat(pB + B, DINF);
+                }
+            }
+            int nd = min(A, dmin_sat(hsl + B, DINF));
+            // right to left: arcs x -> x+1, halo = right column
+            int A2 = nd, B2 = (m[y] & 2) ? 1 : DINF;
+#pragma unroll
+            for (int k = 1; k < 64; k <<= 1) {
+                const int pA = __shfl_down(A2, k, 64), pB = __shfl_down(B2, k, 64);
+                if (lane + k < 64) {
+                    A2 = min(A2, pA + B2);
+                    B2 = dmin_sat(pB + B2, DINF);
+                }
+            }
+            nd = min(A2, dmin_sat(hsr + B2, DINF));
+            if (nd < d[y]) {
+                d[y] = nd;
+                rch |= 1u << y;
+            }
+        }
+        chg |= rch;
+        if (!__any(rch != 0)) return true;
+    }
+    return false;
+}
+
+__global__ __launch_bounds__(64) void k_grw(KParams p) {
+    const int qlen = p.ctl->qlen_gr;
+    if ((int)blockIdx.x >= qlen) return;
+    const int lane = threadIdx.x;
+    int64_t n_tiles = 0, n_rounds = 0;
+    for (int q = blockIdx.x; q < qlen; q += gridDim.x) {
+        const int item = p.queue_gr[q];
+        const int op = item & 7, t = item >> 3;
+        const TileInfo ti = p.tiles[t];
+        const size_t base = (size_t)t * TN + lane;
+        if (op == OP_FIN) {
+            const InstDesc &D = p.desc[ti.inst];
+            int64_t sx = 0;
+            const int X = ti.tx * TW + lane;
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                const int Y = ti.ty * TH + y;
+                if (X < D.W && Y < D.H) {
+                    const size_t g = base + (size_t)y * TW;
+                    const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
+                    D.label_out[(size_t)Y * D.W + X] = (in && p.dist[g] >= DINF) ? 1 : 0;
+                    if (in) sx += max(p.exc[g], 0);
+                }
+            }
+            sx = wave_sum64_w(sx);
+            if (lane == 0 && sx) atomic_add64(&p.state[ti.inst].sum_exc, sx);
+            continue;
+        }
+        const bool init = op == OP_GR_INIT;
+        int d[TH], m[TH];
+        int hU, hD, hs;
+        if (init) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                const size_t g = base + (size_t)y * TW;
+                const int4 c = p.caps[g];
+                const int ee = p.exc[g];
+                m[y] = (c.x > 0) | ((c.y > 0) << 1) | ((c.z > 0) << 2) | ((c.w > 0) << 3) | ((ee > 0) << 4);
+                p.mask[g] = (uint8_t)m[y];
+                d[y] = ee < 0 ? 0 : DINF;
+            }
+            load_halo<true>(p, ti, lane, hU, hD, hs);
+        } else {
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                const size_t g = base + (size_t)y * TW;
+                m[y] = p.mask[g];
+                d[y] = p.dist[g];
+            }
+            load_halo<false>(p, ti, lane, hU, hD, hs);
+        }
+        uint32_t chg = 0;
+        int rounds = 0;
+        const bool conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds);
+        int act = 0;
+#pragma unroll
+        for (int y = 0; y < TH; ++y) {
+            if (init || ((chg >> y) & 1)) p.dist[base + (size_t)y * TW] = d[y];
+            act |= (m[y] & 16) && d[y] < DINF;
+        }
+        const bool chL = __any(lane == 0 && chg != 0);
+        const bool chR = __any(lane == TW - 1 && chg != 0);
+        const bool chU = __any((chg & 1u) != 0);
+        const bool chD = __any((chg >> (TH - 1)) & 1u);
+        const bool any_act = __any(act);
+        if (lane == 0) {
+            if (chL && ti.nbr[0] >= 0) mark_dirty(p, ti.nbr[0]);
+            if (chR && ti.nbr[1] >= 0) mark_dirty(p, ti.nbr[1]);
+            if (chU && ti.nbr[2] >= 0) mark_dirty(p, ti.nbr[2]);
+            if (chD && ti.nbr[3] >= 0) mark_dirty(p, ti.nbr[3]);
+            if (!conv) mark_dirty(p, t);
+            set_active(p, t, any_act);
+        }
+        n_tiles++;
+        n_rounds += rounds;
+    }
+    if (lane == 0) {
+        atomic_add64(&p.ctl->gr_tiles, n_tiles);
+        atomic_add64(&p.ctl->gr_iters, n_rounds);
+        __threadfence();
+        const int participants = min((int)gridDim.x, qlen);
+        if (atomicAdd(&p.ctl->ticket_gr, 1) == participants - 1) {
+            p.ctl->qlen_gr = 0;
+            p.ctl->head_gr = 0;
+            __threadfence();
+            atomicExch(&p.ctl->ticket_gr, 0);
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_prw(KParams p) {
+    const int qlen = p.ctl->qlen_pr;
+    if ((int)blockIdx.x >= qlen) return;
+    const int lane = threadIdx.x;
+    int64_t n_tiles = 0, n_iters = 0;
+    for (int q = blockIdx.x; q < qlen; q += gridDim.x) {
+        const int t = p.queue_pr[q];
+        const TileInfo ti = p.tiles[t];
+        const int dthr = p.desc[ti.inst].n_interior;
+        const size_t base = (size_t)t * TN + lane;
+        int4 c[TH];
+        int e[TH], d[TH], acc[TH];
+#pragma unroll
+        for (int y = 0; y < TH; ++y) {
+            const size_t g = base + (size_t)y * TW;
+            c[y] = p.caps[g];
+            e[y] = p.exc[g];
+            d[y] = p.dist[g];
+            acc[y] = 0;
+        }
+        int hU, hD, hs;
+        load_halo<false>(p, ti, lane, hU, hD, hs);
+        int accU = 0, accD = 0;
+        int it = 0;
+        while (it < p.inner_iters) {
+            ++it;
+            // Push along admissible arcs (d(u) == d(v) + 1) in the order l, r,
+            // u, d. Heights are those at the start of the iteration; flow
+            // received from a lower row can be pushed on in the same sweep.
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                int hl = shr1(d[y]), hr = shl1(d[y]);
+                if (lane == 0) hl = __builtin_amdgcn_readlane(hs, y);
+                if (lane == TW - 1) hr = __builtin_amdgcn_readlane(hs, TH + y);
+                const int hu = y ? d[y - 1] : hU;
+                const int hd = y < TH - 1 ? d[y + 1] : hD;
+                int fl = 0, fr = 0, fu = 0, fd = 0;
+                int ek = e[y];
+                int4 ck = c[y];
+                if (ek > 0 && d[y] < DINF) {
+                    const int tgt = d[y] - 1;
+                    if (ck.x > 0 && hl == tgt) {
+                        fl = min(ek, ck.x);
+                        ek -= fl;
+                        ck.x -= fl;
+                    }
+                    if (ek > 0 && ck.y > 0 && hr == tgt) {
+                        fr = min(ek, ck.y);
+                        ek -= fr;
+                        ck.y -= fr;
+                    }
+                    if (ek > 0 && ck.z > 0 && hu == tgt) {
+                        fu = min(ek, ck.z);
+                        ek -= fu;
+                        ck.z -= fu;
+                    }
+                    if (ek > 0 && ck.w > 0 && hd == tgt) {
+                        fd = min(ek, ck.w);
+                        ek -= fd;
+                        ck.w -= fd;
+                    }
+                }
+                const int inL = shr1(fr), inR = shl1(fl);
+                ck.x += inL;
+                ck.y += inR;
+                c[y] = ck;
+                e[y] = ek + inL + inR;
+                if (lane == 0) acc[y] += fl;
+                if (lane == TW - 1) acc[y] += fr;
+                if (y > 0) {
+                    e[y - 1] += fu;
+                    c[y - 1].w += fu;
+                } else {
+                    accU += fu;
+                }
+                if (y < TH - 1) {
+                    e[y + 1] += fd;
+                    c[y + 1].z += fd;
+                } else {
+                    accD += fd;
+                }
+            }
+            // Relabel nodes left with excess: d = 1 + min{d(v) : r(u,v) > 0}.
+            int act = 0;
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                int hl = shr1(d[y]), hr = shl1(d[y]);
+                if (lane == 0) hl = __builtin_amdgcn_readlane(hs, y);
+                if (lane == TW - 1) hr = __builtin_amdgcn_readlane(hs, TH + y);
+                const int hu = y ? d[y - 1] : hU;
+                const int hd = y < TH - 1 ? d[y + 1] : hD;
+                if (e[y] > 0 && d[y] < DINF) {
+                    int mm = DINF;
+                    if (c[y].x > 0) mm = min(mm, hl);
+                    if (c[y].y > 0) mm = min(mm, hr);
+                    if (c[y].z > 0) mm = min(mm, hu);
+                    if (c[y].w > 0) mm = min(mm, hd);
+                    int nd = mm >= DINF ? DINF : mm + 1;
+                    if (nd >= dthr) nd = DINF;
+                    if (nd > d[y]) d[y] = nd;
+                    act |= d[y] < DINF;
+                }
+            }
+            if (!__any(act)) break;
+        }
+        int mine = 0;
+#pragma unroll
+        for (int y = 0; y < TH; ++y) {
+            const size_t g = base + (size_t)y * TW;
+            p.caps[g] = c[y];
+            p.exc[g] = e[y];
+            p.dist[g] = d[y];
+            mine |= e[y] > 0 && d[y] < DINF;
+        }
+        // Flow pushed into halo nodes (their tiles do not run in this colour
+        // pass; int32 atomics because two same-colour tiles can feed one
+        // corner node).
+        int toL = 0, toR = 0;
+        if (lane == 0 && ti.nbr[0] >= 0) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y)
+                if (acc[y]) {
+                    const size_t gn = (size_t)ti.nbr[0] * TN + (size_t)y * TW + TW - 1;
+                    atomicAdd(&p.exc[gn], acc[y]);
+                    atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 1, acc[y]);
+                    toL = 1;
+                }
+        }
+        if (lane == TW - 1 && ti.nbr[1] >= 0) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y)
+                if (acc[y]) {
+                    const size_t gn = (size_t)ti.nbr[1] * TN + (size_t)y * TW;
+                    atomicAdd(&p.exc[gn], acc[y]);
+                    atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 0, acc[y]);
+                    toR = 1;
+                }
+        }
+        if (accU && ti.nbr[2] >= 0) {
+            const size_t gn = (size_t)ti.nbr[2] * TN + (size_t)(TH - 1) * TW + lane;
+            atomicAdd(&p.exc[gn], accU);
+            atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 3, accU);
+        }
+        if (accD && ti.nbr[3] >= 0) {
+            const size_t gn = (size_t)ti.nbr[3] * TN + lane;
+            atomicAdd(&p.exc[gn], accD);
+            atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 2, accD);
+        }
+        const bool anyL = __any(toL), anyR = __any(toR);
+        const bool anyU = __any(accU != 0 && ti.nbr[2] >= 0), anyD = __any(accD != 0 && ti.nbr[3] >= 0);
+        const bool any_mine = __any(mine);
+        if (lane == 0) {
+            if (anyL) set_active(p, ti.nbr[0], 1);
+            if (anyR) set_active(p, ti.nbr[1], 1);
+            if (anyU) set_active(p, ti.nbr[2], 1);
+            if (anyD) set_active(p, ti.nbr[3], 1);
+            set_active(p, t, any_mine);
+        }
+        n_tiles++;
+        n_iters += it;
+    }
+    if (lane == 0) {
+        atomic_add64(&p.ctl->pr_tiles, n_tiles);
+        atomic_add64(&p.ctl->pr_iters, n_iters);
+        __threadfence();
+        const int participants = min((int)gridDim.x, qlen);
+        if (atomicAdd(&p.ctl->ticket_pr, 1) == participants - 1) {
+            p.ctl->qlen_pr = 0;
+            p.ctl->head_pr = 0;
+            __threadfence();
+            atomicExch(&p.ctl->ticket_pr, 0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront build (GC_SetGraph) fused with the first global-relabel sweep.
+__device__ __forceinline__ int find_inst(const int32_t *prefix, int n, int b) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Tile table and cleared flags of the listed instances (one thread per tile).
+__global__ void k_clear_tiles(KParams p, const int32_t *slots, const int32_t *prefix, int n, int total) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < total; b += gridDim.x * blockDim.x) {
+        const int j = find_inst(prefix, n, b);
+        const int s = slots[j];
+        const InstDesc D = p.desc[s];
+        const int lt = b - prefix[j];
+        const int t = D.tile0 + lt;
+        const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
+        TileInfo ti;
+        ti.inst = s;
+        ti.tx = tx;
+        ti.ty = ty;
+        ti.nbr[0] = tx > 0 ? t - 1 : -1;
+        ti.nbr[1] = tx < D.tiles_x - 1 ? t + 1 : -1;
+        ti.nbr[2] = ty > 0 ? t - D.tiles_x : -1;
+        ti.nbr[3] = ty < D.tiles_y - 1 ? t + D.tiles_x : -1;
+        ti.pad = 0;
+        p.tiles[t] = ti;
+        p.act_flag[t] = 0;
+        p.dirty[t] = 0;
+    }
+}
+
+// Initial BFS distance of an input pixel: 0 if it is an interior node with
+// pos < neg (remaining sink capacity), otherwise unreachable so far.
+__device__ __forceinline__ int input_d0(const InstDesc &D, int X, int Y) {
+    if (X <= 0 || Y <= 0 || X >= D.W - 1 || Y >= D.H - 1) return DINF;
+    const size_t i = (size_t)Y * D.W + X;
+    return D.pos[i] < D.neg[i] ? 0 : DINF;
+}
+
+__global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, const int32_t *prefix, int n,
+                                               int total) {
+    const int lane = threadIdx.x;
+    for (int b = blockIdx.x; b < total; b += gridDim.x) {
+        const int j = find_inst(prefix, n, b);
+        const int s = slots[j];
+        const InstDesc D = p.desc[s];
+        const int lt = b - prefix[j];
+        const int t = D.tile0 + lt;
+        const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
+        const int X = tx * TW + lane, Y0 = ty * TH;
+        const size_t base = (size_t)t * TN + lane;
+        int d[TH], m[TH];
+        int64_t sp = 0;
+        int bad = 0;
+#pragma unroll
+        for (int y = 0; y < TH; ++y) {
+            const int Y = Y0 + y;
+            int4 c = make_int4(0, 0, 0, 0);
+            int e = 0;
+            if (X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1) {
+                const size_t i = (size_t)Y * D.W + X;
+                const int pp = D.pos[i], qq = D.neg[i];
+                if (D.up) {
+                    c.x = D.left[i];
+                    c.y = D.right[i];
+                    c.z = D.up[i];
+                    c.w = D.down[i];
+                } else {
+                    c = make_int4(D.penalty, D.penalty, D.penalty, D.penalty);
+                }
+                if (X == 1) c.x = 0;
+                if (X == D.W - 2) c.y = 0;
+                if (Y == 1) c.z = 0;
+                if (Y == D.H - 2) c.w = 0;
+                bad |= (pp < 0) | (qq < 0) | (c.x < 0) | (c.y < 0) | (c.z < 0) | (c.w < 0);
+                e = pp - qq;
+                sp += pp;
+            }
+            const size_t g = base + (size_t)y * TW;
+            p.caps[g] = c;
+            p.exc[g] = e;
+            m[y] = (c.x > 0) | ((c.y > 0) << 1) | ((c.z > 0) << 2) | ((c.w > 0) << 3) | ((e > 0) << 4);
+            p.mask[g] = (uint8_t)m[y];
+            d[y] = e < 0 ? 0 : DINF;
+        }
+        const int hU = ty > 0 ? input_d0(D, X, Y0 - 1) : DINF;
+        const int hD = ty < D.tiles_y - 1 ? input_d0(D, X, Y0 + TH) : DINF;
+        int hs = DINF;
+        if (lane < TH && tx > 0) hs = input_d0(D, tx * TW - 1, Y0 + lane);
+        else if (lane >= TH && lane < 2 * TH && tx < D.tiles_x - 1) hs = input_d0(D, tx * TW + TW, Y0 + lane - TH);
+        uint32_t chg = 0;
+        int rounds = 0;
+        const bool conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds);
+        int act = 0;
+#pragma unroll
+        for (int y = 0; y < TH; ++y) {
+            p.dist[base + (size_t)y * TW] = d[y];
+            act |= (m[y] & 16) && d[y] < DINF;
+        }
+        const bool chL = __any(lane == 0 && chg != 0);
+        const bool chR = __any(lane == TW - 1 && chg != 0);
+        const bool chU = __any((chg & 1u) != 0);
+        const bool chD = __any((chg >> (TH - 1)) & 1u);
+        const bool any_act = __any(act);
+        const bool any_bad = __any(bad);
+        sp = wave_sum64(sp);
+        if (lane == 0) {
+            if (chL && tx > 0) mark_dirty(p, t - 1);
+            if (chR && tx < D.tiles_x - 1) mark_dirty(p, t + 1);
+            if (chU && ty > 0) mark_dirty(p, t - D.tiles_x);
+            if (chD && ty < D.tiles_y - 1) mark_dirty(p, t + D.tiles_x);
+            if (!conv) mark_dirty(p, t);
+            set_active(p, t, any_act);
+            if (sp) atomic_add64(&p.state[s].sum_pos, sp);
+            if (any_bad) atomicOr(&p.state[s].err, 1);
         }
     }
 }
@@ -708,18 +1200,38 @@ __global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *ex
 // ---------------------------------------------------------------------------
 void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int sweeps, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_arm, dim3((n + 255) / 256), dim3(256), 0, s, p, d_slots, n, warm, sweeps);
+    // The wavefront build performs the first global-relabel sweep itself.
+    const int start = (!warm && p.variant) ? PH_GR : PH_START;
+    hipLaunchKernelGGL(k_arm, dim3((n + 255) / 256), dim3(256), 0, s, p, d_slots, n, warm, sweeps, start);
 }
 
 void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_tiles, hipStream_t s) {
     if (n <= 0 || total_tiles <= 0) return;
+    if (p.variant) {
+        hipLaunchKernelGGL(k_clear_tiles, dim3((total_tiles + 255) / 256), dim3(256), 0, s, p, d_slots, d_slots + n, n,
+                           total_tiles);
+        const int grid = total_tiles < 16384 ? total_tiles : 16384;
+        hipLaunchKernelGGL(k_buildw, dim3(grid), dim3(64), 0, s, p, d_slots, d_slots + n, n, total_tiles);
+        return;
+    }
     const int grid = total_tiles < 8192 ? total_tiles : 8192;
     hipLaunchKernelGGL(k_build, dim3(grid), dim3(NT), 0, s, p, d_slots, d_slots + n, n, total_tiles);
 }
 
 void launch_sched(const KParams &p, hipStream_t s) { hipLaunchKernelGGL(k_sched, dim3(p.n_slots), dim3(NT), 0, s, p); }
-void launch_pr(const KParams &p, int grid, hipStream_t s) { hipLaunchKernelGGL(k_pr, dim3(grid), dim3(NT), 0, s, p); }
-void launch_gr(const KParams &p, int grid, hipStream_t s) { hipLaunchKernelGGL(k_gr, dim3(grid), dim3(NT), 0, s, p); }
+void launch_pr(const KParams &p, int grid, hipStream_t s) {
+    if (p.variant)
+        hipLaunchKernelGGL(k_prw, dim3(grid), dim3(64), 0, s, p);
+    else
+        hipLaunchKernelGGL(k_pr, dim3(grid), dim3(NT), 0, s, p);
+}
+void launch_gr(const KParams &p, int grid, hipStream_t s) {
+    if (p.variant)
+        hipLaunchKernelGGL(k_grw, dim3(grid), dim3(64), 0, s, p);
+    else
+        hipLaunchKernelGGL(k_gr, dim3(grid), dim3(NT), 0, s, p);
+}
+void launch_reset_stats(const KParams &p, hipStream_t s) { hipLaunchKernelGGL(k_reset_stats, dim3(1), dim3(1), 0, s, p); }
 
 void launch_step(const KParams &p, int grid_pr, int grid_gr, hipStream_t s) {
     launch_sched(p, s);

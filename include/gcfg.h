@@ -43,6 +43,7 @@ typedef struct gcfg_options {
     int32_t max_steps;       /* 0 = unbounded; else GCFG_ERR_NOT_CONVERGED past this */
     int32_t verbose;         /* 1: print the reference's "pen"/energy lines */
     int32_t profile;         /* 1: time every push/relabel and global-relabel launch with HIP events */
+    int32_t kernels;         /* 1: wavefront-per-tile kernels (default), 0: workgroup-per-tile LDS kernels */
 } gcfg_options;
 
 typedef struct gcfg_stats {
@@ -56,6 +57,7 @@ typedef struct gcfg_stats {
     /* filled when gcfg_options.profile is set (HIP events around each launch) */
     double pr_kernel_ms, gr_kernel_ms, sched_kernel_ms;
     int64_t launches;    /* steps launched (each = sched + pr + gr) */
+    int64_t gr_iters;    /* tile-local relaxation rounds summed over global-relabel tiles */
 } gcfg_stats;
 
 void gcfg_default_options(gcfg_options *opt);

@@ -27,14 +27,23 @@ def test_debug_kat(g, ora):
     assert label.sum() == 0
 
 
+def _opts(g, kernels, **kw):
+    o = g.default_options()
+    o.kernels = kernels
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+@pytest.mark.parametrize("kernels", [0, 1])
 @pytest.mark.parametrize("shape", [(3, 3), (5, 4), (17, 9), (63, 15), (64, 16), (65, 17), (130, 33), (200, 70),
                                    (31, 120)])
-def test_random_small(g, ora, shape):
+def test_random_small(g, ora, shape, kernels):
     W, H = shape
     rng = np.random.default_rng(W * 1000 + H)
     for trial in range(3):
         pos, neg, E = random_instance(rng, W, H)
-        _check(g, ora, W, H, pos, neg, edges=E)
+        _check(g, ora, W, H, pos, neg, edges=E, options=_opts(g, kernels))
 
 
 def test_degenerate_shapes(g, ora):
@@ -52,13 +61,43 @@ def test_penalty_mode(g, ora):
     _check(g, ora, W, H, pos, neg, penalty=50)
 
 
-def test_debug_random_400(g, ora):
+@pytest.mark.parametrize("kernels", [0, 1])
+def test_debug_random_400(g, ora, kernels):
     """Debug.cu:60-74 random instance (glibc rand, seed 1), penalty 50."""
     pos, neg = glibc_rand_instance(1)
-    _check(g, ora, 400, 400, pos, neg, penalty=50)
+    _check(g, ora, 400, 400, pos, neg, penalty=50, options=_opts(g, kernels))
 
 
-def test_heavy_tailed(g, ora):
+def serpentine(W, H, cap=9):
+    """Every row is a corridor; row r connects to row r+1 only at one column,
+    alternating between the left and right ends, so the only source-sink path
+    snakes through the whole grid (length ~ W * H)."""
+    up = np.zeros((H, W), np.int32)
+    down = np.zeros((H, W), np.int32)
+    left = np.full((H, W), cap, np.int32)
+    right = np.full((H, W), cap, np.int32)
+    for r in range(1, H - 2):
+        gap = W - 2 if r % 2 else 1
+        down[r, gap] = cap
+        up[r + 1, gap] = cap
+    pos = np.zeros((H, W), np.int32)
+    neg = np.zeros((H, W), np.int32)
+    pos[1, 1:4] = 40
+    neg[H - 2, 1:4] = 40
+    return pos, neg, (up, down, left, right)
+
+
+@pytest.mark.parametrize("kernels", [0, 1])
+def test_long_paths(g, ora, kernels):
+    """Serpentine corridor: long BFS distances across many tiles."""
+    W, H = 100, 24
+    pos, neg, (up, down, left, right) = serpentine(W, H)
+    assert ora.maxflow(W, H, pos, neg, edges=(up, down, left, right))[0] == 9
+    _check(g, ora, W, H, pos, neg, edges=(up, down, left, right), options=_opts(g, kernels))
+
+
+@pytest.mark.parametrize("kernels", [0, 1])
+def test_heavy_tailed(g, ora, kernels):
     W, H = 300, 200
     pos, neg = g.gen_stress_tlinks(W, H, 11)
     img = g.gen_perlin_image(W, H, 12)
@@ -67,7 +106,7 @@ def test_heavy_tailed(g, ora):
     dt.set_seeds(np.zeros((H, W), np.uint8))
     dt.compute()
     _, _, up, down, left, right = dt.links()
-    _check(g, ora, W, H, pos, neg, edges=(up, down, left, right))
+    _check(g, ora, W, H, pos, neg, edges=(up, down, left, right), options=_opts(g, kernels))
 
 
 def test_ellipse_c0(g, ora):
@@ -85,7 +124,9 @@ def test_ellipse_c0(g, ora):
 
 
 @pytest.mark.parametrize("opts", [dict(sweeps_per_gr=1, inner_iters=1), dict(sweeps_per_gr=2, inner_iters=8),
-                                  dict(sweeps_per_gr=32, inner_iters=256, steps_per_poll=3, use_graph=0)])
+                                  dict(sweeps_per_gr=32, inner_iters=256, steps_per_poll=3, use_graph=0),
+                                  dict(sweeps_per_gr=4, inner_iters=64, kernels=0),
+                                  dict(sweeps_per_gr=16, inner_iters=200, kernels=1, steps_per_poll=5)])
 def test_options_do_not_change_result(g, ora, opts):
     rng = np.random.default_rng(99)
     W, H = 140, 75
