@@ -48,6 +48,7 @@ void gcfg_default_options(gcfg_options *o) {
     o->verbose = 0;
     o->profile = 0;
     o->kernels = 1;
+    o->gr_rounds = 8;
 }
 
 const char *gcfg_last_error(void) { return gcfg::last_error_cstr(); }
@@ -188,8 +189,8 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
     if (g->opt.verbose) std::printf("Final energy level: %lld\n", (long long)st.flow);
     if (stats) {
         stats->flow = st.flow;
-        stats->pr_tiles = ctl.pr_tiles;
-        stats->gr_tiles = ctl.gr_tiles;
+        stats->pr_tiles = st.pr_tiles;
+        stats->gr_tiles = st.gr_tiles;
         stats->pr_iters = ctl.pr_iters;
         stats->gr_iters = ctl.gr_iters;
         stats->steps = st.steps;
@@ -317,8 +318,6 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
     rc = b->eng.fetch_ctl(&ctl);
     if (rc != GCFG_OK) return rc;
     gcfg_stats agg{};
-    agg.pr_tiles = ctl.pr_tiles;
-    agg.gr_tiles = ctl.gr_tiles;
     agg.pr_iters = ctl.pr_iters;
     agg.gr_iters = ctl.gr_iters;
     for (int i = 0; i < n; ++i) {
@@ -329,6 +328,8 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
         }
         if (flows) flows[i] = st.flow;
         agg.flow += st.flow;
+        agg.pr_tiles += st.pr_tiles;
+        agg.gr_tiles += st.gr_tiles;
         agg.steps = st.steps > agg.steps ? st.steps : agg.steps;
         agg.gr_rounds += st.gr_rounds;
     }

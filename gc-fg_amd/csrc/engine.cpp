@@ -103,8 +103,13 @@ void Engine::apply_options() {
     const int variant = opt.kernels ? 1 : 0;
     const int gpr = variant ? cu * 16 : cu * 4;
     const int ggr = variant ? cu * 16 : cu * 8;
-    if (kp_.inner_iters != opt.inner_iters || kp_.variant != variant || grid_pr_ != gpr || grid_gr_ != ggr) {
+    const int grr = opt.gr_rounds > 0 ? opt.gr_rounds : 48;
+    const int prof = opt.profile ? 1 : 0;
+    if (kp_.inner_iters != opt.inner_iters || kp_.variant != variant || grid_pr_ != gpr || grid_gr_ != ggr ||
+        kp_.gr_rounds != grr || kp_.profile != prof) {
         kp_.inner_iters = opt.inner_iters;
+        kp_.gr_rounds = grr;
+        kp_.profile = prof;
         kp_.variant = variant;
         grid_pr_ = gpr;
         grid_gr_ = ggr;
@@ -293,8 +298,13 @@ int Engine::edit(int slot, const std::vector<int32_t> &strokes, int32_t delta) {
     return GCFG_OK;
 }
 
+// Steps alternate queue-counter sets by parity, so a captured chunk holds an
+// even number of steps starting at parity 0 and always ends where it began.
+static int graph_chunk(int steps_per_poll) { return (steps_per_poll + 1) & ~1; }
+
 int Engine::capture_graph() {
-    if (graph_exec_ && graph_steps_ == opt.steps_per_poll) return GCFG_OK;
+    const int chunk = graph_chunk(opt.steps_per_poll);
+    if (graph_exec_ && graph_steps_ == chunk) return GCFG_OK;
     if (graph_exec_) {
         hipGraphExecDestroy(graph_exec_);
         graph_exec_ = nullptr;
@@ -304,10 +314,10 @@ int Engine::capture_graph() {
         graph_ = nullptr;
     }
     GCFG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < opt.steps_per_poll; ++i) launch_step(kp_, grid_pr_, grid_gr_, stream_);
+    for (int i = 0; i < chunk; ++i) launch_step(kp_, grid_pr_, grid_gr_, i & 1, stream_);
     GCFG_HIP(hipStreamEndCapture(stream_, &graph_));
     GCFG_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-    graph_steps_ = opt.steps_per_poll;
+    graph_steps_ = chunk;
     return GCFG_OK;
 }
 
@@ -323,27 +333,37 @@ int Engine::launch_steps(int n) {
         }
         for (int i = 0; i < n; ++i) {
             GCFG_HIP(hipEventRecord(events_[4 * i], stream_));
-            launch_sched(kp_, stream_);
+            launch_sched(kp_, parity_, stream_);
             GCFG_HIP(hipEventRecord(events_[4 * i + 1], stream_));
-            launch_pr(kp_, grid_pr_, stream_);
+            launch_pr(kp_, grid_pr_, parity_, stream_);
             GCFG_HIP(hipEventRecord(events_[4 * i + 2], stream_));
-            launch_gr(kp_, grid_gr_, stream_);
+            launch_gr(kp_, grid_gr_, parity_, stream_);
             GCFG_HIP(hipEventRecord(events_[4 * i + 3], stream_));
+            parity_ ^= 1;
         }
         events_pending_ = n;
         GCFG_HIP(hipGetLastError());
         steps_run += n;
         return GCFG_OK;
     }
-    if (opt.use_graph && n == opt.steps_per_poll) {
+    if (opt.use_graph) {
+        if (parity_) {  // realign to an even step before replaying the graph
+            launch_step(kp_, grid_pr_, grid_gr_, parity_, stream_);
+            parity_ ^= 1;
+            steps_run += 1;
+        }
         int rc = capture_graph();
         if (rc != GCFG_OK) return rc;
         GCFG_HIP(hipGraphLaunch(graph_exec_, stream_));
+        steps_run += graph_steps_;
     } else {
-        for (int i = 0; i < n; ++i) launch_step(kp_, grid_pr_, grid_gr_, stream_);
+        for (int i = 0; i < n; ++i) {
+            launch_step(kp_, grid_pr_, grid_gr_, parity_, stream_);
+            parity_ ^= 1;
+        }
+        steps_run += n;
     }
     GCFG_HIP(hipGetLastError());
-    steps_run += n;
     return GCFG_OK;
 }
 

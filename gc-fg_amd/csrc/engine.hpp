@@ -61,15 +61,24 @@ struct InstState {
     int64_t sum_pos;       // interior sum of pos (current t-links)
     int64_t sum_exc;       // interior sum of max(e, 0) at FIN
     int64_t flow;
+    int64_t pr_tiles;      // tile push-relabel passes scheduled
+    int64_t gr_tiles;      // tile BFS sweeps scheduled (including FIN)
     int32_t err;
     int32_t sweeps;        // colour-pass pairs between global relabels
 };
 
-struct Ctl {
+// Queue counters of one step. Steps alternate between two sets: the
+// scheduler of step s appends to set s % 2 and clears set (s + 1) % 2, which
+// the consumers of step s - 1 have finished with, so no kernel needs a
+// grid-wide "last workgroup" ticket to reset them.
+struct QCtl {
     int32_t qlen_pr, head_pr, qlen_gr, head_gr;
-    int32_t ticket_pr, ticket_gr, n_live, pad;
-    // work counters since the last reset (all instances of the engine)
-    int64_t pr_tiles, gr_tiles, pr_iters, gr_iters;
+};
+
+struct Ctl {
+    QCtl q[2];
+    int32_t n_live, pad0, pad1, pad2;
+    int64_t pr_iters, gr_iters;  // inner iterations / relaxation rounds (profile mode only)
 };
 
 struct KParams {
@@ -85,15 +94,17 @@ struct KParams {
     int32_t n_slots;
     int32_t inner_iters;
     int32_t variant;  // 0: workgroup-per-tile kernels (LDS), 1: wavefront-per-tile kernels (registers)
+    int32_t gr_rounds;  // max relaxation rounds per tile per global-relabel sweep
+    int32_t profile;    // count inner iterations / relaxation rounds (extra atomics)
 };
 
 // Kernel launchers (solver.hip).
 void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int sweeps, hipStream_t s);
 void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_tiles, hipStream_t s);
-void launch_step(const KParams &p, int grid_pr, int grid_gr, hipStream_t s);
-void launch_sched(const KParams &p, hipStream_t s);
-void launch_pr(const KParams &p, int grid, hipStream_t s);
-void launch_gr(const KParams &p, int grid, hipStream_t s);
+void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s);
+void launch_sched(const KParams &p, int parity, hipStream_t s);
+void launch_pr(const KParams &p, int grid, int parity, hipStream_t s);
+void launch_gr(const KParams &p, int grid, int parity, hipStream_t s);
 void launch_reset_stats(const KParams &p, hipStream_t s);
 void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,
                  hipStream_t s);
@@ -183,6 +194,7 @@ class Engine {
     hipGraphExec_t graph_exec_ = nullptr;
     hipGraph_t graph_ = nullptr;
     int graph_steps_ = 0;
+    int parity_ = 0;  // parity of the next step to launch
     std::vector<void *> allocs_;
     std::vector<hipEvent_t> events_;  // 4 per profiled step
     int events_pending_ = 0;

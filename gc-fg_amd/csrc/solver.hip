@@ -156,11 +156,12 @@ enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2 };
 
 // Append the instance's tiles selected by `kind` to a queue (ballot
 // compaction, one atomic per 256 tiles); returns the number appended.
-__device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int colour, bool to_pr, int grop) {
+__device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int colour, bool to_pr, int grop,
+                          int parity) {
     __shared__ int wcount[NT / 64], wbase[NT / 64], s_qbase, s_total;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int32_t *queue = to_pr ? p.queue_pr : p.queue_gr;
-    int32_t *qlen = to_pr ? &p.ctl->qlen_pr : &p.ctl->qlen_gr;
+    int32_t *qlen = to_pr ? &p.ctl->q[parity].qlen_pr : &p.ctl->q[parity].qlen_gr;
     __syncthreads();  // earlier readers of s_total are done
     if (threadIdx.x == 0) s_total = 0;
     for (int base = 0; base < D.ntiles; base += NT) {
@@ -204,16 +205,18 @@ __device__ bool any_active_tile(const KParams &p, const InstDesc &D) {
     return __syncthreads_or(any) != 0;
 }
 
-__device__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st) {
+__device__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st, int parity) {
     if (threadIdx.x == 0) {
         st.gr_rounds++;
         st.phase = PH_GR;
     }
-    emit_tiles(p, D, EM_ALL, 0, false, OP_GR_INIT);
+    const int n = emit_tiles(p, D, EM_ALL, 0, false, OP_GR_INIT, parity);
+    if (threadIdx.x == 0) st.gr_tiles += n;
 }
 
-__global__ __launch_bounds__(NT) void k_sched(KParams p) {
+__global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
     const int s = blockIdx.x;
+    if (s == 0 && threadIdx.x == 0) p.ctl->q[parity ^ 1] = QCtl{0, 0, 0, 0};
     InstState &st = p.state[s];
     const int phase = st.phase;
     if (phase == PH_IDLE || phase == PH_DONE) return;
@@ -231,15 +234,19 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p) {
     }
     const InstDesc D = p.desc[s];
     if (phase == PH_START) {
-        start_global_relabel(p, D, st);
+        start_global_relabel(p, D, st, parity);
         return;
     }
     if (phase == PH_GR) {
-        if (emit_tiles(p, D, EM_DIRTY, 0, false, OP_GR_RELAX) > 0) return;
+        const int n = emit_tiles(p, D, EM_DIRTY, 0, false, OP_GR_RELAX, parity);
+        if (n > 0) {
+            if (threadIdx.x == 0) st.gr_tiles += n;
+            return;
+        }
         // The BFS has converged: finish, or push from the active tiles.
         if (!any_active_tile(p, D)) {
             if (threadIdx.x == 0) st.phase = PH_FIN;
-            emit_tiles(p, D, EM_ALL, 0, false, OP_FIN);
+            emit_tiles(p, D, EM_ALL, 0, false, OP_FIN, parity);
             return;
         }
         if (threadIdx.x == 0) st.phase = PH_PR;
@@ -247,35 +254,36 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p) {
     }
     // PH_PR: alternate checkerboard colours, then relabel globally again.
     if (pr_iter >= 2 * sweeps) {
-        start_global_relabel(p, D, st);
+        start_global_relabel(p, D, st, parity);
         return;
     }
     const int colour = pr_iter & 1;
-    int n = emit_tiles(p, D, EM_COLOUR, colour, true, 0);
-    if (n == 0) n = emit_tiles(p, D, EM_COLOUR, colour ^ 1, true, 0);
+    int n = emit_tiles(p, D, EM_COLOUR, colour, true, 0, parity);
+    if (n == 0) n = emit_tiles(p, D, EM_COLOUR, colour ^ 1, true, 0, parity);
     if (n == 0) {
-        start_global_relabel(p, D, st);
+        start_global_relabel(p, D, st, parity);
         return;
     }
-    if (threadIdx.x == 0) st.pr_iter = pr_iter + 1;
+    if (threadIdx.x == 0) {
+        st.pr_iter = pr_iter + 1;
+        st.pr_tiles += n;
+    }
 }
 
 __global__ void k_reset_stats(KParams p) {
-    p.ctl->pr_tiles = 0;
-    p.ctl->gr_tiles = 0;
     p.ctl->pr_iters = 0;
     p.ctl->gr_iters = 0;
 }
 
 // ---------------------------------------------------------------------------
 // Tile push-relabel pass.
-__global__ __launch_bounds__(NT) void k_pr(KParams p) {
+__global__ __launch_bounds__(NT) void k_pr(KParams p, int parity) {
     __shared__ int Hs[2][LH][LW];
     __shared__ int FLs[LH][LW], FRs[LH][LW], FUs[LH][LW], FDs[LH][LW];
     __shared__ int s_item;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // Workgroups past the queue length have nothing to take: leave at once.
-    const int qlen = p.ctl->qlen_pr;
+    const int qlen = p.ctl->q[parity].qlen_pr;
     if ((int)blockIdx.x >= qlen) return;
     for (int i = tid; i < LH * LW; i += NT) {
         (&FLs[0][0])[i] = 0;
@@ -285,7 +293,7 @@ __global__ __launch_bounds__(NT) void k_pr(KParams p) {
     }
     for (;;) {
         if (tid == 0) {
-            const int q = atomicAdd(&p.ctl->head_pr, 1);
+            const int q = atomicAdd(&p.ctl->q[parity].head_pr, 1);
             s_item = q < qlen ? p.queue_pr[q] : -1;
         }
         __syncthreads();
@@ -461,37 +469,25 @@ __global__ __launch_bounds__(NT) void k_pr(KParams p) {
         const int any_active = __syncthreads_or(mine);
         if (tid == 0) {
             set_active(p, t, any_active);
-            atomic_add64(&p.ctl->pr_tiles, 1);
-            atomic_add64(&p.ctl->pr_iters, it);
+            if (p.profile) atomic_add64(&p.ctl->pr_iters, it);
         }
         __syncthreads();
-    }
-    // The last participating workgroup resets this queue for the next step.
-    if (tid == 0) {
-        __threadfence();
-        const int participants = min((int)gridDim.x, qlen);
-        if (atomicAdd(&p.ctl->ticket_pr, 1) == participants - 1) {
-            p.ctl->qlen_pr = 0;
-            p.ctl->head_pr = 0;
-            __threadfence();
-            atomicExch(&p.ctl->ticket_pr, 0);
-        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // Global relabel (exact BFS by tile relaxation) and finalisation.
-__global__ __launch_bounds__(NT) void k_gr(KParams p) {
+__global__ __launch_bounds__(NT) void k_gr(KParams p, int parity) {
     __shared__ int Ds[LH][LW];
     __shared__ int s_item;
     __shared__ int64_t red[NT / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lx = lane + 1;
-    const int qlen = p.ctl->qlen_gr;
+    const int qlen = p.ctl->q[parity].qlen_gr;
     if ((int)blockIdx.x >= qlen) return;
     for (;;) {
         if (tid == 0) {
-            const int q = atomicAdd(&p.ctl->head_gr, 1);
+            const int q = atomicAdd(&p.ctl->q[parity].head_gr, 1);
             s_item = q < qlen ? p.queue_gr[q] : -1;
         }
         __syncthreads();
@@ -633,20 +629,8 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p) {
             if (chD && ti.nbr[3] >= 0) mark_dirty(p, ti.nbr[3]);
             if (!conv) mark_dirty(p, t);
             set_active(p, t, total > 0);
-            atomic_add64(&p.ctl->gr_tiles, 1);
         }
         __syncthreads();
-    }
-    // The last participating workgroup resets this queue for the next step.
-    if (tid == 0) {
-        __threadfence();
-        const int participants = min((int)gridDim.x, qlen);
-        if (atomicAdd(&p.ctl->ticket_gr, 1) == participants - 1) {
-            p.ctl->qlen_gr = 0;
-            p.ctl->head_gr = 0;
-            __threadfence();
-            atomicExch(&p.ctl->ticket_gr, 0);
-        }
     }
 }
 
@@ -656,7 +640,6 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p) {
 // reads and horizontal neighbours are one cross-lane move (DPP wave shift for
 // distance-1, ds_bpermute shuffles for the scan). No LDS arrays, no workgroup
 // barriers, no per-tile atomics; flags are plain stores read by the scheduler.
-constexpr int GRW_MAX_ROUNDS = 48;
 
 // lane x <- lane x-1 (lane 0 gets 0) / lane x <- lane x+1 (lane 63 gets 0).
 __device__ __forceinline__ int shr1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false); }
@@ -697,9 +680,9 @@ __device__ __forceinline__ int dmin_sat(int a, int b) { return min(min(a, b), DI
 // the row is already at its fixed point). Returns true when converged; chg
 // collects the rows whose value changed in this lane.
 __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], const int (&m)[TH], int hU, int hD, int hs, int lane,
-                                                uint32_t &chg, int &rounds) {
+                                                uint32_t &chg, int &rounds, int max_rounds) {
     rounds = 0;
-    while (rounds < GRW_MAX_ROUNDS) {
+    while (rounds < max_rounds) {
         ++rounds;
         uint32_t rch = 0;
 #pragma unroll
@@ -763,11 +746,11 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], const int (&m)[TH]
     return false;
 }
 
-__global__ __launch_bounds__(64) void k_grw(KParams p) {
-    const int qlen = p.ctl->qlen_gr;
+__global__ __launch_bounds__(64) void k_grw(KParams p, int parity) {
+    const int qlen = p.ctl->q[parity].qlen_gr;
     if ((int)blockIdx.x >= qlen) return;
     const int lane = threadIdx.x;
-    int64_t n_tiles = 0, n_rounds = 0;
+    int64_t n_rounds = 0;
     for (int q = blockIdx.x; q < qlen; q += gridDim.x) {
         const int item = p.queue_gr[q];
         const int op = item & 7, t = item >> 3;
@@ -816,7 +799,7 @@ __global__ __launch_bounds__(64) void k_grw(KParams p) {
         }
         uint32_t chg = 0;
         int rounds = 0;
-        const bool conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds);
+        const bool conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, p.gr_rounds);
         int act = 0;
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
@@ -836,28 +819,16 @@ __global__ __launch_bounds__(64) void k_grw(KParams p) {
             if (!conv) mark_dirty(p, t);
             set_active(p, t, any_act);
         }
-        n_tiles++;
         n_rounds += rounds;
     }
-    if (lane == 0) {
-        atomic_add64(&p.ctl->gr_tiles, n_tiles);
-        atomic_add64(&p.ctl->gr_iters, n_rounds);
-        __threadfence();
-        const int participants = min((int)gridDim.x, qlen);
-        if (atomicAdd(&p.ctl->ticket_gr, 1) == participants - 1) {
-            p.ctl->qlen_gr = 0;
-            p.ctl->head_gr = 0;
-            __threadfence();
-            atomicExch(&p.ctl->ticket_gr, 0);
-        }
-    }
+    if (p.profile && lane == 0) atomic_add64(&p.ctl->gr_iters, n_rounds);
 }
 
-__global__ __launch_bounds__(64) void k_prw(KParams p) {
-    const int qlen = p.ctl->qlen_pr;
+__global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
+    const int qlen = p.ctl->q[parity].qlen_pr;
     if ((int)blockIdx.x >= qlen) return;
     const int lane = threadIdx.x;
-    int64_t n_tiles = 0, n_iters = 0;
+    int64_t n_iters = 0;
     for (int q = blockIdx.x; q < qlen; q += gridDim.x) {
         const int t = p.queue_pr[q];
         const TileInfo ti = p.tiles[t];
@@ -1011,21 +982,9 @@ __global__ __launch_bounds__(64) void k_prw(KParams p) {
             if (anyD) set_active(p, ti.nbr[3], 1);
             set_active(p, t, any_mine);
         }
-        n_tiles++;
         n_iters += it;
     }
-    if (lane == 0) {
-        atomic_add64(&p.ctl->pr_tiles, n_tiles);
-        atomic_add64(&p.ctl->pr_iters, n_iters);
-        __threadfence();
-        const int participants = min((int)gridDim.x, qlen);
-        if (atomicAdd(&p.ctl->ticket_pr, 1) == participants - 1) {
-            p.ctl->qlen_pr = 0;
-            p.ctl->head_pr = 0;
-            __threadfence();
-            atomicExch(&p.ctl->ticket_pr, 0);
-        }
-    }
+    if (p.profile && lane == 0) atomic_add64(&p.ctl->pr_iters, n_iters);
 }
 
 // ---------------------------------------------------------------------------
@@ -1124,7 +1083,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         else if (lane >= TH && lane < 2 * TH && tx < D.tiles_x - 1) hs = input_d0(D, tx * TW + TW, Y0 + lane - TH);
         uint32_t chg = 0;
         int rounds = 0;
-        const bool conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds);
+        const bool conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, p.gr_rounds);
         int act = 0;
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
@@ -1218,26 +1177,29 @@ void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_til
     hipLaunchKernelGGL(k_build, dim3(grid), dim3(NT), 0, s, p, d_slots, d_slots + n, n, total_tiles);
 }
 
-void launch_sched(const KParams &p, hipStream_t s) { hipLaunchKernelGGL(k_sched, dim3(p.n_slots), dim3(NT), 0, s, p); }
-void launch_pr(const KParams &p, int grid, hipStream_t s) {
-    if (p.variant)
-        hipLaunchKernelGGL(k_prw, dim3(grid), dim3(64), 0, s, p);
-    else
-        hipLaunchKernelGGL(k_pr, dim3(grid), dim3(NT), 0, s, p);
+void launch_sched(const KParams &p, int parity, hipStream_t s) {
+    hipLaunchKernelGGL(k_sched, dim3(p.n_slots), dim3(NT), 0, s, p, parity);
 }
-void launch_gr(const KParams &p, int grid, hipStream_t s) {
+void launch_pr(const KParams &p, int grid, int parity, hipStream_t s) {
     if (p.variant)
-        hipLaunchKernelGGL(k_grw, dim3(grid), dim3(64), 0, s, p);
+        hipLaunchKernelGGL(k_prw, dim3(grid), dim3(64), 0, s, p, parity);
     else
-        hipLaunchKernelGGL(k_gr, dim3(grid), dim3(NT), 0, s, p);
+        hipLaunchKernelGGL(k_pr, dim3(grid), dim3(NT), 0, s, p, parity);
+}
+void launch_gr(const KParams &p, int grid, int parity, hipStream_t s) {
+    if (p.variant)
+        hipLaunchKernelGGL(k_grw, dim3(grid), dim3(64), 0, s, p, parity);
+    else
+        hipLaunchKernelGGL(k_gr, dim3(grid), dim3(NT), 0, s, p, parity);
+}
+
+void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s) {
+    launch_sched(p, parity, s);
+    launch_pr(p, grid_pr, parity, s);
+    launch_gr(p, grid_gr, parity, s);
 }
 void launch_reset_stats(const KParams &p, hipStream_t s) { hipLaunchKernelGGL(k_reset_stats, dim3(1), dim3(1), 0, s, p); }
 
-void launch_step(const KParams &p, int grid_pr, int grid_gr, hipStream_t s) {
-    launch_sched(p, s);
-    launch_pr(p, grid_pr, s);
-    launch_gr(p, grid_gr, s);
-}
 
 void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,
                  hipStream_t s) {

@@ -44,7 +44,7 @@ class Options(ctypes.Structure):
     _fields_ = [("sweeps_per_gr", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
                 ("steps_per_poll", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("max_steps", ctypes.c_int32), ("verbose", ctypes.c_int32), ("profile", ctypes.c_int32),
-                ("kernels", ctypes.c_int32)]
+                ("kernels", ctypes.c_int32), ("gr_rounds", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

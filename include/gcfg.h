@@ -44,6 +44,7 @@ typedef struct gcfg_options {
     int32_t verbose;         /* 1: print the reference's "pen"/energy lines */
     int32_t profile;         /* 1: time every push/relabel and global-relabel launch with HIP events */
     int32_t kernels;         /* 1: wavefront-per-tile kernels (default), 0: workgroup-per-tile LDS kernels */
+    int32_t gr_rounds;       /* max tile-local relaxation rounds per global-relabel sweep (then re-queued) */
 } gcfg_options;
 
 typedef struct gcfg_stats {
