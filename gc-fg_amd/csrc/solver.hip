@@ -650,21 +650,32 @@ __device__ __forceinline__ int64_t wave_sum64_w(int64_t v) { return wave_sum64(v
 // Halo distances: rows of the tiles above/below (one per lane) and the
 // columns of the tiles left/right (lanes 0..15 hold the left column, lanes
 // 16..31 the right column, indexed by row).
+// Halo distances: rows of the tiles above/below (one per lane) and the
+// columns of the tiles left/right (lanes 0..15 hold the left column, lanes
+// 16..31 the right column, indexed by row). The loads are issued
+// unconditionally from clamped addresses (a missing neighbour reads the own
+// tile) and masked afterwards, so they overlap with the tile's own loads.
 template <bool FROM_EXCESS>
-__device__ __forceinline__ int halo_value(const KParams &p, int nbr, size_t off) {
-    if (nbr < 0) return DINF;
-    const size_t g = (size_t)nbr * TN + off;
-    if (FROM_EXCESS) return p.exc[g] < 0 ? 0 : DINF;
-    return p.dist[g];
-}
-
-template <bool FROM_EXCESS>
-__device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, int lane, int &hU, int &hD, int &hs) {
-    hU = halo_value<FROM_EXCESS>(p, ti.nbr[2], (size_t)(TH - 1) * TW + lane);
-    hD = halo_value<FROM_EXCESS>(p, ti.nbr[3], (size_t)lane);
-    hs = DINF;
-    if (lane < TH) hs = halo_value<FROM_EXCESS>(p, ti.nbr[0], (size_t)lane * TW + TW - 1);
-    else if (lane < 2 * TH) hs = halo_value<FROM_EXCESS>(p, ti.nbr[1], (size_t)(lane - TH) * TW);
+__device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, int t, int lane, int &hU, int &hD,
+                                          int &hs) {
+    const int nu = ti.nbr[2], nd = ti.nbr[3];
+    const int side = lane < TH ? ti.nbr[0] : ti.nbr[1];
+    const size_t gu = (size_t)(nu >= 0 ? nu : t) * TN + (size_t)(TH - 1) * TW + lane;
+    const size_t gd = (size_t)(nd >= 0 ? nd : t) * TN + lane;
+    const size_t gs = (size_t)(side >= 0 ? side : t) * TN + (size_t)(lane & (TH - 1)) * TW + (lane < TH ? TW - 1 : 0);
+    int vu, vd, vs;
+    if (FROM_EXCESS) {
+        vu = p.exc[gu] < 0 ? 0 : DINF;
+        vd = p.exc[gd] < 0 ? 0 : DINF;
+        vs = p.exc[gs] < 0 ? 0 : DINF;
+    } else {
+        vu = p.dist[gu];
+        vd = p.dist[gd];
+        vs = p.dist[gs];
+    }
+    hU = nu >= 0 ? vu : DINF;
+    hD = nd >= 0 ? vd : DINF;
+    hs = (lane < 2 * TH && side >= 0) ? vs : DINF;
 }
 
 __device__ __forceinline__ int dmin_sat(int a, int b) { return min(min(a, b), DINF); }
@@ -760,14 +771,20 @@ __global__ __launch_bounds__(64) void k_grw(KParams p, int parity) {
             const InstDesc &D = p.desc[ti.inst];
             int64_t sx = 0;
             const int X = ti.tx * TW + lane;
+            int dd[TH], ex[TH];
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                const size_t g = base + (size_t)y * TW;
+                dd[y] = p.dist[g];
+                ex[y] = p.exc[g];
+            }
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
                 const int Y = ti.ty * TH + y;
                 if (X < D.W && Y < D.H) {
-                    const size_t g = base + (size_t)y * TW;
                     const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
-                    D.label_out[(size_t)Y * D.W + X] = (in && p.dist[g] >= DINF) ? 1 : 0;
-                    if (in) sx += max(p.exc[g], 0);
+                    D.label_out[(size_t)Y * D.W + X] = (in && dd[y] >= DINF) ? 1 : 0;
+                    if (in) sx += max(ex[y], 0);
                 }
             }
             sx = wave_sum64_w(sx);
@@ -778,24 +795,33 @@ __global__ __launch_bounds__(64) void k_grw(KParams p, int parity) {
         int d[TH], m[TH];
         int hU, hD, hs;
         if (init) {
+            // All loads first, then the mask stores (a store between loads would
+            // serialise every row on a possible alias).
+            load_halo<true>(p, ti, t, lane, hU, hD, hs);
+            int4 c[TH];
+            int ee[TH];
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
                 const size_t g = base + (size_t)y * TW;
-                const int4 c = p.caps[g];
-                const int ee = p.exc[g];
-                m[y] = (c.x > 0) | ((c.y > 0) << 1) | ((c.z > 0) << 2) | ((c.w > 0) << 3) | ((ee > 0) << 4);
-                p.mask[g] = (uint8_t)m[y];
-                d[y] = ee < 0 ? 0 : DINF;
+                c[y] = p.caps[g];
+                ee[y] = p.exc[g];
             }
-            load_halo<true>(p, ti, lane, hU, hD, hs);
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                m[y] = (c[y].x > 0) | ((c[y].y > 0) << 1) | ((c[y].z > 0) << 2) | ((c[y].w > 0) << 3) |
+                       ((ee[y] > 0) << 4);
+                d[y] = ee[y] < 0 ? 0 : DINF;
+            }
+#pragma unroll
+            for (int y = 0; y < TH; ++y) p.mask[base + (size_t)y * TW] = (uint8_t)m[y];
         } else {
+            load_halo<false>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
                 const size_t g = base + (size_t)y * TW;
                 m[y] = p.mask[g];
                 d[y] = p.dist[g];
             }
-            load_halo<false>(p, ti, lane, hU, hD, hs);
         }
         uint32_t chg = 0;
         int rounds = 0;
@@ -834,6 +860,8 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
         const TileInfo ti = p.tiles[t];
         const int dthr = p.desc[ti.inst].n_interior;
         const size_t base = (size_t)t * TN + lane;
+        int hU, hD, hs;
+        load_halo<false>(p, ti, t, lane, hU, hD, hs);
         int4 c[TH];
         int e[TH], d[TH], acc[TH];
 #pragma unroll
@@ -844,8 +872,6 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
             d[y] = p.dist[g];
             acc[y] = 0;
         }
-        int hU, hD, hs;
-        load_halo<false>(p, ti, lane, hU, hD, hs);
         int accU = 0, accD = 0;
         int it = 0;
         while (it < p.inner_iters) {
@@ -1025,9 +1051,10 @@ __global__ void k_clear_tiles(KParams p, const int32_t *slots, const int32_t *pr
 // Initial BFS distance of an input pixel: 0 if it is an interior node with
 // pos < neg (remaining sink capacity), otherwise unreachable so far.
 __device__ __forceinline__ int input_d0(const InstDesc &D, int X, int Y) {
-    if (X <= 0 || Y <= 0 || X >= D.W - 1 || Y >= D.H - 1) return DINF;
-    const size_t i = (size_t)Y * D.W + X;
-    return D.pos[i] < D.neg[i] ? 0 : DINF;
+    const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
+    const size_t i = in ? (size_t)Y * D.W + X : 0;  // clamped: the load is always in bounds
+    const int pp = D.pos[i], qq = D.neg[i];
+    return (in && pp < qq) ? 0 : DINF;
 }
 
 __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, const int32_t *prefix, int n,
@@ -1045,42 +1072,55 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         int d[TH], m[TH];
         int64_t sp = 0;
         int bad = 0;
-#pragma unroll
-        for (int y = 0; y < TH; ++y) {
-            const int Y = Y0 + y;
-            int4 c = make_int4(0, 0, 0, 0);
-            int e = 0;
-            if (X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1) {
-                const size_t i = (size_t)Y * D.W + X;
-                const int pp = D.pos[i], qq = D.neg[i];
-                if (D.up) {
-                    c.x = D.left[i];
-                    c.y = D.right[i];
-                    c.z = D.up[i];
-                    c.w = D.down[i];
-                } else {
-                    c = make_int4(D.penalty, D.penalty, D.penalty, D.penalty);
-                }
-                if (X == 1) c.x = 0;
-                if (X == D.W - 2) c.y = 0;
-                if (Y == 1) c.z = 0;
-                if (Y == D.H - 2) c.w = 0;
-                bad |= (pp < 0) | (qq < 0) | (c.x < 0) | (c.y < 0) | (c.z < 0) | (c.w < 0);
-                e = pp - qq;
-                sp += pp;
-            }
-            const size_t g = base + (size_t)y * TW;
-            p.caps[g] = c;
-            p.exc[g] = e;
-            m[y] = (c.x > 0) | ((c.y > 0) << 1) | ((c.z > 0) << 2) | ((c.w > 0) << 3) | ((e > 0) << 4);
-            p.mask[g] = (uint8_t)m[y];
-            d[y] = e < 0 ? 0 : DINF;
-        }
+        // Halo distances from the inputs (issued first, consumed after the rows).
         const int hU = ty > 0 ? input_d0(D, X, Y0 - 1) : DINF;
         const int hD = ty < D.tiles_y - 1 ? input_d0(D, X, Y0 + TH) : DINF;
-        int hs = DINF;
-        if (lane < TH && tx > 0) hs = input_d0(D, tx * TW - 1, Y0 + lane);
-        else if (lane >= TH && lane < 2 * TH && tx < D.tiles_x - 1) hs = input_d0(D, tx * TW + TW, Y0 + lane - TH);
+        const bool hleft = lane < TH && tx > 0, hright = lane >= TH && lane < 2 * TH && tx < D.tiles_x - 1;
+        const int hsv = input_d0(D, lane < TH ? tx * TW - 1 : tx * TW + TW, Y0 + (lane & (TH - 1)));
+        const int hs = (hleft || hright) ? hsv : DINF;
+        // Two halves of 8 rows: all input loads of a half, then compute and
+        // store (clamped in-bounds loads; values outside the interior masked).
+#pragma unroll
+        for (int h = 0; h < TH; h += TH / 2) {
+            int vp[TH / 2], vn[TH / 2];
+            int4 vc[TH / 2];
+            bool vin[TH / 2];
+#pragma unroll
+            for (int k = 0; k < TH / 2; ++k) {
+                const int Y = Y0 + h + k;
+                const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
+                const size_t i = in ? (size_t)Y * D.W + X : 0;
+                vin[k] = in;
+                vp[k] = D.pos[i];
+                vn[k] = D.neg[i];
+                if (D.up)
+                    vc[k] = make_int4(D.left[i], D.right[i], D.up[i], D.down[i]);
+                else
+                    vc[k] = make_int4(D.penalty, D.penalty, D.penalty, D.penalty);
+            }
+#pragma unroll
+            for (int k = 0; k < TH / 2; ++k) {
+                const int y = h + k, Y = Y0 + y;
+                int4 c = make_int4(0, 0, 0, 0);
+                int e = 0;
+                if (vin[k]) {
+                    c = vc[k];
+                    if (X == 1) c.x = 0;
+                    if (X == D.W - 2) c.y = 0;
+                    if (Y == 1) c.z = 0;
+                    if (Y == D.H - 2) c.w = 0;
+                    bad |= (vp[k] < 0) | (vn[k] < 0) | (c.x < 0) | (c.y < 0) | (c.z < 0) | (c.w < 0);
+                    e = vp[k] - vn[k];
+                    sp += vp[k];
+                }
+                const size_t g = base + (size_t)y * TW;
+                p.caps[g] = c;
+                p.exc[g] = e;
+                m[y] = (c.x > 0) | ((c.y > 0) << 1) | ((c.z > 0) << 2) | ((c.w > 0) << 3) | ((e > 0) << 4);
+                p.mask[g] = (uint8_t)m[y];
+                d[y] = e < 0 ? 0 : DINF;
+            }
+        }
         uint32_t chg = 0;
         int rounds = 0;
         const bool conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, p.gr_rounds);
