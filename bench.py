@@ -101,10 +101,9 @@ def run_plain(insts, graphs):
     return (time.perf_counter() - t0) * 1e3, np.array(flows, np.int64)
 
 
-def serve(args, opts):
+def serve(args, opts, qps, n):
     """Poisson stream against the continuous-batching server."""
     rng = np.random.default_rng(args.seed + 7)
-    n = args.serve_requests
     amin, amax = 256.0 ** 2, 2048.0 ** 2
     reqs = []
     sessions = []  # indices of image requests
@@ -126,7 +125,7 @@ def serve(args, opts):
             img = gcfg.gen_ellipse_image(W, H, gcfg.mix_seed(args.seed + 7, i))
             reqs.append(dict(kind="image", W=W, H=H, img=img, seeds=gcfg.gen_seed_map(W, H)))
             sessions.append(i)
-    gaps = rng.exponential(1000.0 / args.qps, n)
+    gaps = rng.exponential(1000.0 / qps, n)
     arrival = np.cumsum(gaps) - gaps[0]
     srv = gcfg.Server(args.serve_budget_gb << 30, opts)
     sess_of = {}
@@ -163,7 +162,7 @@ def serve(args, opts):
     srv.close()
     px = sum(q["W"] * q["H"] for q in reqs)
     L = np.array([lat[k] for k in range(n)])
-    return dict(requests=n, qps=args.qps, edits=sum(q["kind"] == "edit" for q in reqs),
+    return dict(requests=n, qps=qps, edits=sum(q["kind"] == "edit" for q in reqs),
                 p50_ms=round(float(np.percentile(L, 50)), 3), p99_ms=round(float(np.percentile(L, 99)), 3),
                 mean_ms=round(float(L.mean()), 3), mpx_s=round(px / (wall * 1e3), 2), wall_ms=round(wall, 1),
                 hbm_budget_gb=args.serve_budget_gb), reqs, flows
@@ -276,6 +275,13 @@ def main():
     ap.add_argument("--sweeps", type=int, default=0)
     ap.add_argument("--inner", type=int, default=0)
     ap.add_argument("--poll", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune-cache", default=os.environ.get("GCFG_TUNE_CACHE",
+                                                           os.path.expanduser("~/.cache/gcfg/tune.txt")))
+    ap.add_argument("--serve-qps-high", type=float, default=400.0)
+    ap.add_argument("--serve-requests-high", type=int, default=200)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="copy 1 GiB device-to-device first (a known-size kernel for profiler counters)")
     ap.add_argument("--single", choices=["c1", "c2"], default=None,
                     help="time one plain GC_* solve instead: c1 = 1024^2 ellipse image, "
                          "c2 = 4096^2 heavy-tailed t-links on Perlin n-links")
@@ -307,7 +313,34 @@ def main():
     total_px = sum(it.W * it.H for it in insts)
     log(f"batch: {total_px / 1e6:.2f} Mpx, data term {dt_ms:.1f} ms")
 
+    if args.calibrate_copy:
+        a, b = gcfg.DeviceBuffer(1 << 30), gcfg.DeviceBuffer(1 << 30)
+        b.copy_from_device(a.ptr, 1 << 30)
+        gcfg.device_synchronize()
+        a.free()
+        b.free()
     batch = gcfg.Batch(batch_items(insts), opts)
+    tuned = None
+    if not args.no_autotune and not (args.sweeps or args.inner or args.poll):
+        key = f"batch:ellipse:n{args.instances}:seed{args.seed}"
+        cache = args.tune_cache
+        try:
+            os.makedirs(os.path.dirname(cache) or ".", exist_ok=True)
+            tuned = gcfg.tune_lookup(cache, key)
+        except (OSError, gcfg.GcfgError):
+            cache, tuned = None, None
+        if tuned is None:
+            t0 = time.perf_counter()
+            tuned, tms = batch.autotune(trials=2)
+            if cache:
+                try:
+                    gcfg.tune_store(cache, key, tuned)
+                except (OSError, gcfg.GcfgError):
+                    pass
+            log(f"autotune: {gcfg.options_dict(tuned)} ({tms:.2f} ms) in {time.perf_counter() - t0:.1f} s")
+        batch.close()
+        opts = tuned
+        batch = gcfg.Batch(batch_items(insts), opts)
     for _ in range(args.warmup):
         flows_b, _, _ = batch.solve(labels_to_host=False)
     gcfg.device_synchronize()
@@ -349,9 +382,21 @@ def main():
 
     extra = {}
     if not args.no_plain:
+        # The plain path gets its own tuned options (on the median-size instance).
+        popts = gcfg.Options.from_buffer_copy(opts)
+        if tuned is not None:
+            med = sorted(insts, key=lambda it: it.W * it.H)[len(insts) // 2]
+            g = gcfg.Graph(med.W, med.H, gcfg.default_options())
+            g.set_dataterms(med.bufs[0], med.bufs[1])
+            g.set_edges(*med.bufs[2:6])
+            g.build()
+            popts, _ = g.autotune(trials=2)
+            g.close()
+            extra["plain_options"] = {k: popts.__getattribute__(k) for k in ("sweeps_per_gr", "inner_iters",
+                                                                           "gr_rounds", "steps_per_poll")}
         graphs = []
         for it in insts:
-            g = gcfg.Graph(it.W, it.H, opts)
+            g = gcfg.Graph(it.W, it.H, popts)
             g.set_dataterms(it.bufs[0], it.bufs[1])
             g.set_edges(*it.bufs[2:6])
             g.build()
@@ -373,9 +418,13 @@ def main():
 
     if not args.no_serve:
         log(f"serving {args.serve_requests} requests at {args.qps} qps")
-        sv, reqs, sflows = serve(args, opts)
+        sv, reqs, sflows = serve(args, opts, args.qps, args.serve_requests)
         extra["serve"] = sv
         log(f"serve: {sv}")
+        log(f"serving {args.serve_requests_high} requests at {args.serve_qps_high} qps")
+        sv2, _, _ = serve(args, opts, args.serve_qps_high, args.serve_requests_high)
+        extra["serve_high_load"] = sv2
+        log(f"serve (high load): {sv2}")
 
     cpu = None
     if not args.no_cpu:
@@ -392,7 +441,8 @@ def main():
                                f"aspect 1:2..2:1, 16-bin colour data term, lambda 50, cold solves",
                    "instances": args.instances, "total_mpx": round(total_px / 1e6, 3), "seed": args.seed,
                    "sweeps_per_gr": opts.sweeps_per_gr, "inner_iters": opts.inner_iters,
-                   "steps_per_poll": opts.steps_per_poll},
+                   "steps_per_poll": opts.steps_per_poll, "gr_rounds": opts.gr_rounds,
+                   "autotuned": tuned is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "dataterm_mpx_s": round(total_px / dt_ms / 1e3, 2),

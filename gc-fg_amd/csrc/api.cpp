@@ -376,4 +376,41 @@ int gcfg_batch_destroy(gcfg_batch *b) {
     return GCFG_OK;
 }
 
+// ---------------------------------------------------------------------------
+int gcfg_graph_autotune(gcfg_graph *g, int32_t trials, gcfg_options *best, double *best_ms) {
+    if (!g) return GCFG_ERR_INVALID;
+    const gcfg_options start = g->opt;
+    auto solve = [&](const gcfg_options &o) -> double {
+        if (gcfg_graph_set_options(g, &o) != GCFG_OK) return -1.0;
+        const double t0 = now_ms();
+        gcfg_stats st;
+        if (gcfg_graph_build(g) != GCFG_OK || graph_solve(g, g->label_target, &st) != GCFG_OK) return -1.0;
+        return now_ms() - t0;
+    };
+    gcfg_options win;
+    int rc = gcfg::coordinate_search(start, trials, solve, &win, best_ms);
+    if (rc != GCFG_OK) {
+        gcfg_graph_set_options(g, &start);
+        return rc;
+    }
+    if (best) *best = win;
+    return gcfg_graph_set_options(g, &win);
+}
+
+int gcfg_batch_autotune(gcfg_batch *b, int32_t trials, gcfg_options *best, double *best_ms) {
+    if (!b) return GCFG_ERR_INVALID;
+    const gcfg_options start = b->eng.opt;
+    auto solve = [&](const gcfg_options &o) -> double {
+        b->eng.opt = o;
+        const double t0 = now_ms();
+        if (batch_run(b, nullptr, nullptr, nullptr) != GCFG_OK) return -1.0;
+        return now_ms() - t0;
+    };
+    gcfg_options win;
+    int rc = gcfg::coordinate_search(start, trials, solve, &win, best_ms);
+    b->eng.opt = rc == GCFG_OK ? win : start;
+    if (rc == GCFG_OK && best) *best = win;
+    return rc;
+}
+
 }  // extern "C"

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -201,5 +202,7 @@ class Engine {
 };
 
 double now_ms();
+int coordinate_search(gcfg_options start, int trials, const std::function<double(const gcfg_options &)> &solve,
+                      gcfg_options *best, double *best_ms);
 
 }  // namespace gcfg

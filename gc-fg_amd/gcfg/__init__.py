@@ -110,6 +110,21 @@ def now_ms() -> float:
     return float(lib().gcfg_now_ms())
 
 
+def tune_store(path: str, key: str, options: Options):
+    _check(lib().gcfg_tune_store(path.encode(), key.encode(), ctypes.byref(options)), "gcfg_tune_store")
+
+
+def tune_lookup(path: str, key: str):
+    """Stored options for key, or None."""
+    o = Options()
+    rc = lib().gcfg_tune_lookup(path.encode(), key.encode(), ctypes.byref(o))
+    return o if rc == GCFG_OK else None
+
+
+def options_dict(o: Options) -> dict:
+    return {k: getattr(o, k) for k, _ in Options._fields_}
+
+
 def default_options() -> Options:
     o = Options()
     lib().gcfg_default_options(ctypes.byref(o))
@@ -284,6 +299,13 @@ class Graph:
                "gcfg_graph_optimize_device")
         return st
 
+    def autotune(self, trials: int = 2):
+        """Pick solver options by timing cold solves of this graph (built inputs)."""
+        best, ms = Options(), ctypes.c_double()
+        _check(lib().gcfg_graph_autotune(_VP(self.h), int(trials), ctypes.byref(best), ctypes.byref(ms)),
+               "gcfg_graph_autotune")
+        return best, float(ms.value)
+
     def read_state(self):
         caps = np.empty((self.height, self.width, 4), np.int32)
         exc = np.empty((self.height, self.width), np.int32)
@@ -454,6 +476,13 @@ class Batch:
         _check(lib().gcfg_batch_solve_device(_VP(self.h), ptrs, _VP(flows.ctypes.data), ctypes.byref(st)),
                "gcfg_batch_solve_device")
         return flows, None, st
+
+    def autotune(self, trials: int = 2):
+        """Pick solver options by timing cold solves of the whole batch."""
+        best, ms = Options(), ctypes.c_double()
+        _check(lib().gcfg_batch_autotune(_VP(self.h), int(trials), ctypes.byref(best), ctypes.byref(ms)),
+               "gcfg_batch_autotune")
+        return best, float(ms.value)
 
     def device_labels(self, i: int) -> np.ndarray:
         it = self.items[i]

@@ -36,6 +36,7 @@ SOURCES = [
     ("dataterm.hip", "hip"),
     ("server.cpp", "host"),
     ("compat.cpp", "host"),
+    ("tune.cpp", "host"),
 ]
 
 

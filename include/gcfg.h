@@ -136,6 +136,18 @@ int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_sta
 int gcfg_batch_solve_device(gcfg_batch *b, int32_t **d_labels, int64_t *flows, gcfg_stats *stats);
 int gcfg_batch_destroy(gcfg_batch *b);
 
+/* ---- Autotuning of the solver options ----
+ * Coordinate search over inner_iters, sweeps_per_gr, gr_rounds and
+ * steps_per_poll with cold solves of the handle's own inputs (the fastest of
+ * `trials` solves per candidate); the winner is applied to the handle and
+ * returned. Results are bit-identical for every candidate. */
+int gcfg_graph_autotune(gcfg_graph *g, int32_t trials, gcfg_options *best, double *best_ms);
+int gcfg_batch_autotune(gcfg_batch *b, int32_t trials, gcfg_options *best, double *best_ms);
+/* Persisted choices: one "key value..." line per workload class in a text file. */
+int gcfg_tune_store(const char *path, const char *key, const gcfg_options *opt);
+/* GCFG_ERR_STATE if the key is absent. */
+int gcfg_tune_lookup(const char *path, const char *key, gcfg_options *opt);
+
 /* ---- Serving: continuous batching under an HBM budget, warm-started edits ---- */
 typedef struct gcfg_server gcfg_server;
 typedef struct gcfg_result {

@@ -64,3 +64,31 @@ def test_batch_of_ellipse_images_equals_plain(g, ora):
     for k, (f, l) in enumerate(plain):
         assert flows[k] == f and np.array_equal(labels[k], l)
     b.close()
+
+
+def test_autotune_keeps_results(g, ora):
+    rng = np.random.default_rng(8)
+    items, refs = [], []
+    for i, (W, H) in enumerate([(130, 70), (64, 64), (200, 90)]):
+        pos, neg, E = random_instance(rng, W, H, tmax=150, emax=40)
+        items.append(g.BatchItem(W, H, pos, neg, edges=E))
+        refs.append(ora.maxflow(W, H, pos, neg, edges=E))
+    b = g.Batch(items)
+    best, ms = b.autotune(trials=1)
+    assert ms > 0 and best.inner_iters >= 1 and best.sweeps_per_gr >= 1
+    flows, labels, _ = b.solve()
+    for k, (rf, rl) in enumerate(refs):
+        assert flows[k] == rf and np.array_equal(labels[k], rl)
+    b.close()
+    W, H = 150, 80
+    pos, neg, E = random_instance(rng, W, H, tmax=150, emax=40)
+    gr = g.Graph(W, H)
+    gr.set_dataterms(pos, neg)
+    gr.set_edges(*E)
+    gr.build()
+    best, ms = gr.autotune(trials=1)
+    gr.build()
+    label, st = gr.optimize()
+    rf, rl = ora.maxflow(W, H, pos, neg, edges=E)
+    assert st.flow == rf and np.array_equal(label, rl)
+    gr.close()

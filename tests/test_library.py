@@ -74,3 +74,19 @@ def test_library_fails_loudly_without_gpu_work(g):
 
     with pytest.raises(g.GcfgError):
         g.Graph(10, 10)
+
+
+def test_tune_cache_roundtrip(g, tmp_path):
+    path = str(tmp_path / "tune.txt")
+    assert g.tune_lookup(path, "batch:256") is None
+    o = g.default_options()
+    o.inner_iters, o.sweeps_per_gr, o.gr_rounds = 2, 4, 8
+    g.tune_store(path, "batch:256", o)
+    o2 = g.default_options()
+    o2.inner_iters = 16
+    g.tune_store(path, "graph:4096x4096", o2)
+    g.tune_store(path, "batch:256", o)  # replaces, does not duplicate
+    got = g.tune_lookup(path, "batch:256")
+    assert g.options_dict(got) == g.options_dict(o)
+    assert g.tune_lookup(path, "graph:4096x4096").inner_iters == 16
+    assert open(path).read().count("batch:256") == 1
