@@ -39,8 +39,12 @@ import gcfg  # noqa: E402
 
 METRIC = "min-cut Mpixels/s (bit-exact) batched+served vs plain on 1 GPU; p99 latency"
 HBM_PEAK_GBS = 8000.0
-PR_BYTES_PER_NODE = 48   # caps 16 + excess 4 + distance 4, read and written, per tile pass
-GR_BYTES_PER_NODE = 12   # distance read + write, residual mask read, per BFS tile sweep
+# Algorithmic bytes per node of one tile visit (halo rows and columns of a
+# few hundred bytes per 1024-node tile left out).
+PR_BYTES_PER_NODE = 49      # caps 16 + excess 4 + distance 4 read and written, mask 1 written
+GR_INIT_BYTES_PER_NODE = 5  # initial BFS sweep from the kept mask: mask 1 read, distance 4 written
+GR_RELAX_BYTES_PER_NODE = 9  # relaxation sweep: mask 1 + distance 4 read, distance 4 written
+FIN_BYTES_PER_NODE = 12     # labels: distance 4 + excess 4 read, label 4 written
 COMPULSORY_BYTES_PER_PX = 28  # 6 int32 inputs read + int32 label written
 
 
@@ -275,6 +279,8 @@ def main():
     ap.add_argument("--sweeps", type=int, default=0)
     ap.add_argument("--inner", type=int, default=0)
     ap.add_argument("--poll", type=int, default=0)
+    ap.add_argument("--gr-block", default=None, help="global-relabel block WxH in tiles, e.g. 1x4")
+    ap.add_argument("--gr-cap", type=int, default=None, help="distance cap of intermediate global relabels")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune-cache", default=os.environ.get("GCFG_TUNE_CACHE",
                                                            os.path.expanduser("~/.cache/gcfg/tune.txt")))
@@ -299,6 +305,10 @@ def main():
             opts.sweeps_per_gr = args.sweeps
         if args.inner:
             opts.inner_iters = args.inner
+        if args.gr_block:
+            opts.gr_block_w, opts.gr_block_h = (int(v) for v in args.gr_block.split("x"))
+        if args.gr_cap is not None:
+            opts.gr_cap = args.gr_cap
         single(args, opts)
         return
     if args.sweeps:
@@ -307,6 +317,10 @@ def main():
         opts.inner_iters = args.inner
     if args.poll:
         opts.steps_per_poll = args.poll
+    if args.gr_block:
+        opts.gr_block_w, opts.gr_block_h = (int(v) for v in args.gr_block.split("x"))
+    if args.gr_cap is not None:
+        opts.gr_cap = args.gr_cap
 
     log(f"generating {args.instances} instances + device data terms")
     insts, dt_ms = make_batch(args.instances, args.seed)
@@ -321,7 +335,8 @@ def main():
         b.free()
     batch = gcfg.Batch(batch_items(insts), opts)
     tuned = None
-    if not args.no_autotune and not (args.sweeps or args.inner or args.poll):
+    if not args.no_autotune and not (args.sweeps or args.inner or args.poll or args.gr_block or
+                                     args.gr_cap is not None):
         key = f"batch:ellipse:n{args.instances}:seed{args.seed}"
         cache = args.tune_cache
         try:
@@ -361,8 +376,10 @@ def main():
     popts.profile = 1
     pbatch = gcfg.Batch(batch_items(insts), popts)
     flows_p, _, pst = pbatch.solve(labels_to_host=False)
+    n_tiles = sum(((it.W + 63) // 64) * ((it.H + 15) // 16) for it in insts)
     pr_bytes = pst.pr_tiles * 1024 * PR_BYTES_PER_NODE
-    gr_bytes = pst.gr_tiles * 1024 * GR_BYTES_PER_NODE
+    gr_bytes = 1024 * (pst.gr_init_tiles * GR_INIT_BYTES_PER_NODE +
+                       (pst.gr_tiles - pst.gr_init_tiles) * GR_RELAX_BYTES_PER_NODE + n_tiles * FIN_BYTES_PER_NODE)
     kern = {"k_pr": (pst.pr_kernel_ms, pr_bytes), "k_gr": (pst.gr_kernel_ms, gr_bytes)}
     dom = max(kern, key=lambda k: kern[k][0])
     dms, dbytes = kern[dom]
@@ -376,7 +393,8 @@ def main():
                     sched_kernel_ms=round(pst.sched_kernel_ms, 3),
                     solve_alg_frac=round(solve_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     compulsory_frac=round(COMPULSORY_BYTES_PER_PX * total_px / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
-                    pr_tiles=int(pst.pr_tiles), gr_tiles=int(pst.gr_tiles))
+                    pr_tiles=int(pst.pr_tiles), gr_tiles=int(pst.gr_tiles),
+                    gr_init_tiles=int(pst.gr_init_tiles), fin_tiles=int(n_tiles))
     assert np.array_equal(flows_p, flows_b), "profiled solve differs"
     pbatch.close()
 
@@ -392,8 +410,9 @@ def main():
             g.build()
             popts, _ = g.autotune(trials=2)
             g.close()
-            extra["plain_options"] = {k: popts.__getattribute__(k) for k in ("sweeps_per_gr", "inner_iters",
-                                                                           "gr_rounds", "steps_per_poll")}
+            extra["plain_options"] = {k: getattr(popts, k) for k in ("sweeps_per_gr", "inner_iters", "gr_rounds",
+                                                                   "steps_per_poll", "gr_block_w", "gr_block_h",
+                                                                   "gr_cap")}
         graphs = []
         for it in insts:
             g = gcfg.Graph(it.W, it.H, popts)
@@ -442,6 +461,7 @@ def main():
                    "instances": args.instances, "total_mpx": round(total_px / 1e6, 3), "seed": args.seed,
                    "sweeps_per_gr": opts.sweeps_per_gr, "inner_iters": opts.inner_iters,
                    "steps_per_poll": opts.steps_per_poll, "gr_rounds": opts.gr_rounds,
+                   "gr_block": f"{opts.gr_block_w}x{opts.gr_block_h}", "gr_cap": opts.gr_cap,
                    "autotuned": tuned is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -49,6 +49,9 @@ void gcfg_default_options(gcfg_options *o) {
     o->profile = 0;
     o->kernels = 1;
     o->gr_rounds = 8;
+    o->gr_block_w = 1;
+    o->gr_block_h = 1;
+    o->gr_cap = 0;
 }
 
 const char *gcfg_last_error(void) { return gcfg::last_error_cstr(); }
@@ -137,10 +140,11 @@ int gcfg_graph_set_penalty(gcfg_graph *g, int32_t penalty) {
 }
 
 int gcfg_graph_set_options(gcfg_graph *g, const gcfg_options *opt) {
-    if (!g || !opt || opt->sweeps_per_gr <= 0 || opt->inner_iters <= 0 || opt->steps_per_poll <= 0) {
-        set_error("gcfg_graph_set_options: invalid options");
+    if (!g || !opt) {
+        set_error("gcfg_graph_set_options: null argument");
         return GCFG_ERR_INVALID;
     }
+    if (!gcfg::options_valid(*opt)) return GCFG_ERR_INVALID;
     g->opt = *opt;
     g->eng.opt = *opt;
     return GCFG_OK;
@@ -191,6 +195,7 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
         stats->flow = st.flow;
         stats->pr_tiles = st.pr_tiles;
         stats->gr_tiles = st.gr_tiles;
+        stats->gr_init_tiles = st.gr_init_tiles;
         stats->pr_iters = ctl.pr_iters;
         stats->gr_iters = ctl.gr_iters;
         stats->steps = st.steps;
@@ -237,6 +242,18 @@ int gcfg_graph_read_state(gcfg_graph *g, int32_t *h_caps, int32_t *h_excess) {
         return GCFG_ERR_STATE;
     }
     return g->eng.read_state(g->slot, h_caps, h_excess);
+}
+
+int gcfg_graph_read_distances(gcfg_graph *g, int32_t *h_dist) {
+    if (!g || !h_dist) {
+        set_error("gcfg_graph_read_distances: null argument");
+        return GCFG_ERR_INVALID;
+    }
+    if (!g->built) {
+        set_error("gcfg_graph_read_distances: graph not built");
+        return GCFG_ERR_STATE;
+    }
+    return g->eng.read_state(g->slot, nullptr, nullptr, h_dist);
 }
 
 int gcfg_graph_destroy(gcfg_graph *g) {
@@ -330,6 +347,7 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
         agg.flow += st.flow;
         agg.pr_tiles += st.pr_tiles;
         agg.gr_tiles += st.gr_tiles;
+        agg.gr_init_tiles += st.gr_init_tiles;
         agg.steps = st.steps > agg.steps ? st.steps : agg.steps;
         agg.gr_rounds += st.gr_rounds;
     }

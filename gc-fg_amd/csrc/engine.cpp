@@ -5,6 +5,7 @@
 // poll per chunk.
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
 #include <cstring>
 
 #include "engine.hpp"
@@ -35,15 +36,13 @@ Engine::~Engine() {
     if (pin_desc_) hipHostFree(pin_desc_);
     if (pin_slots_) hipHostFree(pin_slots_);
     if (pin_ctl_) hipHostFree(pin_ctl_);
+    if (pin_qtrace_) hipHostFree(pin_qtrace_);
     if (stream_) hipStreamDestroy(stream_);
 }
 
 int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
     opt = o;
-    if (opt.sweeps_per_gr <= 0 || opt.inner_iters <= 0 || opt.steps_per_poll <= 0) {
-        set_error("options: sweeps_per_gr, inner_iters and steps_per_poll must be positive");
-        return GCFG_ERR_INVALID;
-    }
+    if (!options_valid(opt)) return GCFG_ERR_INVALID;
     cap_tiles = std::max(cap_tiles, 1);
     cap_slots = std::max(cap_slots, 1);
     cap_tiles_ = cap_tiles;
@@ -70,6 +69,7 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
     rc |= dalloc((void **)&kp_.tiles, (size_t)cap_tiles * sizeof(TileInfo));
     rc |= dalloc((void **)&kp_.act_flag, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.dirty, (size_t)cap_tiles * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.live, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.queue_pr, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.queue_gr, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.desc, (size_t)cap_slots * sizeof(InstDesc));
@@ -85,6 +85,7 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
     GCFG_HIP(hipMemsetAsync(kp_.ctl, 0, sizeof(Ctl), stream_));
     GCFG_HIP(hipMemsetAsync(kp_.dirty, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
     GCFG_HIP(hipMemsetAsync(kp_.act_flag, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
+    GCFG_HIP(hipMemsetAsync(kp_.live, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
     GCFG_HIP(hipStreamSynchronize(stream_));
     kp_.n_slots = cap_slots;
     apply_options();
@@ -96,21 +97,43 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
     return GCFG_OK;
 }
 
+bool options_valid(const gcfg_options &o) {
+    if (o.sweeps_per_gr <= 0 || o.inner_iters <= 0 || o.steps_per_poll <= 0) {
+        set_error("options: sweeps_per_gr, inner_iters and steps_per_poll must be positive");
+        return false;
+    }
+    if (o.gr_cap < 0) {
+        set_error("options: gr_cap must be >= 0");
+        return false;
+    }
+    if ((o.gr_block_w || o.gr_block_h) && !gr_block_supported(o.gr_block_w, o.gr_block_h)) {
+        set_error("options: unsupported gr_block_w x gr_block_h (1x1, 1x2, 1x4, 1x8, 1x16, 2x2, 2x4, 2x8, 4x4)");
+        return false;
+    }
+    return true;
+}
+
 // Kernel parameters and grid sizes follow the current options; a captured
 // step graph with stale parameters is dropped.
 void Engine::apply_options() {
     const int cu = std::max(cu_count_, 1);
     const int variant = opt.kernels ? 1 : 0;
+    int gbx = opt.gr_block_w > 0 ? opt.gr_block_w : 1, gby = opt.gr_block_h > 0 ? opt.gr_block_h : 1;
+    if (!variant || !gr_block_supported(gbx, gby)) gbx = gby = 1;
     const int gpr = variant ? cu * 16 : cu * 4;
-    const int ggr = variant ? cu * 16 : cu * 8;
+    const int ggr = variant ? std::max(cu, cu * 16 / (gbx * gby)) : cu * 8;
     const int grr = opt.gr_rounds > 0 ? opt.gr_rounds : 48;
     const int prof = opt.profile ? 1 : 0;
+    const int cap = variant && opt.gr_cap > 0 ? opt.gr_cap : 0;
     if (kp_.inner_iters != opt.inner_iters || kp_.variant != variant || grid_pr_ != gpr || grid_gr_ != ggr ||
-        kp_.gr_rounds != grr || kp_.profile != prof) {
+        kp_.gr_rounds != grr || kp_.profile != prof || kp_.gbx != gbx || kp_.gby != gby || kp_.gr_cap != cap) {
+        kp_.gr_cap = cap;
         kp_.inner_iters = opt.inner_iters;
         kp_.gr_rounds = grr;
         kp_.profile = prof;
         kp_.variant = variant;
+        kp_.gbx = gbx;
+        kp_.gby = gby;
         grid_pr_ = gpr;
         grid_gr_ = ggr;
         if (graph_exec_) {
@@ -331,6 +354,13 @@ int Engine::launch_steps(int n) {
                 events_.push_back(e);
             }
         }
+        const bool trace = opt.verbose >= 2;
+        if (trace && qtrace_cap_ < n) {
+            if (pin_qtrace_) hipHostFree(pin_qtrace_);
+            pin_qtrace_ = nullptr;
+            GCFG_HIP(hipHostMalloc((void **)&pin_qtrace_, (size_t)n * sizeof(QCtl), hipHostMallocDefault));
+            qtrace_cap_ = n;
+        }
         for (int i = 0; i < n; ++i) {
             GCFG_HIP(hipEventRecord(events_[4 * i], stream_));
             launch_sched(kp_, parity_, stream_);
@@ -339,6 +369,10 @@ int Engine::launch_steps(int n) {
             GCFG_HIP(hipEventRecord(events_[4 * i + 2], stream_));
             launch_gr(kp_, grid_gr_, parity_, stream_);
             GCFG_HIP(hipEventRecord(events_[4 * i + 3], stream_));
+            // This step's queue lengths, before the next scheduler clears them.
+            if (trace)
+                GCFG_HIP(hipMemcpyAsync(pin_qtrace_ + i, &kp_.ctl->q[parity_], sizeof(QCtl), hipMemcpyDeviceToHost,
+                                        stream_));
             parity_ ^= 1;
         }
         events_pending_ = n;
@@ -395,6 +429,10 @@ int Engine::collect_profile() {
         prof_sched_ms += a;
         prof_pr_ms += b;
         prof_gr_ms += c;
+        if (opt.verbose >= 2 && pin_qtrace_ && i < qtrace_cap_)
+            std::printf("step %lld: pr %d items %.1f us, gr %d items %.1f us, sched %.1f us\n",
+                        (long long)prof_steps, pin_qtrace_[i].qlen_pr, b * 1e3, pin_qtrace_[i].qlen_gr, c * 1e3,
+                        a * 1e3);
         prof_steps++;
     }
     events_pending_ = 0;
@@ -436,21 +474,30 @@ int Engine::fetch_ctl(Ctl *out) {
     return GCFG_OK;
 }
 
-int Engine::read_state(int slot, int32_t *h_caps, int32_t *h_exc) {
+int Engine::read_state(int slot, int32_t *h_caps, int32_t *h_exc, int32_t *h_dist) {
     const InstDesc &d = h_desc_[slot];
     const size_t n = (size_t)d.W * d.H;
-    int32_t *dc = nullptr, *de = nullptr;
-    GCFG_HIP(hipMalloc((void **)&dc, n * 4 * sizeof(int32_t)));
-    GCFG_HIP(hipMalloc((void **)&de, n * sizeof(int32_t)));
-    launch_read_state(kp_, slot, dc, de, stream_);
-    hipError_t e1 = hipMemcpyAsync(h_caps, dc, n * 4 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
-    hipError_t e2 = hipMemcpyAsync(h_exc, de, n * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
-    hipError_t e3 = hipStreamSynchronize(stream_);
-    hipFree(dc);
-    hipFree(de);
+    int32_t *dc = nullptr, *de = nullptr, *dd = nullptr;
+    hipError_t e0 = hipSuccess;
+    if (h_caps && e0 == hipSuccess) e0 = hipMalloc((void **)&dc, n * 4 * sizeof(int32_t));
+    if (h_exc && e0 == hipSuccess) e0 = hipMalloc((void **)&de, n * sizeof(int32_t));
+    if (h_dist && e0 == hipSuccess) e0 = hipMalloc((void **)&dd, n * sizeof(int32_t));
+    hipError_t e1 = hipSuccess, e2 = hipSuccess, e3 = hipSuccess, e4 = hipSuccess;
+    if (e0 == hipSuccess) {
+        launch_read_state(kp_, slot, dc, de, dd, stream_);
+        if (dc) e1 = hipMemcpyAsync(h_caps, dc, n * 4 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
+        if (de) e2 = hipMemcpyAsync(h_exc, de, n * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
+        if (dd) e3 = hipMemcpyAsync(h_dist, dd, n * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
+        e4 = hipStreamSynchronize(stream_);
+    }
+    if (dc) hipFree(dc);
+    if (de) hipFree(de);
+    if (dd) hipFree(dd);
+    GCFG_HIP(e0);
     GCFG_HIP(e1);
     GCFG_HIP(e2);
     GCFG_HIP(e3);
+    GCFG_HIP(e4);
     return GCFG_OK;
 }
 

@@ -4,7 +4,9 @@
 // contiguous range of TILES; a tile is TW x TH nodes stored contiguously
 // (tile-major), row-major inside the tile. Per node: residual capacities
 // int4 {left, right, up, down} (16 B), excess int32 (4 B), distance label
-// int32 (4 B) and a residual mask byte (1 B, global-relabel only).
+// int32 (4 B) and a residual mask byte (1 B: bits 0-3 residual arc l/r/u/d,
+// bit 4 excess > 0, bit 5 excess < 0), kept current by the push-relabel
+// passes so a global relabel starts from the mask alone.
 // Excess < 0 means remaining sink capacity (t-links are pre-saturated,
 // GraphCutKernels.cu:196-197). Pixels outside the interior are inert nodes
 // with zero capacities (GraphCutKernels.cu:122-146).
@@ -38,7 +40,10 @@ constexpr int LW = TW + 2;              // LDS row pitch with halo
 constexpr int LH = TH + 2;
 
 enum Phase : int32_t { PH_IDLE = 0, PH_START = 1, PH_GR = 2, PH_PR = 3, PH_FIN = 4, PH_DONE = 5 };
-enum GrOp : int32_t { OP_GR_INIT = 1, OP_GR_RELAX = 2, OP_FIN = 3 };
+// OP_GR_INIT derives masks from capacities and excess (after a build or an
+// edit); OP_GR_REINIT starts the BFS from the maintained masks.
+enum GrOp : int32_t { OP_GR_INIT = 1, OP_GR_RELAX = 2, OP_FIN = 3, OP_GR_REINIT = 4 };
+constexpr int MB_EPOS = 16, MB_ENEG = 32;  // mask bits: excess > 0, excess < 0
 
 struct TileInfo {
     int32_t inst;
@@ -63,9 +68,12 @@ struct InstState {
     int64_t sum_exc;       // interior sum of max(e, 0) at FIN
     int64_t flow;
     int64_t pr_tiles;      // tile push-relabel passes scheduled
-    int64_t gr_tiles;      // tile BFS sweeps scheduled (including FIN)
+    int64_t gr_tiles;      // tile BFS sweeps scheduled (initial and relaxation)
     int32_t err;
     int32_t sweeps;        // colour-pass pairs between global relabels
+    int64_t gr_init_tiles; // of which initial sweeps (INIT / REINIT)
+    int32_t cap;           // distance cap of the current global relabel (DINF: exact)
+    int32_t pad_;
 };
 
 // Queue counters of one step. Steps alternate between two sets: the
@@ -86,6 +94,7 @@ struct KParams {
     // arena
     int4 *caps;
     int32_t *exc, *dist, *act_flag, *dirty;
+    int32_t *live;  // per tile: some node had a finite distance at its last BFS sweep
     uint8_t *mask;
     TileInfo *tiles;
     InstDesc *desc;
@@ -97,6 +106,8 @@ struct KParams {
     int32_t variant;  // 0: workgroup-per-tile kernels (LDS), 1: wavefront-per-tile kernels (registers)
     int32_t gr_rounds;  // max relaxation rounds per tile per global-relabel sweep
     int32_t profile;    // count inner iterations / relaxation rounds (extra atomics)
+    int32_t gbx, gby;   // global-relabel block, in tiles (one wavefront per tile)
+    int32_t gr_cap;     // distance cap of intermediate global relabels (0: always exact)
 };
 
 // Kernel launchers (solver.hip).
@@ -106,10 +117,13 @@ void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStre
 void launch_sched(const KParams &p, int parity, hipStream_t s);
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s);
 void launch_gr(const KParams &p, int grid, int parity, hipStream_t s);
+bool gr_block_supported(int bx, int by);
+bool options_valid(const gcfg_options &o);  // sets the error message when not
 void launch_reset_stats(const KParams &p, hipStream_t s);
 void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,
                  hipStream_t s);
-void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, hipStream_t s);
+void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, int32_t *d_dist,
+                       hipStream_t s);
 
 void set_error(const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
@@ -150,7 +164,9 @@ class Engine {
     int pump(int steps);
     int live_count(int32_t *out);
     int edit(int slot, const std::vector<int32_t> &strokes, int32_t delta);
-    int read_state(int slot, int32_t *h_caps, int32_t *h_exc);
+    // Residual capacities (W*H*4), excess (W*H) and distance labels (W*H) in
+    // image layout; any output may be null.
+    int read_state(int slot, int32_t *h_caps, int32_t *h_exc, int32_t *h_dist = nullptr);
     int fetch_state(int slot, InstState *out);
     int fetch_states(InstState *out, int n);  // slots [0, n)
     int fetch_ctl(Ctl *out);
@@ -199,6 +215,8 @@ class Engine {
     std::vector<void *> allocs_;
     std::vector<hipEvent_t> events_;  // 4 per profiled step
     int events_pending_ = 0;
+    QCtl *pin_qtrace_ = nullptr;      // queue lengths per profiled step (verbose >= 2)
+    int qtrace_cap_ = 0;
 };
 
 double now_ms();

@@ -71,6 +71,8 @@ __global__ void k_arm(KParams p, const int32_t *slots, int n, int warm, int swee
     z.phase = start_phase;
     z.gr_rounds = start_phase == PH_GR ? 1 : 0;
     z.sweeps = sweeps;
+    // The wavefront build runs the first (possibly capped) relabel sweep.
+    z.cap = (start_phase == PH_GR && p.gr_cap > 0) ? p.gr_cap : DINF;
     st = z;
     if (!was_live) atomicAdd(&p.ctl->n_live, 1);  // re-arming a live instance keeps one count
 }
@@ -105,6 +107,7 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
             p.tiles[t] = ti;
             p.act_flag[t] = 0;
             p.dirty[t] = 0;
+            p.live[t] = 1;
         }
         int64_t sp = 0;
         int bad = 0;
@@ -152,25 +155,34 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
 // ---------------------------------------------------------------------------
 // Scheduler: one workgroup per instance slot. Decisions are block-uniform
 // (taken from block-wide scans of the tile flags); thread 0 writes the state.
-enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2 };
+enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2, EM_LIVE = 3 };
 
-// Append the instance's tiles selected by `kind` to a queue (ballot
-// compaction, one atomic per 256 tiles); returns the number appended.
+// Append the instance's tiles (push/relabel queue) or global-relabel blocks
+// (identified by their anchor tile, whose flags stand for the block)
+// selected by `kind` to a queue (ballot compaction, one atomic per 256
+// entries); returns the number appended.
 __device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int colour, bool to_pr, int grop,
                           int parity) {
     __shared__ int wcount[NT / 64], wbase[NT / 64], s_qbase, s_total;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int32_t *queue = to_pr ? p.queue_pr : p.queue_gr;
     int32_t *qlen = to_pr ? &p.ctl->q[parity].qlen_pr : &p.ctl->q[parity].qlen_gr;
+    const int bx = to_pr ? 1 : p.gbx, by = to_pr ? 1 : p.gby;
+    const int nbx = (D.tiles_x + bx - 1) / bx;
+    const int nunits = bx == 1 && by == 1 ? D.ntiles : nbx * ((D.tiles_y + by - 1) / by);
     __syncthreads();  // earlier readers of s_total are done
     if (threadIdx.x == 0) s_total = 0;
-    for (int base = 0; base < D.ntiles; base += NT) {
-        const int lt = base + threadIdx.x;
+    for (int base = 0; base < nunits; base += NT) {
+        const int lu = base + threadIdx.x;
+        int lt = lu;
+        if (!(bx == 1 && by == 1) && lu < nunits) lt = (lu / nbx) * by * D.tiles_x + (lu % nbx) * bx;
         const int t = D.tile0 + lt;
         int pred = 0;
-        if (lt < D.ntiles) {
+        if (lu < nunits) {
             if (kind == EM_ALL) {
                 pred = 1;
+            } else if (kind == EM_LIVE) {
+                pred = p.live[t];
             } else if (kind == EM_DIRTY) {
                 pred = p.dirty[t];
                 if (pred) p.dirty[t] = 0;
@@ -205,13 +217,34 @@ __device__ bool any_active_tile(const KParams &p, const InstDesc &D) {
     return __syncthreads_or(any) != 0;
 }
 
-__device__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st, int parity) {
+// A new BFS round. After a build or an edit every tile derives its mask from
+// capacities and excess (OP_GR_INIT). Later rounds of the wavefront kernels
+// start from the masks the push passes keep current (OP_GR_REINIT) and skip
+// tiles whose nodes all had infinite distance in the previous round: a node
+// that cannot reach a node with excess < 0 never can again (a push only
+// creates a residual arc towards a node that can reach the sink, and sink
+// nodes only lose deficit), so such a tile's distances stay infinite, no flow
+// enters or leaves it, and its stored distances are already the answer.
+//
+// With a cap K (wavefront kernels, gr_cap > 0) a relabel computes min(d, K)
+// instead of d, starting every non-sink node at K, so BFS fronts stop after K
+// levels. min(d, K) is still a valid labelling; nodes at K are frozen (not
+// active) until the next relabel. Only an exact relabel (cap = DINF) can end
+// the solve, so a capped round that leaves no active node is followed by an
+// exact one.
+__device__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st, int parity, bool reinit,
+                                     int cap) {
     if (threadIdx.x == 0) {
         st.gr_rounds++;
         st.phase = PH_GR;
+        st.cap = cap;
     }
-    const int n = emit_tiles(p, D, EM_ALL, 0, false, OP_GR_INIT, parity);
-    if (threadIdx.x == 0) st.gr_tiles += n;
+    const int n = reinit ? emit_tiles(p, D, EM_LIVE, 0, false, OP_GR_REINIT, parity)
+                         : emit_tiles(p, D, EM_ALL, 0, false, OP_GR_INIT, parity);
+    if (threadIdx.x == 0) {
+        st.gr_tiles += n;
+        st.gr_init_tiles += n;
+    }
 }
 
 __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
@@ -220,7 +253,7 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
     InstState &st = p.state[s];
     const int phase = st.phase;
     if (phase == PH_IDLE || phase == PH_DONE) return;
-    const int err = st.err, sweeps = st.sweeps;
+    const int err = st.err, sweeps = st.sweeps, cur_cap = st.cap;
     int pr_iter = st.pr_iter;
     __syncthreads();  // every thread has read the state before thread 0 writes it
     if (threadIdx.x == 0) st.steps++;
@@ -233,10 +266,12 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
         return;
     }
     const InstDesc D = p.desc[s];
+    const int cap = (p.variant && p.gr_cap > 0) ? p.gr_cap : DINF;
     if (phase == PH_START) {
-        start_global_relabel(p, D, st, parity);
+        start_global_relabel(p, D, st, parity, false, cap);
         return;
     }
+    const bool reinit = p.variant != 0;
     if (phase == PH_GR) {
         const int n = emit_tiles(p, D, EM_DIRTY, 0, false, OP_GR_RELAX, parity);
         if (n > 0) {
@@ -245,6 +280,10 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
         }
         // The BFS has converged: finish, or push from the active tiles.
         if (!any_active_tile(p, D)) {
+            if (cur_cap < DINF) {  // nothing active below the cap: relabel exactly before finishing
+                start_global_relabel(p, D, st, parity, reinit, DINF);
+                return;
+            }
             if (threadIdx.x == 0) st.phase = PH_FIN;
             emit_tiles(p, D, EM_ALL, 0, false, OP_FIN, parity);
             return;
@@ -254,14 +293,14 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
     }
     // PH_PR: alternate checkerboard colours, then relabel globally again.
     if (pr_iter >= 2 * sweeps) {
-        start_global_relabel(p, D, st, parity);
+        start_global_relabel(p, D, st, parity, reinit, cap);
         return;
     }
     const int colour = pr_iter & 1;
     int n = emit_tiles(p, D, EM_COLOUR, colour, true, 0, parity);
     if (n == 0) n = emit_tiles(p, D, EM_COLOUR, colour ^ 1, true, 0, parity);
     if (n == 0) {
-        start_global_relabel(p, D, st, parity);
+        start_global_relabel(p, D, st, parity, reinit, cap);
         return;
     }
     if (threadIdx.x == 0) {
@@ -530,7 +569,8 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p, int parity) {
             if (init) {
                 const int4 cc = p.caps[g];
                 const int ee = p.exc[g];
-                mk = (cc.x > 0) | ((cc.y > 0) << 1) | ((cc.z > 0) << 2) | ((cc.w > 0) << 3) | ((ee > 0) << 4);
+                mk = (cc.x > 0) | ((cc.y > 0) << 1) | ((cc.z > 0) << 2) | ((cc.w > 0) << 3) | ((ee > 0) << 4) |
+                     ((ee < 0) << 5);
                 p.mask[g] = (uint8_t)mk;
                 d0 = ee < 0 ? 0 : DINF;
             } else {
@@ -655,7 +695,9 @@ __device__ __forceinline__ int64_t wave_sum64_w(int64_t v) { return wave_sum64(v
 // 16..31 the right column, indexed by row). The loads are issued
 // unconditionally from clamped addresses (a missing neighbour reads the own
 // tile) and masked afterwards, so they overlap with the tile's own loads.
-template <bool FROM_EXCESS>
+enum HaloSrc : int { HALO_DIST = 0, HALO_EXCESS = 1, HALO_MASK = 2 };
+
+template <int SRC>
 __device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, int t, int lane, int &hU, int &hD,
                                           int &hs) {
     const int nu = ti.nbr[2], nd = ti.nbr[3];
@@ -664,10 +706,14 @@ __device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, 
     const size_t gd = (size_t)(nd >= 0 ? nd : t) * TN + lane;
     const size_t gs = (size_t)(side >= 0 ? side : t) * TN + (size_t)(lane & (TH - 1)) * TW + (lane < TH ? TW - 1 : 0);
     int vu, vd, vs;
-    if (FROM_EXCESS) {
+    if (SRC == HALO_EXCESS) {
         vu = p.exc[gu] < 0 ? 0 : DINF;
         vd = p.exc[gd] < 0 ? 0 : DINF;
         vs = p.exc[gs] < 0 ? 0 : DINF;
+    } else if (SRC == HALO_MASK) {
+        vu = (p.mask[gu] & MB_ENEG) ? 0 : DINF;
+        vd = (p.mask[gd] & MB_ENEG) ? 0 : DINF;
+        vs = (p.mask[gs] & MB_ENEG) ? 0 : DINF;
     } else {
         vu = p.dist[gu];
         vd = p.dist[gd];
@@ -679,6 +725,88 @@ __device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, 
 }
 
 __device__ __forceinline__ int dmin_sat(int a, int b) { return min(min(a, b), DINF); }
+
+__device__ __forceinline__ int node_mask(const int4 &c, int e) {
+    return (c.x > 0) | ((c.y > 0) << 1) | ((c.z > 0) << 2) | ((c.w > 0) << 3) | ((e > 0) ? MB_EPOS : 0) |
+           ((e < 0) ? MB_ENEG : 0);
+}
+
+// Per-tile arrays through a wave-uniform base pointer: row y of lane x is
+// element y * TW + x, so loads and stores take a scalar base, one 32-bit
+// lane offset and immediate row offsets instead of a 64-bit address per row.
+struct TileRef {
+    int4 *c;
+    int32_t *e, *d;
+    uint8_t *m;
+};
+__device__ __forceinline__ TileRef tile_ref(const KParams &p, int t) {
+    const size_t o = (size_t)t * TN;
+    return TileRef{p.caps + o, p.exc + o, p.dist + o, p.mask + o};
+}
+
+// Flow f > 0 was pushed into node g of a tile that is not running: its excess
+// became e_new and the residual arc back to the pusher (bit arc) is open.
+// Within one colour pass such a node only gains excess, so setting "excess
+// > 0" and clearing "excess < 0" are monotone and commute between pushers.
+__device__ __forceinline__ void mask_after_push_in(const KParams &p, size_t g, int arc, int e_new) {
+    uint32_t *w = reinterpret_cast<uint32_t *>(p.mask) + (g >> 2);
+    const int sh = (int)(g & 3) * 8;
+    atomicOr(w, (uint32_t)(arc | (e_new > 0 ? MB_EPOS : 0)) << sh);
+    if (e_new >= 0) atomicAnd(w, ~((uint32_t)MB_ENEG << sh));
+}
+
+// (min, +) composition scans along a 64-lane row. Lane x holds the map
+// f_x(v) = min(A, v + B); the scans leave in lane x the composition of the
+// maps of every lane between x and the row end the values come from (lane 0
+// for scan_from_left, lane 63 for scan_from_right), farthest applied first.
+// Composing g after f: (A, B) = (min(A_g, A_f + B_g), min(B_f + B_g, INF)).
+// Identity: (INF, 0).
+__device__ __forceinline__ void compose_after(int &A, int &B, int pA, int pB) {
+    A = min(A, pA + B);
+    B = min(pB + B, DINF);
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void dpp_compose(int &A, int &B) {
+    // Lanes without a source in this DPP pattern keep the identity (INF, 0).
+    const int pA = __builtin_amdgcn_update_dpp(DINF, A, CTRL, ROWS, 0xF, false);
+    const int pB = __builtin_amdgcn_update_dpp(0, B, CTRL, ROWS, 0xF, false);
+    compose_after(A, B, pA, pB);
+}
+
+// From lane 0 upwards: Hillis-Steele within each 16-lane row (row_shr 1, 2,
+// 4, 8), then row_bcast:15 (row 0 -> row 1, row 2 -> row 3) and
+// row_bcast:31 (rows 0..1 -> rows 2 and 3).
+__device__ __forceinline__ void scan_from_left(int &A, int &B) {
+    dpp_compose<0x111, 0xF>(A, B);
+    dpp_compose<0x112, 0xF>(A, B);
+    dpp_compose<0x114, 0xF>(A, B);
+    dpp_compose<0x118, 0xF>(A, B);
+    dpp_compose<0x142, 0xA>(A, B);
+    dpp_compose<0x143, 0xC>(A, B);
+}
+
+// From lane 63 downwards: within each row by row_shl 1, 2, 4, 8; across rows
+// from the row totals at lanes 16, 32 and 48 (scalar reads).
+__device__ __forceinline__ void scan_from_right(int &A, int &B, int lane) {
+    dpp_compose<0x101, 0xF>(A, B);
+    dpp_compose<0x102, 0xF>(A, B);
+    dpp_compose<0x104, 0xF>(A, B);
+    dpp_compose<0x108, 0xF>(A, B);
+    const int a1 = __builtin_amdgcn_readlane(A, 16), b1 = __builtin_amdgcn_readlane(B, 16);
+    const int a2 = __builtin_amdgcn_readlane(A, 32), b2 = __builtin_amdgcn_readlane(B, 32);
+    const int a3 = __builtin_amdgcn_readlane(A, 48), b3 = __builtin_amdgcn_readlane(B, 48);
+    // Suffixes over whole rows: S2 = row 3, S1 = row 3 then row 2, S0 = rows 3, 2, 1.
+    int sa2 = a3, sb2 = b3;
+    int sa1 = a2, sb1 = b2;
+    compose_after(sa1, sb1, sa2, sb2);
+    int sa0 = a1, sb0 = b1;
+    compose_after(sa0, sb0, sa1, sb1);
+    const int row = lane >> 4;
+    const int pA = row == 0 ? sa0 : (row == 1 ? sa1 : (row == 2 ? sa2 : DINF));
+    const int pB = row == 0 ? sb0 : (row == 1 ? sb1 : (row == 2 ? sb2 : 0));
+    compose_after(A, B, pA, pB);
+}
 
 // Relax d(u) = min(d(u), 1 + d(v)) over residual arcs u->v (mask bits: 1 left,
 // 2 right, 4 up, 8 down) until no value changes. Per round: a downward and an
@@ -726,25 +854,11 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], const int (&m)[TH]
             if (!__any(can)) continue;
             // left to right: arcs x -> x-1, halo = left column
             int A = d[y], B = (m[y] & 1) ? 1 : DINF;
-#pragma unroll
-            for (int k = 1; k < 64; k <<= 1) {
-                const int pA = __shfl_up(A, k, 64), pB = __shfl_up(B, k, 64);
-                if (lane >= k) {
-                    A = min(A, pA + B);
-                    B = dmin_sat(pB + B, DINF);
-                }
-            }
+            scan_from_left(A, B);
             int nd = min(A, dmin_sat(hsl + B, DINF));
             // right to left: arcs x -> x+1, halo = right column
             int A2 = nd, B2 = (m[y] & 2) ? 1 : DINF;
-#pragma unroll
-            for (int k = 1; k < 64; k <<= 1) {
-                const int pA = __shfl_down(A2, k, 64), pB = __shfl_down(B2, k, 64);
-                if (lane + k < 64) {
-                    A2 = min(A2, pA + B2);
-                    B2 = dmin_sat(pB + B2, DINF);
-                }
-            }
+            scan_from_right(A2, B2, lane);
             nd = min(A2, dmin_sat(hsr + B2, DINF));
             if (nd < d[y]) {
                 d[y] = nd;
@@ -757,30 +871,54 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], const int (&m)[TH]
     return false;
 }
 
-__global__ __launch_bounds__(64) void k_grw(KParams p, int parity) {
+// Anchor (top-left tile) of the global-relabel block holding tile (x, y).
+__device__ __forceinline__ int block_anchor(const KParams &p, const InstDesc &D, int x, int y) {
+    return D.tile0 + (y - y % p.gby) * D.tiles_x + (x - x % p.gbx);
+}
+
+constexpr int GR_MAX_EXCHANGES = 64;
+
+// Global relabel over a block of BX x BY tiles per workgroup, one wavefront
+// per tile (lane x owns column x). Each wavefront relaxes its tile in
+// registers; the block then swaps its inner boundary rows and columns
+// through LDS and repeats until no inner halo value drops, so a BFS front
+// crosses the whole block in one launch. Outer halos come from the
+// neighbouring blocks' stored distances (upper bounds: a stale read only
+// delays convergence); a block whose outer boundary changed marks its
+// neighbours for the next step. FIN writes the labels of the block's tiles.
+template <int BX, int BY>
+__global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity) {
+    constexpr int NW = BX * BY;
+    __shared__ int s_top[NW][TW], s_bot[NW][TW], s_side[NW][2 * TH];
     const int qlen = p.ctl->q[parity].qlen_gr;
     if ((int)blockIdx.x >= qlen) return;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wi = w % BX, wj = w / BX;
     int64_t n_rounds = 0;
     for (int q = blockIdx.x; q < qlen; q += gridDim.x) {
         const int item = p.queue_gr[q];
-        const int op = item & 7, t = item >> 3;
+        const int op = item & 7, t0 = item >> 3;
+        const int inst = p.tiles[t0].inst;
+        const InstDesc &D = p.desc[inst];
+        const int ax = p.tiles[t0].tx, ay = p.tiles[t0].ty;
+        const int tx = ax + wi, ty = ay + wj;
+        const bool have = tx < D.tiles_x && ty < D.tiles_y;
+        const int t = have ? t0 + wj * D.tiles_x + wi : t0;
         const TileInfo ti = p.tiles[t];
-        const size_t base = (size_t)t * TN + lane;
+        const TileRef T = tile_ref(p, t);
         if (op == OP_FIN) {
-            const InstDesc &D = p.desc[ti.inst];
+            if (!have) continue;
             int64_t sx = 0;
-            const int X = ti.tx * TW + lane;
+            const int X = tx * TW + lane;
             int dd[TH], ex[TH];
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
-                const size_t g = base + (size_t)y * TW;
-                dd[y] = p.dist[g];
-                ex[y] = p.exc[g];
+                dd[y] = T.d[y * TW + lane];
+                ex[y] = T.e[y * TW + lane];
             }
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
-                const int Y = ti.ty * TH + y;
+                const int Y = ty * TH + y;
                 if (X < D.W && Y < D.H) {
                     const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
                     D.label_out[(size_t)Y * D.W + X] = (in && dd[y] >= DINF) ? 1 : 0;
@@ -788,64 +926,122 @@ __global__ __launch_bounds__(64) void k_grw(KParams p, int parity) {
                 }
             }
             sx = wave_sum64_w(sx);
-            if (lane == 0 && sx) atomic_add64(&p.state[ti.inst].sum_exc, sx);
+            if (lane == 0 && sx) atomic_add64(&p.state[inst].sum_exc, sx);
             continue;
         }
-        const bool init = op == OP_GR_INIT;
+        const bool init = op == OP_GR_INIT, reinit = op == OP_GR_REINIT;
+        const int capv = p.state[inst].cap;  // start value of non-sink nodes (DINF: exact)
         int d[TH], m[TH];
-        int hU, hD, hs;
-        if (init) {
-            // All loads first, then the mask stores (a store between loads would
-            // serialise every row on a possible alias).
-            load_halo<true>(p, ti, t, lane, hU, hD, hs);
-            int4 c[TH];
-            int ee[TH];
+        int hU = DINF, hD = DINF, hs = DINF;
+        if (!have) {
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
-                const size_t g = base + (size_t)y * TW;
-                c[y] = p.caps[g];
-                ee[y] = p.exc[g];
+                d[y] = DINF;
+                m[y] = 0;
             }
+        } else if (init) {
+            // Two halves of 8 rows: all loads of a half first, then its mask
+            // stores (a store between loads would serialise every row on a
+            // possible alias).
+            load_halo<HALO_EXCESS>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
-            for (int y = 0; y < TH; ++y) {
-                m[y] = (c[y].x > 0) | ((c[y].y > 0) << 1) | ((c[y].z > 0) << 2) | ((c[y].w > 0) << 3) |
-                       ((ee[y] > 0) << 4);
-                d[y] = ee[y] < 0 ? 0 : DINF;
+            for (int h = 0; h < TH; h += TH / 2) {
+                int4 c[TH / 2];
+                int ee[TH / 2];
+#pragma unroll
+                for (int k = 0; k < TH / 2; ++k) {
+                    c[k] = T.c[(h + k) * TW + lane];
+                    ee[k] = T.e[(h + k) * TW + lane];
+                }
+#pragma unroll
+                for (int k = 0; k < TH / 2; ++k) {
+                    m[h + k] = node_mask(c[k], ee[k]);
+                    d[h + k] = ee[k] < 0 ? 0 : capv;
+                }
+#pragma unroll
+                for (int k = 0; k < TH / 2; ++k) T.m[(h + k) * TW + lane] = (uint8_t)m[h + k];
             }
+        } else if (reinit) {
+            load_halo<HALO_MASK>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
-            for (int y = 0; y < TH; ++y) p.mask[base + (size_t)y * TW] = (uint8_t)m[y];
+            for (int y = 0; y < TH; ++y) m[y] = T.m[y * TW + lane];
+#pragma unroll
+            for (int y = 0; y < TH; ++y) d[y] = (m[y] & MB_ENEG) ? 0 : capv;
         } else {
-            load_halo<false>(p, ti, t, lane, hU, hD, hs);
+            load_halo<HALO_DIST>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
-                const size_t g = base + (size_t)y * TW;
-                m[y] = p.mask[g];
-                d[y] = p.dist[g];
+                m[y] = T.m[y * TW + lane];
+                d[y] = T.d[y * TW + lane];
             }
         }
+        // Inner neighbours inside the block (the tile exists and is in the block).
+        const bool inU = wj > 0, inD = wj < BY - 1 && ty + 1 < D.tiles_y;
+        const bool inL = wi > 0, inR = wi < BX - 1 && tx + 1 < D.tiles_x;
         uint32_t chg = 0;
-        int rounds = 0;
-        const bool conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, p.gr_rounds);
-        int act = 0;
+        bool capped = false;
+        bool need = have;  // this tile has a new halo value or did not converge yet
+        for (int xr = 0;; ++xr) {
+            int rounds = 0;
+            bool conv = true;
+            if (need) conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, p.gr_rounds);
+            n_rounds += rounds;
+            if (NW == 1) {
+                capped = !conv;
+                break;
+            }
+            s_top[w][lane] = d[0];
+            s_bot[w][lane] = d[TH - 1];
+            if (lane == 0 || lane == TW - 1) {
+                const int o = lane ? TH : 0;  // column 0 at [0, 16), column 63 at [16, 32)
 #pragma unroll
-        for (int y = 0; y < TH; ++y) {
-            if (init || ((chg >> y) & 1)) p.dist[base + (size_t)y * TW] = d[y];
-            act |= (m[y] & 16) && d[y] < DINF;
+                for (int y = 0; y < TH; ++y) s_side[w][o + y] = d[y];
+            }
+            __syncthreads();
+            int nU = hU, nD = hD, nS = hs;
+            if (inU) nU = s_bot[w - BX][lane];
+            if (inD) nD = s_top[w + BX][lane];
+            if (lane < TH && inL) nS = s_side[w - 1][TH + lane];
+            if (lane >= TH && lane < 2 * TH && inR) nS = s_side[w + 1][lane - TH];
+            const bool hch = have && __any(nU < hU || nD < hD || nS < hs);
+            hU = nU;
+            hD = nD;
+            hs = nS;
+            need = hch || !conv;
+            // Also the barrier before the next round's LDS writes.
+            if (!__syncthreads_or(need)) break;
+            if (xr + 1 >= GR_MAX_EXCHANGES) {
+                capped = true;
+                break;
+            }
         }
-        const bool chL = __any(lane == 0 && chg != 0);
-        const bool chR = __any(lane == TW - 1 && chg != 0);
-        const bool chU = __any((chg & 1u) != 0);
-        const bool chD = __any((chg >> (TH - 1)) & 1u);
+        int act = 0, fin = 0;
+        if (have) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                if (init || reinit || ((chg >> y) & 1)) T.d[y * TW + lane] = d[y];
+                act |= (m[y] & MB_EPOS) && d[y] < capv;
+                fin |= d[y] < DINF;
+            }
+        }
+        // Outer boundary changes re-queue the neighbouring blocks.
+        const bool chL = __any(lane == 0 && chg != 0) && !inL && have && ti.nbr[0] >= 0;
+        const bool chR = __any(lane == TW - 1 && chg != 0) && !inR && have && ti.nbr[1] >= 0;
+        const bool chU = __any((chg & 1u) != 0) && !inU && have && ti.nbr[2] >= 0;
+        const bool chD = __any((chg >> (TH - 1)) & 1u) && !inD && have && ti.nbr[3] >= 0;
         const bool any_act = __any(act);
-        if (lane == 0) {
-            if (chL && ti.nbr[0] >= 0) mark_dirty(p, ti.nbr[0]);
-            if (chR && ti.nbr[1] >= 0) mark_dirty(p, ti.nbr[1]);
-            if (chU && ti.nbr[2] >= 0) mark_dirty(p, ti.nbr[2]);
-            if (chD && ti.nbr[3] >= 0) mark_dirty(p, ti.nbr[3]);
-            if (!conv) mark_dirty(p, t);
+        if (lane == 0 && have) {
+            if (chL) mark_dirty(p, block_anchor(p, D, tx - 1, ty));
+            if (chR) mark_dirty(p, block_anchor(p, D, tx + 1, ty));
+            if (chU) mark_dirty(p, block_anchor(p, D, tx, ty - 1));
+            if (chD) mark_dirty(p, block_anchor(p, D, tx, ty + 1));
             set_active(p, t, any_act);
         }
-        n_rounds += rounds;
+        const bool any_fin = NW == 1 ? __any(fin) : __syncthreads_or(fin);
+        if (threadIdx.x == 0) {
+            if (capped) mark_dirty(p, t0);
+            p.live[t0] = any_fin;
+        }
     }
     if (p.profile && lane == 0) atomic_add64(&p.ctl->gr_iters, n_rounds);
 }
@@ -859,18 +1055,20 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
         const int t = p.queue_pr[q];
         const TileInfo ti = p.tiles[t];
         const int dthr = p.desc[ti.inst].n_interior;
-        const size_t base = (size_t)t * TN + lane;
+        const int capv = p.state[ti.inst].cap;  // nodes at the cap are frozen until the next relabel
+        const TileRef T = tile_ref(p, t);
         int hU, hD, hs;
-        load_halo<false>(p, ti, t, lane, hU, hD, hs);
+        load_halo<HALO_DIST>(p, ti, t, lane, hU, hD, hs);
         int4 c[TH];
-        int e[TH], d[TH], acc[TH];
+        int e[TH], d[TH];
+        // Flow pushed out of the tile sideways, one row per lane: lanes 0..15
+        // into the left tile, lanes 16..31 into the right tile.
+        int accS = 0;
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
-            const size_t g = base + (size_t)y * TW;
-            c[y] = p.caps[g];
-            e[y] = p.exc[g];
-            d[y] = p.dist[g];
-            acc[y] = 0;
+            c[y] = T.c[y * TW + lane];
+            e[y] = T.e[y * TW + lane];
+            d[y] = T.d[y * TW + lane];
         }
         int accU = 0, accD = 0;
         int it = 0;
@@ -889,7 +1087,7 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
                 int fl = 0, fr = 0, fu = 0, fd = 0;
                 int ek = e[y];
                 int4 ck = c[y];
-                if (ek > 0 && d[y] < DINF) {
+                if (ek > 0 && d[y] < capv) {
                     const int tgt = d[y] - 1;
                     if (ck.x > 0 && hl == tgt) {
                         fl = min(ek, ck.x);
@@ -917,8 +1115,8 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
                 ck.y += inR;
                 c[y] = ck;
                 e[y] = ek + inL + inR;
-                if (lane == 0) acc[y] += fl;
-                if (lane == TW - 1) acc[y] += fr;
+                const int sfl = __builtin_amdgcn_readlane(fl, 0), sfr = __builtin_amdgcn_readlane(fr, TW - 1);
+                accS += lane == y ? sfl : (lane == TH + y ? sfr : 0);
                 if (y > 0) {
                     e[y - 1] += fu;
                     c[y - 1].w += fu;
@@ -941,16 +1139,20 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
                 if (lane == TW - 1) hr = __builtin_amdgcn_readlane(hs, TH + y);
                 const int hu = y ? d[y - 1] : hU;
                 const int hd = y < TH - 1 ? d[y + 1] : hD;
-                if (e[y] > 0 && d[y] < DINF) {
+                if (e[y] > 0 && d[y] < capv) {
                     int mm = DINF;
                     if (c[y].x > 0) mm = min(mm, hl);
                     if (c[y].y > 0) mm = min(mm, hr);
                     if (c[y].z > 0) mm = min(mm, hu);
                     if (c[y].w > 0) mm = min(mm, hd);
                     int nd = mm >= DINF ? DINF : mm + 1;
+                    // A label of n or more proves the node cannot reach the
+                    // sink; a label past the cap freezes the node at the cap
+                    // (still a valid labelling: cap < nd <= d(v) + 1).
                     if (nd >= dthr) nd = DINF;
+                    else if (nd > capv) nd = capv;
                     if (nd > d[y]) d[y] = nd;
-                    act |= d[y] < DINF;
+                    act |= d[y] < capv;
                 }
             }
             if (!__any(act)) break;
@@ -958,48 +1160,45 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
         int mine = 0;
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
-            const size_t g = base + (size_t)y * TW;
-            p.caps[g] = c[y];
-            p.exc[g] = e[y];
-            p.dist[g] = d[y];
-            mine |= e[y] > 0 && d[y] < DINF;
+            T.c[y * TW + lane] = c[y];
+            T.e[y * TW + lane] = e[y];
+            T.d[y * TW + lane] = d[y];
+            T.m[y * TW + lane] = (uint8_t)node_mask(c[y], e[y]);
+            mine |= e[y] > 0 && d[y] < capv;
         }
         // Flow pushed into halo nodes (their tiles do not run in this colour
         // pass; int32 atomics because two same-colour tiles can feed one
-        // corner node).
-        int toL = 0, toR = 0;
-        if (lane == 0 && ti.nbr[0] >= 0) {
-#pragma unroll
-            for (int y = 0; y < TH; ++y)
-                if (acc[y]) {
-                    const size_t gn = (size_t)ti.nbr[0] * TN + (size_t)y * TW + TW - 1;
-                    atomicAdd(&p.exc[gn], acc[y]);
-                    atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 1, acc[y]);
-                    toL = 1;
-                }
+        // corner node). Lanes 0..31 feed the side tiles (one row each), every
+        // lane may feed the tiles above and below. All excess atomics are
+        // issued before their results are used for the mask updates, so
+        // their latencies overlap.
+        const int side = lane < TH ? ti.nbr[0] : ti.nbr[1];
+        const bool pushS = lane < 2 * TH && accS != 0 && side >= 0;
+        const size_t gS = (size_t)(pushS ? side : 0) * TN + (size_t)(lane & (TH - 1)) * TW + (lane < TH ? TW - 1 : 0);
+        const int capS = lane < TH ? 1 : 0;  // the side node's arc back to us: right (left tile) / left
+        const int arcS = lane < TH ? 2 : 1;
+        const bool pushU = accU != 0 && ti.nbr[2] >= 0, pushD = accD != 0 && ti.nbr[3] >= 0;
+        const size_t gU = (size_t)(pushU ? ti.nbr[2] : 0) * TN + (size_t)(TH - 1) * TW + lane;
+        const size_t gD = (size_t)(pushD ? ti.nbr[3] : 0) * TN + lane;
+        int oldS = 0, oldU = 0, oldD = 0;
+        if (pushS) {
+            oldS = atomicAdd(&p.exc[gS], accS);
+            atomicAdd(reinterpret_cast<int *>(&p.caps[gS]) + capS, accS);
         }
-        if (lane == TW - 1 && ti.nbr[1] >= 0) {
-#pragma unroll
-            for (int y = 0; y < TH; ++y)
-                if (acc[y]) {
-                    const size_t gn = (size_t)ti.nbr[1] * TN + (size_t)y * TW;
-                    atomicAdd(&p.exc[gn], acc[y]);
-                    atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 0, acc[y]);
-                    toR = 1;
-                }
+        if (pushU) {
+            oldU = atomicAdd(&p.exc[gU], accU);
+            atomicAdd(reinterpret_cast<int *>(&p.caps[gU]) + 3, accU);
         }
-        if (accU && ti.nbr[2] >= 0) {
-            const size_t gn = (size_t)ti.nbr[2] * TN + (size_t)(TH - 1) * TW + lane;
-            atomicAdd(&p.exc[gn], accU);
-            atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 3, accU);
+        if (pushD) {
+            oldD = atomicAdd(&p.exc[gD], accD);
+            atomicAdd(reinterpret_cast<int *>(&p.caps[gD]) + 2, accD);
         }
-        if (accD && ti.nbr[3] >= 0) {
-            const size_t gn = (size_t)ti.nbr[3] * TN + lane;
-            atomicAdd(&p.exc[gn], accD);
-            atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + 2, accD);
-        }
+        if (pushS) mask_after_push_in(p, gS, arcS, oldS + accS);
+        if (pushU) mask_after_push_in(p, gU, 8, oldU + accU);
+        if (pushD) mask_after_push_in(p, gD, 4, oldD + accD);
+        const int toL = pushS && lane < TH, toR = pushS && lane >= TH;
         const bool anyL = __any(toL), anyR = __any(toR);
-        const bool anyU = __any(accU != 0 && ti.nbr[2] >= 0), anyD = __any(accD != 0 && ti.nbr[3] >= 0);
+        const bool anyU = __any(pushU), anyD = __any(pushD);
         const bool any_mine = __any(mine);
         if (lane == 0) {
             if (anyL) set_active(p, ti.nbr[0], 1);
@@ -1045,6 +1244,7 @@ __global__ void k_clear_tiles(KParams p, const int32_t *slots, const int32_t *pr
         p.tiles[t] = ti;
         p.act_flag[t] = 0;
         p.dirty[t] = 0;
+        p.live[t] = 1;
     }
 }
 
@@ -1068,8 +1268,9 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         const int t = D.tile0 + lt;
         const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
         const int X = tx * TW + lane, Y0 = ty * TH;
-        const size_t base = (size_t)t * TN + lane;
+        const TileRef T = tile_ref(p, t);
         int d[TH], m[TH];
+        const int capv = p.gr_cap > 0 ? p.gr_cap : DINF;  // the first relabel sweep, capped like k_arm says
         int64_t sp = 0;
         int bad = 0;
         // Halo distances from the inputs (issued first, consumed after the rows).
@@ -1113,12 +1314,11 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
                     e = vp[k] - vn[k];
                     sp += vp[k];
                 }
-                const size_t g = base + (size_t)y * TW;
-                p.caps[g] = c;
-                p.exc[g] = e;
-                m[y] = (c.x > 0) | ((c.y > 0) << 1) | ((c.z > 0) << 2) | ((c.w > 0) << 3) | ((e > 0) << 4);
-                p.mask[g] = (uint8_t)m[y];
-                d[y] = e < 0 ? 0 : DINF;
+                T.c[y * TW + lane] = c;
+                T.e[y * TW + lane] = e;
+                m[y] = node_mask(c, e);
+                T.m[y * TW + lane] = (uint8_t)m[y];
+                d[y] = e < 0 ? 0 : capv;
             }
         }
         uint32_t chg = 0;
@@ -1127,8 +1327,8 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         int act = 0;
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
-            p.dist[base + (size_t)y * TW] = d[y];
-            act |= (m[y] & 16) && d[y] < DINF;
+            T.d[y * TW + lane] = d[y];
+            act |= (m[y] & MB_EPOS) && d[y] < capv;
         }
         const bool chL = __any(lane == 0 && chg != 0);
         const bool chR = __any(lane == TW - 1 && chg != 0);
@@ -1138,11 +1338,12 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         const bool any_bad = __any(bad);
         sp = wave_sum64(sp);
         if (lane == 0) {
-            if (chL && tx > 0) mark_dirty(p, t - 1);
-            if (chR && tx < D.tiles_x - 1) mark_dirty(p, t + 1);
-            if (chU && ty > 0) mark_dirty(p, t - D.tiles_x);
-            if (chD && ty < D.tiles_y - 1) mark_dirty(p, t + D.tiles_x);
-            if (!conv) mark_dirty(p, t);
+            // Dirty marks go to the global-relabel blocks of the affected tiles.
+            if (chL && tx > 0) mark_dirty(p, block_anchor(p, D, tx - 1, ty));
+            if (chR && tx < D.tiles_x - 1) mark_dirty(p, block_anchor(p, D, tx + 1, ty));
+            if (chU && ty > 0) mark_dirty(p, block_anchor(p, D, tx, ty - 1));
+            if (chD && ty < D.tiles_y - 1) mark_dirty(p, block_anchor(p, D, tx, ty + 1));
+            if (!conv) mark_dirty(p, block_anchor(p, D, tx, ty));
             set_active(p, t, any_act);
             if (sp) atomic_add64(&p.state[s].sum_pos, sp);
             if (any_bad) atomicOr(&p.state[s].err, 1);
@@ -1180,7 +1381,7 @@ __global__ __launch_bounds__(NT) void k_edit(KParams p, int slot, const int32_t 
 }
 
 // Residual state in image layout (certificates and debugging).
-__global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *exc_out) {
+__global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *exc_out, int32_t *dist_out) {
     const InstDesc D = p.desc[slot];
     const int64_t n = (int64_t)D.W * D.H;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1188,11 +1389,14 @@ __global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *ex
         const int t = D.tile0 + (Y / TH) * D.tiles_x + X / TW;
         const size_t g = (size_t)t * TN + (Y % TH) * TW + (X % TW);
         const int4 c = p.caps[g];
-        caps_out[4 * i + 0] = c.x;
-        caps_out[4 * i + 1] = c.y;
-        caps_out[4 * i + 2] = c.z;
-        caps_out[4 * i + 3] = c.w;
-        exc_out[i] = p.exc[g];
+        if (caps_out) {
+            caps_out[4 * i + 0] = c.x;
+            caps_out[4 * i + 1] = c.y;
+            caps_out[4 * i + 2] = c.z;
+            caps_out[4 * i + 3] = c.w;
+        }
+        if (exc_out) exc_out[i] = p.exc[g];
+        if (dist_out) dist_out[i] = p.dist[g];
     }
 }
 
@@ -1226,11 +1430,35 @@ void launch_pr(const KParams &p, int grid, int parity, hipStream_t s) {
     else
         hipLaunchKernelGGL(k_pr, dim3(grid), dim3(NT), 0, s, p, parity);
 }
+template <int BX, int BY>
+static void launch_grb(const KParams &p, int grid, int parity, hipStream_t s) {
+    hipLaunchKernelGGL((k_grb<BX, BY>), dim3(grid), dim3(64 * BX * BY), 0, s, p, parity);
+}
+
+bool gr_block_supported(int bx, int by) {
+    switch (bx * 100 + by) {
+        case 101: case 102: case 104: case 108: case 116: case 202: case 204: case 208: case 404: return true;
+        default: return false;
+    }
+}
+
 void launch_gr(const KParams &p, int grid, int parity, hipStream_t s) {
-    if (p.variant)
-        hipLaunchKernelGGL(k_grw, dim3(grid), dim3(64), 0, s, p, parity);
-    else
+    if (!p.variant) {
         hipLaunchKernelGGL(k_gr, dim3(grid), dim3(NT), 0, s, p, parity);
+        return;
+    }
+    switch (p.gbx * 100 + p.gby) {
+        case 101: launch_grb<1, 1>(p, grid, parity, s); break;
+        case 102: launch_grb<1, 2>(p, grid, parity, s); break;
+        case 104: launch_grb<1, 4>(p, grid, parity, s); break;
+        case 108: launch_grb<1, 8>(p, grid, parity, s); break;
+        case 116: launch_grb<1, 16>(p, grid, parity, s); break;
+        case 202: launch_grb<2, 2>(p, grid, parity, s); break;
+        case 204: launch_grb<2, 4>(p, grid, parity, s); break;
+        case 208: launch_grb<2, 8>(p, grid, parity, s); break;
+        case 404: launch_grb<4, 4>(p, grid, parity, s); break;
+        default: break;  // rejected when the options are set
+    }
 }
 
 void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s) {
@@ -1248,8 +1476,9 @@ void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_str
                        n_strokes, delta);
 }
 
-void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, hipStream_t s) {
-    hipLaunchKernelGGL(k_read_state, dim3(1024), dim3(256), 0, s, p, slot, d_caps, d_exc);
+void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, int32_t *d_dist,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_read_state, dim3(1024), dim3(256), 0, s, p, slot, d_caps, d_exc, d_dist);
 }
 
 }  // namespace gcfg

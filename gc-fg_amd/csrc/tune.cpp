@@ -47,9 +47,26 @@ int coordinate_search(gcfg_options start, int trials, const std::function<double
         {&gcfg_options::sweeps_per_gr, {1, 2, 4, 8}},
         {&gcfg_options::gr_rounds, {2, 4, 8, 48}},
         {&gcfg_options::steps_per_poll, {4, 8, 16, 32}},
+        {&gcfg_options::gr_cap, {0, 16, 32, 64, 128}},
     };
+    // Global-relabel block shapes (width, height in tiles) are one knob.
+    const std::vector<std::pair<int32_t, int32_t>> shapes = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 16},
+                                                             {2, 2}, {2, 4}, {2, 8}, {4, 4}};
     for (int pass = 0; pass < 2; ++pass) {
         bool improved = false;
+        for (const auto &sh : shapes) {
+            if (cur.gr_block_w == sh.first && cur.gr_block_h == sh.second) continue;
+            gcfg_options cand = cur;
+            cand.gr_block_w = sh.first;
+            cand.gr_block_h = sh.second;
+            const double ms = measure(cand);
+            if (ms < 0) return GCFG_ERR_HIP;
+            if (ms < cur_ms * 0.98) {
+                cur = cand;
+                cur_ms = ms;
+                improved = true;
+            }
+        }
         for (const Knob &k : knobs) {
             for (int32_t v : k.values) {
                 if (cur.*(k.field) == v) continue;
@@ -77,11 +94,12 @@ extern "C" {
 
 static const char *kFields[] = {"sweeps_per_gr", "inner_iters", "steps_per_poll", "use_graph",
                                 "max_steps",     "verbose",     "profile",        "kernels",
-                                "gr_rounds"};
+                                "gr_rounds",     "gr_block_w",  "gr_block_h",     "gr_cap"};
 
 static std::vector<int32_t *> option_fields(gcfg_options *o) {
-    return {&o->sweeps_per_gr, &o->inner_iters, &o->steps_per_poll, &o->use_graph, &o->max_steps,
-            &o->verbose,       &o->profile,     &o->kernels,        &o->gr_rounds};
+    return {&o->sweeps_per_gr, &o->inner_iters, &o->steps_per_poll, &o->use_graph,
+            &o->max_steps,     &o->verbose,     &o->profile,        &o->kernels,
+            &o->gr_rounds,     &o->gr_block_w,  &o->gr_block_h,     &o->gr_cap};
 }
 
 int gcfg_tune_store(const char *path, const char *key, const gcfg_options *opt) {

@@ -44,14 +44,16 @@ class Options(ctypes.Structure):
     _fields_ = [("sweeps_per_gr", ctypes.c_int32), ("inner_iters", ctypes.c_int32),
                 ("steps_per_poll", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("max_steps", ctypes.c_int32), ("verbose", ctypes.c_int32), ("profile", ctypes.c_int32),
-                ("kernels", ctypes.c_int32), ("gr_rounds", ctypes.c_int32)]
+                ("kernels", ctypes.c_int32), ("gr_rounds", ctypes.c_int32),
+                ("gr_block_w", ctypes.c_int32), ("gr_block_h", ctypes.c_int32), ("gr_cap", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("flow", ctypes.c_int64), ("pr_tiles", ctypes.c_int64), ("gr_tiles", ctypes.c_int64),
                 ("pr_iters", ctypes.c_int64), ("steps", ctypes.c_int32), ("gr_rounds", ctypes.c_int32),
                 ("ms", ctypes.c_double), ("pr_kernel_ms", ctypes.c_double), ("gr_kernel_ms", ctypes.c_double),
-                ("sched_kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("gr_iters", ctypes.c_int64)]
+                ("sched_kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("gr_iters", ctypes.c_int64),
+                ("gr_init_tiles", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -305,6 +307,11 @@ class Graph:
         _check(lib().gcfg_graph_autotune(_VP(self.h), int(trials), ctypes.byref(best), ctypes.byref(ms)),
                "gcfg_graph_autotune")
         return best, float(ms.value)
+
+    def read_distances(self):
+        dist = np.empty((self.height, self.width), np.int32)
+        _check(lib().gcfg_graph_read_distances(_VP(self.h), _VP(dist.ctypes.data)), "gcfg_graph_read_distances")
+        return dist
 
     def read_state(self):
         caps = np.empty((self.height, self.width, 4), np.int32)

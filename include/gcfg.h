@@ -45,6 +45,11 @@ typedef struct gcfg_options {
     int32_t profile;         /* 1: time every push/relabel and global-relabel launch with HIP events */
     int32_t kernels;         /* 1: wavefront-per-tile kernels (default), 0: workgroup-per-tile LDS kernels */
     int32_t gr_rounds;       /* max tile-local relaxation rounds per global-relabel sweep (then re-queued) */
+    int32_t gr_block_w;      /* global-relabel block in tiles (one workgroup, BFS exchanged through LDS): */
+    int32_t gr_block_h;      /*   1x1, 1x2, 1x4, 1x8, 1x16, 2x2, 2x4, 2x8 or 4x4; 0 = default */
+    int32_t gr_cap;          /* >0: intermediate global relabels stop BFS fronts at this distance (nodes
+                                beyond are frozen until the next relabel; the last relabel is exact);
+                                0: every relabel exact */
 } gcfg_options;
 
 typedef struct gcfg_stats {
@@ -59,6 +64,7 @@ typedef struct gcfg_stats {
     double pr_kernel_ms, gr_kernel_ms, sched_kernel_ms;
     int64_t launches;    /* steps launched (each = sched + pr + gr) */
     int64_t gr_iters;    /* tile-local relaxation rounds summed over global-relabel tiles */
+    int64_t gr_init_tiles; /* of gr_tiles, the initial sweeps of a global relabel (the rest relax) */
 } gcfg_stats;
 
 void gcfg_default_options(gcfg_options *opt);
@@ -88,6 +94,11 @@ int gcfg_graph_optimize_device(gcfg_graph *g, int32_t *d_label, gcfg_stats *stat
  * residual capacity from (x,y) towards left/right/up/down (k = 0..3),
  * h_excess[x+y*W] the excess (negative = remaining sink capacity). */
 int gcfg_graph_read_state(gcfg_graph *g, int32_t *h_caps, int32_t *h_excess);
+/* Distance labels (W*H, image layout) after a solve: the exact BFS distance
+ * of every interior node to a node with excess < 0 along residual arcs in the
+ * final residual graph; 0x3fffffff where no such path exists. No counterpart
+ * in the reference (its heights are approximate); exposed for verification. */
+int gcfg_graph_read_distances(gcfg_graph *g, int32_t *h_dist);
 /* GC_End, GraphCut.cu:542-568 (borrowed inputs are not freed). */
 int gcfg_graph_destroy(gcfg_graph *g);
 

@@ -19,7 +19,8 @@ def _ellipse_links(g, W, H, seed):
     return links
 
 
-def test_mixed_batch_matches_oracle(g, ora):
+@pytest.mark.parametrize("cfg", [{}, dict(gr_cap=16, gr_block_w=1, gr_block_h=4), dict(gr_cap=8, kernels=0)])
+def test_mixed_batch_matches_oracle(g, ora, cfg):
     rng = np.random.default_rng(12)
     items, refs = [], []
     shapes = [(70, 40), (3, 3), (1, 9), (257, 130), (64, 16), (129, 17), (50, 300), (400, 200)]
@@ -34,7 +35,10 @@ def test_mixed_batch_matches_oracle(g, ora):
             pos, neg, _ = random_instance(rng, W, H, tmax=200, edges=False)
             items.append(g.BatchItem(W, H, pos, neg, penalty=37))
             refs.append(ora.maxflow(W, H, pos, neg, penalty=37))
-    b = g.Batch(items)
+    o = g.default_options()
+    for k, v in cfg.items():
+        setattr(o, k, v)
+    b = g.Batch(items, o)
     for rep in range(2):  # the same batch object solves again from scratch
         flows, labels, st = b.solve(labels_to_host=True)
         for k, (rf, rl) in enumerate(refs):

@@ -241,3 +241,117 @@ def test_profile_mode_reports_kernel_time(g, ora):
     rflow, rlabel = ora.maxflow(W, H, pos, neg, edges=E)
     assert flow == rflow and np.array_equal(label, rlabel)
     assert st.pr_kernel_ms > 0 and st.gr_kernel_ms > 0 and st.launches >= st.steps
+
+
+GR_BLOCKS = [(1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (2, 2), (2, 4), (2, 8), (4, 4)]
+
+
+@pytest.mark.parametrize("block", GR_BLOCKS)
+def test_gr_block_shapes(g, ora, block):
+    """Every global-relabel block shape (BFS exchanged through LDS inside a
+    block) gives the exact result, on shapes with partial blocks, a corridor
+    that snakes through many tiles and heavy-tailed capacities."""
+    o = g.default_options()
+    o.gr_block_w, o.gr_block_h = block
+    rng = np.random.default_rng(block[0] * 100 + block[1])
+    for W, H in ((200, 70), (130, 33), (65, 300), (5, 4)):
+        pos, neg, E = random_instance(rng, W, H, tmax=150, emax=50)
+        _check(g, ora, W, H, pos, neg, edges=E, options=o)
+    W, H = 100, 40
+    pos, neg, E = serpentine(W, H)
+    _check(g, ora, W, H, pos, neg, edges=E, options=o)
+    W, H = 260, 180
+    pos, neg = g.gen_stress_tlinks(W, H, 5)
+    _, _, E = random_instance(rng, W, H, emax=80)
+    _check(g, ora, W, H, pos, neg, edges=E, options=o)
+
+
+def test_unsupported_gr_block_rejected(g):
+    gr = g.Graph(10, 10)
+    o = g.default_options()
+    o.gr_block_w, o.gr_block_h = 3, 3
+    with pytest.raises(g.GcfgError):
+        gr.set_options(o)
+    gr.close()
+
+
+def _bfs_to_sinks(W, H, caps, exc):
+    """Exact BFS distance of every interior node to a node with excess < 0
+    along residual arcs (numpy level-synchronous relaxation)."""
+    inter = np.zeros((H, W), bool)
+    inter[1:-1, 1:-1] = True
+    INF = 0x3FFFFFFF
+    d = np.where(inter & (exc < 0), 0, INF).astype(np.int64)
+    r = caps.astype(np.int64)
+    while True:
+        nd = d.copy()
+        nd[:, 1:] = np.minimum(nd[:, 1:], np.where(r[:, 1:, 0] > 0, d[:, :-1] + 1, INF))
+        nd[:, :-1] = np.minimum(nd[:, :-1], np.where(r[:, :-1, 1] > 0, d[:, 1:] + 1, INF))
+        nd[1:, :] = np.minimum(nd[1:, :], np.where(r[1:, :, 2] > 0, d[:-1, :] + 1, INF))
+        nd[:-1, :] = np.minimum(nd[:-1, :], np.where(r[:-1, :, 3] > 0, d[1:, :] + 1, INF))
+        nd = np.where(inter, np.minimum(nd, INF), INF)
+        if np.array_equal(nd, d):
+            return d
+        d = nd
+
+
+@pytest.mark.parametrize("cfg", [dict(kernels=0), dict(kernels=1), dict(kernels=1, gr_block_w=1, gr_block_h=4),
+                                 dict(kernels=1, gr_block_w=2, gr_block_h=8)])
+def test_final_distances_are_exact_bfs(g, ora, cfg):
+    """The solve ends with a converged global relabel: the stored distance of
+    every interior node equals its BFS distance to the sink nodes in the final
+    residual graph (checks the tile relaxation and its row scans directly)."""
+    o = g.default_options()
+    for k, v in cfg.items():
+        setattr(o, k, v)
+    rng = np.random.default_rng(17)
+    cases = [(230, 150) + random_instance(rng, 230, 150, tmax=120, emax=40)]
+    W, H = 100, 40
+    cases.append((W, H) + serpentine(W, H))
+    for W, H, pos, neg, E in cases:
+        gr = g.Graph(W, H, o)
+        gr.set_dataterms(pos, neg)
+        gr.set_edges(*E)
+        gr.build()
+        label, st = gr.optimize()
+        caps, exc = gr.read_state()
+        dist = gr.read_distances()
+        gr.close()
+        inter = np.zeros((H, W), bool)
+        inter[1:-1, 1:-1] = True
+        want = _bfs_to_sinks(W, H, caps, exc)
+        assert np.array_equal(dist[inter].astype(np.int64), want[inter]), (W, H, int((dist[inter] != want[inter]).sum()))
+        assert (want[inter] < 0x3FFFFFFF).any() and (want[inter] > 1).any()
+
+
+@pytest.mark.parametrize("cap", [1, 4, 16, 64])
+def test_capped_relabels(g, ora, cap):
+    """Capped intermediate global relabels (BFS fronts stopped at `cap`,
+    nodes beyond frozen) still end with the exact flow, labels and BFS
+    distances: long corridors, random and heavy-tailed instances."""
+    o = g.default_options()
+    o.gr_cap = cap
+    rng = np.random.default_rng(cap)
+    for W, H in ((200, 70), (65, 300), (5, 4)):
+        pos, neg, E = random_instance(rng, W, H, tmax=150, emax=50)
+        _check(g, ora, W, H, pos, neg, edges=E, options=o)
+    W, H = 100, 40
+    pos, neg, E = serpentine(W, H)
+    _check(g, ora, W, H, pos, neg, edges=E, options=o)
+    W, H = 260, 180
+    pos, neg = g.gen_stress_tlinks(W, H, 5)
+    _, _, E = random_instance(rng, W, H, emax=80)
+    _check(g, ora, W, H, pos, neg, edges=E, options=o)
+    o.gr_block_w, o.gr_block_h = 2, 4
+    gr = g.Graph(W, H, o)
+    gr.set_dataterms(pos, neg)
+    gr.set_edges(*E)
+    gr.build()
+    label, st = gr.optimize()
+    caps, exc = gr.read_state()
+    dist = gr.read_distances()
+    gr.close()
+    inter = np.zeros((H, W), bool)
+    inter[1:-1, 1:-1] = True
+    assert np.array_equal(dist[inter].astype(np.int64), _bfs_to_sinks(W, H, caps, exc)[inter])
+    assert st.flow == ora.maxflow(W, H, pos, neg, edges=E)[0]
