@@ -183,6 +183,7 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
     InstState st;
     rc = g->eng.fetch_state(g->slot, &st);
     if (rc != GCFG_OK) return rc;
+    g->eng.steps_hint = st.steps;
     gcfg::Ctl ctl;
     rc = g->eng.fetch_ctl(&ctl);
     if (rc != GCFG_OK) return rc;
@@ -352,6 +353,7 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
         agg.gr_rounds += st.gr_rounds;
     }
     agg.ms = now_ms() - t0;
+    b->eng.steps_hint = agg.steps;
     agg.pr_kernel_ms = b->eng.prof_pr_ms;
     agg.gr_kernel_ms = b->eng.prof_gr_ms;
     agg.sched_kernel_ms = b->eng.prof_sched_ms;

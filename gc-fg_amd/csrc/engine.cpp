@@ -29,8 +29,7 @@ double now_ms() {
 
 Engine::~Engine() {
     if (stream_) hipStreamSynchronize(stream_);
-    if (graph_exec_) hipGraphExecDestroy(graph_exec_);
-    if (graph_) hipGraphDestroy(graph_);
+    drop_graphs();
     for (hipEvent_t e : events_) hipEventDestroy(e);
     for (void *p : allocs_) hipFree(p);
     if (pin_desc_) hipHostFree(pin_desc_);
@@ -136,15 +135,7 @@ void Engine::apply_options() {
         kp_.gby = gby;
         grid_pr_ = gpr;
         grid_gr_ = ggr;
-        if (graph_exec_) {
-            hipGraphExecDestroy(graph_exec_);
-            graph_exec_ = nullptr;
-        }
-        if (graph_) {
-            hipGraphDestroy(graph_);
-            graph_ = nullptr;
-        }
-        graph_steps_ = 0;
+        drop_graphs();
     }
 }
 
@@ -323,24 +314,38 @@ int Engine::edit(int slot, const std::vector<int32_t> &strokes, int32_t delta) {
 
 // Steps alternate queue-counter sets by parity, so a captured chunk holds an
 // even number of steps starting at parity 0 and always ends where it began.
-static int graph_chunk(int steps_per_poll) { return (steps_per_poll + 1) & ~1; }
+static int even_steps(int n) { return (std::max(n, 1) + 1) & ~1; }
 
-int Engine::capture_graph() {
-    const int chunk = graph_chunk(opt.steps_per_poll);
-    if (graph_exec_ && graph_steps_ == chunk) return GCFG_OK;
-    if (graph_exec_) {
-        hipGraphExecDestroy(graph_exec_);
-        graph_exec_ = nullptr;
+void Engine::drop_graphs() {
+    for (StepGraph &g : graphs_) {
+        hipGraphExecDestroy(g.exec);
+        hipGraphDestroy(g.graph);
     }
-    if (graph_) {
-        hipGraphDestroy(graph_);
-        graph_ = nullptr;
+    graphs_.clear();
+}
+
+int Engine::step_graph(int steps, hipGraphExec_t *out) {
+    for (StepGraph &g : graphs_)
+        if (g.steps == steps) {
+            *out = g.exec;
+            return GCFG_OK;
+        }
+    if (graphs_.size() >= 8) {  // keep a handful of chunk lengths
+        hipGraphExecDestroy(graphs_.front().exec);
+        hipGraphDestroy(graphs_.front().graph);
+        graphs_.erase(graphs_.begin());
     }
+    StepGraph g{steps, nullptr, nullptr};
     GCFG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < chunk; ++i) launch_step(kp_, grid_pr_, grid_gr_, i & 1, stream_);
-    GCFG_HIP(hipStreamEndCapture(stream_, &graph_));
-    GCFG_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
-    graph_steps_ = chunk;
+    for (int i = 0; i < steps; ++i) launch_step(kp_, grid_pr_, grid_gr_, i & 1, stream_);
+    GCFG_HIP(hipStreamEndCapture(stream_, &g.graph));
+    hipError_t e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g.graph);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    graphs_.push_back(g);
+    *out = g.exec;
     return GCFG_OK;
 }
 
@@ -386,10 +391,12 @@ int Engine::launch_steps(int n) {
             parity_ ^= 1;
             steps_run += 1;
         }
-        int rc = capture_graph();
+        const int steps = even_steps(n);
+        hipGraphExec_t exec = nullptr;
+        int rc = step_graph(steps, &exec);
         if (rc != GCFG_OK) return rc;
-        GCFG_HIP(hipGraphLaunch(graph_exec_, stream_));
-        steps_run += graph_steps_;
+        GCFG_HIP(hipGraphLaunch(exec, stream_));
+        steps_run += steps;
     } else {
         for (int i = 0; i < n; ++i) {
             launch_step(kp_, grid_pr_, grid_gr_, parity_, stream_);
@@ -403,8 +410,10 @@ int Engine::launch_steps(int n) {
 
 int Engine::run(int64_t max_steps) {
     const int64_t start = steps_run;
+    int chunk = steps_hint > 0 ? std::min(steps_hint, 4096) : opt.steps_per_poll;
     for (;;) {
-        int rc = launch_steps(opt.steps_per_poll);
+        int rc = launch_steps(chunk);
+        chunk = opt.steps_per_poll;
         if (rc != GCFG_OK) return rc;
         GCFG_HIP(hipMemcpyAsync(pin_ctl_, kp_.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, stream_));
         GCFG_HIP(hipStreamSynchronize(stream_));
