@@ -193,10 +193,43 @@ def cpu_baseline(insts, flows_gpu, labels_gpu_fn, budget_s):
         match &= (f == int(flows_gpu[k])) and bool(np.array_equal(lab, labels_gpu_fn(k)))
         if secs > budget_s:
             break
-    return dict(value=round(px / secs / 1e6, 3), unit="Mpx/s", cores=threads, kind="port",
-                sample=f"{len(used)} of the {len(insts)} batch instances ({px / 1e6:.1f} Mpx), "
-                       f"reference push-relabel schedule restated in C/OpenMP",
-                seconds=round(secs, 2), matches_gpu=bool(match))
+    out = dict(value=round(px / secs / 1e6, 3), unit="Mpx/s", cores=threads, kind="port",
+               sample=f"{len(used)} of the {len(insts)} batch instances ({px / 1e6:.1f} Mpx), "
+                      f"reference push-relabel schedule restated in C/OpenMP",
+               seconds=round(secs, 2), matches_gpu=bool(match), cpu_model=cpu_model())
+    # The single-image configurations: 320x240 (1 thread and all threads) and
+    # 1024^2 (all threads), each checked against the GPU drop-in path.
+    per = {}
+    for kind, W, H, thread_list in (("c0", 320, 240, (1, threads)), ("c1", 1024, 1024, (threads,))):
+        img = gcfg.gen_ellipse_image(W, H, 1)
+        dt = gcfg.DataTerm(W, H)
+        dt.set_image(img)
+        dt.set_seeds(gcfg.gen_seed_map(W, H))
+        dt.compute()
+        pos, neg, up, down, left, right = dt.links()
+        dt.close()
+        gflow, glab, _ = gcfg.solve(W, H, pos, neg, edges=(up, down, left, right))
+        row = {}
+        for th in thread_list:
+            t0 = time.perf_counter()
+            f, lab, _, _ = oracle.refpr(W, H, pos, neg, edges=(up, down, left, right), threads=th)
+            dt_s = time.perf_counter() - t0
+            row[f"threads_{th}_mpx_s"] = round(W * H / dt_s / 1e6, 3)
+            row["matches_gpu"] = bool(f == gflow and np.array_equal(lab, glab))
+        per[kind] = row
+    out["configs"] = per
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def single_instance(kind, seed):
@@ -262,6 +295,51 @@ def single(args, opts):
     print(json.dumps(out), flush=True)
 
 
+def pmc_traffic(args, opts, kernel_prefix):
+    """HBM bytes per launch of the dominant kernel from two rocprofv3 counter
+    runs of this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each
+    scaled by what the same counter reads for a 1 GiB device-to-device copy
+    run in the same process. Returns (traffic dict or None, note)."""
+    import shutil
+    import sqlite3
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not found"
+    child = [sys.executable, os.path.abspath(__file__), "--no-cpu", "--no-serve", "--no-plain", "--no-pmc",
+             "--steps", "1", "--warmup", "0", "--calibrate-copy", "--instances", str(args.instances),
+             "--seed", str(args.seed), "--sweeps", str(opts.sweeps_per_gr), "--inner", str(opts.inner_iters),
+             "--poll", str(opts.steps_per_poll), "--gr-block", f"{opts.gr_block_w}x{opts.gr_block_h}",
+             "--gr-cap", str(opts.gr_cap)]
+    got = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="gcfg_pmc_")
+        cmd = ["timeout", "-s", "KILL", "150", exe, "--pmc", counter, "-d", d, "-o", "run", "--"] + child
+        log(f"counter run: {counter}")
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        db = os.path.join(d, "run_results.db")
+        if r.returncode != 0 or not os.path.exists(db):
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"rocprofv3 --pmc {counter} failed (exit {r.returncode})"
+        con = sqlite3.connect(db)
+        rows = con.execute("select kernel_name, value from counters_collection where counter_name = ?",
+                           (counter,)).fetchall()
+        con.close()
+        shutil.rmtree(d, ignore_errors=True)
+        calib = max((v for n, v in rows if "copyBuffer" in n), default=0.0)
+        vals = [v for n, v in rows if kernel_prefix in n]
+        if calib <= 0 or not vals:
+            return None, f"no {counter} samples for the copy or {kernel_prefix}"
+        scale = (1 << 20) / calib  # the copy moves 1 GiB = 2^20 KiB each way
+        got[counter] = (sum(vals) / len(vals) * 1024 * scale, scale, len(vals))
+    fetch, fscale, n = got["FETCH_SIZE"]
+    write, wscale, _ = got["WRITE_SIZE"]
+    return dict(bytes=int(fetch + write), fetch_bytes=int(fetch), write_bytes=int(write),
+                fetch_scale=round(fscale, 4), write_scale=round(wscale, 4), dispatches=n), "ok"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +360,7 @@ def main():
     ap.add_argument("--gr-block", default=None, help="global-relabel block WxH in tiles, e.g. 1x4")
     ap.add_argument("--gr-cap", type=int, default=None, help="distance cap of intermediate global relabels")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs for roofline.traffic")
     ap.add_argument("--tune-cache", default=os.environ.get("GCFG_TUNE_CACHE",
                                                            os.path.expanduser("~/.cache/gcfg/tune.txt")))
     ap.add_argument("--serve-qps-high", type=float, default=400.0)
@@ -375,12 +454,15 @@ def main():
     popts = gcfg.Options.from_buffer_copy(opts)
     popts.profile = 1
     pbatch = gcfg.Batch(batch_items(insts), popts)
+    pbatch.solve(labels_to_host=False)  # the second solve launches exactly the steps the first needed
     flows_p, _, pst = pbatch.solve(labels_to_host=False)
     n_tiles = sum(((it.W + 63) // 64) * ((it.H + 15) // 16) for it in insts)
     pr_bytes = pst.pr_tiles * 1024 * PR_BYTES_PER_NODE
     gr_bytes = 1024 * (pst.gr_init_tiles * GR_INIT_BYTES_PER_NODE +
                        (pst.gr_tiles - pst.gr_init_tiles) * GR_RELAX_BYTES_PER_NODE + n_tiles * FIN_BYTES_PER_NODE)
-    kern = {"k_pr": (pst.pr_kernel_ms, pr_bytes), "k_gr": (pst.gr_kernel_ms, gr_bytes)}
+    wave = opts.kernels != 0
+    kern = {("k_prw" if wave else "k_pr"): (pst.pr_kernel_ms, pr_bytes),
+            ("k_grb" if wave else "k_gr"): (pst.gr_kernel_ms, gr_bytes)}
     dom = max(kern, key=lambda k: kern[k][0])
     dms, dbytes = kern[dom]
     achieved = dbytes / (dms * 1e-3) / 1e9 if dms > 0 else 0.0
@@ -397,6 +479,16 @@ def main():
                     gr_init_tiles=int(pst.gr_init_tiles), fin_tiles=int(n_tiles))
     assert np.array_equal(flows_p, flows_b), "profiled solve differs"
     pbatch.close()
+    if not args.no_pmc:
+        name = {"k_prw": "k_prw(", "k_pr": "k_pr(", "k_grb": "k_grb<", "k_gr": "k_gr("}[dom]
+        traffic, note = pmc_traffic(args, opts, name)
+        if traffic:
+            roofline["traffic"] = traffic["bytes"]
+            roofline["traffic_detail"] = traffic
+            log(f"traffic of {dom}: {traffic}")
+        else:
+            roofline["traffic_note"] = note
+            log(f"traffic not measured: {note}")
 
     extra = {}
     if not args.no_plain:

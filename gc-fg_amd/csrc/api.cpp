@@ -25,6 +25,7 @@ struct gcfg_graph {
                   *right = nullptr;
     int32_t penalty = 0;
     gcfg_options opt{};
+    gcfg::StreamJoin join;
 };
 
 struct gcfg_batch {
@@ -34,6 +35,7 @@ struct gcfg_batch {
     int32_t *d_labels = nullptr;  // internal label arena
     std::vector<size_t> label_off;
     std::vector<InstState> states;
+    gcfg::StreamJoin join;
 };
 
 extern "C" {
@@ -139,6 +141,16 @@ int gcfg_graph_set_penalty(gcfg_graph *g, int32_t penalty) {
     return graph_push_inputs(g);
 }
 
+int gcfg_graph_set_stream(gcfg_graph *g, void *hip_stream) {
+    if (!g) return GCFG_ERR_INVALID;
+    return g->join.set((hipStream_t)hip_stream);
+}
+
+int gcfg_batch_set_stream(gcfg_batch *b, void *hip_stream) {
+    if (!b) return GCFG_ERR_INVALID;
+    return b->join.set((hipStream_t)hip_stream);
+}
+
 int gcfg_graph_set_options(gcfg_graph *g, const gcfg_options *opt) {
     if (!g || !opt) {
         set_error("gcfg_graph_set_options: null argument");
@@ -158,6 +170,7 @@ int gcfg_graph_build(gcfg_graph *g) {
     }
     if (g->opt.verbose) std::printf("pen %d\n", g->penalty);
     int rc = graph_push_inputs(g);
+    if (rc == GCFG_OK) rc = g->join.before(g->eng.stream());
     if (rc == GCFG_OK) rc = g->eng.build({g->slot});
     g->built = rc == GCFG_OK;
     return rc;
@@ -178,7 +191,9 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
     }
     g->eng.reset_profile();
     const int64_t steps0 = g->eng.steps_run;
-    int rc = g->eng.run(g->opt.max_steps);
+    int rc = g->join.before(g->eng.stream());
+    if (rc == GCFG_OK) rc = g->eng.run(g->opt.max_steps);
+    if (rc == GCFG_OK) rc = g->join.after(g->eng.stream());
     if (rc != GCFG_OK) return rc;
     InstState st;
     rc = g->eng.fetch_state(g->slot, &st);
@@ -326,8 +341,10 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
     }
     b->eng.reset_profile();
     const int64_t steps0 = b->eng.steps_run;
-    int rc = b->eng.build(b->slots);
+    int rc = b->join.before(b->eng.stream());
+    if (rc == GCFG_OK) rc = b->eng.build(b->slots);
     if (rc == GCFG_OK) rc = b->eng.run(b->eng.opt.max_steps);
+    if (rc == GCFG_OK) rc = b->join.after(b->eng.stream());
     if (rc != GCFG_OK) return rc;
     // Instance slots were assigned 0..n-1 in order at creation: one copy.
     rc = b->eng.fetch_states(b->states.data(), n);

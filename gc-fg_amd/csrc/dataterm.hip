@@ -322,6 +322,7 @@ int DtContext::unquantized(double *h_tpos, double *h_tneg, double *h_right, doub
 struct gcfg_dataterm {
     hipStream_t stream = nullptr;
     gcfg::DtContext ctx;
+    gcfg::StreamJoin join;
 };
 
 extern "C" {
@@ -394,9 +395,16 @@ int gcfg_dt_set_lambda(gcfg_dataterm *dt, int32_t lambda) {
 
 int gcfg_dt_compute(gcfg_dataterm *dt) {
     if (!dt) return GCFG_ERR_INVALID;
-    int rc = dt->ctx.compute_owned();
+    int rc = dt->join.before(dt->stream);
+    if (rc == GCFG_OK) rc = dt->ctx.compute_owned();
+    if (rc == GCFG_OK) rc = dt->join.after(dt->stream);
     if (rc == GCFG_OK && hipStreamSynchronize(dt->stream) != hipSuccess) return GCFG_ERR_HIP;
     return rc;
+}
+
+int gcfg_dt_set_stream(gcfg_dataterm *dt, void *hip_stream) {
+    if (!dt) return GCFG_ERR_INVALID;
+    return dt->join.set((hipStream_t)hip_stream);
 }
 
 int gcfg_dt_get_links(gcfg_dataterm *dt, int32_t **d_pos, int32_t **d_neg, int32_t **d_up, int32_t **d_down,

@@ -39,6 +39,34 @@ Engine::~Engine() {
     if (stream_) hipStreamDestroy(stream_);
 }
 
+int StreamJoin::set(hipStream_t s) {
+    if (s && !ev_in) {
+        GCFG_HIP(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+        GCFG_HIP(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+    }
+    user = s;
+    return GCFG_OK;
+}
+
+int StreamJoin::before(hipStream_t own) {
+    if (!user) return GCFG_OK;
+    GCFG_HIP(hipEventRecord(ev_in, user));
+    GCFG_HIP(hipStreamWaitEvent(own, ev_in, 0));
+    return GCFG_OK;
+}
+
+int StreamJoin::after(hipStream_t own) {
+    if (!user) return GCFG_OK;
+    GCFG_HIP(hipEventRecord(ev_out, own));
+    GCFG_HIP(hipStreamWaitEvent(user, ev_out, 0));
+    return GCFG_OK;
+}
+
+StreamJoin::~StreamJoin() {
+    if (ev_in) hipEventDestroy(ev_in);
+    if (ev_out) hipEventDestroy(ev_out);
+}
+
 int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
     opt = o;
     if (!options_valid(opt)) return GCFG_ERR_INVALID;

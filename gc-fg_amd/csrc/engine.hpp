@@ -125,6 +125,19 @@ void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_str
 void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, int32_t *d_dist,
                        hipStream_t s);
 
+// Orders a handle's internal stream against a caller stream: `before` makes
+// the internal stream wait for everything already queued on the caller's,
+// `after` makes the caller's wait for everything queued internally so far.
+// With no caller stream both are no-ops.
+struct StreamJoin {
+    hipStream_t user = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    int set(hipStream_t s);
+    int before(hipStream_t own);
+    int after(hipStream_t own);
+    ~StreamJoin();
+};
+
 void set_error(const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 

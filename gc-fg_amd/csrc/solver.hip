@@ -182,7 +182,15 @@ __device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int col
             if (kind == EM_ALL) {
                 pred = 1;
             } else if (kind == EM_LIVE) {
-                pred = p.live[t];
+                // live: 0 all unreachable, 1 some node below the cap, 2 all at or beyond the cap
+                const int lv = p.live[t];
+                pred = colour ? lv != 0 : lv == 1;
+                if (!colour && lv == 2) {
+                    // A skipped block's nodes are all frozen: none is active.
+                    const int tx0 = lt % D.tiles_x, ty0 = lt / D.tiles_x;
+                    for (int j = 0; j < by && ty0 + j < D.tiles_y; ++j)
+                        for (int i = 0; i < bx && tx0 + i < D.tiles_x; ++i) p.act_flag[t + j * D.tiles_x + i] = 0;
+                }
             } else if (kind == EM_DIRTY) {
                 pred = p.dirty[t];
                 if (pred) p.dirty[t] = 0;
@@ -239,7 +247,11 @@ __device__ void start_global_relabel(const KParams &p, const InstDesc &D, InstSt
         st.phase = PH_GR;
         st.cap = cap;
     }
-    const int n = reinit ? emit_tiles(p, D, EM_LIVE, 0, false, OP_GR_REINIT, parity)
+    // An exact round (cap DINF) revisits the tiles left at the cap; a capped
+    // round skips them: exact distances never decrease, so a tile whose nodes
+    // were all at distance >= cap stays there and keeps its labels (>= cap:
+    // frozen, and no push can enter a node at or beyond the cap).
+    const int n = reinit ? emit_tiles(p, D, EM_LIVE, cap >= DINF ? 1 : 0, false, OP_GR_REINIT, parity)
                          : emit_tiles(p, D, EM_ALL, 0, false, OP_GR_INIT, parity);
     if (threadIdx.x == 0) {
         st.gr_tiles += n;
@@ -1015,13 +1027,14 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity) {
                 break;
             }
         }
-        int act = 0, fin = 0;
+        int act = 0, fin = 0, nearv = 0;
         if (have) {
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
                 if (init || reinit || ((chg >> y) & 1)) T.d[y * TW + lane] = d[y];
                 act |= (m[y] & MB_EPOS) && d[y] < capv;
                 fin |= d[y] < DINF;
+                nearv |= d[y] < p.gr_cap;
             }
         }
         // Outer boundary changes re-queue the neighbouring blocks.
@@ -1038,9 +1051,10 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity) {
             set_active(p, t, any_act);
         }
         const bool any_fin = NW == 1 ? __any(fin) : __syncthreads_or(fin);
+        const bool any_near = p.gr_cap <= 0 || (NW == 1 ? __any(nearv) : __syncthreads_or(nearv));
         if (threadIdx.x == 0) {
             if (capped) mark_dirty(p, t0);
-            p.live[t0] = any_fin;
+            p.live[t0] = any_fin ? (any_near ? 1 : 2) : 0;
         }
     }
     if (p.profile && lane == 0) atomic_add64(&p.ctl->gr_iters, n_rounds);

@@ -47,7 +47,7 @@ int coordinate_search(gcfg_options start, int trials, const std::function<double
         {&gcfg_options::sweeps_per_gr, {1, 2, 4, 8}},
         {&gcfg_options::gr_rounds, {2, 4, 8, 48}},
         {&gcfg_options::steps_per_poll, {4, 8, 16, 32}},
-        {&gcfg_options::gr_cap, {0, 16, 32, 64, 128}},
+        {&gcfg_options::gr_cap, {0, 4, 8, 16, 32, 64}},
     };
     // Global-relabel block shapes (width, height in tiles) are one knob.
     const std::vector<std::pair<int32_t, int32_t>> shapes = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 16},

@@ -145,6 +145,10 @@ def hip():
         h.hipMemset.argtypes = [_VP, ctypes.c_int, ctypes.c_size_t]
         h.hipGetErrorString.restype = ctypes.c_char_p
         h.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        h.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        h.hipStreamDestroy.argtypes = [_VP]
+        h.hipStreamSynchronize.argtypes = [_VP]
+        h.hipMemcpyAsync.argtypes = [_VP, _VP, ctypes.c_size_t, ctypes.c_int, _VP]
         _hip = h
     return _hip
 
@@ -164,6 +168,28 @@ def gpu_available() -> bool:
 
 def device_synchronize():
     _hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
+
+
+class Stream:
+    """A HIP stream (``ptr`` is the hipStream_t) for ordering against a caller's work."""
+
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _hip_check(hip().hipStreamCreate(ctypes.byref(p)), "hipStreamCreate")
+        self.ptr = p.value
+
+    def synchronize(self):
+        _hip_check(hip().hipStreamSynchronize(_VP(self.ptr)), "hipStreamSynchronize")
+
+    def copy_async(self, dst_ptr: int, src_ptr: int, nbytes: int, kind: int = 3):
+        """hipMemcpyAsync on this stream (kind 3: device to device)."""
+        _hip_check(hip().hipMemcpyAsync(_VP(dst_ptr), _VP(src_ptr), int(nbytes), int(kind), _VP(self.ptr)),
+                   "hipMemcpyAsync")
+
+    def close(self):
+        if self.ptr:
+            hip().hipStreamDestroy(_VP(self.ptr))
+            self.ptr = None
 
 
 class DeviceBuffer:
@@ -320,6 +346,10 @@ class Graph:
                "gcfg_graph_read_state")
         return caps, exc
 
+    def set_stream(self, stream_ptr: int):
+        """Order this handle's work against a caller HIP stream (0: none)."""
+        _check(lib().gcfg_graph_set_stream(_VP(self.h), _VP(stream_ptr or None)), "gcfg_graph_set_stream")
+
     def close(self):
         if self.h:
             lib().gcfg_graph_destroy(_VP(self.h))
@@ -416,6 +446,10 @@ class DataTerm:
         _check(lib().gcfg_dt_unquantized(_VP(self.h), *[_VP(o.ctypes.data) for o in outs]), "gcfg_dt_unquantized")
         return [o.reshape(self.height, self.width) for o in outs]
 
+    def set_stream(self, stream_ptr: int):
+        """Order this handle's work against a caller HIP stream (0: none)."""
+        _check(lib().gcfg_dt_set_stream(_VP(self.h), _VP(stream_ptr or None)), "gcfg_dt_set_stream")
+
     def close(self):
         if self.h:
             lib().gcfg_dt_destroy(_VP(self.h))
@@ -494,6 +528,10 @@ class Batch:
     def device_labels(self, i: int) -> np.ndarray:
         it = self.items[i]
         return self.labels[i].download(np.int32, it.width * it.height).reshape(it.height, it.width)
+
+    def set_stream(self, stream_ptr: int):
+        """Order this handle's work against a caller HIP stream (0: none)."""
+        _check(lib().gcfg_batch_set_stream(_VP(self.h), _VP(stream_ptr or None)), "gcfg_batch_set_stream")
 
     def close(self):
         if self.h:

@@ -83,6 +83,12 @@ int gcfg_graph_set_edges(gcfg_graph *g, const int32_t *d_up, const int32_t *d_do
                          const int32_t *d_right);
 /* GC_SetPenalty, GraphCut.cu:296-299 (uniform n-link capacity). */
 int gcfg_graph_set_penalty(gcfg_graph *g, int32_t penalty);
+/* Order the handle's work after everything queued on a caller HIP stream
+ * (hipStream_t passed as void*) at each build/solve, and make that stream
+ * wait for the handle's work (device labels included) when the call returns.
+ * NULL restores the default (calls are synchronous to the host only). The
+ * reference runs everything on the legacy default stream (GraphCut.cu). */
+int gcfg_graph_set_stream(gcfg_graph *g, void *hip_stream);
 int gcfg_graph_set_options(gcfg_graph *g, const gcfg_options *opt);
 /* GC_SetGraph, GraphCut.cu:301-319: builds the residual graph on the device. */
 int gcfg_graph_build(gcfg_graph *g);
@@ -122,6 +128,8 @@ int gcfg_dt_reset(gcfg_dataterm *dt);
 int gcfg_dt_set_lambda(gcfg_dataterm *dt, int32_t lambda);
 /* getDataTermsMixture, DataTerm.cpp:129-137: histograms, beta, t-links, n-links. */
 int gcfg_dt_compute(gcfg_dataterm *dt);
+/* As gcfg_graph_set_stream, for gcfg_dt_compute. */
+int gcfg_dt_set_stream(gcfg_dataterm *dt, void *hip_stream);
 /* getPos / getNeg, DataTerm.cpp:12-26, plus the four n-link planes (device, owned by dt). */
 int gcfg_dt_get_links(gcfg_dataterm *dt, int32_t **d_pos, int32_t **d_neg, int32_t **d_up, int32_t **d_down,
                       int32_t **d_left, int32_t **d_right);
@@ -141,6 +149,8 @@ typedef struct gcfg_instance {
 } gcfg_instance;
 typedef struct gcfg_batch gcfg_batch;
 int gcfg_batch_create(int32_t count, const gcfg_instance *inst, const gcfg_options *opt, gcfg_batch **out);
+/* As gcfg_graph_set_stream, for gcfg_batch_solve / gcfg_batch_solve_device. */
+int gcfg_batch_set_stream(gcfg_batch *b, void *hip_stream);
 /* Cold solve of every instance; labels to host (h_labels[i]: W_i*H_i) and flows. */
 int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_stats *stats);
 /* Cold solve with labels written to caller-owned device buffers (d_labels[i]: W_i*H_i). */

@@ -355,3 +355,32 @@ def test_capped_relabels(g, ora, cap):
     inter[1:-1, 1:-1] = True
     assert np.array_equal(dist[inter].astype(np.int64), _bfs_to_sinks(W, H, caps, exc)[inter])
     assert st.flow == ora.maxflow(W, H, pos, neg, edges=E)[0]
+
+
+def test_caller_stream_ordering(g, ora):
+    """With a caller stream set, a device copy of the labels queued on that
+    stream right after optimize_device sees the final labels; results equal
+    the oracle."""
+    rng = np.random.default_rng(33)
+    W, H = 300, 120
+    pos, neg, E = random_instance(rng, W, H, tmax=100, emax=30)
+    rflow, rlabel = ora.maxflow(W, H, pos, neg, edges=E)
+    s = g.Stream()
+    gr = g.Graph(W, H)
+    gr.set_stream(s.ptr)
+    gr.set_dataterms(pos, neg)
+    gr.set_edges(*E)
+    gr.build()
+    d_label = g.DeviceBuffer(4 * W * H)
+    d_copy = g.DeviceBuffer(4 * W * H)
+    st = gr.optimize_device(d_label)
+    s.copy_async(d_copy.ptr, d_label.ptr, 4 * W * H)
+    s.synchronize()
+    assert st.flow == rflow
+    assert np.array_equal(d_copy.download(np.int32, W * H).reshape(H, W), rlabel)
+    gr.set_stream(0)
+    gr.build()
+    label, st2 = gr.optimize()
+    assert st2.flow == rflow and np.array_equal(label, rlabel)
+    gr.close()
+    s.close()
