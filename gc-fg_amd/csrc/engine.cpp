@@ -97,6 +97,8 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
     rc |= dalloc((void **)&kp_.act_flag, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.dirty, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.live, (size_t)cap_tiles * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.stale, (size_t)cap_tiles * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.tsum, (size_t)cap_tiles * 2 * sizeof(int64_t));
     rc |= dalloc((void **)&kp_.queue_pr, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.queue_gr, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.desc, (size_t)cap_slots * sizeof(InstDesc));
@@ -113,6 +115,7 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
     GCFG_HIP(hipMemsetAsync(kp_.dirty, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
     GCFG_HIP(hipMemsetAsync(kp_.act_flag, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
     GCFG_HIP(hipMemsetAsync(kp_.live, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
+    GCFG_HIP(hipMemsetAsync(kp_.stale, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
     GCFG_HIP(hipStreamSynchronize(stream_));
     kp_.n_slots = cap_slots;
     apply_options();
@@ -148,7 +151,7 @@ void Engine::apply_options() {
     int gbx = opt.gr_block_w > 0 ? opt.gr_block_w : 1, gby = opt.gr_block_h > 0 ? opt.gr_block_h : 1;
     if (!variant || !gr_block_supported(gbx, gby)) gbx = gby = 1;
     const int gpr = variant ? cu * 16 : cu * 4;
-    const int ggr = variant ? std::max(cu, cu * 16 / (gbx * gby)) : cu * 8;
+    const int ggr = variant ? std::max(cu, cu * 28 / (gbx * gby)) : cu * 8;
     const int grr = opt.gr_rounds > 0 ? opt.gr_rounds : 48;
     const int prof = opt.profile ? 1 : 0;
     const int cap = variant && opt.gr_cap > 0 ? opt.gr_cap : 0;

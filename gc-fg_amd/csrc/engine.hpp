@@ -40,8 +40,9 @@ constexpr int LW = TW + 2;              // LDS row pitch with halo
 constexpr int LH = TH + 2;
 
 enum Phase : int32_t { PH_IDLE = 0, PH_START = 1, PH_GR = 2, PH_PR = 3, PH_FIN = 4, PH_DONE = 5 };
-// OP_GR_INIT derives masks from capacities and excess (after a build or an
-// edit); OP_GR_REINIT starts the BFS from the maintained masks.
+// OP_GR_INIT (workgroup kernels) derives masks from capacities and excess;
+// OP_GR_REINIT (wavefront kernels) starts the BFS from the kept masks, which
+// the build, the push passes and edits keep current.
 enum GrOp : int32_t { OP_GR_INIT = 1, OP_GR_RELAX = 2, OP_FIN = 3, OP_GR_REINIT = 4 };
 constexpr int MB_EPOS = 16, MB_ENEG = 32;  // mask bits: excess > 0, excess < 0
 
@@ -64,8 +65,8 @@ struct InstState {
     int32_t pr_iter;
     int32_t gr_rounds;
     int32_t steps;
-    int64_t sum_pos;       // interior sum of pos (current t-links)
-    int64_t sum_exc;       // interior sum of max(e, 0) at FIN
+    int64_t sum_pos;       // change of the interior sum of pos by edits since the build
+    int64_t sum_exc;       // unused
     int64_t flow;
     int64_t pr_tiles;      // tile push-relabel passes scheduled
     int64_t gr_tiles;      // tile BFS sweeps scheduled (initial and relaxation)
@@ -94,7 +95,15 @@ struct KParams {
     // arena
     int4 *caps;
     int32_t *exc, *dist, *act_flag, *dirty;
-    int32_t *live;  // per tile: some node had a finite distance at its last BFS sweep
+    // Per global-relabel block (at its anchor tile), from its last BFS sweep:
+    // 0 every node unreachable, 1 all at or beyond the cap, 2 some node below it.
+    int32_t *live;
+    // Per block: within cap distance of a tile changed by a push pass since the
+    // last relabel round (capped rounds recompute only these).
+    int32_t *stale;
+    // Per tile: [2t] interior sum of pos (build), [2t + 1] interior sum of
+    // max(excess, 0) (FIN); the scheduler adds them up per instance.
+    int64_t *tsum;
     uint8_t *mask;
     TileInfo *tiles;
     InstDesc *desc;
@@ -190,7 +199,7 @@ class Engine {
     size_t tiles_free() const;
     int cap_tiles() const { return cap_tiles_; }
     int tiles_for(int W, int H) const { return ((W + TW - 1) / TW) * ((H + TH - 1) / TH); }
-    static size_t bytes_per_tile() { return (size_t)TN * (16 + 4 + 4 + 1) + 64; }
+    static size_t bytes_per_tile() { return (size_t)TN * (16 + 4 + 4 + 1) + 96; }
     gcfg_options opt;
     int steps_run = 0;
     // Steps the previous run needed (set by the caller from the instance

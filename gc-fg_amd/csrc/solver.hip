@@ -107,7 +107,8 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
             p.tiles[t] = ti;
             p.act_flag[t] = 0;
             p.dirty[t] = 0;
-            p.live[t] = 1;
+            p.live[t] = 2;
+            p.stale[t] = 0;
         }
         int64_t sp = 0;
         int bad = 0;
@@ -145,7 +146,7 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
         const int64_t tot = block_sum64(sp, red);
         const int anybad = __syncthreads_or(bad);
         if (threadIdx.x == 0) {
-            if (tot) atomic_add64(&p.state[s].sum_pos, tot);
+            p.tsum[2 * (size_t)t] = tot;
             if (anybad) atomicOr(&p.state[s].err, 1);
         }
         __syncthreads();
@@ -182,10 +183,13 @@ __device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int col
             if (kind == EM_ALL) {
                 pred = 1;
             } else if (kind == EM_LIVE) {
-                // live: 0 all unreachable, 1 some node below the cap, 2 all at or beyond the cap
+                // Exact round (colour 1): every block with a reachable node.
+                // Capped round: blocks with a node below the cap that a push
+                // pass may have changed (the others keep their labels).
                 const int lv = p.live[t];
-                pred = colour ? lv != 0 : lv == 1;
-                if (!colour && lv == 2) {
+                pred = colour ? lv != 0 : (lv == 2 && p.stale[t]);
+                p.stale[t] = 0;
+                if (!colour && lv == 1) {
                     // A skipped block's nodes are all frozen: none is active.
                     const int tx0 = lt % D.tiles_x, ty0 = lt / D.tiles_x;
                     for (int j = 0; j < by && ty0 + j < D.tiles_y; ++j)
@@ -241,18 +245,22 @@ __device__ bool any_active_tile(const KParams &p, const InstDesc &D) {
 // the solve, so a capped round that leaves no active node is followed by an
 // exact one.
 __device__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st, int parity, bool reinit,
-                                     int cap) {
+                                     int cap, bool all = false) {
     if (threadIdx.x == 0) {
         st.gr_rounds++;
         st.phase = PH_GR;
         st.cap = cap;
     }
-    // An exact round (cap DINF) revisits the tiles left at the cap; a capped
-    // round skips them: exact distances never decrease, so a tile whose nodes
-    // were all at distance >= cap stays there and keeps its labels (>= cap:
-    // frozen, and no push can enter a node at or beyond the cap).
-    const int n = reinit ? emit_tiles(p, D, EM_LIVE, cap >= DINF ? 1 : 0, false, OP_GR_REINIT, parity)
-                         : emit_tiles(p, D, EM_ALL, 0, false, OP_GR_INIT, parity);
+    // An exact round (cap DINF) revisits every block with a reachable node. A
+    // capped round with cap K recomputes only blocks that have a node below
+    // K and lie within K of a tile a push pass changed since the last round.
+    // Exact distances never decrease, so a block with all nodes at >= K
+    // stays there (its labels >= K are frozen and no push can enter such a
+    // node), and for a block farther than K from every change each shortest
+    // path of length < K is unchanged, so min(d, K) is unchanged too.
+    const int n = !reinit ? emit_tiles(p, D, EM_ALL, 0, false, OP_GR_INIT, parity)
+                  : all   ? emit_tiles(p, D, EM_ALL, 0, false, OP_GR_REINIT, parity)
+                          : emit_tiles(p, D, EM_LIVE, cap >= DINF ? 1 : 0, false, OP_GR_REINIT, parity);
     if (threadIdx.x == 0) {
         st.gr_tiles += n;
         st.gr_init_tiles += n;
@@ -269,21 +277,33 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
     int pr_iter = st.pr_iter;
     __syncthreads();  // every thread has read the state before thread 0 writes it
     if (threadIdx.x == 0) st.steps++;
+    const InstDesc D = p.desc[s];
     if ((phase == PH_START && err) || phase == PH_FIN) {
+        // flow = sum of pos - sum of max(excess, 0) over the interior, from
+        // the per-tile sums of the build and the FIN sweep.
+        __shared__ int64_t red[NT / 64];
+        int64_t v = 0;
+        if (phase == PH_FIN)
+            for (int lt = threadIdx.x; lt < D.ntiles; lt += NT) {
+                const size_t t = (size_t)(D.tile0 + lt);
+                v += p.tsum[2 * t] - p.tsum[2 * t + 1];
+            }
+        const int64_t tot = block_sum64(v, red);
         if (threadIdx.x == 0) {
-            if (phase == PH_FIN) st.flow = st.sum_pos - st.sum_exc;
+            if (phase == PH_FIN) st.flow = st.sum_pos + tot;
             st.phase = PH_DONE;
             atomicSub(&p.ctl->n_live, 1);
         }
         return;
     }
-    const InstDesc D = p.desc[s];
     const int cap = (p.variant && p.gr_cap > 0) ? p.gr_cap : DINF;
+    const bool reinit = p.variant != 0;
     if (phase == PH_START) {
-        start_global_relabel(p, D, st, parity, false, cap);
+        // A warm start relabels every tile: an edit may have revived tiles
+        // the kept flags call unreachable (edits keep the masks current).
+        start_global_relabel(p, D, st, parity, reinit, cap, true);
         return;
     }
-    const bool reinit = p.variant != 0;
     if (phase == PH_GR) {
         const int n = emit_tiles(p, D, EM_DIRTY, 0, false, OP_GR_RELAX, parity);
         if (n > 0) {
@@ -566,7 +586,7 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p, int parity) {
                 }
             }
             const int64_t tot = block_sum64(sx, red);
-            if (tid == 0 && tot) atomic_add64(&st.sum_exc, tot);
+            if (tid == 0) p.tsum[2 * (size_t)t + 1] = tot;
             __syncthreads();
             continue;
         }
@@ -820,6 +840,18 @@ __device__ __forceinline__ void scan_from_right(int &A, int &B, int lane) {
     compose_after(A, B, pA, pB);
 }
 
+// Residual masks of a lane's 16 rows, packed four rows per register (bits
+// 8k..8k+7 of w[y / 4] for row y = 4 * (y / 4) + k).
+struct Masks {
+    uint32_t w[TH / 4];
+    __device__ __forceinline__ void clear() {
+#pragma unroll
+        for (int i = 0; i < TH / 4; ++i) w[i] = 0;
+    }
+    __device__ __forceinline__ int get(int y) const { return (int)((w[y >> 2] >> ((y & 3) * 8)) & 0xffu); }
+    __device__ __forceinline__ void put(int y, int v) { w[y >> 2] |= (uint32_t)(v & 0xff) << ((y & 3) * 8); }
+};
+
 // Relax d(u) = min(d(u), 1 + d(v)) over residual arcs u->v (mask bits: 1 left,
 // 2 right, 4 up, 8 down) until no value changes. Per round: a downward and an
 // upward sweep of each lane's column in registers, then for each row an exact
@@ -830,15 +862,21 @@ __device__ __forceinline__ void scan_from_right(int &A, int &B, int lane) {
 // only if a one-step check finds a value that can still decrease (otherwise
 // the row is already at its fixed point). Returns true when converged; chg
 // collects the rows whose value changed in this lane.
-__device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], const int (&m)[TH], int hU, int hD, int hs, int lane,
+__device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, int hU, int hD, int hs, int lane,
                                                 uint32_t &chg, int &rounds, int max_rounds) {
     rounds = 0;
     while (rounds < max_rounds) {
         ++rounds;
+        // Keeps the per-row values derived from the masks and the side halo
+        // (arc selects, scalar halo reads) inside the loop: hoisted, they
+        // would hold about 32 vector and 64 scalar registers across it.
+        asm volatile("" : "+v"(hs));
+#pragma unroll
+        for (int i = 0; i < TH / 4; ++i) asm volatile("" : "+v"(m.w[i]));
         uint32_t rch = 0;
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
-            if (m[y] & 4) {
+            if (m.get(y) & 4) {
                 const int cand = (y ? d[y - 1] : hU) + 1;
                 if (cand < d[y]) {
                     d[y] = cand;
@@ -848,7 +886,7 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], const int (&m)[TH]
         }
 #pragma unroll
         for (int y = TH - 1; y >= 0; --y) {
-            if (m[y] & 8) {
+            if (m.get(y) & 8) {
                 const int cand = (y < TH - 1 ? d[y + 1] : hD) + 1;
                 if (cand < d[y]) {
                     d[y] = cand;
@@ -862,14 +900,15 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], const int (&m)[TH]
             const int hsl = __builtin_amdgcn_readlane(hs, y), hsr = __builtin_amdgcn_readlane(hs, TH + y);
             if (lane == 0) hl = hsl;
             if (lane == 63) hr = hsr;
-            const bool can = ((m[y] & 1) && hl + 1 < d[y]) || ((m[y] & 2) && hr + 1 < d[y]);
+            const int my = m.get(y);
+            const bool can = ((my & 1) && hl + 1 < d[y]) || ((my & 2) && hr + 1 < d[y]);
             if (!__any(can)) continue;
             // left to right: arcs x -> x-1, halo = left column
-            int A = d[y], B = (m[y] & 1) ? 1 : DINF;
+            int A = d[y], B = (my & 1) ? 1 : DINF;
             scan_from_left(A, B);
             int nd = min(A, dmin_sat(hsl + B, DINF));
             // right to left: arcs x -> x+1, halo = right column
-            int A2 = nd, B2 = (m[y] & 2) ? 1 : DINF;
+            int A2 = nd, B2 = (my & 2) ? 1 : DINF;
             scan_from_right(A2, B2, lane);
             nd = min(A2, dmin_sat(hsr + B2, DINF));
             if (nd < d[y]) {
@@ -938,52 +977,31 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity) {
                 }
             }
             sx = wave_sum64_w(sx);
-            if (lane == 0 && sx) atomic_add64(&p.state[inst].sum_exc, sx);
+            if (lane == 0) p.tsum[2 * (size_t)t + 1] = sx;
             continue;
         }
-        const bool init = op == OP_GR_INIT, reinit = op == OP_GR_REINIT;
+        // (OP_GR_INIT is the workgroup kernels' op; here masks are rebuilt by
+        // the push kernel's OP_MASK pass and every relabel starts with REINIT.)
+        const bool reinit = op == OP_GR_REINIT;
         const int capv = p.state[inst].cap;  // start value of non-sink nodes (DINF: exact)
-        int d[TH], m[TH];
+        int d[TH];
+        Masks m;
+        m.clear();
         int hU = DINF, hD = DINF, hs = DINF;
         if (!have) {
 #pragma unroll
-            for (int y = 0; y < TH; ++y) {
-                d[y] = DINF;
-                m[y] = 0;
-            }
-        } else if (init) {
-            // Two halves of 8 rows: all loads of a half first, then its mask
-            // stores (a store between loads would serialise every row on a
-            // possible alias).
-            load_halo<HALO_EXCESS>(p, ti, t, lane, hU, hD, hs);
-#pragma unroll
-            for (int h = 0; h < TH; h += TH / 2) {
-                int4 c[TH / 2];
-                int ee[TH / 2];
-#pragma unroll
-                for (int k = 0; k < TH / 2; ++k) {
-                    c[k] = T.c[(h + k) * TW + lane];
-                    ee[k] = T.e[(h + k) * TW + lane];
-                }
-#pragma unroll
-                for (int k = 0; k < TH / 2; ++k) {
-                    m[h + k] = node_mask(c[k], ee[k]);
-                    d[h + k] = ee[k] < 0 ? 0 : capv;
-                }
-#pragma unroll
-                for (int k = 0; k < TH / 2; ++k) T.m[(h + k) * TW + lane] = (uint8_t)m[h + k];
-            }
+            for (int y = 0; y < TH; ++y) d[y] = DINF;
         } else if (reinit) {
             load_halo<HALO_MASK>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
-            for (int y = 0; y < TH; ++y) m[y] = T.m[y * TW + lane];
+            for (int y = 0; y < TH; ++y) m.put(y, T.m[y * TW + lane]);
 #pragma unroll
-            for (int y = 0; y < TH; ++y) d[y] = (m[y] & MB_ENEG) ? 0 : capv;
+            for (int y = 0; y < TH; ++y) d[y] = (m.get(y) & MB_ENEG) ? 0 : capv;
         } else {
             load_halo<HALO_DIST>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
-                m[y] = T.m[y * TW + lane];
+                m.put(y, T.m[y * TW + lane]);
                 d[y] = T.d[y * TW + lane];
             }
         }
@@ -1031,8 +1049,8 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity) {
         if (have) {
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
-                if (init || reinit || ((chg >> y) & 1)) T.d[y * TW + lane] = d[y];
-                act |= (m[y] & MB_EPOS) && d[y] < capv;
+                if (reinit || ((chg >> y) & 1)) T.d[y * TW + lane] = d[y];
+                act |= (m.get(y) & MB_EPOS) && d[y] < capv;
                 fin |= d[y] < DINF;
                 nearv |= d[y] < p.gr_cap;
             }
@@ -1054,7 +1072,7 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity) {
         const bool any_near = p.gr_cap <= 0 || (NW == 1 ? __any(nearv) : __syncthreads_or(nearv));
         if (threadIdx.x == 0) {
             if (capped) mark_dirty(p, t0);
-            p.live[t0] = any_fin ? (any_near ? 1 : 2) : 0;
+            p.live[t0] = any_fin ? (any_near ? 2 : 1) : 0;
         }
     }
     if (p.profile && lane == 0) atomic_add64(&p.ctl->gr_iters, n_rounds);
@@ -1067,10 +1085,10 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
     int64_t n_iters = 0;
     for (int q = blockIdx.x; q < qlen; q += gridDim.x) {
         const int t = p.queue_pr[q];
+        const TileRef T = tile_ref(p, t);
         const TileInfo ti = p.tiles[t];
         const int dthr = p.desc[ti.inst].n_interior;
         const int capv = p.state[ti.inst].cap;  // nodes at the cap are frozen until the next relabel
-        const TileRef T = tile_ref(p, t);
         int hU, hD, hs;
         load_halo<HALO_DIST>(p, ti, t, lane, hU, hD, hs);
         int4 c[TH];
@@ -1221,6 +1239,18 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
             if (anyD) set_active(p, ti.nbr[3], 1);
             set_active(p, t, any_mine);
         }
+        if (p.gr_cap > 0) {
+            // Blocks within cap distance of this tile (and of the tiles it
+            // pushed into) need their capped distances recomputed.
+            const InstDesc &D = p.desc[ti.inst];
+            const int rx = (p.gr_cap + TW - 1) / TW + ((anyL || anyR) ? 1 : 0);
+            const int ry = (p.gr_cap + TH - 1) / TH + ((anyU || anyD) ? 1 : 0);
+            const int span = 2 * rx + 1, cnt = span * (2 * ry + 1);
+            for (int k = lane; k < cnt; k += TW) {
+                const int x = ti.tx + k % span - rx, y = ti.ty + k / span - ry;
+                if (x >= 0 && y >= 0 && x < D.tiles_x && y < D.tiles_y) p.stale[block_anchor(p, D, x, y)] = 1;
+            }
+        }
         n_iters += it;
     }
     if (p.profile && lane == 0) atomic_add64(&p.ctl->pr_iters, n_iters);
@@ -1258,7 +1288,8 @@ __global__ void k_clear_tiles(KParams p, const int32_t *slots, const int32_t *pr
         p.tiles[t] = ti;
         p.act_flag[t] = 0;
         p.dirty[t] = 0;
-        p.live[t] = 1;
+        p.live[t] = 0;  // raised by the build sweep of the block's tiles
+        p.stale[t] = 0;
     }
 }
 
@@ -1283,7 +1314,9 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
         const int X = tx * TW + lane, Y0 = ty * TH;
         const TileRef T = tile_ref(p, t);
-        int d[TH], m[TH];
+        int d[TH];
+        Masks m;
+        m.clear();
         const int capv = p.gr_cap > 0 ? p.gr_cap : DINF;  // the first relabel sweep, capped like k_arm says
         int64_t sp = 0;
         int bad = 0;
@@ -1330,20 +1363,24 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
                 }
                 T.c[y * TW + lane] = c;
                 T.e[y * TW + lane] = e;
-                m[y] = node_mask(c, e);
-                T.m[y * TW + lane] = (uint8_t)m[y];
+                const int mk = node_mask(c, e);
+                m.put(y, mk);
+                T.m[y * TW + lane] = (uint8_t)mk;
                 d[y] = e < 0 ? 0 : capv;
             }
         }
         uint32_t chg = 0;
         int rounds = 0;
         const bool conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, p.gr_rounds);
-        int act = 0;
+        int act = 0, fin = 0, nearv = 0;
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
             T.d[y * TW + lane] = d[y];
-            act |= (m[y] & MB_EPOS) && d[y] < capv;
+            act |= (m.get(y) & MB_EPOS) && d[y] < capv;
+            fin |= d[y] < DINF;
+            nearv |= d[y] < capv;
         }
+        const int rank = __any(fin) ? (__any(nearv) ? 2 : 1) : 0;
         const bool chL = __any(lane == 0 && chg != 0);
         const bool chR = __any(lane == TW - 1 && chg != 0);
         const bool chU = __any((chg & 1u) != 0);
@@ -1359,7 +1396,8 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
             if (chD && ty < D.tiles_y - 1) mark_dirty(p, block_anchor(p, D, tx, ty + 1));
             if (!conv) mark_dirty(p, block_anchor(p, D, tx, ty));
             set_active(p, t, any_act);
-            if (sp) atomic_add64(&p.state[s].sum_pos, sp);
+            if (rank) atomicMax(&p.live[block_anchor(p, D, tx, ty)], rank);
+            p.tsum[2 * (size_t)t] = sp;
             if (any_bad) atomicOr(&p.state[s].err, 1);
         }
     }
@@ -1391,6 +1429,24 @@ __global__ __launch_bounds__(NT) void k_edit(KParams p, int slot, const int32_t 
         const int64_t tot = block_sum64(dp, red);
         if (threadIdx.x == 0 && tot) atomic_add64(&p.state[slot].sum_pos, tot);
         __syncthreads();
+    }
+}
+
+// After k_edit: excess-sign bits of the residual masks under the strokes
+// (idempotent where strokes overlap).
+__global__ __launch_bounds__(NT) void k_edit_masks(KParams p, int slot, const int32_t *strokes, int n) {
+    const InstDesc D = p.desc[slot];
+    for (int s = blockIdx.x; s < n; s += gridDim.x) {
+        const int x0 = strokes[5 * s], y0 = strokes[5 * s + 1], w = strokes[5 * s + 2], h = strokes[5 * s + 3];
+        for (int i = threadIdx.x; i < w * h; i += NT) {
+            const int X = x0 + i % w, Y = y0 + i / w;
+            if (X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1) {
+                const int t = D.tile0 + (Y / TH) * D.tiles_x + X / TW;
+                const size_t g = (size_t)t * TN + (Y % TH) * TW + (X % TW);
+                const int e = p.exc[g];
+                p.mask[g] = (uint8_t)((p.mask[g] & 15) | (e > 0 ? MB_EPOS : 0) | (e < 0 ? MB_ENEG : 0));
+            }
+        }
     }
 }
 
@@ -1488,6 +1544,8 @@ void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_str
     if (n_strokes <= 0) return;
     hipLaunchKernelGGL(k_edit, dim3(n_strokes < 1024 ? n_strokes : 1024), dim3(NT), 0, s, p, slot, d_strokes,
                        n_strokes, delta);
+    hipLaunchKernelGGL(k_edit_masks, dim3(n_strokes < 1024 ? n_strokes : 1024), dim3(NT), 0, s, p, slot, d_strokes,
+                       n_strokes);
 }
 
 void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, int32_t *d_dist,
