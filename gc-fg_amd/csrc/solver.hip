@@ -156,7 +156,7 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
 // ---------------------------------------------------------------------------
 // Scheduler: one workgroup per instance slot. Decisions are block-uniform
 // (taken from block-wide scans of the tile flags); thread 0 writes the state.
-enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2, EM_LIVE = 3 };
+enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2, EM_LIVE = 3, EM_DEAD = 4 };
 
 // Append the instance's tiles (push/relabel queue) or global-relabel blocks
 // (identified by their anchor tile, whose flags stand for the block)
@@ -182,6 +182,8 @@ __device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int col
         if (lu < nunits) {
             if (kind == EM_ALL) {
                 pred = 1;
+            } else if (kind == EM_DEAD) {
+                pred = p.live[t] == 0;
             } else if (kind == EM_LIVE) {
                 // Exact round (colour 1): every block with a reachable node.
                 // Capped round: blocks with a node below the cap that a push
@@ -317,7 +319,9 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
                 return;
             }
             if (threadIdx.x == 0) st.phase = PH_FIN;
-            emit_tiles(p, D, EM_ALL, 0, false, OP_FIN, parity);
+            // Wavefront kernels: the exact round wrote the labels of every
+            // block it visited; FIN labels the blocks it skipped as unreachable.
+            emit_tiles(p, D, p.variant ? EM_DEAD : EM_ALL, 0, false, OP_FIN, parity);
             return;
         }
         if (threadIdx.x == 0) st.phase = PH_PR;
@@ -566,8 +570,7 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p, int parity) {
         if (item < 0) break;
         const int op = item & 7, t = item >> 3;
         const TileInfo ti = p.tiles[t];
-        const InstDesc &D = p.desc[ti.inst];
-        InstState &st = p.state[ti.inst];
+        const InstDesc D = p.desc[ti.inst];  // by value: stores below cannot force reloads
         const size_t base = (size_t)t * TN;
 
         if (op == OP_FIN) {
@@ -922,6 +925,22 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, int hU, 
     return false;
 }
 
+// Labels of one tile from converged exact distances: 1 for interior nodes
+// that cannot reach the sink (SPREAD_ZEROS set, GraphCutKernels.cu:228-351),
+// 0 elsewhere, border included.
+__device__ __forceinline__ void write_labels(const InstDesc &D, int tx, int ty, int lane, const int (&d)[TH]) {
+    const int X = tx * TW + lane;
+    if (X >= D.W) return;
+#pragma unroll
+    for (int y = 0; y < TH; ++y) {
+        const int Y = ty * TH + y;
+        if (Y < D.H) {
+            const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
+            D.label_out[(size_t)Y * D.W + X] = (in && d[y] >= DINF) ? 1 : 0;
+        }
+    }
+}
+
 // Anchor (top-left tile) of the global-relabel block holding tile (x, y).
 __device__ __forceinline__ int block_anchor(const KParams &p, const InstDesc &D, int x, int y) {
     return D.tile0 + (y - y % p.gby) * D.tiles_x + (x - x % p.gbx);
@@ -937,8 +956,16 @@ constexpr int GR_MAX_EXCHANGES = 64;
 // neighbouring blocks' stored distances (upper bounds: a stale read only
 // delays convergence); a block whose outer boundary changed marks its
 // neighbours for the next step. FIN writes the labels of the block's tiles.
+// The queue, tile table, descriptors and states are read-only in the
+// wavefront kernels; passing them as separate restrict arguments lets their
+// uniform loads go through the scalar cache instead of waiting behind the
+// kernel's own vector stores.
+#define GCFG_RO_ARGS                                                                                  \
+    const int32_t *__restrict__ rq, const TileInfo *__restrict__ rtiles, const InstDesc *__restrict__ rdesc, \
+        const InstState *__restrict__ rstate
+
 template <int BX, int BY>
-__global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity) {
+__global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCFG_RO_ARGS) {
     constexpr int NW = BX * BY;
     __shared__ int s_top[NW][TW], s_bot[NW][TW], s_side[NW][2 * TH];
     const int qlen = p.ctl->q[parity].qlen_gr;
@@ -947,43 +974,32 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity) {
     const int wi = w % BX, wj = w / BX;
     int64_t n_rounds = 0;
     for (int q = blockIdx.x; q < qlen; q += gridDim.x) {
-        const int item = p.queue_gr[q];
+        const int item = rq[q];
         const int op = item & 7, t0 = item >> 3;
-        const int inst = p.tiles[t0].inst;
-        const InstDesc &D = p.desc[inst];
-        const int ax = p.tiles[t0].tx, ay = p.tiles[t0].ty;
+        const int inst = rtiles[t0].inst;
+        // By value: with a reference every label or flag store below could
+        // alias the descriptor and force a reload before the next use.
+        const InstDesc D = rdesc[inst];
+        const int ax = rtiles[t0].tx, ay = rtiles[t0].ty;
         const int tx = ax + wi, ty = ay + wj;
         const bool have = tx < D.tiles_x && ty < D.tiles_y;
         const int t = have ? t0 + wj * D.tiles_x + wi : t0;
-        const TileInfo ti = p.tiles[t];
+        const TileInfo ti = rtiles[t];
         const TileRef T = tile_ref(p, t);
         if (op == OP_FIN) {
+            // Only blocks left with every node unreachable come here; every
+            // other tile wrote its labels in the last (exact) relabel round.
             if (!have) continue;
-            int64_t sx = 0;
-            const int X = tx * TW + lane;
-            int dd[TH], ex[TH];
+            int dd[TH];
 #pragma unroll
-            for (int y = 0; y < TH; ++y) {
-                dd[y] = T.d[y * TW + lane];
-                ex[y] = T.e[y * TW + lane];
-            }
-#pragma unroll
-            for (int y = 0; y < TH; ++y) {
-                const int Y = ty * TH + y;
-                if (X < D.W && Y < D.H) {
-                    const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
-                    D.label_out[(size_t)Y * D.W + X] = (in && dd[y] >= DINF) ? 1 : 0;
-                    if (in) sx += max(ex[y], 0);
-                }
-            }
-            sx = wave_sum64_w(sx);
-            if (lane == 0) p.tsum[2 * (size_t)t + 1] = sx;
+            for (int y = 0; y < TH; ++y) dd[y] = DINF;
+            write_labels(D, tx, ty, lane, dd);
             continue;
         }
         // (OP_GR_INIT is the workgroup kernels' op; here masks are rebuilt by
         // the push kernel's OP_MASK pass and every relabel starts with REINIT.)
         const bool reinit = op == OP_GR_REINIT;
-        const int capv = p.state[inst].cap;  // start value of non-sink nodes (DINF: exact)
+        const int capv = rstate[inst].cap;  // start value of non-sink nodes (DINF: exact)
         int d[TH];
         Masks m;
         m.clear();
@@ -1054,6 +1070,9 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity) {
                 fin |= d[y] < DINF;
                 nearv |= d[y] < p.gr_cap;
             }
+            // An exact round writes the labels of every tile it visits; the
+            // last visit of the last round leaves the final ones.
+            if (capv >= DINF) write_labels(D, tx, ty, lane, d);
         }
         // Outer boundary changes re-queue the neighbouring blocks.
         const bool chL = __any(lane == 0 && chg != 0) && !inL && have && ti.nbr[0] >= 0;
@@ -1078,17 +1097,19 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity) {
     if (p.profile && lane == 0) atomic_add64(&p.ctl->gr_iters, n_rounds);
 }
 
-__global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
+__global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS) {
     const int qlen = p.ctl->q[parity].qlen_pr;
     if ((int)blockIdx.x >= qlen) return;
     const int lane = threadIdx.x;
     int64_t n_iters = 0;
     for (int q = blockIdx.x; q < qlen; q += gridDim.x) {
-        const int t = p.queue_pr[q];
+        const int t = rq[q];
         const TileRef T = tile_ref(p, t);
-        const TileInfo ti = p.tiles[t];
-        const int dthr = p.desc[ti.inst].n_interior;
-        const int capv = p.state[ti.inst].cap;  // nodes at the cap are frozen until the next relabel
+        const TileInfo ti = rtiles[t];
+        // Descriptor fields and the cap read once, before any store.
+        const int dthr = rdesc[ti.inst].n_interior;
+        const int tile0 = rdesc[ti.inst].tile0, tiles_x = rdesc[ti.inst].tiles_x, tiles_y = rdesc[ti.inst].tiles_y;
+        const int capv = rstate[ti.inst].cap;  // nodes at the cap are frozen until the next relabel
         int hU, hD, hs;
         load_halo<HALO_DIST>(p, ti, t, lane, hU, hD, hs);
         int4 c[TH];
@@ -1190,6 +1211,7 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
             if (!__any(act)) break;
         }
         int mine = 0;
+        int64_t sx = 0;  // sum of max(e, 0) over the tile (the flow identity's stuck excess)
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
             T.c[y * TW + lane] = c[y];
@@ -1197,6 +1219,7 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
             T.d[y * TW + lane] = d[y];
             T.m[y * TW + lane] = (uint8_t)node_mask(c[y], e[y]);
             mine |= e[y] > 0 && d[y] < capv;
+            sx += max(e[y], 0);
         }
         // Flow pushed into halo nodes (their tiles do not run in this colour
         // pass; int32 atomics because two same-colour tiles can feed one
@@ -1228,6 +1251,11 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
         if (pushS) mask_after_push_in(p, gS, arcS, oldS + accS);
         if (pushU) mask_after_push_in(p, gU, 8, oldU + accU);
         if (pushD) mask_after_push_in(p, gD, 4, oldD + accD);
+        // Stuck-excess sums of the tiles pushed into (they are not running).
+        if (pushS) atomic_add64(&p.tsum[2 * (size_t)side + 1], (int64_t)max(oldS + accS, 0) - max(oldS, 0));
+        if (pushU) atomic_add64(&p.tsum[2 * (size_t)ti.nbr[2] + 1], (int64_t)max(oldU + accU, 0) - max(oldU, 0));
+        if (pushD) atomic_add64(&p.tsum[2 * (size_t)ti.nbr[3] + 1], (int64_t)max(oldD + accD, 0) - max(oldD, 0));
+        sx = wave_sum64(sx);
         const int toL = pushS && lane < TH, toR = pushS && lane >= TH;
         const bool anyL = __any(toL), anyR = __any(toR);
         const bool anyU = __any(pushU), anyD = __any(pushD);
@@ -1238,17 +1266,18 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity) {
             if (anyU) set_active(p, ti.nbr[2], 1);
             if (anyD) set_active(p, ti.nbr[3], 1);
             set_active(p, t, any_mine);
+            p.tsum[2 * (size_t)t + 1] = sx;
         }
         if (p.gr_cap > 0) {
             // Blocks within cap distance of this tile (and of the tiles it
             // pushed into) need their capped distances recomputed.
-            const InstDesc &D = p.desc[ti.inst];
             const int rx = (p.gr_cap + TW - 1) / TW + ((anyL || anyR) ? 1 : 0);
             const int ry = (p.gr_cap + TH - 1) / TH + ((anyU || anyD) ? 1 : 0);
             const int span = 2 * rx + 1, cnt = span * (2 * ry + 1);
             for (int k = lane; k < cnt; k += TW) {
                 const int x = ti.tx + k % span - rx, y = ti.ty + k / span - ry;
-                if (x >= 0 && y >= 0 && x < D.tiles_x && y < D.tiles_y) p.stale[block_anchor(p, D, x, y)] = 1;
+                if (x >= 0 && y >= 0 && x < tiles_x && y < tiles_y)
+                    p.stale[tile0 + (y - y % p.gby) * tiles_x + (x - x % p.gbx)] = 1;
             }
         }
         n_iters += it;
@@ -1318,7 +1347,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         Masks m;
         m.clear();
         const int capv = p.gr_cap > 0 ? p.gr_cap : DINF;  // the first relabel sweep, capped like k_arm says
-        int64_t sp = 0;
+        int64_t sp = 0, sx = 0;
         int bad = 0;
         // Halo distances from the inputs (issued first, consumed after the rows).
         const int hU = ty > 0 ? input_d0(D, X, Y0 - 1) : DINF;
@@ -1360,6 +1389,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
                     bad |= (vp[k] < 0) | (vn[k] < 0) | (c.x < 0) | (c.y < 0) | (c.z < 0) | (c.w < 0);
                     e = vp[k] - vn[k];
                     sp += vp[k];
+                    sx += max(e, 0);
                 }
                 T.c[y * TW + lane] = c;
                 T.e[y * TW + lane] = e;
@@ -1380,6 +1410,9 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
             fin |= d[y] < DINF;
             nearv |= d[y] < capv;
         }
+        // Uncapped, this sweep belongs to an exact relabel round: labels as
+        // of this sweep (rewritten by any later sweep of the tile).
+        if (capv >= DINF) write_labels(D, tx, ty, lane, d);
         const int rank = __any(fin) ? (__any(nearv) ? 2 : 1) : 0;
         const bool chL = __any(lane == 0 && chg != 0);
         const bool chR = __any(lane == TW - 1 && chg != 0);
@@ -1388,6 +1421,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         const bool any_act = __any(act);
         const bool any_bad = __any(bad);
         sp = wave_sum64(sp);
+        sx = wave_sum64(sx);
         if (lane == 0) {
             // Dirty marks go to the global-relabel blocks of the affected tiles.
             if (chL && tx > 0) mark_dirty(p, block_anchor(p, D, tx - 1, ty));
@@ -1398,6 +1432,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
             set_active(p, t, any_act);
             if (rank) atomicMax(&p.live[block_anchor(p, D, tx, ty)], rank);
             p.tsum[2 * (size_t)t] = sp;
+            p.tsum[2 * (size_t)t + 1] = sx;
             if (any_bad) atomicOr(&p.state[s].err, 1);
         }
     }
@@ -1418,12 +1453,12 @@ __global__ __launch_bounds__(NT) void k_edit(KParams p, int slot, const int32_t 
             if (X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1) {
                 const int t = D.tile0 + (Y / TH) * D.tiles_x + X / TW;
                 const size_t g = (size_t)t * TN + (Y % TH) * TW + (X % TW);
-                if (cls == GCFG_SEED_FG) {
-                    atomicAdd(&p.exc[g], delta);
-                    dp += delta;
-                } else {
-                    atomicAdd(&p.exc[g], -delta);
-                }
+                const int de = cls == GCFG_SEED_FG ? delta : -delta;
+                const int old = atomicAdd(&p.exc[g], de);
+                if (cls == GCFG_SEED_FG) dp += delta;
+                // Stuck-excess sum of the tile (wavefront kernels keep it current).
+                const int64_t dx = (int64_t)max(old + de, 0) - max(old, 0);
+                if (dx) atomic_add64(&p.tsum[2 * (size_t)t + 1], dx);
             }
         }
         const int64_t tot = block_sum64(dp, red);
@@ -1496,13 +1531,16 @@ void launch_sched(const KParams &p, int parity, hipStream_t s) {
 }
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s) {
     if (p.variant)
-        hipLaunchKernelGGL(k_prw, dim3(grid), dim3(64), 0, s, p, parity);
+        hipLaunchKernelGGL(k_prw, dim3(grid), dim3(64), 0, s, p, parity, (const int32_t *)p.queue_pr,
+                           (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
     else
         hipLaunchKernelGGL(k_pr, dim3(grid), dim3(NT), 0, s, p, parity);
 }
 template <int BX, int BY>
 static void launch_grb(const KParams &p, int grid, int parity, hipStream_t s) {
-    hipLaunchKernelGGL((k_grb<BX, BY>), dim3(grid), dim3(64 * BX * BY), 0, s, p, parity);
+    hipLaunchKernelGGL((k_grb<BX, BY>), dim3(grid), dim3(64 * BX * BY), 0, s, p, parity,
+                       (const int32_t *)p.queue_gr, (const TileInfo *)p.tiles, (const InstDesc *)p.desc,
+                       (const InstState *)p.state);
 }
 
 bool gr_block_supported(int bx, int by) {

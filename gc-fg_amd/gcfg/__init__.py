@@ -18,7 +18,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgcfg.so")
+# GCFG_LIBRARY selects another build of the library (same ABI).
+LIB_PATH = os.environ.get("GCFG_LIBRARY") or os.path.join(_HERE, "libgcfg.so")
 
 GCFG_OK = 0
 GCFG_ERR_INVALID = -1
