@@ -44,7 +44,7 @@ HBM_PEAK_GBS = 8000.0
 PR_BYTES_PER_NODE = 49      # caps 16 + excess 4 + distance 4 read and written, mask 1 written
 GR_INIT_BYTES_PER_NODE = 5  # initial BFS sweep from the kept mask: mask 1 read, distance 4 written
 GR_RELAX_BYTES_PER_NODE = 9  # relaxation sweep: mask 1 + distance 4 read, distance 4 written
-FIN_BYTES_PER_NODE = 12     # labels: distance 4 + excess 4 read, label 4 written
+LABEL_BYTES_PER_NODE = 4    # label written by every sweep of an exact round and by FIN
 COMPULSORY_BYTES_PER_PX = 28  # 6 int32 inputs read + int32 label written
 
 
@@ -458,8 +458,10 @@ def main():
     flows_p, _, pst = pbatch.solve(labels_to_host=False)
     n_tiles = sum(((it.W + 63) // 64) * ((it.H + 15) // 16) for it in insts)
     pr_bytes = pst.pr_tiles * 1024 * PR_BYTES_PER_NODE
-    gr_bytes = 1024 * (pst.gr_init_tiles * GR_INIT_BYTES_PER_NODE +
-                       (pst.gr_tiles - pst.gr_init_tiles) * GR_RELAX_BYTES_PER_NODE + n_tiles * FIN_BYTES_PER_NODE)
+    per_item = 1024 * max(1, opts.gr_block_w) * max(1, opts.gr_block_h)  # nodes of one relabel item
+    gr_bytes = per_item * (pst.gr_init_tiles * GR_INIT_BYTES_PER_NODE +
+                       (pst.gr_tiles - pst.gr_init_tiles) * GR_RELAX_BYTES_PER_NODE +
+                       (pst.gr_exact_tiles + pst.fin_items) * LABEL_BYTES_PER_NODE)
     wave = opts.kernels != 0
     kern = {("k_prw" if wave else "k_pr"): (pst.pr_kernel_ms, pr_bytes),
             ("k_grb" if wave else "k_gr"): (pst.gr_kernel_ms, gr_bytes)}
@@ -476,7 +478,8 @@ def main():
                     solve_alg_frac=round(solve_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     compulsory_frac=round(COMPULSORY_BYTES_PER_PX * total_px / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                     pr_tiles=int(pst.pr_tiles), gr_tiles=int(pst.gr_tiles),
-                    gr_init_tiles=int(pst.gr_init_tiles), fin_tiles=int(n_tiles))
+                    gr_init_tiles=int(pst.gr_init_tiles), gr_exact_tiles=int(pst.gr_exact_tiles),
+                    fin_items=int(pst.fin_items), arena_tiles=int(n_tiles))
     assert np.array_equal(flows_p, flows_b), "profiled solve differs"
     pbatch.close()
     if not args.no_pmc:

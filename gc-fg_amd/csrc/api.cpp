@@ -212,6 +212,8 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
         stats->pr_tiles = st.pr_tiles;
         stats->gr_tiles = st.gr_tiles;
         stats->gr_init_tiles = st.gr_init_tiles;
+        stats->gr_exact_tiles = st.gr_exact_tiles;
+        stats->fin_items = st.fin_items;
         stats->pr_iters = ctl.pr_iters;
         stats->gr_iters = ctl.gr_iters;
         stats->steps = st.steps;
@@ -366,6 +368,8 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
         agg.pr_tiles += st.pr_tiles;
         agg.gr_tiles += st.gr_tiles;
         agg.gr_init_tiles += st.gr_init_tiles;
+        agg.gr_exact_tiles += st.gr_exact_tiles;
+        agg.fin_items += st.fin_items;
         agg.steps = st.steps > agg.steps ? st.steps : agg.steps;
         agg.gr_rounds += st.gr_rounds;
     }

@@ -75,6 +75,8 @@ struct InstState {
     int64_t gr_init_tiles; // of which initial sweeps (INIT / REINIT)
     int32_t cap;           // distance cap of the current global relabel (DINF: exact)
     int32_t pad_;
+    int64_t gr_exact_tiles;  // relabel items of exact rounds (those write labels)
+    int64_t fin_items;       // FIN items (blocks left unreachable)
 };
 
 // Queue counters of one step. Steps alternate between two sets: the

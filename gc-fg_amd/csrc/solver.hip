@@ -266,6 +266,7 @@ __device__ void start_global_relabel(const KParams &p, const InstDesc &D, InstSt
     if (threadIdx.x == 0) {
         st.gr_tiles += n;
         st.gr_init_tiles += n;
+        if (cap >= DINF) st.gr_exact_tiles += n;
     }
 }
 
@@ -309,7 +310,10 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
     if (phase == PH_GR) {
         const int n = emit_tiles(p, D, EM_DIRTY, 0, false, OP_GR_RELAX, parity);
         if (n > 0) {
-            if (threadIdx.x == 0) st.gr_tiles += n;
+            if (threadIdx.x == 0) {
+                st.gr_tiles += n;
+                if (cur_cap >= DINF) st.gr_exact_tiles += n;
+            }
             return;
         }
         // The BFS has converged: finish, or push from the active tiles.
@@ -321,7 +325,8 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
             if (threadIdx.x == 0) st.phase = PH_FIN;
             // Wavefront kernels: the exact round wrote the labels of every
             // block it visited; FIN labels the blocks it skipped as unreachable.
-            emit_tiles(p, D, p.variant ? EM_DEAD : EM_ALL, 0, false, OP_FIN, parity);
+            const int nf = emit_tiles(p, D, p.variant ? EM_DEAD : EM_ALL, 0, false, OP_FIN, parity);
+            if (threadIdx.x == 0) st.fin_items += nf;
             return;
         }
         if (threadIdx.x == 0) st.phase = PH_PR;
