@@ -36,6 +36,7 @@ constexpr int TN = TW * TH;             // nodes per tile
 constexpr int NT = 256;                 // threads per workgroup (4 wavefronts)
 constexpr int RPT = TN / NT;            // rows per thread (vertical strip)
 constexpr int DINF = 0x3fffffff;        // distance of nodes that cannot reach the sink
+constexpr int NUM_XCD = 8;              // MI355X: 8 accelerator complex dies, one L2 each
 constexpr int LW = TW + 2;              // LDS row pitch with halo
 constexpr int LH = TH + 2;
 

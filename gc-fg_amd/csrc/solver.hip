@@ -930,6 +930,108 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, int hU, 
     return false;
 }
 
+// One bit per row of a lane's column (bit y = row y) for the mask bit `b` of
+// every row: the four rows packed in a mask word sit at bits 0, 8, 16, 24;
+// multiplying by 2^21 + 2^14 + 2^7 + 1 gathers them into bits 21..24
+// without carries (all sixteen partial-product bits are distinct).
+__device__ __forceinline__ uint32_t row_plane(const Masks &m, int b) {
+    uint32_t plane = 0;
+#pragma unroll
+    for (int i = 0; i < TH / 4; ++i) {
+        const uint32_t x = (m.w[i] >> b) & 0x01010101u;
+        plane |= (((x * 0x00204081u) >> 21) & 0xfu) << (4 * i);
+    }
+    return plane;
+}
+
+// Capped BFS of one tile by level sets, 16 rows as bits of one register per
+// lane: L_0 = sink nodes; L_(k+1) = L_k plus every node with a residual arc
+// into L_k or into a halo node whose value is <= k. Then d = the first level
+// holding the node, or K if none of L_0..L_(K-1) does; that is min(K, BFS
+// distance) given the halo, the fixed point the relaxation reaches, in about
+// 25 vector operations per level. Level counts per row are kept bit-sliced
+// (NB bits, K < 2^NB). chg collects rows whose value differs from d on entry.
+template <int NB>
+__device__ __forceinline__ void bfs_levels(int (&d)[TH], const Masks &m, int hU, int hD, int hs, int lane, int K,
+                                           uint32_t &chg, int &levels) {
+    const uint32_t PL = row_plane(m, 0), PR = row_plane(m, 1), PU = row_plane(m, 2), PD = row_plane(m, 3);
+    uint32_t S = row_plane(m, 5);
+    uint32_t c[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) c[b] = 0;
+    auto count = [&](uint32_t v) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const uint32_t t = c[b] & v;
+            c[b] ^= v;
+            v = t;
+        }
+    };
+    count(S);
+    // Largest halo value that can still inject a level (< K - 1); past it and
+    // with no growth, every further level equals the last one.
+    int hm = -1;
+    if (hU < K - 1) hm = max(hm, hU);
+    if (hD < K - 1) hm = max(hm, hD);
+    if (lane < 2 * TH && hs < K - 1) hm = max(hm, hs);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) hm = max(hm, __shfl_xor(hm, o, 64));
+    int j = 0;  // index of the last level computed
+    for (int k = 0; k < K - 1; ++k) {
+        const uint32_t hu = hU <= k ? 1u : 0u;
+        const uint32_t hd = hD <= k ? (1u << (TH - 1)) : 0u;
+        const uint64_t bl = __ballot(lane < TH && hs <= k);
+        const uint64_t br = __ballot(lane >= TH && lane < 2 * TH && hs <= k);
+        const int lm = (int)((uint32_t)bl & 0xffffu), rm = (int)((uint32_t)(br >> TH) & 0xffffu);
+        // lane x <- lane x-1 (lane 0: the left halo column), lane x <- lane x+1 (lane 63: right halo)
+        const uint32_t lf = (uint32_t)__builtin_amdgcn_update_dpp(lm, (int)S, 0x138, 0xF, 0xF, false);
+        const uint32_t rt = (uint32_t)__builtin_amdgcn_update_dpp(rm, (int)S, 0x130, 0xF, 0xF, false);
+        const uint32_t Sn = (S | (((S << 1) | hu) & PU) | (((S >> 1) | hd) & PD) | (lf & PL) | (rt & PR)) & 0xffffu;
+        const bool grew = __any(Sn != S);
+        S = Sn;
+        count(S);
+        j = k + 1;
+        if (!grew && k >= hm) break;
+    }
+    levels = j + 1;
+    uint32_t rch = 0;
+#pragma unroll
+    for (int y = 0; y < TH; ++y) {
+        int cnt = 0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) cnt |= (int)((c[b] >> y) & 1u) << b;
+        const int nd = ((S >> y) & 1u) ? j + 1 - cnt : K;
+        if (nd != d[y]) {
+            d[y] = nd;
+            rch |= 1u << y;
+        }
+    }
+    chg |= rch;
+}
+
+// Capped tile BFS for caps up to 31 by level sets; larger caps and exact
+// rounds use the relaxation. Returns true when the tile converged.
+__device__ __forceinline__ bool tile_bfs(int (&d)[TH], Masks &m, int hU, int hD, int hs, int lane, int capv,
+                                         uint32_t &chg, int &rounds, int max_rounds) {
+    if (capv <= 3) {
+        bfs_levels<2>(d, m, hU, hD, hs, lane, capv, chg, rounds);
+        return true;
+    }
+    if (capv <= 7) {
+        bfs_levels<3>(d, m, hU, hD, hs, lane, capv, chg, rounds);
+        return true;
+    }
+    if (capv <= 15) {
+        bfs_levels<4>(d, m, hU, hD, hs, lane, capv, chg, rounds);
+        return true;
+    }
+    if (capv <= 31) {
+        bfs_levels<5>(d, m, hU, hD, hs, lane, capv, chg, rounds);
+        return true;
+    }
+    return relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, max_rounds);
+}
+
 // Labels of one tile from converged exact distances: 1 for interior nodes
 // that cannot reach the sink (SPREAD_ZEROS set, GraphCutKernels.cu:228-351),
 // 0 elsewhere, border included.
@@ -978,7 +1080,13 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCF
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wi = w % BX, wj = w / BX;
     int64_t n_rounds = 0;
-    for (int q = blockIdx.x; q < qlen; q += gridDim.x) {
+    // Each workgroup takes a contiguous run of the queue: consecutive items
+    // are usually neighbouring tiles of one instance, so their queue words,
+    // tile records and descriptor hit the scalar cache, and their halo rows
+    // are each other's tile rows.
+    const int per = (qlen + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int q_end = min(qlen, ((int)blockIdx.x + 1) * per);
+    for (int q = (int)blockIdx.x * per; q < q_end; ++q) {
         const int item = rq[q];
         const int op = item & 7, t0 = item >> 3;
         const int inst = rtiles[t0].inst;
@@ -1035,7 +1143,7 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCF
         for (int xr = 0;; ++xr) {
             int rounds = 0;
             bool conv = true;
-            if (need) conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, p.gr_rounds);
+            if (need) conv = tile_bfs(d, m, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds);
             n_rounds += rounds;
             if (NW == 1) {
                 capped = !conv;
@@ -1107,7 +1215,13 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
     if ((int)blockIdx.x >= qlen) return;
     const int lane = threadIdx.x;
     int64_t n_iters = 0;
-    for (int q = blockIdx.x; q < qlen; q += gridDim.x) {
+    // Each workgroup takes a contiguous run of the queue: consecutive items
+    // are usually neighbouring tiles of one instance, so their queue words,
+    // tile records and descriptor hit the scalar cache, and their halo rows
+    // are each other's tile rows.
+    const int per = (qlen + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int q_end = min(qlen, ((int)blockIdx.x + 1) * per);
+    for (int q = (int)blockIdx.x * per; q < q_end; ++q) {
         const int t = rq[q];
         const TileRef T = tile_ref(p, t);
         const TileInfo ti = rtiles[t];
@@ -1339,7 +1453,18 @@ __device__ __forceinline__ int input_d0(const InstDesc &D, int X, int Y) {
 __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, const int32_t *prefix, int n,
                                                int total) {
     const int lane = threadIdx.x;
-    for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.
+    // Giving every XCD a contiguous eighth of the tiles keeps horizontally
+    // and vertically adjacent tiles, which share input cache lines (rows of
+    // the caller's row-major planes are not 256-byte aligned), in one L2.
+    const bool by_xcd = gridDim.x % NUM_XCD == 0;
+    const int per = (total + NUM_XCD - 1) / NUM_XCD;
+    const int jstep = by_xcd ? gridDim.x / NUM_XCD : gridDim.x;
+    const int jend = by_xcd ? per : total;
+    const int b0 = by_xcd ? (int)(blockIdx.x % NUM_XCD) * per : 0;
+    for (int jj = by_xcd ? blockIdx.x / NUM_XCD : blockIdx.x; jj < jend; jj += jstep) {
+        const int b = b0 + jj;
+        if (b >= total) break;
         const int j = find_inst(prefix, n, b);
         const int s = slots[j];
         const InstDesc D = p.desc[s];
@@ -1406,7 +1531,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         }
         uint32_t chg = 0;
         int rounds = 0;
-        const bool conv = relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, p.gr_rounds);
+        const bool conv = tile_bfs(d, m, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds);
         int act = 0, fin = 0, nearv = 0;
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
@@ -1523,7 +1648,8 @@ void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_til
     if (p.variant) {
         hipLaunchKernelGGL(k_clear_tiles, dim3((total_tiles + 255) / 256), dim3(256), 0, s, p, d_slots, d_slots + n, n,
                            total_tiles);
-        const int grid = total_tiles < 16384 ? total_tiles : 16384;
+        int grid = total_tiles < 16384 ? total_tiles : 16384;
+        if (grid >= NUM_XCD) grid -= grid % NUM_XCD;  // whole rounds over the XCDs
         hipLaunchKernelGGL(k_buildw, dim3(grid), dim3(64), 0, s, p, d_slots, d_slots + n, n, total_tiles);
         return;
     }
