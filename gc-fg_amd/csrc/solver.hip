@@ -19,6 +19,8 @@
 //    labels (interior and unreachable, the SPREAD_ZEROS set of :228-351)
 //    and the int64 stuck-excess sum for flow = sum pos - sum max(e, 0).
 //  * k_sched: per-instance state machine and ballot-compacted tile queues.
+#include <type_traits>
+
 #include "engine.hpp"
 
 namespace gcfg {
@@ -725,7 +727,40 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p, int parity) {
 __device__ __forceinline__ int shr1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xF, 0xF, false); }
 __device__ __forceinline__ int shl1(int v) { return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xF, 0xF, false); }
 
-__device__ __forceinline__ int64_t wave_sum64_w(int64_t v) { return wave_sum64(v); }
+// 64-bit wavefront sum with DPP moves only (no LDS permutes): Hillis-Steele
+// within 16-lane rows, then row_bcast:15 / row_bcast:31; lane 63 ends with
+// the total, which every lane reads back.
+__device__ __forceinline__ int64_t wave_sum64_dpp(int64_t v) {
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)((uint64_t)v >> 32);
+    auto step = [&](auto ctrl_tag, int rows) {
+        constexpr int ctrl = decltype(ctrl_tag)::value;
+        uint32_t plo, phi;
+        if (rows == 0xF) {
+            plo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, ctrl, 0xF, 0xF, false);
+            phi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, ctrl, 0xF, 0xF, false);
+        } else if (rows == 0xA) {
+            plo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, ctrl, 0xA, 0xF, false);
+            phi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, ctrl, 0xA, 0xF, false);
+        } else {
+            plo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)lo, ctrl, 0xC, 0xF, false);
+            phi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)hi, ctrl, 0xC, 0xF, false);
+        }
+        const uint64_t sum = ((uint64_t)hi << 32 | lo) + ((uint64_t)phi << 32 | plo);
+        lo = (uint32_t)sum;
+        hi = (uint32_t)(sum >> 32);
+    };
+    step(std::integral_constant<int, 0x111>{}, 0xF);
+    step(std::integral_constant<int, 0x112>{}, 0xF);
+    step(std::integral_constant<int, 0x114>{}, 0xF);
+    step(std::integral_constant<int, 0x118>{}, 0xF);
+    step(std::integral_constant<int, 0x142>{}, 0xA);
+    step(std::integral_constant<int, 0x143>{}, 0xC);
+    const uint32_t tlo = (uint32_t)__builtin_amdgcn_readlane((int)lo, 63);
+    const uint32_t thi = (uint32_t)__builtin_amdgcn_readlane((int)hi, 63);
+    return (int64_t)((uint64_t)thi << 32 | tlo);
+}
+
+__device__ __forceinline__ int64_t wave_sum64_w(int64_t v) { return wave_sum64_dpp(v); }
 
 // Halo distances: rows of the tiles above/below (one per lane) and the
 // columns of the tiles left/right (lanes 0..15 hold the left column, lanes
@@ -968,14 +1003,6 @@ __device__ __forceinline__ void bfs_levels(int (&d)[TH], const Masks &m, int hU,
         }
     };
     count(S);
-    // Largest halo value that can still inject a level (< K - 1); past it and
-    // with no growth, every further level equals the last one.
-    int hm = -1;
-    if (hU < K - 1) hm = max(hm, hU);
-    if (hD < K - 1) hm = max(hm, hD);
-    if (lane < 2 * TH && hs < K - 1) hm = max(hm, hs);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) hm = max(hm, __shfl_xor(hm, o, 64));
     int j = 0;  // index of the last level computed
     for (int k = 0; k < K - 1; ++k) {
         const uint32_t hu = hU <= k ? 1u : 0u;
@@ -987,11 +1014,9 @@ __device__ __forceinline__ void bfs_levels(int (&d)[TH], const Masks &m, int hU,
         const uint32_t lf = (uint32_t)__builtin_amdgcn_update_dpp(lm, (int)S, 0x138, 0xF, 0xF, false);
         const uint32_t rt = (uint32_t)__builtin_amdgcn_update_dpp(rm, (int)S, 0x130, 0xF, 0xF, false);
         const uint32_t Sn = (S | (((S << 1) | hu) & PU) | (((S >> 1) | hd) & PD) | (lf & PL) | (rt & PR)) & 0xffffu;
-        const bool grew = __any(Sn != S);
         S = Sn;
         count(S);
         j = k + 1;
-        if (!grew && k >= hm) break;
     }
     levels = j + 1;
     uint32_t rch = 0;
@@ -1009,7 +1034,7 @@ __device__ __forceinline__ void bfs_levels(int (&d)[TH], const Masks &m, int hU,
     chg |= rch;
 }
 
-// Capped tile BFS for caps up to 31 by level sets; larger caps and exact
+// Capped tile BFS for caps up to 15 by level sets; larger caps and exact
 // rounds use the relaxation. Returns true when the tile converged.
 __device__ __forceinline__ bool tile_bfs(int (&d)[TH], Masks &m, int hU, int hD, int hs, int lane, int capv,
                                          uint32_t &chg, int &rounds, int max_rounds) {
@@ -1023,10 +1048,6 @@ __device__ __forceinline__ bool tile_bfs(int (&d)[TH], Masks &m, int hU, int hD,
     }
     if (capv <= 15) {
         bfs_levels<4>(d, m, hU, hD, hs, lane, capv, chg, rounds);
-        return true;
-    }
-    if (capv <= 31) {
-        bfs_levels<5>(d, m, hU, hD, hs, lane, capv, chg, rounds);
         return true;
     }
     return relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, max_rounds);
@@ -1050,6 +1071,7 @@ __device__ __forceinline__ void write_labels(const InstDesc &D, int tx, int ty, 
 
 // Anchor (top-left tile) of the global-relabel block holding tile (x, y).
 __device__ __forceinline__ int block_anchor(const KParams &p, const InstDesc &D, int x, int y) {
+    if (p.gbx == 1 && p.gby == 1) return D.tile0 + y * D.tiles_x + x;  // the common shape: no divisions
     return D.tile0 + (y - y % p.gby) * D.tiles_x + (x - x % p.gbx);
 }
 
@@ -1194,10 +1216,12 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCF
         const bool chD = __any((chg >> (TH - 1)) & 1u) && !inD && have && ti.nbr[3] >= 0;
         const bool any_act = __any(act);
         if (lane == 0 && have) {
-            if (chL) mark_dirty(p, block_anchor(p, D, tx - 1, ty));
-            if (chR) mark_dirty(p, block_anchor(p, D, tx + 1, ty));
-            if (chU) mark_dirty(p, block_anchor(p, D, tx, ty - 1));
-            if (chD) mark_dirty(p, block_anchor(p, D, tx, ty + 1));
+            // Block anchors from the compile-time block shape (no divisions).
+            auto anchor = [&](int x, int y) { return D.tile0 + (y - y % BY) * D.tiles_x + (x - x % BX); };
+            if (chL) mark_dirty(p, anchor(tx - 1, ty));
+            if (chR) mark_dirty(p, anchor(tx + 1, ty));
+            if (chU) mark_dirty(p, anchor(tx, ty - 1));
+            if (chD) mark_dirty(p, anchor(tx, ty + 1));
             set_active(p, t, any_act);
         }
         const bool any_fin = NW == 1 ? __any(fin) : __syncthreads_or(fin);
@@ -1374,7 +1398,7 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         if (pushS) atomic_add64(&p.tsum[2 * (size_t)side + 1], (int64_t)max(oldS + accS, 0) - max(oldS, 0));
         if (pushU) atomic_add64(&p.tsum[2 * (size_t)ti.nbr[2] + 1], (int64_t)max(oldU + accU, 0) - max(oldU, 0));
         if (pushD) atomic_add64(&p.tsum[2 * (size_t)ti.nbr[3] + 1], (int64_t)max(oldD + accD, 0) - max(oldD, 0));
-        sx = wave_sum64(sx);
+        sx = wave_sum64_dpp(sx);
         const int toL = pushS && lane < TH, toR = pushS && lane >= TH;
         const bool anyL = __any(toL), anyR = __any(toR);
         const bool anyU = __any(pushU), anyD = __any(pushD);
@@ -1396,7 +1420,8 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             for (int k = lane; k < cnt; k += TW) {
                 const int x = ti.tx + k % span - rx, y = ti.ty + k / span - ry;
                 if (x >= 0 && y >= 0 && x < tiles_x && y < tiles_y)
-                    p.stale[tile0 + (y - y % p.gby) * tiles_x + (x - x % p.gbx)] = 1;
+                    p.stale[(p.gbx == 1 && p.gby == 1) ? tile0 + y * tiles_x + x
+                                                       : tile0 + (y - y % p.gby) * tiles_x + (x - x % p.gbx)] = 1;
             }
         }
         n_iters += it;
@@ -1550,8 +1575,8 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         const bool chD = __any((chg >> (TH - 1)) & 1u);
         const bool any_act = __any(act);
         const bool any_bad = __any(bad);
-        sp = wave_sum64(sp);
-        sx = wave_sum64(sx);
+        sp = wave_sum64_dpp(sp);
+        sx = wave_sum64_dpp(sx);
         if (lane == 0) {
             // Dirty marks go to the global-relabel blocks of the affected tiles.
             if (chL && tx > 0) mark_dirty(p, block_anchor(p, D, tx - 1, ty));
