@@ -359,6 +359,8 @@ def main():
     ap.add_argument("--poll", type=int, default=0)
     ap.add_argument("--gr-block", default=None, help="global-relabel block WxH in tiles, e.g. 1x4")
     ap.add_argument("--gr-cap", type=int, default=None, help="distance cap of intermediate global relabels")
+    ap.add_argument("--gr-waves", type=int, default=None, help="relabel launch wavefronts per CU")
+    ap.add_argument("--pr-waves", type=int, default=None, help="push/relabel launch wavefronts per CU")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs for roofline.traffic")
     ap.add_argument("--tune-cache", default=os.environ.get("GCFG_TUNE_CACHE",
@@ -400,6 +402,10 @@ def main():
         opts.gr_block_w, opts.gr_block_h = (int(v) for v in args.gr_block.split("x"))
     if args.gr_cap is not None:
         opts.gr_cap = args.gr_cap
+    if args.gr_waves:
+        opts.gr_waves_per_cu = args.gr_waves
+    if args.pr_waves:
+        opts.pr_waves_per_cu = args.pr_waves
 
     log(f"generating {args.instances} instances + device data terms")
     insts, dt_ms = make_batch(args.instances, args.seed)

@@ -54,6 +54,8 @@ void gcfg_default_options(gcfg_options *o) {
     o->gr_block_w = 1;
     o->gr_block_h = 1;
     o->gr_cap = 0;
+    o->pr_waves_per_cu = 0;
+    o->gr_waves_per_cu = 0;
 }
 
 const char *gcfg_last_error(void) { return gcfg::last_error_cstr(); }

@@ -132,6 +132,10 @@ bool options_valid(const gcfg_options &o) {
         set_error("options: sweeps_per_gr, inner_iters and steps_per_poll must be positive");
         return false;
     }
+    if (o.pr_waves_per_cu < 0 || o.gr_waves_per_cu < 0 || o.pr_waves_per_cu > 1024 || o.gr_waves_per_cu > 1024) {
+        set_error("options: pr_waves_per_cu and gr_waves_per_cu must be in [0, 1024]");
+        return false;
+    }
     if (o.gr_cap < 0) {
         set_error("options: gr_cap must be >= 0");
         return false;
@@ -150,8 +154,10 @@ void Engine::apply_options() {
     const int variant = opt.kernels ? 1 : 0;
     int gbx = opt.gr_block_w > 0 ? opt.gr_block_w : 1, gby = opt.gr_block_h > 0 ? opt.gr_block_h : 1;
     if (!variant || !gr_block_supported(gbx, gby)) gbx = gby = 1;
-    const int gpr = variant ? cu * 16 : cu * 4;
-    const int ggr = variant ? std::max(cu, cu * 28 / (gbx * gby)) : cu * 8;
+    const int wpr = opt.pr_waves_per_cu > 0 ? opt.pr_waves_per_cu : 16;
+    const int wgr = opt.gr_waves_per_cu > 0 ? opt.gr_waves_per_cu : 28;
+    const int gpr = variant ? cu * wpr : cu * 4;
+    const int ggr = variant ? std::max(cu, cu * wgr / (gbx * gby)) : cu * 8;
     const int grr = opt.gr_rounds > 0 ? opt.gr_rounds : 48;
     const int prof = opt.profile ? 1 : 0;
     const int cap = variant && opt.gr_cap > 0 ? opt.gr_cap : 0;

@@ -48,6 +48,8 @@ int coordinate_search(gcfg_options start, int trials, const std::function<double
         {&gcfg_options::gr_rounds, {2, 4, 8, 48}},
         {&gcfg_options::steps_per_poll, {4, 8, 16, 32}},
         {&gcfg_options::gr_cap, {0, 4, 8, 16, 32, 64}},
+        {&gcfg_options::gr_waves_per_cu, {8, 16, 28, 56}},
+        {&gcfg_options::pr_waves_per_cu, {8, 16, 32}},
     };
     // Global-relabel block shapes (width, height in tiles) are one knob.
     const std::vector<std::pair<int32_t, int32_t>> shapes = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 16},
@@ -94,12 +96,14 @@ extern "C" {
 
 static const char *kFields[] = {"sweeps_per_gr", "inner_iters", "steps_per_poll", "use_graph",
                                 "max_steps",     "verbose",     "profile",        "kernels",
-                                "gr_rounds",     "gr_block_w",  "gr_block_h",     "gr_cap"};
+                                "gr_rounds",     "gr_block_w",  "gr_block_h",     "gr_cap",
+                                "pr_waves_per_cu", "gr_waves_per_cu"};
 
 static std::vector<int32_t *> option_fields(gcfg_options *o) {
     return {&o->sweeps_per_gr, &o->inner_iters, &o->steps_per_poll, &o->use_graph,
             &o->max_steps,     &o->verbose,     &o->profile,        &o->kernels,
-            &o->gr_rounds,     &o->gr_block_w,  &o->gr_block_h,     &o->gr_cap};
+            &o->gr_rounds,     &o->gr_block_w,  &o->gr_block_h,     &o->gr_cap,
+            &o->pr_waves_per_cu, &o->gr_waves_per_cu};
 }
 
 int gcfg_tune_store(const char *path, const char *key, const gcfg_options *opt) {

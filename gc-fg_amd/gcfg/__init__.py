@@ -46,7 +46,8 @@ class Options(ctypes.Structure):
                 ("steps_per_poll", ctypes.c_int32), ("use_graph", ctypes.c_int32),
                 ("max_steps", ctypes.c_int32), ("verbose", ctypes.c_int32), ("profile", ctypes.c_int32),
                 ("kernels", ctypes.c_int32), ("gr_rounds", ctypes.c_int32),
-                ("gr_block_w", ctypes.c_int32), ("gr_block_h", ctypes.c_int32), ("gr_cap", ctypes.c_int32)]
+                ("gr_block_w", ctypes.c_int32), ("gr_block_h", ctypes.c_int32), ("gr_cap", ctypes.c_int32),
+                ("pr_waves_per_cu", ctypes.c_int32), ("gr_waves_per_cu", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):
@@ -111,6 +112,44 @@ def _check(rc: int, what: str):
 
 def now_ms() -> float:
     return float(lib().gcfg_now_ms())
+
+
+def mrf_load(path: str):
+    """Middlebury 2-label MRF file -> (W, H, pos, neg, up, down, left, right) int32 [H, W] arrays."""
+    W, H, L = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().gcfg_mrf_info(path.encode(), ctypes.byref(W), ctypes.byref(H), ctypes.byref(L)), "gcfg_mrf_info")
+    arrs = [np.empty((H.value, W.value), np.int32) for _ in range(6)]
+    _check(lib().gcfg_mrf_load(path.encode(), W.value, H.value, *[_VP(a.ctypes.data) for a in arrs]), "gcfg_mrf_load")
+    return (W.value, H.value, *arrs)
+
+
+def mrf_write(path: str, pos, neg, hcue, vcue, mult: int = 1):
+    arrs = [np.ascontiguousarray(a, np.int32) for a in (pos, neg, hcue, vcue)]
+    H, W = arrs[0].shape
+    _check(lib().gcfg_mrf_write(path.encode(), W, H, int(mult), *[_VP(a.ctypes.data) for a in arrs]),
+           "gcfg_mrf_write")
+
+
+def read_ppm(path: str) -> np.ndarray:
+    W, H = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().gcfg_ppm_info(path.encode(), ctypes.byref(W), ctypes.byref(H)), "gcfg_ppm_info")
+    img = np.empty((H.value, W.value, 3), np.uint8)
+    _check(lib().gcfg_ppm_read(path.encode(), W.value, H.value, _VP(img.ctypes.data)), "gcfg_ppm_read")
+    return img
+
+
+def write_label_ppm(path: str, label, rgb=None):
+    lab = np.ascontiguousarray(label, np.int32)
+    H, W = lab.shape
+    img = None if rgb is None else np.ascontiguousarray(rgb, np.uint8)
+    _check(lib().gcfg_write_label_ppm(path.encode(), W, H, _VP(lab.ctypes.data),
+                                      _VP(img.ctypes.data) if img is not None else None), "gcfg_write_label_ppm")
+
+
+def write_label_pgm(path: str, label):
+    lab = np.ascontiguousarray(label, np.int32)
+    H, W = lab.shape
+    _check(lib().gcfg_write_label_pgm(path.encode(), W, H, _VP(lab.ctypes.data)), "gcfg_write_label_pgm")
 
 
 def tune_store(path: str, key: str, options: Options):

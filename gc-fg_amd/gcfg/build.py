@@ -37,6 +37,7 @@ SOURCES = [
     ("server.cpp", "host"),
     ("compat.cpp", "host"),
     ("tune.cpp", "host"),
+    ("mrfio.cpp", "host"),
 ]
 
 

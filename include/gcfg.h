@@ -50,6 +50,8 @@ typedef struct gcfg_options {
     int32_t gr_cap;          /* >0: intermediate global relabels stop BFS fronts at this distance (nodes
                                 beyond are frozen until the next relabel; the last relabel is exact);
                                 0: every relabel exact */
+    int32_t pr_waves_per_cu; /* push/relabel launch size in wavefronts per compute unit (0 = 16) */
+    int32_t gr_waves_per_cu; /* global-relabel launch size in wavefronts per compute unit (0 = 28) */
 } gcfg_options;
 
 typedef struct gcfg_stats {
@@ -159,6 +161,28 @@ int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_sta
 /* Cold solve with labels written to caller-owned device buffers (d_labels[i]: W_i*H_i). */
 int gcfg_batch_solve_device(gcfg_batch *b, int32_t **d_labels, int64_t *flows, gcfg_stats *stats);
 int gcfg_batch_destroy(gcfg_batch *b);
+
+/* ---- File formats of the reference's drivers (host only) ----
+ * Middlebury 2-label MRF text files: gcfg_mrf_info reads the header
+ * (solveMRF.cu:16); gcfg_mrf_load fills caller arrays of W*H int32 with the
+ * data terms and the cue planes mapped as the reference does
+ * (solveMRF.cu:10-65, 87-90: up = down = vertical cue, left = right =
+ * horizontal cue, last row of both cues zeroed); gcfg_mrf_write writes one,
+ * repeated `mult` times horizontally (multiplyMRF.cpp:64-92). */
+int gcfg_mrf_info(const char *path, int32_t *width, int32_t *height, int32_t *n_labels);
+int gcfg_mrf_load(const char *path, int32_t width, int32_t height, int32_t *pos, int32_t *neg, int32_t *up,
+                  int32_t *down, int32_t *left, int32_t *right);
+int gcfg_mrf_write(const char *path, int32_t width, int32_t height, int32_t mult, const int32_t *pos,
+                   const int32_t *neg, const int32_t *hcue, const int32_t *vcue);
+/* PPM input images (P3 or P6, maxval 255; Application.cu:55-58) as
+ * interleaved RGB u8. */
+int gcfg_ppm_info(const char *path, int32_t *width, int32_t *height);
+int gcfg_ppm_read(const char *path, int32_t width, int32_t height, uint8_t *rgb);
+/* Label images: text P3 with white (rgb NULL, solveMRF.cu:113-127) or the
+ * input colour (Application.cu:61-75) where label is 1, black elsewhere; and
+ * binary P5 with 255 where label is 1. */
+int gcfg_write_label_ppm(const char *path, int32_t width, int32_t height, const int32_t *label, const uint8_t *rgb);
+int gcfg_write_label_pgm(const char *path, int32_t width, int32_t height, const int32_t *label);
 
 /* ---- Autotuning of the solver options ----
  * Coordinate search over inner_iters, sweeps_per_gr, gr_rounds and

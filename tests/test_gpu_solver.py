@@ -384,3 +384,24 @@ def test_caller_stream_ordering(g, ora):
     assert st2.flow == rflow and np.array_equal(label, rlabel)
     gr.close()
     s.close()
+
+
+def test_middlebury_file_solve(g, ora, tmp_path):
+    """solveMRF.cu's workflow: a 2-label Middlebury file (written here by the
+    multiplyMRF.cpp layout, tiled twice) is loaded with the reference's cue
+    mapping and solved by the drop-in path; the result equals the oracle on
+    the loaded planes, and the label image is written as solveMRF.cu does."""
+    rng = np.random.default_rng(44)
+    H, W = 60, 70
+    pos, neg, hcue, vcue = (rng.integers(0, 200, (H, W)).astype(np.int32) for _ in range(4))
+    path = str(tmp_path / "m.txt")
+    g.mrf_write(path, pos, neg, hcue, vcue, mult=2)
+    W2, H2, p, n, up, down, left, right = g.mrf_load(path)
+    assert (W2, H2) == (2 * W, H)
+    st = _check(g, ora, W2, H2, p, n, edges=(up, down, left, right))
+    flow, label, _ = g.solve(W2, H2, p, n, edges=(up, down, left, right))
+    out = tmp_path / "labelMRF.ppm"
+    g.write_label_ppm(str(out), label)
+    head = out.read_text().split("\n", 1)[0]
+    assert head == f"P3 {W2} {H2} 255 "
+    assert st.steps > 0
