@@ -46,6 +46,7 @@ GR_INIT_BYTES_PER_NODE = 5  # initial BFS sweep from the kept mask: mask 1 read,
 GR_RELAX_BYTES_PER_NODE = 9  # relaxation sweep: mask 1 + distance 4 read, distance 4 written
 LABEL_BYTES_PER_NODE = 4    # label written by every sweep of an exact round and by FIN
 COMPULSORY_BYTES_PER_PX = 28  # 6 int32 inputs read + int32 label written
+CHILD_SOLVES = 3  # solves of the counter child run: --steps 1 --warmup 0, then two event-timed solves
 
 
 def log(msg):
@@ -295,11 +296,15 @@ def single(args, opts):
     print(json.dumps(out), flush=True)
 
 
-def pmc_traffic(args, opts, kernel_prefix):
+def pmc_traffic(args, opts, kernel_prefix, launches_per_solve):
     """HBM bytes per launch of the dominant kernel from two rocprofv3 counter
     runs of this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each
     scaled by what the same counter reads for a 1 GiB device-to-device copy
-    run in the same process. Returns (traffic dict or None, note)."""
+    run in the same process. The child solves the batch CHILD_SOLVES times
+    (one timed solve, two event-timed ones); a first solve may launch a few
+    idle steps whose dispatches exit at once, so bytes per launch are the
+    summed bytes over every dispatch / (solves x launches of one solve).
+    Returns (traffic dict or None, note)."""
     import shutil
     import sqlite3
     import subprocess
@@ -333,7 +338,8 @@ def pmc_traffic(args, opts, kernel_prefix):
         if calib <= 0 or not vals:
             return None, f"no {counter} samples for the copy or {kernel_prefix}"
         scale = (1 << 20) / calib  # the copy moves 1 GiB = 2^20 KiB each way
-        got[counter] = (sum(vals) / len(vals) * 1024 * scale, scale, len(vals))
+        per_launch = sum(vals) / (CHILD_SOLVES * max(1, launches_per_solve))
+        got[counter] = (per_launch * 1024 * scale, scale, len(vals))
     fetch, fscale, n = got["FETCH_SIZE"]
     write, wscale, _ = got["WRITE_SIZE"]
     return dict(bytes=int(fetch + write), fetch_bytes=int(fetch), write_bytes=int(write),
@@ -490,7 +496,7 @@ def main():
     pbatch.close()
     if not args.no_pmc:
         name = {"k_prw": "k_prw(", "k_pr": "k_pr(", "k_grb": "k_grb<", "k_gr": "k_gr("}[dom]
-        traffic, note = pmc_traffic(args, opts, name)
+        traffic, note = pmc_traffic(args, opts, name, int(pst.launches))
         if traffic:
             roofline["traffic"] = traffic["bytes"]
             roofline["traffic_detail"] = traffic

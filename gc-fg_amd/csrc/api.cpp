@@ -24,6 +24,8 @@ struct gcfg_graph {
     const int32_t *pos = nullptr, *neg = nullptr, *up = nullptr, *down = nullptr, *left = nullptr,
                   *right = nullptr;
     int32_t penalty = 0;
+    const int32_t *diag[4] = {nullptr, nullptr, nullptr, nullptr};  // 8-neighbourhood planes
+    int nbhd = 4;
     gcfg_options opt{};
     gcfg::StreamJoin join;
 };
@@ -74,7 +76,7 @@ int gcfg_device_info(char *name, size_t name_len, int32_t *cu_count, size_t *hbm
 }
 
 // ---------------------------------------------------------------------------
-int gcfg_graph_create(int32_t width, int32_t height, gcfg_graph **out) {
+static int graph_create(int32_t width, int32_t height, int nbhd, gcfg_graph **out) {
     if (!out || width <= 0 || height <= 0) {
         set_error("gcfg_graph_create: invalid shape or output pointer");
         return GCFG_ERR_INVALID;
@@ -85,7 +87,8 @@ int gcfg_graph_create(int32_t width, int32_t height, gcfg_graph **out) {
     gcfg_default_options(&g->opt);
     g->W = width;
     g->H = height;
-    int rc = g->eng.init(g->eng.tiles_for(width, height), 1, g->opt);
+    g->nbhd = nbhd;
+    int rc = g->eng.init(g->eng.tiles_for(width, height), 1, g->opt, nbhd);
     if (rc == GCFG_OK && hipMalloc((void **)&g->d_label, (size_t)width * height * sizeof(int32_t)) != hipSuccess) {
         set_error("gcfg_graph_create: label buffer allocation failed");
         rc = GCFG_ERR_NOMEM;
@@ -105,9 +108,17 @@ int gcfg_graph_create(int32_t width, int32_t height, gcfg_graph **out) {
     return GCFG_OK;
 }
 
+int gcfg_graph_create(int32_t width, int32_t height, gcfg_graph **out) {
+    return graph_create(width, height, 4, out);
+}
+
+int gcfg_graph_create_n8(int32_t width, int32_t height, gcfg_graph **out) {
+    return graph_create(width, height, 8, out);
+}
+
 static int graph_push_inputs(gcfg_graph *g) {
     return g->eng.set_inputs(g->slot, g->pos, g->neg, g->up, g->down, g->left, g->right, g->penalty,
-                             g->label_target);
+                             g->label_target, g->diag[0] ? g->diag : nullptr);
 }
 
 int gcfg_graph_set_dataterms(gcfg_graph *g, const int32_t *d_pos, const int32_t *d_neg) {
@@ -130,6 +141,29 @@ int gcfg_graph_set_edges(gcfg_graph *g, const int32_t *d_up, const int32_t *d_do
     g->down = d_down;
     g->left = d_left;
     g->right = d_right;
+    for (auto &d : g->diag) d = nullptr;  // an 8-neighbourhood graph then needs gcfg_graph_set_edges8
+    return graph_push_inputs(g);
+}
+
+int gcfg_graph_set_edges8(gcfg_graph *g, const int32_t *d_up, const int32_t *d_down, const int32_t *d_left,
+                          const int32_t *d_right, const int32_t *d_upleft, const int32_t *d_upright,
+                          const int32_t *d_downleft, const int32_t *d_downright) {
+    if (!g || !d_up || !d_down || !d_left || !d_right || !d_upleft || !d_upright || !d_downleft || !d_downright) {
+        set_error("gcfg_graph_set_edges8: null argument");
+        return GCFG_ERR_INVALID;
+    }
+    if (g->nbhd != 8) {
+        set_error("gcfg_graph_set_edges8: graph was created with the 4-neighbourhood");
+        return GCFG_ERR_STATE;
+    }
+    g->up = d_up;
+    g->down = d_down;
+    g->left = d_left;
+    g->right = d_right;
+    g->diag[0] = d_upleft;
+    g->diag[1] = d_upright;
+    g->diag[2] = d_downleft;
+    g->diag[3] = d_downright;
     return graph_push_inputs(g);
 }
 
@@ -139,6 +173,7 @@ int gcfg_graph_set_penalty(gcfg_graph *g, int32_t penalty) {
         return GCFG_ERR_INVALID;
     }
     g->up = g->down = g->left = g->right = nullptr;
+    for (auto &d : g->diag) d = nullptr;
     g->penalty = penalty;
     return graph_push_inputs(g);
 }
@@ -205,7 +240,7 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
     rc = g->eng.fetch_ctl(&ctl);
     if (rc != GCFG_OK) return rc;
     if (st.err) {
-        set_error("negative capacity in the inputs");
+        set_error("invalid capacities in the inputs (negative n-link or |pos - neg| > 2^30)");
         return GCFG_ERR_INVALID;
     }
     if (g->opt.verbose) std::printf("Final energy level: %lld\n", (long long)st.flow);
@@ -262,6 +297,24 @@ int gcfg_graph_read_state(gcfg_graph *g, int32_t *h_caps, int32_t *h_excess) {
         return GCFG_ERR_STATE;
     }
     return g->eng.read_state(g->slot, h_caps, h_excess);
+}
+
+int gcfg_graph_read_diag_state(gcfg_graph *g, int32_t *h_caps8) {
+    if (!g || !h_caps8) {
+        set_error("gcfg_graph_read_diag_state: null argument");
+        return GCFG_ERR_INVALID;
+    }
+    if (!g->built) {
+        set_error("gcfg_graph_read_diag_state: graph not built");
+        return GCFG_ERR_STATE;
+    }
+    return g->eng.read_state(g->slot, nullptr, nullptr, nullptr, h_caps8);
+}
+
+int gcfg_graph_neighborhood(gcfg_graph *g, int32_t *nbhd) {
+    if (!g || !nbhd) return GCFG_ERR_INVALID;
+    *nbhd = g->nbhd;
+    return GCFG_OK;
 }
 
 int gcfg_graph_read_distances(gcfg_graph *g, int32_t *h_dist) {
@@ -362,7 +415,8 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
     for (int i = 0; i < n; ++i) {
         const InstState &st = b->states[i];
         if (st.err) {
-            set_error("negative capacity in instance " + std::to_string(i));
+            set_error("invalid capacities in instance " + std::to_string(i) +
+                      " (negative n-link or |pos - neg| > 2^30)");
             return GCFG_ERR_INVALID;
         }
         if (flows) flows[i] = st.flow;

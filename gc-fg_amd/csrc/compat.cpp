@@ -15,18 +15,43 @@ static void gc_check(int rc, const char *what) {
     }
 }
 
+// The 8-neighbourhood overloads (the arity GraphCut.h declares when the
+// includer sets NEIGHBORHOOD to 8); this file is built with the default 4.
 GlobalWrapper GC_Init(int width, int height, int *data_positive, int *data_negative, int penalty, int *up,
-                      int *down, int *left, int *right) {
+                      int *down, int *left, int *right, int *upleft, int *upright, int *downleft, int *downright);
+void GC_SetEdges(GlobalWrapper *gw, int *up, int *down, int *left, int *right, int *upleft, int *upright,
+                 int *downleft, int *downright);
+
+static GlobalWrapper gc_init(int width, int height, int nbhd, int penalty, bool varying) {
     GlobalWrapper gw{};
     gw.width = width;
     gw.height = height;
     gw.penalty = penalty;
-    gw.varying_edges = up != NULL;
-    gc_check(gcfg_graph_create(width, height, &gw.graph), "GC_Init");
+    gw.varying_edges = varying;
+    gw.neighborhood = nbhd;
+    gc_check(nbhd == 8 ? gcfg_graph_create_n8(width, height, &gw.graph) : gcfg_graph_create(width, height, &gw.graph),
+             "GC_Init");
     gw.last_flow = new long long(0);
+    return gw;
+}
+
+GlobalWrapper GC_Init(int width, int height, int *data_positive, int *data_negative, int penalty, int *up,
+                      int *down, int *left, int *right) {
+    GlobalWrapper gw = gc_init(width, height, 4, penalty, up != NULL);
     if (data_positive != NULL) {
         GC_SetDataterms(&gw, data_positive, data_negative);
         if (gw.varying_edges) GC_SetEdges(&gw, up, down, left, right);
+        GC_SetGraph(gw);
+    }
+    return gw;
+}
+
+GlobalWrapper GC_Init(int width, int height, int *data_positive, int *data_negative, int penalty, int *up,
+                      int *down, int *left, int *right, int *upleft, int *upright, int *downleft, int *downright) {
+    GlobalWrapper gw = gc_init(width, height, 8, penalty, up != NULL);
+    if (data_positive != NULL) {
+        GC_SetDataterms(&gw, data_positive, data_negative);
+        if (gw.varying_edges) GC_SetEdges(&gw, up, down, left, right, upleft, upright, downleft, downright);
         GC_SetGraph(gw);
     }
     return gw;
@@ -45,6 +70,15 @@ void GC_SetEdges(GlobalWrapper *gw, int *up, int *down, int *left, int *right) {
     gw->right = right;
 }
 
+void GC_SetEdges(GlobalWrapper *gw, int *up, int *down, int *left, int *right, int *upleft, int *upright,
+                 int *downleft, int *downright) {
+    GC_SetEdges(gw, up, down, left, right);
+    gw->upleft = upleft;
+    gw->upright = upright;
+    gw->downleft = downleft;
+    gw->downright = downright;
+}
+
 void GC_SetPenalty(GlobalWrapper *gw, int p) {
     gw->varying_edges = false;
     gw->penalty = p;
@@ -52,7 +86,11 @@ void GC_SetPenalty(GlobalWrapper *gw, int p) {
 
 void GC_SetGraph(GlobalWrapper gw) {
     gc_check(gcfg_graph_set_dataterms(gw.graph, gw.data_positive, gw.data_negative), "GC_SetGraph");
-    if (gw.varying_edges)
+    if (gw.varying_edges && gw.neighborhood == 8)
+        gc_check(gcfg_graph_set_edges8(gw.graph, gw.up, gw.down, gw.left, gw.right, gw.upleft, gw.upright,
+                                       gw.downleft, gw.downright),
+                 "GC_SetGraph");
+    else if (gw.varying_edges)
         gc_check(gcfg_graph_set_edges(gw.graph, gw.up, gw.down, gw.left, gw.right), "GC_SetGraph");
     else
         gc_check(gcfg_graph_set_penalty(gw.graph, gw.penalty), "GC_SetGraph");
@@ -81,6 +119,7 @@ long long GC_GetFlow(GlobalWrapper gw) { return gw.last_flow ? *gw.last_flow : 0
 namespace {
 gcfg_dataterm *g_dt = nullptr;
 int g_w = 0, g_h = 0;
+int g_model = GCFG_DT_MODEL_HISTOGRAM;
 std::vector<int> g_selection;
 
 gcfg_dataterm *dt() {
@@ -110,6 +149,7 @@ int *getRight() { return link(5); }
 void init_data_term(int w, int h) {
     if (g_dt) free_data_term();
     gc_check(gcfg_dt_create(w, h, &g_dt), "DataTerm::init_data_term");
+    if (g_model != GCFG_DT_MODEL_HISTOGRAM) gc_check(gcfg_dt_set_model(g_dt, g_model), "DataTerm::init_data_term");
     g_w = w;
     g_h = h;
     g_selection.assign((size_t)w * h, 0);
@@ -131,9 +171,19 @@ void reset() {
     for (auto &v : g_selection) v = 0;
 }
 
+// DataTerm.cpp:105-127: the 5x5 square at (x..x+4, y..y+4); the sigma
+// argument is unused there too (the kernel width is setSigmaGaussian's).
 void selectOnMap(int x, int y, int sigma) {
     (void)sigma;
-    gc_check(gcfg_dt_add_seed_rect(dt(), x, y, 5, 5, GCFG_SEED_FG), "DataTerm::selectOnMap");
+    if (g_model == GCFG_DT_MODEL_KDE)
+        gc_check(gcfg_dt_kde_select(dt(), x, y), "DataTerm::selectOnMap");
+    else
+        gc_check(gcfg_dt_add_seed_rect(dt(), x, y, 5, 5, GCFG_SEED_FG), "DataTerm::selectOnMap");
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) {
+            const int xx = x + i, yy = y + j;
+            if (xx >= 0 && yy >= 0 && xx < g_w && yy < g_h) g_selection[(size_t)yy * g_w + xx] = 1;
+        }
 }
 
 void selectRect(int x0, int y0, int w, int h, int cls) {
@@ -152,7 +202,15 @@ void drawSelection(unsigned char *inp, unsigned char *out) {
 
 void getDataTermsMixture() { gc_check(gcfg_dt_compute(dt()), "DataTerm::getDataTermsMixture"); }
 
-void setSigmaGaussian(int sigma) { (void)sigma; }
+void setSigmaGaussian(int sigma) { gc_check(gcfg_dt_set_sigma(dt(), sigma), "DataTerm::setSigmaGaussian"); }
+
+// May precede init_data_term; the model then applies from init on.
+void setModel(int model) {
+    if (g_dt) gc_check(gcfg_dt_set_model(g_dt, model), "DataTerm::setModel");
+    else if (model != GCFG_DT_MODEL_HISTOGRAM && model != GCFG_DT_MODEL_KDE)
+        gc_check(GCFG_ERR_INVALID, "DataTerm::setModel");
+    g_model = model;
+}
 
 void setLambda(int lambda) { gc_check(gcfg_dt_set_lambda(dt(), lambda), "DataTerm::setLambda"); }
 

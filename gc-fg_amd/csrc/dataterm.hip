@@ -1,6 +1,7 @@
 // Colour data-term kernels and the DataTerm.h C ABI (DataTerm.cpp:12-165,
 // DataTermKernels.cu:19-150 re-specified as histogram / -log-likelihood /
 // contrast-weighted n-links; formulas in dataterm.hpp).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -98,29 +99,113 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(const uint8_t *img, const in
     }
 }
 
+// ---------------------------------------------------------------------------
+// Kernel-density colour model (the reference's, DataTermKernels.cu:19-150).
+
+// floor(sqrt(s)) for 0 <= s < 2^24. The hardware square root is within an ulp,
+// which can land just below an integer root; the two integer checks make the
+// result exact (the reference truncates a correctly rounded sqrtf of the same
+// integer, DataTermKernels.cu:140,147, which is also exact for these s).
+__device__ __forceinline__ int isqrt24(int s) {
+    int k = (int)__builtin_sqrtf((float)s);
+    if ((k + 1) * (k + 1) <= s) ++k;
+    if (k * k > s) --k;
+    return k;
+}
+
+// p(I) = sum over the seed list of g[floor(|I - s_j|)] for |I - s_j| < 101,
+// in f64 and in list order (the list index is uniform across the wavefront,
+// so the seed colours are scalar loads).
+__device__ __forceinline__ double kde_density(const uint8_t *a, const uint32_t *__restrict__ cols, int n,
+                                              const float *g) {
+    const int c0 = a[0], c1 = a[1], c2 = a[2];
+    double p = 0.0;
+    for (int j = 0; j < n; ++j) {
+        const uint32_t c = cols[j];
+        const int d0 = c0 - (int)(c >> 16), d1 = c1 - (int)((c >> 8) & 255u), d2 = c2 - (int)(c & 255u);
+        const int s = d0 * d0 + d1 * d1 + d2 * d2;
+        if (s < KDE_TAPS * KDE_TAPS) p += (double)g[isqrt24(s)];
+    }
+    return p;
+}
+
+// (int) of a double as C truncates it, saturated to the int32 range (the
+// reference's cast is undefined beyond it).
+__device__ __forceinline__ int32_t trunc_sat(double v) {
+    if (v >= 2147483647.0) return 2147483647;
+    if (v <= -2147483648.0) return (int32_t)0x80000000u;
+    return (int32_t)v;
+}
+
 // Pre-quantisation values in f64 on the device (verification output).
+// Kernel-density model (kde_cols != null): tpos = 300000 p, tneg = 300000 (1 - p).
 __global__ __launch_bounds__(DT_NT) void k_dt_unquantized(const uint8_t *img, const DtStats *st, double beta,
-                                                          int lambda, int W, int H, double *tpos, double *tneg,
-                                                          double *rightf, double *downf) {
+                                                          int lambda, int W, int H, const uint32_t *kde_cols,
+                                                          int kde_n, const float *gauss, double *tpos,
+                                                          double *tneg, double *rightf, double *downf) {
     const int64_t n = (int64_t)W * H;
     for (int64_t i = blockIdx.x * (int64_t)DT_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * DT_NT) {
         const int x = (int)(i % W), y = (int)(i / W);
         const uint8_t *a = img + 3 * i;
-        double cost[2];
-        for (int k = 0; k < 2; ++k) {
-            double p = 1.0;
-            for (int c = 0; c < 3; ++c)
-                p *= (double)(st->hist[k][c][a[c] >> 4] + 1) / (double)(st->counts[k] + 16);
-            cost[k] = 64.0 * -log(p);
+        if (kde_cols) {
+            const double p = kde_density(a, kde_cols, kde_n, gauss);
+            tpos[i] = 300000.0 * p;
+            tneg[i] = 300000.0 * (1.0 - p);
+        } else {
+            double cost[2];
+            for (int k = 0; k < 2; ++k) {
+                double p = 1.0;
+                for (int c = 0; c < 3; ++c)
+                    p *= (double)(st->hist[k][c][a[c] >> 4] + 1) / (double)(st->counts[k] + 16);
+                cost[k] = 64.0 * -log(p);
+            }
+            tpos[i] = cost[1];
+            tneg[i] = cost[0];
         }
-        tpos[i] = cost[1];
-        tneg[i] = cost[0];
         double wr = 0.0, wd = 0.0;
         if (x + 1 < W) wr = beta > 0.0 ? (double)lambda * exp(-(double)sqdiff(a, a + 3) / (2.0 * beta)) : lambda;
         if (y + 1 < H)
             wd = beta > 0.0 ? (double)lambda * exp(-(double)sqdiff(a, a + 3 * (int64_t)W) / (2.0 * beta)) : lambda;
         rightf[i] = wr;
         downf[i] = wd;
+    }
+}
+
+// Seed colours of the in-image part of a selection square, in the
+// reference's launch order (x offset outer, y offset inner, DataTerm.cpp:114-123).
+__global__ void k_dt_kde_gather(const uint8_t *img, int W, int x0, int y0, int ni, int nj, uint32_t *out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * nj) return;
+    const int X = x0 + t / nj, Y = y0 + t % nj;
+    const uint8_t *a = img + 3 * ((int64_t)Y * W + X);
+    out[t] = ((uint32_t)a[0] << 16) | ((uint32_t)a[1] << 8) | (uint32_t)a[2];
+}
+
+// t-links from the kernel density (K_CalcTerms, DataTermKernels.cu:61-83,
+// with p rounded to float as the reference's volume holds it), n-links as in
+// k_dt_links.
+__global__ __launch_bounds__(DT_NT) void k_dt_kde_links(const uint8_t *img, const uint32_t *__restrict__ cols,
+                                                        int ncols, const float *gauss, const int32_t *lut,
+                                                        int lambda, int W, int H, int32_t *pos, int32_t *neg,
+                                                        int32_t *up, int32_t *down, int32_t *left,
+                                                        int32_t *right) {
+    extern __shared__ int32_t thr[];
+    __shared__ float g[KDE_TAPS];
+    const int32_t *gthr = lut + 2 * 4096;
+    for (int i = threadIdx.x; i <= lambda; i += DT_NT) thr[i] = gthr[i];
+    for (int i = threadIdx.x; i < KDE_TAPS; i += DT_NT) g[i] = gauss[i];
+    __syncthreads();
+    const int64_t n = (int64_t)W * H;
+    for (int64_t i = blockIdx.x * (int64_t)DT_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * DT_NT) {
+        const int x = (int)(i % W), y = (int)(i / W);
+        const uint8_t *a = img + 3 * i;
+        const float pf = (float)kde_density(a, cols, ncols, g);
+        pos[i] = trunc_sat(300000.0 * (double)pf);
+        neg[i] = trunc_sat(300000.0 * (double)(1.0f - pf));
+        right[i] = x + 1 < W ? nlink_weight(sqdiff(a, a + 3), thr, lambda) : 0;
+        left[i] = x > 0 ? nlink_weight(sqdiff(a, a - 3), thr, lambda) : 0;
+        down[i] = y + 1 < H ? nlink_weight(sqdiff(a, a + 3 * (int64_t)W), thr, lambda) : 0;
+        up[i] = y > 0 ? nlink_weight(sqdiff(a, a - 3 * (int64_t)W), thr, lambda) : 0;
     }
 }
 
@@ -173,6 +258,8 @@ int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs) {
     rc |= dalloc((void **)&seeds_, n);
     rc |= dalloc((void **)&d_stats_, sizeof(DtStats));
     rc |= dalloc((void **)&d_lut_, (2 * 4096 + MAX_LAMBDA + 1) * sizeof(int32_t));
+    rc |= dalloc((void **)&d_kde_, KDE_MAX_SEEDS * sizeof(uint32_t));
+    rc |= dalloc((void **)&d_gauss_, KDE_TAPS * sizeof(float));
     if (own_outputs) {
         rc |= dalloc((void **)&pos_, n * 4);
         rc |= dalloc((void **)&neg_, n * 4);
@@ -222,10 +309,61 @@ int DtContext::add_seed_rect(int x0, int y0, int w, int h, int cls) {
 }
 
 int DtContext::clear_seeds() {
+    kde_count = 0;
     hipLaunchKernelGGL(k_fill_u8, dim3(grid_for((int64_t)W * H)), dim3(DT_NT), 0, stream_, seeds_, (int64_t)W * H,
                        (uint8_t)0);
     GCFG_HIP(hipGetLastError());
     computed = false;
+    return GCFG_OK;
+}
+
+int DtContext::set_model(int m) {
+    if (m != GCFG_DT_MODEL_HISTOGRAM && m != GCFG_DT_MODEL_KDE) {
+        set_error("data term: unknown colour model");
+        return GCFG_ERR_INVALID;
+    }
+    model = m;
+    computed = false;
+    return GCFG_OK;
+}
+
+int DtContext::set_sigma(int s) {
+    if (s < 1 || s > 1000) {
+        set_error("data term: sigma must be in 1..1000");
+        return GCFG_ERR_INVALID;
+    }
+    sigma = s;
+    computed = false;
+    return GCFG_OK;
+}
+
+int DtContext::kde_select(int x, int y, int size) {
+    const int xa = std::max(x, 0), xb = std::min(x + size, W);
+    const int ya = std::max(y, 0), yb = std::min(y + size, H);
+    const int ni = xb > xa ? xb - xa : 0, nj = yb > ya ? yb - ya : 0;
+    if (ni * nj == 0) return GCFG_OK;
+    if (kde_count + ni * nj > KDE_MAX_SEEDS) {
+        set_error("data term: kernel-density seed list is full (65536 colours)");
+        return GCFG_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(k_dt_kde_gather, dim3(1), dim3(DT_NT), 0, stream_, img_, W, xa, ya, ni, nj,
+                       d_kde_ + kde_count);
+    GCFG_HIP(hipGetLastError());
+    kde_count += ni * nj;
+    computed = false;
+    return GCFG_OK;
+}
+
+// Kernel values of K_InitGaussian (DataTermKernels.cu:19-31) with its mixed
+// float/double arithmetic: x / sigma in float, the exponent in double rounded
+// to float for expf, the factor 1 / (sigma * 2.507) in double, stored as float.
+int DtContext::build_gauss() {
+    for (int k = 0; k < KDE_TAPS; ++k) {
+        const float t = (float)k / (float)sigma;
+        const double arg = -0.5 * (double)t * (double)t;
+        gauss[k] = (float)(1.0 / ((double)(float)sigma * 2.507) * (double)expf((float)arg));
+    }
+    GCFG_HIP(hipMemcpyAsync(d_gauss_, gauss, sizeof(gauss), hipMemcpyHostToDevice, stream_));
     return GCFG_OK;
 }
 
@@ -288,8 +426,15 @@ int DtContext::compute(int32_t *pos, int32_t *neg, int32_t *up, int32_t *down, i
     beta = pairs > 0 ? (double)stats.beta_sum / (double)pairs : 0.0;
     int rc = build_tables();
     if (rc != GCFG_OK) return rc;
-    hipLaunchKernelGGL(k_dt_links, dim3(grid_for(n)), dim3(DT_NT), (lambda + 1) * sizeof(int32_t), stream_, img_,
-                       d_lut_, lambda, W, H, pos, neg, up, down, left, right);
+    if (model == GCFG_DT_MODEL_KDE) {
+        rc = build_gauss();
+        if (rc != GCFG_OK) return rc;
+        hipLaunchKernelGGL(k_dt_kde_links, dim3(grid_for(n)), dim3(DT_NT), (lambda + 1) * sizeof(int32_t), stream_,
+                           img_, d_kde_, kde_count, d_gauss_, d_lut_, lambda, W, H, pos, neg, up, down, left, right);
+    } else {
+        hipLaunchKernelGGL(k_dt_links, dim3(grid_for(n)), dim3(DT_NT), (lambda + 1) * sizeof(int32_t), stream_,
+                           img_, d_lut_, lambda, W, H, pos, neg, up, down, left, right);
+    }
     GCFG_HIP(hipGetLastError());
     computed = true;
     return GCFG_OK;
@@ -303,8 +448,10 @@ int DtContext::unquantized(double *h_tpos, double *h_tneg, double *h_right, doub
     const size_t n = (size_t)W * H;
     double *buf = nullptr;
     GCFG_HIP(hipMalloc((void **)&buf, 4 * n * sizeof(double)));
+    const bool kde = model == GCFG_DT_MODEL_KDE;
     hipLaunchKernelGGL(k_dt_unquantized, dim3(grid_for((int64_t)n)), dim3(DT_NT), 0, stream_, img_, d_stats_, beta,
-                       lambda, W, H, buf, buf + n, buf + 2 * n, buf + 3 * n);
+                       lambda, W, H, kde ? d_kde_ : nullptr, kde_count, d_gauss_, buf, buf + n, buf + 2 * n,
+                       buf + 3 * n);
     hipError_t e = hipGetLastError();
     double *outs[4] = {h_tpos, h_tneg, h_right, h_down};
     for (int k = 0; k < 4 && e == hipSuccess; ++k)
@@ -449,6 +596,27 @@ int gcfg_dt_get_beta(gcfg_dataterm *dt, int64_t *sum, int64_t *pairs, double *be
 int gcfg_dt_unquantized(gcfg_dataterm *dt, double *h_tpos, double *h_tneg, double *h_right, double *h_down) {
     if (!dt) return GCFG_ERR_INVALID;
     return dt->ctx.unquantized(h_tpos, h_tneg, h_right, h_down);
+}
+
+int gcfg_dt_set_model(gcfg_dataterm *dt, int32_t model) {
+    if (!dt) return GCFG_ERR_INVALID;
+    return dt->ctx.set_model(model);
+}
+
+int gcfg_dt_set_sigma(gcfg_dataterm *dt, int32_t sigma) {
+    if (!dt) return GCFG_ERR_INVALID;
+    return dt->ctx.set_sigma(sigma);
+}
+
+int gcfg_dt_kde_select(gcfg_dataterm *dt, int32_t x, int32_t y) {
+    if (!dt) return GCFG_ERR_INVALID;
+    return dt->ctx.kde_select(x, y, 5);
+}
+
+int gcfg_dt_kde_count(gcfg_dataterm *dt, int32_t *count) {
+    if (!dt || !count) return GCFG_ERR_INVALID;
+    *count = dt->ctx.kde_count;
+    return GCFG_OK;
 }
 
 }  // extern "C"

@@ -13,6 +13,19 @@
 // integer thresholds). The LUT and thresholds are built on the host in double
 // from the 96 histogram counts and beta, so the quantised capacities round
 // exactly as a direct per-pixel double evaluation does.
+//
+// The reference's own colour model is kept as a second mode (GCFG_DT_MODEL_KDE,
+// DataTermKernels.cu:19-31,61-83,116-150; DataTerm.cpp:97-137): a Gaussian
+// kernel density over the seed colours,
+//   g[k] = (float)(1 / (sigma * 2.507) * expf(-0.5 (k/sigma)^2)),  k = 0..100,
+//   p(I) = sum_j g[floor(|I - s_j|)]  over seed colours s_j with |I - s_j| < 101,
+//   pos = (int)(300000 * p),  neg = (int)(300000 * (1 - p))   (p rounded to float).
+// The reference scatters every seed into a 256^3 float volume with float
+// atomics (4.3 M atomics per seed pixel, order-dependent sums). Here the
+// density is gathered per pixel instead: the seed colours are captured at
+// selection time into a device list, and each pixel sums its kernel values in
+// f64 in list order, so the result is deterministic and the same value the
+// volume holds up to float rounding.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,6 +36,9 @@
 #include "gcfg.h"
 
 namespace gcfg {
+
+constexpr int KDE_MAX_SEEDS = 65536;  // seed colours kept for the kernel density
+constexpr int KDE_TAPS = 101;         // kernel support: floor(distance) 0..100 (DataTermKernels.cu:11)
 
 struct DtStats {
     int32_t hist[2][3][16];
@@ -43,10 +59,19 @@ class DtContext {
     // Full compute; outputs go to the owned planes or to the given ones.
     int compute(int32_t *pos, int32_t *neg, int32_t *up, int32_t *down, int32_t *left, int32_t *right);
     int compute_owned() { return compute(pos_, neg_, up_, down_, left_, right_); }
+    // Kernel-density mode: selectOnMap's 5 x 5 square at (x..x+4, y..y+4),
+    // colours read from the current image now (pixels outside it are skipped).
+    int kde_select(int x, int y, int size);
+    int set_model(int m);
+    int set_sigma(int s);
     int unquantized(double *h_tpos, double *h_tneg, double *h_right, double *h_down);
 
     int W = 0, H = 0;
     int32_t lambda = 50;
+    int32_t model = GCFG_DT_MODEL_HISTOGRAM;
+    int32_t sigma = 5;     // DEFAULT_SIGMA, DataTerm.cpp:6
+    int32_t kde_count = 0; // seed colours in the list
+    float gauss[KDE_TAPS] = {};
     DtStats stats{};
     int64_t pairs = 0;
     double beta = 0.0;
@@ -57,11 +82,14 @@ class DtContext {
 
    private:
     int build_tables();
+    int build_gauss();
     hipStream_t stream_ = nullptr;
     const uint8_t *img_ = nullptr;
     uint8_t *own_img_ = nullptr, *seeds_ = nullptr;
     DtStats *d_stats_ = nullptr, *h_stats_ = nullptr;
     int32_t *d_lut_ = nullptr, *h_lut_ = nullptr;  // [2*4096 t-link LUT][thresholds]
+    uint32_t *d_kde_ = nullptr;                    // seed colours 0x00RRGGBB-packed (c0 << 16 | c1 << 8 | c2)
+    float *d_gauss_ = nullptr;
     std::vector<void *> allocs_;
     size_t bytes_ = 0;
 };

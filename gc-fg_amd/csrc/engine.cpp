@@ -67,9 +67,15 @@ StreamJoin::~StreamJoin() {
     if (ev_out) hipEventDestroy(ev_out);
 }
 
-int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
+int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o, int nbhd) {
     opt = o;
     if (!options_valid(opt)) return GCFG_ERR_INVALID;
+    if (nbhd != 4 && nbhd != 8) {
+        set_error("neighbourhood must be 4 or 8");
+        return GCFG_ERR_INVALID;
+    }
+    nbhd_ = nbhd;
+    kp_.nbhd = nbhd;
     cap_tiles = std::max(cap_tiles, 1);
     cap_slots = std::max(cap_slots, 1);
     cap_tiles_ = cap_tiles;
@@ -105,6 +111,11 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o) {
     rc |= dalloc((void **)&kp_.state, (size_t)cap_slots * sizeof(InstState));
     rc |= dalloc((void **)&kp_.ctl, sizeof(Ctl));
     rc |= dalloc((void **)&d_slots_, (size_t)(2 * cap_slots + 1) * sizeof(int32_t));
+    if (nbhd == 8) {
+        rc |= dalloc((void **)&kp_.caps8, nodes * sizeof(int4));
+        rc |= dalloc((void **)&kp_.mask8, nodes);
+        rc |= dalloc((void **)&kp_.nbr8, (size_t)cap_tiles * sizeof(int4));
+    }
     if (rc != GCFG_OK) return GCFG_ERR_NOMEM;
     GCFG_HIP(hipHostMalloc((void **)&pin_desc_, (size_t)cap_slots * sizeof(InstDesc), hipHostMallocDefault));
     GCFG_HIP(hipHostMalloc((void **)&pin_slots_, (size_t)(2 * cap_slots + 1) * sizeof(int32_t), hipHostMallocDefault));
@@ -151,7 +162,9 @@ bool options_valid(const gcfg_options &o) {
 // step graph with stale parameters is dropped.
 void Engine::apply_options() {
     const int cu = std::max(cu_count_, 1);
-    const int variant = opt.kernels ? 1 : 0;
+    // The 8-neighbourhood runs the workgroup-per-tile kernels (diagonal arcs
+    // need the corner halo nodes that those kernels stage in LDS).
+    const int variant = (opt.kernels && nbhd_ == 4) ? 1 : 0;
     int gbx = opt.gr_block_w > 0 ? opt.gr_block_w : 1, gby = opt.gr_block_h > 0 ? opt.gr_block_h : 1;
     if (!variant || !gr_block_supported(gbx, gby)) gbx = gby = 1;
     const int wpr = opt.pr_waves_per_cu > 0 ? opt.pr_waves_per_cu : 16;
@@ -159,7 +172,9 @@ void Engine::apply_options() {
     const int gpr = variant ? cu * wpr : cu * 4;
     const int ggr = variant ? std::max(cu, cu * wgr / (gbx * gby)) : cu * 8;
     const int grr = opt.gr_rounds > 0 ? opt.gr_rounds : 48;
-    const int prof = opt.profile ? 1 : 0;
+    // Iteration counters add one same-address atomic per workgroup and launch,
+    // which slows the kernels being timed, so only profile >= 2 turns them on.
+    const int prof = opt.profile >= 2 ? 1 : 0;
     const int cap = variant && opt.gr_cap > 0 ? opt.gr_cap : 0;
     if (kp_.inner_iters != opt.inner_iters || kp_.variant != variant || grid_pr_ != gpr || grid_gr_ != ggr ||
         kp_.gr_rounds != grr || kp_.profile != prof || kp_.gbx != gbx || kp_.gby != gby || kp_.gr_cap != cap) {
@@ -246,8 +261,14 @@ int Engine::add_instance(int W, int H, const int32_t *pos, const int32_t *neg, c
 }
 
 int Engine::set_inputs(int slot, const int32_t *pos, const int32_t *neg, const int32_t *up, const int32_t *down,
-                       const int32_t *left, const int32_t *right, int32_t penalty, int32_t *label_out) {
+                       const int32_t *left, const int32_t *right, int32_t penalty, int32_t *label_out,
+                       const int32_t *const *diag) {
     InstDesc &d = h_desc_[slot];
+    const bool d8 = nbhd_ == 8 && up && diag;
+    d.upleft = d8 ? diag[0] : nullptr;
+    d.upright = d8 ? diag[1] : nullptr;
+    d.downleft = d8 ? diag[2] : nullptr;
+    d.downright = d8 ? diag[3] : nullptr;
     d.pos = pos;
     d.neg = neg;
     d.up = up;
@@ -272,6 +293,10 @@ int Engine::build(const std::vector<int> &slots) {
         const InstDesc &d = h_desc_[s];
         if (!d.pos || !d.neg) {
             set_error("data terms not set");
+            return GCFG_ERR_STATE;
+        }
+        if (nbhd_ == 8 && d.up && !(d.upleft && d.upright && d.downleft && d.downright)) {
+            set_error("8-neighbourhood: per-arc capacities need all eight direction planes");
             return GCFG_ERR_STATE;
         }
         if (!d.label_out) {
@@ -520,29 +545,37 @@ int Engine::fetch_ctl(Ctl *out) {
     return GCFG_OK;
 }
 
-int Engine::read_state(int slot, int32_t *h_caps, int32_t *h_exc, int32_t *h_dist) {
+int Engine::read_state(int slot, int32_t *h_caps, int32_t *h_exc, int32_t *h_dist, int32_t *h_caps8) {
     const InstDesc &d = h_desc_[slot];
     const size_t n = (size_t)d.W * d.H;
-    int32_t *dc = nullptr, *de = nullptr, *dd = nullptr;
+    if (h_caps8 && nbhd_ != 8) {
+        set_error("diagonal capacities exist only in an 8-neighbourhood graph");
+        return GCFG_ERR_STATE;
+    }
+    int32_t *dc = nullptr, *de = nullptr, *dd = nullptr, *d8 = nullptr;
     hipError_t e0 = hipSuccess;
     if (h_caps && e0 == hipSuccess) e0 = hipMalloc((void **)&dc, n * 4 * sizeof(int32_t));
     if (h_exc && e0 == hipSuccess) e0 = hipMalloc((void **)&de, n * sizeof(int32_t));
     if (h_dist && e0 == hipSuccess) e0 = hipMalloc((void **)&dd, n * sizeof(int32_t));
-    hipError_t e1 = hipSuccess, e2 = hipSuccess, e3 = hipSuccess, e4 = hipSuccess;
+    if (h_caps8 && e0 == hipSuccess) e0 = hipMalloc((void **)&d8, n * 4 * sizeof(int32_t));
+    hipError_t e1 = hipSuccess, e2 = hipSuccess, e3 = hipSuccess, e4 = hipSuccess, e5 = hipSuccess;
     if (e0 == hipSuccess) {
-        launch_read_state(kp_, slot, dc, de, dd, stream_);
+        launch_read_state(kp_, slot, dc, de, dd, d8, stream_);
         if (dc) e1 = hipMemcpyAsync(h_caps, dc, n * 4 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
         if (de) e2 = hipMemcpyAsync(h_exc, de, n * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
         if (dd) e3 = hipMemcpyAsync(h_dist, dd, n * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
+        if (d8) e5 = hipMemcpyAsync(h_caps8, d8, n * 4 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
         e4 = hipStreamSynchronize(stream_);
     }
     if (dc) hipFree(dc);
     if (de) hipFree(de);
     if (dd) hipFree(dd);
+    if (d8) hipFree(d8);
     GCFG_HIP(e0);
     GCFG_HIP(e1);
     GCFG_HIP(e2);
     GCFG_HIP(e3);
+    GCFG_HIP(e5);
     GCFG_HIP(e4);
     return GCFG_OK;
 }

@@ -59,6 +59,8 @@ struct InstDesc {
     int32_t tile0, ntiles, n_interior, penalty;
     const int32_t *pos, *neg, *up, *down, *left, *right;
     int32_t *label_out;  // device W*H
+    // 8-neighbourhood only: capacities to (x-1,y-1), (x+1,y-1), (x-1,y+1), (x+1,y+1)
+    const int32_t *upleft, *upright, *downleft, *downright;
 };
 
 struct InstState {
@@ -120,6 +122,14 @@ struct KParams {
     int32_t profile;    // count inner iterations / relaxation rounds (extra atomics)
     int32_t gbx, gby;   // global-relabel block, in tiles (one wavefront per tile)
     int32_t gr_cap;     // distance cap of intermediate global relabels (0: always exact)
+    // 8-neighbourhood (nbhd == 8; workgroup kernels only): per node the
+    // diagonal residual capacities int4 {up-left, up-right, down-left,
+    // down-right} and a second mask byte (bits 0-3: those arcs residual); per
+    // tile the diagonal neighbour tile ids in the same order (-1: none).
+    int32_t nbhd;
+    int4 *caps8;
+    uint8_t *mask8;
+    int4 *nbr8;
 };
 
 // Kernel launchers (solver.hip).
@@ -135,7 +145,7 @@ void launch_reset_stats(const KParams &p, hipStream_t s);
 void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,
                  hipStream_t s);
 void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, int32_t *d_dist,
-                       hipStream_t s);
+                       int32_t *d_caps8, hipStream_t s);
 
 // Orders a handle's internal stream against a caller stream: `before` makes
 // the internal stream wait for everything already queued on the caller's,
@@ -167,15 +177,20 @@ class Engine {
     Engine(const Engine &) = delete;
     Engine &operator=(const Engine &) = delete;
 
-    // Capacity in tiles and instance slots.
-    int init(int cap_tiles, int cap_slots, const gcfg_options &opt);
+    // Capacity in tiles and instance slots; nbhd 4 or 8 (neighbourhood of
+    // every instance of this engine).
+    int init(int cap_tiles, int cap_slots, const gcfg_options &opt, int nbhd = 4);
     // Reserve a tile range and a slot for a W x H instance; returns slot or < 0.
     int add_instance(int W, int H, const int32_t *pos, const int32_t *neg, const int32_t *up,
                      const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
                      int32_t *label_out);
     void remove_instance(int slot);
+    // diag: the four diagonal planes (up-left, up-right, down-left,
+    // down-right) of an 8-neighbourhood instance with per-arc capacities.
     int set_inputs(int slot, const int32_t *pos, const int32_t *neg, const int32_t *up, const int32_t *down,
-                   const int32_t *left, const int32_t *right, int32_t penalty, int32_t *label_out);
+                   const int32_t *left, const int32_t *right, int32_t penalty, int32_t *label_out,
+                   const int32_t *const *diag = nullptr);
+    int neighborhood() const { return nbhd_; }
     // Upload descriptors, re-initialise state and build the residual graphs.
     int build(const std::vector<int> &slots);
     // Upload the descriptor table only (e.g. a new label destination).
@@ -189,9 +204,11 @@ class Engine {
     int pump(int steps);
     int live_count(int32_t *out);
     int edit(int slot, const std::vector<int32_t> &strokes, int32_t delta);
-    // Residual capacities (W*H*4), excess (W*H) and distance labels (W*H) in
-    // image layout; any output may be null.
-    int read_state(int slot, int32_t *h_caps, int32_t *h_exc, int32_t *h_dist = nullptr);
+    // Residual capacities (W*H*4), excess (W*H), distance labels (W*H) and,
+    // for the 8-neighbourhood, diagonal residual capacities (W*H*4) in image
+    // layout; any output may be null.
+    int read_state(int slot, int32_t *h_caps, int32_t *h_exc, int32_t *h_dist = nullptr,
+                   int32_t *h_caps8 = nullptr);
     int fetch_state(int slot, InstState *out);
     int fetch_states(InstState *out, int n);  // slots [0, n)
     int fetch_ctl(Ctl *out);
@@ -202,7 +219,9 @@ class Engine {
     size_t tiles_free() const;
     int cap_tiles() const { return cap_tiles_; }
     int tiles_for(int W, int H) const { return ((W + TW - 1) / TW) * ((H + TH - 1) / TH); }
-    static size_t bytes_per_tile() { return (size_t)TN * (16 + 4 + 4 + 1) + 96; }
+    static size_t bytes_per_tile(int nbhd = 4) {
+        return (size_t)TN * (16 + 4 + 4 + 1) + 96 + (nbhd == 8 ? (size_t)TN * (16 + 1) + 16 : 0);
+    }
     gcfg_options opt;
     int steps_run = 0;
     // Steps the previous run needed (set by the caller from the instance
@@ -224,6 +243,7 @@ class Engine {
 
     int dev_ = 0;
     int cu_count_ = 0;
+    int nbhd_ = 4;
     hipStream_t stream_ = nullptr;
     KParams kp_{};
     int cap_tiles_ = 0, cap_slots_ = 0;

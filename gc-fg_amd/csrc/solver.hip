@@ -57,6 +57,16 @@ __device__ __forceinline__ void atomic_add64(int64_t *p, int64_t v) {
 // Tile flags are plain stores: the scheduler scans them between launches, so
 // concurrent writers of the same value need no atomics.
 __device__ __forceinline__ void set_active(const KParams &p, int t, int v) { p.act_flag[t] = v; }
+
+// Inputs a node accepts: t-links of any sign, as in the reference (it takes
+// excess = pos - neg unchecked, GraphCutKernels.cu:196-197; its own colour
+// data term makes neg negative where p > 1, DataTermKernels.cu:81-82), with
+// |pos - neg| <= 2^30 so that int32 excess keeps room for n-link inflow, and
+// non-negative n-links.
+__device__ __forceinline__ bool tlinks_bad(int pp, int qq) {
+    const long long d = (long long)pp - qq;
+    return d < -(1LL << 30) || d > (1LL << 30);
+}
 __device__ __forceinline__ void mark_dirty(const KParams &p, int t) { p.dirty[t] = 1; }
 
 // ---------------------------------------------------------------------------
@@ -107,6 +117,11 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
             ti.nbr[3] = ty < D.tiles_y - 1 ? t + D.tiles_x : -1;
             ti.pad = 0;
             p.tiles[t] = ti;
+            if (p.nbhd == 8) {
+                const bool l = tx > 0, r = tx < D.tiles_x - 1, u = ty > 0, d = ty < D.tiles_y - 1;
+                p.nbr8[t] = make_int4(l && u ? t - D.tiles_x - 1 : -1, r && u ? t - D.tiles_x + 1 : -1,
+                                      l && d ? t + D.tiles_x - 1 : -1, r && d ? t + D.tiles_x + 1 : -1);
+            }
             p.act_flag[t] = 0;
             p.dirty[t] = 0;
             p.live[t] = 2;
@@ -119,11 +134,24 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
             const int y = wave * RPT + k, x = lane;
             const int X = tx * TW + x, Y = ty * TH + y;
             const size_t g = (size_t)t * TN + y * TW + x;
-            int4 c = make_int4(0, 0, 0, 0);
+            int4 c = make_int4(0, 0, 0, 0), c8 = make_int4(0, 0, 0, 0);
             int e = 0;
             if (X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1) {
                 const size_t i = (size_t)Y * D.W + X;
                 const int pp = D.pos[i], qq = D.neg[i];
+                if (p.nbhd == 8) {
+                    // Diagonal arcs (GraphCutKernels.cu:133-151,190-213): zero
+                    // towards the border like the side arcs.
+                    if (D.up)
+                        c8 = make_int4(D.upleft[i], D.upright[i], D.downleft[i], D.downright[i]);
+                    else
+                        c8 = make_int4(D.penalty, D.penalty, D.penalty, D.penalty);
+                    if (X == 1) c8.x = c8.z = 0;
+                    if (X == D.W - 2) c8.y = c8.w = 0;
+                    if (Y == 1) c8.x = c8.y = 0;
+                    if (Y == D.H - 2) c8.z = c8.w = 0;
+                    bad |= (c8.x < 0) | (c8.y < 0) | (c8.z < 0) | (c8.w < 0);
+                }
                 if (D.up) {
                     c.x = D.left[i];
                     c.y = D.right[i];
@@ -136,7 +164,7 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
                 if (X == D.W - 2) c.y = 0;
                 if (Y == 1) c.z = 0;
                 if (Y == D.H - 2) c.w = 0;
-                bad |= (pp < 0) | (qq < 0) | (c.x < 0) | (c.y < 0) | (c.z < 0) | (c.w < 0);
+                bad |= tlinks_bad(pp, qq) | (c.x < 0) | (c.y < 0) | (c.z < 0) | (c.w < 0);
                 e = pp - qq;
                 sp += pp;
             }
@@ -144,6 +172,10 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
             p.exc[g] = e;
             p.dist[g] = DINF;
             p.mask[g] = 0;
+            if (p.nbhd == 8) {
+                p.caps8[g] = c8;
+                p.mask8[g] = 0;
+            }
         }
         const int64_t tot = block_sum64(sp, red);
         const int anybad = __syncthreads_or(bad);
@@ -203,8 +235,12 @@ __device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int col
                 pred = p.dirty[t];
                 if (pred) p.dirty[t] = 0;
             } else {
+                // 4-neighbourhood: checkerboard (no two side neighbours run
+                // together). 8-neighbourhood: (tx & 1, ty & 1) four colours,
+                // so diagonal neighbours do not run together either.
                 const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
-                pred = (((tx + ty) & 1) == colour) && p.act_flag[t];
+                const int c = p.nbhd == 8 ? ((tx & 1) | ((ty & 1) << 1)) : ((tx + ty) & 1);
+                pred = c == colour && p.act_flag[t];
             }
         }
         const unsigned long long bal = __ballot(pred);
@@ -334,14 +370,15 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
         if (threadIdx.x == 0) st.phase = PH_PR;
         pr_iter = 0;
     }
-    // PH_PR: alternate checkerboard colours, then relabel globally again.
-    if (pr_iter >= 2 * sweeps) {
+    // PH_PR: cycle through the tile colours, then relabel globally again.
+    const int ncol = p.nbhd == 8 ? 4 : 2;
+    if (pr_iter >= ncol * sweeps) {
         start_global_relabel(p, D, st, parity, reinit, cap);
         return;
     }
-    const int colour = pr_iter & 1;
-    int n = emit_tiles(p, D, EM_COLOUR, colour, true, 0, parity);
-    if (n == 0) n = emit_tiles(p, D, EM_COLOUR, colour ^ 1, true, 0, parity);
+    const int colour = pr_iter & (ncol - 1);
+    int n = 0;
+    for (int c = 0; c < ncol && n == 0; ++c) n = emit_tiles(p, D, EM_COLOUR, (colour + c) & (ncol - 1), true, 0, parity);
     if (n == 0) {
         start_global_relabel(p, D, st, parity, reinit, cap);
         return;
@@ -697,6 +734,355 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p, int parity) {
             if (lane == TW - 1) chR |= changed;
             if (y == 0) chU |= changed;
             if (y == TH - 1) chD |= changed;
+            cnt += (m[k] & 16) && dv[k] < DINF;
+        }
+        chL = __syncthreads_or(chL);
+        chR = __syncthreads_or(chR);
+        chU = __syncthreads_or(chU);
+        chD = __syncthreads_or(chD);
+        const int64_t total = block_sum64(cnt, red);
+        if (tid == 0) {
+            if (chL && ti.nbr[0] >= 0) mark_dirty(p, ti.nbr[0]);
+            if (chR && ti.nbr[1] >= 0) mark_dirty(p, ti.nbr[1]);
+            if (chU && ti.nbr[2] >= 0) mark_dirty(p, ti.nbr[2]);
+            if (chD && ti.nbr[3] >= 0) mark_dirty(p, ti.nbr[3]);
+            if (!conv) mark_dirty(p, t);
+            set_active(p, t, total > 0);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 8-neighbourhood (NEIGHBORHOOD == 8, GraphCut.h:22,65-70): the workgroup
+// kernels above with the four diagonal arcs (GraphCutKernels.cu:516-521 push
+// them after the side arcs). Directions 0-7: left, right, up, down, up-left,
+// up-right, down-left, down-right; OPP8 is the reverse arc. The LDS halo holds
+// the four corner nodes too, tiles of one push pass never touch (4 colours),
+// and flow into another tile's node is added there after the pass.
+__device__ constexpr int DX8[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+__device__ constexpr int DY8[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+__device__ constexpr int OPP8[8] = {1, 0, 3, 2, 7, 6, 5, 4};
+
+__device__ __forceinline__ int cap8(const int4 &c, const int4 &c8, int dir) {
+    switch (dir) {
+        case 0: return c.x;
+        case 1: return c.y;
+        case 2: return c.z;
+        case 3: return c.w;
+        case 4: return c8.x;
+        case 5: return c8.y;
+        case 6: return c8.z;
+        default: return c8.w;
+    }
+}
+
+__device__ __forceinline__ void add_cap8(int4 &c, int4 &c8, int dir, int v) {
+    switch (dir) {
+        case 0: c.x += v; break;
+        case 1: c.y += v; break;
+        case 2: c.z += v; break;
+        case 3: c.w += v; break;
+        case 4: c8.x += v; break;
+        case 5: c8.y += v; break;
+        case 6: c8.z += v; break;
+        default: c8.w += v; break;
+    }
+}
+
+// Tile id and in-tile index of the node at tile-local (x, y) one step outside
+// the tile (or inside it: returns t).
+__device__ __forceinline__ int tile_of8(const TileInfo &ti, const int4 &n8, int t, int x, int y) {
+    const int sx = x < 0 ? -1 : (x >= TW ? 1 : 0), sy = y < 0 ? -1 : (y >= TH ? 1 : 0);
+    if (sx == 0 && sy == 0) return t;
+    if (sy == 0) return sx < 0 ? ti.nbr[0] : ti.nbr[1];
+    if (sx == 0) return sy < 0 ? ti.nbr[2] : ti.nbr[3];
+    if (sy < 0) return sx < 0 ? n8.x : n8.y;
+    return sx < 0 ? n8.z : n8.w;
+}
+
+// Load the halo ring (sides and corners) of array `a` of tile t's neighbours
+// into S[LH][LW] (and S2 if given); `init` reads the initial BFS distance (0
+// where excess < 0) instead of the stored distance.
+__device__ __forceinline__ void load_halo8(const KParams &p, const TileInfo &ti, const int4 &n8, int tid,
+                                           int (*S)[LW], int (*S2)[LW], bool init) {
+    // Ring positions: 2 * TW (top, bottom rows) + 2 * TH (left, right columns) + 4 corners.
+    constexpr int RING = 2 * TW + 2 * TH + 4;
+    for (int r = tid; r < RING; r += NT) {
+        int lx, ly;
+        if (r < TW) { lx = r + 1; ly = 0; }
+        else if (r < 2 * TW) { lx = r - TW + 1; ly = TH + 1; }
+        else if (r < 2 * TW + TH) { lx = 0; ly = r - 2 * TW + 1; }
+        else if (r < 2 * TW + 2 * TH) { lx = TW + 1; ly = r - 2 * TW - TH + 1; }
+        else { const int c = r - 2 * TW - 2 * TH; lx = (c & 1) ? TW + 1 : 0; ly = (c & 2) ? TH + 1 : 0; }
+        const int x = lx - 1, y = ly - 1;
+        const int tn = tile_of8(ti, n8, -1, x, y);
+        int v = DINF;
+        if (tn >= 0) {
+            const size_t gn = (size_t)tn * TN + ((y + TH) % TH) * TW + (x + TW) % TW;
+            v = init ? (p.exc[gn] < 0 ? 0 : DINF) : p.dist[gn];
+        }
+        S[ly][lx] = v;
+        if (S2) S2[ly][lx] = v;
+    }
+}
+
+__global__ __launch_bounds__(NT) void k_pr8(KParams p, int parity) {
+    __shared__ int Hs[2][LH][LW];
+    __shared__ int Fs[8][LH][LW];
+    __shared__ int s_item;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int qlen = p.ctl->q[parity].qlen_pr;
+    if ((int)blockIdx.x >= qlen) return;
+    for (int i = tid; i < 8 * LH * LW; i += NT) (&Fs[0][0][0])[i] = 0;  // halo cells stay 0
+    for (;;) {
+        if (tid == 0) {
+            const int q = atomicAdd(&p.ctl->q[parity].head_pr, 1);
+            s_item = q < qlen ? p.queue_pr[q] : -1;
+        }
+        __syncthreads();
+        const int t = s_item;
+        if (t < 0) break;
+        const TileInfo ti = p.tiles[t];
+        const int4 n8 = p.nbr8[t];
+        const int dthr = p.desc[ti.inst].n_interior;
+        const size_t base = (size_t)t * TN;
+        const int lx = lane + 1;
+        int4 c[RPT], c8[RPT];
+        int e[RPT], d[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int y = wave * RPT + k;
+            const size_t g = base + y * TW + lane;
+            c[k] = p.caps[g];
+            c8[k] = p.caps8[g];
+            e[k] = p.exc[g];
+            d[k] = p.dist[g];
+            Hs[0][y + 1][lx] = d[k];
+            Hs[1][y + 1][lx] = d[k];
+        }
+        load_halo8(p, ti, n8, tid, Hs[0], Hs[1], false);
+        __syncthreads();
+
+        // Flow pushed out of the tile over the pass, per boundary node and arc.
+        int acc[RPT][8];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+#pragma unroll
+            for (int dir = 0; dir < 8; ++dir) acc[k][dir] = 0;
+        int cur = 0, it = 0;
+        while (it < p.inner_iters) {
+            ++it;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int ly = wave * RPT + k + 1;
+                int ek = e[k];
+                const bool can = ek > 0 && d[k] < DINF;
+                const int hd = d[k] - 1;
+#pragma unroll
+                for (int dir = 0; dir < 8; ++dir) {
+                    int f = 0;
+                    if (can && ek > 0) {
+                        const int cv = cap8(c[k], c8[k], dir);
+                        if (cv > 0 && Hs[cur][ly + DY8[dir]][lx + DX8[dir]] == hd) {
+                            f = min(ek, cv);
+                            ek -= f;
+                            add_cap8(c[k], c8[k], dir, -f);
+                        }
+                    }
+                    Fs[dir][ly][lx] = f;
+                    acc[k][dir] += f;
+                }
+                e[k] = ek;
+            }
+            __syncthreads();
+            // Receive along every arc, then relabel nodes left with excess.
+            int act = 0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int ly = wave * RPT + k + 1;
+#pragma unroll
+                for (int dir = 0; dir < 8; ++dir) {
+                    const int in = Fs[OPP8[dir]][ly + DY8[dir]][lx + DX8[dir]];
+                    e[k] += in;
+                    add_cap8(c[k], c8[k], dir, in);
+                }
+                if (e[k] > 0 && d[k] < DINF) {
+                    int m = DINF;
+#pragma unroll
+                    for (int dir = 0; dir < 8; ++dir)
+                        if (cap8(c[k], c8[k], dir) > 0) m = min(m, Hs[cur][ly + DY8[dir]][lx + DX8[dir]]);
+                    int nd = m >= DINF ? DINF : m + 1;
+                    if (nd >= dthr) nd = DINF;
+                    if (nd > d[k]) d[k] = nd;
+                    act |= d[k] < DINF;
+                }
+                Hs[cur ^ 1][ly][lx] = d[k];
+            }
+            cur ^= 1;
+            if (!__syncthreads_or(act)) break;
+        }
+
+        int mine = 0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int y = wave * RPT + k;
+            const size_t g = base + y * TW + lane;
+            p.caps[g] = c[k];
+            p.caps8[g] = c8[k];
+            p.exc[g] = e[k];
+            p.dist[g] = d[k];
+            mine |= e[k] > 0 && d[k] < DINF;
+        }
+        // Flow into nodes of other tiles (none of which runs in this pass):
+        // their excess and the reverse residual arc, by int32 atomics (two
+        // tiles of one colour can feed the same node).
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int y = wave * RPT + k;
+#pragma unroll
+            for (int dir = 0; dir < 8; ++dir) {
+                const int a = acc[k][dir];
+                const int x2 = lane + DX8[dir], y2 = y + DY8[dir];
+                if (a == 0 || (x2 >= 0 && x2 < TW && y2 >= 0 && y2 < TH)) continue;
+                const int tn = tile_of8(ti, n8, t, x2, y2);
+                if (tn < 0) continue;
+                const size_t gn = (size_t)tn * TN + ((y2 + TH) % TH) * TW + (x2 + TW) % TW;
+                atomicAdd(&p.exc[gn], a);
+                const int o = OPP8[dir];
+                if (o < 4)
+                    atomicAdd(reinterpret_cast<int *>(&p.caps[gn]) + o, a);
+                else
+                    atomicAdd(reinterpret_cast<int *>(&p.caps8[gn]) + (o - 4), a);
+                set_active(p, tn, 1);
+            }
+        }
+        const int any_active = __syncthreads_or(mine);
+        if (tid == 0) {
+            set_active(p, t, any_active);
+            if (p.profile) atomic_add64(&p.ctl->pr_iters, it);
+        }
+        __syncthreads();
+    }
+}
+
+// Global relabel and finalisation on the 8-neighbourhood (as k_gr).
+__global__ __launch_bounds__(NT) void k_gr8(KParams p, int parity) {
+    __shared__ int Ds[LH][LW];
+    __shared__ int s_item;
+    __shared__ int64_t red[NT / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lx = lane + 1;
+    const int qlen = p.ctl->q[parity].qlen_gr;
+    if ((int)blockIdx.x >= qlen) return;
+    for (;;) {
+        if (tid == 0) {
+            const int q = atomicAdd(&p.ctl->q[parity].head_gr, 1);
+            s_item = q < qlen ? p.queue_gr[q] : -1;
+        }
+        __syncthreads();
+        const int item = s_item;
+        if (item < 0) break;
+        const int op = item & 7, t = item >> 3;
+        const TileInfo ti = p.tiles[t];
+        const int4 n8 = p.nbr8[t];
+        const InstDesc D = p.desc[ti.inst];
+        const size_t base = (size_t)t * TN;
+
+        if (op == OP_FIN) {
+            int64_t sx = 0;
+            int32_t *lab = D.label_out;
+            const int W = D.W, H = D.H;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int y = wave * RPT + k;
+                const int X = ti.tx * TW + lane, Y = ti.ty * TH + y;
+                if (X < W && Y < H) {
+                    const size_t g = base + y * TW + lane;
+                    const bool in = X > 0 && Y > 0 && X < W - 1 && Y < H - 1;
+                    lab[(size_t)Y * W + X] = (in && p.dist[g] >= DINF) ? 1 : 0;
+                    if (in) sx += max(p.exc[g], 0);
+                }
+            }
+            const int64_t tot = block_sum64(sx, red);
+            if (tid == 0) p.tsum[2 * (size_t)t + 1] = tot;
+            __syncthreads();
+            continue;
+        }
+
+        const bool init = op == OP_GR_INIT;
+        int dv[RPT], orig[RPT], m[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int y = wave * RPT + k;
+            const size_t g = base + y * TW + lane;
+            int mk, d0;
+            if (init) {
+                const int4 cc = p.caps[g], c8 = p.caps8[g];
+                const int ee = p.exc[g];
+                mk = (cc.x > 0) | ((cc.y > 0) << 1) | ((cc.z > 0) << 2) | ((cc.w > 0) << 3) | ((ee > 0) << 4) |
+                     ((ee < 0) << 5);
+                const int m8 = (c8.x > 0) | ((c8.y > 0) << 1) | ((c8.z > 0) << 2) | ((c8.w > 0) << 3);
+                p.mask[g] = (uint8_t)mk;
+                p.mask8[g] = (uint8_t)m8;
+                mk |= m8 << 8;
+                d0 = ee < 0 ? 0 : DINF;
+            } else {
+                mk = p.mask[g] | (p.mask8[g] << 8);
+                d0 = p.dist[g];
+            }
+            m[k] = mk;
+            dv[k] = orig[k] = d0;
+            Ds[y + 1][lx] = d0;
+        }
+        // Halo (sides and corners): upper bounds of the BFS distance, so stale
+        // reads only delay convergence.
+        load_halo8(p, ti, n8, tid, Ds, nullptr, init);
+        __syncthreads();
+
+        bool conv = false;
+        for (int it = 0; it < GR_MAX_ITERS; ++it) {
+            int ch = 0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                const int mk = m[k];
+                if ((mk & 0xF0F) && dv[k] > 0) {
+                    const int ly = wave * RPT + k + 1;
+                    int best = dv[k];
+#pragma unroll
+                    for (int dir = 0; dir < 8; ++dir) {
+                        const int bit = dir < 4 ? (1 << dir) : (1 << (dir + 4));
+                        if (mk & bit) best = min(best, Ds[ly + DY8[dir]][lx + DX8[dir]] + 1);
+                    }
+                    if (best < dv[k]) {
+                        dv[k] = best;
+                        Ds[ly][lx] = best;
+                        ch = 1;
+                    }
+                }
+            }
+            if (!__syncthreads_or(ch)) {
+                conv = true;
+                break;
+            }
+        }
+
+        int chL = 0, chR = 0, chU = 0, chD = 0, cnt = 0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            const int y = wave * RPT + k;
+            const size_t g = base + y * TW + lane;
+            const int changed = dv[k] != orig[k];
+            if (changed || init) p.dist[g] = dv[k];
+            if (lane == 0) chL |= changed;
+            if (lane == TW - 1) chR |= changed;
+            if (y == 0) chU |= changed;
+            if (y == TH - 1) chD |= changed;
+            // A changed corner node is in the halo of the diagonal tile.
+            if (changed && (lane == 0 || lane == TW - 1) && (y == 0 || y == TH - 1)) {
+                const int tn = y == 0 ? (lane == 0 ? n8.x : n8.y) : (lane == 0 ? n8.z : n8.w);
+                if (tn >= 0) mark_dirty(p, tn);
+            }
             cnt += (m[k] & 16) && dv[k] < DINF;
         }
         chL = __syncthreads_or(chL);
@@ -1541,7 +1927,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
                     if (X == D.W - 2) c.y = 0;
                     if (Y == 1) c.z = 0;
                     if (Y == D.H - 2) c.w = 0;
-                    bad |= (vp[k] < 0) | (vn[k] < 0) | (c.x < 0) | (c.y < 0) | (c.z < 0) | (c.w < 0);
+                    bad |= tlinks_bad(vp[k], vn[k]) | (c.x < 0) | (c.y < 0) | (c.z < 0) | (c.w < 0);
                     e = vp[k] - vn[k];
                     sp += vp[k];
                     sx += max(e, 0);
@@ -1641,7 +2027,8 @@ __global__ __launch_bounds__(NT) void k_edit_masks(KParams p, int slot, const in
 }
 
 // Residual state in image layout (certificates and debugging).
-__global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *exc_out, int32_t *dist_out) {
+__global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *exc_out, int32_t *dist_out,
+                             int32_t *caps8_out) {
     const InstDesc D = p.desc[slot];
     const int64_t n = (int64_t)D.W * D.H;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1657,6 +2044,13 @@ __global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *ex
         }
         if (exc_out) exc_out[i] = p.exc[g];
         if (dist_out) dist_out[i] = p.dist[g];
+        if (caps8_out) {
+            const int4 c8 = p.caps8[g];
+            caps8_out[4 * i + 0] = c8.x;
+            caps8_out[4 * i + 1] = c8.y;
+            caps8_out[4 * i + 2] = c8.z;
+            caps8_out[4 * i + 3] = c8.w;
+        }
     }
 }
 
@@ -1689,6 +2083,8 @@ void launch_pr(const KParams &p, int grid, int parity, hipStream_t s) {
     if (p.variant)
         hipLaunchKernelGGL(k_prw, dim3(grid), dim3(64), 0, s, p, parity, (const int32_t *)p.queue_pr,
                            (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
+    else if (p.nbhd == 8)
+        hipLaunchKernelGGL(k_pr8, dim3(grid), dim3(NT), 0, s, p, parity);
     else
         hipLaunchKernelGGL(k_pr, dim3(grid), dim3(NT), 0, s, p, parity);
 }
@@ -1708,7 +2104,10 @@ bool gr_block_supported(int bx, int by) {
 
 void launch_gr(const KParams &p, int grid, int parity, hipStream_t s) {
     if (!p.variant) {
-        hipLaunchKernelGGL(k_gr, dim3(grid), dim3(NT), 0, s, p, parity);
+        if (p.nbhd == 8)
+            hipLaunchKernelGGL(k_gr8, dim3(grid), dim3(NT), 0, s, p, parity);
+        else
+            hipLaunchKernelGGL(k_gr, dim3(grid), dim3(NT), 0, s, p, parity);
         return;
     }
     switch (p.gbx * 100 + p.gby) {
@@ -1743,8 +2142,8 @@ void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_str
 }
 
 void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, int32_t *d_dist,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_read_state, dim3(1024), dim3(256), 0, s, p, slot, d_caps, d_exc, d_dist);
+                       int32_t *d_caps8, hipStream_t s) {
+    hipLaunchKernelGGL(k_read_state, dim3(1024), dim3(256), 0, s, p, slot, d_caps, d_exc, d_dist, d_caps8);
 }
 
 }  // namespace gcfg

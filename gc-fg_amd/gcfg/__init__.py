@@ -29,6 +29,7 @@ GCFG_ERR_STATE = -4
 GCFG_ERR_NOT_CONVERGED = -5
 GCFG_ERR_BUDGET = -6
 SEED_NONE, SEED_FG, SEED_BG = 0, 1, 2
+DT_MODEL_HISTOGRAM, DT_MODEL_KDE = 0, 1
 
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -319,10 +320,14 @@ def mix_seed(seed: int, stream: int) -> int:
 class Graph:
     """The GC_* drop-in: one W x H grid graph on the device."""
 
-    def __init__(self, width: int, height: int, options: Options | None = None):
+    def __init__(self, width: int, height: int, options: Options | None = None, neighborhood: int = 4):
         self.width, self.height = int(width), int(height)
+        if neighborhood not in (4, 8):
+            raise ValueError("neighborhood must be 4 or 8")
+        self.neighborhood = neighborhood
         h = ctypes.c_void_p()
-        _check(lib().gcfg_graph_create(self.width, self.height, ctypes.byref(h)), "gcfg_graph_create")
+        create = lib().gcfg_graph_create_n8 if neighborhood == 8 else lib().gcfg_graph_create
+        _check(create(self.width, self.height, ctypes.byref(h)), "gcfg_graph_create")
         self.h = h.value
         self._keep = {}
         if options is not None:
@@ -348,6 +353,13 @@ class Graph:
     def set_edges(self, up, down, left, right):
         ptrs = [_VP(self._dev(n, a)) for n, a in (("up", up), ("down", down), ("left", left), ("right", right))]
         _check(lib().gcfg_graph_set_edges(_VP(self.h), *ptrs), "gcfg_graph_set_edges")
+
+    def set_edges8(self, up, down, left, right, upleft, upright, downleft, downright):
+        """GC_SetEdges of an 8-neighbourhood graph (GraphCut.cu:271-289)."""
+        names = ("up", "down", "left", "right", "upleft", "upright", "downleft", "downright")
+        arrs = (up, down, left, right, upleft, upright, downleft, downright)
+        ptrs = [_VP(self._dev(n, a)) for n, a in zip(names, arrs)]
+        _check(lib().gcfg_graph_set_edges8(_VP(self.h), *ptrs), "gcfg_graph_set_edges8")
 
     def set_penalty(self, penalty: int):
         _check(lib().gcfg_graph_set_penalty(_VP(self.h), int(penalty)), "gcfg_graph_set_penalty")
@@ -386,6 +398,12 @@ class Graph:
                "gcfg_graph_read_state")
         return caps, exc
 
+    def read_diag_state(self):
+        """Diagonal residual capacities (up-left, up-right, down-left, down-right) of an 8-neighbourhood graph."""
+        caps8 = np.empty((self.height, self.width, 4), np.int32)
+        _check(lib().gcfg_graph_read_diag_state(_VP(self.h), _VP(caps8.ctypes.data)), "gcfg_graph_read_diag_state")
+        return caps8
+
     def set_stream(self, stream_ptr: int):
         """Order this handle's work against a caller HIP stream (0: none)."""
         _check(lib().gcfg_graph_set_stream(_VP(self.h), _VP(stream_ptr or None)), "gcfg_graph_set_stream")
@@ -410,6 +428,25 @@ def solve(width, height, pos, neg, edges=None, penalty=0, options: Options | Non
         g.set_dataterms(pos, neg)
         if edges is not None:
             g.set_edges(*edges)
+        else:
+            g.set_penalty(penalty)
+        g.build()
+        label, st = g.optimize()
+        return int(st.flow), label, st
+    finally:
+        g.close()
+
+
+def solve8(width, height, pos, neg, dirs8=None, penalty=0, options: Options | None = None):
+    """One plain 8-neighbourhood solve. ``dirs8`` = (left, right, up, down,
+    upleft, upright, downleft, downright) arrays or None for a uniform
+    ``penalty``. Returns (flow, labels[H, W], stats)."""
+    g = Graph(width, height, options, neighborhood=8)
+    try:
+        g.set_dataterms(pos, neg)
+        if dirs8 is not None:
+            left, right, up, down, ul, ur, dl, dr = dirs8
+            g.set_edges8(up, down, left, right, ul, ur, dl, dr)
         else:
             g.set_penalty(penalty)
         g.build()
@@ -451,6 +488,23 @@ class DataTerm:
 
     def compute(self):
         _check(lib().gcfg_dt_compute(_VP(self.h)), "gcfg_dt_compute")
+
+    def set_model(self, model: int):
+        """DT_MODEL_HISTOGRAM (default) or DT_MODEL_KDE (the reference's kernel density)."""
+        _check(lib().gcfg_dt_set_model(_VP(self.h), int(model)), "gcfg_dt_set_model")
+
+    def set_sigma(self, sigma: int):
+        """setSigmaGaussian (DataTerm.cpp:97-103)."""
+        _check(lib().gcfg_dt_set_sigma(_VP(self.h), int(sigma)), "gcfg_dt_set_sigma")
+
+    def kde_select(self, x: int, y: int):
+        """selectOnMap (DataTerm.cpp:105-127): the 5 x 5 square's colours join the density."""
+        _check(lib().gcfg_dt_kde_select(_VP(self.h), int(x), int(y)), "gcfg_dt_kde_select")
+
+    def kde_count(self) -> int:
+        n = ctypes.c_int32(0)
+        _check(lib().gcfg_dt_kde_count(_VP(self.h), ctypes.byref(n)), "gcfg_dt_kde_count")
+        return int(n.value)
 
     def link_ptrs(self):
         ps = [ctypes.c_void_p() for _ in range(6)]

@@ -2,23 +2,28 @@
  * DataTerm.h -- drop-in C++ host API of the reference colour data term
  * (DataTerm.h:7-38, DataTerm.cpp:12-165), backed by libgcfg.so.
  *
- * Same namespace, function names and single-instance semantics. The colour
- * model behind it is the histogram model of gcfg.h (16 bins per channel and
- * class, +1 smoothing, pos/neg = llround(64 * -ln p), contrast n-links with
- * lambda 50) instead of the reference's 256^3 Gaussian kernel density:
- *  - selectOnMap(x, y, sigma) marks the 5x5 square at (x..x+4, y..y+4) as
- *    foreground seeds (DataTerm.cpp:105-127 selects the same square);
- *    selectRect adds a rectangle of either class;
- *  - getDataTermsMixture() computes histograms, t-links and n-links;
+ * Same namespace, function names and single-instance semantics. Two colour
+ * models (setModel):
+ *  - GCFG_DT_MODEL_HISTOGRAM (default): 16 bins per channel and class, +1
+ *    smoothing, pos/neg = llround(64 * -ln p), contrast n-links with lambda 50;
+ *    selectOnMap(x, y, sigma) marks the 5x5 square at (x..x+4, y..y+4) as
+ *    foreground seeds, selectRect adds a rectangle of either class;
+ *  - GCFG_DT_MODEL_KDE: the reference's Gaussian kernel density
+ *    (DataTermKernels.cu:19-150): selectOnMap adds the colours of the same
+ *    5x5 square to the density, setSigmaGaussian sets its width, and
+ *    pos = (int)(300000 p), neg = (int)(300000 (1 - p)).
+ *  - getDataTermsMixture() computes the t-links (and contrast n-links);
  *  - getPos()/getNeg() return device int* t-links for GC_SetDataterms,
  *    getUp/Down/Left/Right device n-links for GC_SetEdges;
- *  - setSigmaGaussian is accepted and ignored (no kernel density);
- *  - setSelection/drawSelection keep the reference's display behaviour
+ *  - selectOnMap marks the square in the selection map, and
+ *    setSelection/drawSelection keep the reference's display behaviour
  *    (selected pixels painted (0, 0, 255) in BGR order, DataTermKernels.cu:96-113).
  * Errors print a message and abort, as the reference's CUDA_SAFE_CALL does.
  */
 #ifndef DATATERM_H
 #define DATATERM_H
+
+#include "gcfg.h"
 
 namespace DataTerm {
 int *getPos();
@@ -40,6 +45,7 @@ int *getLeft();
 int *getRight();
 void selectRect(int x0, int y0, int w, int h, int cls); /* cls: 1 foreground, 2 background */
 void setLambda(int lambda);
+void setModel(int model); /* GCFG_DT_MODEL_HISTOGRAM or GCFG_DT_MODEL_KDE */
 }  // namespace DataTerm
 
 #endif

@@ -34,6 +34,11 @@ enum {
 };
 
 enum { GCFG_SEED_NONE = 0, GCFG_SEED_FG = 1, GCFG_SEED_BG = 2 };
+/* Colour models of the data term: 16-bin seed histograms with -log-likelihood
+ * t-links (default), or the reference's Gaussian kernel density over selected
+ * colours with pos = (int)(300000 p), neg = (int)(300000 (1 - p))
+ * (DataTermKernels.cu:61-83,116-150). */
+enum { GCFG_DT_MODEL_HISTOGRAM = 0, GCFG_DT_MODEL_KDE = 1 };
 
 typedef struct gcfg_options {
     int32_t sweeps_per_gr;   /* colour-pass pairs of tile push-relabel between global relabels */
@@ -42,7 +47,8 @@ typedef struct gcfg_options {
     int32_t use_graph;       /* capture a chunk of steps in a hipGraph and replay it */
     int32_t max_steps;       /* 0 = unbounded; else GCFG_ERR_NOT_CONVERGED past this */
     int32_t verbose;         /* 1: print the reference's "pen"/energy lines */
-    int32_t profile;         /* 1: time every push/relabel and global-relabel launch with HIP events */
+    int32_t profile;         /* 1: time every push/relabel and global-relabel launch with HIP events;
+                                2: also count inner iterations (pr_iters, gr_iters; adds atomics) */
     int32_t kernels;         /* 1: wavefront-per-tile kernels (default), 0: workgroup-per-tile LDS kernels */
     int32_t gr_rounds;       /* max tile-local relaxation rounds per global-relabel sweep (then re-queued) */
     int32_t gr_block_w;      /* global-relabel block in tiles (one workgroup, BFS exchanged through LDS): */
@@ -58,14 +64,14 @@ typedef struct gcfg_stats {
     int64_t flow;        /* exact max-flow value over the interior */
     int64_t pr_tiles;    /* tile push-relabel passes executed */
     int64_t gr_tiles;    /* tile BFS sweeps executed (global relabel) */
-    int64_t pr_iters;    /* in-LDS push/relabel iterations summed over tiles */
+    int64_t pr_iters;    /* in-LDS push/relabel iterations summed over tiles (profile 2) */
     int32_t steps;       /* scheduler steps until done */
     int32_t gr_rounds;   /* global relabels started */
     double ms;           /* wall time of the call */
     /* filled when gcfg_options.profile is set (HIP events around each launch) */
     double pr_kernel_ms, gr_kernel_ms, sched_kernel_ms;
     int64_t launches;    /* steps launched (each = sched + pr + gr) */
-    int64_t gr_iters;    /* tile-local relaxation rounds summed over global-relabel tiles */
+    int64_t gr_iters;    /* tile-local relaxation rounds summed over global-relabel tiles (profile 2) */
     int64_t gr_init_tiles; /* of gr_tiles, the initial sweeps of a global relabel (the rest relax) */
     int64_t gr_exact_tiles; /* of gr_tiles, those of exact (uncapped) rounds, which also write labels */
     int64_t fin_items;     /* FIN items: blocks left unreachable (wavefront kernels) or every tile */
@@ -81,11 +87,26 @@ int gcfg_device_info(char *name, size_t name_len, int32_t *cu_count, size_t *hbm
 typedef struct gcfg_graph gcfg_graph;
 /* GC_Init allocation part, GraphCut.cu:187-248. */
 int gcfg_graph_create(int32_t width, int32_t height, gcfg_graph **out);
+/* GC_Init with NEIGHBORHOOD == 8 (GraphCut.cu:187-194,223-232; GraphCut.h:22,65-70):
+ * an 8-connected graph. Its n-links come from gcfg_graph_set_edges8 or
+ * gcfg_graph_set_penalty (all eight directions); every other call is as for
+ * the 4-neighbourhood. Solved by the workgroup-per-tile kernels
+ * (gcfg_options.kernels is ignored). */
+int gcfg_graph_create_n8(int32_t width, int32_t height, gcfg_graph **out);
+/* 4 or 8. */
+int gcfg_graph_neighborhood(gcfg_graph *g, int32_t *nbhd);
 /* GC_SetDataterms, GraphCut.cu:265-269 (borrowed device pointers, W*H each). */
 int gcfg_graph_set_dataterms(gcfg_graph *g, const int32_t *d_pos, const int32_t *d_neg);
 /* GC_SetEdges, GraphCut.cu:271-289 (borrowed device pointers, W*H each). */
 int gcfg_graph_set_edges(gcfg_graph *g, const int32_t *d_up, const int32_t *d_down, const int32_t *d_left,
                          const int32_t *d_right);
+/* GC_SetEdges with NEIGHBORHOOD == 8, GraphCut.cu:271-289: the four side planes
+ * and the diagonals, upleft[x+y*W] = capacity from (x,y) to (x-1,y-1),
+ * upright to (x+1,y-1), downleft to (x-1,y+1), downright to (x+1,y+1)
+ * (borrowed device pointers, W*H each; 8-neighbourhood graphs only). */
+int gcfg_graph_set_edges8(gcfg_graph *g, const int32_t *d_up, const int32_t *d_down, const int32_t *d_left,
+                          const int32_t *d_right, const int32_t *d_upleft, const int32_t *d_upright,
+                          const int32_t *d_downleft, const int32_t *d_downright);
 /* GC_SetPenalty, GraphCut.cu:296-299 (uniform n-link capacity). */
 int gcfg_graph_set_penalty(gcfg_graph *g, int32_t penalty);
 /* Order the handle's work after everything queued on a caller HIP stream
@@ -105,6 +126,10 @@ int gcfg_graph_optimize_device(gcfg_graph *g, int32_t *d_label, gcfg_stats *stat
  * residual capacity from (x,y) towards left/right/up/down (k = 0..3),
  * h_excess[x+y*W] the excess (negative = remaining sink capacity). */
 int gcfg_graph_read_state(gcfg_graph *g, int32_t *h_caps, int32_t *h_excess);
+/* Diagonal residual capacities of an 8-neighbourhood graph after optimize:
+ * h_caps8[4*(x+y*W)+k], k = 0..3 towards up-left, up-right, down-left,
+ * down-right. Verification output with no reference counterpart. */
+int gcfg_graph_read_diag_state(gcfg_graph *g, int32_t *h_caps8);
 /* Distance labels (W*H, image layout) after a solve: the exact BFS distance
  * of every interior node to a node with excess < 0 along residual arcs in the
  * final residual graph; 0x3fffffff where no such path exists. No counterpart
@@ -142,8 +167,23 @@ int gcfg_dt_get_links(gcfg_dataterm *dt, int32_t **d_pos, int32_t **d_neg, int32
 int gcfg_dt_get_histograms(gcfg_dataterm *dt, int32_t *h_hist96, int32_t *h_counts2);
 /* Exact sum of squared 4-neighbour colour differences, pair count, beta. */
 int gcfg_dt_get_beta(gcfg_dataterm *dt, int64_t *sum, int64_t *pairs, double *beta);
-/* Pre-quantisation values computed on the device in f64 (W*H each, host out). */
+/* Pre-quantisation values computed on the device in f64 (W*H each, host out).
+ * Kernel-density model: h_tpos = 300000 p, h_tneg = 300000 (1 - p). */
 int gcfg_dt_unquantized(gcfg_dataterm *dt, double *h_tpos, double *h_tneg, double *h_right, double *h_down);
+/* Select the colour model (GCFG_DT_MODEL_*) used by gcfg_dt_compute. No
+ * counterpart in the reference, whose model is the kernel density only. */
+int gcfg_dt_set_model(gcfg_dataterm *dt, int32_t model);
+/* setSigmaGaussian, DataTerm.cpp:97-103 / K_InitGaussian, DataTermKernels.cu:19-31:
+ * kernel width (1..1000, default 5) of the density; applies to later computes. */
+int gcfg_dt_set_sigma(gcfg_dataterm *dt, int32_t sigma);
+/* selectOnMap, DataTerm.cpp:105-127 / K_SelectAndGauss, DataTermKernels.cu:116-150:
+ * adds the colours of the 5 x 5 pixels (x..x+4, y..y+4) of the current image
+ * to the kernel-density seed list (pixels outside the image are skipped; a
+ * pixel selected twice counts twice). Later image changes do not change the
+ * captured colours. At most 65536 seed colours; gcfg_dt_reset clears them. */
+int gcfg_dt_kde_select(gcfg_dataterm *dt, int32_t x, int32_t y);
+/* Number of seed colours in the kernel-density list. */
+int gcfg_dt_kde_count(gcfg_dataterm *dt, int32_t *count);
 
 /* ---- Batched solve: many instances of uneven size in one engine ---- */
 typedef struct gcfg_instance {

@@ -12,9 +12,14 @@ Functions (numpy in / numpy out, ctypes over ``oracle/liboracle.so``):
   only, arcs into the border zeroed); labels as ``GraphCutKernels.cu:228-351``
   with ``SPREAD_ZEROS`` (``GraphCut.h:37``).
 * :func:`cut_value` -- cut value of a labelling on the same graph.
+* :func:`maxflow8` / :func:`cut_value8` -- the same on the 8-neighbourhood
+  grid (``NEIGHBORHOOD == 8``, ``GraphCut.h:22,65-70``).
 * :func:`dataterm` -- per-pixel double-precision colour data term (16-bin
   seed histograms, ``round(64 * -ln p)`` t-links, ``round(lambda *
   exp(-d^2 / 2 beta))`` n-links), the stage that sits behind ``DataTerm.h``.
+* :func:`dataterm_kde` -- the reference's own Gaussian kernel-density colour
+  model (``DataTermKernels.cu:19-150``) as a per-pixel double sum over the
+  selected seed colours.
 * :func:`refpr` -- CPU restatement of the reference push-relabel schedule
   (``GraphCut.cu:385-534``), used as the timed CPU baseline.
 """
@@ -59,6 +64,17 @@ def _load():
         lib.oracle_dataterm.argtypes = [ctypes.c_int, ctypes.c_int, _U8P, _U8P, ctypes.c_int32, _I32P, _I32P,
                                         _I64P, _I64P, _F64P, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P, _F64P,
                                         _F64P, _F64P, _F64P]
+        lib.oracle_maxflow_grid8.restype = ctypes.c_int
+        lib.oracle_maxflow_grid8.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, ctypes.c_void_p,
+                                             ctypes.c_int32, _I32P, _I64P]
+        lib.oracle_cut_value8.restype = ctypes.c_int64
+        lib.oracle_cut_value8.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, ctypes.c_void_p,
+                                          ctypes.c_int32, _I32P]
+        lib.oracle_kde_gauss.restype = None
+        lib.oracle_kde_gauss.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
+        lib.oracle_dataterm_kde.restype = ctypes.c_int
+        lib.oracle_dataterm_kde.argtypes = [ctypes.c_int, ctypes.c_int, _U8P, ctypes.c_int, _U8P, ctypes.c_int,
+                                            _I32P, _I32P, _F64P, _F64P, _F64P]
         lib.oracle_refpr_solve.restype = ctypes.c_int
         lib.oracle_refpr_solve.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P,
                                            ctypes.c_int32, ctypes.c_int, _I32P, _I64P, _I64P]
@@ -111,6 +127,46 @@ def cut_value(width, height, pos, neg, label, edges=None, penalty=0) -> int:
                                     _ptr(label, ctypes.c_int32)))
 
 
+DIRS8 = ("left", "right", "up", "down", "upleft", "upright", "downleft", "downright")
+
+
+def _planes8(dirs8):
+    """(keep-alive arrays, void* to an array of 8 plane pointers) or (None, None)."""
+    if dirs8 is None:
+        return None, None
+    arrs = [_i32(a).ravel() for a in dirs8]
+    assert len(arrs) == 8
+    ptrs = (ctypes.c_void_p * 8)(*[a.ctypes.data for a in arrs])
+    return (arrs, ptrs), ctypes.cast(ptrs, ctypes.c_void_p)
+
+
+def maxflow8(width, height, pos, neg, dirs8=None, penalty=0):
+    """Exact max-flow on the 8-neighbourhood grid. ``dirs8`` = eight arrays in
+    the order of DIRS8 or None for a uniform ``penalty``. Returns (flow, label)."""
+    lib = _load()
+    pos, neg = _i32(pos).ravel(), _i32(neg).ravel()
+    keep, pp = _planes8(dirs8)
+    label = np.zeros(width * height, np.int32)
+    flow = ctypes.c_int64(0)
+    rc = lib.oracle_maxflow_grid8(width, height, _ptr(pos, ctypes.c_int32), _ptr(neg, ctypes.c_int32), pp,
+                                  int(penalty), _ptr(label, ctypes.c_int32), ctypes.byref(flow))
+    del keep
+    if rc != 0:
+        raise RuntimeError(f"oracle_maxflow_grid8 failed: {rc}")
+    return int(flow.value), label.reshape(height, width)
+
+
+def cut_value8(width, height, pos, neg, label, dirs8=None, penalty=0) -> int:
+    lib = _load()
+    pos, neg = _i32(pos).ravel(), _i32(neg).ravel()
+    keep, pp = _planes8(dirs8)
+    label = _i32(label).ravel()
+    v = int(lib.oracle_cut_value8(width, height, _ptr(pos, ctypes.c_int32), _ptr(neg, ctypes.c_int32), pp,
+                                  int(penalty), _ptr(label, ctypes.c_int32)))
+    del keep
+    return v
+
+
 def dataterm(width, height, rgb, seeds, lam=50):
     """Per-pixel double-precision data term. Returns a dict of numpy arrays."""
     lib = _load()
@@ -138,6 +194,40 @@ def dataterm(width, height, rgb, seeds, lam=50):
     for k in ("pos", "neg", "up", "down", "left", "right", "tpos_f", "tneg_f", "right_f", "down_f"):
         out[k] = out[k].reshape(height, width)
     return out
+
+
+def kde_gauss(sigma):
+    """The 101 kernel values of the reference's K_InitGaussian (float32)."""
+    g = np.zeros(101, np.float32)
+    _load().oracle_kde_gauss(int(sigma), g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return g
+
+
+def kde_select_colours(rgb, width, height, x, y, size=5):
+    """Seed colours that selectOnMap(x, y) adds (DataTerm.cpp:111-123: x
+    offset outer, y offset inner), pixels outside the image skipped."""
+    img = np.asarray(rgb, np.uint8).reshape(height, width, 3)
+    out = [img[yy, xx] for xx in range(x, x + size) for yy in range(y, y + size)
+           if 0 <= xx < width and 0 <= yy < height]
+    return np.array(out, np.uint8).reshape(-1, 3)
+
+
+def dataterm_kde(width, height, rgb, seed_colours, sigma=5):
+    """Kernel-density data term of the reference over the seed colours (N x 3
+    u8, selection order). Returns dict with pos, neg (int32), p, tpos_f, tneg_f."""
+    lib = _load()
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint8).ravel()
+    sc = np.ascontiguousarray(seed_colours, dtype=np.uint8).reshape(-1, 3)
+    n = width * height
+    out = {"pos": np.zeros(n, np.int32), "neg": np.zeros(n, np.int32), "tpos_f": np.zeros(n, np.float64),
+           "tneg_f": np.zeros(n, np.float64), "p": np.zeros(n, np.float64)}
+    rc = lib.oracle_dataterm_kde(width, height, _ptr(rgb, ctypes.c_uint8), int(sc.shape[0]),
+                                 _ptr(sc.ravel(), ctypes.c_uint8), int(sigma), _ptr(out["pos"], ctypes.c_int32),
+                                 _ptr(out["neg"], ctypes.c_int32), _ptr(out["tpos_f"], ctypes.c_double),
+                                 _ptr(out["tneg_f"], ctypes.c_double), _ptr(out["p"], ctypes.c_double))
+    if rc != 0:
+        raise RuntimeError(f"oracle_dataterm_kde failed: {rc}")
+    return {k: v.reshape(height, width) for k, v in out.items()}
 
 
 def refpr(width, height, pos, neg, edges=None, penalty=0, threads=0):

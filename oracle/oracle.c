@@ -28,25 +28,26 @@
 
 static inline int is_interior(int x, int y, int w, int h) { return x > 0 && y > 0 && x < w - 1 && y < h - 1; }
 
-/* Capacity of the arc from (x,y) in direction dir (0 left, 1 right, 2 up, 3 down),
- * zero unless both ends are interior. */
-static int64_t arc_cap(int x, int y, int w, int h, int dir, const int32_t *up, const int32_t *down,
-                       const int32_t *left, const int32_t *right, int32_t penalty) {
-    static const int dx[4] = {-1, 1, 0, 0}, dy[4] = {0, 0, -1, 1};
-    if (!is_interior(x, y, w, h) || !is_interior(x + dx[dir], y + dy[dir], w, h)) return 0;
-    if (!up) return penalty;
-    const size_t i = (size_t)y * w + x;
-    switch (dir) {
-        case 0: return left[i];
-        case 1: return right[i];
-        case 2: return up[i];
-        default: return down[i];
-    }
+/* Directions: 0 left, 1 right, 2 up, 3 down, then the diagonals of the
+ * 8-neighbourhood (GraphCut.h:65-70): 4 up-left, 5 up-right, 6 down-left,
+ * 7 down-right. OPP is the reverse arc. */
+static const int DX[8] = {-1, 1, 0, 0, -1, 1, -1, 1};
+static const int DY[8] = {0, 0, -1, 1, -1, -1, 1, 1};
+static const int OPP[8] = {1, 0, 3, 2, 7, 6, 5, 4};
+
+/* Capacity of the arc from (x,y) in direction dir, zero unless both ends are
+ * interior (GraphCutKernels.cu:143-156,200-213). planes[dir] == NULL for all
+ * directions means every n-link is penalty (GraphCutKernels.cu:129-140). */
+static int64_t arc_cap(int x, int y, int w, int h, int dir, const int32_t *const *planes, int32_t penalty) {
+    if (!is_interior(x, y, w, h) || !is_interior(x + DX[dir], y + DY[dir], w, h)) return 0;
+    if (!planes[0]) return penalty;
+    return planes[dir][(size_t)y * w + x];
 }
 
 typedef struct {
-    int w, h, n_interior;
+    int w, h, n_interior, ndir;
     size_t n;
+    long off[8];
     int64_t *e, *rt, *r;
     int32_t *d, *queue;
     uint8_t *inq;
@@ -69,9 +70,7 @@ static int32_t q_pop(pr_t *g) {
 /* Exact BFS distances to the sink over residual arcs (nodes with sink residual
  * are at distance 1); unreachable nodes get DINF. Rebuilds the active queue. */
 static void global_relabel(pr_t *g) {
-    const int w = g->w, h = g->h;
-    const long off[4] = {-1, 1, -(long)w, (long)w};
-    static const int opp[4] = {1, 0, 3, 2};
+    const int w = g->w, h = g->h, K = g->ndir;
     int32_t *bfs = (int32_t *)malloc(sizeof(int32_t) * (g->n ? g->n : 1));
     size_t head = 0, tail = 0;
     for (int y = 0; y < h; ++y)
@@ -85,11 +84,11 @@ static void global_relabel(pr_t *g) {
         }
     while (head < tail) {
         const int32_t v = bfs[head++];
-        for (int dir = 0; dir < 4; ++dir) {
-            const long u = (long)v + off[dir];
+        for (int dir = 0; dir < K; ++dir) {
+            const long u = (long)v + g->off[dir];
             if (u < 0 || (size_t)u >= g->n) continue;
             if (g->d[u] != DINF) continue;
-            if (g->r[(size_t)u * 4 + opp[dir]] <= 0) continue;
+            if (g->r[(size_t)u * K + OPP[dir]] <= 0) continue;
             g->d[u] = g->d[v] + 1;
             bfs[tail++] = (int32_t)u;
         }
@@ -101,25 +100,29 @@ static void global_relabel(pr_t *g) {
         if (g->e[i] > 0 && g->d[i] < DINF) q_push(g, (int32_t)i);
 }
 
-int oracle_maxflow_grid(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *up,
-                        const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
-                        int32_t *label, int64_t *flow_out) {
+static int maxflow_core(int width, int height, const int32_t *pos, const int32_t *neg,
+                        const int32_t *const *planes, int ndir, int32_t penalty, int32_t *label, int64_t *flow_out) {
     if (width <= 0 || height <= 0 || !pos || !neg || !label || !flow_out) return -1;
-    if (up && (!down || !left || !right)) return -1;
+    if (planes[0])
+        for (int k = 1; k < ndir; ++k)
+            if (!planes[k]) return -1;
     const size_t n = (size_t)width * height;
     memset(label, 0, sizeof(int32_t) * n);
     *flow_out = 0;
     if (width < 3 || height < 3) return 0;
+    const int K = ndir;
 
     pr_t g;
     memset(&g, 0, sizeof(g));
     g.w = width;
     g.h = height;
     g.n = n;
+    g.ndir = K;
     g.n_interior = (width - 2) * (height - 2);
+    for (int dir = 0; dir < K; ++dir) g.off[dir] = DX[dir] + (long)DY[dir] * width;
     g.e = (int64_t *)calloc(n, sizeof(int64_t));
     g.rt = (int64_t *)calloc(n, sizeof(int64_t));
-    g.r = (int64_t *)calloc(n * 4, sizeof(int64_t));
+    g.r = (int64_t *)calloc(n * K, sizeof(int64_t));
     g.d = (int32_t *)calloc(n, sizeof(int32_t));
     g.qcap = n + 1;
     g.queue = (int32_t *)calloc(g.qcap, sizeof(int32_t));
@@ -132,13 +135,15 @@ int oracle_maxflow_grid(int width, int height, const int32_t *pos, const int32_t
         for (int x = 0; x < width; ++x) {
             const size_t i = (size_t)y * width + x;
             if (!is_interior(x, y, width, height)) continue;
-            for (int dir = 0; dir < 4; ++dir) {
-                const int64_t c = arc_cap(x, y, width, height, dir, up, down, left, right, penalty);
+            for (int dir = 0; dir < K; ++dir) {
+                const int64_t c = arc_cap(x, y, width, height, dir, planes, penalty);
                 if (c < 0) bad = 1;
-                g.r[i * 4 + dir] = c;
+                g.r[i * K + dir] = c;
             }
+            /* t-links may be negative, as in the reference (GraphCutKernels.cu:196-197):
+             * subtracting min(p, q) from both is a reparametrisation, and the
+             * returned value is the minimum cut energy with the t-links as given. */
             const int64_t p = pos[i], q = neg[i];
-            if (p < 0 || q < 0) bad = 1;
             const int64_t m = p < q ? p : q;
             flow += m;
             g.e[i] = p - m;
@@ -149,8 +154,6 @@ int oracle_maxflow_grid(int width, int height, const int32_t *pos, const int32_t
         return -3;
     }
 
-    const long off[4] = {-1, 1, -(long)width, (long)width};
-    static const int opp[4] = {1, 0, 3, 2};
     global_relabel(&g);
     long relabels = 0;
     const long gr_every = g.n_interior / 2 + 16;
@@ -164,23 +167,23 @@ int oracle_maxflow_grid(int width, int height, const int32_t *pos, const int32_t
                 flow += dlt;
                 continue;
             }
-            for (int dir = 0; dir < 4 && g.e[u] > 0; ++dir) {
-                const int64_t rr = g.r[(size_t)u * 4 + dir];
+            for (int dir = 0; dir < K && g.e[u] > 0; ++dir) {
+                const int64_t rr = g.r[(size_t)u * K + dir];
                 if (rr <= 0) continue;
-                const long v = (long)u + off[dir];
+                const long v = (long)u + g.off[dir];
                 if (g.d[u] != g.d[v] + 1) continue;
                 const int64_t dlt = g.e[u] < rr ? g.e[u] : rr;
-                g.r[(size_t)u * 4 + dir] -= dlt;
-                g.r[(size_t)v * 4 + opp[dir]] += dlt;
+                g.r[(size_t)u * K + dir] -= dlt;
+                g.r[(size_t)v * K + OPP[dir]] += dlt;
                 g.e[u] -= dlt;
                 g.e[v] += dlt;
                 if (!g.inq[v] && g.d[v] < DINF) q_push(&g, (int32_t)v);
             }
             if (g.e[u] > 0) {
                 int32_t m = DINF;
-                for (int dir = 0; dir < 4; ++dir) {
-                    if (g.r[(size_t)u * 4 + dir] <= 0) continue;
-                    const int32_t dv = g.d[(long)u + off[dir]];
+                for (int dir = 0; dir < K; ++dir) {
+                    if (g.r[(size_t)u * K + dir] <= 0) continue;
+                    const int32_t dv = g.d[(long)u + g.off[dir]];
                     if (dv < m) m = dv;
                 }
                 g.d[u] = (m >= DINF - 1 || m + 1 > g.n_interior) ? DINF : m + 1;
@@ -213,27 +216,57 @@ int oracle_maxflow_grid(int width, int height, const int32_t *pos, const int32_t
     return ok ? 0 : -4;
 }
 
-int64_t oracle_cut_value(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *up,
-                         const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
-                         const int32_t *label) {
-    static const int dx[4] = {-1, 1, 0, 0}, dy[4] = {0, 0, -1, 1};
+int oracle_maxflow_grid(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *up,
+                        const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
+                        int32_t *label, int64_t *flow_out) {
+    const int32_t *planes[8] = {left, right, up, down, NULL, NULL, NULL, NULL};
+    if (up && (!down || !left || !right)) return -1;
+    if (!up) planes[0] = planes[1] = planes[2] = planes[3] = NULL;
+    return maxflow_core(width, height, pos, neg, planes, 4, penalty, label, flow_out);
+}
+
+int oracle_maxflow_grid8(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *const *dirs8,
+                         int32_t penalty, int32_t *label, int64_t *flow_out) {
+    const int32_t *planes[8] = {NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL};
+    if (dirs8)
+        for (int k = 0; k < 8; ++k) planes[k] = dirs8[k];
+    return maxflow_core(width, height, pos, neg, planes, 8, penalty, label, flow_out);
+}
+
+static int64_t cut_core(int width, int height, const int32_t *pos, const int32_t *neg,
+                        const int32_t *const *planes, int ndir, int32_t penalty, const int32_t *label) {
     int64_t cut = 0;
     for (int y = 1; y < height - 1; ++y)
         for (int x = 1; x < width - 1; ++x) {
             const size_t i = (size_t)y * width + x;
             if (label[i]) {
                 cut += neg[i];
-                for (int dir = 0; dir < 4; ++dir) {
-                    const int xx = x + dx[dir], yy = y + dy[dir];
+                for (int dir = 0; dir < ndir; ++dir) {
+                    const int xx = x + DX[dir], yy = y + DY[dir];
                     if (!is_interior(xx, yy, width, height)) continue;
                     if (label[(size_t)yy * width + xx]) continue;
-                    cut += arc_cap(x, y, width, height, dir, up, down, left, right, penalty);
+                    cut += arc_cap(x, y, width, height, dir, planes, penalty);
                 }
             } else {
                 cut += pos[i];
             }
         }
     return cut;
+}
+
+int64_t oracle_cut_value(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *up,
+                         const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
+                         const int32_t *label) {
+    const int32_t *planes[8] = {up ? left : NULL, right, up, down, NULL, NULL, NULL, NULL};
+    return cut_core(width, height, pos, neg, planes, 4, penalty, label);
+}
+
+int64_t oracle_cut_value8(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *const *dirs8,
+                          int32_t penalty, const int32_t *label) {
+    const int32_t *planes[8] = {NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL};
+    if (dirs8)
+        for (int k = 0; k < 8; ++k) planes[k] = dirs8[k];
+    return cut_core(width, height, pos, neg, planes, 8, penalty, label);
 }
 
 static int32_t q_round(double v) { return (int32_t)llround(v); }
@@ -326,5 +359,58 @@ int oracle_dataterm(int width, int height, const uint8_t *rgb, const uint8_t *se
         }
     free(wr_q);
     free(wd_q);
+    return 0;
+}
+
+/* Kernel-density data term (the reference's colour model). Restates
+ * K_InitGaussian (DataTermKernels.cu:19-31) for the kernel values, and
+ * K_SelectAndGauss + K_CalcTerms (DataTermKernels.cu:61-83,116-150) as a
+ * per-pixel sum: the volume value pdf[c] after all splats is the sum over
+ * seeds s of g[(int)|c - s|] for |c - s| < 101, here summed in double in seed
+ * order. floor(|c - s|) is taken with an integer square root. */
+static int isqrt_exact(int s) {
+    int k = 0;
+    while ((k + 1) * (k + 1) <= s) ++k;
+    return k;
+}
+
+static int32_t trunc_sat(double v) {
+    if (v >= 2147483647.0) return 2147483647;
+    if (v <= -2147483648.0) return (int32_t)0x80000000u;
+    return (int32_t)v;
+}
+
+void oracle_kde_gauss(int sigma, float *g101) {
+    for (int k = 0; k < 101; ++k) {
+        const float t = (float)k / (float)sigma;
+        const double arg = -0.5 * (double)t * (double)t;
+        g101[k] = (float)(1.0 / ((double)(float)sigma * 2.507) * (double)expf((float)arg));
+    }
+}
+
+int oracle_dataterm_kde(int width, int height, const uint8_t *rgb, int nseeds, const uint8_t *seed_rgb, int sigma,
+                        int32_t *pos, int32_t *neg, double *tpos_f, double *tneg_f, double *p_out) {
+    if (width <= 0 || height <= 0 || !rgb || nseeds < 0 || (nseeds && !seed_rgb) || sigma < 1) return -1;
+    float g[101];
+    oracle_kde_gauss(sigma, g);
+    int root[101 * 101];
+    for (int s = 0; s < 101 * 101; ++s) root[s] = isqrt_exact(s);
+    const size_t n = (size_t)width * height;
+    for (size_t i = 0; i < n; ++i) {
+        const uint8_t *a = rgb + 3 * i;
+        double p = 0.0;
+        for (int j = 0; j < nseeds; ++j) {
+            const uint8_t *b = seed_rgb + 3 * (size_t)j;
+            int s = 0;
+            for (int c = 0; c < 3; ++c) s += (a[c] - b[c]) * (a[c] - b[c]);
+            if (s < 101 * 101) p += (double)g[root[s]];
+        }
+        const float pf = (float)p;
+        if (p_out) p_out[i] = p;
+        if (tpos_f) tpos_f[i] = 300000.0 * p;
+        if (tneg_f) tneg_f[i] = 300000.0 * (1.0 - p);
+        if (pos) pos[i] = trunc_sat(300000.0 * (double)pf);
+        if (neg) neg[i] = trunc_sat(300000.0 * (double)(1.0f - pf));
+    }
     return 0;
 }

@@ -17,6 +17,8 @@
  *  - oracle_dataterm: the colour data term (16-bin per-channel seed
  *    histograms, -log-likelihood t-links, contrast n-links), evaluated per
  *    pixel in double precision, with pre-quantisation values.
+ *  - oracle_dataterm_kde: the reference's own Gaussian kernel-density colour
+ *    model, as a per-pixel double sum over the selected seed colours.
  *  - oracle_refpr_*: a CPU restatement of the reference's own push-relabel
  *    schedule (GraphCut.cu:385-534 over GraphCutKernels.cu), used as the
  *    timed CPU baseline.
@@ -38,6 +40,16 @@ int oracle_maxflow_grid(int width, int height, const int32_t *pos, const int32_t
                         const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
                         int32_t *label, int64_t *flow);
 
+/* 8-neighbourhood graph (NEIGHBORHOOD 8, GraphCut.h:22,65-70): dirs8 holds
+ * eight W*H planes in the order left, right, up, down, up-left (to
+ * (x-1,y-1)), up-right (x+1,y-1), down-left (x-1,y+1), down-right (x+1,y+1),
+ * or is NULL for a uniform penalty on all eight (GraphCutKernels.cu:129-151).
+ * Arcs into the border are zero as in GraphCutKernels.cu:143-156,200-213. */
+int oracle_maxflow_grid8(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *const *dirs8,
+                         int32_t penalty, int32_t *label, int64_t *flow);
+int64_t oracle_cut_value8(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *const *dirs8,
+                          int32_t penalty, const int32_t *label);
+
 /* Cut value of a 0/1 labelling (1 = source side) on the same graph. */
 int64_t oracle_cut_value(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *up,
                          const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
@@ -54,6 +66,17 @@ int oracle_dataterm(int width, int height, const uint8_t *rgb, const uint8_t *se
                     int32_t *hist, int32_t *counts, int64_t *beta_sum, int64_t *pairs, double *beta,
                     int32_t *pos, int32_t *neg, int32_t *up, int32_t *down, int32_t *left, int32_t *right,
                     double *tpos_f, double *tneg_f, double *right_f, double *down_f);
+
+/* Kernel-density colour model of the reference (DataTermKernels.cu:19-31,
+ * 61-83, 116-150): g101 receives the 101 kernel values for sigma; the data
+ * term sums g[floor(|I - s|)] over the nseeds seed colours (seed_rgb, 3 bytes
+ * each, in selection order) with |I - s| < 101 in double; p_out/tpos_f/tneg_f
+ * get p, 300000 p and 300000 (1 - p) before truncation; pos/neg are
+ * (int)(300000 pf) and (int)(300000 (1 - pf)) with pf = (float)p, saturated
+ * to int32. Each output may be NULL. */
+void oracle_kde_gauss(int sigma, float *g101);
+int oracle_dataterm_kde(int width, int height, const uint8_t *rgb, int nseeds, const uint8_t *seed_rgb, int sigma,
+                        int32_t *pos, int32_t *neg, double *tpos_f, double *tneg_f, double *p_out);
 
 /* Reference-schedule push-relabel (CPU restatement). threads <= 0 uses the
  * OpenMP default. Returns the number of outer iterations, or < 0 on error. */

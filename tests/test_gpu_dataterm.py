@@ -82,3 +82,93 @@ def test_constant_image(g):
     assert (up[1:] == 50).all() and (up[0] == 0).all()
     assert dt.beta()[2] == 0.0
     dt.close()
+
+
+# ---------------------------------------------------------------------------
+# The reference's kernel-density model (DataTermKernels.cu:19-150).
+
+def _kde_image(g, W, H, seed):
+    img = g.gen_ellipse_image(W, H, seed)
+    return img
+
+
+def _kde_device(g, W, H, img, clicks, sigma=5):
+    dt = g.DataTerm(W, H)
+    dt.set_model(g.DT_MODEL_KDE)
+    if sigma != 5:
+        dt.set_sigma(sigma)
+    dt.set_image(img)
+    for (x, y) in clicks:
+        dt.kde_select(x, y)
+    dt.compute()
+    return dt
+
+
+def _clicks(W, H, many):
+    c = [(W // 2, H // 2), (W // 2 + 3, H // 2 + 2), (W // 3, H // 2), (W - 3, H - 2), (-2, 1)]
+    if many:  # more clicks inside the ellipse: the unnormalised density exceeds 1 there
+        c += [(W // 2 - 8, H // 2 - 6), (W // 2 + 10, H // 2 + 5), (W // 2 - 4, H // 2 + 9), (W // 2 + 6, H // 2 - 9)]
+    return c
+
+
+@pytest.mark.parametrize("shape,sigma,many", [((320, 240), 5, False), ((97, 61), 3, False),
+                                              ((1024, 768), 8, False), ((320, 240), 30, True),
+                                              ((640, 480), 24, True)])
+def test_kde_dataterm_exact(g, ora, shape, sigma, many):
+    W, H = shape
+    img = _kde_image(g, W, H, 11)
+    clicks = _clicks(W, H, many)
+    dt = _kde_device(g, W, H, img, clicks, sigma)
+    cols = np.concatenate([ora.kde_select_colours(img, W, H, x, y) for (x, y) in clicks])
+    assert dt.kde_count() == len(cols)
+    ref = ora.dataterm_kde(W, H, img, cols, sigma)
+    pos, neg = dt.links()[:2]
+    assert np.array_equal(pos, ref["pos"]) and np.array_equal(neg, ref["neg"])
+    if many:
+        assert (neg < 0).any()  # p > 1 inside the clicked region, as in the reference
+    tpos, tneg, _, _ = dt.unquantized()
+    for a, b in ((tpos, ref["tpos_f"]), (tneg, ref["tneg_f"])):
+        rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+        assert float(rel.max()) <= 1e-5
+    dt.close()
+
+
+def test_kde_colours_captured_at_selection(g, ora):
+    """selectOnMap reads the image at click time (DataTermKernels.cu:133-136):
+    a later setImage (the next video frame) keeps the density."""
+    W, H = 160, 120
+    a = _kde_image(g, W, H, 1)
+    b = _kde_image(g, W, H, 2)
+    dt = g.DataTerm(W, H)
+    dt.set_model(g.DT_MODEL_KDE)
+    dt.set_image(a)
+    dt.kde_select(70, 50)
+    dt.kde_select(10, 10)
+    dt.set_image(b)
+    dt.compute()
+    cols = np.concatenate([ora.kde_select_colours(a, W, H, 70, 50), ora.kde_select_colours(a, W, H, 10, 10)])
+    ref = ora.dataterm_kde(W, H, b, cols, 5)
+    pos, neg = dt.links()[:2]
+    assert np.array_equal(pos, ref["pos"]) and np.array_equal(neg, ref["neg"])
+    dt.reset()
+    assert dt.kde_count() == 0
+    dt.compute()
+    pos, neg = dt.links()[:2]
+    assert (pos == 0).all() and (neg == 300000).all()
+    dt.close()
+
+
+def test_kde_segmentation_negative_tlinks(g, ora):
+    """Kernel-density t-links (neg < 0 where p > 1) with a constant penalty,
+    as ComplexApplication.cu:127-135 uses them: exact flow and labels."""
+    W, H = 200, 150
+    img = _kde_image(g, W, H, 5)
+    clicks = [(95, 70), (100, 75), (90, 72), (80, 60), (110, 80), (98, 55), (92, 88), (105, 66)]
+    dt = _kde_device(g, W, H, img, clicks, sigma=30)
+    pos, neg = dt.links()[:2]
+    assert (neg < 0).any()
+    for pen in (0, 700, 7000):
+        flow, label, _ = g.solve(W, H, pos, neg, penalty=pen)
+        rflow, rlabel = ora.maxflow(W, H, pos, neg, penalty=pen)
+        assert flow == rflow and np.array_equal(label, rlabel), pen
+    dt.close()

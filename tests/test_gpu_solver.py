@@ -137,13 +137,32 @@ def test_options_do_not_change_result(g, ora, opts):
     _check(g, ora, W, H, pos, neg, edges=E, options=o)
 
 
-def test_negative_capacity_rejected(g):
+def test_negative_nlink_rejected(g):
     W, H = 20, 20
     pos = np.zeros((H, W), np.int32)
     neg = np.zeros((H, W), np.int32)
-    pos[5, 5] = -3
+    E = tuple(np.ones((H, W), np.int32) for _ in range(4))
+    E[2][5, 5] = -3
     with pytest.raises(g.GcfgError):
-        g.solve(W, H, pos, neg, penalty=1)
+        g.solve(W, H, pos, neg, edges=E)
+    with pytest.raises(g.GcfgError):  # |pos - neg| beyond 2^30 would leave int32 excess no headroom
+        g.solve(W, H, np.full((H, W), 1 << 30, np.int32), np.full((H, W), -8, np.int32), penalty=1)
+
+
+@pytest.mark.parametrize("kernels", [1, 0])
+def test_negative_tlinks_accepted(g, ora, kernels):
+    """t-links of either sign, as the reference takes them (excess = pos - neg,
+    GraphCutKernels.cu:196-197): the flow is the minimum cut energy with the
+    t-links as given."""
+    rng = np.random.default_rng(31)
+    W, H = 150, 90
+    pos, neg, E = random_instance(rng, W, H, tmax=200, emax=60)
+    pos = pos - rng.integers(0, 120, pos.shape).astype(np.int32)
+    neg = neg - rng.integers(0, 250, neg.shape).astype(np.int32)
+    o = g.default_options()
+    o.kernels = kernels
+    _check(g, ora, W, H, pos, neg, edges=E, options=o)
+    _check(g, ora, W, H, pos, neg, penalty=40, options=o)
 
 
 def test_optimize_before_build_is_state_error(g):
@@ -241,6 +260,11 @@ def test_profile_mode_reports_kernel_time(g, ora):
     rflow, rlabel = ora.maxflow(W, H, pos, neg, edges=E)
     assert flow == rflow and np.array_equal(label, rlabel)
     assert st.pr_kernel_ms > 0 and st.gr_kernel_ms > 0 and st.launches >= st.steps
+    assert st.pr_iters == 0 and st.gr_iters == 0  # counters only with profile 2
+    o.profile = 2
+    flow2, label2, st2 = g.solve(W, H, pos, neg, edges=E, options=o)
+    assert flow2 == rflow and np.array_equal(label2, rlabel)
+    assert st2.pr_iters > 0 and st2.gr_iters > 0 and st2.gr_kernel_ms > 0
 
 
 GR_BLOCKS = [(1, 1), (1, 2), (1, 4), (1, 8), (1, 16), (2, 2), (2, 4), (2, 8), (4, 4)]

@@ -132,10 +132,10 @@ def test_degenerate_and_invalid(ora):
     for W, H in ((1, 1), (2, 5), (5, 2)):
         f, lab = ora.maxflow(W, H, np.ones((H, W)), np.zeros((H, W)), penalty=1)
         assert f == 0 and lab.sum() == 0
-    pos = np.zeros((5, 5), np.int32)
-    pos[2, 2] = -1
+    E = tuple(np.ones((5, 5), np.int32) for _ in range(4))
+    E[1][2, 2] = -1  # negative n-links are invalid (negative t-links are not)
     with pytest.raises(RuntimeError):
-        ora.maxflow(5, 5, pos, np.zeros((5, 5)), penalty=1)
+        ora.maxflow(5, 5, np.zeros((5, 5)), np.zeros((5, 5)), edges=E)
 
 
 def test_reference_restatement_matches_oracle(ora):
@@ -156,3 +156,104 @@ def test_debug_cu_random_instance(ora):
     assert ora.cut_value(400, 400, pos, neg, l, penalty=50) == f
     fr, lr, it, _ = ora.refpr(400, 400, pos, neg, penalty=50, threads=4)
     assert fr == f and np.array_equal(lr, l)
+
+
+# ---------------------------------------------------------------------------
+# 8-neighbourhood (NEIGHBORHOOD == 8, GraphCut.h:22,65-70; diagonal arcs in
+# GraphCutKernels.cu:133-151,190-213).
+D8 = ((-1, 0), (1, 0), (0, -1), (0, 1), (-1, -1), (1, -1), (-1, 1), (1, 1))
+
+
+def random_instance8(rng, W, H, tmax=12, emax=8):
+    pos = rng.integers(0, tmax, (H, W)).astype(np.int32)
+    neg = rng.integers(0, tmax, (H, W)).astype(np.int32)
+    dirs = tuple(rng.integers(0, emax, (H, W)).astype(np.int32) for _ in range(8))
+    return pos, neg, dirs
+
+
+def brute_force8(W, H, pos, neg, dirs):
+    nodes = [(x, y) for y in range(1, H - 1) for x in range(1, W - 1)]
+    idx = {p: i for i, p in enumerate(nodes)}
+    labs = np.array(list(itertools.product([0, 1], repeat=len(nodes))), dtype=np.int64)
+    cost = np.zeros(len(labs), np.int64)
+    for i, (x, y) in enumerate(nodes):
+        s = labs[:, i]
+        cost += np.where(s == 1, neg[y, x], pos[y, x])
+        for (dx, dy), cap in zip(D8, dirs):
+            q = (x + dx, y + dy)
+            if q in idx:
+                cost += (s == 1) * (labs[:, idx[q]] == 0) * int(cap[y, x])
+    best = cost.min()
+    union = labs[cost == best].max(axis=0)
+    lab = np.zeros((H, W), np.int32)
+    for i, (x, y) in enumerate(nodes):
+        lab[y, x] = union[i]
+    return int(best), lab
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_brute_force_tiny8(ora, seed):
+    rng = np.random.default_rng(500 + seed)
+    W, H = [(4, 4), (5, 4), (4, 6), (6, 5), (3, 7)][seed % 5]
+    pos, neg, dirs = random_instance8(rng, W, H)
+    if seed % 2:
+        pos[::2] = neg[::2]
+    best, blab = brute_force8(W, H, pos, neg, dirs)
+    flow, label = ora.maxflow8(W, H, pos, neg, dirs)
+    assert flow == best and np.array_equal(label, blab)
+    assert ora.cut_value8(W, H, pos, neg, label, dirs) == flow
+
+
+def test_diagonal_only_path8(ora):
+    """Source and sink joined only by a diagonal chain: every side n-link is 0,
+    so the flow equals the diagonal bottleneck and a 4-neighbourhood solve is 0."""
+    W, H = 7, 7
+    z = np.zeros((H, W), np.int32)
+    dr = z.copy()
+    for k in range(1, 5):
+        dr[k, k] = 5 + k  # (k,k) -> (k+1,k+1)
+    dr[3, 3] = 4
+    pos, neg = z.copy(), z.copy()
+    pos[1, 1], neg[5, 5] = 100, 100
+    dirs = (z, z, z, z, z, z, z, dr)
+    flow, label = ora.maxflow8(W, H, pos, neg, dirs)
+    assert flow == 4
+    assert ora.maxflow(W, H, pos, neg, edges=(z, z, z, z))[0] == 0
+    # the cut is at (3,3) -> (4,4): (1,1),(2,2),(3,3) are on the source side
+    assert label[1, 1] == label[2, 2] == label[3, 3] == 1 and label[4, 4] == 0
+    # border rule: a diagonal arc into the border carries nothing
+    dirs2 = (z, z, z, z, z, z, z, np.full((H, W), 9, np.int32))
+    p2, n2 = z.copy(), z.copy()
+    p2[5, 5], n2[1, 1] = 50, 50  # (5,5) -> (6,6) is a border pixel
+    assert ora.maxflow8(W, H, p2, n2, dirs2)[0] == 0
+
+
+def test_penalty8_equals_uniform_planes(ora):
+    rng = np.random.default_rng(9)
+    W, H = 23, 17
+    pos, neg, _ = random_instance8(rng, W, H, tmax=60)
+    f1, l1 = ora.maxflow8(W, H, pos, neg, penalty=7)
+    f2, l2 = ora.maxflow8(W, H, pos, neg, tuple(np.full((H, W), 7, np.int32) for _ in range(8)))
+    assert f1 == f2 and np.array_equal(l1, l2)
+
+
+def test_against_networkx8(ora):
+    nx = pytest.importorskip("networkx")
+    rng = np.random.default_rng(17)
+    for trial in range(12):
+        W, H = int(rng.integers(3, 12)), int(rng.integers(3, 12))
+        pos, neg, dirs = random_instance8(rng, W, H, tmax=60, emax=30)
+        G = nx.DiGraph()
+        G.add_node("s")
+        G.add_node("t")
+        for y in range(1, H - 1):
+            for x in range(1, W - 1):
+                G.add_edge("s", (x, y), capacity=int(pos[y, x]))
+                G.add_edge((x, y), "t", capacity=int(neg[y, x]))
+                for (dx, dy), cap in zip(D8, dirs):
+                    xx, yy = x + dx, y + dy
+                    if 0 < xx < W - 1 and 0 < yy < H - 1:
+                        G.add_edge((x, y), (xx, yy), capacity=int(cap[y, x]))
+        want = nx.maximum_flow_value(G, "s", "t") if W > 2 and H > 2 else 0
+        flow, label = ora.maxflow8(W, H, pos, neg, dirs)
+        assert flow == want
