@@ -15,6 +15,9 @@ Reported beside it:
   * serve: continuous-batching server on a Poisson stream (sizes
     log-uniform on [256^2, 2048^2], 30% seed-stroke edits warm-started on
     device-resident residual graphs): p50/p99 latency, Mpx/s;
+  * video: frames of a moving 1024x768 scene on one session, warm-started
+    from the previous frame's residual graph, beside the same frames solved
+    cold as new images;
   * roofline of the dominant kernel from an event-timed solve of the same batch;
   * cpu_baseline: the CPU restatement of the reference push-relabel schedule
     on a bounded sample of the same instances, with its results checked
@@ -171,6 +174,59 @@ def serve(args, opts, qps, n):
                 p50_ms=round(float(np.percentile(L, 50)), 3), p99_ms=round(float(np.percentile(L, 99)), 3),
                 mean_ms=round(float(L.mean()), 3), mpx_s=round(px / (wall * 1e3), 2), wall_ms=round(wall, 1),
                 hbm_budget_gb=args.serve_budget_gb), reqs, flows
+
+
+def video(args, opts, W=1024, H=768, n=16):
+    """Video loop on a serving session: a moving scene (the first image
+    shifted by (3k, 2k) px per frame, 10 % of pixels re-drawn). Warm: every
+    frame after the first is submit_frame on one session (reparametrised from
+    the previous frame's residual graph). Cold: every frame is a new image
+    session, built and solved from scratch. Latency is arrival -> labels on
+    the host; warm flows are checked to repeat exactly across two runs (their
+    exactness against cold solves is a GPU test)."""
+    rng = np.random.default_rng(args.seed + 11)
+    base = gcfg.gen_ellipse_image(W, H, args.seed + 11)
+    frames = [base]
+    for k in range(1, n):
+        f = np.roll(base, (2 * k, 3 * k), axis=(0, 1)).copy()
+        m = rng.random((H, W)) < 0.1
+        f[m] = rng.integers(0, 256, (int(m.sum()), 3))
+        frames.append(f.astype(np.uint8))
+    seeds = gcfg.gen_seed_map(W, H)
+
+    def run(warm):
+        srv = gcfg.Server(args.serve_budget_gb << 30, opts)
+        lat, flows = [], []
+        sess = None
+        for k, f in enumerate(frames):
+            t0 = gcfg.now_ms()
+            if warm and k > 0:
+                rid, _ = srv.submit_frame(sess, f)
+            else:
+                rid, s, _ = srv.submit_image(f, seeds)
+                sess = s if sess is None else sess
+            r = None
+            while r is None:
+                for q in srv.poll(wait=True):
+                    if q["request_id"] == rid:
+                        r = q
+            assert r["status"] == 0
+            lat.append(r["done_ms"] - t0)
+            flows.append(int(r["flow"]))
+            if not warm and k > 0:
+                srv.close_session(r["session"])
+        srv.close()
+        return np.array(lat), flows
+
+    run(True)  # warm-up (first use of the server path)
+    lw, fw = run(True)
+    lc, _ = run(False)
+    lw2, fw2 = run(True)
+    assert fw == fw2, "video frames differ between runs"
+    return dict(frames=n, shape=f"{W}x{H}", warm_frame_ms=round(float(np.mean(lw[1:])), 3),
+                cold_frame_ms=round(float(np.mean(lc[1:])), 3),
+                warm_p99_ms=round(float(np.percentile(lw[1:], 99)), 3),
+                warm_mpx_s=round(W * H * (n - 1) / float(np.sum(lw[1:])) / 1e3, 1))
 
 
 def cpu_baseline(insts, flows_gpu, labels_gpu_fn, budget_s):
@@ -551,6 +607,11 @@ def main():
         sv2, _, _ = serve(args, opts, args.serve_qps_high, args.serve_requests_high)
         extra["serve_high_load"] = sv2
         log(f"serve (high load): {sv2}")
+
+    if not args.no_serve:
+        log("video loop: warm frames on a session vs cold images")
+        extra["video"] = video(args, opts)
+        log(f"video: {extra['video']}")
 
     cpu = None
     if not args.no_cpu:

@@ -422,6 +422,12 @@ int DtContext::compute(int32_t *pos, int32_t *neg, int32_t *up, int32_t *down, i
     GCFG_HIP(hipMemcpyAsync(h_stats_, d_stats_, sizeof(DtStats), hipMemcpyDeviceToHost, stream_));
     GCFG_HIP(hipStreamSynchronize(stream_));
     stats = *h_stats_;
+    if (frozen) {  // keep the model's histograms; beta follows the current image
+        std::memcpy(stats.hist, frozen_stats.hist, sizeof(stats.hist));
+        std::memcpy(stats.counts, frozen_stats.counts, sizeof(stats.counts));
+        std::memcpy(h_stats_, &stats, sizeof(DtStats));
+        GCFG_HIP(hipMemcpyAsync(d_stats_, h_stats_, sizeof(DtStats), hipMemcpyHostToDevice, stream_));
+    }
     pairs = (int64_t)(W - 1) * H + (int64_t)W * (H - 1);
     beta = pairs > 0 ? (double)stats.beta_sum / (double)pairs : 0.0;
     int rc = build_tables();
@@ -437,6 +443,29 @@ int DtContext::compute(int32_t *pos, int32_t *neg, int32_t *up, int32_t *down, i
     }
     GCFG_HIP(hipGetLastError());
     computed = true;
+    return GCFG_OK;
+}
+
+int DtContext::compute_swap() {
+    if (!pos_) {
+        set_error("data term: no owned output planes");
+        return GCFG_ERR_STATE;
+    }
+    const size_t n = (size_t)W * H;
+    for (int k = 0; k < 6; ++k)
+        if (!alt_[k]) {
+            if (hipMalloc((void **)&alt_[k], n * 4) != hipSuccess) {
+                alt_[k] = nullptr;
+                set_error("data term: device allocation failed");
+                return GCFG_ERR_NOMEM;
+            }
+            allocs_.push_back(alt_[k]);
+            bytes_ += n * 4;
+        }
+    int rc = compute(alt_[0], alt_[1], alt_[2], alt_[3], alt_[4], alt_[5]);
+    if (rc != GCFG_OK) return rc;
+    int32_t **cur[6] = {&pos_, &neg_, &up_, &down_, &left_, &right_};
+    for (int k = 0; k < 6; ++k) std::swap(*cur[k], alt_[k]);
     return GCFG_OK;
 }
 

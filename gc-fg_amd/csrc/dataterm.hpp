@@ -62,6 +62,15 @@ class DtContext {
     // Kernel-density mode: selectOnMap's 5 x 5 square at (x..x+4, y..y+4),
     // colours read from the current image now (pixels outside it are skipped).
     int kde_select(int x, int y, int size);
+    // Video frames: keep the colour model of the current seeds (histograms;
+    // the kernel-density seed colours are kept anyway) for later images, as the
+    // reference keeps its density across frames (ComplexApplication.cu:202-207).
+    void freeze_model() { frozen = computed; frozen_stats = stats; }
+    // Compute into the second set of output planes (allocated on first use)
+    // and swap it with the first: the previous outputs stay readable through
+    // prev_*() until the next swap.
+    int compute_swap();
+    const int32_t *prev_pos() const { return alt_[0]; }
     int set_model(int m);
     int set_sigma(int s);
     int unquantized(double *h_tpos, double *h_tneg, double *h_right, double *h_down);
@@ -71,6 +80,8 @@ class DtContext {
     int32_t model = GCFG_DT_MODEL_HISTOGRAM;
     int32_t sigma = 5;     // DEFAULT_SIGMA, DataTerm.cpp:6
     int32_t kde_count = 0; // seed colours in the list
+    bool frozen = false;
+    DtStats frozen_stats{};
     float gauss[KDE_TAPS] = {};
     DtStats stats{};
     int64_t pairs = 0;
@@ -89,6 +100,7 @@ class DtContext {
     DtStats *d_stats_ = nullptr, *h_stats_ = nullptr;
     int32_t *d_lut_ = nullptr, *h_lut_ = nullptr;  // [2*4096 t-link LUT][thresholds]
     uint32_t *d_kde_ = nullptr;                    // seed colours 0x00RRGGBB-packed (c0 << 16 | c1 << 8 | c2)
+    int32_t *alt_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // second output set
     float *d_gauss_ = nullptr;
     std::vector<void *> allocs_;
     size_t bytes_ = 0;

@@ -414,8 +414,8 @@ int Engine::step_graph(int steps, hipGraphExec_t *out) {
 int Engine::launch_steps(int n) {
     apply_options();
     if (opt.profile) {
-        if ((int)events_.size() < 4 * n) {
-            for (int i = (int)events_.size(); i < 4 * n; ++i) {
+        if ((int)events_.size() < 6 * n) {
+            for (int i = (int)events_.size(); i < 6 * n; ++i) {
                 hipEvent_t e;
                 GCFG_HIP(hipEventCreate(&e));
                 events_.push_back(e);
@@ -429,13 +429,10 @@ int Engine::launch_steps(int n) {
             qtrace_cap_ = n;
         }
         for (int i = 0; i < n; ++i) {
-            GCFG_HIP(hipEventRecord(events_[4 * i], stream_));
-            launch_sched(kp_, parity_, stream_);
-            GCFG_HIP(hipEventRecord(events_[4 * i + 1], stream_));
-            launch_pr(kp_, grid_pr_, parity_, stream_);
-            GCFG_HIP(hipEventRecord(events_[4 * i + 2], stream_));
-            launch_gr(kp_, grid_gr_, parity_, stream_);
-            GCFG_HIP(hipEventRecord(events_[4 * i + 3], stream_));
+            hipEvent_t *ev = &events_[6 * i];
+            launch_sched(kp_, parity_, stream_, LaunchEvents{ev[0], ev[1]});
+            launch_pr(kp_, grid_pr_, parity_, stream_, LaunchEvents{ev[2], ev[3]});
+            launch_gr(kp_, grid_gr_, parity_, stream_, LaunchEvents{ev[4], ev[5]});
             // This step's queue lengths, before the next scheduler clears them.
             if (trace)
                 GCFG_HIP(hipMemcpyAsync(pin_qtrace_ + i, &kp_.ctl->q[parity_], sizeof(QCtl), hipMemcpyDeviceToHost,
@@ -494,9 +491,9 @@ int Engine::run(int64_t max_steps) {
 int Engine::collect_profile() {
     for (int i = 0; i < events_pending_; ++i) {
         float a = 0, b = 0, c = 0;
-        GCFG_HIP(hipEventElapsedTime(&a, events_[4 * i], events_[4 * i + 1]));
-        GCFG_HIP(hipEventElapsedTime(&b, events_[4 * i + 1], events_[4 * i + 2]));
-        GCFG_HIP(hipEventElapsedTime(&c, events_[4 * i + 2], events_[4 * i + 3]));
+        GCFG_HIP(hipEventElapsedTime(&a, events_[6 * i], events_[6 * i + 1]));
+        GCFG_HIP(hipEventElapsedTime(&b, events_[6 * i + 2], events_[6 * i + 3]));
+        GCFG_HIP(hipEventElapsedTime(&c, events_[6 * i + 4], events_[6 * i + 5]));
         prof_sched_ms += a;
         prof_pr_ms += b;
         prof_gr_ms += c;
@@ -543,6 +540,30 @@ int Engine::fetch_ctl(Ctl *out) {
     GCFG_HIP(hipStreamSynchronize(stream_));
     *out = *pin_ctl_;
     return GCFG_OK;
+}
+
+int Engine::reframe(int slot, const int32_t *pos, const int32_t *neg, const int32_t *up, const int32_t *down,
+                    const int32_t *left, const int32_t *right, int32_t penalty) {
+    if (slot < 0 || slot >= cap_slots_ || !slot_used_[slot]) {
+        set_error("reframe: unknown instance");
+        return GCFG_ERR_INVALID;
+    }
+    if (nbhd_ != 4) {
+        set_error("reframe: 4-neighbourhood instances only");
+        return GCFG_ERR_STATE;
+    }
+    InstDesc &d = h_desc_[slot];
+    if (!pos || !neg || !d.pos || !d.neg) {
+        set_error("reframe: data terms not set");
+        return GCFG_ERR_STATE;
+    }
+    if (d.W < 3 || d.H < 3) return set_inputs(slot, pos, neg, up, down, left, right, penalty, d.label_out);
+    const int32_t *in1[6] = {pos, neg, up, up ? down : nullptr, up ? left : nullptr, up ? right : nullptr};
+    launch_reframe(kp_, slot, d.ntiles, in1, penalty, stream_);
+    GCFG_HIP(hipGetLastError());
+    int rc = set_inputs(slot, pos, neg, up, down, left, right, penalty, d.label_out);
+    if (rc == GCFG_OK) rc = upload_desc();  // waits for the reframe kernels, which read the old inputs
+    return rc;
 }
 
 int Engine::read_state(int slot, int32_t *h_caps, int32_t *h_exc, int32_t *h_dist, int32_t *h_caps8) {

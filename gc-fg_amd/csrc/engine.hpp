@@ -132,13 +132,17 @@ struct KParams {
     int4 *nbr8;
 };
 
-// Kernel launchers (solver.hip).
+// Kernel launchers (solver.hip). With events, a launch records the
+// kernel's own start and stop times in them (profile mode).
+struct LaunchEvents {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
 void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int sweeps, hipStream_t s);
 void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_tiles, hipStream_t s);
 void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s);
-void launch_sched(const KParams &p, int parity, hipStream_t s);
-void launch_pr(const KParams &p, int grid, int parity, hipStream_t s);
-void launch_gr(const KParams &p, int grid, int parity, hipStream_t s);
+void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvents &ev);
+void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
+void launch_gr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
 bool gr_block_supported(int bx, int by);
 bool options_valid(const gcfg_options &o);  // sets the error message when not
 void launch_reset_stats(const KParams &p, hipStream_t s);
@@ -146,6 +150,9 @@ void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_str
                  hipStream_t s);
 void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, int32_t *d_dist,
                        int32_t *d_caps8, hipStream_t s);
+// in1: pos, neg, up, down, left, right of the new inputs (up == null: penalty pen1).
+void launch_reframe(const KParams &p, int slot, int ntiles, const int32_t *const *in1, int32_t pen1,
+                    hipStream_t s);
 
 // Orders a handle's internal stream against a caller stream: `before` makes
 // the internal stream wait for everything already queued on the caller's,
@@ -204,6 +211,11 @@ class Engine {
     int pump(int steps);
     int live_count(int32_t *out);
     int edit(int slot, const std::vector<int32_t> &strokes, int32_t delta);
+    // Warm start of a solved 4-neighbourhood instance onto new inputs (same
+    // shape): reparametrise its residual graph, then make the new inputs the
+    // instance's own; follow with rearm().
+    int reframe(int slot, const int32_t *pos, const int32_t *neg, const int32_t *up, const int32_t *down,
+                const int32_t *left, const int32_t *right, int32_t penalty);
     // Residual capacities (W*H*4), excess (W*H), distance labels (W*H) and,
     // for the 8-neighbourhood, diagonal residual capacities (W*H*4) in image
     // layout; any output may be null.
@@ -269,7 +281,7 @@ class Engine {
     int step_graph(int steps, hipGraphExec_t *out);
     int parity_ = 0;  // parity of the next step to launch
     std::vector<void *> allocs_;
-    std::vector<hipEvent_t> events_;  // 4 per profiled step
+    std::vector<hipEvent_t> events_;  // 6 per profiled step: start/stop of sched, pr, gr
     int events_pending_ = 0;
     QCtl *pin_qtrace_ = nullptr;      // queue lengths per profiled step (verbose >= 2)
     int qtrace_cap_ = 0;

@@ -24,7 +24,7 @@ using gcfg::set_error;
 
 namespace {
 
-enum { REQ_IMAGE = 0, REQ_EDIT = 1 };
+enum { REQ_IMAGE = 0, REQ_EDIT = 1, REQ_FRAME = 2 };
 
 struct Request {
     int64_t id = 0;
@@ -106,7 +106,7 @@ static bool make_room(gcfg_server *s, size_t need) {
             if (ss->busy) continue;
             bool pending = false;
             for (Request *q : s->queue)
-                if (q->kind == REQ_EDIT && q->session == ss->id) pending = true;
+                if ((q->kind == REQ_EDIT || q->kind == REQ_FRAME) && q->session == ss->id) pending = true;
             if (pending) continue;
             if (!victim || ss->last_used < victim->last_used) victim = ss;
         }
@@ -132,6 +132,7 @@ static int admit_image(gcfg_server *s, Request *r) {
     if (rc == GCFG_OK) rc = ss->dt->upload_image(r->rgb);
     if (rc == GCFG_OK) rc = ss->dt->upload_seeds(r->seeds);
     if (rc == GCFG_OK) rc = ss->dt->compute_owned();
+    if (rc == GCFG_OK) ss->dt->freeze_model();  // later frames of the session reuse this colour model
     if (rc == GCFG_OK) {
         ss->slot = s->eng.add_instance(r->W, r->H, ss->dt->pos_, ss->dt->neg_, ss->dt->up_, ss->dt->down_,
                                        ss->dt->left_, ss->dt->right_, 0, ss->d_label);
@@ -193,7 +194,7 @@ static int admit(gcfg_server *s) {
     while (!s->queue.empty()) {
         Request *r = s->queue.front();
         s->queue.pop_front();
-        if (r->kind == REQ_EDIT) {
+        if (r->kind == REQ_EDIT || r->kind == REQ_FRAME) {
             auto it = s->sessions.find(r->session);
             if (it == s->sessions.end()) {
                 bool image_pending = false;  // the session's image is still queued
@@ -213,7 +214,21 @@ static int admit(gcfg_server *s) {
                 keep.push_back(r);
                 continue;
             }
-            int rc = s->eng.edit(ss->slot, r->strokes, r->delta);
+            int rc = GCFG_OK;
+            if (r->kind == REQ_EDIT) {
+                rc = s->eng.edit(ss->slot, r->strokes, r->delta);
+            } else if (r->W != ss->W || r->H != ss->H) {
+                set_error("server: frame size differs from the session's");
+                rc = GCFG_ERR_INVALID;
+            } else {
+                // Next frame: data term of the new image with the session's
+                // colour model, then the residual graph reparametrised onto it.
+                DtContext *dt = ss->dt;
+                rc = dt->upload_image(r->rgb);
+                if (rc == GCFG_OK) rc = dt->compute_swap();
+                if (rc == GCFG_OK)
+                    rc = s->eng.reframe(ss->slot, dt->pos_, dt->neg_, dt->up_, dt->down_, dt->left_, dt->right_, 0);
+            }
             if (rc == GCFG_OK) rc = s->eng.rearm({ss->slot});
             if (rc != GCFG_OK) {
                 push_result(s, r, rc, 0);
@@ -326,6 +341,27 @@ int gcfg_server_submit_edit(gcfg_server *s, int32_t session, int32_t n_strokes, 
     r->t_submit = now_ms();
     r->strokes.assign(strokes, strokes + 5 * (size_t)n_strokes);
     r->delta = delta;
+    r->h_label = h_label;
+    s->queue.push_back(r);
+    if (request_id) *request_id = r->id;
+    return GCFG_OK;
+}
+
+int gcfg_server_submit_frame(gcfg_server *s, int32_t session, int32_t width, int32_t height, const uint8_t *h_rgb,
+                             int32_t *h_label, int64_t *request_id) {
+    if (!s || width <= 0 || height <= 0 || !h_rgb) {
+        set_error("gcfg_server_submit_frame: invalid arguments");
+        return GCFG_ERR_INVALID;
+    }
+    Request *r = new (std::nothrow) Request();
+    if (!r) return GCFG_ERR_NOMEM;
+    r->id = s->next_id++;
+    r->kind = REQ_FRAME;
+    r->session = session;
+    r->t_submit = now_ms();
+    r->W = width;
+    r->H = height;
+    r->rgb = h_rgb;
     r->h_label = h_label;
     s->queue.push_back(r);
     if (request_id) *request_id = r->id;

@@ -19,6 +19,8 @@
 //    labels (interior and unreachable, the SPREAD_ZEROS set of :228-351)
 //    and the int64 stuck-excess sum for flow = sum pos - sum max(e, 0).
 //  * k_sched: per-instance state machine and ballot-compacted tile queues.
+#include <hip/hip_ext.h>
+
 #include <type_traits>
 
 #include "engine.hpp"
@@ -2026,6 +2028,101 @@ __global__ __launch_bounds__(NT) void k_edit_masks(KParams p, int slot, const in
     }
 }
 
+// ---------------------------------------------------------------------------
+// Warm start onto new capacities (the next video frame of a session): the
+// residual graph of the previous solve is reparametrised to the new inputs
+// instead of rebuilt (the reference rebuilds every frame,
+// ComplexApplication.cu:127-135,202-207 -> GC_SetGraph).
+//  * t-links: excess += (pos1 - neg1) - (pos0 - neg0), and the interior sum
+//    of pos moves by pos1 - pos0;
+//  * n-link pair (u, v): with old capacities c_uv, c_vu and residuals r_uv,
+//    r_vu the net flow u -> v is g = c_uv - r_uv. It is clamped to the new
+//    capacities, g' = min(max(g, -c'_vu), c'_uv); then r_uv = c'_uv - g',
+//    r_vu = c'_vu + g', and the flow no longer carried, g - g', stays at u
+//    (excess_u += g - g', excess_v -= g - g').
+// The result is a preflow of the new graph with the same conservation
+// identity, so flow = sum pos - sum max(excess, 0) still holds at the end.
+// One thread per interior node owns its right and down pairs (each residual
+// component has one writer); excess changes are int32 atomics. Old inputs
+// come from the descriptor, new ones from the arguments (null up = penalty).
+__device__ __forceinline__ int reframe_cap(const int32_t *plane, int32_t pen, size_t i) {
+    return plane ? plane[i] : pen;
+}
+
+__global__ __launch_bounds__(NT) void k_reframe(KParams p, int slot, const int32_t *pos1, const int32_t *neg1,
+                                                const int32_t *up1, const int32_t *down1, const int32_t *left1,
+                                                const int32_t *right1, int32_t pen1) {
+    __shared__ int64_t red[NT / 64];
+    const InstDesc D = p.desc[slot];
+    const int64_t n = (int64_t)D.W * D.H;
+    int64_t dp = 0;
+    int bad = 0;
+    auto node = [&](int X, int Y) -> size_t {
+        const int t = D.tile0 + (Y / TH) * D.tiles_x + X / TW;
+        return (size_t)t * TN + (Y % TH) * TW + (X % TW);
+    };
+    auto inside = [&](int X, int Y) { return X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1; };
+    for (int64_t i = blockIdx.x * (int64_t)NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) {
+        const int X = (int)(i % D.W), Y = (int)(i / D.W);
+        if (!inside(X, Y)) continue;
+        const size_t gu = node(X, Y);
+        const int p0 = D.pos[i], q0 = D.neg[i], p1 = pos1[i], q1 = neg1[i];
+        bad |= tlinks_bad(p1, q1);
+        int de = (p1 - q1) - (p0 - q0);
+        dp += (int64_t)p1 - p0;
+        // pairs (u, right neighbour) and (u, lower neighbour)
+        for (int k = 0; k < 2; ++k) {
+            const int Xv = X + (k == 0), Yv = Y + (k == 1);
+            if (!inside(Xv, Yv)) continue;
+            const size_t iv = (size_t)Yv * D.W + Xv, gv = node(Xv, Yv);
+            const int c_uv0 = reframe_cap(k ? D.down : D.right, D.penalty, i);
+            const int c_vu0 = reframe_cap(k ? D.up : D.left, D.penalty, iv);
+            const int c_uv1 = reframe_cap(k ? down1 : right1, pen1, i);
+            const int c_vu1 = reframe_cap(k ? up1 : left1, pen1, iv);
+            bad |= (c_uv1 < 0) | (c_vu1 < 0);
+            int *ru = reinterpret_cast<int *>(&p.caps[gu]) + (k ? 3 : 1);  // u -> v: down / right
+            int *rv = reinterpret_cast<int *>(&p.caps[gv]) + (k ? 2 : 0);  // v -> u: up / left
+            const int g = c_uv0 - *ru;
+            const int g1 = min(max(g, -c_vu1), c_uv1);
+            *ru = c_uv1 - g1;
+            *rv = c_vu1 + g1;
+            if (g != g1) {
+                de += g - g1;
+                atomicSub(&p.exc[gv], g - g1);
+            }
+        }
+        if (de) atomicAdd(&p.exc[gu], de);
+    }
+    const int64_t tot = block_sum64(dp, red);
+    const int anybad = __syncthreads_or(bad);
+    if (threadIdx.x == 0) {
+        if (tot) atomic_add64(&p.state[slot].sum_pos, tot);
+        if (anybad) atomicOr(&p.state[slot].err, 1);
+    }
+}
+
+// After k_reframe: residual masks and the per-tile stuck-excess sums of the
+// instance from the new residual graph (one workgroup per tile).
+__global__ __launch_bounds__(NT) void k_remask(KParams p, int slot) {
+    __shared__ int64_t red[NT / 64];
+    const InstDesc D = p.desc[slot];
+    for (int lt = blockIdx.x; lt < D.ntiles; lt += gridDim.x) {
+        const int t = D.tile0 + lt;
+        const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
+        int64_t sx = 0;
+        for (int j = threadIdx.x; j < TN; j += NT) {
+            const size_t g = (size_t)t * TN + j;
+            const int X = tx * TW + j % TW, Y = ty * TH + j / TW;
+            const int e = p.exc[g];
+            p.mask[g] = (uint8_t)node_mask(p.caps[g], e);
+            if (X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1) sx += max(e, 0);
+        }
+        const int64_t tot = block_sum64(sx, red);
+        if (threadIdx.x == 0) p.tsum[2 * (size_t)t + 1] = tot;
+        __syncthreads();
+    }
+}
+
 // Residual state in image layout (certificates and debugging).
 __global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *exc_out, int32_t *dist_out,
                              int32_t *caps8_out) {
@@ -2076,23 +2173,34 @@ void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_til
     hipLaunchKernelGGL(k_build, dim3(grid), dim3(NT), 0, s, p, d_slots, d_slots + n, n, total_tiles);
 }
 
-void launch_sched(const KParams &p, int parity, hipStream_t s) {
-    hipLaunchKernelGGL(k_sched, dim3(p.n_slots), dim3(NT), 0, s, p, parity);
-}
-void launch_pr(const KParams &p, int grid, int parity, hipStream_t s) {
-    if (p.variant)
-        hipLaunchKernelGGL(k_prw, dim3(grid), dim3(64), 0, s, p, parity, (const int32_t *)p.queue_pr,
-                           (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
-    else if (p.nbhd == 8)
-        hipLaunchKernelGGL(k_pr8, dim3(grid), dim3(NT), 0, s, p, parity);
+// With events (profile mode) a launch goes through hipExtLaunchKernelGGL,
+// whose start/stop events are stamped by the kernel's own dispatch, so the
+// measured time is the kernel's execution and not the gap between two queued
+// event records.
+template <typename F, typename... Args>
+static void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t s, const LaunchEvents &ev, Args... args) {
+    if (ev.start)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, s, ev.start, ev.stop, 0, args...);
     else
-        hipLaunchKernelGGL(k_pr, dim3(grid), dim3(NT), 0, s, p, parity);
+        hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+}
+
+void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvents &ev) {
+    launch_timed(k_sched, dim3(p.n_slots), dim3(NT), s, ev, p, parity);
+}
+void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
+    if (p.variant)
+        launch_timed(k_prw, dim3(grid), dim3(64), s, ev, p, parity, (const int32_t *)p.queue_pr,
+                     (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
+    else if (p.nbhd == 8)
+        launch_timed(k_pr8, dim3(grid), dim3(NT), s, ev, p, parity);
+    else
+        launch_timed(k_pr, dim3(grid), dim3(NT), s, ev, p, parity);
 }
 template <int BX, int BY>
-static void launch_grb(const KParams &p, int grid, int parity, hipStream_t s) {
-    hipLaunchKernelGGL((k_grb<BX, BY>), dim3(grid), dim3(64 * BX * BY), 0, s, p, parity,
-                       (const int32_t *)p.queue_gr, (const TileInfo *)p.tiles, (const InstDesc *)p.desc,
-                       (const InstState *)p.state);
+static void launch_grb(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
+    launch_timed(k_grb<BX, BY>, dim3(grid), dim3(64 * BX * BY), s, ev, p, parity, (const int32_t *)p.queue_gr,
+                 (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
 }
 
 bool gr_block_supported(int bx, int by) {
@@ -2102,32 +2210,32 @@ bool gr_block_supported(int bx, int by) {
     }
 }
 
-void launch_gr(const KParams &p, int grid, int parity, hipStream_t s) {
+void launch_gr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
     if (!p.variant) {
         if (p.nbhd == 8)
-            hipLaunchKernelGGL(k_gr8, dim3(grid), dim3(NT), 0, s, p, parity);
+            launch_timed(k_gr8, dim3(grid), dim3(NT), s, ev, p, parity);
         else
-            hipLaunchKernelGGL(k_gr, dim3(grid), dim3(NT), 0, s, p, parity);
+            launch_timed(k_gr, dim3(grid), dim3(NT), s, ev, p, parity);
         return;
     }
     switch (p.gbx * 100 + p.gby) {
-        case 101: launch_grb<1, 1>(p, grid, parity, s); break;
-        case 102: launch_grb<1, 2>(p, grid, parity, s); break;
-        case 104: launch_grb<1, 4>(p, grid, parity, s); break;
-        case 108: launch_grb<1, 8>(p, grid, parity, s); break;
-        case 116: launch_grb<1, 16>(p, grid, parity, s); break;
-        case 202: launch_grb<2, 2>(p, grid, parity, s); break;
-        case 204: launch_grb<2, 4>(p, grid, parity, s); break;
-        case 208: launch_grb<2, 8>(p, grid, parity, s); break;
-        case 404: launch_grb<4, 4>(p, grid, parity, s); break;
+        case 101: launch_grb<1, 1>(p, grid, parity, s, ev); break;
+        case 102: launch_grb<1, 2>(p, grid, parity, s, ev); break;
+        case 104: launch_grb<1, 4>(p, grid, parity, s, ev); break;
+        case 108: launch_grb<1, 8>(p, grid, parity, s, ev); break;
+        case 116: launch_grb<1, 16>(p, grid, parity, s, ev); break;
+        case 202: launch_grb<2, 2>(p, grid, parity, s, ev); break;
+        case 204: launch_grb<2, 4>(p, grid, parity, s, ev); break;
+        case 208: launch_grb<2, 8>(p, grid, parity, s, ev); break;
+        case 404: launch_grb<4, 4>(p, grid, parity, s, ev); break;
         default: break;  // rejected when the options are set
     }
 }
 
 void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s) {
-    launch_sched(p, parity, s);
-    launch_pr(p, grid_pr, parity, s);
-    launch_gr(p, grid_gr, parity, s);
+    launch_sched(p, parity, s, LaunchEvents{});
+    launch_pr(p, grid_pr, parity, s, LaunchEvents{});
+    launch_gr(p, grid_gr, parity, s, LaunchEvents{});
 }
 void launch_reset_stats(const KParams &p, hipStream_t s) { hipLaunchKernelGGL(k_reset_stats, dim3(1), dim3(1), 0, s, p); }
 
@@ -2139,6 +2247,13 @@ void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_str
                        n_strokes, delta);
     hipLaunchKernelGGL(k_edit_masks, dim3(n_strokes < 1024 ? n_strokes : 1024), dim3(NT), 0, s, p, slot, d_strokes,
                        n_strokes);
+}
+
+void launch_reframe(const KParams &p, int slot, int ntiles, const int32_t *const *in1, int32_t pen1,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_reframe, dim3(1024), dim3(NT), 0, s, p, slot, in1[0], in1[1], in1[2], in1[3], in1[4],
+                       in1[5], pen1);
+    hipLaunchKernelGGL(k_remask, dim3(ntiles < 4096 ? ntiles : 4096), dim3(NT), 0, s, p, slot);
 }
 
 void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, int32_t *d_dist,

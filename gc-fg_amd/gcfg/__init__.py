@@ -673,6 +673,17 @@ class Server:
         self._pending[rid.value] = (st, label)
         return rid.value, label
 
+    def submit_frame(self, session: int, rgb: np.ndarray):
+        """Next video frame of a session (same W x H): warm start from its residual graph."""
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+        H, W = rgb.shape[:2]
+        label = np.empty((H, W), np.int32)
+        rid = ctypes.c_int64()
+        _check(lib().gcfg_server_submit_frame(_VP(self.h), int(session), W, H, _VP(rgb.ctypes.data),
+                                              _VP(label.ctypes.data), ctypes.byref(rid)), "gcfg_server_submit_frame")
+        self._pending[rid.value] = (rgb, label)
+        return rid.value, label
+
     def poll(self, wait: bool = False, max_results: int = 256):
         res = (Result * max_results)()
         n = ctypes.c_int32()

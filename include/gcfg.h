@@ -260,6 +260,17 @@ int gcfg_server_submit_image(gcfg_server *s, int32_t width, int32_t height, cons
  * on the stroke pixels. Warm-starts from the device-resident residual graph. */
 int gcfg_server_submit_edit(gcfg_server *s, int32_t session, int32_t n_strokes, const int32_t *strokes,
                             int32_t delta, int32_t *h_label, int64_t *request_id);
+/* Next video frame of a live session (the loop of ComplexApplication.cu:202-207,
+ * which re-runs setImage / getDataTermsMixture / GC_SetGraph / GC_Optimize per
+ * frame): a W x H RGB image of the session's size (host, valid until the
+ * request is admitted). The data term is recomputed on the device with the
+ * colour model of the session's first image, and the device-resident residual
+ * graph is reparametrised onto the new capacities (flow beyond a new n-link
+ * capacity returns to its tail node) instead of rebuilt, then solved warm.
+ * Strokes of earlier edits stay applied. Labels and flow are those of a cold
+ * solve of the same capacities. */
+int gcfg_server_submit_frame(gcfg_server *s, int32_t session, int32_t width, int32_t height, const uint8_t *h_rgb,
+                             int32_t *h_label, int64_t *request_id);
 /* Advance the device and collect finished requests (non-blocking if wait == 0). */
 int gcfg_server_poll(gcfg_server *s, int32_t wait, int32_t max_results, gcfg_result *results, int32_t *n_results);
 /* Drop a session and return its memory to the budget. */

@@ -75,6 +75,10 @@ def _load():
         lib.oracle_dataterm_kde.restype = ctypes.c_int
         lib.oracle_dataterm_kde.argtypes = [ctypes.c_int, ctypes.c_int, _U8P, ctypes.c_int, _U8P, ctypes.c_int,
                                             _I32P, _I32P, _F64P, _F64P, _F64P]
+        lib.oracle_dataterm_model.restype = ctypes.c_int
+        lib.oracle_dataterm_model.argtypes = [ctypes.c_int, ctypes.c_int, _U8P, _U8P, _U8P, ctypes.c_int32, _I32P,
+                                              _I32P, _I64P, _I64P, _F64P, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P,
+                                              _F64P, _F64P, _F64P, _F64P]
         lib.oracle_refpr_solve.restype = ctypes.c_int
         lib.oracle_refpr_solve.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P,
                                            ctypes.c_int32, ctypes.c_int, _I32P, _I64P, _I64P]
@@ -167,10 +171,13 @@ def cut_value8(width, height, pos, neg, label, dirs8=None, penalty=0) -> int:
     return v
 
 
-def dataterm(width, height, rgb, seeds, lam=50):
-    """Per-pixel double-precision data term. Returns a dict of numpy arrays."""
+def dataterm(width, height, rgb, seeds, lam=50, model_rgb=None):
+    """Per-pixel double-precision data term. Returns a dict of numpy arrays.
+    ``model_rgb``: take the seed histograms from this image instead (a video
+    session's first frame)."""
     lib = _load()
     rgb = np.ascontiguousarray(rgb, dtype=np.uint8).ravel()
+    model = rgb if model_rgb is None else np.ascontiguousarray(model_rgb, dtype=np.uint8).ravel()
     seeds = np.ascontiguousarray(seeds, dtype=np.uint8).ravel()
     n = width * height
     out = {
@@ -182,7 +189,8 @@ def dataterm(width, height, rgb, seeds, lam=50):
         "right_f": np.zeros(n, np.float64), "down_f": np.zeros(n, np.float64),
     }
     bsum, pairs, beta = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_double(0.0)
-    rc = lib.oracle_dataterm(width, height, _ptr(rgb, ctypes.c_uint8), _ptr(seeds, ctypes.c_uint8), int(lam),
+    rc = lib.oracle_dataterm_model(width, height, _ptr(rgb, ctypes.c_uint8), _ptr(model, ctypes.c_uint8),
+                                   _ptr(seeds, ctypes.c_uint8), int(lam),
                              _ptr(out["hist"], ctypes.c_int32), _ptr(out["counts"], ctypes.c_int32),
                              ctypes.byref(bsum), ctypes.byref(pairs), ctypes.byref(beta),
                              *[_ptr(out[k], ctypes.c_int32) for k in ("pos", "neg", "up", "down", "left", "right")],

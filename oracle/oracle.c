@@ -275,7 +275,17 @@ int oracle_dataterm(int width, int height, const uint8_t *rgb, const uint8_t *se
                     int32_t *hist_out, int32_t *counts_out, int64_t *beta_sum_out, int64_t *pairs_out,
                     double *beta_out, int32_t *pos, int32_t *neg, int32_t *up, int32_t *down, int32_t *left,
                     int32_t *right, double *tpos_f, double *tneg_f, double *right_f, double *down_f) {
-    if (width <= 0 || height <= 0 || !rgb || !seeds) return -1;
+    return oracle_dataterm_model(width, height, rgb, rgb, seeds, lambda, hist_out, counts_out, beta_sum_out,
+                                 pairs_out, beta_out, pos, neg, up, down, left, right, tpos_f, tneg_f, right_f,
+                                 down_f);
+}
+
+int oracle_dataterm_model(int width, int height, const uint8_t *rgb, const uint8_t *model_rgb,
+                          const uint8_t *seeds, int32_t lambda, int32_t *hist_out, int32_t *counts_out,
+                          int64_t *beta_sum_out, int64_t *pairs_out, double *beta_out, int32_t *pos, int32_t *neg,
+                          int32_t *up, int32_t *down, int32_t *left, int32_t *right, double *tpos_f,
+                          double *tneg_f, double *right_f, double *down_f) {
+    if (width <= 0 || height <= 0 || !rgb || !model_rgb || !seeds) return -1;
     int32_t hist[2][3][16];
     int32_t counts[2] = {0, 0};
     memset(hist, 0, sizeof(hist));
@@ -285,7 +295,7 @@ int oracle_dataterm(int width, int height, const uint8_t *rgb, const uint8_t *se
         if (s != 1 && s != 2) continue;
         const int k = s - 1;
         counts[k]++;
-        for (int c = 0; c < 3; ++c) hist[k][c][rgb[3 * i + c] >> 4]++;
+        for (int c = 0; c < 3; ++c) hist[k][c][model_rgb[3 * i + c] >> 4]++;
     }
     int64_t bsum = 0, npairs = 0;
     for (int y = 0; y < height; ++y)

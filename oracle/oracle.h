@@ -67,6 +67,14 @@ int oracle_dataterm(int width, int height, const uint8_t *rgb, const uint8_t *se
                     int32_t *pos, int32_t *neg, int32_t *up, int32_t *down, int32_t *left, int32_t *right,
                     double *tpos_f, double *tneg_f, double *right_f, double *down_f);
 
+/* As oracle_dataterm, with the histograms taken from the seed pixels of
+ * model_rgb (a video session's first frame) and everything else from rgb. */
+int oracle_dataterm_model(int width, int height, const uint8_t *rgb, const uint8_t *model_rgb,
+                          const uint8_t *seeds, int32_t lambda, int32_t *hist, int32_t *counts, int64_t *beta_sum,
+                          int64_t *pairs, double *beta, int32_t *pos, int32_t *neg, int32_t *up, int32_t *down,
+                          int32_t *left, int32_t *right, double *tpos_f, double *tneg_f, double *right_f,
+                          double *down_f);
+
 /* Kernel-density colour model of the reference (DataTermKernels.cu:19-31,
  * 61-83, 116-150): g101 receives the 101 kernel values for sigma; the data
  * term sums g[floor(|I - s|)] over the nseeds seed colours (seed_rgb, 3 bytes
