@@ -50,6 +50,8 @@ GR_RELAX_BYTES_PER_NODE = 9  # relaxation sweep: mask 1 + distance 4 read, dista
 LABEL_BYTES_PER_NODE = 4    # label written by every sweep of an exact round and by FIN
 COMPULSORY_BYTES_PER_PX = 28  # 6 int32 inputs read + int32 label written
 CHILD_SOLVES = 3  # solves of the counter child run: --steps 1 --warmup 0, then two event-timed solves
+BUILD_IN_BYTES_PER_PX = 24  # pos, neg and four n-link planes read by the build
+BUILD_OUT_BYTES_PER_NODE = 25  # caps 16 + excess 4 + distance 4 + mask 1 written per arena node
 
 
 def log(msg):
@@ -531,18 +533,28 @@ def main():
                        (pst.gr_tiles - pst.gr_init_tiles) * GR_RELAX_BYTES_PER_NODE +
                        (pst.gr_exact_tiles + pst.fin_items) * LABEL_BYTES_PER_NODE)
     wave = opts.kernels != 0
-    kern = {("k_prw" if wave else "k_pr"): (pst.pr_kernel_ms, pr_bytes),
-            ("k_grb" if wave else "k_gr"): (pst.gr_kernel_ms, gr_bytes)}
+    # Build: the caller's six input planes read, residual graph written
+    # (arena nodes incl. padding), labels too when the first sweep is exact.
+    build_bytes = BUILD_IN_BYTES_PER_PX * total_px + BUILD_OUT_BYTES_PER_NODE * n_tiles * 1024 + \
+        (LABEL_BYTES_PER_NODE * total_px if opts.gr_cap == 0 else 0)
+    # name: (event-timed ms per solve, algorithmic bytes per solve, launches per solve)
+    kern = {("k_buildw" if wave else "k_build"): (pst.build_kernel_ms, build_bytes, 1),
+            ("k_prw" if wave else "k_pr"): (pst.pr_kernel_ms, pr_bytes, int(pst.launches)),
+            ("k_grb" if wave else "k_gr"): (pst.gr_kernel_ms, gr_bytes, int(pst.launches))}
+    per_kernel = {k: dict(ms=round(v[0], 4), gb_s=round(v[1] / (v[0] * 1e-3) / 1e9, 1) if v[0] > 0 else None,
+                          frac=round(v[1] / (v[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if v[0] > 0 else None,
+                          bytes=int(v[1]), launches=v[2]) for k, v in kern.items()}
     dom = max(kern, key=lambda k: kern[k][0])
-    dms, dbytes = kern[dom]
+    dms, dbytes, dlaunch = kern[dom]
     achieved = dbytes / (dms * 1e-3) / 1e9 if dms > 0 else 0.0
     solve_bytes = COMPULSORY_BYTES_PER_PX * total_px + pr_bytes + gr_bytes
     roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    avg_launch_us=round(dms * 1e3 / max(pst.launches, 1), 2),
-                    bytes_per_launch=int(dbytes / max(pst.launches, 1)), launches=int(pst.launches),
+                    avg_launch_us=round(dms * 1e3 / max(dlaunch, 1), 2),
+                    bytes_per_launch=int(dbytes / max(dlaunch, 1)), launches=int(dlaunch),
+                    kernels=per_kernel, steps=int(pst.launches),
                     pr_kernel_ms=round(pst.pr_kernel_ms, 3), gr_kernel_ms=round(pst.gr_kernel_ms, 3),
-                    sched_kernel_ms=round(pst.sched_kernel_ms, 3),
+                    sched_kernel_ms=round(pst.sched_kernel_ms, 3), build_kernel_ms=round(pst.build_kernel_ms, 3),
                     solve_alg_frac=round(solve_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     compulsory_frac=round(COMPULSORY_BYTES_PER_PX * total_px / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
                     pr_tiles=int(pst.pr_tiles), gr_tiles=int(pst.gr_tiles),
@@ -551,8 +563,9 @@ def main():
     assert np.array_equal(flows_p, flows_b), "profiled solve differs"
     pbatch.close()
     if not args.no_pmc:
-        name = {"k_prw": "k_prw(", "k_pr": "k_pr(", "k_grb": "k_grb<", "k_gr": "k_gr("}[dom]
-        traffic, note = pmc_traffic(args, opts, name, int(pst.launches))
+        name = {"k_prw": "k_prw(", "k_pr": "k_pr(", "k_grb": "k_grb<", "k_gr": "k_gr(", "k_buildw": "k_buildw(",
+                "k_build": "k_build("}[dom]
+        traffic, note = pmc_traffic(args, opts, name, dlaunch)
         if traffic:
             roofline["traffic"] = traffic["bytes"]
             roofline["traffic_detail"] = traffic

@@ -259,6 +259,7 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
         stats->pr_kernel_ms = g->eng.prof_pr_ms;
         stats->gr_kernel_ms = g->eng.prof_gr_ms;
         stats->sched_kernel_ms = g->eng.prof_sched_ms;
+        stats->build_kernel_ms = g->eng.prof_build_ms;
         stats->launches = g->eng.steps_run - steps0;
     }
     return GCFG_OK;
@@ -434,6 +435,7 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
     agg.pr_kernel_ms = b->eng.prof_pr_ms;
     agg.gr_kernel_ms = b->eng.prof_gr_ms;
     agg.sched_kernel_ms = b->eng.prof_sched_ms;
+    agg.build_kernel_ms = b->eng.prof_build_ms;
     agg.launches = b->eng.steps_run - steps0;
     if (stats) *stats = agg;
     return GCFG_OK;

@@ -31,6 +31,8 @@ Engine::~Engine() {
     if (stream_) hipStreamSynchronize(stream_);
     drop_graphs();
     for (hipEvent_t e : events_) hipEventDestroy(e);
+    for (hipEvent_t e : build_ev_)
+        if (e) hipEventDestroy(e);
     for (void *p : allocs_) hipFree(p);
     if (pin_desc_) hipHostFree(pin_desc_);
     if (pin_slots_) hipHostFree(pin_slots_);
@@ -107,6 +109,7 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o, int nbhd) 
     rc |= dalloc((void **)&kp_.tsum, (size_t)cap_tiles * 2 * sizeof(int64_t));
     rc |= dalloc((void **)&kp_.queue_pr, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.queue_gr, (size_t)cap_tiles * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.bmap, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.desc, (size_t)cap_slots * sizeof(InstDesc));
     rc |= dalloc((void **)&kp_.state, (size_t)cap_slots * sizeof(InstState));
     rc |= dalloc((void **)&kp_.ctl, sizeof(Ctl));
@@ -321,7 +324,14 @@ int Engine::build(const std::vector<int> &slots) {
     apply_options();
     launch_reset_stats(kp_, stream_);
     launch_arm(kp_, d_slots_, n, 0, opt.sweeps_per_gr, stream_);
-    launch_build(kp_, d_slots_, n, total, stream_);
+    LaunchEvents ev;
+    if (opt.profile) {
+        for (auto &e : build_ev_)
+            if (!e) GCFG_HIP(hipEventCreate(&e));
+        ev = LaunchEvents{build_ev_[0], build_ev_[1]};
+        build_timed_ = true;
+    }
+    launch_build(kp_, d_slots_, n, total, stream_, ev);
     GCFG_HIP(hipGetLastError());
     return GCFG_OK;
 }
@@ -489,6 +499,12 @@ int Engine::run(int64_t max_steps) {
 }
 
 int Engine::collect_profile() {
+    if (build_timed_) {
+        float b = 0;
+        GCFG_HIP(hipEventElapsedTime(&b, build_ev_[0], build_ev_[1]));
+        prof_build_ms += b;
+        build_timed_ = false;
+    }
     for (int i = 0; i < events_pending_; ++i) {
         float a = 0, b = 0, c = 0;
         GCFG_HIP(hipEventElapsedTime(&a, events_[6 * i], events_[6 * i + 1]));

@@ -115,6 +115,7 @@ struct KParams {
     InstState *state;
     Ctl *ctl;
     int32_t *queue_pr, *queue_gr;
+    int32_t *bmap;  // build list position -> tile id (written by k_clear_tiles for k_buildw)
     int32_t n_slots;
     int32_t inner_iters;
     int32_t variant;  // 0: workgroup-per-tile kernels (LDS), 1: wavefront-per-tile kernels (registers)
@@ -138,7 +139,8 @@ struct LaunchEvents {
     hipEvent_t start = nullptr, stop = nullptr;
 };
 void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int sweeps, hipStream_t s);
-void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_tiles, hipStream_t s);
+void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_tiles, hipStream_t s,
+                  const LaunchEvents &ev);
 void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s);
 void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvents &ev);
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
@@ -242,9 +244,14 @@ class Engine {
     // steps.
     int steps_hint = 0;
     // Event timing of the launches since the last reset (profile mode).
-    double prof_sched_ms = 0, prof_pr_ms = 0, prof_gr_ms = 0;
+    double prof_sched_ms = 0, prof_pr_ms = 0, prof_gr_ms = 0, prof_build_ms = 0;
     int64_t prof_steps = 0;
-    void reset_profile() { prof_sched_ms = prof_pr_ms = prof_gr_ms = 0; prof_steps = 0; }
+    // Keeps a timed build that has not been collected yet: the plain graph
+    // builds in one call and solves (resetting the profile) in the next.
+    void reset_profile() {
+        prof_sched_ms = prof_pr_ms = prof_gr_ms = prof_build_ms = 0;
+        prof_steps = 0;
+    }
 
    private:
     int alloc_tiles(int n);
@@ -284,6 +291,8 @@ class Engine {
     std::vector<hipEvent_t> events_;  // 6 per profiled step: start/stop of sched, pr, gr
     int events_pending_ = 0;
     QCtl *pin_qtrace_ = nullptr;      // queue lengths per profiled step (verbose >= 2)
+    hipEvent_t build_ev_[2] = {nullptr, nullptr};  // the build kernel's start / stop (profile mode)
+    bool build_timed_ = false;
     int qtrace_cap_ = 0;
 };
 

@@ -29,6 +29,17 @@ namespace gcfg {
 
 constexpr int GR_MAX_ITERS = 4 * (TW + TH);
 
+// Caller input planes and label buffers are plain device memory, reached
+// through the descriptor's generic pointers: cast them to the global address
+// space so that the accesses are global (not flat) loads and stores, which do
+// not also count against the LDS/scalar counter and force wider waits.
+__device__ __forceinline__ int ldg_in(const int32_t *p, size_t i) {
+    return ((const __attribute__((address_space(1))) int32_t *)p)[i];
+}
+__device__ __forceinline__ void stg_out(int32_t *p, size_t i, int v) {
+    ((__attribute__((address_space(1))) int32_t *)p)[i] = v;
+}
+
 __device__ __forceinline__ int64_t wave_sum64(int64_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -630,7 +641,7 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p, int parity) {
                 if (X < W && Y < H) {
                     const size_t g = base + y * TW + lane;
                     const bool in = X > 0 && Y > 0 && X < W - 1 && Y < H - 1;
-                    lab[(size_t)Y * W + X] = (in && p.dist[g] >= DINF) ? 1 : 0;
+                    stg_out(lab, (size_t)Y * W + X, (in && p.dist[g] >= DINF) ? 1 : 0);
                     if (in) sx += max(p.exc[g], 0);
                 }
             }
@@ -1002,7 +1013,7 @@ __global__ __launch_bounds__(NT) void k_gr8(KParams p, int parity) {
                 if (X < W && Y < H) {
                     const size_t g = base + y * TW + lane;
                     const bool in = X > 0 && Y > 0 && X < W - 1 && Y < H - 1;
-                    lab[(size_t)Y * W + X] = (in && p.dist[g] >= DINF) ? 1 : 0;
+                    stg_out(lab, (size_t)Y * W + X, (in && p.dist[g] >= DINF) ? 1 : 0);
                     if (in) sx += max(p.exc[g], 0);
                 }
             }
@@ -1452,7 +1463,7 @@ __device__ __forceinline__ void write_labels(const InstDesc &D, int tx, int ty, 
         const int Y = ty * TH + y;
         if (Y < D.H) {
             const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
-            D.label_out[(size_t)Y * D.W + X] = (in && d[y] >= DINF) ? 1 : 0;
+            stg_out(D.label_out, (size_t)Y * D.W + X, (in && d[y] >= DINF) ? 1 : 0);
         }
     }
 }
@@ -1851,15 +1862,17 @@ __global__ void k_clear_tiles(KParams p, const int32_t *slots, const int32_t *pr
         p.dirty[t] = 0;
         p.live[t] = 0;  // raised by the build sweep of the block's tiles
         p.stale[t] = 0;
+        p.bmap[b] = t;
     }
 }
+
 
 // Initial BFS distance of an input pixel: 0 if it is an interior node with
 // pos < neg (remaining sink capacity), otherwise unreachable so far.
 __device__ __forceinline__ int input_d0(const InstDesc &D, int X, int Y) {
     const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
     const size_t i = in ? (size_t)Y * D.W + X : 0;  // clamped: the load is always in bounds
-    const int pp = D.pos[i], qq = D.neg[i];
+    const int pp = ldg_in(D.pos, i), qq = ldg_in(D.neg, i);
     return (in && pp < qq) ? 0 : DINF;
 }
 
@@ -1878,12 +1891,13 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
     for (int jj = by_xcd ? blockIdx.x / NUM_XCD : blockIdx.x; jj < jend; jj += jstep) {
         const int b = b0 + jj;
         if (b >= total) break;
-        const int j = find_inst(prefix, n, b);
-        const int s = slots[j];
+        // Tile, instance and tile coordinates from the map and the tile
+        // records k_clear_tiles wrote just before (two dependent loads).
+        const int t = p.bmap[b];
+        const TileInfo ti = p.tiles[t];
+        const int s = ti.inst;
         const InstDesc D = p.desc[s];
-        const int lt = b - prefix[j];
-        const int t = D.tile0 + lt;
-        const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
+        const int tx = ti.tx, ty = ti.ty;
         const int X = tx * TW + lane, Y0 = ty * TH;
         const TileRef T = tile_ref(p, t);
         int d[TH];
@@ -1911,10 +1925,10 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
                 const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
                 const size_t i = in ? (size_t)Y * D.W + X : 0;
                 vin[k] = in;
-                vp[k] = D.pos[i];
-                vn[k] = D.neg[i];
+                vp[k] = ldg_in(D.pos, i);
+                vn[k] = ldg_in(D.neg, i);
                 if (D.up)
-                    vc[k] = make_int4(D.left[i], D.right[i], D.up[i], D.down[i]);
+                    vc[k] = make_int4(ldg_in(D.left, i), ldg_in(D.right, i), ldg_in(D.up, i), ldg_in(D.down, i));
                 else
                     vc[k] = make_int4(D.penalty, D.penalty, D.penalty, D.penalty);
             }
@@ -2159,18 +2173,23 @@ void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int s
     hipLaunchKernelGGL(k_arm, dim3((n + 255) / 256), dim3(256), 0, s, p, d_slots, n, warm, sweeps, start);
 }
 
-void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_tiles, hipStream_t s) {
+template <typename F, typename... Args>
+static void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t s, const LaunchEvents &ev, Args... args);
+
+void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_tiles, hipStream_t s,
+                  const LaunchEvents &ev) {
     if (n <= 0 || total_tiles <= 0) return;
     if (p.variant) {
         hipLaunchKernelGGL(k_clear_tiles, dim3((total_tiles + 255) / 256), dim3(256), 0, s, p, d_slots, d_slots + n, n,
                            total_tiles);
         int grid = total_tiles < 16384 ? total_tiles : 16384;
         if (grid >= NUM_XCD) grid -= grid % NUM_XCD;  // whole rounds over the XCDs
-        hipLaunchKernelGGL(k_buildw, dim3(grid), dim3(64), 0, s, p, d_slots, d_slots + n, n, total_tiles);
+        launch_timed(k_buildw, dim3(grid), dim3(64), s, ev, p, d_slots, (const int32_t *)(d_slots + n), n,
+                     total_tiles);
         return;
     }
     const int grid = total_tiles < 8192 ? total_tiles : 8192;
-    hipLaunchKernelGGL(k_build, dim3(grid), dim3(NT), 0, s, p, d_slots, d_slots + n, n, total_tiles);
+    launch_timed(k_build, dim3(grid), dim3(NT), s, ev, p, d_slots, (const int32_t *)(d_slots + n), n, total_tiles);
 }
 
 // With events (profile mode) a launch goes through hipExtLaunchKernelGGL,

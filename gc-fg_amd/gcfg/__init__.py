@@ -56,7 +56,8 @@ class Stats(ctypes.Structure):
                 ("pr_iters", ctypes.c_int64), ("steps", ctypes.c_int32), ("gr_rounds", ctypes.c_int32),
                 ("ms", ctypes.c_double), ("pr_kernel_ms", ctypes.c_double), ("gr_kernel_ms", ctypes.c_double),
                 ("sched_kernel_ms", ctypes.c_double), ("launches", ctypes.c_int64), ("gr_iters", ctypes.c_int64),
-                ("gr_init_tiles", ctypes.c_int64), ("gr_exact_tiles", ctypes.c_int64), ("fin_items", ctypes.c_int64)]
+                ("gr_init_tiles", ctypes.c_int64), ("gr_exact_tiles", ctypes.c_int64), ("fin_items", ctypes.c_int64),
+                ("build_kernel_ms", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -75,6 +75,7 @@ typedef struct gcfg_stats {
     int64_t gr_init_tiles; /* of gr_tiles, the initial sweeps of a global relabel (the rest relax) */
     int64_t gr_exact_tiles; /* of gr_tiles, those of exact (uncapped) rounds, which also write labels */
     int64_t fin_items;     /* FIN items: blocks left unreachable (wavefront kernels) or every tile */
+    double build_kernel_ms; /* graph build kernel (GC_SetGraph's work), when profile is set */
     /* gr_* and fin_items count relabel items: blocks of gr_block_w x gr_block_h tiles */
 } gcfg_stats;
 

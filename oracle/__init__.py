@@ -70,6 +70,12 @@ def _load():
         lib.oracle_cut_value8.restype = ctypes.c_int64
         lib.oracle_cut_value8.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, ctypes.c_void_p,
                                           ctypes.c_int32, _I32P]
+        lib.oracle_maxflow_grid_state.restype = ctypes.c_int
+        lib.oracle_maxflow_grid_state.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, _I32P, _I32P, _I32P,
+                                                  _I32P, ctypes.c_int32, _I32P, _I64P, _I32P, _I32P]
+        lib.oracle_certify_grid.restype = ctypes.c_int
+        lib.oracle_certify_grid.argtypes = [ctypes.c_int, ctypes.c_int, _I32P, _I32P, _I32P, _I32P, _I32P, _I32P,
+                                            ctypes.c_int32, _I32P, _I32P, _I32P, ctypes.c_int64]
         lib.oracle_kde_gauss.restype = None
         lib.oracle_kde_gauss.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_float)]
         lib.oracle_dataterm_kde.restype = ctypes.c_int
@@ -120,6 +126,26 @@ def maxflow(width, height, pos, neg, edges=None, penalty=0):
     return int(flow.value), label.reshape(height, width)
 
 
+def maxflow_state(width, height, pos, neg, edges=None, penalty=0):
+    """maxflow plus the final residual state: (flow, label, caps[H, W, 4], excess[H, W])."""
+    lib = _load()
+    pos, neg = _i32(pos).ravel(), _i32(neg).ravel()
+    up, down, left, right = _edges(edges)
+    label = np.zeros(width * height, np.int32)
+    caps = np.zeros(width * height * 4, np.int32)
+    exc = np.zeros(width * height, np.int32)
+    flow = ctypes.c_int64(0)
+    rc = lib.oracle_maxflow_grid_state(width, height, _ptr(pos, ctypes.c_int32), _ptr(neg, ctypes.c_int32),
+                                       _ptr(up, ctypes.c_int32), _ptr(down, ctypes.c_int32),
+                                       _ptr(left, ctypes.c_int32), _ptr(right, ctypes.c_int32), int(penalty),
+                                       _ptr(label, ctypes.c_int32), ctypes.byref(flow), _ptr(caps, ctypes.c_int32),
+                                       _ptr(exc, ctypes.c_int32))
+    if rc != 0:
+        raise RuntimeError(f"oracle_maxflow_grid_state failed: {rc}")
+    return (int(flow.value), label.reshape(height, width), caps.reshape(height, width, 4),
+            exc.reshape(height, width))
+
+
 def cut_value(width, height, pos, neg, label, edges=None, penalty=0) -> int:
     lib = _load()
     pos, neg = _i32(pos).ravel(), _i32(neg).ravel()
@@ -129,6 +155,22 @@ def cut_value(width, height, pos, neg, label, edges=None, penalty=0) -> int:
                                     _ptr(up, ctypes.c_int32), _ptr(down, ctypes.c_int32),
                                     _ptr(left, ctypes.c_int32), _ptr(right, ctypes.c_int32), int(penalty),
                                     _ptr(label, ctypes.c_int32)))
+
+
+def certify(width, height, pos, neg, caps, excess, label, flow, edges=None, penalty=0) -> int:
+    """Optimality certificate of a solver's residual state (caps H x W x 4 in
+    left, right, up, down order; excess; labels; flow). 0 if it holds, else
+    the number of the first failing check (1 feasibility, 2 conservation,
+    3 labels, 4 flow / cut value)."""
+    lib = _load()
+    pos, neg = _i32(pos).ravel(), _i32(neg).ravel()
+    up, down, left, right = _edges(edges)
+    caps, exc, lab = _i32(caps).ravel(), _i32(excess).ravel(), _i32(label).ravel()
+    return int(lib.oracle_certify_grid(width, height, _ptr(pos, ctypes.c_int32), _ptr(neg, ctypes.c_int32),
+                                       _ptr(up, ctypes.c_int32), _ptr(down, ctypes.c_int32),
+                                       _ptr(left, ctypes.c_int32), _ptr(right, ctypes.c_int32), int(penalty),
+                                       _ptr(caps, ctypes.c_int32), _ptr(exc, ctypes.c_int32),
+                                       _ptr(lab, ctypes.c_int32), int(flow)))
 
 
 DIRS8 = ("left", "right", "up", "down", "upleft", "upright", "downleft", "downright")

@@ -101,7 +101,8 @@ static void global_relabel(pr_t *g) {
 }
 
 static int maxflow_core(int width, int height, const int32_t *pos, const int32_t *neg,
-                        const int32_t *const *planes, int ndir, int32_t penalty, int32_t *label, int64_t *flow_out) {
+                        const int32_t *const *planes, int ndir, int32_t penalty, int32_t *label, int64_t *flow_out,
+                        int32_t *caps_out, int32_t *exc_out) {
     if (width <= 0 || height <= 0 || !pos || !neg || !label || !flow_out) return -1;
     if (planes[0])
         for (int k = 1; k < ndir; ++k)
@@ -211,6 +212,12 @@ static int maxflow_core(int width, int height, const int32_t *pos, const int32_t
             se += g.e[i] > 0 ? g.e[i] : 0;
         }
     const int ok = (sp - se == flow);
+    /* Residual state in the solver's form: excess = e - remaining sink
+     * capacity (a node with e > 0 has none left), residual caps per arc. */
+    if (caps_out)
+        for (size_t i = 0; i < n * (size_t)K; ++i) caps_out[i] = (int32_t)g.r[i];
+    if (exc_out)
+        for (size_t i = 0; i < n; ++i) exc_out[i] = (int32_t)(g.e[i] - g.rt[i]);
     free(g.e), free(g.rt), free(g.r), free(g.d), free(g.queue), free(g.inq);
     *flow_out = flow;
     return ok ? 0 : -4;
@@ -222,7 +229,18 @@ int oracle_maxflow_grid(int width, int height, const int32_t *pos, const int32_t
     const int32_t *planes[8] = {left, right, up, down, NULL, NULL, NULL, NULL};
     if (up && (!down || !left || !right)) return -1;
     if (!up) planes[0] = planes[1] = planes[2] = planes[3] = NULL;
-    return maxflow_core(width, height, pos, neg, planes, 4, penalty, label, flow_out);
+    return maxflow_core(width, height, pos, neg, planes, 4, penalty, label, flow_out, NULL, NULL);
+}
+
+int oracle_maxflow_grid_state(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *up,
+                              const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
+                              int32_t *label, int64_t *flow_out, int32_t *caps_out, int32_t *exc_out) {
+    const int32_t *planes[8] = {left, right, up, down, NULL, NULL, NULL, NULL};
+    if (up && (!down || !left || !right)) return -1;
+    if (!up) planes[0] = planes[1] = planes[2] = planes[3] = NULL;
+    if (caps_out) memset(caps_out, 0, sizeof(int32_t) * 4 * (size_t)width * height);
+    if (exc_out) memset(exc_out, 0, sizeof(int32_t) * (size_t)width * height);
+    return maxflow_core(width, height, pos, neg, planes, 4, penalty, label, flow_out, caps_out, exc_out);
 }
 
 int oracle_maxflow_grid8(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *const *dirs8,
@@ -230,7 +248,7 @@ int oracle_maxflow_grid8(int width, int height, const int32_t *pos, const int32_
     const int32_t *planes[8] = {NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL};
     if (dirs8)
         for (int k = 0; k < 8; ++k) planes[k] = dirs8[k];
-    return maxflow_core(width, height, pos, neg, planes, 8, penalty, label, flow_out);
+    return maxflow_core(width, height, pos, neg, planes, 8, penalty, label, flow_out, NULL, NULL);
 }
 
 static int64_t cut_core(int width, int height, const int32_t *pos, const int32_t *neg,
@@ -422,5 +440,95 @@ int oracle_dataterm_kde(int width, int height, const uint8_t *rgb, int nseeds, c
         if (pos) pos[i] = trunc_sat(300000.0 * (double)pf);
         if (neg) neg[i] = trunc_sat(300000.0 * (double)(1.0f - pf));
     }
+    return 0;
+}
+
+/* Optimality certificate of a returned residual state, independent of how it
+ * was computed (for sizes where running oracle_maxflow_grid is too slow):
+ *  1. residuals are >= 0, zero on every arc into or out of the border, and
+ *     r(u,v) + r(v,u) == c(u,v) + c(v,u) for every interior pair;
+ *  2. conservation: excess(v) == pos(v) - neg(v) - sum_u (c(v,u) - r(v,u));
+ *  3. labels == interior nodes with no residual path to a node of excess < 0
+ *     (BFS backwards from those nodes), border labels 0;
+ *  4. the returned flow == sum_interior pos - sum_interior max(excess, 0) ==
+ *     the cut value of the labels.
+ * (1, 2) make the state a preflow of the graph in the reparametrised form
+ * (t-links pre-saturated); (3) says no excess node can reach a deficit node,
+ * so the preflow's value is a maximum flow and the cut is minimum; the label
+ * set is the unique "cannot reach the sink" set. Returns 0 if every check
+ * holds, else the number of the first failing check. caps: 4 per pixel
+ * (left, right, up, down), image layout. */
+int oracle_certify_grid(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *up,
+                        const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
+                        const int32_t *caps, const int32_t *excess, const int32_t *label, int64_t flow) {
+    const int32_t *planes[8] = {up ? left : NULL, right, up, down, NULL, NULL, NULL, NULL};
+    const size_t n = (size_t)width * height;
+    if (width < 3 || height < 3) {
+        for (size_t i = 0; i < n; ++i)
+            if (label[i]) return 3;
+        return flow == 0 ? 0 : 4;
+    }
+    int64_t sp = 0, se = 0;
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const size_t i = (size_t)y * width + x;
+            const int in = is_interior(x, y, width, height);
+            int64_t bal = 0;
+            for (int dir = 0; dir < 4; ++dir) {
+                const int64_t r = caps[4 * i + dir];
+                const int xx = x + DX[dir], yy = y + DY[dir];
+                const int open = in && is_interior(xx, yy, width, height);
+                if (r < 0 || (!open && r != 0)) return 1;
+                if (!open) continue;
+                const size_t j = (size_t)yy * width + xx;
+                const int64_t c = arc_cap(x, y, width, height, dir, planes, penalty);
+                const int64_t cb = arc_cap(xx, yy, width, height, OPP[dir], planes, penalty);
+                if (r + caps[4 * j + OPP[dir]] != c + cb) return 1;
+                bal += c - r; /* net flow out of i along this pair */
+            }
+            if (!in) continue;
+            if ((int64_t)excess[i] != (int64_t)pos[i] - neg[i] - bal) return 2;
+            sp += pos[i];
+            se += excess[i] > 0 ? excess[i] : 0;
+        }
+    /* 3: reverse BFS from deficit nodes over residual arcs u -> v */
+    uint8_t *reach = (uint8_t *)calloc(n, 1);
+    int32_t *q = (int32_t *)malloc(sizeof(int32_t) * n);
+    if (!reach || !q) {
+        free(reach), free(q);
+        return -2;
+    }
+    size_t head = 0, tail = 0;
+    for (int y = 1; y < height - 1; ++y)
+        for (int x = 1; x < width - 1; ++x) {
+            const size_t i = (size_t)y * width + x;
+            if (excess[i] < 0) reach[i] = 1, q[tail++] = (int32_t)i;
+        }
+    while (head < tail) {
+        const int32_t v = q[head++];
+        const int x = v % width, y = v / width;
+        for (int dir = 0; dir < 4; ++dir) {
+            const int ux = x + DX[dir], uy = y + DY[dir];
+            if (!is_interior(ux, uy, width, height)) continue;
+            const size_t u = (size_t)uy * width + ux;
+            if (reach[u] || caps[4 * u + OPP[dir]] <= 0) continue;
+            reach[u] = 1;
+            q[tail++] = (int32_t)u;
+        }
+    }
+    int bad = 0;
+    for (int y = 0; y < height && !bad; ++y)
+        for (int x = 0; x < width; ++x) {
+            const size_t i = (size_t)y * width + x;
+            const int want = is_interior(x, y, width, height) && !reach[i];
+            if (label[i] != want) {
+                bad = 1;
+                break;
+            }
+        }
+    free(reach), free(q);
+    if (bad) return 3;
+    if (flow != sp - se) return 4;
+    if (cut_core(width, height, pos, neg, planes, 4, penalty, label) != flow) return 4;
     return 0;
 }

@@ -50,6 +50,22 @@ int oracle_maxflow_grid8(int width, int height, const int32_t *pos, const int32_
 int64_t oracle_cut_value8(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *const *dirs8,
                           int32_t penalty, const int32_t *label);
 
+/* Optimality certificate of a residual state returned by a solver (caps: 4
+ * per pixel, left/right/up/down; excess; labels; flow): feasibility,
+ * conservation, labels == nodes that cannot reach a deficit node, flow ==
+ * sum pos - sum max(excess, 0) == cut of the labels. 0 if it holds, else the
+ * number (1-4) of the first failing check. */
+int oracle_certify_grid(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *up,
+                        const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
+                        const int32_t *caps, const int32_t *excess, const int32_t *label, int64_t flow);
+
+/* oracle_maxflow_grid that also returns its final residual state in the
+ * solver's form: caps_out[4*i + k] (left, right, up, down), exc_out[i] =
+ * excess (negative: remaining sink capacity). */
+int oracle_maxflow_grid_state(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *up,
+                              const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
+                              int32_t *label, int64_t *flow, int32_t *caps_out, int32_t *exc_out);
+
 /* Cut value of a 0/1 labelling (1 = source side) on the same graph. */
 int64_t oracle_cut_value(int width, int height, const int32_t *pos, const int32_t *neg, const int32_t *up,
                          const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,

@@ -260,6 +260,7 @@ def test_profile_mode_reports_kernel_time(g, ora):
     rflow, rlabel = ora.maxflow(W, H, pos, neg, edges=E)
     assert flow == rflow and np.array_equal(label, rlabel)
     assert st.pr_kernel_ms > 0 and st.gr_kernel_ms > 0 and st.launches >= st.steps
+    assert st.build_kernel_ms > 0
     assert st.pr_iters == 0 and st.gr_iters == 0  # counters only with profile 2
     o.profile = 2
     flow2, label2, st2 = g.solve(W, H, pos, neg, edges=E, options=o)

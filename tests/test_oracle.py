@@ -257,3 +257,69 @@ def test_against_networkx8(ora):
         want = nx.maximum_flow_value(G, "s", "t") if W > 2 and H > 2 else 0
         flow, label = ora.maxflow8(W, H, pos, neg, dirs)
         assert flow == want
+
+
+# ---------------------------------------------------------------------------
+# The residual-state certificate used for full-size GPU results.
+
+def test_certificate_accepts_oracle_states_and_rejects_corruptions(ora):
+    rng = np.random.default_rng(77)
+    for trial in range(8):
+        W, H = int(rng.integers(3, 40)), int(rng.integers(3, 30))
+        pos, neg, E = random_instance(rng, W, H, tmax=80, emax=30)
+        if trial % 2:
+            pos = pos - rng.integers(0, 30, pos.shape).astype(np.int32)  # negative t-links too
+        flow, lab, caps, exc = ora.maxflow_state(W, H, pos, neg, edges=E)
+        assert ora.certify(W, H, pos, neg, caps, exc, lab, flow, edges=E) == 0
+        if W < 4 or H < 4:
+            continue
+        # a flipped label
+        bad = lab.copy()
+        bad[H // 2, W // 2] ^= 1
+        assert ora.certify(W, H, pos, neg, caps, exc, bad, flow, edges=E) == 3
+        # a wrong flow value
+        assert ora.certify(W, H, pos, neg, caps, exc, lab, flow + 1, edges=E) == 4
+        # a residual that breaks the pair sum
+        c2 = caps.copy()
+        c2[H // 2, W // 2, 1] += 1
+        assert ora.certify(W, H, pos, neg, c2, exc, lab, flow, edges=E) == 1
+        # excess that breaks conservation
+        e2 = exc.copy()
+        e2[H // 2, W // 2] += 1
+        assert ora.certify(W, H, pos, neg, caps, e2, lab, flow, edges=E) == 2
+
+
+def test_certificate_rejects_a_non_maximum_preflow(ora):
+    """The untouched pre-saturated graph is a feasible preflow; unless it is
+    already maximum, its value is below the cut of the labels it implies."""
+    rng = np.random.default_rng(5)
+    W, H = 30, 20
+    pos, neg, E = random_instance(rng, W, H, tmax=80, emax=30)
+    up, down, left, right = E
+    inter = np.zeros((H, W), bool)
+    inter[1:-1, 1:-1] = True
+    caps = np.zeros((H, W, 4), np.int32)
+    for k, (a, (dx, dy)) in enumerate(zip((left, right, up, down), ((-1, 0), (1, 0), (0, -1), (0, 1)))):
+        ys, xs = np.mgrid[0:H, 0:W]
+        ok = inter & (xs + dx > 0) & (xs + dx < W - 1) & (ys + dy > 0) & (ys + dy < H - 1)
+        caps[..., k] = np.where(ok, a, 0)
+    exc = np.where(inter, pos - neg, 0).astype(np.int32)
+    # labels: cannot reach a deficit node in this residual graph
+    reach = inter & (exc < 0)
+    while True:
+        new = reach.copy()
+        for k, (dx, dy) in enumerate(((-1, 0), (1, 0), (0, -1), (0, 1))):
+            shifted = np.zeros_like(reach)
+            ys, xs = np.nonzero(reach)
+            us, ux = ys - dy, xs - dx
+            m = (us > 0) & (us < H - 1) & (ux > 0) & (ux < W - 1)
+            shifted[us[m], ux[m]] = True
+            new |= shifted & (caps[..., k] > 0)
+        if (new == reach).all():
+            break
+        reach = new
+    lab = (inter & ~reach).astype(np.int32)
+    value = int(pos[inter].sum() - np.maximum(exc[inter], 0).sum())
+    best = ora.maxflow(W, H, pos, neg, edges=E)[0]
+    assert value < best
+    assert ora.certify(W, H, pos, neg, caps, exc, lab, value, edges=E) == 4
