@@ -51,7 +51,8 @@ LABEL_BYTES_PER_NODE = 4    # label written by every sweep of an exact round and
 COMPULSORY_BYTES_PER_PX = 28  # 6 int32 inputs read + int32 label written
 CHILD_SOLVES = 3  # solves of the counter child run: --steps 1 --warmup 0, then two event-timed solves
 BUILD_IN_BYTES_PER_PX = 24  # pos, neg and four n-link planes read by the build
-BUILD_OUT_BYTES_PER_NODE = 25  # caps 16 + excess 4 + distance 4 + mask 1 written per arena node
+BUILD_OUT_BYTES_PER_NODE = 9  # excess 4 + distance 4 + mask 1 written per arena node (capacities stay lazy)
+BUILD_OUT_BYTES_PER_TILE = 640  # the tile's inflow slots cleared (160 int32)
 
 
 def log(msg):
@@ -536,6 +537,7 @@ def main():
     # Build: the caller's six input planes read, residual graph written
     # (arena nodes incl. padding), labels too when the first sweep is exact.
     build_bytes = BUILD_IN_BYTES_PER_PX * total_px + BUILD_OUT_BYTES_PER_NODE * n_tiles * 1024 + \
+        BUILD_OUT_BYTES_PER_TILE * n_tiles + \
         (LABEL_BYTES_PER_NODE * total_px if opts.gr_cap == 0 else 0)
     # name: (event-timed ms per solve, algorithmic bytes per solve, launches per solve)
     kern = {("k_buildw" if wave else "k_build"): (pst.build_kernel_ms, build_bytes, 1),

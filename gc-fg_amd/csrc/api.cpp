@@ -58,6 +58,7 @@ void gcfg_default_options(gcfg_options *o) {
     o->gr_cap = 0;
     o->pr_waves_per_cu = 0;
     o->gr_waves_per_cu = 0;
+    o->lazy_caps = 0;
 }
 
 const char *gcfg_last_error(void) { return gcfg::last_error_cstr(); }

@@ -110,6 +110,8 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o, int nbhd) 
     rc |= dalloc((void **)&kp_.queue_pr, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.queue_gr, (size_t)cap_tiles * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.bmap, (size_t)cap_tiles * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.mat, (size_t)cap_tiles * sizeof(int32_t));
+    rc |= dalloc((void **)&kp_.inflow, (size_t)cap_tiles * INFLOW * sizeof(int32_t));
     rc |= dalloc((void **)&kp_.desc, (size_t)cap_slots * sizeof(InstDesc));
     rc |= dalloc((void **)&kp_.state, (size_t)cap_slots * sizeof(InstState));
     rc |= dalloc((void **)&kp_.ctl, sizeof(Ctl));
@@ -148,6 +150,10 @@ bool options_valid(const gcfg_options &o) {
     }
     if (o.pr_waves_per_cu < 0 || o.gr_waves_per_cu < 0 || o.pr_waves_per_cu > 1024 || o.gr_waves_per_cu > 1024) {
         set_error("options: pr_waves_per_cu and gr_waves_per_cu must be in [0, 1024]");
+        return false;
+    }
+    if (o.lazy_caps < 0 || o.lazy_caps > 2) {
+        set_error("options: lazy_caps must be 0, 1 or 2");
         return false;
     }
     if (o.gr_cap < 0) {
@@ -322,6 +328,17 @@ int Engine::build(const std::vector<int> &slots) {
     GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)(2 * n) * sizeof(int32_t), hipMemcpyHostToDevice,
                             stream_));
     apply_options();
+    // Lazy capacities pay when the build's 16 B/node of capacity writes (about
+    // 3.7 ns per tile at the build's bandwidth) outweigh one more launch per
+    // step (about 4 us): judged by the previous solve's step count, or without
+    // one by size alone (32 k tiles = 32 M nodes). Large batches with few
+    // steps qualify; single images and long solves do not.
+    const int64_t need = steps_hint > 0 ? (int64_t)steps_hint * 1024 : 32768;
+    const int lazy = kp_.variant && (opt.lazy_caps == 2 || (opt.lazy_caps == 0 && (int64_t)total >= need)) ? 1 : 0;
+    if (lazy != kp_.lazy) {
+        kp_.lazy = lazy;
+        drop_graphs();  // the captured steps carry the flag and the launch sequence
+    }
     launch_reset_stats(kp_, stream_);
     launch_arm(kp_, d_slots_, n, 0, opt.sweeps_per_gr, stream_);
     LaunchEvents ev;
@@ -332,6 +349,16 @@ int Engine::build(const std::vector<int> &slots) {
         build_timed_ = true;
     }
     launch_build(kp_, d_slots_, n, total, stream_, ev);
+    GCFG_HIP(hipGetLastError());
+    if (kp_.lazy) lazy_caps_ = true;
+    return GCFG_OK;
+}
+
+int Engine::materialize(const std::vector<int> &slots) {
+    for (int s : slots) {
+        if (s < 0 || s >= cap_slots_ || !slot_used_[s]) continue;
+        launch_materialize(kp_, s, h_desc_[s].ntiles, stream_);
+    }
     GCFG_HIP(hipGetLastError());
     return GCFG_OK;
 }
@@ -424,8 +451,8 @@ int Engine::step_graph(int steps, hipGraphExec_t *out) {
 int Engine::launch_steps(int n) {
     apply_options();
     if (opt.profile) {
-        if ((int)events_.size() < 6 * n) {
-            for (int i = (int)events_.size(); i < 6 * n; ++i) {
+        if ((int)events_.size() < 8 * n) {
+            for (int i = (int)events_.size(); i < 8 * n; ++i) {
                 hipEvent_t e;
                 GCFG_HIP(hipEventCreate(&e));
                 events_.push_back(e);
@@ -439,8 +466,9 @@ int Engine::launch_steps(int n) {
             qtrace_cap_ = n;
         }
         for (int i = 0; i < n; ++i) {
-            hipEvent_t *ev = &events_[6 * i];
+            hipEvent_t *ev = &events_[8 * i];
             launch_sched(kp_, parity_, stream_, LaunchEvents{ev[0], ev[1]});
+            launch_matq(kp_, grid_pr_, parity_, stream_, LaunchEvents{ev[6], ev[7]});
             launch_pr(kp_, grid_pr_, parity_, stream_, LaunchEvents{ev[2], ev[3]});
             launch_gr(kp_, grid_gr_, parity_, stream_, LaunchEvents{ev[4], ev[5]});
             // This step's queue lengths, before the next scheduler clears them.
@@ -479,6 +507,17 @@ int Engine::launch_steps(int n) {
 
 int Engine::run(int64_t max_steps) {
     const int64_t start = steps_run;
+    apply_options();
+    if (lazy_caps_ && !kp_.variant) {
+        // The workgroup kernels read capacities directly: write out every
+        // instance's (the options switched kernels after a wavefront build).
+        std::vector<int> all;
+        for (int i = 0; i < cap_slots_; ++i)
+            if (slot_used_[i]) all.push_back(i);
+        int rc = materialize(all);
+        if (rc != GCFG_OK) return rc;
+        lazy_caps_ = false;
+    }
     int chunk = steps_hint > 0 ? std::min(steps_hint, 4096) : opt.steps_per_poll;
     for (;;) {
         int rc = launch_steps(chunk);
@@ -507,9 +546,12 @@ int Engine::collect_profile() {
     }
     for (int i = 0; i < events_pending_; ++i) {
         float a = 0, b = 0, c = 0;
-        GCFG_HIP(hipEventElapsedTime(&a, events_[6 * i], events_[6 * i + 1]));
-        GCFG_HIP(hipEventElapsedTime(&b, events_[6 * i + 2], events_[6 * i + 3]));
-        GCFG_HIP(hipEventElapsedTime(&c, events_[6 * i + 4], events_[6 * i + 5]));
+        float m = 0;
+        GCFG_HIP(hipEventElapsedTime(&a, events_[8 * i], events_[8 * i + 1]));
+        GCFG_HIP(hipEventElapsedTime(&b, events_[8 * i + 2], events_[8 * i + 3]));
+        GCFG_HIP(hipEventElapsedTime(&c, events_[8 * i + 4], events_[8 * i + 5]));
+        if (kp_.variant && kp_.lazy) GCFG_HIP(hipEventElapsedTime(&m, events_[8 * i + 6], events_[8 * i + 7]));
+        b += m;
         prof_sched_ms += a;
         prof_pr_ms += b;
         prof_gr_ms += c;
@@ -575,6 +617,7 @@ int Engine::reframe(int slot, const int32_t *pos, const int32_t *neg, const int3
     }
     if (d.W < 3 || d.H < 3) return set_inputs(slot, pos, neg, up, down, left, right, penalty, d.label_out);
     const int32_t *in1[6] = {pos, neg, up, up ? down : nullptr, up ? left : nullptr, up ? right : nullptr};
+    launch_materialize(kp_, slot, d.ntiles, stream_);  // the warm start reads and writes residuals directly
     launch_reframe(kp_, slot, d.ntiles, in1, penalty, stream_);
     GCFG_HIP(hipGetLastError());
     int rc = set_inputs(slot, pos, neg, up, down, left, right, penalty, d.label_out);

@@ -9,7 +9,10 @@
 // passes so a global relabel starts from the mask alone.
 // Excess < 0 means remaining sink capacity (t-links are pre-saturated,
 // GraphCutKernels.cu:196-197). Pixels outside the interior are inert nodes
-// with zero capacities (GraphCutKernels.cu:122-146).
+// with zero capacities (GraphCutKernels.cu:122-146). With the wavefront
+// kernels the capacities are materialised lazily: the build writes excess,
+// distance and mask only, and a tile's capacities are read from the inputs
+// until its first push pass writes them (KParams::mat, ::inflow).
 //
 // Control: a scheduler kernel advances one state machine per instance
 // (START -> global relabel -> push/relabel colour passes -> ... -> FIN ->
@@ -37,6 +40,7 @@ constexpr int NT = 256;                 // threads per workgroup (4 wavefronts)
 constexpr int RPT = TN / NT;            // rows per thread (vertical strip)
 constexpr int DINF = 0x3fffffff;        // distance of nodes that cannot reach the sink
 constexpr int NUM_XCD = 8;              // MI355X: 8 accelerator complex dies, one L2 each
+constexpr int INFLOW = 2 * TH + 2 * TW; // per-tile boundary slots of pending reverse-arc capacity
 constexpr int LW = TW + 2;              // LDS row pitch with halo
 constexpr int LH = TH + 2;
 
@@ -116,6 +120,18 @@ struct KParams {
     Ctl *ctl;
     int32_t *queue_pr, *queue_gr;
     int32_t *bmap;  // build list position -> tile id (written by k_clear_tiles for k_buildw)
+    // Lazily materialised residual capacities (wavefront kernels). mat[t] = 0:
+    // tile t's capacities have not been written since the build and are those
+    // of the inputs (border rule applied); 1: they are in caps. Either way,
+    // inflow[t * INFLOW + k] holds reverse-arc capacity still to be added to a
+    // boundary node by flow that neighbouring tiles pushed into it: slots
+    // [0, TH) column 0 left arcs, [TH, 2 TH) column TW-1 right arcs,
+    // [2 TH, 2 TH + TW) row 0 up arcs, [2 TH + TW, INFLOW) row TH-1 down arcs.
+    int32_t *mat;
+    int32_t *inflow;
+    // 1: the last build left capacities lazy and each step runs k_matq before
+    // the push pass; 0: the build wrote them (chosen per build by the engine).
+    int32_t lazy;
     int32_t n_slots;
     int32_t inner_iters;
     int32_t variant;  // 0: workgroup-per-tile kernels (LDS), 1: wavefront-per-tile kernels (registers)
@@ -144,6 +160,8 @@ void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_til
 void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s);
 void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvents &ev);
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
+// Capacities of the queued push tiles that are still lazy (before launch_pr; no-op unless p.lazy).
+void launch_matq(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
 void launch_gr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
 bool gr_block_supported(int bx, int by);
 bool options_valid(const gcfg_options &o);  // sets the error message when not
@@ -152,6 +170,7 @@ void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_str
                  hipStream_t s);
 void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, int32_t *d_dist,
                        int32_t *d_caps8, hipStream_t s);
+void launch_materialize(const KParams &p, int slot, int ntiles, hipStream_t s);
 // in1: pos, neg, up, down, left, right of the new inputs (up == null: penalty pen1).
 void launch_reframe(const KParams &p, int slot, int ntiles, const int32_t *const *in1, int32_t pen1,
                     hipStream_t s);
@@ -218,6 +237,10 @@ class Engine {
     // instance's own; follow with rearm().
     int reframe(int slot, const int32_t *pos, const int32_t *neg, const int32_t *up, const int32_t *down,
                 const int32_t *left, const int32_t *right, int32_t penalty);
+    // Write every tile's residual capacities of the listed instances to caps
+    // (needed by code that reads caps directly: the workgroup kernels, the
+    // video warm start).
+    int materialize(const std::vector<int> &slots);
     // Residual capacities (W*H*4), excess (W*H), distance labels (W*H) and,
     // for the 8-neighbourhood, diagonal residual capacities (W*H*4) in image
     // layout; any output may be null.
@@ -234,7 +257,7 @@ class Engine {
     int cap_tiles() const { return cap_tiles_; }
     int tiles_for(int W, int H) const { return ((W + TW - 1) / TW) * ((H + TH - 1) / TH); }
     static size_t bytes_per_tile(int nbhd = 4) {
-        return (size_t)TN * (16 + 4 + 4 + 1) + 96 + (nbhd == 8 ? (size_t)TN * (16 + 1) + 16 : 0);
+        return (size_t)TN * (16 + 4 + 4 + 1) + 100 + INFLOW * 4 + (nbhd == 8 ? (size_t)TN * (16 + 1) + 16 : 0);
     }
     gcfg_options opt;
     int steps_run = 0;
@@ -244,6 +267,7 @@ class Engine {
     // steps.
     int steps_hint = 0;
     // Event timing of the launches since the last reset (profile mode).
+    // prof_pr_ms includes k_matq's capacity writes before each push pass.
     double prof_sched_ms = 0, prof_pr_ms = 0, prof_gr_ms = 0, prof_build_ms = 0;
     int64_t prof_steps = 0;
     // Keeps a timed build that has not been collected yet: the plain graph
@@ -263,6 +287,7 @@ class Engine {
     int dev_ = 0;
     int cu_count_ = 0;
     int nbhd_ = 4;
+    bool lazy_caps_ = false;  // some built instance may have unmaterialised capacities
     hipStream_t stream_ = nullptr;
     KParams kp_{};
     int cap_tiles_ = 0, cap_slots_ = 0;
@@ -288,7 +313,7 @@ class Engine {
     int step_graph(int steps, hipGraphExec_t *out);
     int parity_ = 0;  // parity of the next step to launch
     std::vector<void *> allocs_;
-    std::vector<hipEvent_t> events_;  // 6 per profiled step: start/stop of sched, pr, gr
+    std::vector<hipEvent_t> events_;  // 8 per profiled step: start/stop of sched, matq, pr, gr
     int events_pending_ = 0;
     QCtl *pin_qtrace_ = nullptr;      // queue lengths per profiled step (verbose >= 2)
     hipEvent_t build_ev_[2] = {nullptr, nullptr};  // the build kernel's start / stop (profile mode)

@@ -40,6 +40,31 @@ __device__ __forceinline__ void stg_out(int32_t *p, size_t i, int v) {
     ((__attribute__((address_space(1))) int32_t *)p)[i] = v;
 }
 
+// Residual capacities {left, right, up, down} of pixel (X, Y) as the build
+// defines them from the inputs: zero outside the interior and on arcs into the
+// border (GraphCutKernels.cu:122-146, :179-203).
+__device__ __forceinline__ int4 input_caps(const InstDesc &D, int X, int Y) {
+    int4 c = make_int4(0, 0, 0, 0);
+    if (!(X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1)) return c;
+    const size_t i = (size_t)Y * D.W + X;
+    if (D.up)
+        c = make_int4(ldg_in(D.left, i), ldg_in(D.right, i), ldg_in(D.up, i), ldg_in(D.down, i));
+    else
+        c = make_int4(D.penalty, D.penalty, D.penalty, D.penalty);
+    if (X == 1) c.x = 0;
+    if (X == D.W - 2) c.y = 0;
+    if (Y == 1) c.z = 0;
+    if (Y == D.H - 2) c.w = 0;
+    return c;
+}
+
+// Pending reverse-arc capacity of tile-local node (x, y) (see KParams::inflow).
+__device__ __forceinline__ int4 inflow_of(const KParams &p, int t, int x, int y) {
+    const int32_t *f = p.inflow + (size_t)t * INFLOW;
+    return make_int4(x == 0 ? f[y] : 0, x == TW - 1 ? f[TH + y] : 0, y == 0 ? f[2 * TH + x] : 0,
+                     y == TH - 1 ? f[2 * TH + TW + x] : 0);
+}
+
 __device__ __forceinline__ int64_t wave_sum64(int64_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -190,10 +215,13 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
                 p.mask8[g] = 0;
             }
         }
+        // Capacities are written here: the tile is materialised, nothing pending.
+        for (int k = threadIdx.x; k < INFLOW; k += NT) p.inflow[(size_t)t * INFLOW + k] = 0;
         const int64_t tot = block_sum64(sp, red);
         const int anybad = __syncthreads_or(bad);
         if (threadIdx.x == 0) {
             p.tsum[2 * (size_t)t] = tot;
+            p.mat[t] = 1;
             if (anybad) atomicOr(&p.state[s].err, 1);
         }
         __syncthreads();
@@ -1633,6 +1661,30 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCF
     if (p.profile && lane == 0) atomic_add64(&p.ctl->gr_iters, n_rounds);
 }
 
+// Before a push pass of a lazily built engine: write the capacities of the
+// queued tiles that still have them only in the inputs (one wavefront per
+// tile, the queue's own contiguous runs as in k_prw). A launch of its own:
+// inside k_prw the second load path raised the kernel's registers and its
+// per-item latency for every tile.
+__global__ __launch_bounds__(64) void k_matq(KParams p, int parity, GCFG_RO_ARGS) {
+    const int qlen = p.ctl->q[parity].qlen_pr;
+    if ((int)blockIdx.x >= qlen) return;
+    const int lane = threadIdx.x;
+    const int per = (qlen + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int q_end = min(qlen, ((int)blockIdx.x + 1) * per);
+    for (int q = (int)blockIdx.x * per; q < q_end; ++q) {
+        const int t = rq[q];
+        if (p.mat[t]) continue;
+        const TileInfo ti = rtiles[t];
+        const InstDesc D = rdesc[ti.inst];
+        const TileRef T = tile_ref(p, t);
+        const int X = ti.tx * TW + lane;
+#pragma unroll
+        for (int y = 0; y < TH; ++y) T.c[y * TW + lane] = input_caps(D, X, ti.ty * TH + y);
+        if (lane == 0) p.mat[t] = 1;
+    }
+}
+
 __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS) {
     const int qlen = p.ctl->q[parity].qlen_pr;
     if ((int)blockIdx.x >= qlen) return;
@@ -1659,11 +1711,31 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         // Flow pushed out of the tile sideways, one row per lane: lanes 0..15
         // into the left tile, lanes 16..31 into the right tile.
         int accS = 0;
+        // Capacities are in the arena: written by the build, or by k_matq
+        // just before this pass when the build left them lazy.
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
             c[y] = T.c[y * TW + lane];
             e[y] = T.e[y * TW + lane];
             d[y] = T.d[y * TW + lane];
+        }
+        // Reverse-arc capacity that neighbours' pushes left for the boundary
+        // nodes (they do not run in this colour pass): add it, clear the slots.
+        {
+            int32_t *f = p.inflow + (size_t)t * INFLOW;
+            const int fs = lane < 2 * TH ? f[lane] : 0;  // lanes 0..15: column 0, 16..31: column TW-1
+            const int ft = f[2 * TH + lane], fb = f[2 * TH + TW + lane];
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                const int l = __builtin_amdgcn_readlane(fs, y), r = __builtin_amdgcn_readlane(fs, TH + y);
+                if (lane == 0) c[y].x += l;
+                if (lane == TW - 1) c[y].y += r;
+            }
+            c[0].z += ft;
+            c[TH - 1].w += fb;
+            if (lane < 2 * TH && fs) f[lane] = 0;
+            if (ft) f[2 * TH + lane] = 0;
+            if (fb) f[2 * TH + TW + lane] = 0;
         }
         int accU = 0, accD = 0;
         int it = 0;
@@ -1772,23 +1844,28 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         const int side = lane < TH ? ti.nbr[0] : ti.nbr[1];
         const bool pushS = lane < 2 * TH && accS != 0 && side >= 0;
         const size_t gS = (size_t)(pushS ? side : 0) * TN + (size_t)(lane & (TH - 1)) * TW + (lane < TH ? TW - 1 : 0);
-        const int capS = lane < TH ? 1 : 0;  // the side node's arc back to us: right (left tile) / left
+        // The side node's arc back to us: the right arc of the left tile's
+        // column TW-1 (slot TH + row), the left arc of the right tile's column 0 (slot row).
+        const int slotS = lane < TH ? TH + lane : lane - TH;
         const int arcS = lane < TH ? 2 : 1;
         const bool pushU = accU != 0 && ti.nbr[2] >= 0, pushD = accD != 0 && ti.nbr[3] >= 0;
         const size_t gU = (size_t)(pushU ? ti.nbr[2] : 0) * TN + (size_t)(TH - 1) * TW + lane;
         const size_t gD = (size_t)(pushD ? ti.nbr[3] : 0) * TN + lane;
         int oldS = 0, oldU = 0, oldD = 0;
+        // Reverse arcs go to the receiving tile's inflow slots (its capacities
+        // may still live in the inputs): down arcs of the upper tile's last
+        // row, up arcs of the lower tile's first row.
         if (pushS) {
             oldS = atomicAdd(&p.exc[gS], accS);
-            atomicAdd(reinterpret_cast<int *>(&p.caps[gS]) + capS, accS);
+            atomicAdd(&p.inflow[(size_t)side * INFLOW + slotS], accS);
         }
         if (pushU) {
             oldU = atomicAdd(&p.exc[gU], accU);
-            atomicAdd(reinterpret_cast<int *>(&p.caps[gU]) + 3, accU);
+            atomicAdd(&p.inflow[(size_t)ti.nbr[2] * INFLOW + 2 * TH + TW + lane], accU);
         }
         if (pushD) {
             oldD = atomicAdd(&p.exc[gD], accD);
-            atomicAdd(reinterpret_cast<int *>(&p.caps[gD]) + 2, accD);
+            atomicAdd(&p.inflow[(size_t)ti.nbr[3] * INFLOW + 2 * TH + lane], accD);
         }
         if (pushS) mask_after_push_in(p, gS, arcS, oldS + accS);
         if (pushU) mask_after_push_in(p, gU, 8, oldU + accU);
@@ -1948,13 +2025,22 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
                     sp += vp[k];
                     sx += max(e, 0);
                 }
-                T.c[y * TW + lane] = c;
-                T.e[y * TW + lane] = e;
+                // Lazy build: capacities stay in the inputs until a push pass
+                // needs them (most tiles never see one). Excess is not read
+                // again before a push pass either: a streaming store.
+                if (!p.lazy) T.c[y * TW + lane] = c;
+                __builtin_nontemporal_store(e, &T.e[y * TW + lane]);
                 const int mk = node_mask(c, e);
                 m.put(y, mk);
                 T.m[y * TW + lane] = (uint8_t)mk;
                 d[y] = e < 0 ? 0 : capv;
             }
+        }
+        {
+            int32_t *f = p.inflow + (size_t)t * INFLOW;
+            if (lane < 2 * TH) f[lane] = 0;
+            f[2 * TH + lane] = 0;
+            f[2 * TH + TW + lane] = 0;
         }
         uint32_t chg = 0;
         int rounds = 0;
@@ -1990,6 +2076,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
             if (rank) atomicMax(&p.live[block_anchor(p, D, tx, ty)], rank);
             p.tsum[2 * (size_t)t] = sp;
             p.tsum[2 * (size_t)t + 1] = sx;
+            p.mat[t] = p.lazy ? 0 : 1;
             if (any_bad) atomicOr(&p.state[s].err, 1);
         }
     }
@@ -2137,6 +2224,33 @@ __global__ __launch_bounds__(NT) void k_remask(KParams p, int slot) {
     }
 }
 
+// Write the residual capacities of every tile of an instance to caps (from
+// the inputs where a tile has not been pushed since its build) with the
+// pending inflow added, then clear the inflow slots (one workgroup per tile).
+__global__ __launch_bounds__(NT) void k_materialize(KParams p, int slot) {
+    const InstDesc D = p.desc[slot];
+    for (int lt = blockIdx.x; lt < D.ntiles; lt += gridDim.x) {
+        const int t = D.tile0 + lt;
+        const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
+        const bool matd = p.mat[t] != 0;
+        for (int j = threadIdx.x; j < TN; j += NT) {
+            const int x = j % TW, y = j / TW;
+            const size_t g = (size_t)t * TN + j;
+            int4 c = matd ? p.caps[g] : input_caps(D, tx * TW + x, ty * TH + y);
+            const int4 f = inflow_of(p, t, x, y);
+            c.x += f.x;
+            c.y += f.y;
+            c.z += f.z;
+            c.w += f.w;
+            p.caps[g] = c;
+        }
+        __syncthreads();  // every inflow slot has been read
+        for (int k = threadIdx.x; k < INFLOW; k += NT) p.inflow[(size_t)t * INFLOW + k] = 0;
+        if (threadIdx.x == 0) p.mat[t] = 1;
+        __syncthreads();
+    }
+}
+
 // Residual state in image layout (certificates and debugging).
 __global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *exc_out, int32_t *dist_out,
                              int32_t *caps8_out) {
@@ -2146,7 +2260,12 @@ __global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *ex
         const int X = (int)(i % D.W), Y = (int)(i / D.W);
         const int t = D.tile0 + (Y / TH) * D.tiles_x + X / TW;
         const size_t g = (size_t)t * TN + (Y % TH) * TW + (X % TW);
-        const int4 c = p.caps[g];
+        int4 c = p.mat[t] ? p.caps[g] : input_caps(D, X, Y);
+        const int4 f = inflow_of(p, t, X % TW, Y % TH);
+        c.x += f.x;
+        c.y += f.y;
+        c.z += f.z;
+        c.w += f.w;
         if (caps_out) {
             caps_out[4 * i + 0] = c.x;
             caps_out[4 * i + 1] = c.y;
@@ -2182,7 +2301,7 @@ void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_til
     if (p.variant) {
         hipLaunchKernelGGL(k_clear_tiles, dim3((total_tiles + 255) / 256), dim3(256), 0, s, p, d_slots, d_slots + n, n,
                            total_tiles);
-        int grid = total_tiles < 16384 ? total_tiles : 16384;
+        int grid = total_tiles < 65536 ? total_tiles : 65536;
         if (grid >= NUM_XCD) grid -= grid % NUM_XCD;  // whole rounds over the XCDs
         launch_timed(k_buildw, dim3(grid), dim3(64), s, ev, p, d_slots, (const int32_t *)(d_slots + n), n,
                      total_tiles);
@@ -2207,6 +2326,12 @@ static void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t s, const L
 void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvents &ev) {
     launch_timed(k_sched, dim3(p.n_slots), dim3(NT), s, ev, p, parity);
 }
+void launch_matq(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
+    if (!p.variant || !p.lazy) return;  // every capacity was written by the build
+    launch_timed(k_matq, dim3(grid), dim3(64), s, ev, p, parity, (const int32_t *)p.queue_pr,
+                 (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
+}
+
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
     if (p.variant)
         launch_timed(k_prw, dim3(grid), dim3(64), s, ev, p, parity, (const int32_t *)p.queue_pr,
@@ -2253,6 +2378,7 @@ void launch_gr(const KParams &p, int grid, int parity, hipStream_t s, const Laun
 
 void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s) {
     launch_sched(p, parity, s, LaunchEvents{});
+    launch_matq(p, grid_pr, parity, s, LaunchEvents{});
     launch_pr(p, grid_pr, parity, s, LaunchEvents{});
     launch_gr(p, grid_gr, parity, s, LaunchEvents{});
 }
@@ -2266,6 +2392,11 @@ void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_str
                        n_strokes, delta);
     hipLaunchKernelGGL(k_edit_masks, dim3(n_strokes < 1024 ? n_strokes : 1024), dim3(NT), 0, s, p, slot, d_strokes,
                        n_strokes);
+}
+
+void launch_materialize(const KParams &p, int slot, int ntiles, hipStream_t s) {
+    if (ntiles <= 0) return;
+    hipLaunchKernelGGL(k_materialize, dim3(ntiles < 4096 ? ntiles : 4096), dim3(NT), 0, s, p, slot);
 }
 
 void launch_reframe(const KParams &p, int slot, int ntiles, const int32_t *const *in1, int32_t pen1,

@@ -97,13 +97,13 @@ extern "C" {
 static const char *kFields[] = {"sweeps_per_gr", "inner_iters", "steps_per_poll", "use_graph",
                                 "max_steps",     "verbose",     "profile",        "kernels",
                                 "gr_rounds",     "gr_block_w",  "gr_block_h",     "gr_cap",
-                                "pr_waves_per_cu", "gr_waves_per_cu"};
+                                "pr_waves_per_cu", "gr_waves_per_cu", "lazy_caps"};
 
 static std::vector<int32_t *> option_fields(gcfg_options *o) {
     return {&o->sweeps_per_gr, &o->inner_iters, &o->steps_per_poll, &o->use_graph,
             &o->max_steps,     &o->verbose,     &o->profile,        &o->kernels,
             &o->gr_rounds,     &o->gr_block_w,  &o->gr_block_h,     &o->gr_cap,
-            &o->pr_waves_per_cu, &o->gr_waves_per_cu};
+            &o->pr_waves_per_cu, &o->gr_waves_per_cu, &o->lazy_caps};
 }
 
 int gcfg_tune_store(const char *path, const char *key, const gcfg_options *opt) {

@@ -48,7 +48,8 @@ class Options(ctypes.Structure):
                 ("max_steps", ctypes.c_int32), ("verbose", ctypes.c_int32), ("profile", ctypes.c_int32),
                 ("kernels", ctypes.c_int32), ("gr_rounds", ctypes.c_int32),
                 ("gr_block_w", ctypes.c_int32), ("gr_block_h", ctypes.c_int32), ("gr_cap", ctypes.c_int32),
-                ("pr_waves_per_cu", ctypes.c_int32), ("gr_waves_per_cu", ctypes.c_int32)]
+                ("pr_waves_per_cu", ctypes.c_int32), ("gr_waves_per_cu", ctypes.c_int32),
+                ("lazy_caps", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

@@ -54,10 +54,13 @@ def test_alone_equals_mix_and_oracle(g, ora):
     srv.close()
 
 
-def test_warm_edit_equals_cold(g, ora):
+@pytest.mark.parametrize("lazy", [0, 2])
+def test_warm_edit_equals_cold(g, ora, lazy):
     W, H = 360, 240
     img, seeds = _req(g, W, H, 7)
-    srv = g.Server(1 << 30)
+    o = g.default_options()
+    o.lazy_caps = lazy  # 2: capacities written before each tile's first push pass
+    srv = g.Server(1 << 30, o)
     rid, sess, label0 = srv.submit_image(img, seeds)
     _drain(srv, 1)
     rng = np.random.default_rng(3)

@@ -37,13 +37,14 @@ def _cold(ora, frame, model, seeds, W, H, dpos, dneg):
     return ora.maxflow(W, H, ref["pos"] + dpos, ref["neg"] + dneg, edges=E)
 
 
-@pytest.mark.parametrize("kernels", [1, 0])
-def test_video_frames_warm_equal_cold(g, ora, kernels):
+@pytest.mark.parametrize("kernels,lazy", [(1, 0), (1, 2), (0, 0)])
+def test_video_frames_warm_equal_cold(g, ora, kernels, lazy):
     W, H = 320, 240
     frames = _frames(g, W, H, 3, 5)
     seeds = g.gen_seed_map(W, H)
     o = g.default_options()
     o.kernels = kernels
+    o.lazy_caps = lazy
     srv = g.Server(1 << 30, o)
     rid0, sess, lab0 = srv.submit_image(frames[0], seeds)
     ids = [(rid0, lab0)]
