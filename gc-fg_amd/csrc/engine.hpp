@@ -50,6 +50,10 @@ enum Phase : int32_t { PH_IDLE = 0, PH_START = 1, PH_GR = 2, PH_PR = 3, PH_FIN =
 // the build, the push passes and edits keep current.
 enum GrOp : int32_t { OP_GR_INIT = 1, OP_GR_RELAX = 2, OP_FIN = 3, OP_GR_REINIT = 4 };
 constexpr int MB_EPOS = 16, MB_ENEG = 32;  // mask bits: excess > 0, excess < 0
+// InstState::cap of a reachability round (wavefront kernels): a zero-weight
+// flood from the sinks, distances 0 (can reach a node with excess < 0) or
+// DINF, run instead of an exact round to confirm that a capped schedule is done.
+constexpr int CAP_FLOOD = 0;
 
 struct TileInfo {
     int32_t inst;
@@ -80,7 +84,7 @@ struct InstState {
     int32_t err;
     int32_t sweeps;        // colour-pass pairs between global relabels
     int64_t gr_init_tiles; // of which initial sweeps (INIT / REINIT)
-    int32_t cap;           // distance cap of the current global relabel (DINF: exact)
+    int32_t cap;           // distance cap of the current global relabel (DINF: exact, CAP_FLOOD: reachability)
     int32_t pad_;
     int64_t gr_exact_tiles;  // relabel items of exact rounds (those write labels)
     int64_t fin_items;       // FIN items (blocks left unreachable)

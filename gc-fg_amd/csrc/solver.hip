@@ -233,6 +233,12 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
 // (taken from block-wide scans of the tile flags); thread 0 writes the state.
 enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2, EM_LIVE = 3, EM_DEAD = 4 };
 
+// Per-block liveness (KParams::live) as the last relabel sweep of the block
+// left it: 0 every node unreachable, 1 all at or beyond the cap, 2 some node
+// below the cap, 3 (capped and reachability rounds) every interior node
+// reaches a sink -- below the cap, or at 0 in a reachability round. A block
+// at 3 that no push has made stale since stays at 3: a path shorter than the
+// cap changes only through a push within the cap of the block.
 // Append the instance's tiles (push/relabel queue) or global-relabel blocks
 // (identified by their anchor tile, whose flags stand for the block)
 // selected by `kind` to a queue (ballot compaction, one atomic per 256
@@ -258,13 +264,17 @@ __device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int col
             if (kind == EM_ALL) {
                 pred = 1;
             } else if (kind == EM_DEAD) {
-                pred = p.live[t] == 0;
+                // FIN: blocks the last round did not visit (every node
+                // unreachable, or every interior node reaching a sink)
+                pred = p.live[t] == 0 || p.live[t] == 3;
             } else if (kind == EM_LIVE) {
                 // Exact round (colour 1): every block with a reachable node.
-                // Capped round: blocks with a node below the cap that a push
-                // pass may have changed (the others keep their labels).
+                // Reachability round (colour 2): every block with a node that
+                // may not reach a sink. Capped round: blocks with a node below
+                // the cap that a push pass may have changed (the others keep
+                // their labels).
                 const int lv = p.live[t];
-                pred = colour ? lv != 0 : (lv == 2 && p.stale[t]);
+                pred = colour == 1 ? lv != 0 : colour == 2 ? (lv == 1 || lv == 2) : (lv >= 2 && p.stale[t]);
                 p.stale[t] = 0;
                 if (!colour && lv == 1) {
                     // A skipped block's nodes are all frozen: none is active.
@@ -273,8 +283,10 @@ __device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int col
                         for (int i = 0; i < bx && tx0 + i < D.tiles_x; ++i) p.act_flag[t + j * D.tiles_x + i] = 0;
                 }
             } else if (kind == EM_DIRTY) {
-                pred = p.dirty[t];
-                if (pred) p.dirty[t] = 0;
+                // colour 1 (a reachability round): a block whose nodes all
+                // reach a sink already cannot change, whatever its halo does.
+                pred = p.dirty[t] && !(colour && p.live[t] == 3);
+                if (p.dirty[t]) p.dirty[t] = 0;
             } else {
                 // 4-neighbourhood: checkerboard (no two side neighbours run
                 // together). 8-neighbourhood: (tx & 1, ty & 1) four colours,
@@ -339,13 +351,15 @@ __device__ void start_global_relabel(const KParams &p, const InstDesc &D, InstSt
     // stays there (its labels >= K are frozen and no push can enter such a
     // node), and for a block farther than K from every change each shortest
     // path of length < K is unchanged, so min(d, K) is unchanged too.
+    const bool whole = cap >= DINF || cap == CAP_FLOOD;  // exact distances or reachability: every live block
     const int n = !reinit ? emit_tiles(p, D, EM_ALL, 0, false, OP_GR_INIT, parity)
                   : all   ? emit_tiles(p, D, EM_ALL, 0, false, OP_GR_REINIT, parity)
-                          : emit_tiles(p, D, EM_LIVE, cap >= DINF ? 1 : 0, false, OP_GR_REINIT, parity);
+                          : emit_tiles(p, D, EM_LIVE, cap == CAP_FLOOD ? 2 : (whole ? 1 : 0), false, OP_GR_REINIT,
+                                       parity);
     if (threadIdx.x == 0) {
         st.gr_tiles += n;
         st.gr_init_tiles += n;
-        if (cap >= DINF) st.gr_exact_tiles += n;
+        if (whole) st.gr_exact_tiles += n;
     }
 }
 
@@ -387,25 +401,37 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
         return;
     }
     if (phase == PH_GR) {
-        const int n = emit_tiles(p, D, EM_DIRTY, 0, false, OP_GR_RELAX, parity);
+        const int n = emit_tiles(p, D, EM_DIRTY, cur_cap == CAP_FLOOD ? 1 : 0, false, OP_GR_RELAX, parity);
         if (n > 0) {
             if (threadIdx.x == 0) {
                 st.gr_tiles += n;
-                if (cur_cap >= DINF) st.gr_exact_tiles += n;
+                if (cur_cap >= DINF || cur_cap == CAP_FLOOD) st.gr_exact_tiles += n;
             }
             return;
         }
         // The BFS has converged: finish, or push from the active tiles.
+        const bool flood = cur_cap == CAP_FLOOD;
         if (!any_active_tile(p, D)) {
-            if (cur_cap < DINF) {  // nothing active below the cap: relabel exactly before finishing
-                start_global_relabel(p, D, st, parity, reinit, DINF);
+            if (cur_cap < DINF && !flood) {
+                // Nothing active below the cap: check reachability before
+                // finishing (a flood settles in far fewer tile visits than
+                // exact distances; it also writes the labels).
+                start_global_relabel(p, D, st, parity, reinit, CAP_FLOOD);
                 return;
             }
             if (threadIdx.x == 0) st.phase = PH_FIN;
-            // Wavefront kernels: the exact round wrote the labels of every
-            // block it visited; FIN labels the blocks it skipped as unreachable.
+            // Wavefront kernels: the exact or reachability round wrote the
+            // labels of every block it visited; FIN labels the blocks it
+            // skipped (unreachable, or reaching a sink everywhere).
             const int nf = emit_tiles(p, D, p.variant ? EM_DEAD : EM_ALL, 0, false, OP_FIN, parity);
             if (threadIdx.x == 0) st.fin_items += nf;
+            return;
+        }
+        if (flood) {
+            // An excess node can still reach a sink. Flood values are not
+            // valid heights: relabel exactly before pushing (a capped round
+            // could freeze the same nodes again).
+            start_global_relabel(p, D, st, parity, reinit, DINF);
             return;
         }
         if (threadIdx.x == 0) st.phase = PH_PR;
@@ -1461,10 +1487,54 @@ __device__ __forceinline__ void bfs_levels(int (&d)[TH], const Masks &m, int hU,
     chg |= rch;
 }
 
+// Reachability flood of one tile by the same level sets, run to the fixed
+// point: S = sinks, nodes already at 0 and nodes with a residual arc into S or
+// into a halo node at 0; d = 0 on S, DINF elsewhere. Values only go from DINF
+// to 0, so tiles re-queued by changed boundaries converge after few visits,
+// where exact distances would be lowered again and again.
+__device__ __forceinline__ void bfs_flood(int (&d)[TH], const Masks &m, int hU, int hD, int hs, int lane,
+                                          uint32_t &chg, int &levels) {
+    const uint32_t PL = row_plane(m, 0), PR = row_plane(m, 1), PU = row_plane(m, 2), PD = row_plane(m, 3);
+    uint32_t S = row_plane(m, 5);
+#pragma unroll
+    for (int y = 0; y < TH; ++y) S |= (d[y] == 0 ? 1u : 0u) << y;
+    const uint32_t hu = hU == 0 ? 1u : 0u;
+    const uint32_t hd = hD == 0 ? (1u << (TH - 1)) : 0u;
+    const uint64_t bl = __ballot(lane < TH && hs == 0);
+    const uint64_t br = __ballot(lane >= TH && lane < 2 * TH && hs == 0);
+    const int lm = (int)((uint32_t)bl & 0xffffu), rm = (int)((uint32_t)(br >> TH) & 0xffffu);
+    int it = 0;
+    for (;;) {  // S only grows and has at most TN bits: at most TN + 1 trips
+        ++it;
+        const uint32_t lf = (uint32_t)__builtin_amdgcn_update_dpp(lm, (int)S, 0x138, 0xF, 0xF, false);
+        const uint32_t rt = (uint32_t)__builtin_amdgcn_update_dpp(rm, (int)S, 0x130, 0xF, 0xF, false);
+        const uint32_t Sn = (S | (((S << 1) | hu) & PU) | (((S >> 1) | hd) & PD) | (lf & PL) | (rt & PR)) & 0xffffu;
+        const bool grew = __any(Sn != S);
+        S = Sn;
+        if (!grew) break;
+    }
+    levels = it;
+    uint32_t rch = 0;
+#pragma unroll
+    for (int y = 0; y < TH; ++y) {
+        const int nd = ((S >> y) & 1u) ? 0 : DINF;
+        if (nd != d[y]) {
+            d[y] = nd;
+            rch |= 1u << y;
+        }
+    }
+    chg |= rch;
+}
+
 // Capped tile BFS for caps up to 15 by level sets; larger caps and exact
-// rounds use the relaxation. Returns true when the tile converged.
+// rounds use the relaxation, reachability rounds the flood. Returns true when
+// the tile converged.
 __device__ __forceinline__ bool tile_bfs(int (&d)[TH], Masks &m, int hU, int hD, int hs, int lane, int capv,
                                          uint32_t &chg, int &rounds, int max_rounds) {
+    if (capv == CAP_FLOOD) {
+        bfs_flood(d, m, hU, hD, hs, lane, chg, rounds);
+        return true;
+    }
     if (capv <= 3) {
         bfs_levels<2>(d, m, hU, hD, hs, lane, capv, chg, rounds);
         return true;
@@ -1549,19 +1619,23 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCF
         const TileInfo ti = rtiles[t];
         const TileRef T = tile_ref(p, t);
         if (op == OP_FIN) {
-            // Only blocks left with every node unreachable come here; every
-            // other tile wrote its labels in the last (exact) relabel round.
+            // Only blocks the last round skipped come here: every node
+            // unreachable (labels 1) or every interior node reaching a sink
+            // (labels 0); the others wrote their labels in that round.
             if (!have) continue;
+            const int dv = p.live[t0] == 3 ? 0 : DINF;
             int dd[TH];
 #pragma unroll
-            for (int y = 0; y < TH; ++y) dd[y] = DINF;
+            for (int y = 0; y < TH; ++y) dd[y] = dv;
             write_labels(D, tx, ty, lane, dd);
             continue;
         }
         // (OP_GR_INIT is the workgroup kernels' op; here masks are rebuilt by
         // the push kernel's OP_MASK pass and every relabel starts with REINIT.)
         const bool reinit = op == OP_GR_REINIT;
-        const int capv = rstate[inst].cap;  // start value of non-sink nodes (DINF: exact)
+        const int capv = rstate[inst].cap;  // DINF: exact, CAP_FLOOD: reachability, else the cap
+        const bool flood = capv == CAP_FLOOD;
+        const int vinit = flood ? DINF : capv;  // start value of non-sink nodes
         int d[TH];
         Masks m;
         m.clear();
@@ -1574,7 +1648,7 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCF
 #pragma unroll
             for (int y = 0; y < TH; ++y) m.put(y, T.m[y * TW + lane]);
 #pragma unroll
-            for (int y = 0; y < TH; ++y) d[y] = (m.get(y) & MB_ENEG) ? 0 : capv;
+            for (int y = 0; y < TH; ++y) d[y] = (m.get(y) & MB_ENEG) ? 0 : vinit;
         } else {
             load_halo<HALO_DIST>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
@@ -1623,18 +1697,23 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCF
                 break;
             }
         }
-        int act = 0, fin = 0, nearv = 0;
+        int act = 0, fin = 0, nearv = 0, open = 0;
         if (have) {
+            const int X = tx * TW + lane;
+            const bool colin = X > 0 && X < D.W - 1;
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
                 if (reinit || ((chg >> y) & 1)) T.d[y * TW + lane] = d[y];
-                act |= (m.get(y) & MB_EPOS) && d[y] < capv;
+                act |= (m.get(y) & MB_EPOS) && d[y] < vinit;
                 fin |= d[y] < DINF;
                 nearv |= d[y] < p.gr_cap;
+                const int Y = ty * TH + y;
+                // an interior node not known to reach a sink (at or past the cap, or not at 0 in a flood)
+                open |= colin && Y > 0 && Y < D.H - 1 && d[y] >= (flood ? 1 : capv);
             }
-            // An exact round writes the labels of every tile it visits; the
-            // last visit of the last round leaves the final ones.
-            if (capv >= DINF) write_labels(D, tx, ty, lane, d);
+            // An exact or reachability round writes the labels of every tile
+            // it visits; the last visit of the last round leaves the final ones.
+            if (capv >= DINF || flood) write_labels(D, tx, ty, lane, d);
         }
         // Outer boundary changes re-queue the neighbouring blocks.
         const bool chL = __any(lane == 0 && chg != 0) && !inL && have && ti.nbr[0] >= 0;
@@ -1653,9 +1732,10 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCF
         }
         const bool any_fin = NW == 1 ? __any(fin) : __syncthreads_or(fin);
         const bool any_near = p.gr_cap <= 0 || (NW == 1 ? __any(nearv) : __syncthreads_or(nearv));
+        const bool any_open = NW == 1 ? __any(open) : __syncthreads_or(open);
         if (threadIdx.x == 0) {
             if (capped) mark_dirty(p, t0);
-            p.live[t0] = any_fin ? (any_near ? 2 : 1) : 0;
+            p.live[t0] = any_fin ? ((capv < DINF && !any_open) ? 3 : (any_near ? 2 : 1)) : 0;
         }
     }
     if (p.profile && lane == 0) atomic_add64(&p.ctl->gr_iters, n_rounds);
@@ -2045,18 +2125,23 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         uint32_t chg = 0;
         int rounds = 0;
         const bool conv = tile_bfs(d, m, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds);
-        int act = 0, fin = 0, nearv = 0;
+        int act = 0, fin = 0, nearv = 0, open = 0;
+        const bool colin = X > 0 && X < D.W - 1;
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
             T.d[y * TW + lane] = d[y];
             act |= (m.get(y) & MB_EPOS) && d[y] < capv;
             fin |= d[y] < DINF;
             nearv |= d[y] < capv;
+            open |= colin && Y0 + y > 0 && Y0 + y < D.H - 1 && d[y] >= capv;
         }
         // Uncapped, this sweep belongs to an exact relabel round: labels as
         // of this sweep (rewritten by any later sweep of the tile).
         if (capv >= DINF) write_labels(D, tx, ty, lane, d);
-        const int rank = __any(fin) ? (__any(nearv) ? 2 : 1) : 0;
+        // Liveness as k_grb keeps it; 3 (every interior node below the cap)
+        // only for one-tile blocks, whose rank is the tile's own.
+        const bool one = p.gbx == 1 && p.gby == 1;
+        const int rank = __any(fin) ? ((one && capv < DINF && !__any(open)) ? 3 : (__any(nearv) ? 2 : 1)) : 0;
         const bool chL = __any(lane == 0 && chg != 0);
         const bool chR = __any(lane == TW - 1 && chg != 0);
         const bool chU = __any((chg & 1u) != 0);
