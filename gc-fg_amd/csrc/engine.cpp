@@ -294,6 +294,7 @@ void Engine::remove_instance(int slot) {
     free_tiles(h_desc_[slot].tile0, h_desc_[slot].ntiles);
     slot_used_[slot] = 0;
     h_desc_[slot] = InstDesc{};
+    update_matq();
 }
 
 int Engine::build(const std::vector<int> &slots) {
@@ -313,32 +314,32 @@ int Engine::build(const std::vector<int> &slots) {
             return GCFG_ERR_STATE;
         }
     }
-    // The pinned staging buffers are reused; wait for earlier copies out of them.
-    GCFG_HIP(hipStreamSynchronize(stream_));
-    std::memcpy(pin_desc_, h_desc_.data(), (size_t)cap_slots_ * sizeof(InstDesc));
     const int n = (int)slots.size();
     int total = 0;
-    for (int i = 0; i < n; ++i) {
-        pin_slots_[i] = slots[i];
-        pin_slots_[n + i] = total;
-        total += h_desc_[slots[i]].ntiles;
-    }
-    GCFG_HIP(hipMemcpyAsync(kp_.desc, pin_desc_, (size_t)cap_slots_ * sizeof(InstDesc), hipMemcpyHostToDevice,
-                            stream_));
-    GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)(2 * n) * sizeof(int32_t), hipMemcpyHostToDevice,
-                            stream_));
+    for (int i = 0; i < n; ++i) total += h_desc_[slots[i]].ntiles;
     apply_options();
     // Lazy capacities pay when the build's 16 B/node of capacity writes (about
     // 3.7 ns per tile at the build's bandwidth) outweigh one more launch per
     // step (about 4 us): judged by the previous solve's step count, or without
     // one by size alone (32 k tiles = 32 M nodes). Large batches with few
-    // steps qualify; single images and long solves do not.
+    // steps qualify; single images and long solves do not. The descriptors
+    // carry the choice to the kernels.
     const int64_t need = steps_hint > 0 ? (int64_t)steps_hint * 1024 : 32768;
     const int lazy = kp_.variant && (opt.lazy_caps == 2 || (opt.lazy_caps == 0 && (int64_t)total >= need)) ? 1 : 0;
-    if (lazy != kp_.lazy) {
-        kp_.lazy = lazy;
-        drop_graphs();  // the captured steps carry the flag and the launch sequence
+    for (int s : slots) h_desc_[s].lazy = lazy;
+    // The pinned staging buffers are reused; wait for earlier copies out of them.
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    std::memcpy(pin_desc_, h_desc_.data(), (size_t)cap_slots_ * sizeof(InstDesc));
+    for (int i = 0, acc = 0; i < n; ++i) {
+        pin_slots_[i] = slots[i];
+        pin_slots_[n + i] = acc;
+        acc += h_desc_[slots[i]].ntiles;
     }
+    GCFG_HIP(hipMemcpyAsync(kp_.desc, pin_desc_, (size_t)cap_slots_ * sizeof(InstDesc), hipMemcpyHostToDevice,
+                            stream_));
+    GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)(2 * n) * sizeof(int32_t), hipMemcpyHostToDevice,
+                            stream_));
+    update_matq();
     launch_reset_stats(kp_, stream_);
     launch_arm(kp_, d_slots_, n, 0, opt.sweeps_per_gr, stream_);
     LaunchEvents ev;
@@ -350,17 +351,30 @@ int Engine::build(const std::vector<int> &slots) {
     }
     launch_build(kp_, d_slots_, n, total, stream_, ev);
     GCFG_HIP(hipGetLastError());
-    if (kp_.lazy) lazy_caps_ = true;
     return GCFG_OK;
 }
 
+void Engine::update_matq() {
+    int any = 0;
+    for (int i = 0; i < cap_slots_; ++i) any |= slot_used_[i] && h_desc_[i].lazy;
+    if (any != kp_.matq) {
+        kp_.matq = any;
+        drop_graphs();  // the captured steps carry the flag and the launch sequence
+    }
+}
+
 int Engine::materialize(const std::vector<int> &slots) {
+    bool any = false;
     for (int s : slots) {
-        if (s < 0 || s >= cap_slots_ || !slot_used_[s]) continue;
+        if (s < 0 || s >= cap_slots_ || !slot_used_[s] || !h_desc_[s].lazy) continue;
         launch_materialize(kp_, s, h_desc_[s].ntiles, stream_);
+        h_desc_[s].lazy = 0;  // every capacity written, no inflow pending
+        any = true;
     }
     GCFG_HIP(hipGetLastError());
-    return GCFG_OK;
+    if (!any) return GCFG_OK;
+    update_matq();
+    return upload_desc();  // ordered after the materialising kernels on the stream
 }
 
 int Engine::upload_desc() {
@@ -508,15 +522,14 @@ int Engine::launch_steps(int n) {
 int Engine::run(int64_t max_steps) {
     const int64_t start = steps_run;
     apply_options();
-    if (lazy_caps_ && !kp_.variant) {
+    if (kp_.matq && !kp_.variant) {
         // The workgroup kernels read capacities directly: write out every
-        // instance's (the options switched kernels after a wavefront build).
+        // lazy instance's (the options switched kernels after a wavefront build).
         std::vector<int> all;
         for (int i = 0; i < cap_slots_; ++i)
             if (slot_used_[i]) all.push_back(i);
         int rc = materialize(all);
         if (rc != GCFG_OK) return rc;
-        lazy_caps_ = false;
     }
     int chunk = steps_hint > 0 ? std::min(steps_hint, 4096) : opt.steps_per_poll;
     for (;;) {
@@ -550,7 +563,7 @@ int Engine::collect_profile() {
         GCFG_HIP(hipEventElapsedTime(&a, events_[8 * i], events_[8 * i + 1]));
         GCFG_HIP(hipEventElapsedTime(&b, events_[8 * i + 2], events_[8 * i + 3]));
         GCFG_HIP(hipEventElapsedTime(&c, events_[8 * i + 4], events_[8 * i + 5]));
-        if (kp_.variant && kp_.lazy) GCFG_HIP(hipEventElapsedTime(&m, events_[8 * i + 6], events_[8 * i + 7]));
+        if (kp_.variant && kp_.matq) GCFG_HIP(hipEventElapsedTime(&m, events_[8 * i + 6], events_[8 * i + 7]));
         b += m;
         prof_sched_ms += a;
         prof_pr_ms += b;
@@ -617,7 +630,8 @@ int Engine::reframe(int slot, const int32_t *pos, const int32_t *neg, const int3
     }
     if (d.W < 3 || d.H < 3) return set_inputs(slot, pos, neg, up, down, left, right, penalty, d.label_out);
     const int32_t *in1[6] = {pos, neg, up, up ? down : nullptr, up ? left : nullptr, up ? right : nullptr};
-    launch_materialize(kp_, slot, d.ntiles, stream_);  // the warm start reads and writes residuals directly
+    int rc0 = materialize({slot});  // the warm start reads and writes residuals directly
+    if (rc0 != GCFG_OK) return rc0;
     launch_reframe(kp_, slot, d.ntiles, in1, penalty, stream_);
     GCFG_HIP(hipGetLastError());
     int rc = set_inputs(slot, pos, neg, up, down, left, right, penalty, d.label_out);

@@ -10,9 +10,10 @@
 // Excess < 0 means remaining sink capacity (t-links are pre-saturated,
 // GraphCutKernels.cu:196-197). Pixels outside the interior are inert nodes
 // with zero capacities (GraphCutKernels.cu:122-146). With the wavefront
-// kernels the capacities are materialised lazily: the build writes excess,
-// distance and mask only, and a tile's capacities are read from the inputs
-// until its first push pass writes them (KParams::mat, ::inflow).
+// kernels the capacities of a large batch are materialised lazily: the build
+// writes excess, distance and mask only, and a tile's capacities are read from
+// the inputs until its first push pass writes them (InstDesc::lazy,
+// KParams::mat, ::inflow).
 //
 // Control: a scheduler kernel advances one state machine per instance
 // (START -> global relabel -> push/relabel colour passes -> ... -> FIN ->
@@ -69,6 +70,12 @@ struct InstDesc {
     int32_t *label_out;  // device W*H
     // 8-neighbourhood only: capacities to (x-1,y-1), (x+1,y-1), (x-1,y+1), (x+1,y+1)
     const int32_t *upleft, *upright, *downleft, *downright;
+    // 1: built with lazy capacities (wavefront kernels): tiles get their
+    // capacities before their first push pass, and flow pushed into a
+    // neighbour tile adds the reverse arc to its inflow slots. 0: the build
+    // wrote every capacity and pushes update the neighbour's directly.
+    int32_t lazy;
+    int32_t pad_;
 };
 
 struct InstState {
@@ -133,9 +140,9 @@ struct KParams {
     // [2 TH, 2 TH + TW) row 0 up arcs, [2 TH + TW, INFLOW) row TH-1 down arcs.
     int32_t *mat;
     int32_t *inflow;
-    // 1: the last build left capacities lazy and each step runs k_matq before
-    // the push pass; 0: the build wrote them (chosen per build by the engine).
-    int32_t lazy;
+    // 1: some instance was built with lazy capacities, so each step runs
+    // k_matq before the push pass (host-side launch choice).
+    int32_t matq;
     int32_t n_slots;
     int32_t inner_iters;
     int32_t variant;  // 0: workgroup-per-tile kernels (LDS), 1: wavefront-per-tile kernels (registers)
@@ -164,7 +171,7 @@ void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_til
 void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s);
 void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvents &ev);
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
-// Capacities of the queued push tiles that are still lazy (before launch_pr; no-op unless p.lazy).
+// Capacities of the queued push tiles that are still lazy (before launch_pr; no-op unless p.matq).
 void launch_matq(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
 void launch_gr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
 bool gr_block_supported(int bx, int by);
@@ -291,7 +298,7 @@ class Engine {
     int dev_ = 0;
     int cu_count_ = 0;
     int nbhd_ = 4;
-    bool lazy_caps_ = false;  // some built instance may have unmaterialised capacities
+    void update_matq();       // kp_.matq from the instances' lazy flags (drops stale step graphs)
     hipStream_t stream_ = nullptr;
     KParams kp_{};
     int cap_tiles_ = 0, cap_slots_ = 0;

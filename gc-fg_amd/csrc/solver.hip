@@ -1784,6 +1784,7 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         const int dthr = rdesc[ti.inst].n_interior;
         const int tile0 = rdesc[ti.inst].tile0, tiles_x = rdesc[ti.inst].tiles_x, tiles_y = rdesc[ti.inst].tiles_y;
         const int capv = rstate[ti.inst].cap;  // nodes at the cap are frozen until the next relabel
+        const bool lz = rdesc[ti.inst].lazy != 0;
         int hU, hD, hs;
         load_halo<HALO_DIST>(p, ti, t, lane, hU, hD, hs);
         int4 c[TH];
@@ -1799,9 +1800,10 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             e[y] = T.e[y * TW + lane];
             d[y] = T.d[y * TW + lane];
         }
-        // Reverse-arc capacity that neighbours' pushes left for the boundary
-        // nodes (they do not run in this colour pass): add it, clear the slots.
-        {
+        // Lazy instance: reverse-arc capacity that neighbours' pushes left for
+        // the boundary nodes (they do not run in this colour pass): add it,
+        // clear the slots.
+        if (lz) {
             int32_t *f = p.inflow + (size_t)t * INFLOW;
             const int fs = lane < 2 * TH ? f[lane] : 0;  // lanes 0..15: column 0, 16..31: column TW-1
             const int ft = f[2 * TH + lane], fb = f[2 * TH + TW + lane];
@@ -1932,20 +1934,27 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         const size_t gU = (size_t)(pushU ? ti.nbr[2] : 0) * TN + (size_t)(TH - 1) * TW + lane;
         const size_t gD = (size_t)(pushD ? ti.nbr[3] : 0) * TN + lane;
         int oldS = 0, oldU = 0, oldD = 0;
-        // Reverse arcs go to the receiving tile's inflow slots (its capacities
-        // may still live in the inputs): down arcs of the upper tile's last
-        // row, up arcs of the lower tile's first row.
+        // Reverse arcs: in a lazy instance to the receiving tile's inflow
+        // slots (its capacities may still live in the inputs; down arcs of the
+        // upper tile's last row, up arcs of the lower tile's first row),
+        // otherwise to its capacities directly.
+        int *revS = lz ? &p.inflow[(size_t)(pushS ? side : 0) * INFLOW + slotS]
+                       : reinterpret_cast<int *>(&p.caps[gS]) + (lane < TH ? 1 : 0);
+        int *revU = lz ? &p.inflow[(size_t)(pushU ? ti.nbr[2] : 0) * INFLOW + 2 * TH + TW + lane]
+                       : reinterpret_cast<int *>(&p.caps[gU]) + 3;
+        int *revD = lz ? &p.inflow[(size_t)(pushD ? ti.nbr[3] : 0) * INFLOW + 2 * TH + lane]
+                       : reinterpret_cast<int *>(&p.caps[gD]) + 2;
         if (pushS) {
             oldS = atomicAdd(&p.exc[gS], accS);
-            atomicAdd(&p.inflow[(size_t)side * INFLOW + slotS], accS);
+            atomicAdd(revS, accS);
         }
         if (pushU) {
             oldU = atomicAdd(&p.exc[gU], accU);
-            atomicAdd(&p.inflow[(size_t)ti.nbr[2] * INFLOW + 2 * TH + TW + lane], accU);
+            atomicAdd(revU, accU);
         }
         if (pushD) {
             oldD = atomicAdd(&p.exc[gD], accD);
-            atomicAdd(&p.inflow[(size_t)ti.nbr[3] * INFLOW + 2 * TH + lane], accD);
+            atomicAdd(revD, accD);
         }
         if (pushS) mask_after_push_in(p, gS, arcS, oldS + accS);
         if (pushU) mask_after_push_in(p, gU, 8, oldU + accU);
@@ -2108,7 +2117,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
                 // Lazy build: capacities stay in the inputs until a push pass
                 // needs them (most tiles never see one). Excess is not read
                 // again before a push pass either: a streaming store.
-                if (!p.lazy) T.c[y * TW + lane] = c;
+                if (!D.lazy) T.c[y * TW + lane] = c;
                 __builtin_nontemporal_store(e, &T.e[y * TW + lane]);
                 const int mk = node_mask(c, e);
                 m.put(y, mk);
@@ -2161,7 +2170,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
             if (rank) atomicMax(&p.live[block_anchor(p, D, tx, ty)], rank);
             p.tsum[2 * (size_t)t] = sp;
             p.tsum[2 * (size_t)t + 1] = sx;
-            p.mat[t] = p.lazy ? 0 : 1;
+            p.mat[t] = D.lazy ? 0 : 1;
             if (any_bad) atomicOr(&p.state[s].err, 1);
         }
     }
@@ -2412,7 +2421,7 @@ void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvent
     launch_timed(k_sched, dim3(p.n_slots), dim3(NT), s, ev, p, parity);
 }
 void launch_matq(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
-    if (!p.variant || !p.lazy) return;  // every capacity was written by the build
+    if (!p.variant || !p.matq) return;  // every capacity was written by the build
     launch_timed(k_matq, dim3(grid), dim3(64), s, ev, p, parity, (const int32_t *)p.queue_pr,
                  (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
 }
