@@ -243,62 +243,89 @@ enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2, EM_LIVE = 3, EM_D
 // (identified by their anchor tile, whose flags stand for the block)
 // selected by `kind` to a queue (ballot compaction, one atomic per 256
 // entries); returns the number appended.
-__device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int colour, bool to_pr, int grop,
-                          int parity) {
+constexpr int EMIT_U = 8;  // units per thread in one pass of emit_tiles
+
+__device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int colour, bool to_pr,
+                                          int grop, int parity) {
     __shared__ int wcount[NT / 64], wbase[NT / 64], s_qbase, s_total;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int32_t *queue = to_pr ? p.queue_pr : p.queue_gr;
     int32_t *qlen = to_pr ? &p.ctl->q[parity].qlen_pr : &p.ctl->q[parity].qlen_gr;
     const int bx = to_pr ? 1 : p.gbx, by = to_pr ? 1 : p.gby;
+    const bool unit = bx == 1 && by == 1;
     const int nbx = (D.tiles_x + bx - 1) / bx;
-    const int nunits = bx == 1 && by == 1 ? D.ntiles : nbx * ((D.tiles_y + by - 1) / by);
+    const int nunits = unit ? D.ntiles : nbx * ((D.tiles_y + by - 1) / by);
+    // The flag a unit's predicate reads first, and the second one if any.
+    const int32_t *f0 = kind == EM_DIRTY ? p.dirty : kind == EM_COLOUR ? p.act_flag : p.live;
+    const int32_t *f1 = kind == EM_LIVE ? p.stale : kind == EM_DIRTY ? p.live : nullptr;
     __syncthreads();  // earlier readers of s_total are done
     if (threadIdx.x == 0) s_total = 0;
-    for (int base = 0; base < nunits; base += NT) {
-        const int lu = base + threadIdx.x;
-        int lt = lu;
-        if (!(bx == 1 && by == 1) && lu < nunits) lt = (lu / nbx) * by * D.tiles_x + (lu % nbx) * bx;
-        const int t = D.tile0 + lt;
-        int pred = 0;
-        if (lu < nunits) {
-            if (kind == EM_ALL) {
-                pred = 1;
-            } else if (kind == EM_DEAD) {
-                // FIN: blocks the last round did not visit (every node
-                // unreachable, or every interior node reaching a sink)
-                pred = p.live[t] == 0 || p.live[t] == 3;
-            } else if (kind == EM_LIVE) {
-                // Exact round (colour 1): every block with a reachable node.
-                // Reachability round (colour 2): every block with a node that
-                // may not reach a sink. Capped round: blocks with a node below
-                // the cap that a push pass may have changed (the others keep
-                // their labels).
-                const int lv = p.live[t];
-                pred = colour == 1 ? lv != 0 : colour == 2 ? (lv == 1 || lv == 2) : (lv >= 2 && p.stale[t]);
-                p.stale[t] = 0;
-                if (!colour && lv == 1) {
-                    // A skipped block's nodes are all frozen: none is active.
-                    const int tx0 = lt % D.tiles_x, ty0 = lt / D.tiles_x;
-                    for (int j = 0; j < by && ty0 + j < D.tiles_y; ++j)
-                        for (int i = 0; i < bx && tx0 + i < D.tiles_x; ++i) p.act_flag[t + j * D.tiles_x + i] = 0;
-                }
-            } else if (kind == EM_DIRTY) {
-                // colour 1 (a reachability round): a block whose nodes all
-                // reach a sink already cannot change, whatever its halo does.
-                pred = p.dirty[t] && !(colour && p.live[t] == 3);
-                if (p.dirty[t]) p.dirty[t] = 0;
-            } else {
-                // 4-neighbourhood: checkerboard (no two side neighbours run
-                // together). 8-neighbourhood: (tx & 1, ty & 1) four colours,
-                // so diagonal neighbours do not run together either.
-                const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
-                const int c = p.nbhd == 8 ? ((tx & 1) | ((ty & 1) << 1)) : ((tx + ty) & 1);
-                pred = c == colour && p.act_flag[t];
+    if (nunits == 0) __syncthreads();
+    // Passes of NT * EMIT_U units: wave w takes the contiguous units
+    // [base + w * 64 * EMIT_U, + 64 * EMIT_U), so the queue keeps the units in
+    // order, and one queue reservation covers the pass. Every flag is loaded
+    // before any is cleared, so the loads of a pass are in flight together.
+    for (int base = 0; base < nunits; base += NT * EMIT_U) {
+        int lt[EMIT_U], v0[EMIT_U], v1[EMIT_U];
+#pragma unroll
+        for (int k = 0; k < EMIT_U; ++k) {
+            const int lu = base + (wave * EMIT_U + k) * 64 + lane;
+            lt[k] = -1;
+            v0[k] = v1[k] = 0;
+            if (lu < nunits) {
+                lt[k] = unit ? lu : (lu / nbx) * by * D.tiles_x + (lu % nbx) * bx;
+                const int t = D.tile0 + lt[k];
+                if (kind != EM_ALL) v0[k] = f0[t];
+                if (f1) v1[k] = f1[t];
             }
         }
-        const unsigned long long bal = __ballot(pred);
-        const int wpre = __popcll(bal & ((1ull << lane) - 1ull));
-        if (lane == 0) wcount[wave] = __popcll(bal);
+        uint64_t bal[EMIT_U];
+        int wsum = 0;
+#pragma unroll
+        for (int k = 0; k < EMIT_U; ++k) {
+            int pred = 0;
+            if (lt[k] >= 0) {
+                const int t = D.tile0 + lt[k];
+                if (kind == EM_ALL) {
+                    pred = 1;
+                } else if (kind == EM_DEAD) {
+                    // FIN: blocks the last round did not visit (every node
+                    // unreachable, or every interior node reaching a sink)
+                    pred = v0[k] == 0 || v0[k] == 3;
+                } else if (kind == EM_LIVE) {
+                    // Exact round (colour 1): every block with a reachable node.
+                    // Reachability round (colour 2): every block with a node
+                    // that may not reach a sink. Capped round: blocks with a
+                    // node below the cap that a push pass may have changed
+                    // (the others keep their labels).
+                    const int lv = v0[k];
+                    pred = colour == 1 ? lv != 0 : colour == 2 ? (lv == 1 || lv == 2) : (lv >= 2 && v1[k]);
+                    if (v1[k]) p.stale[t] = 0;
+                    if (!colour && lv == 1) {
+                        // A skipped block's nodes are all frozen: none is active.
+                        const int tx0 = lt[k] % D.tiles_x, ty0 = lt[k] / D.tiles_x;
+                        for (int j = 0; j < by && ty0 + j < D.tiles_y; ++j)
+                            for (int i = 0; i < bx && tx0 + i < D.tiles_x; ++i)
+                                p.act_flag[t + j * D.tiles_x + i] = 0;
+                    }
+                } else if (kind == EM_DIRTY) {
+                    // colour 1 (a reachability round): a block whose nodes all
+                    // reach a sink already cannot change, whatever its halo does.
+                    pred = v0[k] && !(colour && v1[k] == 3);
+                    if (v0[k]) p.dirty[t] = 0;
+                } else {
+                    // 4-neighbourhood: checkerboard (no two side neighbours
+                    // run together). 8-neighbourhood: (tx & 1, ty & 1) four
+                    // colours, so diagonal neighbours do not run together either.
+                    const int tx = lt[k] % D.tiles_x, ty = lt[k] / D.tiles_x;
+                    const int c = p.nbhd == 8 ? ((tx & 1) | ((ty & 1) << 1)) : ((tx + ty) & 1);
+                    pred = c == colour && v0[k];
+                }
+            }
+            bal[k] = __ballot(pred);
+            wsum += __popcll(bal[k]);
+        }
+        if (lane == 0) wcount[wave] = wsum;
         __syncthreads();
         if (threadIdx.x == 0) {
             int tot = 0;
@@ -310,14 +337,24 @@ __device__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int col
             s_total += tot;
         }
         __syncthreads();
-        if (pred) queue[s_qbase + wbase[wave] + wpre] = to_pr ? t : ((t << 3) | grop);
+        int pos = s_qbase + wbase[wave];
+        const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+        for (int k = 0; k < EMIT_U; ++k) {
+            if ((bal[k] >> lane) & 1ull) {
+                const int t = D.tile0 + lt[k];
+                queue[pos + __popcll(bal[k] & below)] = to_pr ? t : ((t << 3) | grop);
+            }
+            pos += __popcll(bal[k]);
+        }
         __syncthreads();
     }
     return s_total;
 }
 
-__device__ bool any_active_tile(const KParams &p, const InstDesc &D) {
+__device__ __forceinline__ bool any_active_tile(const KParams &p, const InstDesc &D) {
     int any = 0;
+#pragma unroll 8
     for (int lt = threadIdx.x; lt < D.ntiles; lt += NT) any |= p.act_flag[D.tile0 + lt];
     return __syncthreads_or(any) != 0;
 }
@@ -337,8 +374,8 @@ __device__ bool any_active_tile(const KParams &p, const InstDesc &D) {
 // active) until the next relabel. Only an exact relabel (cap = DINF) can end
 // the solve, so a capped round that leaves no active node is followed by an
 // exact one.
-__device__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st, int parity, bool reinit,
-                                     int cap, bool all = false) {
+__device__ __forceinline__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st, int parity,
+                                                     bool reinit, int cap, bool all = false) {
     if (threadIdx.x == 0) {
         st.gr_rounds++;
         st.phase = PH_GR;
