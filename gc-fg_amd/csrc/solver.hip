@@ -2079,8 +2079,13 @@ __device__ __forceinline__ int input_d0(const InstDesc &D, int X, int Y) {
     return (in && pp < qq) ? 0 : DINF;
 }
 
-__global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, const int32_t *prefix, int n,
-                                               int total) {
+// The build map, tile records and descriptors come in as restrict read-only
+// arguments, like the solve kernels' queue arguments: their loads are uniform
+// and go through the scalar cache, so a tile's prologue is one scalar round
+// trip per level instead of vector loads and waits.
+__global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restrict__ rbmap,
+                                               const TileInfo *__restrict__ rtiles,
+                                               const InstDesc *__restrict__ rdesc, int total) {
     const int lane = threadIdx.x;
     // Workgroups are dealt round-robin to the 8 XCDs, each with its own L2.
     // Giving every XCD a contiguous eighth of the tiles keeps horizontally
@@ -2096,10 +2101,10 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         if (b >= total) break;
         // Tile, instance and tile coordinates from the map and the tile
         // records k_clear_tiles wrote just before (two dependent loads).
-        const int t = p.bmap[b];
-        const TileInfo ti = p.tiles[t];
+        const int t = rbmap[b];
+        const TileInfo ti = rtiles[t];
         const int s = ti.inst;
-        const InstDesc D = p.desc[s];
+        const InstDesc D = rdesc[s];
         const int tx = ti.tx, ty = ti.ty;
         const int X = tx * TW + lane, Y0 = ty * TH;
         const TileRef T = tile_ref(p, t);
@@ -2109,9 +2114,12 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *slots, 
         const int capv = p.gr_cap > 0 ? p.gr_cap : DINF;  // the first relabel sweep, capped like k_arm says
         int64_t sp = 0, sx = 0;
         int bad = 0;
-        // Halo distances from the inputs (issued first, consumed after the rows).
-        const int hU = ty > 0 ? input_d0(D, X, Y0 - 1) : DINF;
-        const int hD = ty < D.tiles_y - 1 ? input_d0(D, X, Y0 + TH) : DINF;
+        // Halo distances from the inputs (issued first, consumed after the
+        // rows; loaded unconditionally, from clamped in-bounds addresses, so
+        // that no branch waits for them).
+        const int hUv = input_d0(D, X, Y0 - 1), hDv = input_d0(D, X, Y0 + TH);
+        const int hU = ty > 0 ? hUv : DINF;
+        const int hD = ty < D.tiles_y - 1 ? hDv : DINF;
         const bool hleft = lane < TH && tx > 0, hright = lane >= TH && lane < 2 * TH && tx < D.tiles_x - 1;
         const int hsv = input_d0(D, lane < TH ? tx * TW - 1 : tx * TW + TW, Y0 + (lane & (TH - 1)));
         const int hs = (hleft || hright) ? hsv : DINF;
@@ -2434,8 +2442,8 @@ void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_til
                            total_tiles);
         int grid = total_tiles < 65536 ? total_tiles : 65536;
         if (grid >= NUM_XCD) grid -= grid % NUM_XCD;  // whole rounds over the XCDs
-        launch_timed(k_buildw, dim3(grid), dim3(64), s, ev, p, d_slots, (const int32_t *)(d_slots + n), n,
-                     total_tiles);
+        launch_timed(k_buildw, dim3(grid), dim3(64), s, ev, p, (const int32_t *)p.bmap, (const TileInfo *)p.tiles,
+                     (const InstDesc *)p.desc, total_tiles);
         return;
     }
     const int grid = total_tiles < 8192 ? total_tiles : 8192;
