@@ -294,7 +294,6 @@ void Engine::remove_instance(int slot) {
     free_tiles(h_desc_[slot].tile0, h_desc_[slot].ntiles);
     slot_used_[slot] = 0;
     h_desc_[slot] = InstDesc{};
-    update_matq();
 }
 
 int Engine::build(const std::vector<int> &slots) {
@@ -339,7 +338,6 @@ int Engine::build(const std::vector<int> &slots) {
                             stream_));
     GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)(2 * n) * sizeof(int32_t), hipMemcpyHostToDevice,
                             stream_));
-    update_matq();
     launch_reset_stats(kp_, stream_);
     launch_arm(kp_, d_slots_, n, 0, opt.sweeps_per_gr, stream_);
     LaunchEvents ev;
@@ -354,13 +352,10 @@ int Engine::build(const std::vector<int> &slots) {
     return GCFG_OK;
 }
 
-void Engine::update_matq() {
-    int any = 0;
-    for (int i = 0; i < cap_slots_; ++i) any |= slot_used_[i] && h_desc_[i].lazy;
-    if (any != kp_.matq) {
-        kp_.matq = any;
-        drop_graphs();  // the captured steps carry the flag and the launch sequence
-    }
+bool Engine::any_lazy() const {
+    for (int i = 0; i < cap_slots_; ++i)
+        if (slot_used_[i] && h_desc_[i].lazy) return true;
+    return false;
 }
 
 int Engine::materialize(const std::vector<int> &slots) {
@@ -373,7 +368,6 @@ int Engine::materialize(const std::vector<int> &slots) {
     }
     GCFG_HIP(hipGetLastError());
     if (!any) return GCFG_OK;
-    update_matq();
     return upload_desc();  // ordered after the materialising kernels on the stream
 }
 
@@ -465,8 +459,8 @@ int Engine::step_graph(int steps, hipGraphExec_t *out) {
 int Engine::launch_steps(int n) {
     apply_options();
     if (opt.profile) {
-        if ((int)events_.size() < 8 * n) {
-            for (int i = (int)events_.size(); i < 8 * n; ++i) {
+        if ((int)events_.size() < 6 * n) {
+            for (int i = (int)events_.size(); i < 6 * n; ++i) {
                 hipEvent_t e;
                 GCFG_HIP(hipEventCreate(&e));
                 events_.push_back(e);
@@ -480,9 +474,8 @@ int Engine::launch_steps(int n) {
             qtrace_cap_ = n;
         }
         for (int i = 0; i < n; ++i) {
-            hipEvent_t *ev = &events_[8 * i];
+            hipEvent_t *ev = &events_[6 * i];
             launch_sched(kp_, parity_, stream_, LaunchEvents{ev[0], ev[1]});
-            launch_matq(kp_, grid_pr_, parity_, stream_, LaunchEvents{ev[6], ev[7]});
             launch_pr(kp_, grid_pr_, parity_, stream_, LaunchEvents{ev[2], ev[3]});
             launch_gr(kp_, grid_gr_, parity_, stream_, LaunchEvents{ev[4], ev[5]});
             // This step's queue lengths, before the next scheduler clears them.
@@ -522,7 +515,7 @@ int Engine::launch_steps(int n) {
 int Engine::run(int64_t max_steps) {
     const int64_t start = steps_run;
     apply_options();
-    if (kp_.matq && !kp_.variant) {
+    if (!kp_.variant && any_lazy()) {
         // The workgroup kernels read capacities directly: write out every
         // lazy instance's (the options switched kernels after a wavefront build).
         std::vector<int> all;
@@ -559,12 +552,9 @@ int Engine::collect_profile() {
     }
     for (int i = 0; i < events_pending_; ++i) {
         float a = 0, b = 0, c = 0;
-        float m = 0;
-        GCFG_HIP(hipEventElapsedTime(&a, events_[8 * i], events_[8 * i + 1]));
-        GCFG_HIP(hipEventElapsedTime(&b, events_[8 * i + 2], events_[8 * i + 3]));
-        GCFG_HIP(hipEventElapsedTime(&c, events_[8 * i + 4], events_[8 * i + 5]));
-        if (kp_.variant && kp_.matq) GCFG_HIP(hipEventElapsedTime(&m, events_[8 * i + 6], events_[8 * i + 7]));
-        b += m;
+        GCFG_HIP(hipEventElapsedTime(&a, events_[6 * i], events_[6 * i + 1]));
+        GCFG_HIP(hipEventElapsedTime(&b, events_[6 * i + 2], events_[6 * i + 3]));
+        GCFG_HIP(hipEventElapsedTime(&c, events_[6 * i + 4], events_[6 * i + 5]));
         prof_sched_ms += a;
         prof_pr_ms += b;
         prof_gr_ms += c;

@@ -140,9 +140,6 @@ struct KParams {
     // [2 TH, 2 TH + TW) row 0 up arcs, [2 TH + TW, INFLOW) row TH-1 down arcs.
     int32_t *mat;
     int32_t *inflow;
-    // 1: some instance was built with lazy capacities, so each step runs
-    // k_matq before the push pass (host-side launch choice).
-    int32_t matq;
     int32_t n_slots;
     int32_t inner_iters;
     int32_t variant;  // 0: workgroup-per-tile kernels (LDS), 1: wavefront-per-tile kernels (registers)
@@ -171,8 +168,6 @@ void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_til
 void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s);
 void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvents &ev);
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
-// Capacities of the queued push tiles that are still lazy (before launch_pr; no-op unless p.matq).
-void launch_matq(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
 void launch_gr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
 bool gr_block_supported(int bx, int by);
 bool options_valid(const gcfg_options &o);  // sets the error message when not
@@ -278,7 +273,6 @@ class Engine {
     // steps.
     int steps_hint = 0;
     // Event timing of the launches since the last reset (profile mode).
-    // prof_pr_ms includes k_matq's capacity writes before each push pass.
     double prof_sched_ms = 0, prof_pr_ms = 0, prof_gr_ms = 0, prof_build_ms = 0;
     int64_t prof_steps = 0;
     // Keeps a timed build that has not been collected yet: the plain graph
@@ -298,7 +292,7 @@ class Engine {
     int dev_ = 0;
     int cu_count_ = 0;
     int nbhd_ = 4;
-    void update_matq();       // kp_.matq from the instances' lazy flags (drops stale step graphs)
+    bool any_lazy() const;    // some live instance still has capacities only in its inputs
     hipStream_t stream_ = nullptr;
     KParams kp_{};
     int cap_tiles_ = 0, cap_slots_ = 0;
@@ -324,7 +318,7 @@ class Engine {
     int step_graph(int steps, hipGraphExec_t *out);
     int parity_ = 0;  // parity of the next step to launch
     std::vector<void *> allocs_;
-    std::vector<hipEvent_t> events_;  // 8 per profiled step: start/stop of sched, matq, pr, gr
+    std::vector<hipEvent_t> events_;  // 6 per profiled step: start/stop of sched, pr, gr
     int events_pending_ = 0;
     QCtl *pin_qtrace_ = nullptr;      // queue lengths per profiled step (verbose >= 2)
     hipEvent_t build_ev_[2] = {nullptr, nullptr};  // the build kernel's start / stop (profile mode)

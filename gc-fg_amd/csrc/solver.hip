@@ -244,6 +244,9 @@ enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2, EM_LIVE = 3, EM_D
 // selected by `kind` to a queue (ballot compaction, one atomic per 256
 // entries); returns the number appended.
 constexpr int EMIT_U = 8;  // units per thread in one pass of emit_tiles
+// Push-queue item flag: the tile's capacities are still only in the inputs
+// (lazy build); the push pass reads them from there and writes them out.
+constexpr int PR_FRESH = 1 << 30;
 
 __device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int colour, bool to_pr,
                                           int grop, int parity) {
@@ -255,9 +258,14 @@ __device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, i
     const bool unit = bx == 1 && by == 1;
     const int nbx = (D.tiles_x + bx - 1) / bx;
     const int nunits = unit ? D.ntiles : nbx * ((D.tiles_y + by - 1) / by);
-    // The flag a unit's predicate reads first, and the second one if any.
+    // The flag a unit's predicate reads first, and the second one if any
+    // (push items of a lazy instance: whether the tile's capacities are
+    // materialised yet).
     const int32_t *f0 = kind == EM_DIRTY ? p.dirty : kind == EM_COLOUR ? p.act_flag : p.live;
-    const int32_t *f1 = kind == EM_LIVE ? p.stale : kind == EM_DIRTY ? p.live : nullptr;
+    const int32_t *f1 = kind == EM_LIVE    ? p.stale
+                        : kind == EM_DIRTY ? p.live
+                        : (kind == EM_COLOUR && D.lazy) ? p.mat
+                                                        : nullptr;
     __syncthreads();  // earlier readers of s_total are done
     if (threadIdx.x == 0) s_total = 0;
     if (nunits == 0) __syncthreads();
@@ -343,7 +351,8 @@ __device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, i
         for (int k = 0; k < EMIT_U; ++k) {
             if ((bal[k] >> lane) & 1ull) {
                 const int t = D.tile0 + lt[k];
-                queue[pos + __popcll(bal[k] & below)] = to_pr ? t : ((t << 3) | grop);
+                const int fresh = (kind == EM_COLOUR && D.lazy && !v1[k]) ? PR_FRESH : 0;
+                queue[pos + __popcll(bal[k] & below)] = to_pr ? (t | fresh) : ((t << 3) | grop);
             }
             pos += __popcll(bal[k]);
         }
@@ -1778,30 +1787,6 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCF
     if (p.profile && lane == 0) atomic_add64(&p.ctl->gr_iters, n_rounds);
 }
 
-// Before a push pass of a lazily built engine: write the capacities of the
-// queued tiles that still have them only in the inputs (one wavefront per
-// tile, the queue's own contiguous runs as in k_prw). A launch of its own:
-// inside k_prw the second load path raised the kernel's registers and its
-// per-item latency for every tile.
-__global__ __launch_bounds__(64) void k_matq(KParams p, int parity, GCFG_RO_ARGS) {
-    const int qlen = p.ctl->q[parity].qlen_pr;
-    if ((int)blockIdx.x >= qlen) return;
-    const int lane = threadIdx.x;
-    const int per = (qlen + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int q_end = min(qlen, ((int)blockIdx.x + 1) * per);
-    for (int q = (int)blockIdx.x * per; q < q_end; ++q) {
-        const int t = rq[q];
-        if (p.mat[t]) continue;
-        const TileInfo ti = rtiles[t];
-        const InstDesc D = rdesc[ti.inst];
-        const TileRef T = tile_ref(p, t);
-        const int X = ti.tx * TW + lane;
-#pragma unroll
-        for (int y = 0; y < TH; ++y) T.c[y * TW + lane] = input_caps(D, X, ti.ty * TH + y);
-        if (lane == 0) p.mat[t] = 1;
-    }
-}
-
 __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS) {
     const int qlen = p.ctl->q[parity].qlen_pr;
     if ((int)blockIdx.x >= qlen) return;
@@ -1814,7 +1799,9 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
     const int per = (qlen + (int)gridDim.x - 1) / (int)gridDim.x;
     const int q_end = min(qlen, ((int)blockIdx.x + 1) * per);
     for (int q = (int)blockIdx.x * per; q < q_end; ++q) {
-        const int t = rq[q];
+        const int item = rq[q];
+        const int t = item & (PR_FRESH - 1);
+        const bool fresh = (item & PR_FRESH) != 0;
         const TileRef T = tile_ref(p, t);
         const TileInfo ti = rtiles[t];
         // Descriptor fields and the cap read once, before any store.
@@ -1829,14 +1816,24 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         // Flow pushed out of the tile sideways, one row per lane: lanes 0..15
         // into the left tile, lanes 16..31 into the right tile.
         int accS = 0;
-        // Capacities are in the arena: written by the build, or by k_matq
-        // just before this pass when the build left them lazy.
+        // Capacities are in the arena (written by the build or by an earlier
+        // push pass). A fresh tile of a lazy build still has them only in
+        // the inputs: copy them into the arena first (its own stores, read
+        // back below in program order). Loading them from the inputs
+        // straight into registers would keep a second set of 64 loads in
+        // flight beside the arena path and lower the kernel's occupancy.
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
-            c[y] = T.c[y * TW + lane];
             e[y] = T.e[y * TW + lane];
             d[y] = T.d[y * TW + lane];
         }
+        if (fresh) {
+            const InstDesc D = rdesc[ti.inst];
+#pragma unroll
+            for (int y = 0; y < TH; ++y) T.c[y * TW + lane] = input_caps(D, ti.tx * TW + lane, ti.ty * TH + y);
+        }
+#pragma unroll
+        for (int y = 0; y < TH; ++y) c[y] = T.c[y * TW + lane];
         // Lazy instance: reverse-arc capacity that neighbours' pushes left for
         // the boundary nodes (they do not run in this colour pass): add it,
         // clear the slots.
@@ -2012,6 +2009,7 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             if (anyD) set_active(p, ti.nbr[3], 1);
             set_active(p, t, any_mine);
             p.tsum[2 * (size_t)t + 1] = sx;
+            if (fresh) p.mat[t] = 1;
         }
         if (p.gr_cap > 0) {
             // Blocks within cap distance of this tile (and of the tiles it
@@ -2465,11 +2463,6 @@ static void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t s, const L
 void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvents &ev) {
     launch_timed(k_sched, dim3(p.n_slots), dim3(NT), s, ev, p, parity);
 }
-void launch_matq(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
-    if (!p.variant || !p.matq) return;  // every capacity was written by the build
-    launch_timed(k_matq, dim3(grid), dim3(64), s, ev, p, parity, (const int32_t *)p.queue_pr,
-                 (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
-}
 
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
     if (p.variant)
@@ -2517,7 +2510,6 @@ void launch_gr(const KParams &p, int grid, int parity, hipStream_t s, const Laun
 
 void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s) {
     launch_sched(p, parity, s, LaunchEvents{});
-    launch_matq(p, grid_pr, parity, s, LaunchEvents{});
     launch_pr(p, grid_pr, parity, s, LaunchEvents{});
     launch_gr(p, grid_gr, parity, s, LaunchEvents{});
 }
