@@ -1817,11 +1817,11 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         // into the left tile, lanes 16..31 into the right tile.
         int accS = 0;
         // Capacities are in the arena (written by the build or by an earlier
-        // push pass). A fresh tile of a lazy build still has them only in
-        // the inputs: copy them into the arena first (its own stores, read
-        // back below in program order). Loading them from the inputs
-        // straight into registers would keep a second set of 64 loads in
-        // flight beside the arena path and lower the kernel's occupancy.
+        // push pass), or for a fresh tile of a lazy build still only in the
+        // inputs; this pass writes them to the arena either way. Excess and
+        // distances are loaded first: issued after the branch, the compiler
+        // kept both branches' loads in flight at once and the kernel's
+        // registers rose past two wavefronts per SIMD.
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
             e[y] = T.e[y * TW + lane];
@@ -1830,10 +1830,11 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         if (fresh) {
             const InstDesc D = rdesc[ti.inst];
 #pragma unroll
-            for (int y = 0; y < TH; ++y) T.c[y * TW + lane] = input_caps(D, ti.tx * TW + lane, ti.ty * TH + y);
-        }
+            for (int y = 0; y < TH; ++y) c[y] = input_caps(D, ti.tx * TW + lane, ti.ty * TH + y);
+        } else {
 #pragma unroll
-        for (int y = 0; y < TH; ++y) c[y] = T.c[y * TW + lane];
+            for (int y = 0; y < TH; ++y) c[y] = T.c[y * TW + lane];
+        }
         // Lazy instance: reverse-arc capacity that neighbours' pushes left for
         // the boundary nodes (they do not run in this colour pass): add it,
         // clear the slots.
