@@ -232,14 +232,12 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
     int rc = g->join.before(g->eng.stream());
     if (rc == GCFG_OK) rc = g->eng.run(g->opt.max_steps);
     if (rc == GCFG_OK) rc = g->join.after(g->eng.stream());
+    if (rc == GCFG_OK) rc = g->eng.fetch_states_pinned(g->slot, 1);
     if (rc != GCFG_OK) return rc;
-    InstState st;
-    rc = g->eng.fetch_state(g->slot, &st);
-    if (rc != GCFG_OK) return rc;
+    // The counters as of the final poll (nothing has run since).
+    const InstState st = g->eng.pinned_states()[0];
     g->eng.steps_hint = st.steps;
-    gcfg::Ctl ctl;
-    rc = g->eng.fetch_ctl(&ctl);
-    if (rc != GCFG_OK) return rc;
+    const gcfg::Ctl ctl = g->eng.polled_ctl();
     if (st.err) {
         set_error("invalid capacities in the inputs (negative n-link or |pos - neg| > 2^30)");
         return GCFG_ERR_INVALID;
@@ -404,13 +402,14 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
     if (rc == GCFG_OK) rc = b->eng.build(b->slots);
     if (rc == GCFG_OK) rc = b->eng.run(b->eng.opt.max_steps);
     if (rc == GCFG_OK) rc = b->join.after(b->eng.stream());
-    if (rc != GCFG_OK) return rc;
     // Instance slots were assigned 0..n-1 in order at creation: one copy.
-    rc = b->eng.fetch_states(b->states.data(), n);
+    // (Copying them with every completion poll instead cost a one-off stall
+    // of several ms a few solves into a process.)
+    if (rc == GCFG_OK) rc = b->eng.fetch_states_pinned(0, n);
     if (rc != GCFG_OK) return rc;
-    gcfg::Ctl ctl;
-    rc = b->eng.fetch_ctl(&ctl);
-    if (rc != GCFG_OK) return rc;
+    std::memcpy(b->states.data(), b->eng.pinned_states(), (size_t)n * sizeof(InstState));
+    // The counters as of the final poll (nothing has run since).
+    const gcfg::Ctl ctl = b->eng.polled_ctl();
     gcfg_stats agg{};
     agg.pr_iters = ctl.pr_iters;
     agg.gr_iters = ctl.gr_iters;

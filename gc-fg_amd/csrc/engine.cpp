@@ -37,6 +37,7 @@ Engine::~Engine() {
     if (pin_desc_) hipHostFree(pin_desc_);
     if (pin_slots_) hipHostFree(pin_slots_);
     if (pin_ctl_) hipHostFree(pin_ctl_);
+    if (pin_states_) hipHostFree(pin_states_);
     if (pin_qtrace_) hipHostFree(pin_qtrace_);
     if (stream_) hipStreamDestroy(stream_);
 }
@@ -125,6 +126,7 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o, int nbhd) 
     GCFG_HIP(hipHostMalloc((void **)&pin_desc_, (size_t)cap_slots * sizeof(InstDesc), hipHostMallocDefault));
     GCFG_HIP(hipHostMalloc((void **)&pin_slots_, (size_t)(2 * cap_slots + 1) * sizeof(int32_t), hipHostMallocDefault));
     GCFG_HIP(hipHostMalloc((void **)&pin_ctl_, sizeof(Ctl), hipHostMallocDefault));
+    GCFG_HIP(hipHostMalloc((void **)&pin_states_, (size_t)cap_slots * sizeof(InstState), hipHostMallocDefault));
     GCFG_HIP(hipMemsetAsync(kp_.state, 0, (size_t)cap_slots * sizeof(InstState), stream_));
     GCFG_HIP(hipMemsetAsync(kp_.desc, 0, (size_t)cap_slots * sizeof(InstDesc), stream_));
     GCFG_HIP(hipMemsetAsync(kp_.ctl, 0, sizeof(Ctl), stream_));
@@ -338,8 +340,7 @@ int Engine::build(const std::vector<int> &slots) {
                             stream_));
     GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)(2 * n) * sizeof(int32_t), hipMemcpyHostToDevice,
                             stream_));
-    launch_reset_stats(kp_, stream_);
-    launch_arm(kp_, d_slots_, n, 0, opt.sweeps_per_gr, stream_);
+    launch_arm(kp_, d_slots_, n, 0, opt.sweeps_per_gr, stream_, kp_.variant ? total : 0);
     LaunchEvents ev;
     if (opt.profile) {
         for (auto &e : build_ev_)
@@ -386,8 +387,7 @@ int Engine::rearm(const std::vector<int> &slots) {
     for (int i = 0; i < n; ++i) pin_slots_[i] = slots[i];
     GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
     apply_options();
-    launch_reset_stats(kp_, stream_);
-    launch_arm(kp_, d_slots_, n, 1, opt.sweeps_per_gr, stream_);
+    launch_arm(kp_, d_slots_, n, 1, opt.sweeps_per_gr, stream_, 0);
     GCFG_HIP(hipGetLastError());
     return GCFG_OK;
 }
@@ -592,6 +592,17 @@ int Engine::fetch_state(int slot, InstState *out) {
 
 int Engine::fetch_states(InstState *out, int n) {
     GCFG_HIP(hipMemcpyAsync(out, kp_.state, (size_t)n * sizeof(InstState), hipMemcpyDeviceToHost, stream_));
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    return GCFG_OK;
+}
+
+int Engine::fetch_states_pinned(int first, int n) {
+    if (first < 0 || n < 0 || first + n > cap_slots_) {
+        set_error("fetch_states_pinned: range outside the instance slots");
+        return GCFG_ERR_INVALID;
+    }
+    GCFG_HIP(hipMemcpyAsync(pin_states_, kp_.state + first, (size_t)n * sizeof(InstState), hipMemcpyDeviceToHost,
+                            stream_));
     GCFG_HIP(hipStreamSynchronize(stream_));
     return GCFG_OK;
 }

@@ -130,7 +130,7 @@ struct KParams {
     InstState *state;
     Ctl *ctl;
     int32_t *queue_pr, *queue_gr;
-    int32_t *bmap;  // build list position -> tile id (written by k_clear_tiles for k_buildw)
+    int32_t *bmap;  // build list position -> tile id (written by k_arm for k_buildw)
     // Lazily materialised residual capacities (wavefront kernels). mat[t] = 0:
     // tile t's capacities have not been written since the build and are those
     // of the inputs (border rule applied); 1: they are in caps. Either way,
@@ -162,7 +162,11 @@ struct KParams {
 struct LaunchEvents {
     hipEvent_t start = nullptr, stop = nullptr;
 };
-void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int sweeps, hipStream_t s);
+// Arms instances d_slots[0, n) and clears the iteration counters; with
+// clear_tiles > 0 also writes the tile records and build map of the first
+// clear_tiles build-list positions (d_slots[n, 2n) = tile offsets).
+void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int sweeps, hipStream_t s,
+                int clear_tiles);
 void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_tiles, hipStream_t s,
                   const LaunchEvents &ev);
 void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStream_t s);
@@ -171,7 +175,6 @@ void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const Laun
 void launch_gr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
 bool gr_block_supported(int bx, int by);
 bool options_valid(const gcfg_options &o);  // sets the error message when not
-void launch_reset_stats(const KParams &p, hipStream_t s);
 void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,
                  hipStream_t s);
 void launch_read_state(const KParams &p, int slot, int32_t *d_caps, int32_t *d_exc, int32_t *d_dist,
@@ -231,8 +234,14 @@ class Engine {
     int upload_desc();
     // Re-arm instances from their current residual graph (warm start).
     int rearm(const std::vector<int> &slots);
-    // Run steps until every armed instance is done.
+    // Run steps until every armed instance is done. polled_ctl() then holds
+    // the counters as of the final completion poll.
     int run(int64_t max_steps);
+    const Ctl &polled_ctl() const { return *pin_ctl_; }
+    // States of slots [first, first + n) into pinned memory (one copy, one
+    // synchronisation); pinned_states() points at them.
+    int fetch_states_pinned(int first, int n);
+    const InstState *pinned_states() const { return pin_states_; }
     // Launch up to `steps` steps without waiting; returns live count seen at
     // the last completed poll via live_out (non-blocking).
     int pump(int steps);
@@ -304,6 +313,7 @@ class Engine {
     int32_t *pin_slots_ = nullptr;
     int32_t *d_slots_ = nullptr;
     Ctl *pin_ctl_ = nullptr;
+    InstState *pin_states_ = nullptr;  // cap_slots_ states (fetch_states_pinned)
     int32_t *d_strokes_ = nullptr;
     int strokes_cap_ = 0;
     // Captured step chunks, one per chunk length (even: every chunk starts at

@@ -109,8 +109,18 @@ __device__ __forceinline__ void mark_dirty(const KParams &p, int t) { p.dirty[t]
 
 // ---------------------------------------------------------------------------
 // Arm: reset the per-instance state and mark it live.
-__global__ void k_arm(KParams p, const int32_t *slots, int n, int warm, int sweeps, int start_phase) {
+__device__ void clear_tile(const KParams &p, const int32_t *slots, const int32_t *prefix, int n, int b);
+
+// Arms the listed instances (thread i: slots[i]), clears the iteration
+// counters and, before a wavefront build, the first `clear` tile records of
+// the listed instances (build-list positions, prefix = tile offsets).
+__global__ void k_arm(KParams p, const int32_t *slots, int n, int warm, int sweeps, int start_phase, int clear) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        p.ctl->pr_iters = 0;
+        p.ctl->gr_iters = 0;
+    }
+    for (int b = i; b < clear; b += gridDim.x * blockDim.x) clear_tile(p, slots, slots + n, n, b);
     if (i >= n) return;
     const int s = slots[i];
     InstState &st = p.state[s];
@@ -500,11 +510,6 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
         st.pr_iter = pr_iter + 1;
         st.pr_tiles += n;
     }
-}
-
-__global__ void k_reset_stats(KParams p) {
-    p.ctl->pr_iters = 0;
-    p.ctl->gr_iters = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -2042,30 +2047,29 @@ __device__ __forceinline__ int find_inst(const int32_t *prefix, int n, int b) {
 }
 
 // Tile table and cleared flags of the listed instances (one thread per tile).
-__global__ void k_clear_tiles(KParams p, const int32_t *slots, const int32_t *prefix, int n, int total) {
-    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < total; b += gridDim.x * blockDim.x) {
-        const int j = find_inst(prefix, n, b);
-        const int s = slots[j];
-        const InstDesc D = p.desc[s];
-        const int lt = b - prefix[j];
-        const int t = D.tile0 + lt;
-        const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
-        TileInfo ti;
-        ti.inst = s;
-        ti.tx = tx;
-        ti.ty = ty;
-        ti.nbr[0] = tx > 0 ? t - 1 : -1;
-        ti.nbr[1] = tx < D.tiles_x - 1 ? t + 1 : -1;
-        ti.nbr[2] = ty > 0 ? t - D.tiles_x : -1;
-        ti.nbr[3] = ty < D.tiles_y - 1 ? t + D.tiles_x : -1;
-        ti.pad = 0;
-        p.tiles[t] = ti;
-        p.act_flag[t] = 0;
-        p.dirty[t] = 0;
-        p.live[t] = 0;  // raised by the build sweep of the block's tiles
-        p.stale[t] = 0;
-        p.bmap[b] = t;
-    }
+// Tile record, map entry and flags of build-list position b.
+__device__ void clear_tile(const KParams &p, const int32_t *slots, const int32_t *prefix, int n, int b) {
+    const int j = find_inst(prefix, n, b);
+    const int s = slots[j];
+    const InstDesc D = p.desc[s];
+    const int lt = b - prefix[j];
+    const int t = D.tile0 + lt;
+    const int tx = lt % D.tiles_x, ty = lt / D.tiles_x;
+    TileInfo ti;
+    ti.inst = s;
+    ti.tx = tx;
+    ti.ty = ty;
+    ti.nbr[0] = tx > 0 ? t - 1 : -1;
+    ti.nbr[1] = tx < D.tiles_x - 1 ? t + 1 : -1;
+    ti.nbr[2] = ty > 0 ? t - D.tiles_x : -1;
+    ti.nbr[3] = ty < D.tiles_y - 1 ? t + D.tiles_x : -1;
+    ti.pad = 0;
+    p.tiles[t] = ti;
+    p.act_flag[t] = 0;
+    p.dirty[t] = 0;
+    p.live[t] = 0;  // raised by the build sweep of the block's tiles
+    p.stale[t] = 0;
+    p.bmap[b] = t;
 }
 
 
@@ -2099,7 +2103,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
         const int b = b0 + jj;
         if (b >= total) break;
         // Tile, instance and tile coordinates from the map and the tile
-        // records k_clear_tiles wrote just before (two dependent loads).
+        // records k_arm wrote just before (two dependent loads).
         const int t = rbmap[b];
         const TileInfo ti = rtiles[t];
         const int s = ti.inst;
@@ -2423,11 +2427,14 @@ __global__ void k_read_state(KParams p, int slot, int32_t *caps_out, int32_t *ex
 }
 
 // ---------------------------------------------------------------------------
-void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int sweeps, hipStream_t s) {
+void launch_arm(const KParams &p, const int32_t *d_slots, int n, int warm, int sweeps, hipStream_t s,
+                int clear_tiles) {
     if (n <= 0) return;
     // The wavefront build performs the first global-relabel sweep itself.
     const int start = (!warm && p.variant) ? PH_GR : PH_START;
-    hipLaunchKernelGGL(k_arm, dim3((n + 255) / 256), dim3(256), 0, s, p, d_slots, n, warm, sweeps, start);
+    const int work = clear_tiles > n ? clear_tiles : n;
+    const int grid = (work + 255) / 256 < 65536 ? (work + 255) / 256 : 65536;
+    hipLaunchKernelGGL(k_arm, dim3(grid), dim3(256), 0, s, p, d_slots, n, warm, sweeps, start, clear_tiles);
 }
 
 template <typename F, typename... Args>
@@ -2437,8 +2444,6 @@ void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_til
                   const LaunchEvents &ev) {
     if (n <= 0 || total_tiles <= 0) return;
     if (p.variant) {
-        hipLaunchKernelGGL(k_clear_tiles, dim3((total_tiles + 255) / 256), dim3(256), 0, s, p, d_slots, d_slots + n, n,
-                           total_tiles);
         int grid = total_tiles < 65536 ? total_tiles : 65536;
         if (grid >= NUM_XCD) grid -= grid % NUM_XCD;  // whole rounds over the XCDs
         launch_timed(k_buildw, dim3(grid), dim3(64), s, ev, p, (const int32_t *)p.bmap, (const TileInfo *)p.tiles,
@@ -2514,7 +2519,6 @@ void launch_step(const KParams &p, int grid_pr, int grid_gr, int parity, hipStre
     launch_pr(p, grid_pr, parity, s, LaunchEvents{});
     launch_gr(p, grid_gr, parity, s, LaunchEvents{});
 }
-void launch_reset_stats(const KParams &p, hipStream_t s) { hipLaunchKernelGGL(k_reset_stats, dim3(1), dim3(1), 0, s, p); }
 
 
 void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,
