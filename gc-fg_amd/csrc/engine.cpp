@@ -296,6 +296,7 @@ void Engine::remove_instance(int slot) {
     free_tiles(h_desc_[slot].tile0, h_desc_[slot].ntiles);
     slot_used_[slot] = 0;
     h_desc_[slot] = InstDesc{};
+    update_sched_width();
 }
 
 int Engine::build(const std::vector<int> &slots) {
@@ -319,6 +320,7 @@ int Engine::build(const std::vector<int> &slots) {
     int total = 0;
     for (int i = 0; i < n; ++i) total += h_desc_[slots[i]].ntiles;
     apply_options();
+    update_sched_width();
     // Lazy capacities pay when the build's 16 B/node of capacity writes (about
     // 3.7 ns per tile at the build's bandwidth) outweigh one more launch per
     // step (about 4 us): judged by the previous solve's step count, or without
@@ -351,6 +353,15 @@ int Engine::build(const std::vector<int> &slots) {
     launch_build(kp_, d_slots_, n, total, stream_, ev);
     GCFG_HIP(hipGetLastError());
     return GCFG_OK;
+}
+
+void Engine::update_sched_width() {
+    int wide = 0;
+    for (int i = 0; i < cap_slots_; ++i) wide |= slot_used_[i] && h_desc_[i].ntiles > 2048;
+    if (wide != kp_.sched_wide) {
+        kp_.sched_wide = wide;
+        drop_graphs();  // the captured steps carry the launch shape
+    }
 }
 
 bool Engine::any_lazy() const {

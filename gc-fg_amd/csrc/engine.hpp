@@ -140,6 +140,9 @@ struct KParams {
     // [2 TH, 2 TH + TW) row 0 up arcs, [2 TH + TW, INFLOW) row TH-1 down arcs.
     int32_t *mat;
     int32_t *inflow;
+    // 1: some instance has more than 2048 tiles, so the scheduler runs
+    // 1024-thread workgroups (host-side launch choice).
+    int32_t sched_wide;
     int32_t n_slots;
     int32_t inner_iters;
     int32_t variant;  // 0: workgroup-per-tile kernels (LDS), 1: wavefront-per-tile kernels (registers)
@@ -302,6 +305,7 @@ class Engine {
     int cu_count_ = 0;
     int nbhd_ = 4;
     bool any_lazy() const;    // some live instance still has capacities only in its inputs
+    void update_sched_width();  // kp_.sched_wide from the live instances (drops stale step graphs)
     hipStream_t stream_ = nullptr;
     KParams kp_{};
     int cap_tiles_ = 0, cap_slots_ = 0;

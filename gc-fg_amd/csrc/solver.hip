@@ -76,6 +76,7 @@ __device__ __forceinline__ int64_t wave_sum64(int64_t v) {
 }
 
 // Block (NT threads) sum; result valid in every thread. Uses red[NT/64].
+template <int NTHR = NT>
 __device__ __forceinline__ int64_t block_sum64(int64_t v, int64_t *red) {
     v = wave_sum64(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -84,7 +85,7 @@ __device__ __forceinline__ int64_t block_sum64(int64_t v, int64_t *red) {
     __syncthreads();
     int64_t s = 0;
 #pragma unroll
-    for (int w = 0; w < NT / 64; ++w) s += red[w];
+    for (int w = 0; w < NTHR / 64; ++w) s += red[w];
     return s;
 }
 
@@ -253,14 +254,19 @@ enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2, EM_LIVE = 3, EM_D
 // (identified by their anchor tile, whose flags stand for the block)
 // selected by `kind` to a queue (ballot compaction, one atomic per 256
 // entries); returns the number appended.
+// The scheduler runs one workgroup of SNT threads per instance slot, a pass
+// of emit_tiles covering SNT * EMIT_U tiles with one queue reservation:
+// 256 threads (2 k tiles a pass) while every instance has at most 2 k tiles,
+// 1024 (8 k a pass: a 4096^2 image in two) otherwise.
 constexpr int EMIT_U = 8;  // units per thread in one pass of emit_tiles
 // Push-queue item flag: the tile's capacities are still only in the inputs
 // (lazy build); the push pass reads them from there and writes them out.
 constexpr int PR_FRESH = 1 << 30;
 
+template <int SNT>
 __device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int colour, bool to_pr,
                                           int grop, int parity) {
-    __shared__ int wcount[NT / 64], wbase[NT / 64], s_qbase, s_total;
+    __shared__ int wcount[SNT / 64], wbase[SNT / 64], s_qbase, s_total;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int32_t *queue = to_pr ? p.queue_pr : p.queue_gr;
     int32_t *qlen = to_pr ? &p.ctl->q[parity].qlen_pr : &p.ctl->q[parity].qlen_gr;
@@ -279,11 +285,11 @@ __device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, i
     __syncthreads();  // earlier readers of s_total are done
     if (threadIdx.x == 0) s_total = 0;
     if (nunits == 0) __syncthreads();
-    // Passes of NT * EMIT_U units: wave w takes the contiguous units
+    // Passes of SNT * EMIT_U units: wave w takes the contiguous units
     // [base + w * 64 * EMIT_U, + 64 * EMIT_U), so the queue keeps the units in
     // order, and one queue reservation covers the pass. Every flag is loaded
     // before any is cleared, so the loads of a pass are in flight together.
-    for (int base = 0; base < nunits; base += NT * EMIT_U) {
+    for (int base = 0; base < nunits; base += SNT * EMIT_U) {
         int lt[EMIT_U], v0[EMIT_U], v1[EMIT_U];
 #pragma unroll
         for (int k = 0; k < EMIT_U; ++k) {
@@ -347,7 +353,7 @@ __device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, i
         __syncthreads();
         if (threadIdx.x == 0) {
             int tot = 0;
-            for (int w = 0; w < NT / 64; ++w) {
+            for (int w = 0; w < SNT / 64; ++w) {
                 wbase[w] = tot;
                 tot += wcount[w];
             }
@@ -371,10 +377,11 @@ __device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, i
     return s_total;
 }
 
+template <int SNT>
 __device__ __forceinline__ bool any_active_tile(const KParams &p, const InstDesc &D) {
     int any = 0;
 #pragma unroll 8
-    for (int lt = threadIdx.x; lt < D.ntiles; lt += NT) any |= p.act_flag[D.tile0 + lt];
+    for (int lt = threadIdx.x; lt < D.ntiles; lt += SNT) any |= p.act_flag[D.tile0 + lt];
     return __syncthreads_or(any) != 0;
 }
 
@@ -393,6 +400,7 @@ __device__ __forceinline__ bool any_active_tile(const KParams &p, const InstDesc
 // active) until the next relabel. Only an exact relabel (cap = DINF) can end
 // the solve, so a capped round that leaves no active node is followed by an
 // exact one.
+template <int SNT>
 __device__ __forceinline__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st, int parity,
                                                      bool reinit, int cap, bool all = false) {
     if (threadIdx.x == 0) {
@@ -408,9 +416,9 @@ __device__ __forceinline__ void start_global_relabel(const KParams &p, const Ins
     // node), and for a block farther than K from every change each shortest
     // path of length < K is unchanged, so min(d, K) is unchanged too.
     const bool whole = cap >= DINF || cap == CAP_FLOOD;  // exact distances or reachability: every live block
-    const int n = !reinit ? emit_tiles(p, D, EM_ALL, 0, false, OP_GR_INIT, parity)
-                  : all   ? emit_tiles(p, D, EM_ALL, 0, false, OP_GR_REINIT, parity)
-                          : emit_tiles(p, D, EM_LIVE, cap == CAP_FLOOD ? 2 : (whole ? 1 : 0), false, OP_GR_REINIT,
+    const int n = !reinit ? emit_tiles<SNT>(p, D, EM_ALL, 0, false, OP_GR_INIT, parity)
+                  : all   ? emit_tiles<SNT>(p, D, EM_ALL, 0, false, OP_GR_REINIT, parity)
+                          : emit_tiles<SNT>(p, D, EM_LIVE, cap == CAP_FLOOD ? 2 : (whole ? 1 : 0), false, OP_GR_REINIT,
                                        parity);
     if (threadIdx.x == 0) {
         st.gr_tiles += n;
@@ -419,7 +427,8 @@ __device__ __forceinline__ void start_global_relabel(const KParams &p, const Ins
     }
 }
 
-__global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
+template <int SNT>
+__global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     const int s = blockIdx.x;
     if (s == 0 && threadIdx.x == 0) p.ctl->q[parity ^ 1] = QCtl{0, 0, 0, 0};
     InstState &st = p.state[s];
@@ -433,14 +442,14 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
     if ((phase == PH_START && err) || phase == PH_FIN) {
         // flow = sum of pos - sum of max(excess, 0) over the interior, from
         // the per-tile sums of the build and the FIN sweep.
-        __shared__ int64_t red[NT / 64];
+        __shared__ int64_t red[SNT / 64];
         int64_t v = 0;
         if (phase == PH_FIN)
-            for (int lt = threadIdx.x; lt < D.ntiles; lt += NT) {
+            for (int lt = threadIdx.x; lt < D.ntiles; lt += SNT) {
                 const size_t t = (size_t)(D.tile0 + lt);
                 v += p.tsum[2 * t] - p.tsum[2 * t + 1];
             }
-        const int64_t tot = block_sum64(v, red);
+        const int64_t tot = block_sum64<SNT>(v, red);
         if (threadIdx.x == 0) {
             if (phase == PH_FIN) st.flow = st.sum_pos + tot;
             st.phase = PH_DONE;
@@ -453,11 +462,11 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
     if (phase == PH_START) {
         // A warm start relabels every tile: an edit may have revived tiles
         // the kept flags call unreachable (edits keep the masks current).
-        start_global_relabel(p, D, st, parity, reinit, cap, true);
+        start_global_relabel<SNT>(p, D, st, parity, reinit, cap, true);
         return;
     }
     if (phase == PH_GR) {
-        const int n = emit_tiles(p, D, EM_DIRTY, cur_cap == CAP_FLOOD ? 1 : 0, false, OP_GR_RELAX, parity);
+        const int n = emit_tiles<SNT>(p, D, EM_DIRTY, cur_cap == CAP_FLOOD ? 1 : 0, false, OP_GR_RELAX, parity);
         if (n > 0) {
             if (threadIdx.x == 0) {
                 st.gr_tiles += n;
@@ -467,19 +476,19 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
         }
         // The BFS has converged: finish, or push from the active tiles.
         const bool flood = cur_cap == CAP_FLOOD;
-        if (!any_active_tile(p, D)) {
+        if (!any_active_tile<SNT>(p, D)) {
             if (cur_cap < DINF && !flood) {
                 // Nothing active below the cap: check reachability before
                 // finishing (a flood settles in far fewer tile visits than
                 // exact distances; it also writes the labels).
-                start_global_relabel(p, D, st, parity, reinit, CAP_FLOOD);
+                start_global_relabel<SNT>(p, D, st, parity, reinit, CAP_FLOOD);
                 return;
             }
             if (threadIdx.x == 0) st.phase = PH_FIN;
             // Wavefront kernels: the exact or reachability round wrote the
             // labels of every block it visited; FIN labels the blocks it
             // skipped (unreachable, or reaching a sink everywhere).
-            const int nf = emit_tiles(p, D, p.variant ? EM_DEAD : EM_ALL, 0, false, OP_FIN, parity);
+            const int nf = emit_tiles<SNT>(p, D, p.variant ? EM_DEAD : EM_ALL, 0, false, OP_FIN, parity);
             if (threadIdx.x == 0) st.fin_items += nf;
             return;
         }
@@ -487,7 +496,7 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
             // An excess node can still reach a sink. Flood values are not
             // valid heights: relabel exactly before pushing (a capped round
             // could freeze the same nodes again).
-            start_global_relabel(p, D, st, parity, reinit, DINF);
+            start_global_relabel<SNT>(p, D, st, parity, reinit, DINF);
             return;
         }
         if (threadIdx.x == 0) st.phase = PH_PR;
@@ -496,14 +505,15 @@ __global__ __launch_bounds__(NT) void k_sched(KParams p, int parity) {
     // PH_PR: cycle through the tile colours, then relabel globally again.
     const int ncol = p.nbhd == 8 ? 4 : 2;
     if (pr_iter >= ncol * sweeps) {
-        start_global_relabel(p, D, st, parity, reinit, cap);
+        start_global_relabel<SNT>(p, D, st, parity, reinit, cap);
         return;
     }
     const int colour = pr_iter & (ncol - 1);
     int n = 0;
-    for (int c = 0; c < ncol && n == 0; ++c) n = emit_tiles(p, D, EM_COLOUR, (colour + c) & (ncol - 1), true, 0, parity);
+    for (int c = 0; c < ncol && n == 0; ++c)
+        n = emit_tiles<SNT>(p, D, EM_COLOUR, (colour + c) & (ncol - 1), true, 0, parity);
     if (n == 0) {
-        start_global_relabel(p, D, st, parity, reinit, cap);
+        start_global_relabel<SNT>(p, D, st, parity, reinit, cap);
         return;
     }
     if (threadIdx.x == 0) {
@@ -2467,7 +2477,10 @@ static void launch_timed(F kernel, dim3 grid, dim3 block, hipStream_t s, const L
 }
 
 void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvents &ev) {
-    launch_timed(k_sched, dim3(p.n_slots), dim3(NT), s, ev, p, parity);
+    if (p.sched_wide)
+        launch_timed(k_sched<1024>, dim3(p.n_slots), dim3(1024), s, ev, p, parity);
+    else
+        launch_timed(k_sched<256>, dim3(p.n_slots), dim3(256), s, ev, p, parity);
 }
 
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
