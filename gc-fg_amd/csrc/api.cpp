@@ -78,8 +78,8 @@ int gcfg_device_info(char *name, size_t name_len, int32_t *cu_count, size_t *hbm
 
 // ---------------------------------------------------------------------------
 static int graph_create(int32_t width, int32_t height, int nbhd, gcfg_graph **out) {
-    if (!out || width <= 0 || height <= 0) {
-        set_error("gcfg_graph_create: invalid shape or output pointer");
+    if (!out || !Engine::shape_supported(width, height)) {
+        set_error("gcfg_graph_create: invalid shape (positive, at most 2^30 pixels) or output pointer");
         return GCFG_ERR_INVALID;
     }
     *out = nullptr;
@@ -352,20 +352,26 @@ int gcfg_batch_create(int32_t count, const gcfg_instance *inst, const gcfg_optio
     gcfg_batch *b = new (std::nothrow) gcfg_batch();
     if (!b) return GCFG_ERR_NOMEM;
     b->inst.assign(inst, inst + count);
-    int tiles = 0;
+    int64_t tiles = 0;
     size_t labels = 0;
     for (int i = 0; i < count; ++i) {
         const gcfg_instance &in = inst[i];
-        if (in.width <= 0 || in.height <= 0 || !in.d_pos || !in.d_neg) {
-            set_error("gcfg_batch_create: instance " + std::to_string(i) + " has no shape or data terms");
+        if (!Engine::shape_supported(in.width, in.height) || !in.d_pos || !in.d_neg) {
+            set_error("gcfg_batch_create: instance " + std::to_string(i) +
+                      " has no data terms or an invalid shape (positive, at most 2^30 pixels)");
             delete b;
             return GCFG_ERR_INVALID;
         }
         tiles += b->eng.tiles_for(in.width, in.height);
+        if (tiles >= (int64_t)1 << 28) {
+            set_error("gcfg_batch_create: more than 2^28 tiles in one batch");
+            delete b;
+            return GCFG_ERR_INVALID;
+        }
         b->label_off.push_back(labels);
         labels += (size_t)in.width * in.height;
     }
-    int rc = b->eng.init(tiles, count, o);
+    int rc = b->eng.init((int)tiles, count, o);
     if (rc == GCFG_OK && hipMalloc((void **)&b->d_labels, labels * sizeof(int32_t)) != hipSuccess) {
         set_error("gcfg_batch_create: label arena allocation failed");
         rc = GCFG_ERR_NOMEM;

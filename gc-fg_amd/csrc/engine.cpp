@@ -81,6 +81,13 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o, int nbhd) 
     kp_.nbhd = nbhd;
     cap_tiles = std::max(cap_tiles, 1);
     cap_slots = std::max(cap_slots, 1);
+    // Relabel queue items hold a tile id shifted by 3 bits (an op in the low
+    // bits), push items a flag in bit 30: tile ids stay below 2^28 (275 G
+    // nodes, far beyond one GPU's memory).
+    if (cap_tiles >= (1 << 28)) {
+        set_error("arena of more than 2^28 tiles");
+        return GCFG_ERR_INVALID;
+    }
     cap_tiles_ = cap_tiles;
     cap_slots_ = cap_slots;
     GCFG_HIP(hipGetDevice(&dev_));
@@ -237,8 +244,8 @@ void Engine::free_tiles(int start, int n) {
 int Engine::add_instance(int W, int H, const int32_t *pos, const int32_t *neg, const int32_t *up,
                          const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
                          int32_t *label_out) {
-    if (W <= 0 || H <= 0) {
-        set_error("instance shape must be positive");
+    if (!shape_supported(W, H)) {
+        set_error("instance shape must be positive and at most 2^30 pixels");
         return GCFG_ERR_INVALID;
     }
     int slot = -1;

@@ -274,6 +274,9 @@ class Engine {
     size_t tiles_free() const;
     int cap_tiles() const { return cap_tiles_; }
     int tiles_for(int W, int H) const { return ((W + TW - 1) / TW) * ((H + TH - 1) / TH); }
+    // Instances hold at most 2^30 pixels (int32 node counts and 32-bit tile
+    // arithmetic throughout).
+    static bool shape_supported(int W, int H) { return W > 0 && H > 0 && (int64_t)W * H <= (int64_t)1 << 30; }
     static size_t bytes_per_tile(int nbhd = 4) {
         return (size_t)TN * (16 + 4 + 4 + 1) + 100 + INFLOW * 4 + (nbhd == 8 ? (size_t)TN * (16 + 1) + 16 : 0);
     }

@@ -306,8 +306,8 @@ int gcfg_server_create(size_t hbm_budget_bytes, const gcfg_options *opt, gcfg_se
 
 int gcfg_server_submit_image(gcfg_server *s, int32_t width, int32_t height, const uint8_t *h_rgb,
                              const uint8_t *h_seeds, int32_t *h_label, int64_t *request_id, int32_t *session) {
-    if (!s || width <= 0 || height <= 0 || !h_rgb || !h_seeds) {
-        set_error("gcfg_server_submit_image: invalid arguments");
+    if (!s || !Engine::shape_supported(width, height) || !h_rgb || !h_seeds) {
+        set_error("gcfg_server_submit_image: invalid arguments (shape positive, at most 2^30 pixels)");
         return GCFG_ERR_INVALID;
     }
     Request *r = new (std::nothrow) Request();

@@ -461,3 +461,11 @@ def test_lazy_capacities(g, ora, shape, lazy):
         caps, exc = gr.read_state()
         assert ora.certify(W, H, pos, neg, caps, exc, label, st.flow, edges=E) == 0, rep
     gr.close()
+
+
+def test_oversize_instance_rejected(g):
+    """Instances hold at most 2^30 pixels (int32 node counts, 32-bit tile
+    arithmetic); the shape is checked before anything is allocated."""
+    with pytest.raises(g.GcfgError) as e:
+        g.Graph(32769, 32768)
+    assert e.value.rc == g.GCFG_ERR_INVALID
