@@ -1802,6 +1802,7 @@ __global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCF
     if (p.profile && lane == 0) atomic_add64(&p.ctl->gr_iters, n_rounds);
 }
 
+template <bool ONE>
 __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS) {
     const int qlen = p.ctl->q[parity].qlen_pr;
     if ((int)blockIdx.x >= qlen) return;
@@ -1870,102 +1871,125 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             if (fb) f[2 * TH + TW + lane] = 0;
         }
         int accU = 0, accD = 0;
-        int it = 0;
-        while (it < p.inner_iters) {
-            ++it;
-            // Push along admissible arcs (d(u) == d(v) + 1) in the order l, r,
-            // u, d. Heights are those at the start of the iteration; flow
-            // received from a lower row can be pushed on in the same sweep.
-#pragma unroll
-            for (int y = 0; y < TH; ++y) {
-                int hl = shr1(d[y]), hr = shl1(d[y]);
-                if (lane == 0) hl = __builtin_amdgcn_readlane(hs, y);
-                if (lane == TW - 1) hr = __builtin_amdgcn_readlane(hs, TH + y);
-                const int hu = y ? d[y - 1] : hU;
-                const int hd = y < TH - 1 ? d[y + 1] : hD;
-                int fl = 0, fr = 0, fu = 0, fd = 0;
-                int ek = e[y];
-                int4 ck = c[y];
-                if (ek > 0 && d[y] < capv) {
-                    const int tgt = d[y] - 1;
-                    if (ck.x > 0 && hl == tgt) {
-                        fl = min(ek, ck.x);
-                        ek -= fl;
-                        ck.x -= fl;
-                    }
-                    if (ek > 0 && ck.y > 0 && hr == tgt) {
-                        fr = min(ek, ck.y);
-                        ek -= fr;
-                        ck.y -= fr;
-                    }
-                    if (ek > 0 && ck.z > 0 && hu == tgt) {
-                        fu = min(ek, ck.z);
-                        ek -= fu;
-                        ck.z -= fu;
-                    }
-                    if (ek > 0 && ck.w > 0 && hd == tgt) {
-                        fd = min(ek, ck.w);
-                        ek -= fd;
-                        ck.w -= fd;
-                    }
+        // Push along admissible arcs (d(u) == d(v) + 1) in the order l, r, u,
+        // d. Heights are those at the start of the iteration; flow received
+        // from a lower row can be pushed on in the same sweep.
+        auto push_row = [&](const int y) {
+            int hl = shr1(d[y]), hr = shl1(d[y]);
+            if (lane == 0) hl = __builtin_amdgcn_readlane(hs, y);
+            if (lane == TW - 1) hr = __builtin_amdgcn_readlane(hs, TH + y);
+            const int hu = y ? d[y - 1] : hU;
+            const int hd = y < TH - 1 ? d[y + 1] : hD;
+            int fl = 0, fr = 0, fu = 0, fd = 0;
+            int ek = e[y];
+            int4 ck = c[y];
+            if (ek > 0 && d[y] < capv) {
+                const int tgt = d[y] - 1;
+                if (ck.x > 0 && hl == tgt) {
+                    fl = min(ek, ck.x);
+                    ek -= fl;
+                    ck.x -= fl;
                 }
-                const int inL = shr1(fr), inR = shl1(fl);
-                ck.x += inL;
-                ck.y += inR;
-                c[y] = ck;
-                e[y] = ek + inL + inR;
-                const int sfl = __builtin_amdgcn_readlane(fl, 0), sfr = __builtin_amdgcn_readlane(fr, TW - 1);
-                accS += lane == y ? sfl : (lane == TH + y ? sfr : 0);
-                if (y > 0) {
-                    e[y - 1] += fu;
-                    c[y - 1].w += fu;
-                } else {
-                    accU += fu;
+                if (ek > 0 && ck.y > 0 && hr == tgt) {
+                    fr = min(ek, ck.y);
+                    ek -= fr;
+                    ck.y -= fr;
                 }
-                if (y < TH - 1) {
-                    e[y + 1] += fd;
-                    c[y + 1].z += fd;
-                } else {
-                    accD += fd;
+                if (ek > 0 && ck.z > 0 && hu == tgt) {
+                    fu = min(ek, ck.z);
+                    ek -= fu;
+                    ck.z -= fu;
+                }
+                if (ek > 0 && ck.w > 0 && hd == tgt) {
+                    fd = min(ek, ck.w);
+                    ek -= fd;
+                    ck.w -= fd;
                 }
             }
-            // Relabel nodes left with excess: d = 1 + min{d(v) : r(u,v) > 0}.
-            int act = 0;
-#pragma unroll
-            for (int y = 0; y < TH; ++y) {
-                int hl = shr1(d[y]), hr = shl1(d[y]);
-                if (lane == 0) hl = __builtin_amdgcn_readlane(hs, y);
-                if (lane == TW - 1) hr = __builtin_amdgcn_readlane(hs, TH + y);
-                const int hu = y ? d[y - 1] : hU;
-                const int hd = y < TH - 1 ? d[y + 1] : hD;
-                if (e[y] > 0 && d[y] < capv) {
-                    int mm = DINF;
-                    if (c[y].x > 0) mm = min(mm, hl);
-                    if (c[y].y > 0) mm = min(mm, hr);
-                    if (c[y].z > 0) mm = min(mm, hu);
-                    if (c[y].w > 0) mm = min(mm, hd);
-                    int nd = mm >= DINF ? DINF : mm + 1;
-                    // A label of n or more proves the node cannot reach the
-                    // sink; a label past the cap freezes the node at the cap
-                    // (still a valid labelling: cap < nd <= d(v) + 1).
-                    if (nd >= dthr) nd = DINF;
-                    else if (nd > capv) nd = capv;
-                    if (nd > d[y]) d[y] = nd;
-                    act |= d[y] < capv;
-                }
+            const int inL = shr1(fr), inR = shl1(fl);
+            ck.x += inL;
+            ck.y += inR;
+            c[y] = ck;
+            e[y] = ek + inL + inR;
+            const int sfl = __builtin_amdgcn_readlane(fl, 0), sfr = __builtin_amdgcn_readlane(fr, TW - 1);
+            accS += lane == y ? sfl : (lane == TH + y ? sfr : 0);
+            if (y > 0) {
+                e[y - 1] += fu;
+                c[y - 1].w += fu;
+            } else {
+                accU += fu;
             }
-            if (!__any(act)) break;
-        }
+            if (y < TH - 1) {
+                e[y + 1] += fd;
+                c[y + 1].z += fd;
+            } else {
+                accD += fd;
+            }
+        };
+        // Relabel a node left with excess: d = 1 + min{d(v) : r(u,v) > 0}.
+        // Rows are relabelled in order, each with the new height of the row
+        // above and the old one of the row below. Returns whether the node
+        // is still active (below the cap).
+        auto relabel_row = [&](const int y) -> int {
+            int hl = shr1(d[y]), hr = shl1(d[y]);
+            if (lane == 0) hl = __builtin_amdgcn_readlane(hs, y);
+            if (lane == TW - 1) hr = __builtin_amdgcn_readlane(hs, TH + y);
+            const int hu = y ? d[y - 1] : hU;
+            const int hd = y < TH - 1 ? d[y + 1] : hD;
+            if (!(e[y] > 0 && d[y] < capv)) return 0;
+            int mm = DINF;
+            if (c[y].x > 0) mm = min(mm, hl);
+            if (c[y].y > 0) mm = min(mm, hr);
+            if (c[y].z > 0) mm = min(mm, hu);
+            if (c[y].w > 0) mm = min(mm, hd);
+            int nd = mm >= DINF ? DINF : mm + 1;
+            // A label of n or more proves the node cannot reach the sink; a
+            // label past the cap freezes the node at the cap (still a valid
+            // labelling: cap < nd <= d(v) + 1).
+            if (nd >= dthr) nd = DINF;
+            else if (nd > capv) nd = capv;
+            if (nd > d[y]) d[y] = nd;
+            return d[y] < capv;
+        };
         int mine = 0;
         int64_t sx = 0;  // sum of max(e, 0) over the tile (the flow identity's stuck excess)
-#pragma unroll
-        for (int y = 0; y < TH; ++y) {
+        auto store_row = [&](const int y) {
             T.c[y * TW + lane] = c[y];
             T.e[y * TW + lane] = e[y];
             T.d[y * TW + lane] = d[y];
             T.m[y * TW + lane] = (uint8_t)node_mask(c[y], e[y]);
             mine |= e[y] > 0 && d[y] < capv;
             sx += max(e[y], 0);
+        };
+        int it = 0;
+        if (ONE) {
+            // One iteration: row y - 1 is final once row y has pushed (only
+            // rows y - 2 .. y push into it), so it is relabelled and stored
+            // right then, with exactly the values of the general order, and
+            // its registers are free for the rows still to come.
+            it = 1;
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                push_row(y);
+                if (y > 0) {
+                    relabel_row(y - 1);
+                    store_row(y - 1);
+                }
+            }
+            relabel_row(TH - 1);
+            store_row(TH - 1);
+        } else {
+            while (it < p.inner_iters) {
+                ++it;
+#pragma unroll
+                for (int y = 0; y < TH; ++y) push_row(y);
+                int act = 0;
+#pragma unroll
+                for (int y = 0; y < TH; ++y) act |= relabel_row(y);
+                if (!__any(act)) break;
+            }
+#pragma unroll
+            for (int y = 0; y < TH; ++y) store_row(y);
         }
         // Flow pushed into halo nodes (their tiles do not run in this colour
         // pass; int32 atomics because two same-colour tiles can feed one
@@ -2484,8 +2508,11 @@ void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvent
 }
 
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
-    if (p.variant)
-        launch_timed(k_prw, dim3(grid), dim3(64), s, ev, p, parity, (const int32_t *)p.queue_pr,
+    if (p.variant && p.inner_iters == 1)
+        launch_timed(k_prw<true>, dim3(grid), dim3(64), s, ev, p, parity, (const int32_t *)p.queue_pr,
+                     (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
+    else if (p.variant)
+        launch_timed(k_prw<false>, dim3(grid), dim3(64), s, ev, p, parity, (const int32_t *)p.queue_pr,
                      (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
     else if (p.nbhd == 8)
         launch_timed(k_pr8, dim3(grid), dim3(NT), s, ev, p, parity);
