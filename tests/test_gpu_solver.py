@@ -438,18 +438,21 @@ def test_middlebury_file_solve(g, ora, tmp_path):
     assert st.steps > 0
 
 
+@pytest.mark.parametrize("inner", [1, 4])
 @pytest.mark.parametrize("lazy", [1, 2])
 @pytest.mark.parametrize("shape", [(5, 4), (130, 33), (257, 140)])
-def test_lazy_capacities(g, ora, shape, lazy):
-    """Capacities written by the build (1) or before each tile's first push
-    pass, with neighbours' pushes held in per-tile inflow slots (2): exact
-    results, and the residual state read back (inputs + inflow for tiles never
-    pushed) is a valid optimality certificate."""
+def test_lazy_capacities(g, ora, shape, lazy, inner):
+    """Capacities written by the build (1) or by each tile's first push pass,
+    read there from the inputs, with neighbours' pushes held in per-tile
+    inflow slots (2), through the single-iteration push pass (inner 1) and the
+    general one: exact results, and the residual state read back (inputs +
+    inflow for tiles never pushed) is a valid optimality certificate."""
     W, H = shape
     rng = np.random.default_rng(W + 7 * H + lazy)
     pos, neg, E = random_instance(rng, W, H, tmax=120, emax=40)
     o = g.default_options()
     o.lazy_caps = lazy
+    o.inner_iters = inner
     gr = g.Graph(W, H, o)
     gr.set_dataterms(pos, neg)
     gr.set_edges(*E)
