@@ -1,6 +1,6 @@
 /*
- * CPU restatement of the reference push-relabel schedule -- TEST / BASELINE
- * INFRASTRUCTURE ONLY (timed as the CPU baseline; never linked into the
+ * CPU restatement of the reference push-relabel schedule -- TEST
+ * INFRASTRUCTURE ONLY (also timed as the CPU baseline; never linked into the
  * product).
  *
  * It follows GraphCut.cu:385-534 (GC_Optimize) driving the kernels of
