@@ -412,8 +412,8 @@ def pmc_traffic(args, opts, kernel_prefix, launches_per_solve):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--qps", type=float, default=40.0)
