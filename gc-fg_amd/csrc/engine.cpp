@@ -136,6 +136,7 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o, int nbhd) 
     GCFG_HIP(hipHostMalloc((void **)&pin_states_, (size_t)cap_slots * sizeof(InstState), hipHostMallocDefault));
     GCFG_HIP(hipMemsetAsync(kp_.state, 0, (size_t)cap_slots * sizeof(InstState), stream_));
     GCFG_HIP(hipMemsetAsync(kp_.desc, 0, (size_t)cap_slots * sizeof(InstDesc), stream_));
+    dev_desc_.assign(cap_slots, InstDesc{});
     GCFG_HIP(hipMemsetAsync(kp_.ctl, 0, sizeof(Ctl), stream_));
     GCFG_HIP(hipMemsetAsync(kp_.dirty, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
     GCFG_HIP(hipMemsetAsync(kp_.act_flag, 0, (size_t)cap_tiles * sizeof(int32_t), stream_));
@@ -328,27 +329,43 @@ int Engine::build(const std::vector<int> &slots) {
     for (int i = 0; i < n; ++i) total += h_desc_[slots[i]].ntiles;
     apply_options();
     update_sched_width();
-    // Lazy capacities pay when the build's 16 B/node of capacity writes (about
-    // 3.7 ns per tile at the build's bandwidth) outweigh one more launch per
-    // step (about 4 us): judged by the previous solve's step count, or without
-    // one by size alone (32 k tiles = 32 M nodes). Large batches with few
-    // steps qualify; single images and long solves do not. The descriptors
-    // carry the choice to the kernels.
+    // Lazy capacities pay off for large builds solved in few steps: the build
+    // skips 16 B/node of capacity writes (about 3.7 ns per tile at its
+    // bandwidth) while the tiles of the push passes read their capacities
+    // from the caller's planes instead of the arena. Taken as: at least 1024
+    // tiles per step the previous solve needed, or without one 32 k tiles
+    // (32 M nodes). Large batches qualify; single images and long solves do
+    // not. The descriptors carry the choice to the kernels.
     const int64_t need = steps_hint > 0 ? (int64_t)steps_hint * 1024 : 32768;
     const int lazy = kp_.variant && (opt.lazy_caps == 2 || (opt.lazy_caps == 0 && (int64_t)total >= need)) ? 1 : 0;
     for (int s : slots) h_desc_[s].lazy = lazy;
-    // The pinned staging buffers are reused; wait for earlier copies out of them.
-    GCFG_HIP(hipStreamSynchronize(stream_));
-    std::memcpy(pin_desc_, h_desc_.data(), (size_t)cap_slots_ * sizeof(InstDesc));
+    // Build list: slots, then each one's tile offset. The descriptor table and
+    // the list are uploaded only when they differ from what the device holds
+    // (a repeated solve of the same batch uploads nothing).
+    std::vector<int32_t> list(2 * n);
     for (int i = 0, acc = 0; i < n; ++i) {
-        pin_slots_[i] = slots[i];
-        pin_slots_[n + i] = acc;
+        list[i] = slots[i];
+        list[n + i] = acc;
         acc += h_desc_[slots[i]].ntiles;
     }
-    GCFG_HIP(hipMemcpyAsync(kp_.desc, pin_desc_, (size_t)cap_slots_ * sizeof(InstDesc), hipMemcpyHostToDevice,
-                            stream_));
-    GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)(2 * n) * sizeof(int32_t), hipMemcpyHostToDevice,
-                            stream_));
+    const size_t dbytes = (size_t)cap_slots_ * sizeof(InstDesc);
+    const bool desc_same = std::memcmp(dev_desc_.data(), h_desc_.data(), dbytes) == 0;
+    const bool list_same = dev_list_ == list;
+    if (!desc_same || !list_same) {
+        // The pinned staging buffers are reused; wait for earlier copies out of them.
+        GCFG_HIP(hipStreamSynchronize(stream_));
+        if (!desc_same) {
+            std::memcpy(pin_desc_, h_desc_.data(), dbytes);
+            GCFG_HIP(hipMemcpyAsync(kp_.desc, pin_desc_, dbytes, hipMemcpyHostToDevice, stream_));
+            dev_desc_ = h_desc_;
+        }
+        if (!list_same) {
+            std::memcpy(pin_slots_, list.data(), (size_t)(2 * n) * sizeof(int32_t));
+            GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)(2 * n) * sizeof(int32_t), hipMemcpyHostToDevice,
+                                    stream_));
+            dev_list_ = list;
+        }
+    }
     launch_arm(kp_, d_slots_, n, 0, opt.sweeps_per_gr, stream_, kp_.variant ? total : 0);
     LaunchEvents ev;
     if (opt.profile) {
@@ -395,6 +412,7 @@ int Engine::upload_desc() {
     std::memcpy(pin_desc_, h_desc_.data(), (size_t)cap_slots_ * sizeof(InstDesc));
     GCFG_HIP(hipMemcpyAsync(kp_.desc, pin_desc_, (size_t)cap_slots_ * sizeof(InstDesc), hipMemcpyHostToDevice,
                             stream_));
+    dev_desc_ = h_desc_;
     return GCFG_OK;
 }
 
@@ -404,6 +422,7 @@ int Engine::rearm(const std::vector<int> &slots) {
     const int n = (int)slots.size();
     for (int i = 0; i < n; ++i) pin_slots_[i] = slots[i];
     GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+    dev_list_.clear();  // the build list on the device is overwritten
     apply_options();
     launch_arm(kp_, d_slots_, n, 1, opt.sweeps_per_gr, stream_, 0);
     GCFG_HIP(hipGetLastError());

@@ -314,6 +314,8 @@ class Engine {
     int cap_tiles_ = 0, cap_slots_ = 0;
     int grid_pr_ = 0, grid_gr_ = 0;
     std::vector<InstDesc> h_desc_;
+    std::vector<InstDesc> dev_desc_;  // what the device descriptor table holds
+    std::vector<int32_t> dev_list_;   // what d_slots_ holds after a build (empty: unknown)
     std::vector<char> slot_used_;
     std::vector<std::pair<int, int>> free_ranges_;  // (start, count)
     InstDesc *pin_desc_ = nullptr;
