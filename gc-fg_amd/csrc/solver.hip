@@ -1651,8 +1651,11 @@ constexpr int GR_MAX_EXCHANGES = 64;
     const int32_t *__restrict__ rq, const TileInfo *__restrict__ rtiles, const InstDesc *__restrict__ rdesc, \
         const InstState *__restrict__ rstate
 
+// One-tile blocks fit 96 VGPRs without spilling: five wavefronts per SIMD
+// instead of four (the bigger blocks would spill, so they keep the default).
 template <int BX, int BY>
-__global__ __launch_bounds__(64 * BX * BY) void k_grb(KParams p, int parity, GCFG_RO_ARGS) {
+__global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX * BY == 1 ? 5 : 1))) void k_grb(
+    KParams p, int parity, GCFG_RO_ARGS) {
     constexpr int NW = BX * BY;
     __shared__ int s_top[NW][TW], s_bot[NW][TW], s_side[NW][2 * TH];
     const int qlen = p.ctl->q[parity].qlen_gr;
