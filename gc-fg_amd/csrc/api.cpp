@@ -100,7 +100,7 @@ static int graph_create(int32_t width, int32_t height, int nbhd, gcfg_graph **ou
         if (g->slot < 0) rc = g->slot;
     }
     if (rc != GCFG_OK) {
-        if (g->d_label) hipFree(g->d_label);
+        if (g->d_label) (void)hipFree(g->d_label);
         delete g;
         return rc;
     }
@@ -332,7 +332,7 @@ int gcfg_graph_read_distances(gcfg_graph *g, int32_t *h_dist) {
 int gcfg_graph_destroy(gcfg_graph *g) {
     if (!g) return GCFG_OK;
     g->eng.sync();
-    if (g->d_label) hipFree(g->d_label);
+    if (g->d_label) (void)hipFree(g->d_label);
     delete g;
     return GCFG_OK;
 }
@@ -384,7 +384,7 @@ int gcfg_batch_create(int32_t count, const gcfg_instance *inst, const gcfg_optio
         b->slots.push_back(s);
     }
     if (rc != GCFG_OK) {
-        if (b->d_labels) hipFree(b->d_labels);
+        if (b->d_labels) (void)hipFree(b->d_labels);
         delete b;
         return rc;
     }
@@ -476,7 +476,7 @@ int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_sta
 int gcfg_batch_destroy(gcfg_batch *b) {
     if (!b) return GCFG_OK;
     b->eng.sync();
-    if (b->d_labels) hipFree(b->d_labels);
+    if (b->d_labels) (void)hipFree(b->d_labels);
     delete b;
     return GCFG_OK;
 }

@@ -228,10 +228,11 @@ static int grid_for(int64_t n) {
 
 // ---------------------------------------------------------------------------
 DtContext::~DtContext() {
-    if (stream_) hipStreamSynchronize(stream_);
-    for (void *p : allocs_) hipFree(p);
-    if (h_stats_) hipHostFree(h_stats_);
-    if (h_lut_) hipHostFree(h_lut_);
+    // Teardown: errors here have nowhere to go.
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void *p : allocs_) (void)hipFree(p);
+    if (h_stats_) (void)hipHostFree(h_stats_);
+    if (h_lut_) (void)hipHostFree(h_lut_);
 }
 
 int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs) {
@@ -486,7 +487,7 @@ int DtContext::unquantized(double *h_tpos, double *h_tneg, double *h_right, doub
     for (int k = 0; k < 4 && e == hipSuccess; ++k)
         if (outs[k]) e = hipMemcpyAsync(outs[k], buf + k * n, n * sizeof(double), hipMemcpyDeviceToHost, stream_);
     if (e == hipSuccess) e = hipStreamSynchronize(stream_);
-    hipFree(buf);
+    (void)hipFree(buf);
     GCFG_HIP(e);
     return GCFG_OK;
 }
@@ -515,7 +516,7 @@ int gcfg_dt_create(int32_t width, int32_t height, gcfg_dataterm **out) {
     }
     int rc = dt->ctx.init(width, height, dt->stream, true);
     if (rc != GCFG_OK) {
-        hipStreamDestroy(dt->stream);
+        (void)hipStreamDestroy(dt->stream);
         delete dt;
         return rc;
     }
@@ -526,9 +527,9 @@ int gcfg_dt_create(int32_t width, int32_t height, gcfg_dataterm **out) {
 int gcfg_dt_destroy(gcfg_dataterm *dt) {
     if (!dt) return GCFG_OK;
     hipStream_t s = dt->stream;
-    hipStreamSynchronize(s);
+    (void)hipStreamSynchronize(s);
     delete dt;
-    hipStreamDestroy(s);
+    (void)hipStreamDestroy(s);
     return GCFG_OK;
 }
 

@@ -28,18 +28,18 @@ double now_ms() {
 }
 
 Engine::~Engine() {
-    if (stream_) hipStreamSynchronize(stream_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
     drop_graphs();
-    for (hipEvent_t e : events_) hipEventDestroy(e);
+    for (hipEvent_t e : events_) (void)hipEventDestroy(e);
     for (hipEvent_t e : build_ev_)
-        if (e) hipEventDestroy(e);
-    for (void *p : allocs_) hipFree(p);
-    if (pin_desc_) hipHostFree(pin_desc_);
-    if (pin_slots_) hipHostFree(pin_slots_);
-    if (pin_ctl_) hipHostFree(pin_ctl_);
-    if (pin_states_) hipHostFree(pin_states_);
-    if (pin_qtrace_) hipHostFree(pin_qtrace_);
-    if (stream_) hipStreamDestroy(stream_);
+        if (e) (void)hipEventDestroy(e);
+    for (void *p : allocs_) (void)hipFree(p);
+    if (pin_desc_) (void)hipHostFree(pin_desc_);
+    if (pin_slots_) (void)hipHostFree(pin_slots_);
+    if (pin_ctl_) (void)hipHostFree(pin_ctl_);
+    if (pin_states_) (void)hipHostFree(pin_states_);
+    if (pin_qtrace_) (void)hipHostFree(pin_qtrace_);
+    if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 int StreamJoin::set(hipStream_t s) {
@@ -66,8 +66,8 @@ int StreamJoin::after(hipStream_t own) {
 }
 
 StreamJoin::~StreamJoin() {
-    if (ev_in) hipEventDestroy(ev_in);
-    if (ev_out) hipEventDestroy(ev_out);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_out) (void)hipEventDestroy(ev_out);
 }
 
 int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o, int nbhd) {
@@ -435,7 +435,7 @@ int Engine::edit(int slot, const std::vector<int32_t> &strokes, int32_t delta) {
     if ((int)strokes.size() > strokes_cap_) {
         GCFG_HIP(hipStreamSynchronize(stream_));
         if (d_strokes_) {
-            hipFree(d_strokes_);
+            (void)hipFree(d_strokes_);
             allocs_.erase(std::remove(allocs_.begin(), allocs_.end(), (void *)d_strokes_), allocs_.end());
         }
         strokes_cap_ = std::max<int>((int)strokes.size(), 5 * 64);
@@ -462,8 +462,8 @@ static int even_steps(int n) { return (std::max(n, 1) + 1) & ~1; }
 
 void Engine::drop_graphs() {
     for (StepGraph &g : graphs_) {
-        hipGraphExecDestroy(g.exec);
-        hipGraphDestroy(g.graph);
+        (void)hipGraphExecDestroy(g.exec);
+        (void)hipGraphDestroy(g.graph);
     }
     graphs_.clear();
 }
@@ -475,8 +475,8 @@ int Engine::step_graph(int steps, hipGraphExec_t *out) {
             return GCFG_OK;
         }
     if (graphs_.size() >= 8) {  // keep a handful of chunk lengths
-        hipGraphExecDestroy(graphs_.front().exec);
-        hipGraphDestroy(graphs_.front().graph);
+        (void)hipGraphExecDestroy(graphs_.front().exec);
+        (void)hipGraphDestroy(graphs_.front().graph);
         graphs_.erase(graphs_.begin());
     }
     StepGraph g{steps, nullptr, nullptr};
@@ -485,7 +485,7 @@ int Engine::step_graph(int steps, hipGraphExec_t *out) {
     GCFG_HIP(hipStreamEndCapture(stream_, &g.graph));
     hipError_t e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        hipGraphDestroy(g.graph);
+        (void)hipGraphDestroy(g.graph);
         return hip_fail(e, "hipGraphInstantiate");
     }
     graphs_.push_back(g);
@@ -505,7 +505,7 @@ int Engine::launch_steps(int n) {
         }
         const bool trace = opt.verbose >= 2;
         if (trace && qtrace_cap_ < n) {
-            if (pin_qtrace_) hipHostFree(pin_qtrace_);
+            if (pin_qtrace_) (void)hipHostFree(pin_qtrace_);
             pin_qtrace_ = nullptr;
             GCFG_HIP(hipHostMalloc((void **)&pin_qtrace_, (size_t)n * sizeof(QCtl), hipHostMallocDefault));
             qtrace_cap_ = n;
@@ -699,10 +699,10 @@ int Engine::read_state(int slot, int32_t *h_caps, int32_t *h_exc, int32_t *h_dis
         if (d8) e5 = hipMemcpyAsync(h_caps8, d8, n * 4 * sizeof(int32_t), hipMemcpyDeviceToHost, stream_);
         e4 = hipStreamSynchronize(stream_);
     }
-    if (dc) hipFree(dc);
-    if (de) hipFree(de);
-    if (dd) hipFree(dd);
-    if (d8) hipFree(d8);
+    if (dc) (void)hipFree(dc);
+    if (de) (void)hipFree(de);
+    if (dd) (void)hipFree(dd);
+    if (d8) (void)hipFree(d8);
     GCFG_HIP(e0);
     GCFG_HIP(e1);
     GCFG_HIP(e2);

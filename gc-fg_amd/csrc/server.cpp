@@ -78,7 +78,7 @@ static void destroy_session(gcfg_server *s, Session *ss) {
         s->by_slot.erase(ss->slot);
     }
     delete ss->dt;
-    if (ss->d_label) hipFree(ss->d_label);
+    if (ss->d_label) (void)hipFree(ss->d_label);
     s->used -= std::min(s->used, ss->bytes);
     s->sessions.erase(ss->id);
     delete ss;
@@ -142,7 +142,7 @@ static int admit_image(gcfg_server *s, Request *r) {
     if (rc != GCFG_OK) {
         if (ss->slot >= 0) s->eng.remove_instance(ss->slot);
         delete ss->dt;
-        if (ss->d_label) hipFree(ss->d_label);
+        if (ss->d_label) (void)hipFree(ss->d_label);
         delete ss;
         return rc;
     }

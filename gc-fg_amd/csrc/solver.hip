@@ -2355,8 +2355,9 @@ __global__ __launch_bounds__(NT) void k_reframe(KParams p, int slot, const int32
             const int Xv = X + (k == 0), Yv = Y + (k == 1);
             if (!inside(Xv, Yv)) continue;
             const size_t iv = (size_t)Yv * D.W + Xv, gv = node(Xv, Yv);
+            // Net flow u -> v from one side of the pair (the pair's residual
+            // sum is the sum of its two old capacities).
             const int c_uv0 = reframe_cap(k ? D.down : D.right, D.penalty, i);
-            const int c_vu0 = reframe_cap(k ? D.up : D.left, D.penalty, iv);
             const int c_uv1 = reframe_cap(k ? down1 : right1, pen1, i);
             const int c_vu1 = reframe_cap(k ? up1 : left1, pen1, iv);
             bad |= (c_uv1 < 0) | (c_vu1 < 0);
