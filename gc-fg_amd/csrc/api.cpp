@@ -55,7 +55,7 @@ void gcfg_default_options(gcfg_options *o) {
     o->gr_rounds = 8;
     o->gr_block_w = 1;
     o->gr_block_h = 1;
-    o->gr_cap = 0;
+    o->gr_cap = 8;  // capped while flow moves locally; exact after 4 rounds (see the scheduler)
     o->pr_waves_per_cu = 0;
     o->gr_waves_per_cu = 0;
     o->lazy_caps = 0;

@@ -259,6 +259,7 @@ enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2, EM_LIVE = 3, EM_D
 // 256 threads (2 k tiles a pass) while every instance has at most 2 k tiles,
 // 1024 (8 k a pass: a 4096^2 image in two) otherwise.
 constexpr int EMIT_U = 8;  // units per thread in one pass of emit_tiles
+constexpr int CAP_ROUNDS = 4;  // capped relabel rounds before an instance switches to exact ones
 // Push-queue item flag: the tile's capacities are still only in the inputs
 // (lazy build); the push pass reads them from there and writes them out.
 constexpr int PR_FRESH = 1 << 30;
@@ -434,7 +435,7 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     InstState &st = p.state[s];
     const int phase = st.phase;
     if (phase == PH_IDLE || phase == PH_DONE) return;
-    const int err = st.err, sweeps = st.sweeps, cur_cap = st.cap;
+    const int err = st.err, sweeps = st.sweeps, cur_cap = st.cap, rounds0 = st.gr_rounds;
     int pr_iter = st.pr_iter;
     __syncthreads();  // every thread has read the state before thread 0 writes it
     if (threadIdx.x == 0) st.steps++;
@@ -457,7 +458,11 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
         }
         return;
     }
-    const int cap = (p.variant && p.gr_cap > 0) ? p.gr_cap : DINF;
+    // Capped relabels pay when flow moves locally (few rounds); an instance
+    // still unfinished after CAP_ROUNDS relabel rounds has flow travelling
+    // far, where capped rounds multiply the steps: it relabels exactly from
+    // then on.
+    const int cap = (p.variant && p.gr_cap > 0 && rounds0 < CAP_ROUNDS) ? p.gr_cap : DINF;
     const bool reinit = p.variant != 0;
     if (phase == PH_START) {
         // A warm start relabels every tile: an edit may have revived tiles

@@ -53,10 +53,11 @@ typedef struct gcfg_options {
     int32_t gr_rounds;       /* max tile-local relaxation rounds per global-relabel sweep (then re-queued) */
     int32_t gr_block_w;      /* global-relabel block in tiles (one workgroup, BFS exchanged through LDS): */
     int32_t gr_block_h;      /*   1x1, 1x2, 1x4, 1x8, 1x16, 2x2, 2x4, 2x8 or 4x4; 0 = default */
-    int32_t gr_cap;          /* >0: intermediate global relabels stop BFS fronts at this distance (nodes
-                                beyond are frozen until the next relabel); the solve ends with a
-                                reachability round (and an exact round if that finds work left);
-                                0: every relabel exact */
+    int32_t gr_cap;          /* >0 (default 8): the first 4 global relabels of an instance stop BFS
+                                fronts at this distance (nodes beyond are frozen until the next
+                                relabel), later ones are exact; an instance finished within capped
+                                rounds ends with a reachability round (and an exact round if that
+                                finds work left); 0: every relabel exact */
     int32_t pr_waves_per_cu; /* push/relabel launch size in wavefronts per compute unit (0 = 16) */
     int32_t gr_waves_per_cu; /* global-relabel launch size in wavefronts per compute unit (0 = 28) */
     int32_t lazy_caps;       /* residual capacities of the wavefront kernels: 0 = chosen per build (lazy for
@@ -137,8 +138,9 @@ int gcfg_graph_read_state(gcfg_graph *g, int32_t *h_caps, int32_t *h_excess);
 int gcfg_graph_read_diag_state(gcfg_graph *g, int32_t *h_caps8);
 /* Distance labels (W*H, image layout) after a solve: the exact BFS distance
  * of every interior node to a node with excess < 0 along residual arcs in the
- * final residual graph (gr_cap 0, or the workgroup kernels); 0x3fffffff where
- * no such path exists. With gr_cap > 0 the solve ends with a reachability
+ * final residual graph (gr_cap 0, the workgroup kernels, or an instance that
+ * outlasted its capped rounds); 0x3fffffff where no such path exists. An
+ * instance that finished within capped rounds ends with a reachability
  * round instead: 0 for the nodes with such a path, 0x3fffffff for the others.
  * No counterpart in the reference (its heights are approximate); exposed for
  * verification. */
