@@ -45,14 +45,14 @@ extern "C" {
 void gcfg_default_options(gcfg_options *o) {
     if (!o) return;
     o->sweeps_per_gr = 1;
-    o->inner_iters = 4;
+    o->inner_iters = 8;
     o->steps_per_poll = 8;
     o->use_graph = 1;
     o->max_steps = 200000;
     o->verbose = 0;
     o->profile = 0;
     o->kernels = 1;
-    o->gr_rounds = 8;
+    o->gr_rounds = 48;
     o->gr_block_w = 1;
     o->gr_block_h = 1;
     o->gr_cap = 8;  // capped while flow moves locally; exact after 4 rounds (see the scheduler)

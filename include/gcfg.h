@@ -42,7 +42,7 @@ enum { GCFG_DT_MODEL_HISTOGRAM = 0, GCFG_DT_MODEL_KDE = 1 };
 
 typedef struct gcfg_options {
     int32_t sweeps_per_gr;   /* colour-pass pairs of tile push-relabel between global relabels */
-    int32_t inner_iters;     /* max in-LDS push/relabel iterations per tile pass */
+    int32_t inner_iters;     /* max push/relabel iterations per tile pass (default 8) */
     int32_t steps_per_poll;  /* scheduler steps launched between two host polls */
     int32_t use_graph;       /* capture a chunk of steps in a hipGraph and replay it */
     int32_t max_steps;       /* 0 = unbounded; else GCFG_ERR_NOT_CONVERGED past this */
@@ -50,7 +50,8 @@ typedef struct gcfg_options {
     int32_t profile;         /* 1: time every push/relabel and global-relabel launch with HIP events;
                                 2: also count inner iterations (pr_iters, gr_iters; adds atomics) */
     int32_t kernels;         /* 1: wavefront-per-tile kernels (default), 0: workgroup-per-tile LDS kernels */
-    int32_t gr_rounds;       /* max tile-local relaxation rounds per global-relabel sweep (then re-queued) */
+    int32_t gr_rounds;       /* max tile-local relaxation rounds per global-relabel sweep (then re-queued;
+                                default 48) */
     int32_t gr_block_w;      /* global-relabel block in tiles (one workgroup, BFS exchanged through LDS): */
     int32_t gr_block_h;      /*   1x1, 1x2, 1x4, 1x8, 1x16, 2x2, 2x4, 2x8 or 4x4; 0 = default */
     int32_t gr_cap;          /* >0 (default 8): the first 4 global relabels of an instance stop BFS
