@@ -322,6 +322,58 @@ def single_instance(kind, seed):
     return W, H, bufs
 
 
+def extras(args):
+    """Rows beyond the batch: the two data-term models and the 4- vs
+    8-neighbourhood solver on the 1024^2 ellipse image (default options,
+    penalty 50 for both neighbourhoods; medians of 8 runs after 2 warm-ups)."""
+    def timeit(fn, reps=8):
+        fn()
+        fn()
+        ts = []
+        for _ in range(reps):
+            gcfg.device_synchronize()
+            t0 = time.perf_counter()
+            fn()
+            gcfg.device_synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return round(float(np.median(ts)), 3)
+
+    W = H = 1024
+    img = gcfg.gen_ellipse_image(W, H, args.seed)
+    seeds = gcfg.gen_seed_map(W, H)
+    out = dict(workload="extras", W=W, H=H)
+    dt = gcfg.DataTerm(W, H)
+    dt.set_image(img)
+    dt.set_seeds(seeds)
+    out["dataterm_histogram_ms"] = timeit(dt.compute)
+    pos, neg = dt.links()[:2]
+    dt.close()
+    dk = gcfg.DataTerm(W, H)
+    dk.set_model(gcfg.DT_MODEL_KDE)
+    dk.set_sigma(30)
+    dk.set_image(img)
+    for k in range(8):
+        dk.kde_select(100 + 100 * k, 500)
+    out["dataterm_kde_seeds"] = dk.kde_count()
+    out["dataterm_kde_ms"] = timeit(dk.compute)
+    dk.close()
+    for nb in (4, 8):
+        g = gcfg.Graph(W, H, gcfg.default_options(), neighborhood=nb)
+        g.set_dataterms(pos, neg)
+        g.set_penalty(50)
+        d_label = gcfg.DeviceBuffer(4 * W * H)
+
+        def run():
+            g.build()
+            g.optimize_device(d_label)
+        out[f"n{nb}_penalty_ms"] = timeit(run)
+        st = g.optimize_device(d_label)
+        out[f"n{nb}_steps"] = int(st.steps)
+        out[f"n{nb}_flow"] = int(st.flow)
+        g.close()
+    print(json.dumps(out), flush=True)
+
+
 def single(args, opts):
     """One plain GC_* drop-in solve at C1 or C2, with labels left on the device."""
     W, H, bufs = single_instance(args.single, args.seed)
@@ -438,6 +490,8 @@ def main():
     ap.add_argument("--serve-requests-high", type=int, default=200)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="copy 1 GiB device-to-device first (a known-size kernel for profiler counters)")
+    ap.add_argument("--extras", action="store_true",
+                    help="time the two data-term models and the 4- vs 8-neighbourhood solver at 1024^2 instead")
     ap.add_argument("--single", choices=["c1", "c2"], default=None,
                     help="time one plain GC_* solve instead: c1 = 1024^2 ellipse image, "
                          "c2 = 4096^2 heavy-tailed t-links on Perlin n-links")
@@ -450,6 +504,9 @@ def main():
         raise SystemExit("no GPU visible: bench.py needs an MI355X")
 
     opts = gcfg.default_options()
+    if args.extras:
+        extras(args)
+        return
     if args.single:
         if args.sweeps:
             opts.sweeps_per_gr = args.sweeps
