@@ -458,7 +458,6 @@ int Engine::edit(int slot, const std::vector<int32_t> &strokes, int32_t delta) {
 
 // Steps alternate queue-counter sets by parity, so a captured chunk holds an
 // even number of steps starting at parity 0 and always ends where it began.
-static int even_steps(int n) { return (std::max(n, 1) + 1) & ~1; }
 
 void Engine::drop_graphs() {
     for (StepGraph &g : graphs_) {
@@ -468,20 +467,20 @@ void Engine::drop_graphs() {
     graphs_.clear();
 }
 
-int Engine::step_graph(int steps, hipGraphExec_t *out) {
+int Engine::step_graph(int steps, int parity, hipGraphExec_t *out) {
     for (StepGraph &g : graphs_)
-        if (g.steps == steps) {
+        if (g.steps == steps && g.parity == parity) {
             *out = g.exec;
             return GCFG_OK;
         }
-    if (graphs_.size() >= 8) {  // keep a handful of chunk lengths
+    if (graphs_.size() >= 16) {  // keep a handful of chunk lengths and parities
         (void)hipGraphExecDestroy(graphs_.front().exec);
         (void)hipGraphDestroy(graphs_.front().graph);
         graphs_.erase(graphs_.begin());
     }
-    StepGraph g{steps, nullptr, nullptr};
+    StepGraph g{steps, parity, nullptr, nullptr};
     GCFG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < steps; ++i) launch_step(kp_, grid_pr_, grid_gr_, i & 1, stream_);
+    for (int i = 0; i < steps; ++i) launch_step(kp_, grid_pr_, grid_gr_, (parity + i) & 1, stream_);
     GCFG_HIP(hipStreamEndCapture(stream_, &g.graph));
     hipError_t e = hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
@@ -527,16 +526,12 @@ int Engine::launch_steps(int n) {
         return GCFG_OK;
     }
     if (opt.use_graph) {
-        if (parity_) {  // realign to an even step before replaying the graph
-            launch_step(kp_, grid_pr_, grid_gr_, parity_, stream_);
-            parity_ ^= 1;
-            steps_run += 1;
-        }
-        const int steps = even_steps(n);
+        const int steps = std::max(n, 1);
         hipGraphExec_t exec = nullptr;
-        int rc = step_graph(steps, &exec);
+        int rc = step_graph(steps, parity_, &exec);
         if (rc != GCFG_OK) return rc;
         GCFG_HIP(hipGraphLaunch(exec, stream_));
+        parity_ ^= steps & 1;
         steps_run += steps;
     } else {
         for (int i = 0; i < n; ++i) {

@@ -325,16 +325,18 @@ class Engine {
     InstState *pin_states_ = nullptr;  // cap_slots_ states (fetch_states_pinned)
     int32_t *d_strokes_ = nullptr;
     int strokes_cap_ = 0;
-    // Captured step chunks, one per chunk length (even: every chunk starts at
-    // queue parity 0); dropped when the options change.
+    // Captured step chunks, one per chunk length and starting queue parity
+    // (a chunk of any length continues the parity sequence, so a solve
+    // replays exactly the steps it needs); dropped when the options change.
     struct StepGraph {
         int steps;
+        int parity;
         hipGraph_t graph;
         hipGraphExec_t exec;
     };
     std::vector<StepGraph> graphs_;
     void drop_graphs();
-    int step_graph(int steps, hipGraphExec_t *out);
+    int step_graph(int steps, int parity, hipGraphExec_t *out);
     int parity_ = 0;  // parity of the next step to launch
     std::vector<void *> allocs_;
     std::vector<hipEvent_t> events_;  // 6 per profiled step: start/stop of sched, pr, gr
