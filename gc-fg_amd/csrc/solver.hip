@@ -1289,7 +1289,10 @@ __device__ __forceinline__ int64_t wave_sum64_w(int64_t v) { return wave_sum64_d
 // 16..31 the right column, indexed by row). The loads are issued
 // unconditionally from clamped addresses (a missing neighbour reads the own
 // tile) and masked afterwards, so they overlap with the tile's own loads.
-enum HaloSrc : int { HALO_DIST = 0, HALO_EXCESS = 1, HALO_MASK = 2 };
+// HALO_WITNESS (reachability rounds): 0 where the stored distance is below the
+// cap of the capped rounds (a residual path to a sink that is still intact,
+// see k_grb), DINF elsewhere.
+enum HaloSrc : int { HALO_DIST = 0, HALO_EXCESS = 1, HALO_MASK = 2, HALO_WITNESS = 3 };
 
 template <int SRC>
 __device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, int t, int lane, int &hU, int &hD,
@@ -1308,6 +1311,10 @@ __device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, 
         vu = (p.mask[gu] & MB_ENEG) ? 0 : DINF;
         vd = (p.mask[gd] & MB_ENEG) ? 0 : DINF;
         vs = (p.mask[gs] & MB_ENEG) ? 0 : DINF;
+    } else if (SRC == HALO_WITNESS) {
+        vu = p.dist[gu] < p.gr_cap ? 0 : DINF;
+        vd = p.dist[gd] < p.gr_cap ? 0 : DINF;
+        vs = p.dist[gs] < p.gr_cap ? 0 : DINF;
     } else {
         vu = p.dist[gu];
         vd = p.dist[gd];
@@ -1709,15 +1716,42 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
         Masks m;
         m.clear();
         int hU = DINF, hD = DINF, hs = DINF;
+        // A reachability round follows a capped round that converged with no
+        // push pass in between, and skips the blocks at live 3 (every interior
+        // node below the cap). A stored distance below the cap is then the
+        // length of a residual path to a sink that still exists: the capped
+        // round recomputed every block within the cap of a push since, and a
+        // path shorter than the cap lies within the cap of its block. The
+        // flood therefore starts from those nodes as well as from the sinks,
+        // and reads its halos the same way, so a node whose only route to a
+        // sink enters a skipped block through a node that is not itself a
+        // sink still reaches it (the skipped block never marks anyone dirty).
         if (!have) {
 #pragma unroll
             for (int y = 0; y < TH; ++y) d[y] = DINF;
+        } else if (reinit && flood) {
+            load_halo<HALO_WITNESS>(p, ti, t, lane, hU, hD, hs);
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                m.put(y, T.m[y * TW + lane]);
+                d[y] = T.d[y * TW + lane];
+            }
+#pragma unroll
+            for (int y = 0; y < TH; ++y) d[y] = ((m.get(y) & MB_ENEG) || d[y] < p.gr_cap) ? 0 : DINF;
         } else if (reinit) {
             load_halo<HALO_MASK>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
             for (int y = 0; y < TH; ++y) m.put(y, T.m[y * TW + lane]);
 #pragma unroll
             for (int y = 0; y < TH; ++y) d[y] = (m.get(y) & MB_ENEG) ? 0 : vinit;
+        } else if (flood) {
+            // Visited blocks hold 0 or DINF by now; skipped ones their capped distances.
+            load_halo<HALO_WITNESS>(p, ti, t, lane, hU, hD, hs);
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                m.put(y, T.m[y * TW + lane]);
+                d[y] = T.d[y * TW + lane];
+            }
         } else {
             load_halo<HALO_DIST>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll

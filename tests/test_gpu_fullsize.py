@@ -7,7 +7,10 @@
     (feasible preflow, conservation, labels = nodes that cannot reach a
     deficit node, flow = sum pos - sum max(excess, 0) = cut of the labels);
   * the 256-image batch of areas 128^2..1024^2: every instance equals the
-    plain path bit for bit, and a sample equals the oracle."""
+    plain path and the oracle bit for bit, with the default options, the
+    benchmark's tuned options and both capacity layouts, each solved twice."""
+import os
+
 import numpy as np
 import pytest
 
@@ -60,22 +63,66 @@ def test_c2_stress_4096_certificate(g, ora, cap):
     assert ora.certify(W, H, pos, neg, caps, exc, label, st.flow, edges=E) == 0
 
 
-def test_c3_batch_256(g, ora):
+@pytest.fixture(scope="module")
+def c3_batch(g, ora):
+    """The 256-image benchmark batch (device data term) with every instance's
+    oracle flow and labels (solved on the host cores in parallel)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     n = 256
     Ws, Hs = g.gen_batch_shapes(n, 2024, 128.0 ** 2, 1024.0 ** 2)
     print(f"[fullsize] batch of {n}, {int((Ws * Hs).sum())} px", flush=True)
-    items, host = [], []
+    host = []
     for i in range(n):
         W, H = int(Ws[i]), int(Hs[i])
-        links = _links(g, W, H, g.gen_ellipse_image(W, H, g.mix_seed(2024, i)), g.gen_seed_map(W, H))
-        host.append((W, H, links))
-        items.append(g.BatchItem(W, H, links[0], links[1], edges=tuple(links[2:6])))
+        host.append((W, H, _links(g, W, H, g.gen_ellipse_image(W, H, g.mix_seed(2024, i)), g.gen_seed_map(W, H))))
+
+    def ref(h):
+        W, H, (pos, neg, up, down, left, right) = h
+        return ora.maxflow(W, H, pos, neg, edges=(up, down, left, right))
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        refs = list(ex.map(ref, host))
+    return host, refs
+
+
+def test_c3_batch_256(g, ora, c3_batch):
+    host, refs = c3_batch
+    items = [g.BatchItem(W, H, L[0], L[1], edges=tuple(L[2:6])) for W, H, L in host]
     b = g.Batch(items)
     flows, labels, _ = b.solve(labels_to_host=True)
     b.close()
     for i, (W, H, (pos, neg, up, down, left, right)) in enumerate(host):
         f, lab, _ = g.solve(W, H, pos, neg, edges=(up, down, left, right))
         assert flows[i] == f and np.array_equal(labels[i], lab), i
-        if i % 32 == 0:
-            rf, rl = ora.maxflow(W, H, pos, neg, edges=(up, down, left, right))
-            assert f == rf and np.array_equal(lab, rl), i
+        rf, rl = refs[i]
+        assert f == rf and np.array_equal(lab, rl), i
+
+
+# The options the benchmark's tuner picks for this batch (one push iteration
+# per pass, one sweep pair per relabel, cap 4, 56 relabel wavefronts per CU),
+# with the capacity layout left to the engine, forced to the build, and forced
+# lazy. Each option set solves the batch twice on one handle: the second
+# solve starts with the first solve's step count as its first chunk, which
+# may switch the capacity layout and changes the poll schedule.
+C3_OPTION_SETS = [dict(), dict(sweeps_per_gr=1, inner_iters=1, gr_cap=4, gr_waves_per_cu=56),
+                  dict(sweeps_per_gr=1, inner_iters=1, gr_cap=4, gr_waves_per_cu=56, lazy_caps=1),
+                  dict(sweeps_per_gr=1, inner_iters=1, gr_cap=4, gr_waves_per_cu=56, lazy_caps=2),
+                  dict(gr_cap=8, lazy_caps=1), dict(gr_cap=8, lazy_caps=2)]
+
+
+@pytest.mark.parametrize("opts", C3_OPTION_SETS)
+def test_c3_batch_every_instance_vs_oracle(g, c3_batch, opts):
+    host, refs = c3_batch
+    o = g.default_options()
+    for k, v in opts.items():
+        setattr(o, k, v)
+    items = [g.BatchItem(W, H, L[0], L[1], edges=tuple(L[2:6])) for W, H, L in host]
+    b = g.Batch(items, o)
+    for rep in range(2):
+        print(f"[fullsize] batch {opts} solve {rep}", flush=True)
+        flows, labels, _ = b.solve(labels_to_host=True)
+        for i, (rf, rl) in enumerate(refs):
+            assert flows[i] == rf, (opts, rep, i, int(flows[i]), rf)
+            assert np.array_equal(labels[i], rl), (opts, rep, i, int((labels[i] != rl).sum()))
+    b.close()
