@@ -239,7 +239,8 @@ static int graph_solve(gcfg_graph *g, int32_t *d_target, gcfg_stats *stats) {
     g->eng.steps_hint = st.steps;
     const gcfg::Ctl ctl = g->eng.polled_ctl();
     if (st.err) {
-        set_error("invalid capacities in the inputs (negative n-link or |pos - neg| > 2^30)");
+        set_error("invalid capacities in the inputs (|pos - neg| > 2^29, or an n-link outside [0, 2^28) "
+                  "(4-neighbourhood) / [0, 2^27) (8-neighbourhood))");
         return GCFG_ERR_INVALID;
     }
     if (g->opt.verbose) std::printf("Final energy level: %lld\n", (long long)st.flow);
@@ -423,7 +424,7 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
         const InstState &st = b->states[i];
         if (st.err) {
             set_error("invalid capacities in instance " + std::to_string(i) +
-                      " (negative n-link or |pos - neg| > 2^30)");
+                      " (|pos - neg| > 2^29, or an n-link outside [0, 2^28))");
             return GCFG_ERR_INVALID;
         }
         if (flows) flows[i] = st.flow;

@@ -119,7 +119,10 @@ long long GC_GetFlow(GlobalWrapper gw) { return gw.last_flow ? *gw.last_flow : 0
 namespace {
 gcfg_dataterm *g_dt = nullptr;
 int g_w = 0, g_h = 0;
-int g_model = GCFG_DT_MODEL_HISTOGRAM;
+// The reference's only colour model is the kernel density
+// (DataTerm.cpp:105-137, DataTermKernels.cu:61-83): a driver that switches
+// headers gets the same t-links unless it asks for the histogram model.
+int g_model = GCFG_DT_MODEL_KDE;
 std::vector<int> g_selection;
 
 gcfg_dataterm *dt() {
@@ -149,7 +152,7 @@ int *getRight() { return link(5); }
 void init_data_term(int w, int h) {
     if (g_dt) free_data_term();
     gc_check(gcfg_dt_create(w, h, &g_dt), "DataTerm::init_data_term");
-    if (g_model != GCFG_DT_MODEL_HISTOGRAM) gc_check(gcfg_dt_set_model(g_dt, g_model), "DataTerm::init_data_term");
+    gc_check(gcfg_dt_set_model(g_dt, g_model), "DataTerm::init_data_term");
     g_w = w;
     g_h = h;
     g_selection.assign((size_t)w * h, 0);

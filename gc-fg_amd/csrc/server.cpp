@@ -47,7 +47,12 @@ struct Session {
     size_t bytes = 0;
     bool busy = false;
     double last_used = 0;
+    // Sum over the session's edits of delta x strokes: a bound on how far the
+    // edits have moved any node's pos - neg. Kept at most 2^29 so that the
+    // excess of every node stays inside int32 (solver.hip, tlinks_bad).
+    int64_t edit_mass = 0;
 };
+constexpr int64_t EDIT_MASS_MAX = (int64_t)1 << 29;
 
 }  // namespace
 
@@ -216,7 +221,15 @@ static int admit(gcfg_server *s) {
             }
             int rc = GCFG_OK;
             if (r->kind == REQ_EDIT) {
-                rc = s->eng.edit(ss->slot, r->strokes, r->delta);
+                const int64_t mass = (int64_t)r->delta * (int64_t)(r->strokes.size() / 5);
+                if (ss->edit_mass + mass > EDIT_MASS_MAX) {
+                    set_error("server: the session's edits would move pos - neg by more than 2^29 "
+                              "(sum of delta x strokes over its edits)");
+                    rc = GCFG_ERR_INVALID;
+                } else {
+                    ss->edit_mass += mass;
+                    rc = s->eng.edit(ss->slot, r->strokes, r->delta);
+                }
             } else if (r->W != ss->W || r->H != ss->H) {
                 set_error("server: frame size differs from the session's");
                 rc = GCFG_ERR_INVALID;

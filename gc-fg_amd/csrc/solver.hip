@@ -100,11 +100,20 @@ __device__ __forceinline__ void set_active(const KParams &p, int t, int v) { p.a
 // Inputs a node accepts: t-links of any sign, as in the reference (it takes
 // excess = pos - neg unchecked, GraphCutKernels.cu:196-197; its own colour
 // data term makes neg negative where p > 1, DataTermKernels.cu:81-82), with
-// |pos - neg| <= 2^30 so that int32 excess keeps room for n-link inflow, and
-// non-negative n-links.
+// |pos - neg| <= 2^29, and n-links in [0, 2^28) (4-neighbourhood) or
+// [0, 2^27) (8-neighbourhood). Excess then stays inside int32 for good: with
+// e0 = pos - neg plus the accumulated edits of a session (at most 2^29, kept
+// by the server), a node's excess is e0 plus the net flow of its pairs, each
+// at most the pair's capacity into the node, so |e| < 2^29 + 2^29 + 2^30;
+// and a pair's residual capacity is at most the sum of its two capacities.
+constexpr int TLINK_DIFF_MAX = 1 << 29;
 __device__ __forceinline__ bool tlinks_bad(int pp, int qq) {
     const long long d = (long long)pp - qq;
-    return d < -(1LL << 30) || d > (1LL << 30);
+    return d < -(long long)TLINK_DIFF_MAX || d > TLINK_DIFF_MAX;
+}
+__device__ __forceinline__ bool nlink_bad(int c, int nbhd) { return c < 0 || c >= (nbhd == 8 ? 1 << 27 : 1 << 28); }
+__device__ __forceinline__ bool nlinks_bad(const int4 &c, int nbhd) {
+    return nlink_bad(c.x, nbhd) | nlink_bad(c.y, nbhd) | nlink_bad(c.z, nbhd) | nlink_bad(c.w, nbhd);
 }
 __device__ __forceinline__ void mark_dirty(const KParams &p, int t) { p.dirty[t] = 1; }
 
@@ -199,7 +208,7 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
                     if (X == D.W - 2) c8.y = c8.w = 0;
                     if (Y == 1) c8.x = c8.y = 0;
                     if (Y == D.H - 2) c8.z = c8.w = 0;
-                    bad |= (c8.x < 0) | (c8.y < 0) | (c8.z < 0) | (c8.w < 0);
+                    bad |= nlinks_bad(c8, 8);
                 }
                 if (D.up) {
                     c.x = D.left[i];
@@ -213,7 +222,7 @@ __global__ __launch_bounds__(NT) void k_build(KParams p, const int32_t *slots, c
                 if (X == D.W - 2) c.y = 0;
                 if (Y == 1) c.z = 0;
                 if (Y == D.H - 2) c.w = 0;
-                bad |= tlinks_bad(pp, qq) | (c.x < 0) | (c.y < 0) | (c.z < 0) | (c.w < 0);
+                bad |= tlinks_bad(pp, qq) | nlinks_bad(c, p.nbhd);
                 e = pp - qq;
                 sp += pp;
             }
@@ -2233,7 +2242,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
                     if (X == D.W - 2) c.y = 0;
                     if (Y == 1) c.z = 0;
                     if (Y == D.H - 2) c.w = 0;
-                    bad |= tlinks_bad(vp[k], vn[k]) | (c.x < 0) | (c.y < 0) | (c.z < 0) | (c.w < 0);
+                    bad |= tlinks_bad(vp[k], vn[k]) | nlinks_bad(c, 4);
                     e = vp[k] - vn[k];
                     sp += vp[k];
                     sx += max(e, 0);
@@ -2387,7 +2396,11 @@ __global__ __launch_bounds__(NT) void k_reframe(KParams p, int slot, const int32
         const size_t gu = node(X, Y);
         const int p0 = D.pos[i], q0 = D.neg[i], p1 = pos1[i], q1 = neg1[i];
         bad |= tlinks_bad(p1, q1);
-        int de = (p1 - q1) - (p0 - q0);
+        // The change of a node's excess can leave int32 on its own (t-link
+        // differences up to 2^29 each, pair flows up to 2^28); the final
+        // excess is in range (see tlinks_bad), so the parts are summed in 64
+        // bits and applied with 32-bit atomics, which wrap, in any order.
+        int64_t de = ((int64_t)p1 - q1) - ((int64_t)p0 - q0);
         dp += (int64_t)p1 - p0;
         // pairs (u, right neighbour) and (u, lower neighbour)
         for (int k = 0; k < 2; ++k) {
@@ -2399,7 +2412,7 @@ __global__ __launch_bounds__(NT) void k_reframe(KParams p, int slot, const int32
             const int c_uv0 = reframe_cap(k ? D.down : D.right, D.penalty, i);
             const int c_uv1 = reframe_cap(k ? down1 : right1, pen1, i);
             const int c_vu1 = reframe_cap(k ? up1 : left1, pen1, iv);
-            bad |= (c_uv1 < 0) | (c_vu1 < 0);
+            bad |= nlink_bad(c_uv1, 4) | nlink_bad(c_vu1, 4);
             int *ru = reinterpret_cast<int *>(&p.caps[gu]) + (k ? 3 : 1);  // u -> v: down / right
             int *rv = reinterpret_cast<int *>(&p.caps[gv]) + (k ? 2 : 0);  // v -> u: up / left
             const int g = c_uv0 - *ru;
@@ -2407,11 +2420,11 @@ __global__ __launch_bounds__(NT) void k_reframe(KParams p, int slot, const int32
             *ru = c_uv1 - g1;
             *rv = c_vu1 + g1;
             if (g != g1) {
-                de += g - g1;
-                atomicSub(&p.exc[gv], g - g1);
+                de += (int64_t)g - g1;
+                atomicAdd(reinterpret_cast<unsigned *>(&p.exc[gv]), (unsigned)(uint32_t)(g1 - (int64_t)g));
             }
         }
-        if (de) atomicAdd(&p.exc[gu], de);
+        if (de) atomicAdd(reinterpret_cast<unsigned *>(&p.exc[gu]), (unsigned)(uint32_t)de);
     }
     const int64_t tot = block_sum64(dp, red);
     const int anybad = __syncthreads_or(bad);

@@ -4,14 +4,15 @@
  *
  * Same namespace, function names and single-instance semantics. Two colour
  * models (setModel):
- *  - GCFG_DT_MODEL_HISTOGRAM (default): 16 bins per channel and class, +1
+ *  - GCFG_DT_MODEL_KDE (default, the reference's model): the Gaussian kernel
+ *    density of DataTermKernels.cu:19-150: selectOnMap(x, y, sigma) adds the
+ *    colours of the 5x5 square at (x..x+4, y..y+4) to the density,
+ *    setSigmaGaussian sets its width, and pos = (int)(300000 p),
+ *    neg = (int)(300000 (1 - p));
+ *  - GCFG_DT_MODEL_HISTOGRAM (opt-in): 16 bins per channel and class, +1
  *    smoothing, pos/neg = llround(64 * -ln p), contrast n-links with lambda 50;
- *    selectOnMap(x, y, sigma) marks the 5x5 square at (x..x+4, y..y+4) as
- *    foreground seeds, selectRect adds a rectangle of either class;
- *  - GCFG_DT_MODEL_KDE: the reference's Gaussian kernel density
- *    (DataTermKernels.cu:19-150): selectOnMap adds the colours of the same
- *    5x5 square to the density, setSigmaGaussian sets its width, and
- *    pos = (int)(300000 p), neg = (int)(300000 (1 - p)).
+ *    selectOnMap marks the same square as foreground seeds, selectRect adds a
+ *    rectangle of either class.
  *  - getDataTermsMixture() computes the t-links (and contrast n-links);
  *  - getPos()/getNeg() return device int* t-links for GC_SetDataterms,
  *    getUp/Down/Left/Right device n-links for GC_SetEdges;
@@ -45,7 +46,7 @@ int *getLeft();
 int *getRight();
 void selectRect(int x0, int y0, int w, int h, int cls); /* cls: 1 foreground, 2 background */
 void setLambda(int lambda);
-void setModel(int model); /* GCFG_DT_MODEL_HISTOGRAM or GCFG_DT_MODEL_KDE */
+void setModel(int model); /* GCFG_DT_MODEL_KDE (default) or GCFG_DT_MODEL_HISTOGRAM */
 }  // namespace DataTerm
 
 #endif

@@ -11,6 +11,12 @@
  * pixels 0<x<W-1, 0<y<H-1 are graph nodes (GraphCutKernels.cu:122-123,
  * :143-146). Labels: 1 <=> interior node that cannot reach the sink in the
  * final residual graph (SPREAD_ZEROS, GraphCut.h:37), border pixels 0.
+ * Input range (checked on the device; a solve of inputs outside it returns
+ * GCFG_ERR_INVALID): |pos - neg| <= 2^29 per pixel (t-links of either sign
+ * are accepted, as the reference takes them), n-links in [0, 2^28) for the
+ * 4-neighbourhood and [0, 2^27) for the 8-neighbourhood. Within it every
+ * node's int32 excess and every residual capacity stay in range for any
+ * sequence of pushes.
  * Calls on one handle must not overlap; distinct handles are independent.
  */
 #ifndef GCFG_H
@@ -268,7 +274,10 @@ int gcfg_server_submit_image(gcfg_server *s, int32_t width, int32_t height, cons
                              const uint8_t *h_seeds, int32_t *h_label, int64_t *request_id, int32_t *session);
 /* Edit of a live session: n strokes of 5 ints (x0, y0, w, h, cls); a
  * foreground stroke adds delta to pos, a background stroke adds delta to neg,
- * on the stroke pixels. Warm-starts from the device-resident residual graph. */
+ * on the stroke pixels. Warm-starts from the device-resident residual graph.
+ * A session takes edits while the sum of delta x n_strokes over them stays at
+ * most 2^29; the request past that fails with GCFG_ERR_INVALID (the int32
+ * excess bound below). */
 int gcfg_server_submit_edit(gcfg_server *s, int32_t session, int32_t n_strokes, const int32_t *strokes,
                             int32_t delta, int32_t *h_label, int64_t *request_id);
 /* Next video frame of a live session (the loop of ComplexApplication.cu:202-207,

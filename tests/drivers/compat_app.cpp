@@ -5,7 +5,7 @@
 // hipMalloc in place of cudaMalloc.
 //
 // Input file (little-endian int32 header, then bytes):
-//   W, H, penalty, sigma, model, nclicks, nframes,
+//   W, H, penalty, sigma, model (-1: leave the default), nclicks, nframes,
 //   nclicks x (x, y)            selectOnMap clicks on frame 0
 //   nframes x W*H*3 u8          frames (BGR as stored by the caller)
 // For every frame: setImage(frame), getDataTermsMixture(), then
@@ -49,7 +49,7 @@ int main(int argc, char **argv) {
     if (!out) die("cannot open output");
 
     GlobalWrapper gw = GC_Init(W, H);  // data terms and penalty set later (ComplexApplication.cu:105)
-    DataTerm::setModel(model);
+    if (model >= 0) DataTerm::setModel(model);  // -1: the drop-in's default model
     DataTerm::init_data_term(W, H);
     DataTerm::setSigmaGaussian(sigma);
     std::vector<int> labels(n), hpos(n), hneg(n);

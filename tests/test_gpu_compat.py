@@ -41,14 +41,17 @@ def _run(tmp_path, W, H, penalty, sigma, model, clicks, frames):
     return out
 
 
-def test_kde_video_loop_penalty(g, ora, tmp_path):
+@pytest.mark.parametrize("model", ["kde", "default"])
+def test_kde_video_loop_penalty(g, ora, tmp_path, model):
+    """The reference's colour model, asked for explicitly and as the drop-in's
+    default (a driver that only switches headers gets the same t-links)."""
     W, H = 160, 120
     frames = [g.gen_ellipse_image(W, H, s) for s in (21, 22, 23)]
     # eight clicks inside the ellipse; a kernel width of 30 grey levels covers
     # the generator's colour noise (sigma 20 per channel)
     clicks = [(75, 55), (80, 60), (70, 58), (60, 50), (90, 65), (78, 40), (72, 75), (85, 52)]
     sigma, pen = 30, 3000
-    res = _run(tmp_path, W, H, pen, sigma, g.DT_MODEL_KDE, clicks, frames)
+    res = _run(tmp_path, W, H, pen, sigma, g.DT_MODEL_KDE if model == "kde" else -1, clicks, frames)
     cols = np.concatenate([ora.kde_select_colours(frames[0], W, H, x, y) for (x, y) in clicks])
     for k, (flow, pos, neg, label, shown) in enumerate(res):
         ref = ora.dataterm_kde(W, H, frames[k], cols, sigma)

@@ -157,6 +157,22 @@ def test_certificate8(g, ora):
     assert ora.cut_value8(W, H, pos, neg, label, dirs8=dirs) == st.flow
 
 
+def test_input_range_limits8(g, ora):
+    """8-neighbourhood limits: n-links in [0, 2^27), |pos - neg| <= 2^29."""
+    rng = np.random.default_rng(78)
+    W, H = 66, 34
+    big = 1 << 29
+    pos = np.where(rng.random((H, W)) < 0.5, big, 0).astype(np.int32)
+    neg = np.where(pos == 0, big, 0).astype(np.int32)
+    dirs = tuple(np.where(rng.random((H, W)) < 0.7, (1 << 27) - 1, rng.integers(0, 1000, (H, W))).astype(np.int32)
+                 for _ in range(8))
+    check8(g, ora, W, H, pos, neg, dirs)
+    d2 = tuple(d.copy() for d in dirs)
+    d2[6][9, 9] = 1 << 27
+    with pytest.raises(g.GcfgError):
+        g.solve8(W, H, pos, neg, dirs8=d2)
+
+
 def test_n8_api_errors(g):
     W, H = 20, 20
     z = np.zeros((H, W), np.int32)

@@ -124,3 +124,22 @@ def test_budget_refusal_and_eviction(g):
     r = _drain(srv, 1)[rid]
     assert r["status"] == g.GCFG_ERR_STATE  # the first session was evicted
     srv.close()
+
+
+def test_edit_budget(g):
+    """A session takes edits while the sum of delta x strokes stays within
+    2^29 (the int32 excess bound); the edit past it fails, the session stays
+    usable."""
+    W, H = 130, 70
+    img, seeds = _req(g, W, H, 3)
+    srv = g.Server(1 << 30)
+    rid, sess, _ = srv.submit_image(img, seeds)
+    assert _drain(srv, 1)[rid]["status"] == 0
+    stroke = [[10, 10, 16, 16, 1]]
+    rid, _ = srv.submit_edit(sess, stroke * 4, (1 << 27) - 1, shape=(H, W))
+    assert _drain(srv, 1)[rid]["status"] == 0
+    rid, _ = srv.submit_edit(sess, stroke, 8, shape=(H, W))
+    assert _drain(srv, 1)[rid]["status"] == g.GCFG_ERR_INVALID
+    rid, _ = srv.submit_edit(sess, stroke, 4, shape=(H, W))
+    assert _drain(srv, 1)[rid]["status"] == 0
+    srv.close()

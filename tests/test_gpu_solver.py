@@ -145,8 +145,38 @@ def test_negative_nlink_rejected(g):
     E[2][5, 5] = -3
     with pytest.raises(g.GcfgError):
         g.solve(W, H, pos, neg, edges=E)
-    with pytest.raises(g.GcfgError):  # |pos - neg| beyond 2^30 would leave int32 excess no headroom
-        g.solve(W, H, np.full((H, W), 1 << 30, np.int32), np.full((H, W), -8, np.int32), penalty=1)
+    with pytest.raises(g.GcfgError):  # |pos - neg| beyond 2^29 would leave int32 excess no headroom
+        g.solve(W, H, np.full((H, W), 1 << 29, np.int32), np.full((H, W), -8, np.int32), penalty=1)
+
+
+@pytest.mark.parametrize("kernels", [1, 0])
+def test_input_range_limits(g, ora, kernels):
+    """At the documented limits (|pos - neg| = 2^29, n-links 2^28 - 1) the
+    excess of a node can approach 2^31 during the solve; results stay exact.
+    One step past either limit is rejected."""
+    o = g.default_options()
+    o.kernels = kernels
+    rng = np.random.default_rng(77)
+    W, H = 70, 40
+    big = 1 << 29
+    pos = np.where(rng.random((H, W)) < 0.5, big, 0).astype(np.int32)
+    neg = np.where(pos == 0, big, 0).astype(np.int32)
+    E = tuple(np.where(rng.random((H, W)) < 0.7, (1 << 28) - 1, rng.integers(0, 1000, (H, W))).astype(np.int32)
+              for _ in range(4))
+    flow, label, _ = g.solve(W, H, pos, neg, edges=E, options=o)
+    rflow, rlabel = ora.maxflow(W, H, pos, neg, edges=E)
+    assert flow == rflow and np.array_equal(label, rlabel)
+    with pytest.raises(g.GcfgError):
+        p2 = pos.copy()
+        p2[5, 5], neg[5, 5] = big, -1
+        g.solve(W, H, p2, neg, edges=E, options=o)
+    neg[5, 5] = 0
+    E2 = tuple(e.copy() for e in E)
+    E2[1][7, 9] = 1 << 28
+    with pytest.raises(g.GcfgError):
+        g.solve(W, H, pos, neg, edges=E2, options=o)
+    with pytest.raises(g.GcfgError):
+        g.solve(W, H, pos, neg, penalty=1 << 28, options=o)
 
 
 @pytest.mark.parametrize("kernels", [1, 0])
