@@ -1,5 +1,6 @@
 // C ABI: plain graph (GraphCut.cu GC_*), batched solve, options and errors.
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -36,6 +37,11 @@ struct gcfg_batch {
     std::vector<int> slots;
     int32_t *d_labels = nullptr;  // internal label arena
     std::vector<size_t> label_off;
+    // Packed labels (gcfg_batch_solve_bits): device words of every instance
+    // and their pinned host copy, allocated on first use.
+    uint64_t *d_bits = nullptr, *h_bits = nullptr;
+    std::vector<size_t> bits_off;
+    size_t bits_words = 0;
     std::vector<InstState> states;
     gcfg::StreamJoin join;
 };
@@ -394,14 +400,19 @@ int gcfg_batch_create(int32_t count, const gcfg_instance *inst, const gcfg_optio
     return GCFG_OK;
 }
 
-static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_stats *stats) {
+// Cold solve of every instance. Labels go to d_targets (device int32), or
+// with bits to the packed arena, whose host copy is queued before the final
+// state copy so one synchronisation covers both; otherwise to the internal
+// int32 arena.
+static int batch_run(gcfg_batch *b, int32_t **d_targets, bool bits, int64_t *flows, gcfg_stats *stats) {
     const double t0 = now_ms();
     const int n = (int)b->slots.size();
     for (int i = 0; i < n; ++i) {
         const gcfg_instance &in = b->inst[i];
-        int32_t *lab = d_targets ? d_targets[i] : b->d_labels + b->label_off[i];
+        int32_t *lab = bits ? nullptr : d_targets ? d_targets[i] : b->d_labels + b->label_off[i];
         b->eng.set_inputs(b->slots[i], in.d_pos, in.d_neg, in.d_up, in.d_down, in.d_left, in.d_right, in.penalty,
                           lab);
+        b->eng.set_label_bits(b->slots[i], bits ? b->d_bits + b->bits_off[i] : nullptr);
     }
     b->eng.reset_profile();
     const int64_t steps0 = b->eng.steps_run;
@@ -409,6 +420,12 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, int64_t *flows, gcfg_st
     if (rc == GCFG_OK) rc = b->eng.build(b->slots);
     if (rc == GCFG_OK) rc = b->eng.run(b->eng.opt.max_steps);
     if (rc == GCFG_OK) rc = b->join.after(b->eng.stream());
+    if (rc == GCFG_OK && bits &&
+        hipMemcpyAsync(b->h_bits, b->d_bits, b->bits_words * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                       b->eng.stream()) != hipSuccess) {
+        set_error("gcfg_batch_solve_bits: label copy failed");
+        rc = GCFG_ERR_HIP;
+    }
     // Instance slots were assigned 0..n-1 in order at creation: one copy.
     // (Copying them with every completion poll instead cost a one-off stall
     // of several ms a few solves into a process.)
@@ -453,13 +470,13 @@ int gcfg_batch_solve_device(gcfg_batch *b, int32_t **d_labels, int64_t *flows, g
         set_error("gcfg_batch_solve_device: null argument");
         return GCFG_ERR_INVALID;
     }
-    return batch_run(b, d_labels, flows, stats);
+    return batch_run(b, d_labels, false, flows, stats);
 }
 
 int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_stats *stats) {
     if (!b) return GCFG_ERR_INVALID;
     const double t0 = now_ms();
-    int rc = batch_run(b, nullptr, flows, stats);
+    int rc = batch_run(b, nullptr, false, flows, stats);
     if (rc != GCFG_OK) return rc;
     if (h_labels) {
         for (size_t i = 0; i < b->inst.size(); ++i) {
@@ -474,9 +491,40 @@ int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_sta
     return GCFG_OK;
 }
 
+int gcfg_batch_solve_bits(gcfg_batch *b, const uint64_t **h_bits, int64_t *flows, gcfg_stats *stats) {
+    if (!b) return GCFG_ERR_INVALID;
+    const double t0 = now_ms();
+    if (!b->d_bits) {
+        size_t words = 0;
+        b->bits_off.clear();
+        for (const gcfg_instance &in : b->inst) {
+            b->bits_off.push_back(words);
+            words += Engine::label_words(in.width, in.height);
+        }
+        if (hipMalloc((void **)&b->d_bits, std::max<size_t>(words, 1) * sizeof(uint64_t)) != hipSuccess ||
+            hipHostMalloc((void **)&b->h_bits, std::max<size_t>(words, 1) * sizeof(uint64_t),
+                          hipHostMallocDefault) != hipSuccess) {
+            if (b->d_bits) (void)hipFree(b->d_bits);
+            b->d_bits = nullptr;
+            b->h_bits = nullptr;
+            set_error("gcfg_batch_solve_bits: packed label allocation failed");
+            return GCFG_ERR_NOMEM;
+        }
+        b->bits_words = words;
+    }
+    const int rc = batch_run(b, nullptr, true, flows, stats);
+    if (rc != GCFG_OK) return rc;
+    if (h_bits)
+        for (size_t i = 0; i < b->inst.size(); ++i) h_bits[i] = b->h_bits + b->bits_off[i];
+    if (stats) stats->ms = now_ms() - t0;
+    return GCFG_OK;
+}
+
 int gcfg_batch_destroy(gcfg_batch *b) {
     if (!b) return GCFG_OK;
     b->eng.sync();
+    if (b->d_bits) (void)hipFree(b->d_bits);
+    if (b->h_bits) (void)hipHostFree(b->h_bits);
     if (b->d_labels) (void)hipFree(b->d_labels);
     delete b;
     return GCFG_OK;
@@ -509,7 +557,7 @@ int gcfg_batch_autotune(gcfg_batch *b, int32_t trials, gcfg_options *best, doubl
     auto solve = [&](const gcfg_options &o) -> double {
         b->eng.opt = o;
         const double t0 = now_ms();
-        if (batch_run(b, nullptr, nullptr, nullptr) != GCFG_OK) return -1.0;
+        if (batch_run(b, nullptr, false, nullptr, nullptr) != GCFG_OK) return -1.0;
         return now_ms() - t0;
     };
     gcfg_options win;

@@ -319,7 +319,7 @@ int Engine::build(const std::vector<int> &slots) {
             set_error("8-neighbourhood: per-arc capacities need all eight direction planes");
             return GCFG_ERR_STATE;
         }
-        if (!d.label_out) {
+        if (!d.label_out && !d.label_bits) {
             set_error("label buffer not set");
             return GCFG_ERR_STATE;
         }

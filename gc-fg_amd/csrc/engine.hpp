@@ -76,6 +76,11 @@ struct InstDesc {
     // wrote every capacity and pushes update the neighbour's directly.
     int32_t lazy;
     int32_t pad_;
+    // Packed labels (device, may be null): row Y of the image is tiles_x
+    // 64-bit words at label_bits + Y * tiles_x, bit x of word k = label of
+    // pixel (64 k + x, Y); bits past W are 0. Written with, or instead of,
+    // label_out.
+    uint64_t *label_bits;
 };
 
 struct InstState {
@@ -225,6 +230,10 @@ class Engine {
                      const int32_t *down, const int32_t *left, const int32_t *right, int32_t penalty,
                      int32_t *label_out);
     void remove_instance(int slot);
+    // Packed label destination of an instance (see InstDesc::label_bits); null: none.
+    void set_label_bits(int slot, uint64_t *bits) { h_desc_[slot].label_bits = bits; }
+    // Words of an instance's packed label map.
+    static size_t label_words(int W, int H) { return (size_t)((W + TW - 1) / TW) * (size_t)H; }
     // diag: the four diagonal planes (up-left, up-right, down-left,
     // down-right) of an 8-neighbourhood instance with per-arc capacities.
     int set_inputs(int slot, const int32_t *pos, const int32_t *neg, const int32_t *up, const int32_t *down,

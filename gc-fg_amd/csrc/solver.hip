@@ -39,6 +39,33 @@ __device__ __forceinline__ int ldg_in(const int32_t *p, size_t i) {
 __device__ __forceinline__ void stg_out(int32_t *p, size_t i, int v) {
     ((__attribute__((address_space(1))) int32_t *)p)[i] = v;
 }
+__device__ __forceinline__ void stg_out64(uint64_t *p, size_t i, uint64_t v) {
+    ((__attribute__((address_space(1))) uint64_t *)p)[i] = v;
+}
+
+// FIN of the workgroup kernels (k_gr, k_gr8): labels of the RPT rows that
+// one wavefront owns (lane = column) from the final distances, as int32 per
+// pixel and/or one packed word per row (InstDesc::label_bits), and the
+// wavefront's share of the tile's stuck excess.
+__device__ __forceinline__ int64_t fin_rows_wg(const KParams &p, const InstDesc &D, const TileInfo &ti,
+                                               size_t base, int wave, int lane) {
+    int64_t sx = 0;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        const int y = wave * RPT + k;
+        const int X = ti.tx * TW + lane, Y = ti.ty * TH + y;
+        const size_t g = base + y * TW + lane;
+        const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
+        const int lab = (in && p.dist[g] >= DINF) ? 1 : 0;
+        if (D.label_out && X < D.W && Y < D.H) stg_out(D.label_out, (size_t)Y * D.W + X, lab);
+        if (in) sx += max(p.exc[g], 0);
+        if (D.label_bits) {
+            const uint64_t b = __ballot(lab);
+            if (lane == 0 && Y < D.H) stg_out64(D.label_bits, (size_t)Y * D.tiles_x + ti.tx, b);
+        }
+    }
+    return sx;
+}
 
 // Residual capacities {left, right, up, down} of pixel (X, Y) as the build
 // defines them from the inputs: zero outside the interior and on arcs into the
@@ -760,20 +787,7 @@ __global__ __launch_bounds__(NT) void k_gr(KParams p, int parity) {
         const size_t base = (size_t)t * TN;
 
         if (op == OP_FIN) {
-            int64_t sx = 0;
-            int32_t *lab = D.label_out;
-            const int W = D.W, H = D.H;
-#pragma unroll
-            for (int k = 0; k < RPT; ++k) {
-                const int y = wave * RPT + k;
-                const int X = ti.tx * TW + lane, Y = ti.ty * TH + y;
-                if (X < W && Y < H) {
-                    const size_t g = base + y * TW + lane;
-                    const bool in = X > 0 && Y > 0 && X < W - 1 && Y < H - 1;
-                    stg_out(lab, (size_t)Y * W + X, (in && p.dist[g] >= DINF) ? 1 : 0);
-                    if (in) sx += max(p.exc[g], 0);
-                }
-            }
+            const int64_t sx = fin_rows_wg(p, D, ti, base, wave, lane);
             const int64_t tot = block_sum64(sx, red);
             if (tid == 0) p.tsum[2 * (size_t)t + 1] = tot;
             __syncthreads();
@@ -1132,20 +1146,7 @@ __global__ __launch_bounds__(NT) void k_gr8(KParams p, int parity) {
         const size_t base = (size_t)t * TN;
 
         if (op == OP_FIN) {
-            int64_t sx = 0;
-            int32_t *lab = D.label_out;
-            const int W = D.W, H = D.H;
-#pragma unroll
-            for (int k = 0; k < RPT; ++k) {
-                const int y = wave * RPT + k;
-                const int X = ti.tx * TW + lane, Y = ti.ty * TH + y;
-                if (X < W && Y < H) {
-                    const size_t g = base + y * TW + lane;
-                    const bool in = X > 0 && Y > 0 && X < W - 1 && Y < H - 1;
-                    stg_out(lab, (size_t)Y * W + X, (in && p.dist[g] >= DINF) ? 1 : 0);
-                    if (in) sx += max(p.exc[g], 0);
-                }
-            }
+            const int64_t sx = fin_rows_wg(p, D, ti, base, wave, lane);
             const int64_t tot = block_sum64(sx, red);
             if (tid == 0) p.tsum[2 * (size_t)t + 1] = tot;
             __syncthreads();
@@ -1637,15 +1638,20 @@ __device__ __forceinline__ bool tile_bfs(int (&d)[TH], Masks &m, int hU, int hD,
 // 0 elsewhere, border included.
 __device__ __forceinline__ void write_labels(const InstDesc &D, int tx, int ty, int lane, const int (&d)[TH]) {
     const int X = tx * TW + lane;
-    if (X >= D.W) return;
+    const bool colin = X > 0 && X < D.W - 1;
+    uint64_t word = 0;  // lane y: the packed word of row y
 #pragma unroll
     for (int y = 0; y < TH; ++y) {
         const int Y = ty * TH + y;
-        if (Y < D.H) {
-            const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
-            stg_out(D.label_out, (size_t)Y * D.W + X, (in && d[y] >= DINF) ? 1 : 0);
+        const bool lab = colin && Y > 0 && Y < D.H - 1 && d[y] >= DINF;
+        if (D.label_out && X < D.W && Y < D.H) stg_out(D.label_out, (size_t)Y * D.W + X, lab ? 1 : 0);
+        if (D.label_bits) {
+            const uint64_t b = __ballot(lab);
+            if (lane == y) word = b;
         }
     }
+    if (D.label_bits && lane < TH && ty * TH + lane < D.H)
+        stg_out64(D.label_bits, (size_t)(ty * TH + lane) * D.tiles_x + tx, word);
 }
 
 // Anchor (top-left tile) of the global-relabel block holding tile (x, y).

@@ -166,6 +166,23 @@ def tune_lookup(path: str, key: str):
     return o if rc == GCFG_OK else None
 
 
+def unpack_bits(words: np.ndarray, width: int, height: int) -> np.ndarray:
+    """Packed label map (rows of ceil(W/64) uint64 words, bit x of word k =
+    pixel 64 k + x) -> int32 labels [H, W]."""
+    w = np.ascontiguousarray(words, dtype=np.uint64).reshape(height, -1)
+    bits = np.unpackbits(w.view(np.uint8), axis=1, bitorder="little")
+    return bits[:, :width].astype(np.int32)
+
+
+def pack_bits(label: np.ndarray) -> np.ndarray:
+    """int32 labels [H, W] -> packed words [H, ceil(W/64)] (inverse of unpack_bits)."""
+    H, W = label.shape
+    k = (W + 63) // 64
+    padded = np.zeros((H, 64 * k), np.uint8)
+    padded[:, :W] = label != 0
+    return np.packbits(padded, axis=1, bitorder="little").view(np.uint64).reshape(H, k)
+
+
 def options_dict(o: Options) -> dict:
     return {k: getattr(o, k) for k, _ in Options._fields_}
 
@@ -613,6 +630,23 @@ class Batch:
         _check(lib().gcfg_batch_solve_device(_VP(self.h), ptrs, _VP(flows.ctypes.data), ctypes.byref(st)),
                "gcfg_batch_solve_device")
         return flows, None, st
+
+    def solve_bits(self, copy: bool = True):
+        """Cold solve with the labels on the host packed one bit per pixel
+        (gcfg_batch_solve_bits). Returns (flows, [uint64 array [H, ceil(W/64)]
+        per instance], stats); with copy=False the arrays view the batch's
+        pinned buffer, valid until the next solve."""
+        n = len(self.items)
+        flows = np.zeros(n, np.int64)
+        st = Stats()
+        ptrs = (ctypes.POINTER(ctypes.c_uint64) * n)()
+        _check(lib().gcfg_batch_solve_bits(_VP(self.h), ptrs, _VP(flows.ctypes.data), ctypes.byref(st)),
+               "gcfg_batch_solve_bits")
+        out = []
+        for it, p in zip(self.items, ptrs):
+            a = np.ctypeslib.as_array(p, shape=(it.height, (it.width + 63) // 64))
+            out.append(a.copy() if copy else a)
+        return flows, out, st
 
     def autotune(self, trials: int = 2):
         """Pick solver options by timing cold solves of the whole batch."""

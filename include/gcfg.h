@@ -217,6 +217,14 @@ int gcfg_batch_set_stream(gcfg_batch *b, void *hip_stream);
 int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_stats *stats);
 /* Cold solve with labels written to caller-owned device buffers (d_labels[i]: W_i*H_i). */
 int gcfg_batch_solve_device(gcfg_batch *b, int32_t **d_labels, int64_t *flows, gcfg_stats *stats);
+/* Cold solve with the labels on the host packed one bit per pixel: h_bits[i]
+ * receives a pointer to instance i's map in pinned host memory owned by the
+ * batch, valid until the next solve or destroy. Row y of instance i is the
+ * ceil(W_i/64) words h_bits[i][y * ceil(W_i/64) + k], bit x of word k being
+ * the label of pixel (64 k + x, y); bits past W_i are 0. The GC_Optimize
+ * result (GraphCut.cu:528-529: int32 labels copied to the host) in 1/32 of
+ * the bytes: one device-to-host copy for the whole batch. */
+int gcfg_batch_solve_bits(gcfg_batch *b, const uint64_t **h_bits, int64_t *flows, gcfg_stats *stats);
 int gcfg_batch_destroy(gcfg_batch *b);
 
 /* ---- File formats of the reference's drivers (host only) ----

@@ -19,7 +19,8 @@ def _ellipse_links(g, W, H, seed):
     return links
 
 
-@pytest.mark.parametrize("cfg", [{}, dict(gr_cap=16, gr_block_w=1, gr_block_h=4), dict(gr_cap=8, kernels=0)])
+@pytest.mark.parametrize("cfg", [{}, dict(gr_cap=16, gr_block_w=1, gr_block_h=4), dict(gr_cap=8, kernels=0),
+                                 dict(gr_cap=0), dict(gr_cap=0, gr_block_w=2, gr_block_h=2)])
 def test_mixed_batch_matches_oracle(g, ora, cfg):
     rng = np.random.default_rng(12)
     items, refs = [], []
@@ -48,6 +49,14 @@ def test_mixed_batch_matches_oracle(g, ora, cfg):
     assert np.array_equal(flows_d, flows)
     for k, (rf, rl) in enumerate(refs):
         assert np.array_equal(b.device_labels(k), rl)
+    # Packed labels on the host: the same maps, bits past W zero.
+    for rep in range(2):
+        flows_b, words, _ = b.solve_bits()
+        assert np.array_equal(flows_b, flows), rep
+        for k, (rf, rl) in enumerate(refs):
+            W, H = shapes[k]
+            assert words[k].shape == (H, (W + 63) // 64)
+            assert np.array_equal(words[k], g.pack_bits(rl)), (rep, k)
     b.close()
 
 

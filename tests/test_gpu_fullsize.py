@@ -119,9 +119,13 @@ def test_c3_batch_every_instance_vs_oracle(g, c3_batch, opts):
         setattr(o, k, v)
     items = [g.BatchItem(W, H, L[0], L[1], edges=tuple(L[2:6])) for W, H, L in host]
     b = g.Batch(items, o)
-    for rep in range(2):
+    for rep in range(3):
         print(f"[fullsize] batch {opts} solve {rep}", flush=True)
-        flows, labels, _ = b.solve(labels_to_host=True)
+        if rep < 2:
+            flows, labels, _ = b.solve(labels_to_host=True)
+        else:  # labels packed one bit per pixel on the host
+            flows, words, _ = b.solve_bits()
+            labels = [g.unpack_bits(w, W, H) for w, (W, H, _) in zip(words, host)]
         for i, (rf, rl) in enumerate(refs):
             assert flows[i] == rf, (opts, rep, i, int(flows[i]), rf)
             assert np.array_equal(labels[i], rl), (opts, rep, i, int((labels[i] != rl).sum()))
