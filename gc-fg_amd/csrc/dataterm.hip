@@ -1,6 +1,6 @@
 // Colour data-term kernels and the DataTerm.h C ABI (DataTerm.cpp:12-165,
 // DataTermKernels.cu:19-150 re-specified as histogram / -log-likelihood /
-// contrast-weighted n-links; formulas in dataterm.hpp).
+// contrast-weighted n-links; formulas and pass structure in dataterm.hpp).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -12,59 +12,73 @@
 namespace gcfg {
 
 constexpr int DT_NT = 256;
-constexpr int MAX_LAMBDA = 4096;  // thresholds live in LDS: (MAX_LAMBDA + 1) * 4 bytes
 constexpr int MAX_SQDIFF = 3 * 255 * 255;
+constexpr int DT_ROUNDS = DT_CHUNK / (4 * DT_NT);  // quads of 4 pixels per lane in one chunk
 
-__global__ void k_dt_zero(DtStats *s) {
-    int *p = reinterpret_cast<int *>(s);
-    for (int i = threadIdx.x; i < (int)(sizeof(DtStats) / sizeof(int)); i += blockDim.x) p[i] = 0;
-}
+struct Rgb {
+    int c0, c1, c2;
+};
 
-__device__ __forceinline__ int sqdiff(const uint8_t *a, const uint8_t *b) {
-    const int d0 = (int)a[0] - b[0], d1 = (int)a[1] - b[1], d2 = (int)a[2] - b[2];
+__device__ __forceinline__ int sq(const Rgb &a, const Rgb &b) {
+    const int d0 = a.c0 - b.c0, d1 = a.c1 - b.c1, d2 = a.c2 - b.c2;
     return d0 * d0 + d1 * d1 + d2 * d2;
 }
 
-// One pass over the image: seed histograms (LDS int atomics) and the exact sum
-// of squared colour differences over right and down neighbour pairs.
-__global__ __launch_bounds__(DT_NT) void k_dt_stats(const uint8_t *img, const uint8_t *seeds, int W, int H,
-                                                    DtStats *out) {
-    __shared__ int hist[2 * 3 * 16 + 2];
-    __shared__ unsigned long long red[DT_NT / 64];
-    for (int i = threadIdx.x; i < 2 * 3 * 16 + 2; i += DT_NT) hist[i] = 0;
-    __syncthreads();
-    const int64_t n = (int64_t)W * H;
-    unsigned long long bs = 0;
-    for (int64_t i = blockIdx.x * (int64_t)DT_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * DT_NT) {
-        const int x = (int)(i % W), y = (int)(i / W);
-        const uint8_t *a = img + 3 * i;
-        if (x + 1 < W) bs += sqdiff(a, a + 3);
-        if (y + 1 < H) bs += sqdiff(a, a + 3 * (int64_t)W);
-        const int s = seeds[i];
-        if (s == GCFG_SEED_FG || s == GCFG_SEED_BG) {
-            const int k = s - 1;
-            atomicAdd(&hist[k * 48 + 0 * 16 + (a[0] >> 4)], 1);
-            atomicAdd(&hist[k * 48 + 1 * 16 + (a[1] >> 4)], 1);
-            atomicAdd(&hist[k * 48 + 2 * 16 + (a[2] >> 4)], 1);
-            atomicAdd(&hist[96 + k], 1);
+// Byte b of a little-endian word array (b is a constant after unrolling).
+template <int N>
+__device__ __forceinline__ int byte_at(const uint32_t (&w)[N], int b) {
+    return (int)((w[b >> 2] >> (8 * (b & 3))) & 255u);
+}
+template <int N>
+__device__ __forceinline__ Rgb px_at(const uint32_t (&w)[N], int b) {
+    return Rgb{byte_at(w, b), byte_at(w, b + 1), byte_at(w, b + 2)};
+}
+
+// Bytes [o, o + 4N) of an image of `total` bytes as N words (o 4-byte
+// aligned): dword loads when the range is inside the image, otherwise byte
+// loads with the bytes outside it read as 0 (first and last lanes only).
+template <int N>
+__device__ __forceinline__ void load_words(const uint8_t *img, int64_t o, int64_t total, uint32_t (&w)[N]) {
+    if (o >= 0 && o + 4 * N <= total) {
+        const __attribute__((address_space(1))) uint32_t *p =
+            (const __attribute__((address_space(1))) uint32_t *)(img + o);
+#pragma unroll
+        for (int k = 0; k < N; ++k) w[k] = p[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int64_t q = o + 4 * k + b;
+                if (q >= 0 && q < total) v |= (uint32_t)img[q] << (8 * b);
+            }
+            w[k] = v;
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned lo = __shfl_xor((unsigned)bs, o, 64), hi = __shfl_xor((unsigned)(bs >> 32), o, 64);
-        bs += ((unsigned long long)hi << 32) | lo;
+}
+
+// 12 bytes (4 pixels) at any byte offset o: four aligned words realigned by
+// o & 3, which is uniform across an image's lanes for a row offset.
+__device__ __forceinline__ void load_px12(const uint8_t *img, int64_t o, int64_t total, uint32_t (&r)[3]) {
+    const int64_t a = o & ~(int64_t)3;
+    const unsigned s = (unsigned)(o - a);
+    uint32_t w[4];
+    load_words<4>(img, a, total, w);
+    r[0] = __builtin_amdgcn_alignbyte(w[1], w[0], s);
+    r[1] = __builtin_amdgcn_alignbyte(w[2], w[1], s);
+    r[2] = __builtin_amdgcn_alignbyte(w[3], w[2], s);
+}
+
+// The job of a chunk: the single job, or the job table through the chunk map.
+__device__ __forceinline__ void dt_job(const DtLaunch &L, int b, DtJob &J, int &c) {
+    if (L.njobs == 0) {
+        J = L.one;
+        c = b;
+        return;
     }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bs;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int w = 0; w < DT_NT / 64; ++w) t += red[w];
-        if (t) atomicAdd(&out->beta_sum, t);
-    }
-    int *oh = &out->hist[0][0][0];
-    for (int i = threadIdx.x; i < 96; i += DT_NT)
-        if (hist[i]) atomicAdd(&oh[i], hist[i]);
-    if (threadIdx.x < 2 && hist[96 + threadIdx.x]) atomicAdd(&out->counts[threadIdx.x], hist[96 + threadIdx.x]);
+    J = L.jobs[L.cmap[b]];
+    c = b - J.chunk0;
 }
 
 // n-link weight from the integer thresholds: w(s) = #{k in 1..lambda : s <= thr[k]}
@@ -78,24 +92,147 @@ __device__ __forceinline__ int nlink_weight(int s, const int32_t *thr, int lambd
     return lo;
 }
 
-__global__ __launch_bounds__(DT_NT) void k_dt_links(const uint8_t *img, const int32_t *lut, int lambda, int W,
-                                                    int H, int32_t *pos, int32_t *neg, int32_t *up, int32_t *down,
-                                                    int32_t *left, int32_t *right) {
-    extern __shared__ int32_t thr[];
-    const int32_t *gthr = lut + 2 * 4096;
-    for (int i = threadIdx.x; i <= lambda; i += DT_NT) thr[i] = gthr[i];
+// Stats pass: seed histograms and the chunk's sum of squared colour
+// differences over right and down pairs (each pair once).
+__global__ __launch_bounds__(DT_NT) void k_dt_stats(DtLaunch L) {
+    __shared__ int hist[DT_HIST];
+    __shared__ unsigned long long red[DT_NT / 64];
+    DtJob J;
+    int c;
+    dt_job(L, blockIdx.x, J, c);
+    for (int i = threadIdx.x; i < DT_HIST; i += DT_NT) hist[i] = 0;
     __syncthreads();
-    const int64_t n = (int64_t)W * H;
-    for (int64_t i = blockIdx.x * (int64_t)DT_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * DT_NT) {
-        const int x = (int)(i % W), y = (int)(i / W);
-        const uint8_t *a = img + 3 * i;
-        const int idx = ((a[0] >> 4) << 8) | ((a[1] >> 4) << 4) | (a[2] >> 4);
-        pos[i] = lut[4096 + idx];  // cost of the background label
-        neg[i] = lut[idx];         // cost of the foreground label
-        right[i] = x + 1 < W ? nlink_weight(sqdiff(a, a + 3), thr, lambda) : 0;
-        left[i] = x > 0 ? nlink_weight(sqdiff(a, a - 3), thr, lambda) : 0;
-        down[i] = y + 1 < H ? nlink_weight(sqdiff(a, a + 3 * (int64_t)W), thr, lambda) : 0;
-        up[i] = y > 0 ? nlink_weight(sqdiff(a, a - 3 * (int64_t)W), thr, lambda) : 0;
+    const int W = J.W, H = J.H, n = J.W * J.H;
+    const int64_t total = 3 * (int64_t)n;
+    unsigned long long bs = 0;
+    int seen = 0;
+    for (int r = 0; r < DT_ROUNDS; ++r) {
+        const int i0 = c * DT_CHUNK + (r * DT_NT + (int)threadIdx.x) * 4;
+        if (i0 >= n) break;
+        uint32_t w[5], b[3];
+        load_words<5>(J.img, 3 * (int64_t)i0 - 4, total, w);  // p(-1) .. p(4)
+        load_px12(J.img, 3 * ((int64_t)i0 + W), total, b);     // the four pixels below
+        uint32_t sd = 0;
+        if (J.seeds) {
+            if (i0 + 4 <= n) {
+                sd = *(const __attribute__((address_space(1))) uint32_t *)(J.seeds + i0);
+            } else {
+                for (int k = 0; i0 + k < n; ++k) sd |= (uint32_t)J.seeds[i0 + k] << (8 * k);
+            }
+        }
+        int x = i0 % W, y = i0 / W;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (i0 + k < n) {
+                const Rgb pk = px_at(w, 4 + 3 * k);
+                if (x + 1 < W) bs += (unsigned)sq(pk, px_at(w, 7 + 3 * k));
+                if (y + 1 < H) bs += (unsigned)sq(pk, px_at(b, 3 * k));
+                const int s = (int)((sd >> (8 * k)) & 255u);
+                if (s == GCFG_SEED_FG || s == GCFG_SEED_BG) {
+                    const int cl = s - 1;
+                    atomicAdd(&hist[cl * 48 + 0 * 16 + (pk.c0 >> 4)], 1);
+                    atomicAdd(&hist[cl * 48 + 1 * 16 + (pk.c1 >> 4)], 1);
+                    atomicAdd(&hist[cl * 48 + 2 * 16 + (pk.c2 >> 4)], 1);
+                    atomicAdd(&hist[96 + cl], 1);
+                    seen = 1;
+                }
+            }
+            if (++x == W) {
+                x = 0;
+                ++y;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned lo = __shfl_xor((unsigned)bs, o, 64), hi = __shfl_xor((unsigned)(bs >> 32), o, 64);
+        bs += ((unsigned long long)hi << 32) | lo;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bs;
+    const int any_seed = __syncthreads_or(seen);
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int k = 0; k < DT_NT / 64; ++k) t += red[k];
+        J.beta_part[c] = t;
+    }
+    // Only the few chunks that hold seed pixels touch the image's counters.
+    if (any_seed)
+        for (int i = threadIdx.x; i < DT_HIST; i += DT_NT)
+            if (hist[i]) atomicAdd(&J.hist_acc[i], hist[i]);
+}
+
+// n-link weight of a squared difference, evaluated as the oracle does.
+__device__ __forceinline__ long long nlink_exact(int s, int lambda, double beta) {
+    return beta > 0.0 ? llround((double)lambda * exp(-(double)s / (2.0 * beta))) : (long long)lambda;
+}
+
+// Table pass (one workgroup per image): reduce the partials, keep the
+// statistics, build the t-link costs of the 16^3 colour bins of both classes
+// and the n-link thresholds thr[k] = largest squared difference s with
+// w(s) >= k (thr[0] = +inf, -1 where no s qualifies).
+__global__ __launch_bounds__(DT_NT) void k_dt_table(DtLaunch L) {
+    __shared__ int hist[DT_HIST];
+    __shared__ double rr[2][3][16];
+    __shared__ unsigned long long red[DT_NT / 64];
+    const DtJob J = L.njobs ? L.jobs[blockIdx.x] : L.one;
+    const int nch = dt_chunks(J.W, J.H);
+    unsigned long long bs = 0;
+    for (int k = threadIdx.x; k < nch; k += DT_NT) bs += J.beta_part[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned lo = __shfl_xor((unsigned)bs, o, 64), hi = __shfl_xor((unsigned)(bs >> 32), o, 64);
+        bs += ((unsigned long long)hi << 32) | lo;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bs;
+    for (int i = threadIdx.x; i < DT_HIST; i += DT_NT) {
+        const int acc = J.hist_acc[i];
+        J.hist_acc[i] = 0;  // ready for the next compute
+        hist[i] = J.frozen ? (i < 96 ? (&J.frozen->hist[0][0][0])[i] : J.frozen->counts[i - 96]) : acc;
+    }
+    __syncthreads();
+    unsigned long long tot = 0;
+    for (int k = 0; k < DT_NT / 64; ++k) tot += red[k];
+    for (int i = threadIdx.x; i < DT_HIST; i += DT_NT) {
+        if (i < 96) (&J.stats->hist[0][0][0])[i] = hist[i];
+        else J.stats->counts[i - 96] = hist[i];
+    }
+    if (threadIdx.x == 0) J.stats->beta_sum = tot;
+    if (threadIdx.x < 96) {
+        const int k = threadIdx.x / 48;
+        rr[k][(threadIdx.x / 16) % 3][threadIdx.x % 16] =
+            (double)(hist[threadIdx.x] + 1) / (double)(hist[96 + k] + 16);
+    }
+    __syncthreads();
+    if (J.model == GCFG_DT_MODEL_HISTOGRAM) {
+        for (int e = threadIdx.x; e < DT_LUT; e += DT_NT) {
+            const int k = e >> 12, idx = e & 4095;
+            double p = 1.0;
+            p *= rr[k][0][(idx >> 8) & 15];
+            p *= rr[k][1][(idx >> 4) & 15];
+            p *= rr[k][2][idx & 15];
+            J.tlut[e] = (uint16_t)llround(64.0 * -log(p));
+        }
+    }
+    const int64_t pairs = (int64_t)(J.W - 1) * J.H + (int64_t)J.W * (J.H - 1);
+    const double beta = pairs > 0 ? (double)tot / (double)pairs : 0.0;
+    const int lambda = J.lambda;
+    for (int k = threadIdx.x; k <= lambda; k += DT_NT) {
+        int t;
+        if (k == 0) {
+            t = 0x7fffffff;
+        } else if (nlink_exact(MAX_SQDIFF, lambda, beta) >= k) {
+            t = MAX_SQDIFF;
+        } else if (nlink_exact(0, lambda, beta) < k) {
+            t = -1;
+        } else {
+            int lo = 0, hi = MAX_SQDIFF;  // w(lo) >= k > w(hi)
+            while (hi - lo > 1) {
+                const int mid = lo + (hi - lo) / 2;
+                if (nlink_exact(mid, lambda, beta) >= k) lo = mid; else hi = mid;
+            }
+            t = lo;
+        }
+        J.thr[k] = t;
     }
 }
 
@@ -116,13 +253,12 @@ __device__ __forceinline__ int isqrt24(int s) {
 // p(I) = sum over the seed list of g[floor(|I - s_j|)] for |I - s_j| < 101,
 // in f64 and in list order (the list index is uniform across the wavefront,
 // so the seed colours are scalar loads).
-__device__ __forceinline__ double kde_density(const uint8_t *a, const uint32_t *__restrict__ cols, int n,
+__device__ __forceinline__ double kde_density(const Rgb &a, const uint32_t *__restrict__ cols, int n,
                                               const float *g) {
-    const int c0 = a[0], c1 = a[1], c2 = a[2];
     double p = 0.0;
     for (int j = 0; j < n; ++j) {
         const uint32_t c = cols[j];
-        const int d0 = c0 - (int)(c >> 16), d1 = c1 - (int)((c >> 8) & 255u), d2 = c2 - (int)(c & 255u);
+        const int d0 = a.c0 - (int)(c >> 16), d1 = a.c1 - (int)((c >> 8) & 255u), d2 = a.c2 - (int)(c & 255u);
         const int s = d0 * d0 + d1 * d1 + d2 * d2;
         if (s < KDE_TAPS * KDE_TAPS) p += (double)g[isqrt24(s)];
     }
@@ -137,6 +273,93 @@ __device__ __forceinline__ int32_t trunc_sat(double v) {
     return (int32_t)v;
 }
 
+// Links pass: six planes for four pixels per lane. The weight of each
+// horizontal pair is computed once (left of pixel k = right of pixel k - 1).
+template <bool KDE>
+__global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
+    extern __shared__ int32_t thr[];
+    __shared__ uint32_t lut2[KDE ? 1 : DT_LUT / 2];  // u16 costs, two per word
+    __shared__ float g[KDE ? KDE_TAPS : 1];
+    DtJob J;
+    int c;
+    dt_job(L, blockIdx.x, J, c);
+    const int lambda = J.lambda;
+    for (int i = threadIdx.x; i <= lambda; i += DT_NT) thr[i] = J.thr[i];
+    if (KDE) {
+        for (int i = threadIdx.x; i < KDE_TAPS; i += DT_NT) g[i] = J.gauss[i];
+    } else {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(J.tlut);
+        for (int i = threadIdx.x; i < DT_LUT / 2; i += DT_NT) lut2[i] = src[i];
+    }
+    __syncthreads();
+    const int W = J.W, H = J.H, n = J.W * J.H;
+    const int64_t total = 3 * (int64_t)n;
+    for (int r = 0; r < DT_ROUNDS; ++r) {
+        const int i0 = c * DT_CHUNK + (r * DT_NT + (int)threadIdx.x) * 4;
+        if (i0 >= n) break;
+        uint32_t w[5], a[3], b[3];
+        load_words<5>(J.img, 3 * (int64_t)i0 - 4, total, w);  // p(-1) .. p(4)
+        load_px12(J.img, 3 * ((int64_t)i0 - W), total, a);     // above
+        load_px12(J.img, 3 * ((int64_t)i0 + W), total, b);     // below
+        int hw[5];  // weight of the pair (k - 1, k)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) hw[k] = nlink_weight(sq(px_at(w, 1 + 3 * k), px_at(w, 4 + 3 * k)), thr, lambda);
+        int vp[4], vn[4], vu[4], vd[4], vl[4], vr[4];
+        int x = i0 % W, y = i0 / W;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const Rgb pk = px_at(w, 4 + 3 * k);
+            if (KDE) {
+                const float pf = (float)kde_density(pk, J.kde, J.kde_n, g);
+                vp[k] = trunc_sat(300000.0 * (double)pf);
+                vn[k] = trunc_sat(300000.0 * (double)(1.0f - pf));
+            } else {
+                const int idx = ((pk.c0 >> 4) << 8) | ((pk.c1 >> 4) << 4) | (pk.c2 >> 4);
+                const int e0 = idx, e1 = 4096 + idx;
+                vn[k] = (int)((lut2[e0 >> 1] >> (16 * (e0 & 1))) & 0xffffu);  // cost of the foreground label
+                vp[k] = (int)((lut2[e1 >> 1] >> (16 * (e1 & 1))) & 0xffffu);  // cost of the background label
+            }
+            vl[k] = x > 0 ? hw[k] : 0;
+            vr[k] = x + 1 < W ? hw[k + 1] : 0;
+            vu[k] = y > 0 ? nlink_weight(sq(px_at(a, 3 * k), pk), thr, lambda) : 0;
+            vd[k] = y + 1 < H ? nlink_weight(sq(pk, px_at(b, 3 * k)), thr, lambda) : 0;
+            if (++x == W) {
+                x = 0;
+                ++y;
+            }
+        }
+        const int *vals[6] = {vp, vn, vu, vd, vl, vr};
+        if (J.aligned && i0 + 4 <= n) {
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+                *reinterpret_cast<int4 *>(J.out[j] + i0) = make_int4(vals[j][0], vals[j][1], vals[j][2], vals[j][3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 6; ++j)
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (i0 + k < n) J.out[j][i0 + k] = vals[j][k];
+        }
+    }
+}
+
+void dt_launch(const DtLaunch &L, int total_chunks, hipStream_t s) {
+    const int jobs = L.njobs ? L.njobs : 1;
+    const bool kde = L.njobs == 0 && L.one.model == GCFG_DT_MODEL_KDE;
+    hipLaunchKernelGGL(k_dt_stats, dim3(total_chunks), dim3(DT_NT), 0, s, L);
+    hipLaunchKernelGGL(k_dt_table, dim3(jobs), dim3(DT_NT), 0, s, L);
+    const size_t lds = (size_t)(L.lds_lambda + 1) * sizeof(int32_t);
+    if (kde)
+        hipLaunchKernelGGL(k_dt_links<true>, dim3(total_chunks), dim3(DT_NT), lds, s, L);
+    else
+        hipLaunchKernelGGL(k_dt_links<false>, dim3(total_chunks), dim3(DT_NT), lds, s, L);
+}
+
+__device__ __forceinline__ int sqdiff(const uint8_t *a, const uint8_t *b) {
+    const int d0 = (int)a[0] - b[0], d1 = (int)a[1] - b[1], d2 = (int)a[2] - b[2];
+    return d0 * d0 + d1 * d1 + d2 * d2;
+}
+
 // Pre-quantisation values in f64 on the device (verification output).
 // Kernel-density model (kde_cols != null): tpos = 300000 p, tneg = 300000 (1 - p).
 __global__ __launch_bounds__(DT_NT) void k_dt_unquantized(const uint8_t *img, const DtStats *st, double beta,
@@ -148,7 +371,7 @@ __global__ __launch_bounds__(DT_NT) void k_dt_unquantized(const uint8_t *img, co
         const int x = (int)(i % W), y = (int)(i / W);
         const uint8_t *a = img + 3 * i;
         if (kde_cols) {
-            const double p = kde_density(a, kde_cols, kde_n, gauss);
+            const double p = kde_density(Rgb{a[0], a[1], a[2]}, kde_cols, kde_n, gauss);
             tpos[i] = 300000.0 * p;
             tneg[i] = 300000.0 * (1.0 - p);
         } else {
@@ -181,34 +404,6 @@ __global__ void k_dt_kde_gather(const uint8_t *img, int W, int x0, int y0, int n
     out[t] = ((uint32_t)a[0] << 16) | ((uint32_t)a[1] << 8) | (uint32_t)a[2];
 }
 
-// t-links from the kernel density (K_CalcTerms, DataTermKernels.cu:61-83,
-// with p rounded to float as the reference's volume holds it), n-links as in
-// k_dt_links.
-__global__ __launch_bounds__(DT_NT) void k_dt_kde_links(const uint8_t *img, const uint32_t *__restrict__ cols,
-                                                        int ncols, const float *gauss, const int32_t *lut,
-                                                        int lambda, int W, int H, int32_t *pos, int32_t *neg,
-                                                        int32_t *up, int32_t *down, int32_t *left,
-                                                        int32_t *right) {
-    extern __shared__ int32_t thr[];
-    __shared__ float g[KDE_TAPS];
-    const int32_t *gthr = lut + 2 * 4096;
-    for (int i = threadIdx.x; i <= lambda; i += DT_NT) thr[i] = gthr[i];
-    for (int i = threadIdx.x; i < KDE_TAPS; i += DT_NT) g[i] = gauss[i];
-    __syncthreads();
-    const int64_t n = (int64_t)W * H;
-    for (int64_t i = blockIdx.x * (int64_t)DT_NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * DT_NT) {
-        const int x = (int)(i % W), y = (int)(i / W);
-        const uint8_t *a = img + 3 * i;
-        const float pf = (float)kde_density(a, cols, ncols, g);
-        pos[i] = trunc_sat(300000.0 * (double)pf);
-        neg[i] = trunc_sat(300000.0 * (double)(1.0f - pf));
-        right[i] = x + 1 < W ? nlink_weight(sqdiff(a, a + 3), thr, lambda) : 0;
-        left[i] = x > 0 ? nlink_weight(sqdiff(a, a - 3), thr, lambda) : 0;
-        down[i] = y + 1 < H ? nlink_weight(sqdiff(a, a + 3 * (int64_t)W), thr, lambda) : 0;
-        up[i] = y > 0 ? nlink_weight(sqdiff(a, a - 3 * (int64_t)W), thr, lambda) : 0;
-    }
-}
-
 __global__ void k_fill_rect_u8(uint8_t *map, int W, int H, int x0, int y0, int w, int h, uint8_t v) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < w * h; i += gridDim.x * blockDim.x) {
         const int x = x0 + i % w, y = y0 + i / w;
@@ -221,10 +416,17 @@ __global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v) {
         p[i] = v;
 }
 
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
 static int grid_for(int64_t n) {
     const int64_t g = (n + DT_NT - 1) / DT_NT;
     return (int)(g < 4096 ? (g > 0 ? g : 1) : 4096);
 }
+
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 // ---------------------------------------------------------------------------
 DtContext::~DtContext() {
@@ -232,12 +434,11 @@ DtContext::~DtContext() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (void *p : allocs_) (void)hipFree(p);
     if (h_stats_) (void)hipHostFree(h_stats_);
-    if (h_lut_) (void)hipHostFree(h_lut_);
 }
 
 int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs) {
-    if (w <= 0 || h <= 0) {
-        set_error("data term: shape must be positive");
+    if (w <= 0 || h <= 0 || (int64_t)w * h > ((int64_t)1 << 30)) {
+        set_error("data term: shape must be positive and at most 2^30 pixels");
         return GCFG_ERR_INVALID;
     }
     W = w;
@@ -258,7 +459,11 @@ int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs) {
     rc |= dalloc((void **)&own_img_, 3 * n);
     rc |= dalloc((void **)&seeds_, n);
     rc |= dalloc((void **)&d_stats_, sizeof(DtStats));
-    rc |= dalloc((void **)&d_lut_, (2 * 4096 + MAX_LAMBDA + 1) * sizeof(int32_t));
+    rc |= dalloc((void **)&d_frozen_, sizeof(DtStats));
+    rc |= dalloc((void **)&d_beta_part_, (size_t)dt_chunks(W, H) * sizeof(unsigned long long));
+    rc |= dalloc((void **)&d_hist_acc_, DT_HIST * sizeof(int32_t));
+    rc |= dalloc((void **)&d_tlut_, DT_LUT * sizeof(uint16_t));
+    rc |= dalloc((void **)&d_thr_, (DT_MAX_LAMBDA + 1) * sizeof(int32_t));
     rc |= dalloc((void **)&d_kde_, KDE_MAX_SEEDS * sizeof(uint32_t));
     rc |= dalloc((void **)&d_gauss_, KDE_TAPS * sizeof(float));
     if (own_outputs) {
@@ -271,7 +476,8 @@ int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs) {
     }
     if (rc != GCFG_OK) return GCFG_ERR_NOMEM;
     GCFG_HIP(hipHostMalloc((void **)&h_stats_, sizeof(DtStats), hipHostMallocDefault));
-    GCFG_HIP(hipHostMalloc((void **)&h_lut_, (2 * 4096 + MAX_LAMBDA + 1) * sizeof(int32_t), hipHostMallocDefault));
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(DT_NT), 0, stream_, d_hist_acc_, (int64_t)DT_HIST, 0);
+    GCFG_HIP(hipGetLastError());
     img_ = own_img_;
     return clear_seeds();
 }
@@ -283,6 +489,16 @@ int DtContext::upload_image(const uint8_t *h_rgb) {
     }
     GCFG_HIP(hipMemcpyAsync(own_img_, h_rgb, (size_t)3 * W * H, hipMemcpyHostToDevice, stream_));
     img_ = own_img_;
+    computed = false;
+    return GCFG_OK;
+}
+
+int DtContext::use_image(const uint8_t *d_rgb) {
+    if (!d_rgb || ((uintptr_t)d_rgb & 3)) {
+        set_error("data term: the device image must be 4-byte aligned");
+        return GCFG_ERR_INVALID;
+    }
+    img_ = d_rgb;
     computed = false;
     return GCFG_OK;
 }
@@ -364,86 +580,63 @@ int DtContext::build_gauss() {
         const double arg = -0.5 * (double)t * (double)t;
         gauss[k] = (float)(1.0 / ((double)(float)sigma * 2.507) * (double)expf((float)arg));
     }
-    GCFG_HIP(hipMemcpyAsync(d_gauss_, gauss, sizeof(gauss), hipMemcpyHostToDevice, stream_));
+    if (gauss_sigma_ != sigma) {
+        // The host table is pageable memory: the copy has read it when the call returns.
+        GCFG_HIP(hipMemcpy(d_gauss_, gauss, sizeof(gauss), hipMemcpyHostToDevice));
+        gauss_sigma_ = sigma;
+    }
     return GCFG_OK;
 }
 
-// t-link LUT over the 16^3 colour bins (entries [0,4096): foreground cost,
-// [4096, 8192): background cost) and the n-link thresholds thr[k] = largest
-// squared difference s with llround(lambda * exp(-s / (2 beta))) >= k.
-int DtContext::build_tables() {
-    for (int k = 0; k < 2; ++k) {
-        double r[3][16];
-        for (int c = 0; c < 3; ++c)
-            for (int b = 0; b < 16; ++b)
-                r[c][b] = (double)(stats.hist[k][c][b] + 1) / (double)(stats.counts[k] + 16);
-        for (int idx = 0; idx < 4096; ++idx) {
-            double p = 1.0;
-            p *= r[0][(idx >> 8) & 15];
-            p *= r[1][(idx >> 4) & 15];
-            p *= r[2][idx & 15];
-            h_lut_[k * 4096 + idx] = (int32_t)llround(64.0 * -log(p));
-        }
+DtJob DtContext::job(int32_t *const *outs) const {
+    DtJob j{};
+    j.img = img_;
+    j.seeds = seeds_;
+    bool al = true;
+    for (int k = 0; k < 6; ++k) {
+        j.out[k] = outs[k];
+        al &= aligned16(outs[k]);
     }
-    int32_t *thr = h_lut_ + 2 * 4096;
-    thr[0] = 0x7fffffff;
-    auto weight = [&](int s) -> long long {
-        return beta > 0.0 ? llround((double)lambda * exp(-(double)s / (2.0 * beta))) : (long long)lambda;
-    };
-    for (int k = 1; k <= lambda; ++k) {
-        if (weight(MAX_SQDIFF) >= k) {
-            thr[k] = MAX_SQDIFF;
-            continue;
-        }
-        if (weight(0) < k) {
-            thr[k] = -1;
-            continue;
-        }
-        int lo = 0, hi = MAX_SQDIFF;  // weight(lo) >= k > weight(hi)
-        while (hi - lo > 1) {
-            const int mid = lo + (hi - lo) / 2;
-            if (weight(mid) >= k) lo = mid; else hi = mid;
-        }
-        thr[k] = lo;
-    }
-    GCFG_HIP(hipMemcpyAsync(d_lut_, h_lut_, (2 * 4096 + lambda + 1) * sizeof(int32_t), hipMemcpyHostToDevice,
-                            stream_));
-    return GCFG_OK;
+    j.W = W;
+    j.H = H;
+    j.chunk0 = 0;
+    j.lambda = lambda;
+    j.model = model;
+    j.kde_n = kde_count;
+    j.aligned = al ? 1 : 0;
+    j.kde = d_kde_;
+    j.gauss = d_gauss_;
+    j.beta_part = d_beta_part_;
+    j.hist_acc = d_hist_acc_;
+    j.stats = d_stats_;
+    j.frozen = frozen ? d_frozen_ : nullptr;
+    j.tlut = d_tlut_;
+    j.thr = d_thr_;
+    return j;
 }
 
 int DtContext::compute(int32_t *pos, int32_t *neg, int32_t *up, int32_t *down, int32_t *left, int32_t *right) {
-    if (lambda < 0 || lambda > MAX_LAMBDA) {
+    if (lambda < 0 || lambda > DT_MAX_LAMBDA) {
         set_error("data term: lambda out of range");
         return GCFG_ERR_INVALID;
     }
-    const int64_t n = (int64_t)W * H;
-    hipLaunchKernelGGL(k_dt_zero, dim3(1), dim3(128), 0, stream_, d_stats_);
-    hipLaunchKernelGGL(k_dt_stats, dim3(grid_for(n)), dim3(DT_NT), 0, stream_, img_, seeds_, W, H, d_stats_);
-    GCFG_HIP(hipGetLastError());
-    GCFG_HIP(hipMemcpyAsync(h_stats_, d_stats_, sizeof(DtStats), hipMemcpyDeviceToHost, stream_));
-    GCFG_HIP(hipStreamSynchronize(stream_));
-    stats = *h_stats_;
-    if (frozen) {  // keep the model's histograms; beta follows the current image
-        std::memcpy(stats.hist, frozen_stats.hist, sizeof(stats.hist));
-        std::memcpy(stats.counts, frozen_stats.counts, sizeof(stats.counts));
-        std::memcpy(h_stats_, &stats, sizeof(DtStats));
-        GCFG_HIP(hipMemcpyAsync(d_stats_, h_stats_, sizeof(DtStats), hipMemcpyHostToDevice, stream_));
-    }
-    pairs = (int64_t)(W - 1) * H + (int64_t)W * (H - 1);
-    beta = pairs > 0 ? (double)stats.beta_sum / (double)pairs : 0.0;
-    int rc = build_tables();
-    if (rc != GCFG_OK) return rc;
     if (model == GCFG_DT_MODEL_KDE) {
-        rc = build_gauss();
+        int rc = build_gauss();
         if (rc != GCFG_OK) return rc;
-        hipLaunchKernelGGL(k_dt_kde_links, dim3(grid_for(n)), dim3(DT_NT), (lambda + 1) * sizeof(int32_t), stream_,
-                           img_, d_kde_, kde_count, d_gauss_, d_lut_, lambda, W, H, pos, neg, up, down, left, right);
-    } else {
-        hipLaunchKernelGGL(k_dt_links, dim3(grid_for(n)), dim3(DT_NT), (lambda + 1) * sizeof(int32_t), stream_,
-                           img_, d_lut_, lambda, W, H, pos, neg, up, down, left, right);
     }
+    int32_t *outs[6] = {pos, neg, up, down, left, right};
+    DtLaunch L{};
+    L.one = job(outs);
+    L.lds_lambda = lambda;
+    dt_launch(L, dt_chunks(W, H), stream_);
     GCFG_HIP(hipGetLastError());
     computed = true;
+    return GCFG_OK;
+}
+
+int DtContext::freeze_model() {
+    GCFG_HIP(hipMemcpyAsync(d_frozen_, d_stats_, sizeof(DtStats), hipMemcpyDeviceToDevice, stream_));
+    frozen = true;
     return GCFG_OK;
 }
 
@@ -470,11 +663,22 @@ int DtContext::compute_swap() {
     return GCFG_OK;
 }
 
-int DtContext::unquantized(double *h_tpos, double *h_tneg, double *h_right, double *h_down) {
+int DtContext::fetch_stats() {
     if (!computed) {
-        set_error("data term: compute before reading unquantised values");
+        set_error("data term: compute first");
         return GCFG_ERR_STATE;
     }
+    GCFG_HIP(hipMemcpyAsync(h_stats_, d_stats_, sizeof(DtStats), hipMemcpyDeviceToHost, stream_));
+    GCFG_HIP(hipStreamSynchronize(stream_));
+    stats = *h_stats_;
+    pairs = (int64_t)(W - 1) * H + (int64_t)W * (H - 1);
+    beta = pairs > 0 ? (double)stats.beta_sum / (double)pairs : 0.0;
+    return GCFG_OK;
+}
+
+int DtContext::unquantized(double *h_tpos, double *h_tneg, double *h_right, double *h_down) {
+    int rc = fetch_stats();
+    if (rc != GCFG_OK) return rc;
     const size_t n = (size_t)W * H;
     double *buf = nullptr;
     GCFG_HIP(hipMalloc((void **)&buf, 4 * n * sizeof(double)));
@@ -500,6 +704,23 @@ struct gcfg_dataterm {
     hipStream_t stream = nullptr;
     gcfg::DtContext ctx;
     gcfg::StreamJoin join;
+};
+
+// Batched histogram data term: every image's stats, table and links passes
+// in one launch each.
+struct gcfg_dt_batch {
+    hipStream_t stream = nullptr;
+    gcfg::StreamJoin join;
+    int count = 0, total_chunks = 0;
+    int32_t lambda = 50;
+    gcfg::DtJob *d_jobs = nullptr;
+    int32_t *d_cmap = nullptr;
+    std::vector<void *> allocs;
+    ~gcfg_dt_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
 };
 
 extern "C" {
@@ -539,10 +760,8 @@ int gcfg_dt_set_image(gcfg_dataterm *dt, const uint8_t *h_rgb) {
 }
 
 int gcfg_dt_set_image_device(gcfg_dataterm *dt, const uint8_t *d_rgb) {
-    if (!dt || !d_rgb) return GCFG_ERR_INVALID;
-    dt->ctx.use_image(d_rgb);
-    dt->ctx.computed = false;
-    return GCFG_OK;
+    if (!dt) return GCFG_ERR_INVALID;
+    return dt->ctx.use_image(d_rgb);
 }
 
 int gcfg_dt_set_seeds(gcfg_dataterm *dt, const uint8_t *h_seeds) {
@@ -561,7 +780,7 @@ int gcfg_dt_reset(gcfg_dataterm *dt) {
 }
 
 int gcfg_dt_set_lambda(gcfg_dataterm *dt, int32_t lambda) {
-    if (!dt || lambda < 0 || lambda > gcfg::MAX_LAMBDA) {
+    if (!dt || lambda < 0 || lambda > gcfg::DT_MAX_LAMBDA) {
         gcfg::set_error("gcfg_dt_set_lambda: lambda out of range");
         return GCFG_ERR_INVALID;
     }
@@ -602,10 +821,8 @@ int gcfg_dt_get_links(gcfg_dataterm *dt, int32_t **d_pos, int32_t **d_neg, int32
 
 int gcfg_dt_get_histograms(gcfg_dataterm *dt, int32_t *h_hist96, int32_t *h_counts2) {
     if (!dt) return GCFG_ERR_INVALID;
-    if (!dt->ctx.computed) {
-        gcfg::set_error("gcfg_dt_get_histograms: call gcfg_dt_compute first");
-        return GCFG_ERR_STATE;
-    }
+    const int rc = dt->ctx.fetch_stats();
+    if (rc != GCFG_OK) return rc;
     if (h_hist96) std::memcpy(h_hist96, &dt->ctx.stats.hist[0][0][0], 96 * sizeof(int32_t));
     if (h_counts2) std::memcpy(h_counts2, dt->ctx.stats.counts, 2 * sizeof(int32_t));
     return GCFG_OK;
@@ -613,10 +830,8 @@ int gcfg_dt_get_histograms(gcfg_dataterm *dt, int32_t *h_hist96, int32_t *h_coun
 
 int gcfg_dt_get_beta(gcfg_dataterm *dt, int64_t *sum, int64_t *pairs, double *beta) {
     if (!dt) return GCFG_ERR_INVALID;
-    if (!dt->ctx.computed) {
-        gcfg::set_error("gcfg_dt_get_beta: call gcfg_dt_compute first");
-        return GCFG_ERR_STATE;
-    }
+    const int rc = dt->ctx.fetch_stats();
+    if (rc != GCFG_OK) return rc;
     if (sum) *sum = (int64_t)dt->ctx.stats.beta_sum;
     if (pairs) *pairs = dt->ctx.pairs;
     if (beta) *beta = dt->ctx.beta;
@@ -646,6 +861,138 @@ int gcfg_dt_kde_select(gcfg_dataterm *dt, int32_t x, int32_t y) {
 int gcfg_dt_kde_count(gcfg_dataterm *dt, int32_t *count) {
     if (!dt || !count) return GCFG_ERR_INVALID;
     *count = dt->ctx.kde_count;
+    return GCFG_OK;
+}
+
+int gcfg_dt_batch_create(int32_t count, const gcfg_dt_image *images, int32_t lambda, gcfg_dt_batch **out) {
+    using namespace gcfg;
+    if (!out || count <= 0 || !images || lambda < 0 || lambda > DT_MAX_LAMBDA) {
+        set_error("gcfg_dt_batch_create: invalid arguments (count > 0, lambda in 0..4096)");
+        return GCFG_ERR_INVALID;
+    }
+    *out = nullptr;
+    int64_t chunks = 0;
+    for (int i = 0; i < count; ++i) {
+        const gcfg_dt_image &im = images[i];
+        const bool outs = im.d_pos && im.d_neg && im.d_up && im.d_down && im.d_left && im.d_right;
+        if (im.width <= 0 || im.height <= 0 || (int64_t)im.width * im.height > ((int64_t)1 << 30) || !im.d_rgb ||
+            ((uintptr_t)im.d_rgb & 3) || ((uintptr_t)im.d_seeds & 3) || !outs) {
+            set_error("gcfg_dt_batch_create: image " + std::to_string(i) +
+                      " needs a positive shape of at most 2^30 pixels, a 4-byte aligned image (and seed map) "
+                      "and six output planes");
+            return GCFG_ERR_INVALID;
+        }
+        chunks += dt_chunks(im.width, im.height);
+        if (chunks >= ((int64_t)1 << 31) - 1) {
+            set_error("gcfg_dt_batch_create: too many pixels in one batch");
+            return GCFG_ERR_INVALID;
+        }
+    }
+    gcfg_dt_batch *b = new (std::nothrow) gcfg_dt_batch();
+    if (!b) return GCFG_ERR_NOMEM;
+    b->count = count;
+    b->lambda = lambda;
+    b->total_chunks = (int)chunks;
+    auto dalloc = [&](void **p, size_t bytes) -> int {
+        if (hipMalloc(p, bytes ? bytes : 16) != hipSuccess) {
+            *p = nullptr;
+            set_error("gcfg_dt_batch_create: device allocation failed");
+            return GCFG_ERR_NOMEM;
+        }
+        b->allocs.push_back(*p);
+        return GCFG_OK;
+    };
+    unsigned long long *beta_part = nullptr;
+    int32_t *hist_acc = nullptr, *thr = nullptr;
+    DtStats *stats = nullptr;
+    uint16_t *tlut = nullptr;
+    int rc = GCFG_OK;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("gcfg_dt_batch_create: stream creation failed");
+        rc = GCFG_ERR_HIP;
+    }
+    if (rc == GCFG_OK) rc = dalloc((void **)&b->d_jobs, (size_t)count * sizeof(DtJob));
+    if (rc == GCFG_OK) rc = dalloc((void **)&b->d_cmap, (size_t)chunks * sizeof(int32_t));
+    if (rc == GCFG_OK) rc = dalloc((void **)&beta_part, (size_t)chunks * sizeof(unsigned long long));
+    if (rc == GCFG_OK) rc = dalloc((void **)&hist_acc, (size_t)count * DT_HIST * sizeof(int32_t));
+    if (rc == GCFG_OK) rc = dalloc((void **)&stats, (size_t)count * sizeof(DtStats));
+    if (rc == GCFG_OK) rc = dalloc((void **)&tlut, (size_t)count * DT_LUT * sizeof(uint16_t));
+    if (rc == GCFG_OK) rc = dalloc((void **)&thr, (size_t)count * (lambda + 1) * sizeof(int32_t));
+    if (rc != GCFG_OK) {
+        delete b;
+        return rc;
+    }
+    std::vector<DtJob> jobs(count);
+    std::vector<int32_t> cmap((size_t)chunks);
+    int c0 = 0;
+    for (int i = 0; i < count; ++i) {
+        const gcfg_dt_image &im = images[i];
+        DtJob &j = jobs[i];
+        j = DtJob{};
+        j.img = im.d_rgb;
+        j.seeds = im.d_seeds;
+        int32_t *o[6] = {im.d_pos, im.d_neg, im.d_up, im.d_down, im.d_left, im.d_right};
+        bool al = true;
+        for (int k = 0; k < 6; ++k) {
+            j.out[k] = o[k];
+            al &= aligned16(o[k]);
+        }
+        j.aligned = al ? 1 : 0;
+        j.W = im.width;
+        j.H = im.height;
+        j.chunk0 = c0;
+        j.lambda = lambda;
+        j.model = GCFG_DT_MODEL_HISTOGRAM;
+        j.beta_part = beta_part + c0;
+        j.hist_acc = hist_acc + (size_t)i * DT_HIST;
+        j.stats = stats + i;
+        j.tlut = tlut + (size_t)i * DT_LUT;
+        j.thr = thr + (size_t)i * (lambda + 1);
+        const int nc = dt_chunks(im.width, im.height);
+        for (int k = 0; k < nc; ++k) cmap[(size_t)c0 + k] = i;
+        c0 += nc;
+    }
+    hipError_t e = hipMemcpy(b->d_jobs, jobs.data(), jobs.size() * sizeof(DtJob), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_cmap, cmap.data(), cmap.size() * sizeof(int32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for((int64_t)count * DT_HIST)), dim3(DT_NT), 0, b->stream, hist_acc,
+                           (int64_t)count * DT_HIST, 0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) {
+        delete b;
+        return hip_fail(e, "gcfg_dt_batch_create");
+    }
+    *out = b;
+    return GCFG_OK;
+}
+
+int gcfg_dt_batch_set_stream(gcfg_dt_batch *b, void *hip_stream) {
+    if (!b) return GCFG_ERR_INVALID;
+    return b->join.set((hipStream_t)hip_stream);
+}
+
+int gcfg_dt_batch_compute(gcfg_dt_batch *b) {
+    using namespace gcfg;
+    if (!b) return GCFG_ERR_INVALID;
+    int rc = b->join.before(b->stream);
+    if (rc != GCFG_OK) return rc;
+    DtLaunch L{};
+    L.jobs = b->d_jobs;
+    L.cmap = b->d_cmap;
+    L.njobs = b->count;
+    L.lds_lambda = b->lambda;
+    dt_launch(L, b->total_chunks, b->stream);
+    GCFG_HIP(hipGetLastError());
+    rc = b->join.after(b->stream);
+    if (rc != GCFG_OK) return rc;
+    GCFG_HIP(hipStreamSynchronize(b->stream));
+    return GCFG_OK;
+}
+
+int gcfg_dt_batch_destroy(gcfg_dt_batch *b) {
+    delete b;
     return GCFG_OK;
 }
 

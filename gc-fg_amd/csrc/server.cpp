@@ -137,7 +137,7 @@ static int admit_image(gcfg_server *s, Request *r) {
     if (rc == GCFG_OK) rc = ss->dt->upload_image(r->rgb);
     if (rc == GCFG_OK) rc = ss->dt->upload_seeds(r->seeds);
     if (rc == GCFG_OK) rc = ss->dt->compute_owned();
-    if (rc == GCFG_OK) ss->dt->freeze_model();  // later frames of the session reuse this colour model
+    if (rc == GCFG_OK) rc = ss->dt->freeze_model();  // later frames of the session reuse this colour model
     if (rc == GCFG_OK) {
         ss->slot = s->eng.add_instance(r->W, r->H, ss->dt->pos_, ss->dt->neg_, ss->dt->up_, ss->dt->down_,
                                        ss->dt->left_, ss->dt->right_, 0, ss->d_label);

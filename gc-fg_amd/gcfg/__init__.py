@@ -575,6 +575,58 @@ class DataTerm:
             pass
 
 
+class DtImage(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("d_rgb", _VP), ("d_seeds", _VP),
+                ("d_pos", _VP), ("d_neg", _VP), ("d_up", _VP), ("d_down", _VP), ("d_left", _VP), ("d_right", _VP)]
+
+
+class DataTermBatch:
+    """Histogram data term of many images in one launch per pass
+    (gcfg_dt_batch_*). ``images`` = list of (rgb [H, W, 3] u8, seeds [H, W] u8
+    or None); the device planes of image i are ``planes[i]`` (pos, neg, up,
+    down, left, right DeviceBuffers)."""
+
+    def __init__(self, images, lam: int = 50):
+        n = len(images)
+        arr = (DtImage * n)()
+        self.inputs, self.planes, self.shapes = [], [], []
+        for i, (rgb, seeds) in enumerate(images):
+            rgb = np.ascontiguousarray(rgb, np.uint8)
+            H, W = rgb.shape[:2]
+            drgb = DeviceBuffer.from_array(rgb)
+            dseeds = DeviceBuffer.from_array(np.ascontiguousarray(seeds, np.uint8)) if seeds is not None else None
+            outs = [DeviceBuffer(4 * W * H) for _ in range(6)]
+            self.inputs.append((drgb, dseeds))
+            self.planes.append(outs)
+            self.shapes.append((W, H))
+            arr[i].width, arr[i].height = W, H
+            arr[i].d_rgb = drgb.ptr
+            arr[i].d_seeds = dseeds.ptr if dseeds is not None else None
+            arr[i].d_pos, arr[i].d_neg, arr[i].d_up, arr[i].d_down, arr[i].d_left, arr[i].d_right = \
+                (o.ptr for o in outs)
+        h = ctypes.c_void_p()
+        _check(lib().gcfg_dt_batch_create(n, arr, int(lam), ctypes.byref(h)), "gcfg_dt_batch_create")
+        self.h = h.value
+
+    def compute(self):
+        _check(lib().gcfg_dt_batch_compute(_VP(self.h)), "gcfg_dt_batch_compute")
+
+    def links(self, i: int):
+        W, H = self.shapes[i]
+        return [b.download(np.int32, W * H).reshape(H, W) for b in self.planes[i]]
+
+    def close(self):
+        if self.h:
+            lib().gcfg_dt_batch_destroy(_VP(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ---------------------------------------------------------------------------
 @dataclass
 class BatchItem:

@@ -202,6 +202,27 @@ int gcfg_dt_kde_select(gcfg_dataterm *dt, int32_t x, int32_t y);
 /* Number of seed colours in the kernel-density list. */
 int gcfg_dt_kde_count(gcfg_dataterm *dt, int32_t *count);
 
+/* Batched data term: getDataTermsMixture (DataTerm.cpp:129-137) of many
+ * images at once, histogram model, with one launch per pass for the whole
+ * batch (no counterpart in the reference, which computes one image per
+ * call). Each image: caller-owned device RGB (W*H*3 interleaved, 4-byte
+ * aligned), its seed map (W*H classes GCFG_SEED_*, 4-byte aligned; NULL: no
+ * seeds) and six output planes (W*H int32 each; 16-byte aligned planes are
+ * written with vector stores). Outputs equal gcfg_dt_compute's per image. */
+typedef struct gcfg_dt_image {
+    int32_t width, height;
+    const uint8_t *d_rgb;
+    const uint8_t *d_seeds;
+    int32_t *d_pos, *d_neg, *d_up, *d_down, *d_left, *d_right;
+} gcfg_dt_image;
+typedef struct gcfg_dt_batch gcfg_dt_batch;
+int gcfg_dt_batch_create(int32_t count, const gcfg_dt_image *images, int32_t lambda, gcfg_dt_batch **out);
+/* As gcfg_graph_set_stream, for gcfg_dt_batch_compute. */
+int gcfg_dt_batch_set_stream(gcfg_dt_batch *b, void *hip_stream);
+/* Computes every image's data term; returns when the outputs are written. */
+int gcfg_dt_batch_compute(gcfg_dt_batch *b);
+int gcfg_dt_batch_destroy(gcfg_dt_batch *b);
+
 /* ---- Batched solve: many instances of uneven size in one engine ---- */
 typedef struct gcfg_instance {
     int32_t width, height;

@@ -172,3 +172,63 @@ def test_kde_segmentation_negative_tlinks(g, ora):
         rflow, rlabel = ora.maxflow(W, H, pos, neg, penalty=pen)
         assert flow == rflow and np.array_equal(label, rlabel), pen
     dt.close()
+
+
+def _odd_shapes():
+    # widths that are not multiples of 4 (rows start at every byte alignment),
+    # widths below 4 (several rows per group of four pixels), one-pixel images,
+    # sizes that end inside a chunk of 4096 pixels and exactly at one.
+    return [(1, 1), (2, 3), (3, 7), (5, 2), (7, 9), (13, 5), (64, 64), (65, 63), (97, 61), (333, 17), (1023, 5),
+            (4096, 1), (1, 4097)]
+
+
+@pytest.mark.parametrize("lam", [50, 3])
+def test_batch_dataterm_exact(g, ora, lam):
+    """One launch per pass over images of every alignment: equal to the
+    single-image path and to the oracle, bit for bit."""
+    rng = np.random.default_rng(lam)
+    images = []
+    for i, (W, H) in enumerate(_odd_shapes()):
+        img = g.gen_ellipse_image(W, H, 40 + i) if W >= 8 and H >= 8 else \
+            rng.integers(0, 256, (H, W, 3)).astype(np.uint8)
+        seeds = (rng.integers(0, 3, (H, W)) * (rng.random((H, W)) < 0.3)).astype(np.uint8)
+        images.append((img, seeds if i % 3 else None))
+    b = g.DataTermBatch(images, lam)
+    for rep in range(2):  # the per-image counters are cleared for the next compute
+        b.compute()
+        for i, ((img, seeds), (W, H)) in enumerate(zip(images, b.shapes)):
+            sd = seeds if seeds is not None else np.zeros((H, W), np.uint8)
+            ref = ora.dataterm(W, H, img, sd, lam)
+            got = b.links(i)
+            for name, a in zip(("pos", "neg", "up", "down", "left", "right"), got):
+                assert np.array_equal(a, ref[name]), (rep, i, W, H, name)
+            if rep == 0:
+                dt = _device(g, W, H, img, sd, lam)
+                for a, c in zip(dt.links(), got):
+                    assert np.array_equal(a, c), (i, W, H)
+                dt.close()
+    b.close()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_every_colour_bin(g, ora, seed):
+    """Images holding all 16^3 colour bins, with random seed histograms: every
+    t-link table entry the device builds is compared with the oracle's
+    per-pixel f64 evaluation."""
+    rng = np.random.default_rng(100 + seed)
+    W, H = 128, 64  # 8192 pixels: each bin twice, at random offsets inside the bin
+    idx = np.arange(W * H) % 4096
+    img = np.stack([((idx >> 8) & 15) * 16, ((idx >> 4) & 15) * 16, (idx & 15) * 16], -1).astype(np.int32)
+    img = (img + rng.integers(0, 16, img.shape)).astype(np.uint8).reshape(H, W, 3)
+    img = img.reshape(-1, 3)[rng.permutation(W * H)].reshape(H, W, 3)
+    seeds = np.zeros((H, W), np.uint8)
+    k = rng.integers(1, 3000)
+    pick = rng.choice(W * H, size=int(k), replace=False)
+    seeds.reshape(-1)[pick] = rng.integers(1, 3, int(k))
+    dt = _device(g, W, H, img, seeds)
+    ref = ora.dataterm(W, H, img, seeds, 50)
+    for name, a in zip(("pos", "neg", "up", "down", "left", "right"), dt.links()):
+        assert np.array_equal(a, ref[name]), name
+    hist, counts = dt.histograms()
+    assert np.array_equal(hist, ref["hist"]) and np.array_equal(counts, ref["counts"])
+    dt.close()
