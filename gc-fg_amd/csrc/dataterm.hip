@@ -436,15 +436,16 @@ DtContext::~DtContext() {
     if (h_stats_) (void)hipHostFree(h_stats_);
 }
 
-int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs) {
-    if (w <= 0 || h <= 0 || (int64_t)w * h > ((int64_t)1 << 30)) {
+int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs, size_t cap_px) {
+    if (w <= 0 || h <= 0 || (int64_t)w * h > ((int64_t)1 << 30) || cap_px > ((size_t)1 << 30)) {
         set_error("data term: shape must be positive and at most 2^30 pixels");
         return GCFG_ERR_INVALID;
     }
     W = w;
     H = h;
     stream_ = stream;
-    const size_t n = (size_t)W * H;
+    const size_t n = std::max((size_t)W * H, cap_px);
+    cap_px_ = n;
     auto dalloc = [&](void **p, size_t bytes) -> int {
         if (hipMalloc(p, bytes) != hipSuccess) {
             *p = nullptr;
@@ -460,7 +461,7 @@ int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs) {
     rc |= dalloc((void **)&seeds_, n);
     rc |= dalloc((void **)&d_stats_, sizeof(DtStats));
     rc |= dalloc((void **)&d_frozen_, sizeof(DtStats));
-    rc |= dalloc((void **)&d_beta_part_, (size_t)dt_chunks(W, H) * sizeof(unsigned long long));
+    rc |= dalloc((void **)&d_beta_part_, (n + DT_CHUNK - 1) / DT_CHUNK * sizeof(unsigned long long));
     rc |= dalloc((void **)&d_hist_acc_, DT_HIST * sizeof(int32_t));
     rc |= dalloc((void **)&d_tlut_, DT_LUT * sizeof(uint16_t));
     rc |= dalloc((void **)&d_thr_, (DT_MAX_LAMBDA + 1) * sizeof(int32_t));
@@ -480,6 +481,32 @@ int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs) {
     GCFG_HIP(hipGetLastError());
     img_ = own_img_;
     return clear_seeds();
+}
+
+int DtContext::reshape(int w, int h) {
+    if (w <= 0 || h <= 0 || (size_t)w * h > cap_px_) {
+        set_error("data term: shape beyond the buffers' capacity");
+        return GCFG_ERR_INVALID;
+    }
+    W = w;
+    H = h;
+    frozen = false;
+    kde_count = 0;
+    computed = false;
+    img_ = own_img_;
+    return GCFG_OK;
+}
+
+int DtContext::upload_on(hipStream_t s, const uint8_t *h_rgb, const uint8_t *h_seeds) {
+    if (!h_rgb) {
+        set_error("data term: null image");
+        return GCFG_ERR_INVALID;
+    }
+    GCFG_HIP(hipMemcpyAsync(own_img_, h_rgb, (size_t)3 * W * H, hipMemcpyHostToDevice, s));
+    if (h_seeds) GCFG_HIP(hipMemcpyAsync(seeds_, h_seeds, (size_t)W * H, hipMemcpyHostToDevice, s));
+    img_ = own_img_;
+    computed = false;
+    return GCFG_OK;
 }
 
 int DtContext::upload_image(const uint8_t *h_rgb) {
@@ -645,7 +672,7 @@ int DtContext::compute_swap() {
         set_error("data term: no owned output planes");
         return GCFG_ERR_STATE;
     }
-    const size_t n = (size_t)W * H;
+    const size_t n = cap_px_;  // the planes serve any later shape within the capacity
     for (int k = 0; k < 6; ++k)
         if (!alt_[k]) {
             if (hipMalloc((void **)&alt_[k], n * 4) != hipSuccess) {

@@ -97,7 +97,16 @@ void dt_launch(const DtLaunch &L, int total_chunks, hipStream_t s);
 class DtContext {
    public:
     ~DtContext();
-    int init(int W, int H, hipStream_t stream, bool own_outputs);
+    // Buffers for images of up to max(W * H, cap_px) pixels.
+    int init(int W, int H, hipStream_t stream, bool own_outputs, size_t cap_px = 0);
+    // Reuse the buffers for a W x H image (W * H <= capacity()): the
+    // kernel-density colours and the frozen colour model are dropped; the
+    // seed map is left as it is (upload or clear one before computing).
+    int reshape(int W, int H);
+    size_t capacity() const { return cap_px_; }
+    // Image and seed map copied from the host on another stream (the copies
+    // are ordered before this context's later work by the caller).
+    int upload_on(hipStream_t s, const uint8_t *h_rgb, const uint8_t *h_seeds);
     // Inputs (device pointers; the image may be owned or borrowed).
     int upload_image(const uint8_t *h_rgb);
     int use_image(const uint8_t *d_rgb);
@@ -157,6 +166,7 @@ class DtContext {
     int32_t *alt_[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // second output set
     float *d_gauss_ = nullptr;
     int gauss_sigma_ = 0;  // sigma the device kernel table was built for (0: none)
+    size_t cap_px_ = 0;
     std::vector<void *> allocs_;
     size_t bytes_ = 0;
 };

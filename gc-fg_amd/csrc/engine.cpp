@@ -34,8 +34,7 @@ Engine::~Engine() {
     for (hipEvent_t e : build_ev_)
         if (e) (void)hipEventDestroy(e);
     for (void *p : allocs_) (void)hipFree(p);
-    if (pin_desc_) (void)hipHostFree(pin_desc_);
-    if (pin_slots_) (void)hipHostFree(pin_slots_);
+    if (ring_) (void)hipHostFree(ring_);
     if (pin_ctl_) (void)hipHostFree(pin_ctl_);
     if (pin_states_) (void)hipHostFree(pin_states_);
     if (pin_qtrace_) (void)hipHostFree(pin_qtrace_);
@@ -130,8 +129,9 @@ int Engine::init(int cap_tiles, int cap_slots, const gcfg_options &o, int nbhd) 
         rc |= dalloc((void **)&kp_.nbr8, (size_t)cap_tiles * sizeof(int4));
     }
     if (rc != GCFG_OK) return GCFG_ERR_NOMEM;
-    GCFG_HIP(hipHostMalloc((void **)&pin_desc_, (size_t)cap_slots * sizeof(InstDesc), hipHostMallocDefault));
-    GCFG_HIP(hipHostMalloc((void **)&pin_slots_, (size_t)(2 * cap_slots + 1) * sizeof(int32_t), hipHostMallocDefault));
+    ring_cap_ = std::max<size_t>((size_t)4 << 20, 2 * ((size_t)cap_slots * sizeof(InstDesc) + 4096));
+    GCFG_HIP(hipHostMalloc((void **)&ring_, ring_cap_, hipHostMallocDefault));
+    ring_off_ = 0;
     GCFG_HIP(hipHostMalloc((void **)&pin_ctl_, sizeof(Ctl), hipHostMallocDefault));
     GCFG_HIP(hipHostMalloc((void **)&pin_states_, (size_t)cap_slots * sizeof(InstState), hipHostMallocDefault));
     GCFG_HIP(hipMemsetAsync(kp_.state, 0, (size_t)cap_slots * sizeof(InstState), stream_));
@@ -348,23 +348,14 @@ int Engine::build(const std::vector<int> &slots) {
         list[n + i] = acc;
         acc += h_desc_[slots[i]].ntiles;
     }
-    const size_t dbytes = (size_t)cap_slots_ * sizeof(InstDesc);
-    const bool desc_same = std::memcmp(dev_desc_.data(), h_desc_.data(), dbytes) == 0;
-    const bool list_same = dev_list_ == list;
-    if (!desc_same || !list_same) {
-        // The pinned staging buffers are reused; wait for earlier copies out of them.
-        GCFG_HIP(hipStreamSynchronize(stream_));
-        if (!desc_same) {
-            std::memcpy(pin_desc_, h_desc_.data(), dbytes);
-            GCFG_HIP(hipMemcpyAsync(kp_.desc, pin_desc_, dbytes, hipMemcpyHostToDevice, stream_));
-            dev_desc_ = h_desc_;
-        }
-        if (!list_same) {
-            std::memcpy(pin_slots_, list.data(), (size_t)(2 * n) * sizeof(int32_t));
-            GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)(2 * n) * sizeof(int32_t), hipMemcpyHostToDevice,
-                                    stream_));
-            dev_list_ = list;
-        }
+    for (int s : slots) {
+        const int rc = upload_desc_slot(s);
+        if (rc != GCFG_OK) return rc;
+    }
+    if (dev_list_ != list) {
+        const int rc = stage(d_slots_, list.data(), (size_t)(2 * n) * sizeof(int32_t));
+        if (rc != GCFG_OK) return rc;
+        dev_list_ = list;
     }
     launch_arm(kp_, d_slots_, n, 0, opt.sweeps_per_gr, stream_, kp_.variant ? total : 0);
     LaunchEvents ev;
@@ -382,10 +373,7 @@ int Engine::build(const std::vector<int> &slots) {
 void Engine::update_sched_width() {
     int wide = 0;
     for (int i = 0; i < cap_slots_; ++i) wide |= slot_used_[i] && h_desc_[i].ntiles > 2048;
-    if (wide != kp_.sched_wide) {
-        kp_.sched_wide = wide;
-        drop_graphs();  // the captured steps carry the launch shape
-    }
+    kp_.sched_wide = wide;  // captured chunks are kept per width (StepGraph::wide)
 }
 
 bool Engine::any_lazy() const {
@@ -407,21 +395,57 @@ int Engine::materialize(const std::vector<int> &slots) {
     return upload_desc();  // ordered after the materialising kernels on the stream
 }
 
+int Engine::stage(void *d_dst, const void *h_src, size_t bytes) {
+    if (bytes == 0) return GCFG_OK;
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (need > ring_cap_) {  // larger than the ring: a synchronous copy
+        GCFG_HIP(hipStreamSynchronize(stream_));
+        GCFG_HIP(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+        return GCFG_OK;
+    }
+    if (ring_off_ + need > ring_cap_) {
+        GCFG_HIP(hipStreamSynchronize(stream_));
+        ring_off_ = 0;
+    }
+    std::memcpy(ring_ + ring_off_, h_src, bytes);
+    GCFG_HIP(hipMemcpyAsync(d_dst, ring_ + ring_off_, bytes, hipMemcpyHostToDevice, stream_));
+    ring_off_ += need;
+    return GCFG_OK;
+}
+
+int Engine::upload_desc_slot(int slot) {
+    if (std::memcmp(&dev_desc_[slot], &h_desc_[slot], sizeof(InstDesc)) == 0) return GCFG_OK;
+    const int rc = stage(kp_.desc + slot, &h_desc_[slot], sizeof(InstDesc));
+    if (rc == GCFG_OK) dev_desc_[slot] = h_desc_[slot];
+    return rc;
+}
+
+// Descriptors that differ from the device's, queued after the work already
+// on the stream.
 int Engine::upload_desc() {
-    GCFG_HIP(hipStreamSynchronize(stream_));
-    std::memcpy(pin_desc_, h_desc_.data(), (size_t)cap_slots_ * sizeof(InstDesc));
-    GCFG_HIP(hipMemcpyAsync(kp_.desc, pin_desc_, (size_t)cap_slots_ * sizeof(InstDesc), hipMemcpyHostToDevice,
-                            stream_));
-    dev_desc_ = h_desc_;
+    for (int s = 0; s < cap_slots_; ++s) {
+        const int rc = upload_desc_slot(s);
+        if (rc != GCFG_OK) return rc;
+    }
+    return GCFG_OK;
+}
+
+int Engine::snapshot_states(InstState *h_dst, int n, hipEvent_t ev) {
+    if (n < 0 || n > cap_slots_) {
+        set_error("snapshot_states: range outside the instance slots");
+        return GCFG_ERR_INVALID;
+    }
+    if (n > 0)
+        GCFG_HIP(hipMemcpyAsync(h_dst, kp_.state, (size_t)n * sizeof(InstState), hipMemcpyDeviceToHost, stream_));
+    GCFG_HIP(hipEventRecord(ev, stream_));
     return GCFG_OK;
 }
 
 int Engine::rearm(const std::vector<int> &slots) {
     if (slots.empty()) return GCFG_OK;
-    GCFG_HIP(hipStreamSynchronize(stream_));
     const int n = (int)slots.size();
-    for (int i = 0; i < n; ++i) pin_slots_[i] = slots[i];
-    GCFG_HIP(hipMemcpyAsync(d_slots_, pin_slots_, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+    const int rc = stage(d_slots_, slots.data(), (size_t)n * sizeof(int32_t));
+    if (rc != GCFG_OK) return rc;
     dev_list_.clear();  // the build list on the device is overwritten
     apply_options();
     launch_arm(kp_, d_slots_, n, 1, opt.sweeps_per_gr, stream_, 0);
@@ -442,15 +466,8 @@ int Engine::edit(int slot, const std::vector<int32_t> &strokes, int32_t delta) {
         GCFG_HIP(hipMalloc((void **)&d_strokes_, (size_t)strokes_cap_ * sizeof(int32_t)));
         allocs_.push_back(d_strokes_);
     }
-    GCFG_HIP(hipStreamSynchronize(stream_));
-    std::memcpy(pin_slots_, strokes.data(), std::min<size_t>(strokes.size(), (size_t)(2 * cap_slots_ + 1)) *
-                                                 sizeof(int32_t));
-    if (strokes.size() <= (size_t)(2 * cap_slots_ + 1)) {
-        GCFG_HIP(hipMemcpyAsync(d_strokes_, pin_slots_, strokes.size() * sizeof(int32_t), hipMemcpyHostToDevice,
-                                stream_));
-    } else {
-        GCFG_HIP(hipMemcpy(d_strokes_, strokes.data(), strokes.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    }
+    const int rc = stage(d_strokes_, strokes.data(), strokes.size() * sizeof(int32_t));
+    if (rc != GCFG_OK) return rc;
     launch_edit(kp_, slot, d_strokes_, n, delta, stream_);
     GCFG_HIP(hipGetLastError());
     return GCFG_OK;
@@ -469,16 +486,16 @@ void Engine::drop_graphs() {
 
 int Engine::step_graph(int steps, int parity, hipGraphExec_t *out) {
     for (StepGraph &g : graphs_)
-        if (g.steps == steps && g.parity == parity) {
+        if (g.steps == steps && g.parity == parity && g.wide == kp_.sched_wide) {
             *out = g.exec;
             return GCFG_OK;
         }
-    if (graphs_.size() >= 16) {  // keep a handful of chunk lengths and parities
+    if (graphs_.size() >= 16) {  // keep a handful of chunk lengths, parities and widths
         (void)hipGraphExecDestroy(graphs_.front().exec);
         (void)hipGraphDestroy(graphs_.front().graph);
         graphs_.erase(graphs_.begin());
     }
-    StepGraph g{steps, parity, nullptr, nullptr};
+    StepGraph g{steps, parity, kp_.sched_wide, nullptr, nullptr};
     GCFG_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < steps; ++i) launch_step(kp_, grid_pr_, grid_gr_, (parity + i) & 1, stream_);
     GCFG_HIP(hipStreamEndCapture(stream_, &g.graph));

@@ -242,8 +242,10 @@ class Engine {
     int neighborhood() const { return nbhd_; }
     // Upload descriptors, re-initialise state and build the residual graphs.
     int build(const std::vector<int> &slots);
-    // Upload the descriptor table only (e.g. a new label destination).
+    // Upload the descriptors that changed (e.g. a new label destination),
+    // queued after the work already on the stream.
     int upload_desc();
+    int upload_desc_slot(int slot);
     // Re-arm instances from their current residual graph (warm start).
     int rearm(const std::vector<int> &slots);
     // Run steps until every armed instance is done. polled_ctl() then holds
@@ -273,6 +275,9 @@ class Engine {
     // layout; any output may be null.
     int read_state(int slot, int32_t *h_caps, int32_t *h_exc, int32_t *h_dist = nullptr,
                    int32_t *h_caps8 = nullptr);
+    // Queue a copy of the states of slots [0, n) into pinned host memory and
+    // record ev after it on the engine stream (non-blocking).
+    int snapshot_states(InstState *h_dst, int n, hipEvent_t ev);
     int fetch_state(int slot, InstState *out);
     int fetch_states(InstState *out, int n);  // slots [0, n)
     int fetch_ctl(Ctl *out);
@@ -327,19 +332,25 @@ class Engine {
     std::vector<int32_t> dev_list_;   // what d_slots_ holds after a build (empty: unknown)
     std::vector<char> slot_used_;
     std::vector<std::pair<int, int>> free_ranges_;  // (start, count)
-    InstDesc *pin_desc_ = nullptr;
-    int32_t *pin_slots_ = nullptr;
     int32_t *d_slots_ = nullptr;
     Ctl *pin_ctl_ = nullptr;
     InstState *pin_states_ = nullptr;  // cap_slots_ states (fetch_states_pinned)
     int32_t *d_strokes_ = nullptr;
     int strokes_cap_ = 0;
+    // Pinned ring for small host-to-device uploads (descriptors, slot lists,
+    // strokes): each upload takes the next bytes of the ring; a wrap waits for
+    // the stream, after which every copy queued from the ring has read its
+    // bytes. Uploads therefore never wait for queued kernels.
+    uint8_t *ring_ = nullptr;
+    size_t ring_cap_ = 0, ring_off_ = 0;
+    int stage(void *d_dst, const void *h_src, size_t bytes);
     // Captured step chunks, one per chunk length and starting queue parity
     // (a chunk of any length continues the parity sequence, so a solve
     // replays exactly the steps it needs); dropped when the options change.
     struct StepGraph {
         int steps;
         int parity;
+        int wide;  // scheduler launch width the chunk was captured with
         hipGraph_t graph;
         hipGraphExec_t exec;
     };

@@ -7,7 +7,23 @@
 // budget (other instances keep solving), and an edit (seed strokes) changes
 // the excess of the touched nodes and re-arms the instance from its current
 // residual graph (warm start: global relabel, then push-relabel).
+//
+// The host never waits for the device on the request path. Three streams:
+//  * the engine stream runs data terms, builds and chunks of solver steps;
+//    after each chunk the instance states are copied into one of two pinned
+//    snapshots and an event is recorded, so the next chunk runs while the
+//    host reads the previous snapshot;
+//  * an upload stream copies request images and seed maps; the engine stream
+//    waits for them with an event;
+//  * a download stream copies a finished instance's labels (int32 or packed
+//    one bit per pixel) into pinned staging; the request completes when that
+//    copy's event has passed, and the labels then go to the caller's buffer.
+// Small uploads (descriptors, slot lists, strokes) go through the engine's
+// pinned ring. Session buffers (image, seed map, data term, labels) return to
+// a pool when a session closes and serve later sessions of at most as many
+// pixels, so admission rarely allocates.
 #include <algorithm>
+#include <cstring>
 #include <deque>
 #include <map>
 #include <new>
@@ -18,6 +34,7 @@
 
 using gcfg::DtContext;
 using gcfg::Engine;
+using gcfg::InstDesc;
 using gcfg::InstState;
 using gcfg::now_ms;
 using gcfg::set_error;
@@ -26,6 +43,23 @@ namespace {
 
 enum { REQ_IMAGE = 0, REQ_EDIT = 1, REQ_FRAME = 2 };
 
+// Device buffers of one session.
+struct Buffers {
+    DtContext dt;
+    void *d_label = nullptr;  // int32 labels or packed words
+    size_t label_bytes = 0;
+    size_t bytes() const { return dt.device_bytes() + label_bytes; }
+    ~Buffers() {
+        if (d_label) (void)hipFree(d_label);
+    }
+};
+
+// Pinned host memory that a finished request's labels are copied into.
+struct Staging {
+    void *h = nullptr;
+    size_t bytes = 0;
+};
+
 struct Request {
     int64_t id = 0;
     int kind = REQ_IMAGE;
@@ -33,18 +67,23 @@ struct Request {
     double t_submit = 0, t_admit = 0;
     int W = 0, H = 0;
     const uint8_t *rgb = nullptr, *seeds = nullptr;
-    int32_t *h_label = nullptr;
+    void *h_label = nullptr;
+    int32_t format = GCFG_LABELS_INT32;
     std::vector<int32_t> strokes;
     int32_t delta = 0;
+    int64_t first_snapshot = 0;  // snapshots from this sequence number on see the admitted instance
+    int64_t flow = 0;
+    Staging stg;
+    size_t label_bytes = 0;
+    hipEvent_t ev = nullptr;  // label copy done
 };
 
 struct Session {
     int32_t id = 0;
     int slot = -1;
     int W = 0, H = 0;
-    DtContext *dt = nullptr;
-    int32_t *d_label = nullptr;
-    size_t bytes = 0;
+    Buffers *buf = nullptr;
+    size_t tile_bytes = 0;
     bool busy = false;
     double last_used = 0;
     // Sum over the session's edits of delta x strokes: a bound on how far the
@@ -53,6 +92,19 @@ struct Session {
     int64_t edit_mass = 0;
 };
 constexpr int64_t EDIT_MASS_MAX = (int64_t)1 << 29;
+
+struct Snapshot {
+    InstState *h = nullptr;  // pinned
+    hipEvent_t ev = nullptr;
+    int64_t seq = 0;
+    bool pending = false;
+};
+constexpr int SNAPSHOTS = 2;  // chunks of steps in flight
+
+size_t label_bytes_for(int W, int H, int32_t format) {
+    return format == GCFG_LABELS_BITS ? Engine::label_words(W, H) * sizeof(uint64_t)
+                                      : (size_t)W * H * sizeof(int32_t);
+}
 
 }  // namespace
 
@@ -63,28 +115,47 @@ struct gcfg_server {
     int cap_slots = 0;
     int64_t next_id = 1;
     int32_t next_session = 1;
+    int32_t label_format = GCFG_LABELS_INT32;
+    hipStream_t up = nullptr, down = nullptr;
+    hipEvent_t up_ev = nullptr;
     std::deque<Request *> queue;
-    std::vector<Request *> running;
+    std::vector<Request *> running, copying;
     std::map<int32_t, Session *> sessions;
     std::map<int, Session *> by_slot;
-    std::vector<InstState> states;
+    std::vector<Buffers *> pool;      // idle session buffers (counted in used)
+    std::vector<Staging> staging;     // idle pinned label staging
+    std::vector<hipEvent_t> events;   // idle events
+    Snapshot snaps[SNAPSHOTS];
+    int64_t snap_seq = 0;             // sequence number of the next snapshot
+    int hi_slot = 0;                  // slots [0, hi_slot) may be in use
     std::deque<gcfg_result> ready;
 };
 
-static size_t session_cost(const Engine &eng, int W, int H) {
+static size_t tile_bytes_for(const Engine &eng, int W, int H) {
+    return (size_t)eng.tiles_for(W, H) * Engine::bytes_per_tile();
+}
+
+// Device bytes a new session's buffers take (data term with its planes and
+// the video frame planes allocated on first use, labels).
+static size_t buffer_estimate(int W, int H) {
     const size_t n = (size_t)W * H;
-    return (size_t)eng.tiles_for(W, H) * Engine::bytes_per_tile() + n * (3 + 1 + 24 + 4) + 64 * 1024;
+    return n * (3 + 1 + 24 + 4) + 2 * ((size_t)H + n / 64) * 8 + 512 * 1024;
+}
+
+static void free_pool_entry(gcfg_server *s, size_t i) {
+    Buffers *b = s->pool[i];
+    s->used -= std::min(s->used, b->bytes());
+    delete b;
+    s->pool.erase(s->pool.begin() + (long)i);
 }
 
 static void destroy_session(gcfg_server *s, Session *ss) {
     if (ss->slot >= 0) {
-        s->eng.sync();
         s->eng.remove_instance(ss->slot);
         s->by_slot.erase(ss->slot);
     }
-    delete ss->dt;
-    if (ss->d_label) (void)hipFree(ss->d_label);
-    s->used -= std::min(s->used, ss->bytes);
+    s->used -= std::min(s->used, ss->tile_bytes);
+    if (ss->buf) s->pool.push_back(ss->buf);  // its bytes stay counted until the pool entry is freed
     s->sessions.erase(ss->id);
     delete ss;
 }
@@ -102,17 +173,28 @@ static void push_result(gcfg_server *s, Request *r, int status, int64_t flow) {
     s->ready.push_back(res);
 }
 
-// Evict idle sessions, least recently used first, until `need` bytes fit.
-static bool make_room(gcfg_server *s, size_t need) {
+static bool session_has_queued(gcfg_server *s, int32_t id) {
+    for (Request *q : s->queue)
+        if ((q->kind == REQ_EDIT || q->kind == REQ_FRAME) && q->session == id) return true;
+    return false;
+}
+
+// Free pooled buffers (except `keep`, which the pending request will use),
+// then evict idle sessions least recently used first, until `need` more
+// bytes fit.
+static bool make_room(gcfg_server *s, size_t need, const Buffers *keep) {
     while (s->used + need > s->budget) {
+        int big = -1;
+        for (size_t i = 0; i < s->pool.size(); ++i)
+            if (s->pool[i] != keep && (big < 0 || s->pool[i]->bytes() > s->pool[big]->bytes())) big = (int)i;
+        if (big >= 0) {
+            free_pool_entry(s, (size_t)big);
+            continue;
+        }
         Session *victim = nullptr;
         for (auto &kv : s->sessions) {
             Session *ss = kv.second;
-            if (ss->busy) continue;
-            bool pending = false;
-            for (Request *q : s->queue)
-                if ((q->kind == REQ_EDIT || q->kind == REQ_FRAME) && q->session == ss->id) pending = true;
-            if (pending) continue;
+            if (ss->busy || session_has_queued(s, ss->id)) continue;
             if (!victim || ss->last_used < victim->last_used) victim = ss;
         }
         if (!victim) return false;
@@ -121,75 +203,281 @@ static bool make_room(gcfg_server *s, size_t need) {
     return true;
 }
 
+// The smallest pooled buffer set that holds a W x H session and is not more
+// than twice its size, or -1.
+static int pooled_for(const gcfg_server *s, int W, int H) {
+    const size_t n = (size_t)W * H;
+    int best = -1;
+    for (size_t i = 0; i < s->pool.size(); ++i) {
+        const size_t cap = s->pool[i]->dt.capacity();
+        if (cap >= n && cap <= 2 * n && (best < 0 || cap < s->pool[best]->dt.capacity())) best = (int)i;
+    }
+    return best;
+}
+
+// Buffers for a W x H session: a pooled set (pooled_for) or a new one.
+static int take_buffers(gcfg_server *s, int W, int H, Buffers **out) {
+    const size_t n = (size_t)W * H;
+    const size_t lb = std::max(label_bytes_for(W, H, GCFG_LABELS_INT32), label_bytes_for(W, H, GCFG_LABELS_BITS));
+    const int best = pooled_for(s, W, H);
+    Buffers *b = nullptr;
+    if (best >= 0) {
+        b = s->pool[best];
+        s->pool.erase(s->pool.begin() + best);
+        s->used -= std::min(s->used, b->bytes());
+        int rc = b->dt.reshape(W, H);
+        if (rc == GCFG_OK && b->label_bytes < lb) {
+            (void)hipFree(b->d_label);
+            b->d_label = nullptr;
+            b->label_bytes = 0;
+            if (hipMalloc(&b->d_label, lb) != hipSuccess) {
+                b->d_label = nullptr;
+                set_error("server: label buffer allocation failed");
+                rc = GCFG_ERR_NOMEM;
+            } else {
+                b->label_bytes = lb;
+            }
+        }
+        if (rc != GCFG_OK) {
+            delete b;
+            return rc;
+        }
+    } else {
+        b = new (std::nothrow) Buffers();
+        if (!b) return GCFG_ERR_NOMEM;
+        int rc = b->dt.init(W, H, s->eng.stream(), true, n);
+        if (rc == GCFG_OK) {
+            if (hipMalloc(&b->d_label, lb) != hipSuccess) {
+                b->d_label = nullptr;
+                set_error("server: label buffer allocation failed");
+                rc = GCFG_ERR_NOMEM;
+            } else {
+                b->label_bytes = lb;
+            }
+        }
+        if (rc != GCFG_OK) {
+            delete b;
+            return rc;
+        }
+    }
+    s->used += b->bytes();
+    *out = b;
+    return GCFG_OK;
+}
+
 static int admit_image(gcfg_server *s, Request *r) {
     Session *ss = new (std::nothrow) Session();
     if (!ss) return GCFG_ERR_NOMEM;
     ss->id = r->session;
     ss->W = r->W;
     ss->H = r->H;
-    ss->bytes = session_cost(s->eng, r->W, r->H);
-    ss->dt = new (std::nothrow) DtContext();
-    int rc = ss->dt ? ss->dt->init(r->W, r->H, s->eng.stream(), true) : GCFG_ERR_NOMEM;
-    if (rc == GCFG_OK && hipMalloc((void **)&ss->d_label, (size_t)r->W * r->H * sizeof(int32_t)) != hipSuccess) {
-        set_error("server: label buffer allocation failed");
-        rc = GCFG_ERR_NOMEM;
+    int rc = take_buffers(s, r->W, r->H, &ss->buf);
+    if (rc != GCFG_OK) {
+        delete ss;
+        return rc;
     }
-    if (rc == GCFG_OK) rc = ss->dt->upload_image(r->rgb);
-    if (rc == GCFG_OK) rc = ss->dt->upload_seeds(r->seeds);
-    if (rc == GCFG_OK) rc = ss->dt->compute_owned();
-    if (rc == GCFG_OK) rc = ss->dt->freeze_model();  // later frames of the session reuse this colour model
+    DtContext &dt = ss->buf->dt;
+    rc = dt.upload_on(s->up, r->rgb, r->seeds);
+    if (rc == GCFG_OK && (hipEventRecord(s->up_ev, s->up) != hipSuccess ||
+                          hipStreamWaitEvent(s->eng.stream(), s->up_ev, 0) != hipSuccess)) {
+        set_error("server: ordering the upload failed");
+        rc = GCFG_ERR_HIP;
+    }
+    if (rc == GCFG_OK) rc = dt.compute_owned();
+    if (rc == GCFG_OK) rc = dt.freeze_model();  // later frames of the session reuse this colour model
     if (rc == GCFG_OK) {
-        ss->slot = s->eng.add_instance(r->W, r->H, ss->dt->pos_, ss->dt->neg_, ss->dt->up_, ss->dt->down_,
-                                       ss->dt->left_, ss->dt->right_, 0, ss->d_label);
+        const bool bits = r->format == GCFG_LABELS_BITS;
+        ss->slot = s->eng.add_instance(r->W, r->H, dt.pos_, dt.neg_, dt.up_, dt.down_, dt.left_, dt.right_, 0,
+                                       bits ? nullptr : (int32_t *)ss->buf->d_label);
         if (ss->slot < 0) rc = ss->slot;
+        else s->eng.set_label_bits(ss->slot, bits ? (uint64_t *)ss->buf->d_label : nullptr);
     }
     if (rc == GCFG_OK) rc = s->eng.build({ss->slot});
     if (rc != GCFG_OK) {
         if (ss->slot >= 0) s->eng.remove_instance(ss->slot);
-        delete ss->dt;
-        if (ss->d_label) (void)hipFree(ss->d_label);
+        (void)s->eng.sync();  // work queued on the buffers ends before they serve another session
+        s->pool.push_back(ss->buf);
         delete ss;
         return rc;
     }
-    s->used += ss->bytes;
+    ss->tile_bytes = tile_bytes_for(s->eng, r->W, r->H);
+    s->used += ss->tile_bytes;
     s->sessions[ss->id] = ss;
     s->by_slot[ss->slot] = ss;
+    s->hi_slot = std::max(s->hi_slot, ss->slot + 1);
     ss->busy = true;
     ss->last_used = now_ms();
     return GCFG_OK;
 }
 
-static int collect(gcfg_server *s) {
-    if (s->running.empty()) return GCFG_OK;
-    int rc = s->eng.fetch_states(s->states.data(), s->cap_slots);
-    if (rc != GCFG_OK) return rc;
+// Label destination of a session's next solve in the request's format.
+static void point_labels(gcfg_server *s, Session *ss, int32_t format) {
+    const bool bits = format == GCFG_LABELS_BITS;
+    const InstDesc &d = s->eng.desc(ss->slot);
+    s->eng.set_inputs(ss->slot, d.pos, d.neg, d.up, d.down, d.left, d.right, d.penalty,
+                      bits ? nullptr : (int32_t *)ss->buf->d_label);
+    s->eng.set_label_bits(ss->slot, bits ? (uint64_t *)ss->buf->d_label : nullptr);
+}
+
+static hipEvent_t take_event(gcfg_server *s) {
+    if (!s->events.empty()) {
+        hipEvent_t e = s->events.back();
+        s->events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return e;
+}
+
+static int take_staging(gcfg_server *s, size_t bytes, Staging *out) {
+    int best = -1;
+    for (size_t i = 0; i < s->staging.size(); ++i)
+        if (s->staging[i].bytes >= bytes && (best < 0 || s->staging[i].bytes < s->staging[best].bytes))
+            best = (int)i;
+    if (best >= 0) {
+        *out = s->staging[best];
+        s->staging.erase(s->staging.begin() + best);
+        return GCFG_OK;
+    }
+    Staging st;
+    st.bytes = std::max<size_t>(bytes, 4096);
+    if (hipHostMalloc(&st.h, st.bytes, hipHostMallocDefault) != hipSuccess) {
+        set_error("server: pinned staging allocation failed");
+        return GCFG_ERR_NOMEM;
+    }
+    *out = st;
+    return GCFG_OK;
+}
+
+static void finish(gcfg_server *s, Request *r, int status) {
+    auto it = s->sessions.find(r->session);
+    if (it != s->sessions.end()) {
+        it->second->busy = false;
+        it->second->last_used = now_ms();
+    }
+    push_result(s, r, status, status == GCFG_OK ? r->flow : 0);
+    if (r->stg.h) s->staging.push_back(r->stg);
+    if (r->ev) s->events.push_back(r->ev);
+    delete r;
+}
+
+// Requests whose instance a completed snapshot shows done: queue the label
+// copy on the download stream.
+static int process_snapshot(gcfg_server *s, const Snapshot &sn) {
     std::vector<Request *> still;
     for (Request *r : s->running) {
         auto it = s->sessions.find(r->session);
         if (it == s->sessions.end()) {
+            set_error("server: session vanished while solving");
             push_result(s, r, GCFG_ERR_STATE, 0);
             delete r;
             continue;
         }
         Session *ss = it->second;
-        const InstState &st = s->states[ss->slot];
-        if (st.phase != gcfg::PH_DONE) {
+        const InstState &st = sn.h[ss->slot];
+        if (sn.seq < r->first_snapshot || st.phase != gcfg::PH_DONE) {
             still.push_back(r);
             continue;
         }
-        int status = st.err ? GCFG_ERR_INVALID : GCFG_OK;
-        if (status == GCFG_OK && r->h_label) {
-            if (hipMemcpyAsync(r->h_label, ss->d_label, (size_t)ss->W * ss->H * sizeof(int32_t),
-                               hipMemcpyDeviceToHost, s->eng.stream()) != hipSuccess ||
-                hipStreamSynchronize(s->eng.stream()) != hipSuccess)
-                status = GCFG_ERR_HIP;
+        if (st.err) {
+            set_error("server: request inputs outside the supported range");
+            finish(s, r, GCFG_ERR_INVALID);
+            continue;
         }
-        ss->busy = false;
-        ss->last_used = now_ms();
-        push_result(s, r, status, st.flow);
-        delete r;
+        r->flow = st.flow;
+        if (!r->h_label) {
+            finish(s, r, GCFG_OK);
+            continue;
+        }
+        r->label_bytes = label_bytes_for(ss->W, ss->H, r->format);
+        int rc = take_staging(s, r->label_bytes, &r->stg);
+        r->ev = rc == GCFG_OK ? take_event(s) : nullptr;
+        if (rc == GCFG_OK && !r->ev) {
+            set_error("server: event creation failed");
+            rc = GCFG_ERR_HIP;
+        }
+        if (rc == GCFG_OK &&
+            (hipStreamWaitEvent(s->down, sn.ev, 0) != hipSuccess ||
+             hipMemcpyAsync(r->stg.h, ss->buf->d_label, r->label_bytes, hipMemcpyDeviceToHost, s->down) !=
+                 hipSuccess ||
+             hipEventRecord(r->ev, s->down) != hipSuccess)) {
+            set_error("server: label copy failed");
+            rc = GCFG_ERR_HIP;
+        }
+        if (rc != GCFG_OK) {
+            finish(s, r, rc);
+            continue;
+        }
+        s->copying.push_back(r);
     }
     s->running.swap(still);
+    return GCFG_OK;
+}
+
+// Label copies that have landed: hand the labels to the caller.
+static bool finish_copies(gcfg_server *s, bool block_first) {
+    bool any = false;
+    std::vector<Request *> still;
+    for (size_t i = 0; i < s->copying.size(); ++i) {
+        Request *r = s->copying[i];
+        hipError_t q = (block_first && i == 0) ? hipEventSynchronize(r->ev) : hipEventQuery(r->ev);
+        if (q == hipErrorNotReady) {
+            still.push_back(r);
+            continue;
+        }
+        if (q != hipSuccess) {
+            set_error("server: label copy failed");
+            finish(s, r, GCFG_ERR_HIP);
+            any = true;
+            continue;
+        }
+        std::memcpy(r->h_label, r->stg.h, r->label_bytes);
+        finish(s, r, GCFG_OK);
+        any = true;
+    }
+    s->copying.swap(still);
+    return any;
+}
+
+// Oldest pending snapshot index, or -1.
+static int oldest_snapshot(gcfg_server *s) {
+    int best = -1;
+    for (int i = 0; i < SNAPSHOTS; ++i)
+        if (s->snaps[i].pending && (best < 0 || s->snaps[i].seq < s->snaps[best].seq)) best = i;
+    return best;
+}
+
+static int drain_snapshots(gcfg_server *s, bool block, bool *progress) {
+    for (;;) {
+        const int i = oldest_snapshot(s);
+        if (i < 0) return GCFG_OK;
+        Snapshot &sn = s->snaps[i];
+        const hipError_t q = block ? hipEventSynchronize(sn.ev) : hipEventQuery(sn.ev);
+        if (q == hipErrorNotReady) return GCFG_OK;
+        if (q != hipSuccess) return gcfg::hip_fail(q, "server snapshot");
+        sn.pending = false;
+        block = false;
+        const int rc = process_snapshot(s, sn);
+        if (rc != GCFG_OK) return rc;
+        *progress = true;
+    }
+}
+
+// Launch a chunk of steps followed by a snapshot, while fewer than
+// SNAPSHOTS chunks are in flight.
+static int feed(gcfg_server *s) {
+    if (s->running.empty()) return GCFG_OK;
+    for (int i = 0; i < SNAPSHOTS; ++i) {
+        Snapshot &sn = s->snaps[i];
+        if (sn.pending) continue;
+        int rc = s->eng.pump(s->opt.steps_per_poll);
+        if (rc == GCFG_OK) rc = s->eng.snapshot_states(sn.h, s->hi_slot, sn.ev);
+        if (rc != GCFG_OK) return rc;
+        sn.seq = s->snap_seq++;
+        sn.pending = true;
+        return GCFG_OK;  // one chunk per call: admissions in between join the next chunk
+    }
     return GCFG_OK;
 }
 
@@ -236,12 +524,21 @@ static int admit(gcfg_server *s) {
             } else {
                 // Next frame: data term of the new image with the session's
                 // colour model, then the residual graph reparametrised onto it.
-                DtContext *dt = ss->dt;
-                rc = dt->upload_image(r->rgb);
-                if (rc == GCFG_OK) rc = dt->compute_swap();
+                DtContext &dt = ss->buf->dt;
+                const size_t before = ss->buf->bytes();
+                rc = dt.upload_on(s->up, r->rgb, nullptr);
+                if (rc == GCFG_OK && (hipEventRecord(s->up_ev, s->up) != hipSuccess ||
+                                      hipStreamWaitEvent(s->eng.stream(), s->up_ev, 0) != hipSuccess)) {
+                    set_error("server: ordering the upload failed");
+                    rc = GCFG_ERR_HIP;
+                }
+                if (rc == GCFG_OK) rc = dt.compute_swap();
+                s->used += ss->buf->bytes() - before;  // frame planes allocated on the first frame
                 if (rc == GCFG_OK)
-                    rc = s->eng.reframe(ss->slot, dt->pos_, dt->neg_, dt->up_, dt->down_, dt->left_, dt->right_, 0);
+                    rc = s->eng.reframe(ss->slot, dt.pos_, dt.neg_, dt.up_, dt.down_, dt.left_, dt.right_, 0);
             }
+            if (rc == GCFG_OK) point_labels(s, ss, r->format);
+            if (rc == GCFG_OK) rc = s->eng.upload_desc_slot(ss->slot);
             if (rc == GCFG_OK) rc = s->eng.rearm({ss->slot});
             if (rc != GCFG_OK) {
                 push_result(s, r, rc, 0);
@@ -249,24 +546,30 @@ static int admit(gcfg_server *s) {
                 continue;
             }
             r->t_admit = now_ms();
+            r->first_snapshot = s->snap_seq;
             ss->busy = true;
             ss->last_used = r->t_admit;
             s->running.push_back(r);
             continue;
         }
-        const size_t need = session_cost(s->eng, r->W, r->H);
-        if (need > s->budget || s->eng.tiles_for(r->W, r->H) > s->eng.cap_tiles()) {
+        const size_t full = tile_bytes_for(s->eng, r->W, r->H) + buffer_estimate(r->W, r->H);
+        if (full > s->budget || s->eng.tiles_for(r->W, r->H) > s->eng.cap_tiles()) {
             set_error("server: request larger than the HBM budget");
             push_result(s, r, GCFG_ERR_BUDGET, 0);
             delete r;
             continue;
         }
-        if (blocked || !make_room(s, need) || (size_t)s->eng.tiles_for(r->W, r->H) > s->eng.tiles_free()) {
+        // A pooled buffer set is already counted: only the tiles are new then.
+        const int pb = blocked ? -1 : pooled_for(s, r->W, r->H);
+        const Buffers *reuse = pb >= 0 ? s->pool[pb] : nullptr;
+        const size_t need = reuse ? tile_bytes_for(s->eng, r->W, r->H) : full;
+        if (blocked || !make_room(s, need, reuse) || (size_t)s->eng.tiles_for(r->W, r->H) > s->eng.tiles_free()) {
             blocked = true;
             keep.push_back(r);
             continue;
         }
         r->t_admit = now_ms();
+        r->first_snapshot = s->snap_seq;
         int rc = admit_image(s, r);
         if (rc == GCFG_ERR_BUDGET) {  // tile range fragmented: wait for frees
             blocked = true;
@@ -307,13 +610,34 @@ int gcfg_server_create(size_t hbm_budget_bytes, const gcfg_options *opt, gcfg_se
     const size_t per_tile = Engine::bytes_per_tile() + (size_t)gcfg::TN * 32;
     const int tiles = (int)std::max<size_t>(1, hbm_budget_bytes / per_tile);
     int rc = s->eng.init(tiles, s->cap_slots, s->opt);
+    if (rc == GCFG_OK && (hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess ||
+                          hipStreamCreateWithFlags(&s->down, hipStreamNonBlocking) != hipSuccess ||
+                          hipEventCreateWithFlags(&s->up_ev, hipEventDisableTiming) != hipSuccess)) {
+        set_error("gcfg_server_create: stream or event creation failed");
+        rc = GCFG_ERR_HIP;
+    }
+    for (int i = 0; rc == GCFG_OK && i < SNAPSHOTS; ++i) {
+        if (hipHostMalloc((void **)&s->snaps[i].h, (size_t)s->cap_slots * sizeof(InstState),
+                          hipHostMallocDefault) != hipSuccess ||
+            hipEventCreateWithFlags(&s->snaps[i].ev, hipEventDisableTiming) != hipSuccess) {
+            set_error("gcfg_server_create: snapshot allocation failed");
+            rc = GCFG_ERR_NOMEM;
+        }
+    }
     if (rc != GCFG_OK) {
-        delete s;
+        gcfg_server_destroy(s);
         return rc;
     }
-    s->used = 0;
-    s->states.resize(s->cap_slots);
     *out = s;
+    return GCFG_OK;
+}
+
+int gcfg_server_set_label_format(gcfg_server *s, int32_t format) {
+    if (!s || (format != GCFG_LABELS_INT32 && format != GCFG_LABELS_BITS)) {
+        set_error("gcfg_server_set_label_format: format must be GCFG_LABELS_INT32 or GCFG_LABELS_BITS");
+        return GCFG_ERR_INVALID;
+    }
+    s->label_format = format;
     return GCFG_OK;
 }
 
@@ -334,6 +658,7 @@ int gcfg_server_submit_image(gcfg_server *s, int32_t width, int32_t height, cons
     r->rgb = h_rgb;
     r->seeds = h_seeds;
     r->h_label = h_label;
+    r->format = s->label_format;
     s->queue.push_back(r);
     if (request_id) *request_id = r->id;
     if (session) *session = r->session;
@@ -355,6 +680,7 @@ int gcfg_server_submit_edit(gcfg_server *s, int32_t session, int32_t n_strokes, 
     r->strokes.assign(strokes, strokes + 5 * (size_t)n_strokes);
     r->delta = delta;
     r->h_label = h_label;
+    r->format = s->label_format;
     s->queue.push_back(r);
     if (request_id) *request_id = r->id;
     return GCFG_OK;
@@ -376,6 +702,7 @@ int gcfg_server_submit_frame(gcfg_server *s, int32_t session, int32_t width, int
     r->H = height;
     r->rgb = h_rgb;
     r->h_label = h_label;
+    r->format = s->label_format;
     s->queue.push_back(r);
     if (request_id) *request_id = r->id;
     return GCFG_OK;
@@ -386,15 +713,28 @@ int gcfg_server_poll(gcfg_server *s, int32_t wait, int32_t max_results, gcfg_res
     if (!s || (max_results > 0 && !results)) return GCFG_ERR_INVALID;
     int rc = GCFG_OK;
     for (;;) {
-        rc = collect(s);
+        bool progress = finish_copies(s, false);
+        rc = drain_snapshots(s, false, &progress);
         if (rc != GCFG_OK) break;
         rc = admit(s);
         if (rc != GCFG_OK) break;
-        if (!s->running.empty()) {
-            rc = s->eng.pump(s->opt.steps_per_poll);
-            if (rc != GCFG_OK) break;
+        rc = feed(s);
+        if (rc != GCFG_OK) break;
+        const bool idle = s->running.empty() && s->queue.empty() && s->copying.empty();
+        if (!wait || !s->ready.empty() || idle) break;
+        if (!progress) {
+            // Nothing moved: wait for the oldest chunk in flight, else the oldest label copy.
+            if (oldest_snapshot(s) >= 0) {
+                rc = drain_snapshots(s, true, &progress);
+                if (rc != GCFG_OK) break;
+            } else if (!s->copying.empty()) {
+                finish_copies(s, true);
+            } else if (!s->queue.empty()) {
+                set_error("server: queued requests cannot be admitted (budget held by busy sessions)");
+                rc = GCFG_ERR_BUDGET;
+                break;
+            }
         }
-        if (!wait || !s->ready.empty() || (s->running.empty() && s->queue.empty())) break;
     }
     int n = 0;
     while (n < max_results && !s->ready.empty()) {
@@ -419,7 +759,7 @@ int gcfg_server_close_session(gcfg_server *s, int32_t session) {
 
 int gcfg_server_load(gcfg_server *s, int32_t *in_flight, int32_t *queued, size_t *bytes_in_use) {
     if (!s) return GCFG_ERR_INVALID;
-    if (in_flight) *in_flight = (int32_t)s->running.size();
+    if (in_flight) *in_flight = (int32_t)(s->running.size() + s->copying.size());
     if (queued) *queued = (int32_t)s->queue.size();
     if (bytes_in_use) *bytes_in_use = s->used;
     return GCFG_OK;
@@ -427,12 +767,29 @@ int gcfg_server_load(gcfg_server *s, int32_t *in_flight, int32_t *queued, size_t
 
 int gcfg_server_destroy(gcfg_server *s) {
     if (!s) return GCFG_OK;
-    s->eng.sync();
+    (void)s->eng.sync();
+    if (s->up) (void)hipStreamSynchronize(s->up);
+    if (s->down) (void)hipStreamSynchronize(s->down);
     for (Request *r : s->queue) delete r;
     for (Request *r : s->running) delete r;
+    for (Request *r : s->copying) {
+        if (r->stg.h) (void)hipHostFree(r->stg.h);
+        if (r->ev) (void)hipEventDestroy(r->ev);
+        delete r;
+    }
     std::vector<Session *> all;
     for (auto &kv : s->sessions) all.push_back(kv.second);
     for (Session *ss : all) destroy_session(s, ss);
+    for (Buffers *b : s->pool) delete b;
+    for (Staging &st : s->staging) (void)hipHostFree(st.h);
+    for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
+    for (Snapshot &sn : s->snaps) {
+        if (sn.h) (void)hipHostFree(sn.h);
+        if (sn.ev) (void)hipEventDestroy(sn.ev);
+    }
+    if (s->up_ev) (void)hipEventDestroy(s->up_ev);
+    if (s->up) (void)hipStreamDestroy(s->up);
+    if (s->down) (void)hipStreamDestroy(s->down);
     delete s;
     return GCFG_OK;
 }

@@ -30,6 +30,7 @@ GCFG_ERR_NOT_CONVERGED = -5
 GCFG_ERR_BUDGET = -6
 SEED_NONE, SEED_FG, SEED_BG = 0, 1, 2
 DT_MODEL_HISTOGRAM, DT_MODEL_KDE = 0, 1
+LABELS_INT32, LABELS_BITS = 0, 1
 
 _I32P = ctypes.POINTER(ctypes.c_int32)
 _I64P = ctypes.POINTER(ctypes.c_int64)
@@ -731,18 +732,31 @@ class Batch:
 class Server:
     """Continuous-batching segmentation server with device-resident sessions."""
 
-    def __init__(self, hbm_budget_bytes: int, options: Options | None = None):
+    def __init__(self, hbm_budget_bytes: int, options: Options | None = None, label_format: int = LABELS_INT32):
         h = ctypes.c_void_p()
         _check(lib().gcfg_server_create(int(hbm_budget_bytes), ctypes.byref(options) if options is not None else None,
                                         ctypes.byref(h)), "gcfg_server_create")
         self.h = h.value
         self._pending = {}
+        self.label_format = LABELS_INT32
+        if label_format != LABELS_INT32:
+            self.set_label_format(label_format)
+
+    def set_label_format(self, fmt: int):
+        """LABELS_INT32 (int32 [H, W]) or LABELS_BITS (uint64 [H, ceil(W/64)]) for later requests."""
+        _check(lib().gcfg_server_set_label_format(_VP(self.h), int(fmt)), "gcfg_server_set_label_format")
+        self.label_format = fmt
+
+    def _label(self, H, W):
+        if self.label_format == LABELS_BITS:
+            return np.empty((H, (W + 63) // 64), np.uint64)
+        return np.empty((H, W), np.int32)
 
     def submit_image(self, rgb: np.ndarray, seeds: np.ndarray):
         rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
         seeds = np.ascontiguousarray(seeds, dtype=np.uint8)
         H, W = seeds.shape
-        label = np.empty((H, W), np.int32)
+        label = self._label(H, W)
         rid, sess = ctypes.c_int64(), ctypes.c_int32()
         _check(lib().gcfg_server_submit_image(_VP(self.h), W, H, _VP(rgb.ctypes.data), _VP(seeds.ctypes.data),
                                               _VP(label.ctypes.data), ctypes.byref(rid), ctypes.byref(sess)),
@@ -754,7 +768,7 @@ class Server:
         st = np.ascontiguousarray(np.asarray(strokes, dtype=np.int32).reshape(-1, 5))
         if shape is None:
             raise ValueError("shape (H, W) of the session image is required for the label buffer")
-        label = np.empty(shape, np.int32)
+        label = self._label(*shape)
         rid = ctypes.c_int64()
         _check(lib().gcfg_server_submit_edit(_VP(self.h), int(session), st.shape[0], _VP(st.ctypes.data), int(delta),
                                              _VP(label.ctypes.data), ctypes.byref(rid)), "gcfg_server_submit_edit")
@@ -765,7 +779,7 @@ class Server:
         """Next video frame of a session (same W x H): warm start from its residual graph."""
         rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
         H, W = rgb.shape[:2]
-        label = np.empty((H, W), np.int32)
+        label = self._label(H, W)
         rid = ctypes.c_int64()
         _check(lib().gcfg_server_submit_frame(_VP(self.h), int(session), W, H, _VP(rgb.ctypes.data),
                                               _VP(label.ctypes.data), ctypes.byref(rid)), "gcfg_server_submit_frame")

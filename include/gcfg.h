@@ -284,6 +284,10 @@ int gcfg_tune_lookup(const char *path, const char *key, gcfg_options *opt);
 
 /* ---- Serving: continuous batching under an HBM budget, warm-started edits ---- */
 typedef struct gcfg_server gcfg_server;
+/* Label layouts of served requests: W*H int32 (the GC_Optimize layout), or
+ * packed one bit per pixel as gcfg_batch_solve_bits writes them (H rows of
+ * ceil(W/64) uint64 words; h_label then points to that many words). */
+enum { GCFG_LABELS_INT32 = 0, GCFG_LABELS_BITS = 1 };
 typedef struct gcfg_result {
     int64_t request_id;
     int32_t session;
@@ -296,6 +300,9 @@ typedef struct gcfg_result {
 /* Monotonic clock (ms) shared with gcfg_result.done_ms. */
 double gcfg_now_ms(void);
 int gcfg_server_create(size_t hbm_budget_bytes, const gcfg_options *opt, gcfg_server **out);
+/* Label layout (GCFG_LABELS_*) of the requests submitted after this call
+ * (default GCFG_LABELS_INT32). No counterpart in the reference. */
+int gcfg_server_set_label_format(gcfg_server *s, int32_t format);
 /* New session: image + seed map (host), data term and solve on the device.
  * h_label (W*H int32, host) receives the labels; it must stay valid until the
  * request's result is returned by gcfg_server_poll. */
@@ -320,7 +327,11 @@ int gcfg_server_submit_edit(gcfg_server *s, int32_t session, int32_t n_strokes, 
  * solve of the same capacities. */
 int gcfg_server_submit_frame(gcfg_server *s, int32_t session, int32_t width, int32_t height, const uint8_t *h_rgb,
                              int32_t *h_label, int64_t *request_id);
-/* Advance the device and collect finished requests (non-blocking if wait == 0). */
+/* Advance the device and collect finished requests. wait == 0: one pass that
+ * never waits for the device (it admits queued requests, launches the next
+ * chunk of solver steps and returns what has finished); wait != 0: until at
+ * least one result is ready or nothing is left. A request's labels are in
+ * its h_label when its result is returned. */
 int gcfg_server_poll(gcfg_server *s, int32_t wait, int32_t max_results, gcfg_result *results, int32_t *n_results);
 /* Drop a session and return its memory to the budget. */
 int gcfg_server_close_session(gcfg_server *s, int32_t session);

@@ -143,3 +143,73 @@ def test_edit_budget(g):
     rid, _ = srv.submit_edit(sess, stroke, 4, shape=(H, W))
     assert _drain(srv, 1)[rid]["status"] == 0
     srv.close()
+
+
+def _strokes(rng, W, H, k):
+    out = []
+    for _ in range(k):
+        out.append([int(rng.integers(0, W - 16)), int(rng.integers(0, H - 16)), 16, 16, int(rng.integers(1, 3))])
+    return out
+
+
+def _apply(dpos, dneg, strokes, delta):
+    for x0, y0, w, h, cls in strokes:
+        (dpos if cls == 1 else dneg)[y0:y0 + h, x0:x0 + w] += delta
+
+
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_serving_sizes_vs_cold(g, ora, fmt):
+    """Requests of 256^2..2048^2 (two of at least 1536^2) in one stream, with
+    edits of 1-5 strokes of +2^16 on the two large sessions and on a small
+    one, all submitted together: every result (flow and labels, int32 or
+    packed) equals a cold solve of the same capacities by the oracle."""
+    rng = np.random.default_rng(40 + fmt)
+    shapes = [(2048, 2048), (1600, 1536), (1024, 1024), (300, 256), (700, 900), (256, 512)]
+    srv = g.Server(8 << 30, label_format=fmt)
+    reqs = []  # (rid, label, session index, strokes so far)
+    imgs = []
+    for i, (W, H) in enumerate(shapes):
+        img, seeds = _req(g, W, H, 60 + i)
+        imgs.append((W, H, img, seeds))
+        rid, sess, label = srv.submit_image(img, seeds)
+        reqs.append((rid, label, i, []))
+        if i in (0, 1, 3):
+            done = []
+            for e in range(2):
+                st = _strokes(rng, W, H, int(rng.integers(1, 6)))
+                done = done + st
+                rid, label = srv.submit_edit(sess, st, 1 << 16, shape=(H, W))
+                reqs.append((rid, label, i, list(done)))
+    print(f"[server] {len(reqs)} requests, label format {fmt}", flush=True)
+    res = _drain(srv, len(reqs))
+    for rid, label, i, strokes in reqs:
+        W, H, img, seeds = imgs[i]
+        assert res[rid]["status"] == 0, (i, res[rid])
+        dpos = np.zeros((H, W), np.int64)
+        dneg = np.zeros((H, W), np.int64)
+        _apply(dpos, dneg, strokes, 1 << 16)
+        rf, rl = _cold(g, ora, img, seeds, W, H, dpos, dneg)
+        got = g.unpack_bits(label, W, H) if fmt == 1 else label
+        assert res[rid]["flow"] == rf, (i, len(strokes))
+        assert np.array_equal(got, rl), (i, len(strokes), int((got != rl).sum()))
+    srv.close()
+
+
+def test_pooled_buffers_serve_later_sessions(g, ora):
+    """A closed session's buffers serve the next, smaller session: its result
+    equals a cold solve (nothing of the earlier image leaks in)."""
+    srv = g.Server(1 << 30)
+    W, H = 400, 300
+    img, seeds = _req(g, W, H, 71)
+    rid, sess, _ = srv.submit_image(img, seeds)
+    assert _drain(srv, 1)[rid]["status"] == 0
+    srv.close_session(sess)
+    for W2, H2, sd in ((390, 290, 72), (300, 380, 73)):
+        img2, seeds2 = _req(g, W2, H2, sd)
+        rid, sess2, label = srv.submit_image(img2, seeds2)
+        r = _drain(srv, 1)[rid]
+        assert r["status"] == 0
+        rf, rl = _cold(g, ora, img2, seeds2, W2, H2)
+        assert r["flow"] == rf and np.array_equal(label, rl)
+        srv.close_session(sess2)
+    srv.close()
