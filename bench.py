@@ -4,17 +4,27 @@
 Workload (the batch the headline metric is quoted on): 256 instances from the
 ellipse-image generator, areas log-uniform on [128^2, 1024^2], aspect ratios
 log-uniform on [1/2, 2], 16-bin colour data term with lambda = 50 computed on
-the device (untimed setup; timed separately as `dataterm`).
+the device by the batched data term (timed separately as `dataterm`).
 
 A step is one cold solve of the whole batch: inputs resident in HBM ->
-residual graphs built -> max-flow -> labels in HBM and int64 flows on the host.
-`value` is batched Mpx/s (pixels = W*H of every instance, border included).
+residual graphs built -> max-flow -> labels on the host (packed one bit per
+pixel, one transfer for the batch) and int64 flows on the host. `value` is
+that rate in Mpx/s (pixels = W*H of every instance, border included).
 Reported beside it:
-  * plain: the GC_* drop-in (GC_SetGraph + GC_Optimize, labels left on the
-    device) looped over the same instances one at a time, and the ratio;
-  * serve: continuous-batching server on a Poisson stream (sizes
-    log-uniform on [256^2, 2048^2], 30% seed-stroke edits warm-started on
-    device-resident residual graphs): p50/p99 latency, Mpx/s;
+  * batched with the labels left in HBM, and with int32 labels copied to the
+    host;
+  * plain: the GC_* drop-in (GC_SetGraph + GC_Optimize with int32 labels to
+    the host, as the reference returns them) looped over the same instances
+    one at a time, and the ratios;
+  * first_solve_unseen: one solve of a batch from another seed on a fresh
+    handle (no step-count hint from an earlier solve), checked against the
+    oracle;
+  * serve: the continuous-batching server on Poisson streams of 1000 requests
+    per point (sizes log-uniform on [256^2, 2048^2], 30% edits of 1-5 seed
+    strokes warm-started on device-resident residual graphs), swept over
+    arrival rates until it saturates: p50/p99 latency (arrival -> labels on
+    the host), served Mpx/s; every result's flow and labels are compared with
+    a cold solve of the same capacities (serve.matches_cold);
   * video: frames of a moving 1024x768 scene on one session, warm-started
     from the previous frame's residual graph, beside the same frames solved
     cold as new images;
@@ -22,11 +32,13 @@ Reported beside it:
   * cpu_baseline: the CPU restatement of the reference push-relabel schedule
     on a bounded sample of the same instances, with its results checked
     against the GPU's.
-Every run checks that batched, plain and served flows agree bit-exactly.
+Every run checks that batched (packed and int32 labels), plain, served and
+oracle results agree bit-exactly where it compares them.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -49,10 +61,12 @@ GR_INIT_BYTES_PER_NODE = 5  # initial BFS sweep from the kept mask: mask 1 read,
 GR_RELAX_BYTES_PER_NODE = 9  # relaxation sweep: mask 1 + distance 4 read, distance 4 written
 LABEL_BYTES_PER_NODE = 4    # label written by every sweep of an exact round and by FIN
 COMPULSORY_BYTES_PER_PX = 28  # 6 int32 inputs read + int32 label written
+DATATERM_BYTES_PER_PX = 30  # RGB read by two passes + six int32 planes written
 CHILD_SOLVES = 3  # solves of the counter child run: --steps 1 --warmup 0, then two event-timed solves
 BUILD_IN_BYTES_PER_PX = 24  # pos, neg and four n-link planes read by the build
 BUILD_OUT_BYTES_PER_NODE = 9  # excess 4 + distance 4 + mask 1 written per arena node (capacities stay lazy)
 BUILD_OUT_BYTES_PER_TILE = 640  # the tile's inflow slots cleared (160 int32)
+_VP = ctypes.c_void_p
 
 
 def log(msg):
@@ -67,29 +81,25 @@ class Inst:
 
 
 def make_batch(n, seed):
+    """Instances and their data terms (one batched data-term launch per
+    pass). Returns (instances, data-term handle, data-term ms)."""
     Ws, Hs = gcfg.gen_batch_shapes(n, seed, 128.0 ** 2, 1024.0 ** 2)
-    insts = []
-    dt_ms = 0.0
+    insts, images = [], []
     for i in range(n):
         it = Inst(Ws[i], Hs[i], gcfg.mix_seed(seed, i))
-        img = gcfg.gen_ellipse_image(it.W, it.H, it.seed)
-        seeds = gcfg.gen_seed_map(it.W, it.H)
-        dt = gcfg.DataTerm(it.W, it.H)
-        dt.set_image(img)
-        dt.set_seeds(seeds)
-        t0 = time.perf_counter()
-        dt.compute()
-        dt_ms += (time.perf_counter() - t0) * 1e3
-        nbytes = 4 * it.W * it.H
-        it.bufs = []
-        for p in dt.link_ptrs():
-            b = gcfg.DeviceBuffer(nbytes)
-            b.copy_from_device(p, nbytes)
-            it.bufs.append(b)
-        dt.close()
-        it.d_label = gcfg.DeviceBuffer(nbytes)
+        images.append((gcfg.gen_ellipse_image(it.W, it.H, it.seed), gcfg.gen_seed_map(it.W, it.H)))
         insts.append(it)
-    return insts, dt_ms
+    dtb = gcfg.DataTermBatch(images)
+    dtb.compute()  # first launch of the kernels
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        dtb.compute()
+        times.append((time.perf_counter() - t0) * 1e3)
+    for i, it in enumerate(insts):
+        it.bufs = dtb.planes[i]
+        it.d_label = gcfg.DeviceBuffer(4 * it.W * it.H)
+    return insts, dtb, float(np.median(times))
 
 
 def batch_items(insts):
@@ -101,69 +111,204 @@ def host_links(it):
     return [b.download(np.int32, n).reshape(it.H, it.W) for b in it.bufs]
 
 
-def run_plain(insts, graphs):
-    flows = []
-    t0 = time.perf_counter()
-    for it, g in zip(insts, graphs):
-        g.build()
-        st = g.optimize_device(it.d_label)
-        flows.append(st.flow)
+class BitsSolver:
+    """gcfg_batch_solve_bits through preallocated ctypes arguments, so the
+    timed region is the library call."""
+
+    def __init__(self, batch):
+        self.b = batch
+        self.n = len(batch.items)
+        self.flows = np.zeros(self.n, np.int64)
+        self.ptrs = (ctypes.POINTER(ctypes.c_uint64) * self.n)()
+        self.st = gcfg.Stats()
+
+    def solve(self):
+        rc = gcfg.lib().gcfg_batch_solve_bits(_VP(self.b.h), self.ptrs, _VP(self.flows.ctypes.data),
+                                              ctypes.byref(self.st))
+        if rc != 0:
+            raise RuntimeError(f"gcfg_batch_solve_bits failed ({rc}): {gcfg.lib().gcfg_last_error().decode()}")
+        return self.flows
+
+    def words(self, i):
+        it = self.b.items[i]
+        return np.ctypeslib.as_array(self.ptrs[i], shape=(it.height, (it.width + 63) // 64))
+
+
+class Int32Solver:
+    """gcfg_batch_solve with int32 labels into preallocated host arrays."""
+
+    def __init__(self, batch):
+        self.b = batch
+        n = len(batch.items)
+        self.host = [np.empty((it.height, it.width), np.int32) for it in batch.items]
+        self.ptrs = (ctypes.c_void_p * n)(*[a.ctypes.data for a in self.host])
+        self.flows = np.zeros(n, np.int64)
+        self.st = gcfg.Stats()
+
+    def solve(self):
+        rc = gcfg.lib().gcfg_batch_solve(_VP(self.b.h), self.ptrs, _VP(self.flows.ctypes.data), ctypes.byref(self.st))
+        if rc != 0:
+            raise RuntimeError(f"gcfg_batch_solve failed ({rc}): {gcfg.lib().gcfg_last_error().decode()}")
+        return self.flows
+
+
+def time_steps(fn, steps, warmup, label):
+    for _ in range(warmup):
+        fn()
     gcfg.device_synchronize()
+    times = []
+    for s in range(steps):
+        gcfg.device_synchronize()
+        t0 = time.perf_counter()
+        fn()
+        gcfg.device_synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+        log(f"{label} step {s}: {times[-1]:.3f} ms")
+    return float(np.mean(times)), times
+
+
+def run_plain(insts, graphs, hosts):
+    """GC_SetGraph + GC_Optimize per instance, int32 labels to the host."""
+    flows = []
+    st = gcfg.Stats()
+    t0 = time.perf_counter()
+    for it, g, h in zip(insts, graphs, hosts):
+        g.build()
+        rc = gcfg.lib().gcfg_graph_optimize(_VP(g.h), _VP(h.ctypes.data), ctypes.byref(st))
+        if rc != 0:
+            raise RuntimeError(f"gcfg_graph_optimize failed ({rc})")
+        flows.append(st.flow)
     return (time.perf_counter() - t0) * 1e3, np.array(flows, np.int64)
 
 
-def serve(args, opts, qps, n):
-    """Poisson stream against the continuous-batching server."""
+# ---------------------------------------------------------------------------
+# Serving
+def serve_workload(args, n):
+    """A deterministic request list: image requests drawn from a pool of
+    distinct images (areas log-uniform on [256^2, 2048^2], aspect 1:2..2:1,
+    ellipse generator, in pinned host memory), 30% edits of 1-5 random 16x16
+    strokes on one of the 8 most recent sessions."""
     rng = np.random.default_rng(args.seed + 7)
     amin, amax = 256.0 ** 2, 2048.0 ** 2
-    reqs = []
-    sessions = []  # indices of image requests
+    pool = []
+    for k in range(args.serve_pool):
+        a = math.exp(math.log(amin) + rng.random() * (math.log(amax) - math.log(amin)))
+        r = math.exp(math.log(0.5) + rng.random() * math.log(4.0))
+        W, H = int(round(math.sqrt(a * r))), int(round(math.sqrt(a / r)))
+        img = gcfg.pinned_empty((H, W, 3), np.uint8)
+        img[...] = gcfg.gen_ellipse_image(W, H, gcfg.mix_seed(args.seed + 7, k))
+        seeds = gcfg.pinned_empty((H, W), np.uint8)
+        seeds[...] = gcfg.gen_seed_map(W, H)
+        pool.append((W, H, img, seeds))
+    reqs, live = [], []
     for i in range(n):
-        if sessions and rng.random() < 0.3:
-            j = int(rng.choice(sessions))
+        if live and rng.random() < 0.3:
+            j = live[int(rng.integers(len(live)))]
             W, H = reqs[j]["W"], reqs[j]["H"]
-            k = int(rng.integers(1, 6))
-            strokes = []
-            for _ in range(k):
-                x0 = int(rng.integers(0, max(1, W - 16)))
-                y0 = int(rng.integers(0, max(1, H - 16)))
-                strokes.append([x0, y0, 16, 16, int(rng.integers(1, 3))])
+            strokes = [[int(rng.integers(0, max(1, W - 16))), int(rng.integers(0, max(1, H - 16))), 16, 16,
+                        int(rng.integers(1, 3))] for _ in range(int(rng.integers(1, 6)))]
             reqs.append(dict(kind="edit", target=j, W=W, H=H, strokes=np.array(strokes, np.int32)))
         else:
-            a = math.exp(math.log(amin) + rng.random() * (math.log(amax) - math.log(amin)))
-            r = math.exp(math.log(0.5) + rng.random() * math.log(4.0))
-            W, H = int(round(math.sqrt(a * r))), int(round(math.sqrt(a / r)))
-            img = gcfg.gen_ellipse_image(W, H, gcfg.mix_seed(args.seed + 7, i))
-            reqs.append(dict(kind="image", W=W, H=H, img=img, seeds=gcfg.gen_seed_map(W, H)))
-            sessions.append(i)
+            p = int(rng.integers(len(pool)))
+            reqs.append(dict(kind="image", pool=p, W=pool[p][0], H=pool[p][1]))
+            live.append(i)
+            if len(live) > 8:
+                live.pop(0)
+    last = {}
+    for i, q in enumerate(reqs):
+        last[i if q["kind"] == "image" else q["target"]] = i
+    return pool, reqs, last
+
+
+def serve_expected(pool, reqs):
+    """Flow and packed labels of a cold plain solve of every request's
+    capacities (an edit: its session's data term plus every stroke applied to
+    the session so far)."""
+    links = []
+    for W, H, img, seeds in pool:
+        dt = gcfg.DataTerm(W, H)
+        dt.set_image(img)
+        dt.set_seeds(seeds)
+        dt.compute()
+        bufs = []
+        for p in dt.link_ptrs():
+            b = gcfg.DeviceBuffer(4 * W * H)
+            b.copy_from_device(p, 4 * W * H)
+            bufs.append(b)
+        dt.close()
+        links.append(bufs)
+    host_pn = {}
+    strokes_of = {}
+    out = []
+    for i, q in enumerate(reqs):
+        W, H = q["W"], q["H"]
+        if q["kind"] == "image":
+            bufs = links[q["pool"]]
+            strokes_of[i] = []
+            pos_b, neg_b = bufs[0], bufs[1]
+            tmp = None
+        else:
+            j = q["target"]
+            strokes_of[j].extend(q["strokes"].tolist())
+            bufs = links[reqs[j]["pool"]]
+            if reqs[j]["pool"] not in host_pn:
+                host_pn[reqs[j]["pool"]] = (bufs[0].download(np.int32, W * H).reshape(H, W),
+                                            bufs[1].download(np.int32, W * H).reshape(H, W))
+            pos, neg = (a.astype(np.int64) for a in host_pn[reqs[j]["pool"]])
+            for x0, y0, w, h, cls in strokes_of[j]:
+                (pos if cls == 1 else neg)[y0:y0 + h, x0:x0 + w] += 1 << 16
+            pos_b = gcfg.DeviceBuffer.from_array(pos.astype(np.int32))
+            neg_b = gcfg.DeviceBuffer.from_array(neg.astype(np.int32))
+            tmp = (pos_b, neg_b)
+        g = gcfg.Graph(W, H)
+        g.set_dataterms(pos_b, neg_b)
+        g.set_edges(*bufs[2:6])
+        g.build()
+        label, st = g.optimize()
+        g.close()
+        out.append((int(st.flow), gcfg.pack_bits(label)))
+        del tmp
+    return out
+
+
+def serve_point(args, opts, pool, reqs, last, expected, qps):
+    """One Poisson stream at `qps` against a fresh server (packed labels)."""
+    n = len(reqs)
+    rng = np.random.default_rng(args.seed + 13)
     gaps = rng.exponential(1000.0 / qps, n)
     arrival = np.cumsum(gaps) - gaps[0]
-    srv = gcfg.Server(args.serve_budget_gb << 30, opts)
-    sess_of = {}
-    rid_of = {}
-    lat = {}
-    flows = {}
+    srv = gcfg.Server(args.serve_budget_gb << 30, opts, label_format=gcfg.LABELS_BITS)
+    sess_of, rid_of, lat, labels = {}, {}, {}, {}
+    matches = True
     t0 = gcfg.now_ms()
-    i = 0
-    done = 0
+    i = done = 0
     while done < n:
         now = gcfg.now_ms() - t0
         while i < n and arrival[i] <= now:
             q = reqs[i]
             if q["kind"] == "image":
-                rid, sess, _ = srv.submit_image(q["img"], q["seeds"])
+                W, H, img, seeds = pool[q["pool"]]
+                rid, sess, lab = srv.submit_image(img, seeds)
                 sess_of[i] = sess
             else:
-                rid, _ = srv.submit_edit(sess_of[q["target"]], q["strokes"], 1 << 16, shape=(q["H"], q["W"]))
+                rid, lab = srv.submit_edit(sess_of[q["target"]], q["strokes"], 1 << 16, shape=(q["H"], q["W"]))
             rid_of[rid] = i
+            labels[i] = lab
             i += 1
         for r in srv.poll(wait=False):
             k = rid_of[r["request_id"]]
             if r["status"] != 0:
                 raise RuntimeError(f"serving request {k} failed with status {r['status']}")
             lat[k] = r["done_ms"] - (t0 + arrival[k])
-            flows[k] = r["flow"]
+            ef, ew = expected[k]
+            if r["flow"] != ef or not np.array_equal(labels[k], ew):
+                matches = False
+                log(f"serve: request {k} differs from its cold solve (flow {r['flow']} vs {ef})")
+            labels.pop(k)
             done += 1
+            sess = k if reqs[k]["kind"] == "image" else reqs[k]["target"]
+            if last[sess] == k:
+                srv.close_session(sess_of[sess])
         inflight, queued, _ = srv.load()
         if inflight == 0 and queued == 0 and i < n:
             wait = arrival[i] - (gcfg.now_ms() - t0)
@@ -173,10 +318,38 @@ def serve(args, opts, qps, n):
     srv.close()
     px = sum(q["W"] * q["H"] for q in reqs)
     L = np.array([lat[k] for k in range(n)])
-    return dict(requests=n, qps=qps, edits=sum(q["kind"] == "edit" for q in reqs),
+    offered = float(px / (arrival[-1] * 1e3)) if arrival[-1] > 0 else float("inf")
+    return dict(qps=qps, requests=n, edits=sum(q["kind"] == "edit" for q in reqs),
                 p50_ms=round(float(np.percentile(L, 50)), 3), p99_ms=round(float(np.percentile(L, 99)), 3),
-                mean_ms=round(float(L.mean()), 3), mpx_s=round(px / (wall * 1e3), 2), wall_ms=round(wall, 1),
-                hbm_budget_gb=args.serve_budget_gb), reqs, flows
+                mean_ms=round(float(L.mean()), 3), served_mpx_s=round(px / (wall * 1e3), 1),
+                offered_mpx_s=round(offered, 1), wall_ms=round(wall, 1), matches_cold=bool(matches))
+
+
+def serve_sweep(args, opts):
+    log(f"serving workload: {args.serve_requests} requests, pool of {args.serve_pool} images")
+    pool, reqs, last = serve_workload(args, args.serve_requests)
+    t0 = time.perf_counter()
+    expected = serve_expected(pool, reqs)
+    log(f"cold solves of every request: {time.perf_counter() - t0:.1f} s")
+    points = []
+    saturated = 0
+    for qps in args.serve_qps:
+        pt = serve_point(args, opts, pool, reqs, last, expected, qps)
+        log(f"serve: {pt}")
+        points.append(pt)
+        if pt["served_mpx_s"] < 0.8 * pt["offered_mpx_s"]:
+            saturated += 1
+            if saturated >= 2:
+                break
+    for _, _, img, seeds in pool:
+        gcfg.pinned_free(img)
+        gcfg.pinned_free(seeds)
+    px = sum(q["W"] * q["H"] for q in reqs)
+    best = max(points, key=lambda p: p["served_mpx_s"])
+    return dict(points=points, capacity_mpx_s=best["served_mpx_s"], capacity_qps=best["qps"],
+                capacity_p99_ms=best["p99_ms"], matches_cold=all(p["matches_cold"] for p in points),
+                requests_per_point=len(reqs), mean_request_mpx=round(px / len(reqs) / 1e6, 3),
+                hbm_budget_gb=args.serve_budget_gb, label_format="packed bits")
 
 
 def video(args, opts, W=1024, H=768, n=16):
@@ -232,11 +405,40 @@ def video(args, opts, W=1024, H=768, n=16):
                 warm_mpx_s=round(W * H * (n - 1) / float(np.sum(lw[1:])) / 1e3, 1))
 
 
+# ---------------------------------------------------------------------------
+# CPU baseline
+def host_cores():
+    """(threads used, nproc, OMP_NUM_THREADS): nproc as the affinity mask
+    shows it, capped by OMP_NUM_THREADS where the environment sets the
+    process's share of the host."""
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        nproc = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS")
+    try:
+        omp_n = int(omp) if omp else None
+    except ValueError:
+        omp_n = None
+    return (min(nproc, omp_n) if omp_n else nproc), nproc, omp_n
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(insts, flows_gpu, labels_gpu_fn, budget_s):
     """Reference-schedule restatement on the host cores over a bounded sample."""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
+    threads, nproc, omp_n = host_cores()
     # Size-stratified order (every 8th instance by area, then the next
     # offset, ...), so that a sample cut short by the time budget still spans
     # the size range; with the default budget it covers most of the batch.
@@ -259,8 +461,9 @@ def cpu_baseline(insts, flows_gpu, labels_gpu_fn, budget_s):
             break
     out = dict(value=round(px / secs / 1e6, 3), unit="Mpx/s", cores=threads, kind="port",
                sample=f"{len(used)} of the {len(insts)} batch instances ({px / 1e6:.1f} Mpx), "
-                      f"reference push-relabel schedule restated in C/OpenMP",
-               seconds=round(secs, 2), matches_gpu=bool(match), cpu_model=cpu_model())
+                      f"reference push-relabel schedule restated in C/OpenMP on {threads} threads",
+               seconds=round(secs, 2), matches_gpu=bool(match), cpu_model=cpu_model(), nproc=nproc,
+               omp_num_threads=omp_n, os_cpu_count=os.cpu_count())
     # The single-image configurations: 320x240 (1 thread and all threads) and
     # 1024^2 (all threads), each checked against the GPU drop-in path.
     per = {}
@@ -285,17 +488,7 @@ def cpu_baseline(insts, flows_gpu, labels_gpu_fn, budget_s):
     return out
 
 
-def cpu_model():
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return None
-
-
+# ---------------------------------------------------------------------------
 def single_instance(kind, seed):
     if kind == "c1":
         W = H = 1024
@@ -375,38 +568,48 @@ def extras(args):
 
 
 def single(args, opts):
-    """One plain GC_* drop-in solve at C1 or C2, with labels left on the device."""
+    """One plain GC_* drop-in solve at C1 or C2, labels to the host (int32)."""
     W, H, bufs = single_instance(args.single, args.seed)
     g = gcfg.Graph(W, H, opts)
     g.set_dataterms(bufs[0], bufs[1])
     g.set_edges(*bufs[2:6])
     d_label = gcfg.DeviceBuffer(4 * W * H)
+    host = np.empty((H, W), np.int32)
+    st = gcfg.Stats()
     g.build()
     g.optimize_device(d_label)
-    times, st = [], None
+    times, dev_times = [], []
     for s in range(args.warmup + args.steps):
         gcfg.device_synchronize()
         t0 = time.perf_counter()
         g.build()
-        st = g.optimize_device(d_label)
+        rc = gcfg.lib().gcfg_graph_optimize(_VP(g.h), _VP(host.ctypes.data), ctypes.byref(st))
+        if rc != 0:
+            raise RuntimeError("gcfg_graph_optimize failed")
+        t1 = time.perf_counter()
+        g.build()
+        std = g.optimize_device(d_label)
         gcfg.device_synchronize()
+        t2 = time.perf_counter()
         if s >= args.warmup:
-            times.append((time.perf_counter() - t0) * 1e3)
-        log(f"{args.single} solve {s}: {(time.perf_counter() - t0) * 1e3:.2f} ms flow {st.flow} steps {st.steps} "
-            f"gr_rounds {st.gr_rounds} pr_tiles {st.pr_tiles} gr_tiles {st.gr_tiles}")
-    ms = float(np.mean(times))
-    out = dict(workload=args.single, W=W, H=H, ms=round(ms, 3), mpx_s=round(W * H / ms / 1e3, 2), flow=int(st.flow),
-               steps=int(st.steps), gr_rounds=int(st.gr_rounds), pr_tiles=int(st.pr_tiles),
-               gr_tiles=int(st.gr_tiles), sweeps_per_gr=opts.sweeps_per_gr, inner_iters=opts.inner_iters)
+            times.append((t1 - t0) * 1e3)
+            dev_times.append((t2 - t1) * 1e3)
+        log(f"{args.single} solve {s}: {(t1 - t0) * 1e3:.2f} ms (labels to host), {(t2 - t1) * 1e3:.2f} ms "
+            f"(labels in HBM), flow {st.flow} steps {st.steps} gr_rounds {st.gr_rounds}")
+    ms, dms = float(np.mean(times)), float(np.mean(dev_times))
+    out = dict(workload=args.single, W=W, H=H, ms=round(ms, 3), mpx_s=round(W * H / ms / 1e3, 2),
+               device_resident_ms=round(dms, 3), device_resident_mpx_s=round(W * H / dms / 1e3, 2),
+               flow=int(st.flow), steps=int(st.steps), gr_rounds=int(st.gr_rounds), pr_tiles=int(st.pr_tiles),
+               gr_tiles=int(st.gr_tiles), sweeps_per_gr=opts.sweeps_per_gr, inner_iters=opts.inner_iters,
+               gr_cap=opts.gr_cap)
     if not args.no_cpu:
         import oracle
         n = W * H
-        host = [b.download(np.int32, n).reshape(H, W) for b in bufs]
+        h = [b.download(np.int32, n).reshape(H, W) for b in bufs]
         t0 = time.perf_counter()
-        rf, rl = oracle.maxflow(W, H, host[0], host[1], edges=tuple(host[2:6]))
+        rf, rl = oracle.maxflow(W, H, h[0], h[1], edges=tuple(h[2:6]))
         out["oracle_s"] = round(time.perf_counter() - t0, 2)
-        lab = d_label.download(np.int32, n).reshape(H, W)
-        out["matches_oracle"] = bool(rf == st.flow and np.array_equal(rl, lab))
+        out["matches_oracle"] = bool(rf == st.flow and np.array_equal(rl, host) and std.flow == rf)
     g.close()
     print(json.dumps(out), flush=True)
 
@@ -429,10 +632,12 @@ def pmc_traffic(args, opts, kernel_prefix, launches_per_solve):
     if not exe:
         return None, "rocprofv3 not found"
     child = [sys.executable, os.path.abspath(__file__), "--no-cpu", "--no-serve", "--no-plain", "--no-pmc",
-             "--steps", "1", "--warmup", "0", "--calibrate-copy", "--instances", str(args.instances),
+             "--no-unseen", "--steps", "1", "--warmup", "0", "--calibrate-copy", "--instances", str(args.instances),
              "--seed", str(args.seed), "--sweeps", str(opts.sweeps_per_gr), "--inner", str(opts.inner_iters),
              "--poll", str(opts.steps_per_poll), "--gr-block", f"{opts.gr_block_w}x{opts.gr_block_h}",
              "--gr-cap", str(opts.gr_cap)]
+    if opts.gr_waves_per_cu:
+        child += ["--gr-waves", str(opts.gr_waves_per_cu)]
     got = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="gcfg_pmc_")
@@ -461,6 +666,42 @@ def pmc_traffic(args, opts, kernel_prefix, launches_per_solve):
                 fetch_scale=round(fscale, 4), write_scale=round(wscale, 4), dispatches=n), "ok"
 
 
+def unseen_first_solve(args, opts):
+    """A batch from another seed, solved once on a fresh handle (the first
+    step chunk is the default length, no hint from an earlier solve; tuned
+    options of this workload class), labels packed to the host; every flow
+    and label map checked against the oracle."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+
+    insts, dtb, _ = make_batch(args.instances, args.seed + 1)
+    px = sum(it.W * it.H for it in insts)
+    b = gcfg.Batch(batch_items(insts), opts)
+    bs = BitsSolver(b)
+    gcfg.device_synchronize()
+    t0 = time.perf_counter()
+    flows = bs.solve().copy()
+    ms = (time.perf_counter() - t0) * 1e3
+    steps = int(bs.st.steps)
+    words = [bs.words(i).copy() for i in range(len(insts))]
+    bs.solve()
+    second_ms = bs.st.ms
+    b.close()
+    threads, _, _ = host_cores()
+
+    def check(k):
+        it = insts[k]
+        pos, neg, up, down, left, right = host_links(it)
+        rf, rl = oracle.maxflow(it.W, it.H, pos, neg, edges=(up, down, left, right))
+        return rf == int(flows[k]) and np.array_equal(gcfg.pack_bits(rl), words[k])
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        ok = all(ex.map(check, range(len(insts))))
+    dtb.close()
+    return dict(seed=args.seed + 1, ms=round(ms, 3), mpx_s=round(px / ms / 1e3, 1), steps=steps,
+                second_solve_ms=round(second_ms, 3), matches_oracle=bool(ok), total_mpx=round(px / 1e6, 3))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -468,13 +709,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--instances", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2024)
-    ap.add_argument("--qps", type=float, default=40.0)
-    ap.add_argument("--serve-requests", type=int, default=120)
-    ap.add_argument("--serve-budget-gb", type=int, default=32)
+    ap.add_argument("--serve-qps", type=float, nargs="+", default=[250, 500, 1000, 2000, 4000, 8000, 16000])
+    ap.add_argument("--serve-requests", type=int, default=1000)
+    ap.add_argument("--serve-pool", type=int, default=96, help="distinct images the served requests draw from")
+    ap.add_argument("--serve-budget-gb", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="time budget of the CPU baseline's batch sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-unseen", action="store_true")
     ap.add_argument("--sweeps", type=int, default=0)
     ap.add_argument("--inner", type=int, default=0)
     ap.add_argument("--poll", type=int, default=0)
@@ -486,8 +729,6 @@ def main():
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs for roofline.traffic")
     ap.add_argument("--tune-cache", default=os.environ.get("GCFG_TUNE_CACHE",
                                                            os.path.expanduser("~/.cache/gcfg/tune.txt")))
-    ap.add_argument("--serve-qps-high", type=float, default=400.0)
-    ap.add_argument("--serve-requests-high", type=int, default=200)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="copy 1 GiB device-to-device first (a known-size kernel for profiler counters)")
     ap.add_argument("--extras", action="store_true",
@@ -504,20 +745,6 @@ def main():
         raise SystemExit("no GPU visible: bench.py needs an MI355X")
 
     opts = gcfg.default_options()
-    if args.extras:
-        extras(args)
-        return
-    if args.single:
-        if args.sweeps:
-            opts.sweeps_per_gr = args.sweeps
-        if args.inner:
-            opts.inner_iters = args.inner
-        if args.gr_block:
-            opts.gr_block_w, opts.gr_block_h = (int(v) for v in args.gr_block.split("x"))
-        if args.gr_cap is not None:
-            opts.gr_cap = args.gr_cap
-        single(args, opts)
-        return
     if args.sweeps:
         opts.sweeps_per_gr = args.sweeps
     if args.inner:
@@ -532,11 +759,18 @@ def main():
         opts.gr_waves_per_cu = args.gr_waves
     if args.pr_waves:
         opts.pr_waves_per_cu = args.pr_waves
+    if args.extras:
+        extras(args)
+        return
+    if args.single:
+        single(args, opts)
+        return
 
-    log(f"generating {args.instances} instances + device data terms")
-    insts, dt_ms = make_batch(args.instances, args.seed)
+    log(f"generating {args.instances} instances + batched device data term")
+    insts, dtb, dt_ms = make_batch(args.instances, args.seed)
     total_px = sum(it.W * it.H for it in insts)
-    log(f"batch: {total_px / 1e6:.2f} Mpx, data term {dt_ms:.1f} ms")
+    dt_gbs = DATATERM_BYTES_PER_PX * total_px / (dt_ms * 1e-3) / 1e9
+    log(f"batch: {total_px / 1e6:.2f} Mpx, data term {dt_ms:.3f} ms ({dt_gbs:.0f} GB/s at 30 B/px)")
 
     if args.calibrate_copy:
         a, b = gcfg.DeviceBuffer(1 << 30), gcfg.DeviceBuffer(1 << 30)
@@ -548,7 +782,8 @@ def main():
     tuned = None
     if not args.no_autotune and not (args.sweeps or args.inner or args.poll or args.gr_block or
                                      args.gr_cap is not None):
-        key = f"batch:ellipse:n{args.instances}:seed{args.seed}"
+        # One cache entry per workload class (instance count), not per seed.
+        key = f"batch:ellipse:n{args.instances}"
         cache = args.tune_cache
         try:
             os.makedirs(os.path.dirname(cache) or ".", exist_ok=True)
@@ -567,20 +802,26 @@ def main():
         batch.close()
         opts = tuned
         batch = gcfg.Batch(batch_items(insts), opts)
-    for _ in range(args.warmup):
-        flows_b, _, _ = batch.solve(labels_to_host=False)
-    gcfg.device_synchronize()
-    times = []
-    st = None
-    for s in range(args.steps):
-        gcfg.device_synchronize()
-        t0 = time.perf_counter()
-        flows_b, _, st = batch.solve(labels_to_host=False)
-        gcfg.device_synchronize()
-        times.append((time.perf_counter() - t0) * 1e3)
-        log(f"batched step {s}: {times[-1]:.2f} ms, {total_px / times[-1] / 1e3:.1f} Mpx/s, steps {st.steps}")
-    ms = float(np.mean(times))
+
+    # Headline: labels on the host, packed one bit per pixel.
+    bits = BitsSolver(batch)
+    ms, _ = time_steps(bits.solve, args.steps, args.warmup, "batched (labels on host, packed)")
     value = total_px / ms / 1e3
+    flows_b = bits.flows.copy()
+    st = bits.st
+    words_b = [bits.words(i).copy() for i in range(len(insts))]
+    # The same solves with int32 labels to the host, and with labels left in HBM.
+    i32 = Int32Solver(batch)
+    ms_i32, _ = time_steps(i32.solve, max(3, args.steps // 4), 1, "batched (int32 labels to host)")
+    assert np.array_equal(i32.flows, flows_b), "int32-label and packed-label solves differ"
+    for k in range(len(insts)):
+        assert np.array_equal(gcfg.pack_bits(i32.host[k]), words_b[k]), f"packed labels differ on instance {k}"
+    dev_flows = {}
+
+    def dev_solve():
+        dev_flows["f"] = batch.solve(labels_to_host=False)[0]
+    ms_dev, _ = time_steps(dev_solve, args.steps, 1, "batched (labels in HBM)")
+    assert np.array_equal(dev_flows["f"], flows_b)
 
     # Event-timed solve of the same batch for the roofline.
     popts = gcfg.Options.from_buffer_copy(opts)
@@ -592,8 +833,8 @@ def main():
     pr_bytes = pst.pr_tiles * 1024 * PR_BYTES_PER_NODE
     per_item = 1024 * max(1, opts.gr_block_w) * max(1, opts.gr_block_h)  # nodes of one relabel item
     gr_bytes = per_item * (pst.gr_init_tiles * GR_INIT_BYTES_PER_NODE +
-                       (pst.gr_tiles - pst.gr_init_tiles) * GR_RELAX_BYTES_PER_NODE +
-                       (pst.gr_exact_tiles + pst.fin_items) * LABEL_BYTES_PER_NODE)
+                           (pst.gr_tiles - pst.gr_init_tiles) * GR_RELAX_BYTES_PER_NODE +
+                           (pst.gr_exact_tiles + pst.fin_items) * LABEL_BYTES_PER_NODE)
     wave = opts.kernels != 0
     # Build: the caller's six input planes read, residual graph written
     # (arena nodes incl. padding), labels too when the first sweep is exact.
@@ -618,15 +859,16 @@ def main():
                     kernels=per_kernel, steps=int(pst.launches),
                     pr_kernel_ms=round(pst.pr_kernel_ms, 3), gr_kernel_ms=round(pst.gr_kernel_ms, 3),
                     sched_kernel_ms=round(pst.sched_kernel_ms, 3), build_kernel_ms=round(pst.build_kernel_ms, 3),
-                    solve_alg_frac=round(solve_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    compulsory_frac=round(COMPULSORY_BYTES_PER_PX * total_px / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+                    solve_alg_frac=round(solve_bytes / (ms_dev * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    compulsory_frac=round(COMPULSORY_BYTES_PER_PX * total_px / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                          5),
                     pr_tiles=int(pst.pr_tiles), gr_tiles=int(pst.gr_tiles),
                     gr_init_tiles=int(pst.gr_init_tiles), gr_exact_tiles=int(pst.gr_exact_tiles),
                     fin_items=int(pst.fin_items), arena_tiles=int(n_tiles))
     assert np.array_equal(flows_p, flows_b), "profiled solve differs"
     pbatch.close()
     if not args.no_pmc:
-        name = {"k_prw": "k_prw(", "k_pr": "k_pr(", "k_grb": "k_grb<", "k_gr": "k_gr(", "k_buildw": "k_buildw(",
+        name = {"k_prw": "k_prw", "k_pr": "k_pr(", "k_grb": "k_grb<", "k_gr": "k_gr(", "k_buildw": "k_buildw(",
                 "k_build": "k_build("}[dom]
         traffic, note = pmc_traffic(args, opts, name, dlaunch)
         if traffic:
@@ -637,7 +879,10 @@ def main():
             roofline["traffic_note"] = note
             log(f"traffic not measured: {note}")
 
-    extra = {}
+    extra = dict(batched_labels_int32_to_host_mpx_s=round(total_px / ms_i32 / 1e3, 1),
+                 batched_labels_int32_to_host_ms=round(ms_i32, 3),
+                 batched_device_resident_mpx_s=round(total_px / ms_dev / 1e3, 1),
+                 batched_device_resident_ms=round(ms_dev, 3))
     if not args.no_plain:
         # The plain path gets its own tuned options (on the median-size instance).
         popts = gcfg.Options.from_buffer_copy(opts)
@@ -652,7 +897,7 @@ def main():
             extra["plain_options"] = {k: getattr(popts, k) for k in ("sweeps_per_gr", "inner_iters", "gr_rounds",
                                                                    "steps_per_poll", "gr_block_w", "gr_block_h",
                                                                    "gr_cap")}
-        graphs = []
+        graphs, hosts = [], []
         for it in insts:
             g = gcfg.Graph(it.W, it.H, popts)
             g.set_dataterms(it.bufs[0], it.bufs[1])
@@ -660,31 +905,37 @@ def main():
             g.build()
             g.optimize_device(it.d_label)
             graphs.append(g)
-        plain_ms, flows_plain = run_plain(insts, graphs)
-        log(f"plain pass: {plain_ms:.1f} ms, {total_px / plain_ms / 1e3:.2f} Mpx/s")
+            hosts.append(np.empty((it.H, it.W), np.int32))
+        run_plain(insts, graphs, hosts)  # warm-up pass
+        plain_ms, flows_plain = run_plain(insts, graphs, hosts)
+        log(f"plain pass (int32 labels to host): {plain_ms:.1f} ms, {total_px / plain_ms / 1e3:.2f} Mpx/s")
         assert np.array_equal(flows_plain, flows_b), "plain and batched flows differ"
-        for k in range(0, len(insts), max(1, len(insts) // 8)):
-            it = insts[k]
-            a = it.d_label.download(np.int32, it.W * it.H)
-            b = batch.device_labels(k).ravel()
-            assert np.array_equal(a, b), f"plain and batched labels differ on instance {k}"
+        for k in range(len(insts)):
+            assert np.array_equal(gcfg.pack_bits(hosts[k]), words_b[k]), f"plain and batched labels differ ({k})"
+        t0 = time.perf_counter()
+        for it, g in zip(insts, graphs):
+            g.build()
+            g.optimize_device(it.d_label)
+        gcfg.device_synchronize()
+        plain_dev_ms = (time.perf_counter() - t0) * 1e3
         extra["plain_mpx_s"] = round(total_px / plain_ms / 1e3, 3)
         extra["plain_ms"] = round(plain_ms, 2)
+        extra["plain_device_resident_mpx_s"] = round(total_px / plain_dev_ms / 1e3, 3)
         extra["speedup_vs_plain"] = round(value / (total_px / plain_ms / 1e3), 2)
+        extra["speedup_vs_plain_device_resident"] = round(ms_dev and (plain_dev_ms / ms_dev), 2)
         for g in graphs:
             g.close()
 
-    if not args.no_serve:
-        log(f"serving {args.serve_requests} requests at {args.qps} qps")
-        sv, reqs, sflows = serve(args, opts, args.qps, args.serve_requests)
-        extra["serve"] = sv
-        log(f"serve: {sv}")
-        log(f"serving {args.serve_requests_high} requests at {args.serve_qps_high} qps")
-        sv2, _, _ = serve(args, opts, args.serve_qps_high, args.serve_requests_high)
-        extra["serve_high_load"] = sv2
-        log(f"serve (high load): {sv2}")
+    if not args.no_unseen:
+        log("first solve of an unseen batch")
+        extra["first_solve_unseen"] = unseen_first_solve(args, opts)
+        log(f"unseen: {extra['first_solve_unseen']}")
 
     if not args.no_serve:
+        sv = serve_sweep(args, opts)
+        extra["serve"] = sv
+        if "plain_mpx_s" in extra:
+            extra["served_speedup_vs_plain"] = round(sv["capacity_mpx_s"] / extra["plain_mpx_s"], 2)
         log("video loop: warm frames on a session vs cold images")
         extra["video"] = video(args, opts)
         log(f"video: {extra['video']}")
@@ -692,7 +943,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (reference schedule restated on the host)")
-        cpu = cpu_baseline(insts, flows_b, lambda k: batch.device_labels(k), args.cpu_seconds)
+        cpu = cpu_baseline(insts, flows_b, lambda k: gcfg.unpack_bits(words_b[k], insts[k].W, insts[k].H),
+                           args.cpu_seconds)
         log(f"cpu: {cpu}")
     batch.close()
 
@@ -701,18 +953,22 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": f"batch of {args.instances} ellipse images, area log-uniform 128^2..1024^2, "
-                               f"aspect 1:2..2:1, 16-bin colour data term, lambda 50, cold solves",
+                               f"aspect 1:2..2:1, 16-bin colour data term, lambda 50, cold solves, labels to the "
+                               f"host packed one bit per pixel",
                    "instances": args.instances, "total_mpx": round(total_px / 1e6, 3), "seed": args.seed,
                    "sweeps_per_gr": opts.sweeps_per_gr, "inner_iters": opts.inner_iters,
                    "steps_per_poll": opts.steps_per_poll, "gr_rounds": opts.gr_rounds,
                    "gr_block": f"{opts.gr_block_w}x{opts.gr_block_h}", "gr_cap": opts.gr_cap,
-                   "autotuned": tuned is not None},
+                   "gr_waves_per_cu": opts.gr_waves_per_cu, "autotuned": tuned is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "dataterm_mpx_s": round(total_px / dt_ms / 1e3, 2),
-        "scheduler_steps": int(st.steps) if st is not None else None,
+        "dataterm": dict(ms=round(dt_ms, 3), mpx_s=round(total_px / dt_ms / 1e3, 1), alg_gb_s=round(dt_gbs, 1),
+                         frac=round(dt_gbs / HBM_PEAK_GBS, 4), bytes_per_px=DATATERM_BYTES_PER_PX,
+                         note="gcfg_dt_batch_compute wall time (three launches and a synchronisation)"),
+        "scheduler_steps": int(st.steps),
     }
     line.update(extra)
+    dtb.close()
     print(json.dumps(line), flush=True)
 
 

@@ -430,13 +430,38 @@ static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 // ---------------------------------------------------------------------------
 DtContext::~DtContext() {
-    // Teardown: errors here have nowhere to go.
-    if (stream_) (void)hipStreamSynchronize(stream_);
+    // Teardown: errors here have nowhere to go. Carved buffers belong to the
+    // caller, who frees them once no work on them is queued.
+    if (stream_ && !allocs_.empty()) (void)hipStreamSynchronize(stream_);
     for (void *p : allocs_) (void)hipFree(p);
     if (h_stats_) (void)hipHostFree(h_stats_);
 }
 
-int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs, size_t cap_px) {
+// Sizes of the buffers init() carves, in order (256-byte aligned each).
+static size_t carve_sizes(size_t n, size_t *sz) {
+    size_t k = 0;
+    sz[k++] = 3 * n;                                         // image
+    sz[k++] = n;                                             // seeds
+    sz[k++] = sizeof(DtStats);                               // stats
+    sz[k++] = sizeof(DtStats);                               // frozen model
+    sz[k++] = (n + DT_CHUNK - 1) / DT_CHUNK * sizeof(unsigned long long);
+    sz[k++] = DT_HIST * sizeof(int32_t);
+    sz[k++] = DT_LUT * sizeof(uint16_t);
+    sz[k++] = (DT_MAX_LAMBDA + 1) * sizeof(int32_t);
+    sz[k++] = KDE_MAX_SEEDS * sizeof(uint32_t);
+    sz[k++] = KDE_TAPS * sizeof(float);
+    for (int i = 0; i < 6; ++i) sz[k++] = n * 4;             // output planes
+    size_t tot = 0;
+    for (size_t i = 0; i < k; ++i) tot += (sz[i] + 255) & ~(size_t)255;
+    return tot;
+}
+
+size_t DtContext::block_bytes(size_t cap_px) {
+    size_t sz[32];
+    return carve_sizes(cap_px, sz);
+}
+
+int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs, size_t cap_px, void *block) {
     if (w <= 0 || h <= 0 || (int64_t)w * h > ((int64_t)1 << 30) || cap_px > ((size_t)1 << 30)) {
         set_error("data term: shape must be positive and at most 2^30 pixels");
         return GCFG_ERR_INVALID;
@@ -446,6 +471,38 @@ int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs, size_t c
     stream_ = stream;
     const size_t n = std::max((size_t)W * H, cap_px);
     cap_px_ = n;
+    if (block) {
+        size_t sz[32];
+        carve_sizes(n, sz);
+        uint8_t *p = static_cast<uint8_t *>(block);
+        size_t k = 0;
+        auto take = [&](void **dst) {
+            *dst = p;
+            p += (sz[k] + 255) & ~(size_t)255;
+            bytes_ += sz[k++];
+        };
+        take((void **)&own_img_);
+        take((void **)&seeds_);
+        take((void **)&d_stats_);
+        take((void **)&d_frozen_);
+        take((void **)&d_beta_part_);
+        take((void **)&d_hist_acc_);
+        take((void **)&d_tlut_);
+        take((void **)&d_thr_);
+        take((void **)&d_kde_);
+        take((void **)&d_gauss_);
+        take((void **)&pos_);
+        take((void **)&neg_);
+        take((void **)&up_);
+        take((void **)&down_);
+        take((void **)&left_);
+        take((void **)&right_);
+        carved_ = true;
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(DT_NT), 0, stream_, d_hist_acc_, (int64_t)DT_HIST, 0);
+        GCFG_HIP(hipGetLastError());
+        img_ = own_img_;
+        return GCFG_OK;
+    }
     auto dalloc = [&](void **p, size_t bytes) -> int {
         if (hipMalloc(p, bytes) != hipSuccess) {
             *p = nullptr;
@@ -476,7 +533,6 @@ int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs, size_t c
         rc |= dalloc((void **)&right_, n * 4);
     }
     if (rc != GCFG_OK) return GCFG_ERR_NOMEM;
-    GCFG_HIP(hipHostMalloc((void **)&h_stats_, sizeof(DtStats), hipHostMallocDefault));
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(DT_NT), 0, stream_, d_hist_acc_, (int64_t)DT_HIST, 0);
     GCFG_HIP(hipGetLastError());
     img_ = own_img_;
@@ -695,6 +751,7 @@ int DtContext::fetch_stats() {
         set_error("data term: compute first");
         return GCFG_ERR_STATE;
     }
+    if (!h_stats_) GCFG_HIP(hipHostMalloc((void **)&h_stats_, sizeof(DtStats), hipHostMallocDefault));
     GCFG_HIP(hipMemcpyAsync(h_stats_, d_stats_, sizeof(DtStats), hipMemcpyDeviceToHost, stream_));
     GCFG_HIP(hipStreamSynchronize(stream_));
     stats = *h_stats_;

@@ -97,8 +97,12 @@ void dt_launch(const DtLaunch &L, int total_chunks, hipStream_t s);
 class DtContext {
    public:
     ~DtContext();
-    // Buffers for images of up to max(W * H, cap_px) pixels.
-    int init(int W, int H, hipStream_t stream, bool own_outputs, size_t cap_px = 0);
+    // Buffers for images of up to max(W * H, cap_px) pixels. With `block`
+    // (device memory of at least block_bytes(cap) bytes, owned by the caller)
+    // every buffer is carved from it and nothing is allocated, and the seed
+    // map is not cleared (the caller uploads one before computing).
+    int init(int W, int H, hipStream_t stream, bool own_outputs, size_t cap_px = 0, void *block = nullptr);
+    static size_t block_bytes(size_t cap_px);
     // Reuse the buffers for a W x H image (W * H <= capacity()): the
     // kernel-density colours and the frozen colour model are dropped; the
     // seed map is left as it is (upload or clear one before computing).
@@ -169,6 +173,7 @@ class DtContext {
     size_t cap_px_ = 0;
     std::vector<void *> allocs_;
     size_t bytes_ = 0;
+    bool carved_ = false;
 };
 
 }  // namespace gcfg

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <cstring>
 #include <deque>
+#include <iterator>
 #include <map>
 #include <new>
 #include <vector>
@@ -43,15 +44,69 @@ namespace {
 
 enum { REQ_IMAGE = 0, REQ_EDIT = 1, REQ_FRAME = 2 };
 
-// Device buffers of one session.
+// One device allocation made with the server and carved into session
+// blocks (first fit, neighbouring free ranges merged), so that admitting a
+// session allocates nothing.
+class DevHeap {
+   public:
+    ~DevHeap() {
+        if (base_) (void)hipFree(base_);
+    }
+    int init(size_t bytes) {
+        bytes &= ~(size_t)255;
+        if (bytes && hipMalloc((void **)&base_, bytes) != hipSuccess) {
+            base_ = nullptr;
+            set_error("server: session heap allocation failed");
+            return GCFG_ERR_NOMEM;
+        }
+        if (bytes) free_[0] = bytes;
+        return GCFG_OK;
+    }
+    void *alloc(size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const size_t off = it->first, rem = it->second - bytes;
+            free_.erase(it);
+            if (rem) free_[off + bytes] = rem;
+            return base_ + off;
+        }
+        return nullptr;
+    }
+    void release(void *p, size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        size_t off = (size_t)(static_cast<uint8_t *>(p) - base_);
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && off + bytes == next->first) {
+            bytes += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += bytes;
+                return;
+            }
+        }
+        free_[off] = bytes;
+    }
+
+   private:
+    uint8_t *base_ = nullptr;
+    std::map<size_t, size_t> free_;  // offset -> bytes
+};
+
+// Device buffers of one session: the data term and the label map, carved
+// from one heap block.
 struct Buffers {
     DtContext dt;
+    void *block = nullptr;
+    size_t block_bytes = 0;
     void *d_label = nullptr;  // int32 labels or packed words
     size_t label_bytes = 0;
-    size_t bytes() const { return dt.device_bytes() + label_bytes; }
-    ~Buffers() {
-        if (d_label) (void)hipFree(d_label);
-    }
+    size_t dt_bytes0 = 0;  // the data term's bytes right after init (all in the block)
+    // Block plus what the data term allocated itself (video frame planes).
+    size_t bytes() const { return block_bytes + (dt.device_bytes() - dt_bytes0); }
 };
 
 // Pinned host memory that a finished request's labels are copied into.
@@ -122,6 +177,7 @@ struct gcfg_server {
     std::vector<Request *> running, copying;
     std::map<int32_t, Session *> sessions;
     std::map<int, Session *> by_slot;
+    DevHeap heap;                     // session buffers
     std::vector<Buffers *> pool;      // idle session buffers (counted in used)
     std::vector<Staging> staging;     // idle pinned label staging
     std::vector<hipEvent_t> events;   // idle events
@@ -135,18 +191,16 @@ static size_t tile_bytes_for(const Engine &eng, int W, int H) {
     return (size_t)eng.tiles_for(W, H) * Engine::bytes_per_tile();
 }
 
-// Device bytes a new session's buffers take (data term with its planes and
-// the video frame planes allocated on first use, labels).
-static size_t buffer_estimate(int W, int H) {
-    const size_t n = (size_t)W * H;
-    return n * (3 + 1 + 24 + 4) + 2 * ((size_t)H + n / 64) * 8 + 512 * 1024;
+static void free_buffers(gcfg_server *s, Buffers *b) {
+    s->used -= std::min(s->used, b->bytes());
+    if (b->block) s->heap.release(b->block, b->block_bytes);
+    delete b;
 }
 
 static void free_pool_entry(gcfg_server *s, size_t i) {
     Buffers *b = s->pool[i];
-    s->used -= std::min(s->used, b->bytes());
-    delete b;
     s->pool.erase(s->pool.begin() + (long)i);
+    free_buffers(s, b);
 }
 
 static void destroy_session(gcfg_server *s, Session *ss) {
@@ -203,6 +257,11 @@ static bool make_room(gcfg_server *s, size_t need, const Buffers *keep) {
     return true;
 }
 
+// Label bytes a W x H session needs in either layout.
+static size_t label_need(int W, int H) {
+    return std::max(label_bytes_for(W, H, GCFG_LABELS_INT32), label_bytes_for(W, H, GCFG_LABELS_BITS));
+}
+
 // The smallest pooled buffer set that holds a W x H session and is not more
 // than twice its size, or -1.
 static int pooled_for(const gcfg_server *s, int W, int H) {
@@ -210,55 +269,57 @@ static int pooled_for(const gcfg_server *s, int W, int H) {
     int best = -1;
     for (size_t i = 0; i < s->pool.size(); ++i) {
         const size_t cap = s->pool[i]->dt.capacity();
-        if (cap >= n && cap <= 2 * n && (best < 0 || cap < s->pool[best]->dt.capacity())) best = (int)i;
+        if (cap >= n && cap <= 2 * n && s->pool[i]->label_bytes >= label_need(W, H) &&
+            (best < 0 || cap < s->pool[best]->dt.capacity()))
+            best = (int)i;
     }
     return best;
 }
 
-// Buffers for a W x H session: a pooled set (pooled_for) or a new one.
+// Buffers for a W x H session: a pooled set (pooled_for) or a new block
+// from the heap (GCFG_ERR_BUDGET if the heap has no room for it now).
 static int take_buffers(gcfg_server *s, int W, int H, Buffers **out) {
     const size_t n = (size_t)W * H;
-    const size_t lb = std::max(label_bytes_for(W, H, GCFG_LABELS_INT32), label_bytes_for(W, H, GCFG_LABELS_BITS));
     const int best = pooled_for(s, W, H);
     Buffers *b = nullptr;
     if (best >= 0) {
         b = s->pool[best];
         s->pool.erase(s->pool.begin() + best);
         s->used -= std::min(s->used, b->bytes());
-        int rc = b->dt.reshape(W, H);
-        if (rc == GCFG_OK && b->label_bytes < lb) {
-            (void)hipFree(b->d_label);
-            b->d_label = nullptr;
-            b->label_bytes = 0;
-            if (hipMalloc(&b->d_label, lb) != hipSuccess) {
-                b->d_label = nullptr;
-                set_error("server: label buffer allocation failed");
-                rc = GCFG_ERR_NOMEM;
-            } else {
-                b->label_bytes = lb;
-            }
-        }
+        const int rc = b->dt.reshape(W, H);
         if (rc != GCFG_OK) {
-            delete b;
+            s->used += b->bytes();
+            free_buffers(s, b);
             return rc;
         }
     } else {
-        b = new (std::nothrow) Buffers();
-        if (!b) return GCFG_ERR_NOMEM;
-        int rc = b->dt.init(W, H, s->eng.stream(), true, n);
-        if (rc == GCFG_OK) {
-            if (hipMalloc(&b->d_label, lb) != hipSuccess) {
-                b->d_label = nullptr;
-                set_error("server: label buffer allocation failed");
-                rc = GCFG_ERR_NOMEM;
-            } else {
-                b->label_bytes = lb;
-            }
+        const size_t dtb = DtContext::block_bytes(n), lb = label_need(W, H);
+        void *block = s->heap.alloc(dtb + lb);
+        for (size_t i = 0; !block && !s->pool.empty();) {  // pooled sets that do not fit make room
+            (void)i;
+            free_pool_entry(s, 0);
+            block = s->heap.alloc(dtb + lb);
         }
+        if (!block) {
+            set_error("server: no heap block for the session (fragmented or held by live sessions)");
+            return GCFG_ERR_BUDGET;
+        }
+        b = new (std::nothrow) Buffers();
+        if (!b) {
+            s->heap.release(block, dtb + lb);
+            return GCFG_ERR_NOMEM;
+        }
+        b->block = block;
+        b->block_bytes = dtb + lb;
+        b->d_label = static_cast<uint8_t *>(block) + dtb;
+        b->label_bytes = lb;
+        const int rc = b->dt.init(W, H, s->eng.stream(), true, n, block);
         if (rc != GCFG_OK) {
+            s->heap.release(block, dtb + lb);
             delete b;
             return rc;
         }
+        b->dt_bytes0 = b->dt.device_bytes();
     }
     s->used += b->bytes();
     *out = b;
@@ -296,7 +357,7 @@ static int admit_image(gcfg_server *s, Request *r) {
     if (rc != GCFG_OK) {
         if (ss->slot >= 0) s->eng.remove_instance(ss->slot);
         (void)s->eng.sync();  // work queued on the buffers ends before they serve another session
-        s->pool.push_back(ss->buf);
+        free_buffers(s, ss->buf);
         delete ss;
         return rc;
     }
@@ -552,7 +613,8 @@ static int admit(gcfg_server *s) {
             s->running.push_back(r);
             continue;
         }
-        const size_t full = tile_bytes_for(s->eng, r->W, r->H) + buffer_estimate(r->W, r->H);
+        const size_t full = tile_bytes_for(s->eng, r->W, r->H) + DtContext::block_bytes((size_t)r->W * r->H) +
+                            label_need(r->W, r->H);
         if (full > s->budget || s->eng.tiles_for(r->W, r->H) > s->eng.cap_tiles()) {
             set_error("server: request larger than the HBM budget");
             push_result(s, r, GCFG_ERR_BUDGET, 0);
@@ -606,10 +668,16 @@ int gcfg_server_create(size_t hbm_budget_bytes, const gcfg_options *opt, gcfg_se
         gcfg_default_options(&s->opt);
     s->budget = hbm_budget_bytes;
     s->cap_slots = 1024;
-    // Arena sized for the budget if every byte went to tiles and their inputs.
+    // The budget is split between the tile arena and the session heap in the
+    // proportion one session needs them: per 1024 pixels a tile and about
+    // 32 bytes per pixel of data term, inputs and labels.
     const size_t per_tile = Engine::bytes_per_tile() + (size_t)gcfg::TN * 32;
     const int tiles = (int)std::max<size_t>(1, hbm_budget_bytes / per_tile);
     int rc = s->eng.init(tiles, s->cap_slots, s->opt);
+    if (rc == GCFG_OK) {
+        const size_t arena = (size_t)tiles * Engine::bytes_per_tile();
+        rc = s->heap.init(hbm_budget_bytes > arena ? hbm_budget_bytes - arena : 0);
+    }
     if (rc == GCFG_OK && (hipStreamCreateWithFlags(&s->up, hipStreamNonBlocking) != hipSuccess ||
                           hipStreamCreateWithFlags(&s->down, hipStreamNonBlocking) != hipSuccess ||
                           hipEventCreateWithFlags(&s->up_ev, hipEventDisableTiming) != hipSuccess)) {
@@ -780,7 +848,7 @@ int gcfg_server_destroy(gcfg_server *s) {
     std::vector<Session *> all;
     for (auto &kv : s->sessions) all.push_back(kv.second);
     for (Session *ss : all) destroy_session(s, ss);
-    for (Buffers *b : s->pool) delete b;
+    while (!s->pool.empty()) free_pool_entry(s, s->pool.size() - 1);
     for (Staging &st : s->staging) (void)hipHostFree(st.h);
     for (hipEvent_t e : s->events) (void)hipEventDestroy(e);
     for (Snapshot &sn : s->snaps) {

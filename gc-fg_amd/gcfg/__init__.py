@@ -227,6 +227,32 @@ def gpu_available() -> bool:
         return False
 
 
+_pinned_keep = {}
+
+
+def pinned_empty(shape, dtype) -> np.ndarray:
+    """A numpy array in pinned host memory (hipHostMalloc), for inputs that
+    the library copies to the device asynchronously. Freed with
+    pinned_free(array) or at exit."""
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    p = ctypes.c_void_p()
+    h = hip()
+    h.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+    _hip_check(h.hipHostMalloc(ctypes.byref(p), max(n, 1), 0), "hipHostMalloc")
+    buf = (ctypes.c_uint8 * max(n, 1)).from_address(p.value)
+    a = np.frombuffer(buf, dtype=np.uint8, count=n).view(dt).reshape(shape)
+    _pinned_keep[a.ctypes.data] = p.value
+    return a
+
+
+def pinned_free(a: np.ndarray):
+    p = _pinned_keep.pop(a.ctypes.data, None)
+    if p:
+        hip().hipHostFree.argtypes = [_VP]
+        hip().hipHostFree(_VP(p))
+
+
 def device_synchronize():
     _hip_check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 

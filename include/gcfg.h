@@ -304,8 +304,11 @@ int gcfg_server_create(size_t hbm_budget_bytes, const gcfg_options *opt, gcfg_se
  * (default GCFG_LABELS_INT32). No counterpart in the reference. */
 int gcfg_server_set_label_format(gcfg_server *s, int32_t format);
 /* New session: image + seed map (host), data term and solve on the device.
- * h_label (W*H int32, host) receives the labels; it must stay valid until the
- * request's result is returned by gcfg_server_poll. */
+ * h_label (host, in the server's label format) receives the labels. h_rgb,
+ * h_seeds and h_label must stay valid, and the inputs unchanged, until the
+ * request's result is returned by gcfg_server_poll (the image is copied to
+ * the device asynchronously; from pinned memory the copy does not hold up
+ * the calling thread). */
 int gcfg_server_submit_image(gcfg_server *s, int32_t width, int32_t height, const uint8_t *h_rgb,
                              const uint8_t *h_seeds, int32_t *h_label, int64_t *request_id, int32_t *session);
 /* Edit of a live session: n strokes of 5 ints (x0, y0, w, h, cls); a
@@ -318,8 +321,8 @@ int gcfg_server_submit_edit(gcfg_server *s, int32_t session, int32_t n_strokes, 
                             int32_t delta, int32_t *h_label, int64_t *request_id);
 /* Next video frame of a live session (the loop of ComplexApplication.cu:202-207,
  * which re-runs setImage / getDataTermsMixture / GC_SetGraph / GC_Optimize per
- * frame): a W x H RGB image of the session's size (host, valid until the
- * request is admitted). The data term is recomputed on the device with the
+ * frame): a W x H RGB image of the session's size (host, valid and unchanged
+ * until the request's result is returned). The data term is recomputed on the device with the
  * colour model of the session's first image, and the device-resident residual
  * graph is reparametrised onto the new capacities (flow beyond a new n-link
  * capacity returns to its tail node) instead of rebuilt, then solved warm.

@@ -213,3 +213,31 @@ def test_pooled_buffers_serve_later_sessions(g, ora):
         assert r["flow"] == rf and np.array_equal(label, rl)
         srv.close_session(sess2)
     srv.close()
+
+
+def test_admission_while_chunks_run(g, ora):
+    """New sessions admitted while chunks of solver steps for earlier ones
+    are still queued on the device (non-blocking polls between
+    submissions): each result equals a cold solve. This is the order in
+    which a session's data term could see a seed map that work still queued
+    ahead of it had not finished with."""
+    shapes = [(2048, 1536), (640, 480), (512, 300), (900, 700), (333, 777), (1024, 1024), (256, 256), (800, 600)]
+    srv = g.Server(8 << 30, label_format=g.LABELS_BITS)
+    subs = []
+    res = {}
+    for i, (W, H) in enumerate(shapes):
+        img, seeds = _req(g, W, H, 90 + i)
+        rid, sess, label = srv.submit_image(img, seeds)
+        subs.append((rid, label, W, H, img, seeds))
+        for _ in range(3):  # admits, launches a chunk, returns without waiting
+            for r in srv.poll(wait=False):
+                res[r["request_id"]] = r
+    while len(res) < len(subs):
+        for r in srv.poll(wait=True):
+            res[r["request_id"]] = r
+    for rid, label, W, H, img, seeds in subs:
+        assert res[rid]["status"] == 0
+        rf, rl = _cold(g, ora, img, seeds, W, H)
+        assert res[rid]["flow"] == rf, (W, H)
+        assert np.array_equal(g.unpack_bits(label, W, H), rl), (W, H)
+    srv.close()
