@@ -569,6 +569,12 @@ def extras(args):
 
 def single(args, opts):
     """One plain GC_* drop-in solve at C1 or C2, labels to the host (int32)."""
+    if args.calibrate_copy:
+        a, b = gcfg.DeviceBuffer(1 << 30), gcfg.DeviceBuffer(1 << 30)
+        b.copy_from_device(a.ptr, 1 << 30)
+        gcfg.device_synchronize()
+        a.free()
+        b.free()
     W, H, bufs = single_instance(args.single, args.seed)
     g = gcfg.Graph(W, H, opts)
     g.set_dataterms(bufs[0], bufs[1])

@@ -106,28 +106,37 @@ __global__ __launch_bounds__(DT_NT) void k_dt_stats(DtLaunch L) {
     const int64_t total = 3 * (int64_t)n;
     unsigned long long bs = 0;
     int seen = 0;
+    // Every round's loads are issued before any is used: four times the bytes
+    // in flight per lane that a load-then-compute loop keeps.
+    uint32_t w[DT_ROUNDS][5], b[DT_ROUNDS][3], sd[DT_ROUNDS];
+#pragma unroll
+    for (int r = 0; r < DT_ROUNDS; ++r) {
+        const int i0 = c * DT_CHUNK + (r * DT_NT + (int)threadIdx.x) * 4;
+        sd[r] = 0;
+        if (i0 < n) {
+            load_words<5>(J.img, 3 * (int64_t)i0 - 4, total, w[r]);  // p(-1) .. p(4)
+            load_px12(J.img, 3 * ((int64_t)i0 + W), total, b[r]);     // the four pixels below
+            if (J.seeds) {
+                if (i0 + 4 <= n) {
+                    sd[r] = *(const __attribute__((address_space(1))) uint32_t *)(J.seeds + i0);
+                } else {
+                    for (int k = 0; i0 + k < n; ++k) sd[r] |= (uint32_t)J.seeds[i0 + k] << (8 * k);
+                }
+            }
+        }
+    }
+#pragma unroll
     for (int r = 0; r < DT_ROUNDS; ++r) {
         const int i0 = c * DT_CHUNK + (r * DT_NT + (int)threadIdx.x) * 4;
         if (i0 >= n) break;
-        uint32_t w[5], b[3];
-        load_words<5>(J.img, 3 * (int64_t)i0 - 4, total, w);  // p(-1) .. p(4)
-        load_px12(J.img, 3 * ((int64_t)i0 + W), total, b);     // the four pixels below
-        uint32_t sd = 0;
-        if (J.seeds) {
-            if (i0 + 4 <= n) {
-                sd = *(const __attribute__((address_space(1))) uint32_t *)(J.seeds + i0);
-            } else {
-                for (int k = 0; i0 + k < n; ++k) sd |= (uint32_t)J.seeds[i0 + k] << (8 * k);
-            }
-        }
         int x = i0 % W, y = i0 / W;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (i0 + k < n) {
-                const Rgb pk = px_at(w, 4 + 3 * k);
-                if (x + 1 < W) bs += (unsigned)sq(pk, px_at(w, 7 + 3 * k));
-                if (y + 1 < H) bs += (unsigned)sq(pk, px_at(b, 3 * k));
-                const int s = (int)((sd >> (8 * k)) & 255u);
+                const Rgb pk = px_at(w[r], 4 + 3 * k);
+                if (x + 1 < W) bs += (unsigned)sq(pk, px_at(w[r], 7 + 3 * k));
+                if (y + 1 < H) bs += (unsigned)sq(pk, px_at(b[r], 3 * k));
+                const int s = (int)((sd[r] >> (8 * k)) & 255u);
                 if (s == GCFG_SEED_FG || s == GCFG_SEED_BG) {
                     const int cl = s - 1;
                     atomicAdd(&hist[cl * 48 + 0 * 16 + (pk.c0 >> 4)], 1);
@@ -225,10 +234,28 @@ __global__ __launch_bounds__(DT_NT) void k_dt_table(DtLaunch L) {
         } else if (nlink_exact(0, lambda, beta) < k) {
             t = -1;
         } else {
-            int lo = 0, hi = MAX_SQDIFF;  // w(lo) >= k > w(hi)
-            while (hi - lo > 1) {
-                const int mid = lo + (hi - lo) / 2;
-                if (nlink_exact(mid, lambda, beta) >= k) lo = mid; else hi = mid;
+            // w(s) >= k <=> lambda exp(-s / 2 beta) >= k - 1/2 <=> s <= 2 beta ln(lambda / (k - 1/2)):
+            // start there and step to the exact boundary (w(lo) >= k > w(lo + 1)) by evaluating w
+            // itself; the binary search is kept for a start that is far off.
+            const double est = 2.0 * beta * log((double)lambda / ((double)k - 0.5));
+            int lo = (int)fmin(fmax(floor(est), 0.0), (double)(MAX_SQDIFF - 1));
+            int steps = 0;
+            while (steps < 8 && nlink_exact(lo, lambda, beta) < k) {
+                --lo;
+                ++steps;
+            }
+            while (steps < 8 && lo + 1 <= MAX_SQDIFF && nlink_exact(lo + 1, lambda, beta) >= k) {
+                ++lo;
+                ++steps;
+            }
+            if (steps >= 8 || lo < 0 || nlink_exact(lo, lambda, beta) < k ||
+                (lo + 1 <= MAX_SQDIFF && nlink_exact(lo + 1, lambda, beta) >= k)) {
+                lo = 0;
+                int hi = MAX_SQDIFF;  // w(lo) >= k > w(hi)
+                while (hi - lo > 1) {
+                    const int mid = lo + (hi - lo) / 2;
+                    if (nlink_exact(mid, lambda, beta) >= k) lo = mid; else hi = mid;
+                }
             }
             t = lo;
         }
@@ -294,13 +321,27 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
     __syncthreads();
     const int W = J.W, H = J.H, n = J.W * J.H;
     const int64_t total = 3 * (int64_t)n;
-    for (int r = 0; r < DT_ROUNDS; ++r) {
-        const int i0 = c * DT_CHUNK + (r * DT_NT + (int)threadIdx.x) * 4;
+    // Loads of two rounds are in flight before either is used (the stores of
+    // the six planes then overlap the next pair's loads).
+#pragma unroll
+    for (int r2 = 0; r2 < DT_ROUNDS; r2 += 2) {
+    uint32_t ww[2][5], aa[2][3], bb[2][3];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int i0 = c * DT_CHUNK + ((r2 + h) * DT_NT + (int)threadIdx.x) * 4;
+        if (i0 < n) {
+            load_words<5>(J.img, 3 * (int64_t)i0 - 4, total, ww[h]);  // p(-1) .. p(4)
+            load_px12(J.img, 3 * ((int64_t)i0 - W), total, aa[h]);     // above
+            load_px12(J.img, 3 * ((int64_t)i0 + W), total, bb[h]);     // below
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int i0 = c * DT_CHUNK + ((r2 + h) * DT_NT + (int)threadIdx.x) * 4;
         if (i0 >= n) break;
-        uint32_t w[5], a[3], b[3];
-        load_words<5>(J.img, 3 * (int64_t)i0 - 4, total, w);  // p(-1) .. p(4)
-        load_px12(J.img, 3 * ((int64_t)i0 - W), total, a);     // above
-        load_px12(J.img, 3 * ((int64_t)i0 + W), total, b);     // below
+        const uint32_t(&w)[5] = ww[h];
+        const uint32_t(&a)[3] = aa[h];
+        const uint32_t(&b)[3] = bb[h];
         int hw[5];  // weight of the pair (k - 1, k)
 #pragma unroll
         for (int k = 0; k < 5; ++k) hw[k] = nlink_weight(sq(px_at(w, 1 + 3 * k), px_at(w, 4 + 3 * k)), thr, lambda);
@@ -340,6 +381,7 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
                 for (int k = 0; k < 4; ++k)
                     if (i0 + k < n) J.out[j][i0 + k] = vals[j][k];
         }
+    }
     }
 }
 

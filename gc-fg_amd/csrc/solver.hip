@@ -1570,6 +1570,52 @@ __device__ __forceinline__ void bfs_levels(int (&d)[TH], const Masks &m, int hU,
     chg |= rch;
 }
 
+// BFS of one tile whose every source is at distance 0 (the sinks and the
+// halo nodes at 0: a REINIT sweep, whose halos come from the masks), by the
+// same level sets run until they stop growing or reach the cap K (DINF: no
+// cap): d = the first level holding the node, K for the nodes no level
+// reaches. Exact for any K, in as many levels as the tile's longest BFS
+// distance, where the relaxation (relax_tile_wave) needs a round per turn of
+// a shortest path and more work per round. chg collects the rows whose value
+// differs from d on entry.
+__device__ __forceinline__ void bfs_levels_zero(int (&d)[TH], const Masks &m, int hU, int hD, int hs, int lane,
+                                                int K, uint32_t &chg, int &levels) {
+    const uint32_t PL = row_plane(m, 0), PR = row_plane(m, 1), PU = row_plane(m, 2), PD = row_plane(m, 3);
+    uint32_t S = row_plane(m, 5);
+    const uint32_t hu = hU == 0 ? 1u : 0u;
+    const uint32_t hd = hD == 0 ? (1u << (TH - 1)) : 0u;
+    const uint64_t bl = __ballot(lane < TH && hs == 0);
+    const uint64_t br = __ballot(lane >= TH && lane < 2 * TH && hs == 0);
+    const int lm = (int)((uint32_t)bl & 0xffffu), rm = (int)((uint32_t)(br >> TH) & 0xffffu);
+    int nd[TH];
+#pragma unroll
+    for (int y = 0; y < TH; ++y) nd[y] = ((S >> y) & 1u) ? 0 : K;
+    int L = 0;
+    // S only grows and has at most TN bits; a level is computed only while L + 1 < K.
+    while (L + 1 < K) {
+        const uint32_t lf = (uint32_t)__builtin_amdgcn_update_dpp(lm, (int)S, 0x138, 0xF, 0xF, false);
+        const uint32_t rt = (uint32_t)__builtin_amdgcn_update_dpp(rm, (int)S, 0x130, 0xF, 0xF, false);
+        const uint32_t Sn = (S | (((S << 1) | hu) & PU) | (((S >> 1) | hd) & PD) | (lf & PL) | (rt & PR)) & 0xffffu;
+        const uint32_t fresh = Sn & ~S;
+        if (!__any(fresh != 0)) break;
+        ++L;
+#pragma unroll
+        for (int y = 0; y < TH; ++y)
+            if ((fresh >> y) & 1u) nd[y] = L;
+        S = Sn;
+    }
+    levels = L + 1;
+    uint32_t rch = 0;
+#pragma unroll
+    for (int y = 0; y < TH; ++y) {
+        if (nd[y] != d[y]) {
+            d[y] = nd[y];
+            rch |= 1u << y;
+        }
+    }
+    chg |= rch;
+}
+
 // Reachability flood of one tile by the same level sets, run to the fixed
 // point: S = sinks, nodes already at 0 and nodes with a residual arc into S or
 // into a halo node at 0; d = 0 on S, DINF elsewhere. Values only go from DINF
@@ -1613,7 +1659,7 @@ __device__ __forceinline__ void bfs_flood(int (&d)[TH], const Masks &m, int hU, 
 // rounds use the relaxation, reachability rounds the flood. Returns true when
 // the tile converged.
 __device__ __forceinline__ bool tile_bfs(int (&d)[TH], Masks &m, int hU, int hD, int hs, int lane, int capv,
-                                         uint32_t &chg, int &rounds, int max_rounds) {
+                                         uint32_t &chg, int &rounds, int max_rounds, bool zero_sources = false) {
     if (capv == CAP_FLOOD) {
         bfs_flood(d, m, hU, hD, hs, lane, chg, rounds);
         return true;
@@ -1628,6 +1674,10 @@ __device__ __forceinline__ bool tile_bfs(int (&d)[TH], Masks &m, int hU, int hD,
     }
     if (capv <= 15) {
         bfs_levels<4>(d, m, hU, hD, hs, lane, capv, chg, rounds);
+        return true;
+    }
+    if (zero_sources) {
+        bfs_levels_zero(d, m, hU, hD, hs, lane, capv, chg, rounds);
         return true;
     }
     return relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, max_rounds);
@@ -1784,7 +1834,9 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
         for (int xr = 0;; ++xr) {
             int rounds = 0;
             bool conv = true;
-            if (need) conv = tile_bfs(d, m, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds);
+            // The first sweep of a REINIT starts from sinks and mask halos only (every
+            // source at 0); later exchanges bring block-inner halos of any value.
+            if (need) conv = tile_bfs(d, m, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds, reinit && !flood && xr == 0);
             n_rounds += rounds;
             if (NW == 1) {
                 capped = !conv;
@@ -2272,7 +2324,8 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
         }
         uint32_t chg = 0;
         int rounds = 0;
-        const bool conv = tile_bfs(d, m, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds);
+        // Sources: the sinks and the halo sinks read from the inputs, all at 0.
+        const bool conv = tile_bfs(d, m, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds, true);
         int act = 0, fin = 0, nearv = 0, open = 0;
         const bool colin = X > 0 && X < D.W - 1;
 #pragma unroll
