@@ -24,8 +24,18 @@ void GC_SetEdges(GlobalWrapper *gw, int *up, int *down, int *left, int *right, i
 
 static GlobalWrapper gc_init(int width, int height, int nbhd, int penalty, bool varying) {
     GlobalWrapper gw{};
-    gw.width = width;
-    gw.height = height;
+    // Geometry of the tile grid (64 x 16 tiles stand where the reference's
+    // 32 x 8 thread blocks did, GraphCut.cu:203-209).
+    const int tw = 64, th = 16;
+    gw.k.g.width = width;
+    gw.k.g.height = height;
+    gw.k.g.size = width * height;
+    gw.k.block_x = width > 0 ? (width + tw - 1) / tw : 0;
+    gw.block_y = height > 0 ? (height + th - 1) / th : 0;
+    gw.block_count = gw.k.block_x * gw.block_y;
+    gw.k.g.width_ex = gw.k.block_x * tw;
+    gw.k.g.height_ex = gw.block_y * th;
+    gw.k.g.size_ex = gw.k.g.width_ex * gw.k.g.height_ex;
     gw.penalty = penalty;
     gw.varying_edges = varying;
     gw.neighborhood = nbhd;

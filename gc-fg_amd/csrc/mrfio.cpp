@@ -1,7 +1,9 @@
 // Host-side file formats of the reference's drivers: the Middlebury 2-label
-// MRF text file (solveMRF.cu:10-65 reader, multiplyMRF.cpp:64-92 writer) and
-// the plain PPM/PGM images the drivers read and write (Application.cu:55-75,
-// solveMRF.cu:113-127). Pure host code, no device work.
+// MRF text file (solveMRF.cu:10-65 reader, multiplyMRF.cpp:64-92 writer), the
+// data-term file and the tiled constant-penalty MRF built from it
+// (multiplyMRF.cpp:94-145, Application.cu:15-24) and the plain PPM/PGM images
+// the drivers read and write (Application.cu:55-75, solveMRF.cu:113-127).
+// Pure host code, no device work.
 #include <cctype>
 #include <cstdint>
 #include <cstdio>
@@ -157,6 +159,68 @@ int gcfg_mrf_write(const char *path, int32_t width, int32_t height, int32_t mult
     const bool ok = std::ferror(f) == 0;
     std::fclose(f);
     return ok ? GCFG_OK : fail("gcfg_mrf_write: write error");
+}
+
+int gcfg_dat_info(const char *path, int32_t *width, int32_t *height) {
+    IntReader r;
+    if (!path || !r.open(path)) return fail(std::string("gcfg_dat_info: cannot open ") + (path ? path : "(null)"));
+    long long h, w;
+    if (!r.next_int(h) || !r.next_int(w) || w <= 0 || h <= 0 || w * h > (1ll << 30))
+        return fail("gcfg_dat_info: bad header (expected 'height width')");
+    if (width) *width = (int32_t)w;
+    if (height) *height = (int32_t)h;
+    return GCFG_OK;
+}
+
+int gcfg_dat_load(const char *path, int32_t width, int32_t height, int32_t *pos, int32_t *neg) {
+    IntReader r;
+    if (!path || !r.open(path)) return fail(std::string("gcfg_dat_load: cannot open ") + (path ? path : "(null)"));
+    long long h, w;
+    if (!r.next_int(h) || !r.next_int(w)) return fail("gcfg_dat_load: bad header");
+    if (w != width || h != height) return fail("gcfg_dat_load: shape differs from gcfg_dat_info");
+    const size_t n = (size_t)w * (size_t)h;
+    // Application.cu:18-23 / multiplyMRF.cpp:138-143: every neg value first, then every pos value.
+    if (!read_ints(r, neg, n) || !read_ints(r, pos, n)) return fail("gcfg_dat_load: truncated data terms");
+    return GCFG_OK;
+}
+
+int gcfg_dat_write(const char *path, int32_t width, int32_t height, const int32_t *pos, const int32_t *neg) {
+    if (!path || width <= 0 || height <= 0 || !pos || !neg) return fail("gcfg_dat_write: invalid arguments");
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(std::string("gcfg_dat_write: cannot create ") + path);
+    const size_t n = (size_t)width * height;
+    std::fprintf(f, "%d %d\n", height, width);
+    for (size_t i = 0; i < n; ++i) std::fprintf(f, "%d%c", neg[i], (i + 1) % (size_t)width ? ' ' : '\n');
+    for (size_t i = 0; i < n; ++i) std::fprintf(f, "%d%c", pos[i], (i + 1) % (size_t)width ? ' ' : '\n');
+    const bool ok = std::ferror(f) == 0;
+    std::fclose(f);
+    return ok ? GCFG_OK : fail("gcfg_dat_write: write error");
+}
+
+int gcfg_mrf_write_tiled(const char *path, int32_t out_w, int32_t out_h, int32_t width, int32_t height,
+                         const int32_t *pos, const int32_t *neg, int32_t penalty) {
+    if (!path || out_w < 1 || out_h < 1 || width < 1 || height < 1 || !pos || !neg)
+        return fail("gcfg_mrf_write_tiled: invalid arguments");
+    FILE *f = std::fopen(path, "wb");
+    if (!f) return fail(std::string("gcfg_mrf_write_tiled: cannot create ") + path);
+    // Layout of writeRihannaMRFData (multiplyMRF.cpp:94-122): header, ground
+    // truth all 1, the data terms repeated over the output (source column
+    // x mod width; source row y, which the reference indexes without a wrap
+    // and so only defines for out_h <= height: rows past the source repeat it
+    // here), then every horizontal and vertical cue equal to the penalty.
+    std::fprintf(f, "%d %d 2 ", out_w, out_h);
+    const long long N = (long long)out_w * out_h;
+    for (long long i = 0; i < N; ++i) std::fputs("1 ", f);
+    const int32_t *planes[2] = {pos, neg};
+    for (const int32_t *pl : planes)
+        for (int y = 0; y < out_h; ++y)
+            for (int x = 0; x < out_w; ++x)
+                std::fprintf(f, "%d ", pl[(size_t)(y % height) * width + (size_t)(x % width)]);
+    for (long long i = 0; i < (long long)out_h * (out_w - 1); ++i) std::fprintf(f, "%d ", penalty);
+    for (long long i = 0; i < (long long)(out_h - 1) * out_w; ++i) std::fprintf(f, "%d ", penalty);
+    const bool ok = std::ferror(f) == 0;
+    std::fclose(f);
+    return ok ? GCFG_OK : fail("gcfg_mrf_write_tiled: write error");
 }
 
 int gcfg_ppm_info(const char *path, int32_t *width, int32_t *height) {

@@ -134,6 +134,33 @@ def mrf_write(path: str, pos, neg, hcue, vcue, mult: int = 1):
            "gcfg_mrf_write")
 
 
+def dat_load(path: str):
+    """Data-term file (Application.cu:15-24) -> (W, H, pos, neg) int32 [H, W]."""
+    W, H = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib().gcfg_dat_info(path.encode(), ctypes.byref(W), ctypes.byref(H)), "gcfg_dat_info")
+    pos = np.empty((H.value, W.value), np.int32)
+    neg = np.empty((H.value, W.value), np.int32)
+    _check(lib().gcfg_dat_load(path.encode(), W.value, H.value, _VP(pos.ctypes.data), _VP(neg.ctypes.data)),
+           "gcfg_dat_load")
+    return W.value, H.value, pos, neg
+
+
+def dat_write(path: str, pos, neg):
+    pos = np.ascontiguousarray(pos, np.int32)
+    neg = np.ascontiguousarray(neg, np.int32)
+    H, W = pos.shape
+    _check(lib().gcfg_dat_write(path.encode(), W, H, _VP(pos.ctypes.data), _VP(neg.ctypes.data)), "gcfg_dat_write")
+
+
+def mrf_write_tiled(path: str, out_w: int, out_h: int, pos, neg, penalty: int = 600):
+    """writeRihannaMRFData (multiplyMRF.cpp:94-122) at out_w x out_h."""
+    pos = np.ascontiguousarray(pos, np.int32)
+    neg = np.ascontiguousarray(neg, np.int32)
+    H, W = pos.shape
+    _check(lib().gcfg_mrf_write_tiled(path.encode(), int(out_w), int(out_h), W, H, _VP(pos.ctypes.data),
+                                      _VP(neg.ctypes.data), int(penalty)), "gcfg_mrf_write_tiled")
+
+
 def read_ppm(path: str) -> np.ndarray:
     W, H = ctypes.c_int32(), ctypes.c_int32()
     _check(lib().gcfg_ppm_info(path.encode(), ctypes.byref(W), ctypes.byref(H)), "gcfg_ppm_info")

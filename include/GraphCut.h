@@ -31,10 +31,54 @@
 #endif
 #define SPREAD_ZEROS
 
-/* Handle plus the borrowed inputs, mirroring GlobalWrapper (GraphCut.h:110-128). */
+/* The reference's wrapper structs (GraphCut.h:60-128) with their field names,
+ * so that code reading them compiles. The geometry fields describe this
+ * library's padded tile grid (tiles of 64 x 16 nodes, the counterpart of the
+ * reference's 32 x 8 thread blocks): width_ex, height_ex and size_ex are the
+ * padded extents, block_x x block_y the tile grid and block_count the tiles.
+ * The residual state lives in the library's tiled arena, not in W*H planes:
+ * the NodeWrapper pointers and KernelWrapper::active are NULL (the state is
+ * read with gcfg_graph_read_state / gcfg_graph_read_distances). The diagonal
+ * fields, under NEIGHBORHOOD == 8 only in the reference, are present in both
+ * builds: one library serves both arities, so the layout is the same. */
+struct NodeWrapper {
+    int *edge_l;
+    int *edge_r;
+    int *edge_u;
+    int *edge_d;
+    int *edge_ul;
+    int *edge_dr;
+    int *edge_ur;
+    int *edge_dl;
+    int *height;
+    int *excess;
+    int *status;
+    int *comp_h;
+    int *comp_n;
+    int *energy_sum;
+};
+
+struct GraphWrapper {
+    int width;     /* graph width */
+    int height;    /* graph height */
+    int size;      /* width * height */
+    int width_ex;  /* width padded to whole tiles */
+    int height_ex; /* height padded to whole tiles */
+    int size_ex;   /* width_ex * height_ex */
+    NodeWrapper n;
+};
+
+struct KernelWrapper {
+    int block_x; /* tiles across */
+    int *active;
+    GraphWrapper g;
+};
+
+/* Handle plus the borrowed inputs (GraphCut.h:110-128); the last three
+ * fields are this library's. */
 struct GlobalWrapper {
-    int width;             /* graph width (GraphWrapper::width) */
-    int height;            /* graph height (GraphWrapper::height) */
+    int block_y;           /* tiles down */
+    int block_count;       /* tiles */
     int *data_positive;    /* source t-links (device) */
     int *data_negative;    /* sink t-links (device) */
     int *up;               /* capacity to (x, y-1) (device) */
@@ -47,6 +91,7 @@ struct GlobalWrapper {
     int *downright;        /* capacity to (x+1, y+1) */
     int penalty;           /* uniform n-link capacity when varying_edges is false */
     bool varying_edges;    /* whether per-direction capacities were given */
+    KernelWrapper k;       /* geometry (k.g.width, k.g.height, ...) */
     int neighborhood;      /* 4 or 8, fixed by GC_Init */
     gcfg_graph *graph;     /* device state */
     long long *last_flow;  /* exact max-flow of the last GC_Optimize (shared by copies) */

@@ -260,6 +260,20 @@ int gcfg_mrf_load(const char *path, int32_t width, int32_t height, int32_t *pos,
                   int32_t *down, int32_t *left, int32_t *right);
 int gcfg_mrf_write(const char *path, int32_t width, int32_t height, int32_t mult, const int32_t *pos,
                    const int32_t *neg, const int32_t *hcue, const int32_t *vcue);
+/* Data-term files (rihanna.dat / face.dat of Application.cu:15-24 and
+ * multiplyMRF.cpp:130-145): "height width", then all W*H neg values, then all
+ * W*H pos values, row-major, whitespace separated. */
+int gcfg_dat_info(const char *path, int32_t *width, int32_t *height);
+int gcfg_dat_load(const char *path, int32_t width, int32_t height, int32_t *pos, int32_t *neg);
+int gcfg_dat_write(const char *path, int32_t width, int32_t height, const int32_t *pos, const int32_t *neg);
+/* writeRihannaMRFData (multiplyMRF.cpp:94-122): a Middlebury 2-label MRF file
+ * of out_w x out_h pixels (the reference: 3600 x 3600) whose data terms repeat
+ * the width x height source (pixel (x, y) takes source pixel
+ * (x mod width, y mod height); the reference indexes rows without the wrap,
+ * which agrees wherever it reads inside the source) and whose horizontal and
+ * vertical cues all equal `penalty` (the reference: 600). */
+int gcfg_mrf_write_tiled(const char *path, int32_t out_w, int32_t out_h, int32_t width, int32_t height,
+                         const int32_t *pos, const int32_t *neg, int32_t penalty);
 /* PPM input images (P3 or P6, maxval 255; Application.cu:55-58) as
  * interleaved RGB u8. */
 int gcfg_ppm_info(const char *path, int32_t *width, int32_t *height);

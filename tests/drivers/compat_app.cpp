@@ -49,6 +49,11 @@ int main(int argc, char **argv) {
     if (!out) die("cannot open output");
 
     GlobalWrapper gw = GC_Init(W, H);  // data terms and penalty set later (ComplexApplication.cu:105)
+    // The reference's wrapper fields (GraphCut.h:60-128) read as its callers read them.
+    if (gw.k.g.width != W || gw.k.g.height != H || gw.k.g.size != W * H || gw.k.g.width_ex < W ||
+        gw.k.g.height_ex < H || gw.block_count != gw.k.block_x * gw.block_y ||
+        gw.k.g.size_ex != gw.k.g.width_ex * gw.k.g.height_ex)
+        die("GlobalWrapper geometry fields");
     if (model >= 0) DataTerm::setModel(model);  // -1: the drop-in's default model
     DataTerm::init_data_term(W, H);
     DataTerm::setSigmaGaussian(sigma);

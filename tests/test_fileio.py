@@ -1,6 +1,8 @@
 """Host-side file formats of the reference's drivers (no GPU): the Middlebury
-2-label MRF text file (solveMRF.cu:10-65, multiplyMRF.cpp:64-92) and the
-PPM/PGM label images (solveMRF.cu:113-127, Application.cu:55-75)."""
+2-label MRF text file (solveMRF.cu:10-65, multiplyMRF.cpp:64-92), the
+data-term file and the tiled constant-penalty MRF built from it
+(multiplyMRF.cpp:94-145, Application.cu:15-24), and the PPM/PGM label images
+(solveMRF.cu:113-127, Application.cu:55-75)."""
 import numpy as np
 import pytest
 
@@ -90,3 +92,46 @@ def test_label_pgm(g, tmp_path):
     p = tmp_path / "l.pgm"
     g.write_label_pgm(str(p), lab)
     assert p.read_bytes() == b"P5\n3 2\n255\n" + bytes([0, 255, 255, 255, 0, 0])
+
+
+def test_dat_known_file_and_roundtrip(g, tmp_path):
+    """'height width', then every neg value, then every pos value."""
+    p = tmp_path / "face.dat"
+    p.write_text("2 3\n1 2 3\n4 5 6\n10 20 30 40 50 60\n")
+    W, H, pos, neg = g.dat_load(str(p))
+    assert (W, H) == (3, 2)
+    assert neg.tolist() == [[1, 2, 3], [4, 5, 6]] and pos.tolist() == [[10, 20, 30], [40, 50, 60]]
+    rng = np.random.default_rng(5)
+    pos2, neg2 = rng.integers(-50, 900, (2, 9, 13)).astype(np.int32)
+    q = str(tmp_path / "r.dat")
+    g.dat_write(q, pos2, neg2)
+    W, H, a, b = g.dat_load(q)
+    assert (W, H) == (13, 9) and np.array_equal(a, pos2) and np.array_equal(b, neg2)
+    p.write_text("2 3\n1 2 3\n4 5 6\n10 20\n")
+    with pytest.raises(g.GcfgError):
+        g.dat_load(str(p))
+
+
+@pytest.mark.parametrize("out", [(7, 5), (20, 11), (4, 3)])
+def test_tiled_penalty_mrf(g, tmp_path, out):
+    """writeRihannaMRFData's layout read back by the solveMRF reader: data
+    terms repeat the source (column x mod W, row y mod H), every cue equals
+    the penalty, and the reader's zeroed last row and last column apply."""
+    rng = np.random.default_rng(out[0])
+    H, W = 5, 7
+    pos, neg = rng.integers(0, 1000, (2, H, W)).astype(np.int32)
+    path = str(tmp_path / "t.txt")
+    g.mrf_write_tiled(path, out[0], out[1], pos, neg, penalty=600)
+    OW, OH, P, N, up, down, left, right = g.mrf_load(path)
+    assert (OW, OH) == out
+    ys, xs = np.meshgrid(np.arange(OH), np.arange(OW), indexing="ij")
+    assert np.array_equal(P, pos[ys % H, xs % W]) and np.array_equal(N, neg[ys % H, xs % W])
+    hc = np.full((OH, OW), 600, np.int32)
+    hc[:, OW - 1] = 0
+    hc[OH - 1, :] = 0
+    vc = np.full((OH, OW), 600, np.int32)
+    vc[OH - 1, :] = 0
+    assert np.array_equal(left, hc) and np.array_equal(right, hc)
+    assert np.array_equal(up, vc) and np.array_equal(down, vc)
+    text = open(path).read().split()
+    assert text[:3] == [str(out[0]), str(out[1]), "2"] and text[3:3 + OW * OH] == ["1"] * (OW * OH)

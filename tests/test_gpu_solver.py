@@ -474,6 +474,31 @@ def test_middlebury_file_solve(g, ora, tmp_path):
     assert st.steps > 0
 
 
+def test_face_dat_and_tiled_mrf_solve(g, ora, tmp_path):
+    """Application.cu's workflow (a face.dat file of data terms solved with
+    penalty 600) and multiplyMRF.cpp's Rihanna path (the same data terms
+    tiled into a larger constant-penalty MRF file, read back by the
+    solveMRF.cu reader): both equal the oracle."""
+    rng = np.random.default_rng(45)
+    H, W = 220, 150
+    yy, xx = np.mgrid[0:H, 0:W]
+    inside = ((xx - W / 2) / (0.3 * W)) ** 2 + ((yy - H / 2) / (0.35 * H)) ** 2 < 1
+    base = rng.integers(0, 2000, (H, W))
+    pos = np.where(inside, base + 3000, base).astype(np.int32)
+    neg = np.where(inside, base, base + 3000).astype(np.int32)
+    dat = str(tmp_path / "face.dat")
+    g.dat_write(dat, pos, neg)
+    W1, H1, p1, n1 = g.dat_load(dat)
+    assert (W1, H1) == (W, H)
+    st = _check(g, ora, W, H, p1, n1, penalty=600)
+    mrf = str(tmp_path / "tiled.txt")
+    g.mrf_write_tiled(mrf, 3 * W, H + 37, p1, n1, penalty=600)
+    W2, H2, p2, n2, up, down, left, right = g.mrf_load(mrf)
+    assert (W2, H2) == (3 * W, H + 37)
+    st2 = _check(g, ora, W2, H2, p2, n2, edges=(up, down, left, right))
+    assert st.steps > 0 and st2.steps > 0
+
+
 @pytest.mark.parametrize("inner", [1, 4])
 @pytest.mark.parametrize("lazy", [1, 2])
 @pytest.mark.parametrize("shape", [(5, 4), (130, 33), (257, 140)])
