@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "dataterm.hpp"
@@ -822,6 +823,45 @@ int DtContext::unquantized(double *h_tpos, double *h_tneg, double *h_right, doub
     return GCFG_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Fixed scratch of gcfg_dt_compute_noalloc: the per-chunk beta partials, the
+// histogram counters, the statistics, the t-link table and the n-link
+// thresholds of one image of at most GCFG_DT_NOALLOC_MAX_PIXELS pixels
+// (161 KiB). Every GPU the library runs on holds its own copy until the
+// process ends; calls that overlap on one GPU share it and race.
+constexpr size_t NA_CHUNKS = ((size_t)GCFG_DT_NOALLOC_MAX_PIXELS + DT_CHUNK - 1) / DT_CHUNK;
+constexpr size_t NA_OFF_BETA = 0;
+constexpr size_t NA_OFF_HIST = NA_OFF_BETA + ((NA_CHUNKS * sizeof(unsigned long long) + 255) & ~(size_t)255);
+constexpr size_t NA_OFF_STATS = NA_OFF_HIST + ((DT_HIST * sizeof(int32_t) + 255) & ~(size_t)255);
+constexpr size_t NA_OFF_TLUT = NA_OFF_STATS + ((sizeof(DtStats) + 255) & ~(size_t)255);
+constexpr size_t NA_OFF_THR = NA_OFF_TLUT + ((DT_LUT * sizeof(uint16_t) + 255) & ~(size_t)255);
+constexpr size_t NA_BYTES = NA_OFF_THR + (((DT_MAX_LAMBDA + 1) * sizeof(int32_t) + 255) & ~(size_t)255);
+
+static __device__ __align__(256) uint8_t scratch[NA_BYTES];
+
+// Device address of `scratch` on the current device, looked up once per
+// device (host code must never pass the symbol itself to a kernel).
+constexpr int NA_MAX_DEVICES = 64;
+static std::mutex g_na_mu;
+static void *g_na_ptr[NA_MAX_DEVICES];
+
+static int noalloc_scratch(uint8_t **out) {
+    int dev = 0;
+    GCFG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NA_MAX_DEVICES) {
+        set_error("gcfg_dt_compute_noalloc: device index beyond the supported range");
+        return GCFG_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        GCFG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_na_ptr[dev] = p;
+    }
+    *out = static_cast<uint8_t *>(g_na_ptr[dev]);
+    return GCFG_OK;
+}
+
 }  // namespace gcfg
 
 // ---------------------------------------------------------------------------
@@ -987,6 +1027,57 @@ int gcfg_dt_kde_select(gcfg_dataterm *dt, int32_t x, int32_t y) {
 int gcfg_dt_kde_count(gcfg_dataterm *dt, int32_t *count) {
     if (!dt || !count) return GCFG_ERR_INVALID;
     *count = dt->ctx.kde_count;
+    return GCFG_OK;
+}
+
+int gcfg_dt_compute_noalloc(int32_t width, int32_t height, const uint8_t *d_rgb, const uint8_t *d_seeds,
+                            int32_t lambda, int32_t *d_pos, int32_t *d_neg, int32_t *d_up, int32_t *d_down,
+                            int32_t *d_left, int32_t *d_right, void *hip_stream) {
+    using namespace gcfg;
+    if (width <= 0 || height <= 0 || (int64_t)width * height > (int64_t)GCFG_DT_NOALLOC_MAX_PIXELS) {
+        set_error("gcfg_dt_compute_noalloc: the image must have 1 .. GCFG_DT_NOALLOC_MAX_PIXELS (2^26) pixels");
+        return GCFG_ERR_INVALID;
+    }
+    if (lambda < 0 || lambda > DT_MAX_LAMBDA) {
+        set_error("gcfg_dt_compute_noalloc: lambda out of range (0..4096)");
+        return GCFG_ERR_INVALID;
+    }
+    if (!d_rgb || ((uintptr_t)d_rgb & 3) || ((uintptr_t)d_seeds & 3) || !d_pos || !d_neg || !d_up || !d_down ||
+        !d_left || !d_right) {
+        set_error("gcfg_dt_compute_noalloc: needs a 4-byte aligned image (and seed map) and six output planes");
+        return GCFG_ERR_INVALID;
+    }
+    uint8_t *base = nullptr;
+    int rc = noalloc_scratch(&base);
+    if (rc != GCFG_OK) return rc;
+    hipStream_t st = (hipStream_t)hip_stream;
+    DtJob j{};
+    j.img = d_rgb;
+    j.seeds = d_seeds;
+    int32_t *o[6] = {d_pos, d_neg, d_up, d_down, d_left, d_right};
+    bool al = true;
+    for (int k = 0; k < 6; ++k) {
+        j.out[k] = o[k];
+        al &= aligned16(o[k]);
+    }
+    j.aligned = al ? 1 : 0;
+    j.W = width;
+    j.H = height;
+    j.lambda = lambda;
+    j.model = GCFG_DT_MODEL_HISTOGRAM;
+    j.beta_part = reinterpret_cast<unsigned long long *>(base + NA_OFF_BETA);
+    j.hist_acc = reinterpret_cast<int32_t *>(base + NA_OFF_HIST);
+    j.stats = reinterpret_cast<DtStats *>(base + NA_OFF_STATS);
+    j.tlut = reinterpret_cast<uint16_t *>(base + NA_OFF_TLUT);
+    j.thr = reinterpret_cast<int32_t *>(base + NA_OFF_THR);
+    // The histogram counters must start at zero: a kernel clears them (no
+    // memset node, so the call can be captured into a graph).
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(DT_NT), 0, st, j.hist_acc, (int64_t)DT_HIST, 0);
+    DtLaunch L{};
+    L.one = j;
+    L.lds_lambda = lambda;
+    dt_launch(L, dt_chunks(width, height), st);
+    GCFG_HIP(hipGetLastError());
     return GCFG_OK;
 }
 

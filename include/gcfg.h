@@ -202,6 +202,23 @@ int gcfg_dt_kde_select(gcfg_dataterm *dt, int32_t x, int32_t y);
 /* Number of seed colours in the kernel-density list. */
 int gcfg_dt_kde_count(gcfg_dataterm *dt, int32_t *count);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): gcfg_dt_compute's histogram data term (lambda, 16-bin
+ * seed histograms; same outputs) of one image of at most
+ * GCFG_DT_NOALLOC_MAX_PIXELS pixels, from a caller-owned device image
+ * (W*H*3, 4-byte aligned) and seed map (W*H, 4-byte aligned, or NULL) into
+ * six caller-owned W*H int32 planes, with every scratch buffer in a fixed
+ * device array of the library (161 KiB per GPU, held until the process ends).
+ * It allocates nothing, issues no memset and does not synchronise: the work
+ * is queued on hip_stream (NULL: the default stream) and the outputs are
+ * ready when the stream reaches it. Calls that overlap on one GPU share the
+ * scratch and race; larger images fail with GCFG_ERR_INVALID. */
+#define GCFG_DT_NOALLOC_MAX_PIXELS (1 << 26)
+int gcfg_dt_compute_noalloc(int32_t width, int32_t height, const uint8_t *d_rgb, const uint8_t *d_seeds,
+                            int32_t lambda, int32_t *d_pos, int32_t *d_neg, int32_t *d_up, int32_t *d_down,
+                            int32_t *d_left, int32_t *d_right, void *hip_stream);
+
 /* Batched data term: getDataTermsMixture (DataTerm.cpp:129-137) of many
  * images at once, histogram model, with one launch per pass for the whole
  * batch (no counterpart in the reference, which computes one image per
