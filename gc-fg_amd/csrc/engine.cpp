@@ -181,11 +181,11 @@ bool options_valid(const gcfg_options &o) {
 // step graph with stale parameters is dropped.
 void Engine::apply_options() {
     const int cu = std::max(cu_count_, 1);
-    // The 8-neighbourhood runs the workgroup-per-tile kernels (diagonal arcs
-    // need the corner halo nodes that those kernels stage in LDS).
-    const int variant = (opt.kernels && nbhd_ == 4) ? 1 : 0;
+    const int variant = opt.kernels ? 1 : 0;
     int gbx = opt.gr_block_w > 0 ? opt.gr_block_w : 1, gby = opt.gr_block_h > 0 ? opt.gr_block_h : 1;
-    if (!variant || !gr_block_supported(gbx, gby)) gbx = gby = 1;
+    // 8-neighbourhood relabels run one-tile blocks (a block's inner exchange
+    // would need its corner nodes too).
+    if (!variant || nbhd_ == 8 || !gr_block_supported(gbx, gby)) gbx = gby = 1;
     const int wpr = opt.pr_waves_per_cu > 0 ? opt.pr_waves_per_cu : 16;
     const int wgr = opt.gr_waves_per_cu > 0 ? opt.gr_waves_per_cu : 28;
     const int gpr = variant ? cu * wpr : cu * 4;
@@ -337,7 +337,9 @@ int Engine::build(const std::vector<int> &slots) {
     // (32 M nodes). Large batches qualify; single images and long solves do
     // not. The descriptors carry the choice to the kernels.
     const int64_t need = steps_hint > 0 ? (int64_t)steps_hint * 1024 : 32768;
-    const int lazy = kp_.variant && (opt.lazy_caps == 2 || (opt.lazy_caps == 0 && (int64_t)total >= need)) ? 1 : 0;
+    // (8-neighbourhood: never; diagonal reverse arcs go to the capacities.)
+    const int lazy =
+        kp_.variant && nbhd_ == 4 && (opt.lazy_caps == 2 || (opt.lazy_caps == 0 && (int64_t)total >= need)) ? 1 : 0;
     for (int s : slots) h_desc_[s].lazy = lazy;
     // Build list: slots, then each one's tile offset. The descriptor table and
     // the list are uploaded only when they differ from what the device holds

@@ -1304,14 +1304,45 @@ __device__ __forceinline__ int64_t wave_sum64_w(int64_t v) { return wave_sum64_d
 // see k_grb), DINF elsewhere.
 enum HaloSrc : int { HALO_DIST = 0, HALO_EXCESS = 1, HALO_MASK = 2, HALO_WITNESS = 3 };
 
-template <int SRC>
+// 8-neighbourhood: lanes 32..35 of the side halo register hold the four corner
+// nodes, up-left (node (TW-1, TH-1) of the up-left tile), up-right ((0, TH-1)
+// of the up-right tile), down-left ((TW-1, 0)) and down-right ((0, 0)).
+constexpr int HS_UL = 2 * TH, HS_UR = 2 * TH + 1, HS_DL = 2 * TH + 2, HS_DR = 2 * TH + 3;
+
+// Diagonal neighbour tiles from the side ones: an instance's tiles are
+// row-major and contiguous, so the up-left tile is the up tile's left
+// neighbour, and it exists when both the up and the left tile do.
+__device__ __forceinline__ int tile_ul(const TileInfo &ti) { return ti.nbr[2] >= 0 && ti.nbr[0] >= 0 ? ti.nbr[2] - 1 : -1; }
+__device__ __forceinline__ int tile_ur(const TileInfo &ti) { return ti.nbr[2] >= 0 && ti.nbr[1] >= 0 ? ti.nbr[2] + 1 : -1; }
+__device__ __forceinline__ int tile_dl(const TileInfo &ti) { return ti.nbr[3] >= 0 && ti.nbr[0] >= 0 ? ti.nbr[3] - 1 : -1; }
+__device__ __forceinline__ int tile_dr(const TileInfo &ti) { return ti.nbr[3] >= 0 && ti.nbr[1] >= 0 ? ti.nbr[3] + 1 : -1; }
+
+// Side-halo node of a lane: its tile (-1: none) and in-tile index. Lanes
+// 0..15 the left tile's column TW-1, 16..31 the right tile's column 0 (by
+// row), and for the 8-neighbourhood lanes 32..35 the corner nodes.
+template <int NB>
+__device__ __forceinline__ int side_halo(const TileInfo &ti, int lane, int &idx) {
+    if (lane < 2 * TH) {
+        idx = (lane & (TH - 1)) * TW + (lane < TH ? TW - 1 : 0);
+        return lane < TH ? ti.nbr[0] : ti.nbr[1];
+    }
+    if (NB == 8 && lane <= HS_DR) {
+        idx = (lane == HS_UL ? (TH - 1) * TW + TW - 1 : lane == HS_UR ? (TH - 1) * TW : lane == HS_DL ? TW - 1 : 0);
+        return lane == HS_UL ? tile_ul(ti) : lane == HS_UR ? tile_ur(ti) : lane == HS_DL ? tile_dl(ti) : tile_dr(ti);
+    }
+    idx = 0;
+    return -1;
+}
+
+template <int SRC, int NB = 4>
 __device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, int t, int lane, int &hU, int &hD,
                                           int &hs) {
     const int nu = ti.nbr[2], nd = ti.nbr[3];
-    const int side = lane < TH ? ti.nbr[0] : ti.nbr[1];
+    int sidx;
+    const int side = side_halo<NB>(ti, lane, sidx);
     const size_t gu = (size_t)(nu >= 0 ? nu : t) * TN + (size_t)(TH - 1) * TW + lane;
     const size_t gd = (size_t)(nd >= 0 ? nd : t) * TN + lane;
-    const size_t gs = (size_t)(side >= 0 ? side : t) * TN + (size_t)(lane & (TH - 1)) * TW + (lane < TH ? TW - 1 : 0);
+    const size_t gs = (size_t)(side >= 0 ? side : t) * TN + (size_t)sidx;
     int vu, vd, vs;
     if (SRC == HALO_EXCESS) {
         vu = p.exc[gu] < 0 ? 0 : DINF;
@@ -1332,7 +1363,7 @@ __device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, 
     }
     hU = nu >= 0 ? vu : DINF;
     hD = nd >= 0 ? vd : DINF;
-    hs = (lane < 2 * TH && side >= 0) ? vs : DINF;
+    hs = side >= 0 ? vs : DINF;
 }
 
 __device__ __forceinline__ int dmin_sat(int a, int b) { return min(min(a, b), DINF); }
@@ -1364,6 +1395,14 @@ __device__ __forceinline__ void mask_after_push_in(const KParams &p, size_t g, i
     const int sh = (int)(g & 3) * 8;
     atomicOr(w, (uint32_t)(arc | (e_new > 0 ? MB_EPOS : 0)) << sh);
     if (e_new >= 0) atomicAnd(w, ~((uint32_t)MB_ENEG << sh));
+}
+
+// Diagonal arcs back to the pushers opened on a node of a tile that is not
+// running (8-neighbourhood mask8 bits: 1 up-left, 2 up-right, 4 down-left,
+// 8 down-right).
+__device__ __forceinline__ void mask8_after_push_in(const KParams &p, size_t g, int arcs) {
+    uint32_t *w = reinterpret_cast<uint32_t *>(p.mask8) + (g >> 2);
+    atomicOr(w, (uint32_t)arcs << ((int)(g & 3) * 8));
 }
 
 // (min, +) composition scans along a 64-lane row. Lane x holds the map
@@ -1441,7 +1480,11 @@ struct Masks {
 // only if a one-step check finds a value that can still decrease (otherwise
 // the row is already at its fixed point). Returns true when converged; chg
 // collects the rows whose value changed in this lane.
-__device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, int hU, int hD, int hs, int lane,
+// 8-neighbourhood (m8: bits 1 up-left, 2 up-right, 4 down-left, 8 down-right):
+// the downward sweep also relaxes the arcs into the row above's left and
+// right neighbours, the upward sweep those into the row below's.
+template <int NB = 4>
+__device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, Masks &m8, int hU, int hD, int hs, int lane,
                                                 uint32_t &chg, int &rounds, int max_rounds) {
     rounds = 0;
     while (rounds < max_rounds) {
@@ -1452,25 +1495,67 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, int hU, 
         asm volatile("" : "+v"(hs));
 #pragma unroll
         for (int i = 0; i < TH / 4; ++i) asm volatile("" : "+v"(m.w[i]));
+        if (NB == 8) {
+#pragma unroll
+            for (int i = 0; i < TH / 4; ++i) asm volatile("" : "+v"(m8.w[i]));
+        }
         uint32_t rch = 0;
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
-            if (m.get(y) & 4) {
-                const int cand = (y ? d[y - 1] : hU) + 1;
-                if (cand < d[y]) {
-                    d[y] = cand;
-                    rch |= 1u << y;
+            const int U = y ? d[y - 1] : hU;
+            if (NB == 4) {
+                if (m.get(y) & 4) {
+                    const int cand = U + 1;
+                    if (cand < d[y]) {
+                        d[y] = cand;
+                        rch |= 1u << y;
+                    }
                 }
+                continue;
+            }
+            int cand = (m.get(y) & 4) ? U + 1 : DINF;
+            {
+                // lane x <- lane x - 1 / x + 1 of the row above; lanes 0 and 63
+                // take the side halo's row y - 1 (the corners for row 0).
+                const int sl = __builtin_amdgcn_readlane(hs, y ? y - 1 : HS_UL);
+                const int sr = __builtin_amdgcn_readlane(hs, y ? TH + y - 1 : HS_UR);
+                const int ul = __builtin_amdgcn_update_dpp(sl, U, 0x138, 0xF, 0xF, false);
+                const int ur = __builtin_amdgcn_update_dpp(sr, U, 0x130, 0xF, 0xF, false);
+                const int m8y = m8.get(y);
+                if (m8y & 1) cand = min(cand, ul + 1);
+                if (m8y & 2) cand = min(cand, ur + 1);
+            }
+            if (cand < d[y]) {
+                d[y] = cand;
+                rch |= 1u << y;
             }
         }
 #pragma unroll
         for (int y = TH - 1; y >= 0; --y) {
-            if (m.get(y) & 8) {
-                const int cand = (y < TH - 1 ? d[y + 1] : hD) + 1;
-                if (cand < d[y]) {
-                    d[y] = cand;
-                    rch |= 1u << y;
+            const int Dn = y < TH - 1 ? d[y + 1] : hD;
+            if (NB == 4) {
+                if (m.get(y) & 8) {
+                    const int cand = Dn + 1;
+                    if (cand < d[y]) {
+                        d[y] = cand;
+                        rch |= 1u << y;
+                    }
                 }
+                continue;
+            }
+            int cand = (m.get(y) & 8) ? Dn + 1 : DINF;
+            {
+                const int sl = __builtin_amdgcn_readlane(hs, y < TH - 1 ? y + 1 : HS_DL);
+                const int sr = __builtin_amdgcn_readlane(hs, y < TH - 1 ? TH + y + 1 : HS_DR);
+                const int dl = __builtin_amdgcn_update_dpp(sl, Dn, 0x138, 0xF, 0xF, false);
+                const int dr = __builtin_amdgcn_update_dpp(sr, Dn, 0x130, 0xF, 0xF, false);
+                const int m8y = m8.get(y);
+                if (m8y & 4) cand = min(cand, dl + 1);
+                if (m8y & 8) cand = min(cand, dr + 1);
+            }
+            if (cand < d[y]) {
+                d[y] = cand;
+                rch |= 1u << y;
             }
         }
 #pragma unroll
@@ -1515,24 +1600,78 @@ __device__ __forceinline__ uint32_t row_plane(const Masks &m, int b) {
     return plane;
 }
 
+// Diagonal arcs of a lane's 16 rows as row bit planes (8-neighbourhood).
+struct DiagPlanes {
+    uint32_t ul, ur, dl, dr;
+};
+
+template <int NB>
+__device__ __forceinline__ DiagPlanes diag_planes(const Masks &m8) {
+    if (NB != 8) return DiagPlanes{0, 0, 0, 0};
+    return DiagPlanes{row_plane(m8, 0), row_plane(m8, 1), row_plane(m8, 2), row_plane(m8, 3)};
+}
+
+// One level-set step: S plus every node with a residual arc into S or into a
+// halo node in the set. hv holds this lane's halo bits (bit 0 the node above
+// row 0, bit 15 the node below row 15), lm / rm the side halo columns' bits
+// (lane 0 takes lm as its left neighbour's set, lane 63 takes rm), and with
+// diagonals cl / cr the corner bits lanes 0 / 63 take as the halo bits of
+// the lane beyond the tile (bit 0 up-left / up-right, bit 15 down-left /
+// down-right).
+template <int NB>
+__device__ __forceinline__ uint32_t level_step(uint32_t S, uint32_t PL, uint32_t PR, uint32_t PU, uint32_t PD,
+                                               const DiagPlanes &P8, uint32_t hv, int lm, int rm, uint32_t cl,
+                                               uint32_t cr) {
+    // lane x <- lane x-1 (lane 0: the left halo column), lane x <- lane x+1 (lane 63: right halo)
+    const uint32_t lf = (uint32_t)__builtin_amdgcn_update_dpp(lm, (int)S, 0x138, 0xF, 0xF, false);
+    const uint32_t rt = (uint32_t)__builtin_amdgcn_update_dpp(rm, (int)S, 0x130, 0xF, 0xF, false);
+    uint32_t Sn = S | (((S << 1) | (hv & 1u)) & PU) | (((S >> 1) | (hv & 0x8000u)) & PD) | (lf & PL) | (rt & PR);
+    if (NB == 8) {
+        const uint32_t hvl = (uint32_t)__builtin_amdgcn_update_dpp((int)cl, (int)hv, 0x138, 0xF, 0xF, false);
+        const uint32_t hvr = (uint32_t)__builtin_amdgcn_update_dpp((int)cr, (int)hv, 0x130, 0xF, 0xF, false);
+        Sn |= (((lf << 1) | (hvl & 1u)) & P8.ul) | (((lf >> 1) | (hvl & 0x8000u)) & P8.dl) |
+              (((rt << 1) | (hvr & 1u)) & P8.ur) | (((rt >> 1) | (hvr & 0x8000u)) & P8.dr);
+    }
+    return Sn & 0xffffu;
+}
+
+// Halo bits of one level: nodes whose value passes `in` (<= k, or == 0).
+template <int NB, typename In>
+__device__ __forceinline__ void level_halo(int hU, int hD, int hs, int lane, In in, uint32_t &hv, int &lm, int &rm,
+                                           uint32_t &cl, uint32_t &cr) {
+    hv = (in(hU) ? 1u : 0u) | (in(hD) ? 0x8000u : 0u);
+    const uint64_t bl = __ballot(lane < TH && in(hs));
+    const uint64_t br = __ballot(lane >= TH && lane < 2 * TH && in(hs));
+    lm = (int)((uint32_t)bl & 0xffffu);
+    rm = (int)((uint32_t)(br >> TH) & 0xffffu);
+    cl = cr = 0;
+    if (NB == 8) {
+        const uint64_t bc = __ballot(lane >= HS_UL && lane <= HS_DR && in(hs));
+        cl = ((bc >> HS_UL) & 1u) | (((bc >> HS_DL) & 1u) << 15);
+        cr = ((bc >> HS_UR) & 1u) | (((bc >> HS_DR) & 1u) << 15);
+    }
+}
+
 // Capped BFS of one tile by level sets, 16 rows as bits of one register per
 // lane: L_0 = sink nodes; L_(k+1) = L_k plus every node with a residual arc
 // into L_k or into a halo node whose value is <= k. Then d = the first level
 // holding the node, or K if none of L_0..L_(K-1) does; that is min(K, BFS
 // distance) given the halo, the fixed point the relaxation reaches, in about
-// 25 vector operations per level. Level counts per row are kept bit-sliced
-// (NB bits, K < 2^NB). chg collects rows whose value differs from d on entry.
-template <int NB>
-__device__ __forceinline__ void bfs_levels(int (&d)[TH], const Masks &m, int hU, int hD, int hs, int lane, int K,
-                                           uint32_t &chg, int &levels) {
+// 25 vector operations per level (4-neighbourhood). Level counts per row are
+// kept bit-sliced (BITS bits, K < 2^BITS). chg collects rows whose value
+// differs from d on entry.
+template <int BITS, int NB = 4>
+__device__ __forceinline__ void bfs_levels(int (&d)[TH], const Masks &m, const Masks &m8, int hU, int hD, int hs,
+                                           int lane, int K, uint32_t &chg, int &levels) {
     const uint32_t PL = row_plane(m, 0), PR = row_plane(m, 1), PU = row_plane(m, 2), PD = row_plane(m, 3);
+    const DiagPlanes P8 = diag_planes<NB>(m8);
     uint32_t S = row_plane(m, 5);
-    uint32_t c[NB];
+    uint32_t c[BITS];
 #pragma unroll
-    for (int b = 0; b < NB; ++b) c[b] = 0;
+    for (int b = 0; b < BITS; ++b) c[b] = 0;
     auto count = [&](uint32_t v) {
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
+        for (int b = 0; b < BITS; ++b) {
             const uint32_t t = c[b] & v;
             c[b] ^= v;
             v = t;
@@ -1541,16 +1680,10 @@ __device__ __forceinline__ void bfs_levels(int (&d)[TH], const Masks &m, int hU,
     count(S);
     int j = 0;  // index of the last level computed
     for (int k = 0; k < K - 1; ++k) {
-        const uint32_t hu = hU <= k ? 1u : 0u;
-        const uint32_t hd = hD <= k ? (1u << (TH - 1)) : 0u;
-        const uint64_t bl = __ballot(lane < TH && hs <= k);
-        const uint64_t br = __ballot(lane >= TH && lane < 2 * TH && hs <= k);
-        const int lm = (int)((uint32_t)bl & 0xffffu), rm = (int)((uint32_t)(br >> TH) & 0xffffu);
-        // lane x <- lane x-1 (lane 0: the left halo column), lane x <- lane x+1 (lane 63: right halo)
-        const uint32_t lf = (uint32_t)__builtin_amdgcn_update_dpp(lm, (int)S, 0x138, 0xF, 0xF, false);
-        const uint32_t rt = (uint32_t)__builtin_amdgcn_update_dpp(rm, (int)S, 0x130, 0xF, 0xF, false);
-        const uint32_t Sn = (S | (((S << 1) | hu) & PU) | (((S >> 1) | hd) & PD) | (lf & PL) | (rt & PR)) & 0xffffu;
-        S = Sn;
+        uint32_t hv, cl, cr;
+        int lm, rm;
+        level_halo<NB>(hU, hD, hs, lane, [k](int v) { return v <= k; }, hv, lm, rm, cl, cr);
+        S = level_step<NB>(S, PL, PR, PU, PD, P8, hv, lm, rm, cl, cr);
         count(S);
         j = k + 1;
     }
@@ -1560,7 +1693,7 @@ __device__ __forceinline__ void bfs_levels(int (&d)[TH], const Masks &m, int hU,
     for (int y = 0; y < TH; ++y) {
         int cnt = 0;
 #pragma unroll
-        for (int b = 0; b < NB; ++b) cnt |= (int)((c[b] >> y) & 1u) << b;
+        for (int b = 0; b < BITS; ++b) cnt |= (int)((c[b] >> y) & 1u) << b;
         const int nd = ((S >> y) & 1u) ? j + 1 - cnt : K;
         if (nd != d[y]) {
             d[y] = nd;
@@ -1578,24 +1711,22 @@ __device__ __forceinline__ void bfs_levels(int (&d)[TH], const Masks &m, int hU,
 // distance, where the relaxation (relax_tile_wave) needs a round per turn of
 // a shortest path and more work per round. chg collects the rows whose value
 // differs from d on entry.
-__device__ __forceinline__ void bfs_levels_zero(int (&d)[TH], const Masks &m, int hU, int hD, int hs, int lane,
-                                                int K, uint32_t &chg, int &levels) {
+template <int NB = 4>
+__device__ __forceinline__ void bfs_levels_zero(int (&d)[TH], const Masks &m, const Masks &m8, int hU, int hD, int hs,
+                                                int lane, int K, uint32_t &chg, int &levels) {
     const uint32_t PL = row_plane(m, 0), PR = row_plane(m, 1), PU = row_plane(m, 2), PD = row_plane(m, 3);
+    const DiagPlanes P8 = diag_planes<NB>(m8);
     uint32_t S = row_plane(m, 5);
-    const uint32_t hu = hU == 0 ? 1u : 0u;
-    const uint32_t hd = hD == 0 ? (1u << (TH - 1)) : 0u;
-    const uint64_t bl = __ballot(lane < TH && hs == 0);
-    const uint64_t br = __ballot(lane >= TH && lane < 2 * TH && hs == 0);
-    const int lm = (int)((uint32_t)bl & 0xffffu), rm = (int)((uint32_t)(br >> TH) & 0xffffu);
+    uint32_t hv, cl, cr;
+    int lm, rm;
+    level_halo<NB>(hU, hD, hs, lane, [](int v) { return v == 0; }, hv, lm, rm, cl, cr);
     int nd[TH];
 #pragma unroll
     for (int y = 0; y < TH; ++y) nd[y] = ((S >> y) & 1u) ? 0 : K;
     int L = 0;
     // S only grows and has at most TN bits; a level is computed only while L + 1 < K.
     while (L + 1 < K) {
-        const uint32_t lf = (uint32_t)__builtin_amdgcn_update_dpp(lm, (int)S, 0x138, 0xF, 0xF, false);
-        const uint32_t rt = (uint32_t)__builtin_amdgcn_update_dpp(rm, (int)S, 0x130, 0xF, 0xF, false);
-        const uint32_t Sn = (S | (((S << 1) | hu) & PU) | (((S >> 1) | hd) & PD) | (lf & PL) | (rt & PR)) & 0xffffu;
+        const uint32_t Sn = level_step<NB>(S, PL, PR, PU, PD, P8, hv, lm, rm, cl, cr);
         const uint32_t fresh = Sn & ~S;
         if (!__any(fresh != 0)) break;
         ++L;
@@ -1621,23 +1752,21 @@ __device__ __forceinline__ void bfs_levels_zero(int (&d)[TH], const Masks &m, in
 // into a halo node at 0; d = 0 on S, DINF elsewhere. Values only go from DINF
 // to 0, so tiles re-queued by changed boundaries converge after few visits,
 // where exact distances would be lowered again and again.
-__device__ __forceinline__ void bfs_flood(int (&d)[TH], const Masks &m, int hU, int hD, int hs, int lane,
-                                          uint32_t &chg, int &levels) {
+template <int NB = 4>
+__device__ __forceinline__ void bfs_flood(int (&d)[TH], const Masks &m, const Masks &m8, int hU, int hD, int hs,
+                                          int lane, uint32_t &chg, int &levels) {
     const uint32_t PL = row_plane(m, 0), PR = row_plane(m, 1), PU = row_plane(m, 2), PD = row_plane(m, 3);
+    const DiagPlanes P8 = diag_planes<NB>(m8);
     uint32_t S = row_plane(m, 5);
 #pragma unroll
     for (int y = 0; y < TH; ++y) S |= (d[y] == 0 ? 1u : 0u) << y;
-    const uint32_t hu = hU == 0 ? 1u : 0u;
-    const uint32_t hd = hD == 0 ? (1u << (TH - 1)) : 0u;
-    const uint64_t bl = __ballot(lane < TH && hs == 0);
-    const uint64_t br = __ballot(lane >= TH && lane < 2 * TH && hs == 0);
-    const int lm = (int)((uint32_t)bl & 0xffffu), rm = (int)((uint32_t)(br >> TH) & 0xffffu);
+    uint32_t hv, cl, cr;
+    int lm, rm;
+    level_halo<NB>(hU, hD, hs, lane, [](int v) { return v == 0; }, hv, lm, rm, cl, cr);
     int it = 0;
     for (;;) {  // S only grows and has at most TN bits: at most TN + 1 trips
         ++it;
-        const uint32_t lf = (uint32_t)__builtin_amdgcn_update_dpp(lm, (int)S, 0x138, 0xF, 0xF, false);
-        const uint32_t rt = (uint32_t)__builtin_amdgcn_update_dpp(rm, (int)S, 0x130, 0xF, 0xF, false);
-        const uint32_t Sn = (S | (((S << 1) | hu) & PU) | (((S >> 1) | hd) & PD) | (lf & PL) | (rt & PR)) & 0xffffu;
+        const uint32_t Sn = level_step<NB>(S, PL, PR, PU, PD, P8, hv, lm, rm, cl, cr);
         const bool grew = __any(Sn != S);
         S = Sn;
         if (!grew) break;
@@ -1658,29 +1787,31 @@ __device__ __forceinline__ void bfs_flood(int (&d)[TH], const Masks &m, int hU, 
 // Capped tile BFS for caps up to 15 by level sets; larger caps and exact
 // rounds use the relaxation, reachability rounds the flood. Returns true when
 // the tile converged.
-__device__ __forceinline__ bool tile_bfs(int (&d)[TH], Masks &m, int hU, int hD, int hs, int lane, int capv,
-                                         uint32_t &chg, int &rounds, int max_rounds, bool zero_sources = false) {
+template <int NB = 4>
+__device__ __forceinline__ bool tile_bfs(int (&d)[TH], Masks &m, Masks &m8, int hU, int hD, int hs, int lane,
+                                         int capv, uint32_t &chg, int &rounds, int max_rounds,
+                                         bool zero_sources = false) {
     if (capv == CAP_FLOOD) {
-        bfs_flood(d, m, hU, hD, hs, lane, chg, rounds);
+        bfs_flood<NB>(d, m, m8, hU, hD, hs, lane, chg, rounds);
         return true;
     }
     if (capv <= 3) {
-        bfs_levels<2>(d, m, hU, hD, hs, lane, capv, chg, rounds);
+        bfs_levels<2, NB>(d, m, m8, hU, hD, hs, lane, capv, chg, rounds);
         return true;
     }
     if (capv <= 7) {
-        bfs_levels<3>(d, m, hU, hD, hs, lane, capv, chg, rounds);
+        bfs_levels<3, NB>(d, m, m8, hU, hD, hs, lane, capv, chg, rounds);
         return true;
     }
     if (capv <= 15) {
-        bfs_levels<4>(d, m, hU, hD, hs, lane, capv, chg, rounds);
+        bfs_levels<4, NB>(d, m, m8, hU, hD, hs, lane, capv, chg, rounds);
         return true;
     }
     if (zero_sources) {
-        bfs_levels_zero(d, m, hU, hD, hs, lane, capv, chg, rounds);
+        bfs_levels_zero<NB>(d, m, m8, hU, hD, hs, lane, capv, chg, rounds);
         return true;
     }
-    return relax_tile_wave(d, m, hU, hD, hs, lane, chg, rounds, max_rounds);
+    return relax_tile_wave<NB>(d, m, m8, hU, hD, hs, lane, chg, rounds, max_rounds);
 }
 
 // Labels of one tile from converged exact distances: 1 for interior nodes
@@ -1729,11 +1860,13 @@ constexpr int GR_MAX_EXCHANGES = 64;
         const InstState *__restrict__ rstate
 
 // One-tile blocks fit 96 VGPRs without spilling: five wavefronts per SIMD
-// instead of four (the bigger blocks would spill, so they keep the default).
-template <int BX, int BY>
-__global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX * BY == 1 ? 5 : 1))) void k_grb(
+// instead of four (the bigger blocks would spill, so they keep the default;
+// so would the 8-neighbourhood's one-tile blocks at five, they run four).
+template <int BX, int BY, int NB = 4>
+__global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX * BY == 1 ? (NB == 8 ? 4 : 5) : 1))) void k_grb(
     KParams p, int parity, GCFG_RO_ARGS) {
     constexpr int NW = BX * BY;
+    static_assert(NB == 4 || NW == 1, "8-neighbourhood relabels run one-tile blocks");
     __shared__ int s_top[NW][TW], s_bot[NW][TW], s_side[NW][2 * TH];
     const int qlen = p.ctl->q[parity].qlen_gr;
     if ((int)blockIdx.x >= qlen) return;
@@ -1778,8 +1911,9 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
         const bool flood = capv == CAP_FLOOD;
         const int vinit = flood ? DINF : capv;  // start value of non-sink nodes
         int d[TH];
-        Masks m;
+        Masks m, m8;
         m.clear();
+        m8.clear();
         int hU = DINF, hD = DINF, hs = DINF;
         // A reachability round follows a capped round that converged with no
         // push pass in between, and skips the blocks at live 3 (every interior
@@ -1795,33 +1929,39 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
 #pragma unroll
             for (int y = 0; y < TH; ++y) d[y] = DINF;
         } else if (reinit && flood) {
-            load_halo<HALO_WITNESS>(p, ti, t, lane, hU, hD, hs);
+            load_halo<HALO_WITNESS, NB>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
                 m.put(y, T.m[y * TW + lane]);
+                if (NB == 8) m8.put(y, p.mask8[(size_t)t * TN + y * TW + lane]);
                 d[y] = T.d[y * TW + lane];
             }
 #pragma unroll
             for (int y = 0; y < TH; ++y) d[y] = ((m.get(y) & MB_ENEG) || d[y] < p.gr_cap) ? 0 : DINF;
         } else if (reinit) {
-            load_halo<HALO_MASK>(p, ti, t, lane, hU, hD, hs);
+            load_halo<HALO_MASK, NB>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
-            for (int y = 0; y < TH; ++y) m.put(y, T.m[y * TW + lane]);
+            for (int y = 0; y < TH; ++y) {
+                m.put(y, T.m[y * TW + lane]);
+                if (NB == 8) m8.put(y, p.mask8[(size_t)t * TN + y * TW + lane]);
+            }
 #pragma unroll
             for (int y = 0; y < TH; ++y) d[y] = (m.get(y) & MB_ENEG) ? 0 : vinit;
         } else if (flood) {
             // Visited blocks hold 0 or DINF by now; skipped ones their capped distances.
-            load_halo<HALO_WITNESS>(p, ti, t, lane, hU, hD, hs);
+            load_halo<HALO_WITNESS, NB>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
                 m.put(y, T.m[y * TW + lane]);
+                if (NB == 8) m8.put(y, p.mask8[(size_t)t * TN + y * TW + lane]);
                 d[y] = T.d[y * TW + lane];
             }
         } else {
-            load_halo<HALO_DIST>(p, ti, t, lane, hU, hD, hs);
+            load_halo<HALO_DIST, NB>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
                 m.put(y, T.m[y * TW + lane]);
+                if (NB == 8) m8.put(y, p.mask8[(size_t)t * TN + y * TW + lane]);
                 d[y] = T.d[y * TW + lane];
             }
         }
@@ -1836,7 +1976,8 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
             bool conv = true;
             // The first sweep of a REINIT starts from sinks and mask halos only (every
             // source at 0); later exchanges bring block-inner halos of any value.
-            if (need) conv = tile_bfs(d, m, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds, reinit && !flood && xr == 0);
+            if (need)
+                conv = tile_bfs<NB>(d, m, m8, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds, reinit && !flood && xr == 0);
             n_rounds += rounds;
             if (NW == 1) {
                 capped = !conv;
@@ -1890,6 +2031,15 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
         const bool chR = __any(lane == TW - 1 && chg != 0) && !inR && have && ti.nbr[1] >= 0;
         const bool chU = __any((chg & 1u) != 0) && !inU && have && ti.nbr[2] >= 0;
         const bool chD = __any((chg >> (TH - 1)) & 1u) && !inD && have && ti.nbr[3] >= 0;
+        // 8-neighbourhood (one-tile blocks): a changed corner node is a halo
+        // node of the diagonal tile as well.
+        bool chUL = false, chUR = false, chDL = false, chDR = false;
+        if (NB == 8 && have) {
+            chUL = __any(lane == 0 && (chg & 1u)) && tile_ul(ti) >= 0;
+            chUR = __any(lane == TW - 1 && (chg & 1u)) && tile_ur(ti) >= 0;
+            chDL = __any(lane == 0 && ((chg >> (TH - 1)) & 1u)) && tile_dl(ti) >= 0;
+            chDR = __any(lane == TW - 1 && ((chg >> (TH - 1)) & 1u)) && tile_dr(ti) >= 0;
+        }
         const bool any_act = __any(act);
         if (lane == 0 && have) {
             // Block anchors from the compile-time block shape (no divisions).
@@ -1898,6 +2048,10 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
             if (chR) mark_dirty(p, anchor(tx + 1, ty));
             if (chU) mark_dirty(p, anchor(tx, ty - 1));
             if (chD) mark_dirty(p, anchor(tx, ty + 1));
+            if (chUL) mark_dirty(p, tile_ul(ti));
+            if (chUR) mark_dirty(p, tile_ur(ti));
+            if (chDL) mark_dirty(p, tile_dl(ti));
+            if (chDR) mark_dirty(p, tile_dr(ti));
             set_active(p, t, any_act);
         }
         const bool any_fin = NW == 1 ? __any(fin) : __syncthreads_or(fin);
@@ -1911,7 +2065,12 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
     if (p.profile && lane == 0) atomic_add64(&p.ctl->gr_iters, n_rounds);
 }
 
-template <bool ONE>
+// 8-neighbourhood (NB == 8): every node also holds the diagonal capacities
+// {up-left, up-right, down-left, down-right} (KParams::caps8, mask bits in
+// KParams::mask8); tiles run in four colours, so no two running tiles touch,
+// corners included; instances are never lazy (the build writes every
+// capacity, and flow into another tile adds its reverse arcs there).
+template <bool ONE, int NB = 4>
 __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS) {
     const int qlen = p.ctl->q[parity].qlen_pr;
     if ((int)blockIdx.x >= qlen) return;
@@ -1935,12 +2094,16 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         const int capv = rstate[ti.inst].cap;  // nodes at the cap are frozen until the next relabel
         const bool lz = rdesc[ti.inst].lazy != 0;
         int hU, hD, hs;
-        load_halo<HALO_DIST>(p, ti, t, lane, hU, hD, hs);
-        int4 c[TH];
+        load_halo<HALO_DIST, NB>(p, ti, t, lane, hU, hD, hs);
+        int4 c[TH], c8[TH];
         int e[TH], d[TH];
         // Flow pushed out of the tile sideways, one row per lane: lanes 0..15
-        // into the left tile, lanes 16..31 into the right tile.
-        int accS = 0;
+        // into the left tile, lanes 16..31 into the right tile. With diagonals
+        // accSa holds what reached those nodes from the row below them (up-left
+        // / up-right arcs) and accSb from the row above (down-left /
+        // down-right), and lanes 32..35 the corner nodes' flow (the side-halo
+        // lane order).
+        int accS = 0, accSa = 0, accSb = 0;
         // Capacities are in the arena (written by the build or by an earlier
         // push pass), or for a fresh tile of a lazy build still only in the
         // inputs; this pass writes them to the arena either way. Excess and
@@ -1959,6 +2122,10 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         } else {
 #pragma unroll
             for (int y = 0; y < TH; ++y) c[y] = T.c[y * TW + lane];
+        }
+        if (NB == 8) {
+#pragma unroll
+            for (int y = 0; y < TH; ++y) c8[y] = p.caps8[(size_t)t * TN + y * TW + lane];
         }
         // Lazy instance: reverse-arc capacity that neighbours' pushes left for
         // the boundary nodes (they do not run in this colour pass): add it,
@@ -1979,19 +2146,39 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             if (ft) f[2 * TH + lane] = 0;
             if (fb) f[2 * TH + TW + lane] = 0;
         }
-        int accU = 0, accD = 0;
+        // Flow into the rows above and below the tile: straight (u / d arcs),
+        // and with diagonals Ua / Da from lane x + 1 (up-left / down-left
+        // arcs) and Ub / Db from lane x - 1 (up-right / down-right arcs).
+        int accU = 0, accD = 0, accUa = 0, accUb = 0, accDa = 0, accDb = 0;
+        // Heights of the diagonal neighbours of row y: lanes x - 1 / x + 1 of
+        // the rows above and below; lanes 0 and 63 take the side halo's rows
+        // y - 1 and y + 1 (the corners for rows 0 and TH - 1).
+        auto diag_heights = [&](const int y, int hu, int hd, int &hul, int &hur, int &hdl, int &hdr) {
+            const int sul = __builtin_amdgcn_readlane(hs, y ? y - 1 : HS_UL);
+            const int sur = __builtin_amdgcn_readlane(hs, y ? TH + y - 1 : HS_UR);
+            const int sdl = __builtin_amdgcn_readlane(hs, y < TH - 1 ? y + 1 : HS_DL);
+            const int sdr = __builtin_amdgcn_readlane(hs, y < TH - 1 ? TH + y + 1 : HS_DR);
+            hul = __builtin_amdgcn_update_dpp(sul, hu, 0x138, 0xF, 0xF, false);
+            hur = __builtin_amdgcn_update_dpp(sur, hu, 0x130, 0xF, 0xF, false);
+            hdl = __builtin_amdgcn_update_dpp(sdl, hd, 0x138, 0xF, 0xF, false);
+            hdr = __builtin_amdgcn_update_dpp(sdr, hd, 0x130, 0xF, 0xF, false);
+        };
         // Push along admissible arcs (d(u) == d(v) + 1) in the order l, r, u,
-        // d. Heights are those at the start of the iteration; flow received
-        // from a lower row can be pushed on in the same sweep.
+        // d (then up-left, up-right, down-left, down-right). Heights are those
+        // at the start of the iteration; flow received from a lower row can be
+        // pushed on in the same sweep.
         auto push_row = [&](const int y) {
             int hl = shr1(d[y]), hr = shl1(d[y]);
             if (lane == 0) hl = __builtin_amdgcn_readlane(hs, y);
             if (lane == TW - 1) hr = __builtin_amdgcn_readlane(hs, TH + y);
             const int hu = y ? d[y - 1] : hU;
             const int hd = y < TH - 1 ? d[y + 1] : hD;
-            int fl = 0, fr = 0, fu = 0, fd = 0;
+            int hul = 0, hur = 0, hdl = 0, hdr = 0;
+            if (NB == 8) diag_heights(y, hu, hd, hul, hur, hdl, hdr);
+            int fl = 0, fr = 0, fu = 0, fd = 0, ful = 0, fur = 0, fdl = 0, fdr = 0;
             int ek = e[y];
             int4 ck = c[y];
+            int4 c8k = NB == 8 ? c8[y] : make_int4(0, 0, 0, 0);
             if (ek > 0 && d[y] < capv) {
                 const int tgt = d[y] - 1;
                 if (ck.x > 0 && hl == tgt) {
@@ -2014,6 +2201,28 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
                     ek -= fd;
                     ck.w -= fd;
                 }
+                if (NB == 8) {
+                    if (ek > 0 && c8k.x > 0 && hul == tgt) {
+                        ful = min(ek, c8k.x);
+                        ek -= ful;
+                        c8k.x -= ful;
+                    }
+                    if (ek > 0 && c8k.y > 0 && hur == tgt) {
+                        fur = min(ek, c8k.y);
+                        ek -= fur;
+                        c8k.y -= fur;
+                    }
+                    if (ek > 0 && c8k.z > 0 && hdl == tgt) {
+                        fdl = min(ek, c8k.z);
+                        ek -= fdl;
+                        c8k.z -= fdl;
+                    }
+                    if (ek > 0 && c8k.w > 0 && hdr == tgt) {
+                        fdr = min(ek, c8k.w);
+                        ek -= fdr;
+                        c8k.w -= fdr;
+                    }
+                }
             }
             const int inL = shr1(fr), inR = shl1(fl);
             ck.x += inL;
@@ -2022,6 +2231,38 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             e[y] = ek + inL + inR;
             const int sfl = __builtin_amdgcn_readlane(fl, 0), sfr = __builtin_amdgcn_readlane(fr, TW - 1);
             accS += lane == y ? sfl : (lane == TH + y ? sfr : 0);
+            if (NB == 8) {
+                c8[y] = c8k;
+                // Diagonal flow out of lanes 0 / 63 enters the side columns one
+                // row up or down (a corner tile past the first or last row).
+                const int sful = __builtin_amdgcn_readlane(ful, 0), sfur = __builtin_amdgcn_readlane(fur, TW - 1);
+                const int sfdl = __builtin_amdgcn_readlane(fdl, 0), sfdr = __builtin_amdgcn_readlane(fdr, TW - 1);
+                if (y > 0) accSa += lane == y - 1 ? sful : (lane == TH + y - 1 ? sfur : 0);
+                else accSa += lane == HS_UL ? sful : (lane == HS_UR ? sfur : 0);
+                if (y < TH - 1) accSb += lane == y + 1 ? sfdl : (lane == TH + y + 1 ? sfdr : 0);
+                else accSb += lane == HS_DL ? sfdl : (lane == HS_DR ? sfdr : 0);
+                // Inside the tile: lane x - 1 / x + 1 of the row above and below
+                // gains the flow and the reverse arc (down-right / down-left
+                // for up-left / up-right pushes, up-right / up-left for
+                // down-left / down-right ones).
+                const int iul = shl1(ful), iur = shr1(fur), idl = shl1(fdl), idr = shr1(fdr);
+                if (y > 0) {
+                    e[y - 1] += iul + iur;
+                    c8[y - 1].w += iul;
+                    c8[y - 1].z += iur;
+                } else {
+                    accUa += iul;
+                    accUb += iur;
+                }
+                if (y < TH - 1) {
+                    e[y + 1] += idl + idr;
+                    c8[y + 1].y += idl;
+                    c8[y + 1].x += idr;
+                } else {
+                    accDa += idl;
+                    accDb += idr;
+                }
+            }
             if (y > 0) {
                 e[y - 1] += fu;
                 c[y - 1].w += fu;
@@ -2045,12 +2286,20 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             if (lane == TW - 1) hr = __builtin_amdgcn_readlane(hs, TH + y);
             const int hu = y ? d[y - 1] : hU;
             const int hd = y < TH - 1 ? d[y + 1] : hD;
+            int hul = 0, hur = 0, hdl = 0, hdr = 0;
+            if (NB == 8) diag_heights(y, hu, hd, hul, hur, hdl, hdr);
             if (!(e[y] > 0 && d[y] < capv)) return 0;
             int mm = DINF;
             if (c[y].x > 0) mm = min(mm, hl);
             if (c[y].y > 0) mm = min(mm, hr);
             if (c[y].z > 0) mm = min(mm, hu);
             if (c[y].w > 0) mm = min(mm, hd);
+            if (NB == 8) {
+                if (c8[y].x > 0) mm = min(mm, hul);
+                if (c8[y].y > 0) mm = min(mm, hur);
+                if (c8[y].z > 0) mm = min(mm, hdl);
+                if (c8[y].w > 0) mm = min(mm, hdr);
+            }
             int nd = mm >= DINF ? DINF : mm + 1;
             // A label of n or more proves the node cannot reach the sink; a
             // label past the cap freezes the node at the cap (still a valid
@@ -2067,6 +2316,11 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             T.e[y * TW + lane] = e[y];
             T.d[y * TW + lane] = d[y];
             T.m[y * TW + lane] = (uint8_t)node_mask(c[y], e[y]);
+            if (NB == 8) {
+                const size_t g = (size_t)t * TN + y * TW + lane;
+                p.caps8[g] = c8[y];
+                p.mask8[g] = (uint8_t)node_mask(c8[y], 0);
+            }
             mine |= e[y] > 0 && d[y] < capv;
             sx += max(e[y], 0);
         };
@@ -2102,25 +2356,32 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         }
         // Flow pushed into halo nodes (their tiles do not run in this colour
         // pass; int32 atomics because two same-colour tiles can feed one
-        // corner node). Lanes 0..31 feed the side tiles (one row each), every
-        // lane may feed the tiles above and below. All excess atomics are
-        // issued before their results are used for the mask updates, so
-        // their latencies overlap.
-        const int side = lane < TH ? ti.nbr[0] : ti.nbr[1];
-        const bool pushS = lane < 2 * TH && accS != 0 && side >= 0;
-        const size_t gS = (size_t)(pushS ? side : 0) * TN + (size_t)(lane & (TH - 1)) * TW + (lane < TH ? TW - 1 : 0);
+        // corner node). Lanes 0..31 feed the side tiles (one row each), lanes
+        // 32..35 the corner tiles (8-neighbourhood), every lane may feed the
+        // tiles above and below. All excess atomics are issued before their
+        // results are used for the mask updates, so their latencies overlap.
+        int sidx;
+        const int side = side_halo<NB>(ti, lane, sidx);
+        const int totS = accS + accSa + accSb, totU = accU + accUa + accUb, totD = accD + accDa + accDb;
+        const bool pushS = totS != 0 && side >= 0;
+        const size_t gS = (size_t)(pushS ? side : 0) * TN + (size_t)sidx;
+        // Receiving nodes left of the tile: the left column and the left corners.
+        const bool leftS = lane < TH || lane == HS_UL || lane == HS_DL;
         // The side node's arc back to us: the right arc of the left tile's
         // column TW-1 (slot TH + row), the left arc of the right tile's column 0 (slot row).
         const int slotS = lane < TH ? TH + lane : lane - TH;
         const int arcS = lane < TH ? 2 : 1;
-        const bool pushU = accU != 0 && ti.nbr[2] >= 0, pushD = accD != 0 && ti.nbr[3] >= 0;
+        const bool pushU = totU != 0 && ti.nbr[2] >= 0, pushD = totD != 0 && ti.nbr[3] >= 0;
         const size_t gU = (size_t)(pushU ? ti.nbr[2] : 0) * TN + (size_t)(TH - 1) * TW + lane;
         const size_t gD = (size_t)(pushD ? ti.nbr[3] : 0) * TN + lane;
         int oldS = 0, oldU = 0, oldD = 0;
         // Reverse arcs: in a lazy instance to the receiving tile's inflow
         // slots (its capacities may still live in the inputs; down arcs of the
         // upper tile's last row, up arcs of the lower tile's first row),
-        // otherwise to its capacities directly.
+        // otherwise to its capacities directly. Diagonal reverse arcs (never
+        // lazy) go to the diagonal capacities {ul, ur, dl, dr}: flow from the
+        // row below a node opens its down-left or down-right arc, flow from
+        // the row above its up-left or up-right arc.
         int *revS = lz ? &p.inflow[(size_t)(pushS ? side : 0) * INFLOW + slotS]
                        : reinterpret_cast<int *>(&p.caps[gS]) + (lane < TH ? 1 : 0);
         int *revU = lz ? &p.inflow[(size_t)(pushU ? ti.nbr[2] : 0) * INFLOW + 2 * TH + TW + lane]
@@ -2128,43 +2389,74 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         int *revD = lz ? &p.inflow[(size_t)(pushD ? ti.nbr[3] : 0) * INFLOW + 2 * TH + lane]
                        : reinterpret_cast<int *>(&p.caps[gD]) + 2;
         if (pushS) {
-            oldS = atomicAdd(&p.exc[gS], accS);
-            atomicAdd(revS, accS);
+            oldS = atomicAdd(&p.exc[gS], totS);
+            if (accS) atomicAdd(revS, accS);
+            if (NB == 8) {
+                int *r8 = reinterpret_cast<int *>(&p.caps8[gS]);
+                if (accSa) atomicAdd(r8 + (leftS ? 3 : 2), accSa);
+                if (accSb) atomicAdd(r8 + (leftS ? 1 : 0), accSb);
+            }
         }
         if (pushU) {
-            oldU = atomicAdd(&p.exc[gU], accU);
-            atomicAdd(revU, accU);
+            oldU = atomicAdd(&p.exc[gU], totU);
+            if (accU) atomicAdd(revU, accU);
+            if (NB == 8) {
+                int *r8 = reinterpret_cast<int *>(&p.caps8[gU]);
+                if (accUa) atomicAdd(r8 + 3, accUa);
+                if (accUb) atomicAdd(r8 + 2, accUb);
+            }
         }
         if (pushD) {
-            oldD = atomicAdd(&p.exc[gD], accD);
-            atomicAdd(revD, accD);
+            oldD = atomicAdd(&p.exc[gD], totD);
+            if (accD) atomicAdd(revD, accD);
+            if (NB == 8) {
+                int *r8 = reinterpret_cast<int *>(&p.caps8[gD]);
+                if (accDa) atomicAdd(r8 + 1, accDa);
+                if (accDb) atomicAdd(r8 + 0, accDb);
+            }
         }
-        if (pushS) mask_after_push_in(p, gS, arcS, oldS + accS);
-        if (pushU) mask_after_push_in(p, gU, 8, oldU + accU);
-        if (pushD) mask_after_push_in(p, gD, 4, oldD + accD);
+        if (pushS) mask_after_push_in(p, gS, accS ? arcS : 0, oldS + totS);
+        if (pushU) mask_after_push_in(p, gU, accU ? 8 : 0, oldU + totU);
+        if (pushD) mask_after_push_in(p, gD, accD ? 4 : 0, oldD + totD);
+        if (NB == 8) {
+            const int bS = (accSa ? (leftS ? 8 : 4) : 0) | (accSb ? (leftS ? 2 : 1) : 0);
+            const int bU = (accUa ? 8 : 0) | (accUb ? 4 : 0), bD = (accDa ? 2 : 0) | (accDb ? 1 : 0);
+            if (pushS && bS) mask8_after_push_in(p, gS, bS);
+            if (pushU && bU) mask8_after_push_in(p, gU, bU);
+            if (pushD && bD) mask8_after_push_in(p, gD, bD);
+        }
         // Stuck-excess sums of the tiles pushed into (they are not running).
-        if (pushS) atomic_add64(&p.tsum[2 * (size_t)side + 1], (int64_t)max(oldS + accS, 0) - max(oldS, 0));
-        if (pushU) atomic_add64(&p.tsum[2 * (size_t)ti.nbr[2] + 1], (int64_t)max(oldU + accU, 0) - max(oldU, 0));
-        if (pushD) atomic_add64(&p.tsum[2 * (size_t)ti.nbr[3] + 1], (int64_t)max(oldD + accD, 0) - max(oldD, 0));
+        if (pushS) atomic_add64(&p.tsum[2 * (size_t)side + 1], (int64_t)max(oldS + totS, 0) - max(oldS, 0));
+        if (pushU) atomic_add64(&p.tsum[2 * (size_t)ti.nbr[2] + 1], (int64_t)max(oldU + totU, 0) - max(oldU, 0));
+        if (pushD) atomic_add64(&p.tsum[2 * (size_t)ti.nbr[3] + 1], (int64_t)max(oldD + totD, 0) - max(oldD, 0));
         sx = wave_sum64_dpp(sx);
-        const int toL = pushS && lane < TH, toR = pushS && lane >= TH;
+        const int toL = pushS && lane < TH, toR = pushS && lane >= TH && lane < 2 * TH;
         const bool anyL = __any(toL), anyR = __any(toR);
         const bool anyU = __any(pushU), anyD = __any(pushD);
+        // 8-neighbourhood: the corner lanes' receiving tiles (side-halo order).
+        const uint64_t corners = NB == 8 ? __ballot(pushS && lane >= HS_UL) >> HS_UL : 0;
         const bool any_mine = __any(mine);
         if (lane == 0) {
             if (anyL) set_active(p, ti.nbr[0], 1);
             if (anyR) set_active(p, ti.nbr[1], 1);
             if (anyU) set_active(p, ti.nbr[2], 1);
             if (anyD) set_active(p, ti.nbr[3], 1);
+            if (NB == 8) {
+                if (corners & 1) set_active(p, tile_ul(ti), 1);
+                if (corners & 2) set_active(p, tile_ur(ti), 1);
+                if (corners & 4) set_active(p, tile_dl(ti), 1);
+                if (corners & 8) set_active(p, tile_dr(ti), 1);
+            }
             set_active(p, t, any_mine);
             p.tsum[2 * (size_t)t + 1] = sx;
             if (fresh) p.mat[t] = 1;
         }
         if (p.gr_cap > 0) {
             // Blocks within cap distance of this tile (and of the tiles it
-            // pushed into) need their capped distances recomputed.
-            const int rx = (p.gr_cap + TW - 1) / TW + ((anyL || anyR) ? 1 : 0);
-            const int ry = (p.gr_cap + TH - 1) / TH + ((anyU || anyD) ? 1 : 0);
+            // pushed into) need their capped distances recomputed. A path of
+            // length K moves at most K columns and K rows, diagonals included.
+            const int rx = (p.gr_cap + TW - 1) / TW + ((anyL || anyR || corners) ? 1 : 0);
+            const int ry = (p.gr_cap + TH - 1) / TH + ((anyU || anyD || corners) ? 1 : 0);
             const int span = 2 * rx + 1, cnt = span * (2 * ry + 1);
             for (int k = lane; k < cnt; k += TW) {
                 const int x = ti.tx + k % span - rx, y = ti.ty + k / span - ry;
@@ -2229,6 +2521,7 @@ __device__ __forceinline__ int input_d0(const InstDesc &D, int X, int Y) {
 // arguments, like the solve kernels' queue arguments: their loads are uniform
 // and go through the scalar cache, so a tile's prologue is one scalar round
 // trip per level instead of vector loads and waits.
+template <int NB = 4>
 __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restrict__ rbmap,
                                                const TileInfo *__restrict__ rtiles,
                                                const InstDesc *__restrict__ rdesc, int total) {
@@ -2255,8 +2548,9 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
         const int X = tx * TW + lane, Y0 = ty * TH;
         const TileRef T = tile_ref(p, t);
         int d[TH];
-        Masks m;
+        Masks m, m8;
         m.clear();
+        m8.clear();
         const int capv = p.gr_cap > 0 ? p.gr_cap : DINF;  // the first relabel sweep, capped like k_arm says
         int64_t sp = 0, sx = 0;
         int bad = 0;
@@ -2267,14 +2561,21 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
         const int hU = ty > 0 ? hUv : DINF;
         const int hD = ty < D.tiles_y - 1 ? hDv : DINF;
         const bool hleft = lane < TH && tx > 0, hright = lane >= TH && lane < 2 * TH && tx < D.tiles_x - 1;
-        const int hsv = input_d0(D, lane < TH ? tx * TW - 1 : tx * TW + TW, Y0 + (lane & (TH - 1)));
-        const int hs = (hleft || hright) ? hsv : DINF;
+        // 8-neighbourhood: lanes 32..35 the corner nodes (up-left, up-right,
+        // down-left, down-right), where the diagonal tile exists.
+        const bool cl = lane == HS_UL || lane == HS_DL, cu = lane == HS_UL || lane == HS_UR;
+        const bool corner = NB == 8 && lane >= HS_UL && lane <= HS_DR && (cl ? tx > 0 : tx < D.tiles_x - 1) &&
+                            (cu ? ty > 0 : ty < D.tiles_y - 1);
+        const int hsx = lane < 2 * TH ? (lane < TH ? tx * TW - 1 : tx * TW + TW) : (cl ? tx * TW - 1 : tx * TW + TW);
+        const int hsy = lane < 2 * TH ? Y0 + (lane & (TH - 1)) : (cu ? Y0 - 1 : Y0 + TH);
+        const int hsv = input_d0(D, hsx, hsy);
+        const int hs = (hleft || hright || corner) ? hsv : DINF;
         // Two halves of 8 rows: all input loads of a half, then compute and
         // store (clamped in-bounds loads; values outside the interior masked).
 #pragma unroll
         for (int h = 0; h < TH; h += TH / 2) {
             int vp[TH / 2], vn[TH / 2];
-            int4 vc[TH / 2];
+            int4 vc[TH / 2], vc8[TH / 2];
             bool vin[TH / 2];
 #pragma unroll
             for (int k = 0; k < TH / 2; ++k) {
@@ -2288,11 +2589,18 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
                     vc[k] = make_int4(ldg_in(D.left, i), ldg_in(D.right, i), ldg_in(D.up, i), ldg_in(D.down, i));
                 else
                     vc[k] = make_int4(D.penalty, D.penalty, D.penalty, D.penalty);
+                if (NB == 8) {
+                    if (D.up)
+                        vc8[k] = make_int4(ldg_in(D.upleft, i), ldg_in(D.upright, i), ldg_in(D.downleft, i),
+                                           ldg_in(D.downright, i));
+                    else
+                        vc8[k] = make_int4(D.penalty, D.penalty, D.penalty, D.penalty);
+                }
             }
 #pragma unroll
             for (int k = 0; k < TH / 2; ++k) {
                 const int y = h + k, Y = Y0 + y;
-                int4 c = make_int4(0, 0, 0, 0);
+                int4 c = make_int4(0, 0, 0, 0), c8 = make_int4(0, 0, 0, 0);
                 int e = 0;
                 if (vin[k]) {
                     c = vc[k];
@@ -2300,7 +2608,17 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
                     if (X == D.W - 2) c.y = 0;
                     if (Y == 1) c.z = 0;
                     if (Y == D.H - 2) c.w = 0;
-                    bad |= tlinks_bad(vp[k], vn[k]) | nlinks_bad(c, 4);
+                    if (NB == 8) {
+                        // Diagonal arcs towards the border are zero like the
+                        // side arcs (GraphCutKernels.cu:133-151,190-213).
+                        c8 = vc8[k];
+                        if (X == 1) c8.x = c8.z = 0;
+                        if (X == D.W - 2) c8.y = c8.w = 0;
+                        if (Y == 1) c8.x = c8.y = 0;
+                        if (Y == D.H - 2) c8.z = c8.w = 0;
+                        bad |= nlinks_bad(c8, 8);
+                    }
+                    bad |= tlinks_bad(vp[k], vn[k]) | nlinks_bad(c, NB);
                     e = vp[k] - vn[k];
                     sp += vp[k];
                     sx += max(e, 0);
@@ -2313,6 +2631,14 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
                 const int mk = node_mask(c, e);
                 m.put(y, mk);
                 T.m[y * TW + lane] = (uint8_t)mk;
+                if (NB == 8) {
+                    // Never lazy: the diagonal capacities are written here.
+                    const size_t g = (size_t)t * TN + y * TW + lane;
+                    const int mk8 = node_mask(c8, 0);
+                    m8.put(y, mk8);
+                    p.caps8[g] = c8;
+                    p.mask8[g] = (uint8_t)mk8;
+                }
                 d[y] = e < 0 ? 0 : capv;
             }
         }
@@ -2325,7 +2651,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
         uint32_t chg = 0;
         int rounds = 0;
         // Sources: the sinks and the halo sinks read from the inputs, all at 0.
-        const bool conv = tile_bfs(d, m, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds, true);
+        const bool conv = tile_bfs<NB>(d, m, m8, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds, true);
         int act = 0, fin = 0, nearv = 0, open = 0;
         const bool colin = X > 0 && X < D.W - 1;
 #pragma unroll
@@ -2347,6 +2673,11 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
         const bool chR = __any(lane == TW - 1 && chg != 0);
         const bool chU = __any((chg & 1u) != 0);
         const bool chD = __any((chg >> (TH - 1)) & 1u);
+        // 8-neighbourhood: corner nodes are halo nodes of the diagonal tiles.
+        const bool chC0 = NB == 8 && __any(lane == 0 && (chg & 1u));
+        const bool chC1 = NB == 8 && __any(lane == TW - 1 && (chg & 1u));
+        const bool chC2 = NB == 8 && __any(lane == 0 && ((chg >> (TH - 1)) & 1u));
+        const bool chC3 = NB == 8 && __any(lane == TW - 1 && ((chg >> (TH - 1)) & 1u));
         const bool any_act = __any(act);
         const bool any_bad = __any(bad);
         sp = wave_sum64_dpp(sp);
@@ -2357,6 +2688,10 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
             if (chR && tx < D.tiles_x - 1) mark_dirty(p, block_anchor(p, D, tx + 1, ty));
             if (chU && ty > 0) mark_dirty(p, block_anchor(p, D, tx, ty - 1));
             if (chD && ty < D.tiles_y - 1) mark_dirty(p, block_anchor(p, D, tx, ty + 1));
+            if (chC0 && tx > 0 && ty > 0) mark_dirty(p, block_anchor(p, D, tx - 1, ty - 1));
+            if (chC1 && tx < D.tiles_x - 1 && ty > 0) mark_dirty(p, block_anchor(p, D, tx + 1, ty - 1));
+            if (chC2 && tx > 0 && ty < D.tiles_y - 1) mark_dirty(p, block_anchor(p, D, tx - 1, ty + 1));
+            if (chC3 && tx < D.tiles_x - 1 && ty < D.tiles_y - 1) mark_dirty(p, block_anchor(p, D, tx + 1, ty + 1));
             if (!conv) mark_dirty(p, block_anchor(p, D, tx, ty));
             set_active(p, t, any_act);
             if (rank) atomicMax(&p.live[block_anchor(p, D, tx, ty)], rank);
@@ -2595,8 +2930,12 @@ void launch_build(const KParams &p, const int32_t *d_slots, int n, int total_til
     if (p.variant) {
         int grid = total_tiles < 65536 ? total_tiles : 65536;
         if (grid >= NUM_XCD) grid -= grid % NUM_XCD;  // whole rounds over the XCDs
-        launch_timed(k_buildw, dim3(grid), dim3(64), s, ev, p, (const int32_t *)p.bmap, (const TileInfo *)p.tiles,
-                     (const InstDesc *)p.desc, total_tiles);
+        if (p.nbhd == 8)
+            launch_timed(k_buildw<8>, dim3(grid), dim3(64), s, ev, p, (const int32_t *)p.bmap,
+                         (const TileInfo *)p.tiles, (const InstDesc *)p.desc, total_tiles);
+        else
+            launch_timed(k_buildw<4>, dim3(grid), dim3(64), s, ev, p, (const int32_t *)p.bmap,
+                         (const TileInfo *)p.tiles, (const InstDesc *)p.desc, total_tiles);
         return;
     }
     const int grid = total_tiles < 8192 ? total_tiles : 8192;
@@ -2623,20 +2962,26 @@ void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvent
 }
 
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
-    if (p.variant && p.inner_iters == 1)
-        launch_timed(k_prw<true>, dim3(grid), dim3(64), s, ev, p, parity, (const int32_t *)p.queue_pr,
-                     (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
+    const int32_t *q = p.queue_pr;
+    const TileInfo *ti = p.tiles;
+    const InstDesc *de = p.desc;
+    const InstState *st = p.state;
+    if (p.variant && p.nbhd == 8 && p.inner_iters == 1)
+        launch_timed(k_prw<true, 8>, dim3(grid), dim3(64), s, ev, p, parity, q, ti, de, st);
+    else if (p.variant && p.nbhd == 8)
+        launch_timed(k_prw<false, 8>, dim3(grid), dim3(64), s, ev, p, parity, q, ti, de, st);
+    else if (p.variant && p.inner_iters == 1)
+        launch_timed(k_prw<true, 4>, dim3(grid), dim3(64), s, ev, p, parity, q, ti, de, st);
     else if (p.variant)
-        launch_timed(k_prw<false>, dim3(grid), dim3(64), s, ev, p, parity, (const int32_t *)p.queue_pr,
-                     (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
+        launch_timed(k_prw<false, 4>, dim3(grid), dim3(64), s, ev, p, parity, q, ti, de, st);
     else if (p.nbhd == 8)
         launch_timed(k_pr8, dim3(grid), dim3(NT), s, ev, p, parity);
     else
         launch_timed(k_pr, dim3(grid), dim3(NT), s, ev, p, parity);
 }
-template <int BX, int BY>
+template <int BX, int BY, int NB = 4>
 static void launch_grb(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
-    launch_timed(k_grb<BX, BY>, dim3(grid), dim3(64 * BX * BY), s, ev, p, parity, (const int32_t *)p.queue_gr,
+    launch_timed(k_grb<BX, BY, NB>, dim3(grid), dim3(64 * BX * BY), s, ev, p, parity, (const int32_t *)p.queue_gr,
                  (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
 }
 
@@ -2653,6 +2998,10 @@ void launch_gr(const KParams &p, int grid, int parity, hipStream_t s, const Laun
             launch_timed(k_gr8, dim3(grid), dim3(NT), s, ev, p, parity);
         else
             launch_timed(k_gr, dim3(grid), dim3(NT), s, ev, p, parity);
+        return;
+    }
+    if (p.nbhd == 8) {  // one-tile blocks only
+        launch_grb<1, 1, 8>(p, grid, parity, s, ev);
         return;
     }
     switch (p.gbx * 100 + p.gby) {

@@ -41,6 +41,32 @@ def test_random8(g, ora, shape):
         check8(g, ora, W, H, pos, neg, dirs)
 
 
+@pytest.mark.parametrize("shape", [(17, 9), (65, 17), (200, 70)])
+def test_random8_workgroup_kernels(g, ora, shape):
+    """The workgroup-per-tile kernels (kernels=0) on the same instances."""
+    W, H = shape
+    rng = np.random.default_rng(7000 + W * 100 + H)
+    o = g.default_options()
+    o.kernels = 0
+    for trial in range(2):
+        pos, neg, dirs = rand8(rng, W, H)
+        check8(g, ora, W, H, pos, neg, dirs, options=o)
+
+
+def test_corner_flows8(g, ora):
+    """Instances whose flow has to cross many tile corners diagonally:
+    t-links only on a sparse lattice, n-links only on the diagonals, on a
+    grid of 3 x 4 tiles (the corner nodes of every tile carry flow)."""
+    rng = np.random.default_rng(88)
+    W, H = 3 * 64 + 5, 4 * 16 + 3
+    z = np.zeros((H, W), np.int32)
+    for trial in range(3):
+        pos = np.where(rng.random((H, W)) < 0.05, rng.integers(0, 500, (H, W)), 0).astype(np.int32)
+        neg = np.where(rng.random((H, W)) < 0.05, rng.integers(0, 500, (H, W)), 0).astype(np.int32)
+        diag = tuple(rng.integers(0, 80, (H, W)).astype(np.int32) for _ in range(4))
+        check8(g, ora, W, H, pos, neg, (z, z, z, z) + diag)
+
+
 def test_penalty8_and_glibc_sized(g, ora):
     rng = np.random.default_rng(3)
     W, H = 400, 400
@@ -92,8 +118,12 @@ def test_options8(g, ora):
     rng = np.random.default_rng(12)
     W, H = 150, 80
     pos, neg, dirs = rand8(rng, W, H, tmax=200, emax=60)
+    # kernels=1 (the default): wavefront kernels, capped relabels with gr_cap;
+    # kernels=0: the workgroup-per-tile kernels, always exact relabels.
     for kw in (dict(sweeps_per_gr=1, inner_iters=1), dict(sweeps_per_gr=8, inner_iters=64, use_graph=0),
-               dict(kernels=1, gr_cap=4)):  # kernels/gr_cap do not apply to the 8-neighbourhood
+               dict(gr_cap=4), dict(gr_cap=0), dict(gr_cap=16, inner_iters=1), dict(gr_cap=0, gr_rounds=1),
+               dict(kernels=0), dict(kernels=0, sweeps_per_gr=2, inner_iters=4),
+               dict(gr_block_w=2, gr_block_h=2)):  # relabel blocks are one tile in the 8-neighbourhood
         o = g.default_options()
         for k, v in kw.items():
             setattr(o, k, v)
