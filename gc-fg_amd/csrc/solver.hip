@@ -2150,31 +2150,42 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         // and with diagonals Ua / Da from lane x + 1 (up-left / down-left
         // arcs) and Ub / Db from lane x - 1 (up-right / down-right arcs).
         int accU = 0, accD = 0, accUa = 0, accUb = 0, accDa = 0, accDb = 0;
-        // Heights of the diagonal neighbours of row y: lanes x - 1 / x + 1 of
-        // the rows above and below; lanes 0 and 63 take the side halo's rows
-        // y - 1 and y + 1 (the corners for rows 0 and TH - 1).
-        auto diag_heights = [&](const int y, int hu, int hd, int &hul, int &hur, int &hdl, int &hdr) {
-            const int sul = __builtin_amdgcn_readlane(hs, y ? y - 1 : HS_UL);
-            const int sur = __builtin_amdgcn_readlane(hs, y ? TH + y - 1 : HS_UR);
-            const int sdl = __builtin_amdgcn_readlane(hs, y < TH - 1 ? y + 1 : HS_DL);
-            const int sdr = __builtin_amdgcn_readlane(hs, y < TH - 1 ? TH + y + 1 : HS_DR);
-            hul = __builtin_amdgcn_update_dpp(sul, hu, 0x138, 0xF, 0xF, false);
-            hur = __builtin_amdgcn_update_dpp(sur, hu, 0x130, 0xF, 0xF, false);
-            hdl = __builtin_amdgcn_update_dpp(sdl, hd, 0x138, 0xF, 0xF, false);
-            hdr = __builtin_amdgcn_update_dpp(sdr, hd, 0x130, 0xF, 0xF, false);
+        // Row y's heights seen one lane over: lane x <- lane x - 1 (lane 0: the
+        // left halo column's row y) and lane x <- lane x + 1 (lane 63: the
+        // right column's). With diagonals they are also the up-left / up-right
+        // heights of row y + 1 and the down-left / down-right ones of row y - 1,
+        // so the callers below compute each row's once and pass it on; the
+        // halo rows' (with the corner nodes at lanes 0 and 63) are fixed per tile.
+        auto shl_row = [&](const int y) {
+            return __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(hs, y), d[y], 0x138, 0xF, 0xF, false);
         };
+        auto shr_row = [&](const int y) {
+            return __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(hs, TH + y), d[y], 0x130, 0xF, 0xF, false);
+        };
+        int hUL = 0, hUR = 0, hDL = 0, hDR = 0;
+        if (NB == 8) {
+            hUL = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(hs, HS_UL), hU, 0x138, 0xF, 0xF, false);
+            hUR = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(hs, HS_UR), hU, 0x130, 0xF, 0xF, false);
+            hDL = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(hs, HS_DL), hD, 0x138, 0xF, 0xF, false);
+            hDR = __builtin_amdgcn_update_dpp(__builtin_amdgcn_readlane(hs, HS_DR), hD, 0x130, 0xF, 0xF, false);
+        }
         // Push along admissible arcs (d(u) == d(v) + 1) in the order l, r, u,
         // d (then up-left, up-right, down-left, down-right). Heights are those
         // at the start of the iteration; flow received from a lower row can be
         // pushed on in the same sweep.
-        auto push_row = [&](const int y) {
-            int hl = shr1(d[y]), hr = shl1(d[y]);
-            if (lane == 0) hl = __builtin_amdgcn_readlane(hs, y);
-            if (lane == TW - 1) hr = __builtin_amdgcn_readlane(hs, TH + y);
+        // 8-neighbourhood: (hl, hr) are row y's shifted heights, (hul, hur) row
+        // y - 1's and (hdl, hdr) row y + 1's; the 4-neighbourhood computes
+        // (hl, hr) itself, only for rows it does not skip.
+        auto push_row = [&](const int y, int hl, int hr, int hul, int hur, int hdl, int hdr) {
+            // A row with no active node pushes nothing (and receives nothing
+            // from within itself): skipped whole, a uniform branch.
+            if (!__any(e[y] > 0 && d[y] < capv)) return;
+            if (NB == 4) {
+                hl = shl_row(y);
+                hr = shr_row(y);
+            }
             const int hu = y ? d[y - 1] : hU;
             const int hd = y < TH - 1 ? d[y + 1] : hD;
-            int hul = 0, hur = 0, hdl = 0, hdr = 0;
-            if (NB == 8) diag_heights(y, hu, hd, hul, hur, hdl, hdr);
             int fl = 0, fr = 0, fu = 0, fd = 0, ful = 0, fur = 0, fdl = 0, fdr = 0;
             int ek = e[y];
             int4 ck = c[y];
@@ -2280,14 +2291,16 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
         // Rows are relabelled in order, each with the new height of the row
         // above and the old one of the row below. Returns whether the node
         // is still active (below the cap).
-        auto relabel_row = [&](const int y) -> int {
-            int hl = shr1(d[y]), hr = shl1(d[y]);
-            if (lane == 0) hl = __builtin_amdgcn_readlane(hs, y);
-            if (lane == TW - 1) hr = __builtin_amdgcn_readlane(hs, TH + y);
+        // (8-neighbourhood: hul, hur from the new heights of row y - 1, hdl,
+        // hdr from the old ones of row y + 1.)
+        auto relabel_row = [&](const int y, int hl, int hr, int hul, int hur, int hdl, int hdr) -> int {
+            if (!__any(e[y] > 0 && d[y] < capv)) return 0;
+            if (NB == 4) {
+                hl = shl_row(y);
+                hr = shr_row(y);
+            }
             const int hu = y ? d[y - 1] : hU;
             const int hd = y < TH - 1 ? d[y + 1] : hD;
-            int hul = 0, hur = 0, hdl = 0, hdr = 0;
-            if (NB == 8) diag_heights(y, hu, hd, hul, hur, hdl, hdr);
             if (!(e[y] > 0 && d[y] < capv)) return 0;
             int mm = DINF;
             if (c[y].x > 0) mm = min(mm, hl);
@@ -2325,30 +2338,80 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             sx += max(e[y], 0);
         };
         int it = 0;
+        // Shifted heights carried along the rows (8-neighbourhood only): pl/pr
+        // row y - 1's, cl/cr row y's, nl/nr row y + 1's, all as of the start
+        // of the push phase; npl/npr the relabelled row above's.
+        int pl = hUL, pr = hUR, cl = 0, cr = 0, nl = 0, nr = 0, npl = hUL, npr = hUR;
+        auto next_row = [&](const int y) {  // nl/nr <- row y + 1 (the halo row below past the last)
+            if (NB != 8) return;
+            nl = y < TH - 1 ? shl_row(y + 1) : hDL;
+            nr = y < TH - 1 ? shr_row(y + 1) : hDR;
+        };
         if (ONE) {
             // One iteration: row y - 1 is final once row y has pushed (only
             // rows y - 2 .. y push into it), so it is relabelled and stored
             // right then, with exactly the values of the general order, and
             // its registers are free for the rows still to come.
             it = 1;
+            if (NB == 8) {
+                cl = shl_row(0);
+                cr = shr_row(0);
+            }
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
-                push_row(y);
+                next_row(y);
+                push_row(y, cl, cr, pl, pr, nl, nr);
                 if (y > 0) {
-                    relabel_row(y - 1);
+                    relabel_row(y - 1, pl, pr, npl, npr, cl, cr);
+                    if (NB == 8) {
+                        npl = shl_row(y - 1);
+                        npr = shr_row(y - 1);
+                    }
                     store_row(y - 1);
                 }
+                pl = cl;
+                pr = cr;
+                cl = nl;
+                cr = nr;
             }
-            relabel_row(TH - 1);
+            relabel_row(TH - 1, pl, pr, npl, npr, hDL, hDR);
             store_row(TH - 1);
         } else {
             while (it < p.inner_iters) {
                 ++it;
+                if (NB == 8) {
+                    pl = hUL;
+                    pr = hUR;
+                    cl = shl_row(0);
+                    cr = shr_row(0);
+                }
 #pragma unroll
-                for (int y = 0; y < TH; ++y) push_row(y);
+                for (int y = 0; y < TH; ++y) {
+                    next_row(y);
+                    push_row(y, cl, cr, pl, pr, nl, nr);
+                    pl = cl;
+                    pr = cr;
+                    cl = nl;
+                    cr = nr;
+                }
                 int act = 0;
+                if (NB == 8) {
+                    npl = hUL;
+                    npr = hUR;
+                    cl = shl_row(0);
+                    cr = shr_row(0);
+                }
 #pragma unroll
-                for (int y = 0; y < TH; ++y) act |= relabel_row(y);
+                for (int y = 0; y < TH; ++y) {
+                    next_row(y);
+                    act |= relabel_row(y, cl, cr, npl, npr, nl, nr);
+                    if (NB == 8) {
+                        npl = shl_row(y);
+                        npr = shr_row(y);
+                    }
+                    cl = nl;
+                    cr = nr;
+                }
                 if (!__any(act)) break;
             }
 #pragma unroll
