@@ -186,8 +186,18 @@ void Engine::apply_options() {
     // 8-neighbourhood relabels run one-tile blocks (a block's inner exchange
     // would need its corner nodes too).
     if (!variant || nbhd_ == 8 || !gr_block_supported(gbx, gby)) gbx = gby = 1;
-    const int wpr = opt.pr_waves_per_cu > 0 ? opt.pr_waves_per_cu : 16;
-    const int wgr = opt.gr_waves_per_cu > 0 ? opt.gr_waves_per_cu : 28;
+    // Push launches default to the wavefronts the kernel keeps resident per
+    // CU (its register use sets it: two to five per SIMD); more would wait for
+    // a slot with their contiguous run of the queue. Relabel launches default
+    // to 48 per CU, well beyond residency: their items are short and uneven,
+    // queued workgroups take the place of finished ones, and a queue longer
+    // than the grid would give workgroups runs of two or more items in a row
+    // (a 4096^2 image's relabel sweeps have up to 16 k items).
+    KParams pk = kp_;
+    pk.variant = variant;
+    pk.inner_iters = opt.inner_iters;
+    const int wpr = opt.pr_waves_per_cu > 0 ? opt.pr_waves_per_cu : (variant ? pr_resident_waves_per_cu(pk) : 16);
+    const int wgr = opt.gr_waves_per_cu > 0 ? opt.gr_waves_per_cu : 48;
     const int gpr = variant ? cu * wpr : cu * 4;
     const int ggr = variant ? std::max(cu, cu * wgr / (gbx * gby)) : cu * 8;
     const int grr = opt.gr_rounds > 0 ? opt.gr_rounds : 48;

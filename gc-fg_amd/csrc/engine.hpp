@@ -182,6 +182,9 @@ void launch_sched(const KParams &p, int parity, hipStream_t s, const LaunchEvent
 void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
 void launch_gr(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev);
 bool gr_block_supported(int bx, int by);
+// Wavefronts of the push kernel that the kernel parameters select (variant,
+// neighbourhood, inner iterations) resident on one CU at once.
+int pr_resident_waves_per_cu(const KParams &p);
 bool options_valid(const gcfg_options &o);  // sets the error message when not
 void launch_edit(const KParams &p, int slot, const int32_t *d_strokes, int n_strokes, int32_t delta,
                  hipStream_t s);

@@ -21,6 +21,7 @@
 //  * k_sched: per-instance state machine and ballot-compacted tile queues.
 #include <hip/hip_ext.h>
 
+#include <atomic>
 #include <type_traits>
 
 #include "engine.hpp"
@@ -292,9 +293,11 @@ enum EmitKind : int { EM_ALL = 0, EM_DIRTY = 1, EM_COLOUR = 2, EM_LIVE = 3, EM_D
 // entries); returns the number appended.
 // The scheduler runs one workgroup of SNT threads per instance slot, a pass
 // of emit_tiles covering SNT * EMIT_U tiles with one queue reservation:
-// 256 threads (2 k tiles a pass) while every instance has at most 2 k tiles,
-// 1024 (8 k a pass: a 4096^2 image in two) otherwise.
-constexpr int EMIT_U = 8;  // units per thread in one pass of emit_tiles
+// 256 threads (8 units each, 2 k tiles a pass) while every instance has at
+// most 2 k tiles, 1024 (16 units each, 16 k a pass: a 4096^2 image in one)
+// otherwise.
+template <int SNT>
+constexpr int emit_units() { return SNT >= 1024 ? 16 : 8; }
 constexpr int CAP_ROUNDS = 4;  // capped relabel rounds before an instance switches to exact ones
 // Push-queue item flag: the tile's capacities are still only in the inputs
 // (lazy build); the push pass reads them from there and writes them out.
@@ -303,6 +306,7 @@ constexpr int PR_FRESH = 1 << 30;
 template <int SNT>
 __device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, int kind, int colour, bool to_pr,
                                           int grop, int parity) {
+    constexpr int EMIT_U = emit_units<SNT>();
     __shared__ int wcount[SNT / 64], wbase[SNT / 64], s_qbase, s_total;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int32_t *queue = to_pr ? p.queue_pr : p.queue_gr;
@@ -3046,6 +3050,24 @@ template <int BX, int BY, int NB = 4>
 static void launch_grb(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
     launch_timed(k_grb<BX, BY, NB>, dim3(grid), dim3(64 * BX * BY), s, ev, p, parity, (const int32_t *)p.queue_gr,
                  (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
+}
+
+int pr_resident_waves_per_cu(const KParams &p) {
+    // One value per kernel (the code objects are the same on every device of
+    // this architecture); a racing first call computes the same number.
+    static std::atomic<int> cache[4];
+    const int key = (p.nbhd == 8 ? 2 : 0) + (p.inner_iters == 1 ? 1 : 0);
+    int v = cache[key].load(std::memory_order_relaxed);
+    if (v > 0) return v;
+    const void *k = p.nbhd == 8 ? (p.inner_iters == 1 ? (const void *)k_prw<true, 8> : (const void *)k_prw<false, 8>)
+                                : (p.inner_iters == 1 ? (const void *)k_prw<true, 4> : (const void *)k_prw<false, 4>);
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 64, 0) != hipSuccess || blocks <= 0) {
+        (void)hipGetLastError();
+        return 16;
+    }
+    cache[key].store(blocks, std::memory_order_relaxed);
+    return blocks;
 }
 
 bool gr_block_supported(int bx, int by) {

@@ -65,8 +65,8 @@ typedef struct gcfg_options {
                                 relabel), later ones are exact; an instance finished within capped
                                 rounds ends with a reachability round (and an exact round if that
                                 finds work left); 0: every relabel exact */
-    int32_t pr_waves_per_cu; /* push/relabel launch size in wavefronts per compute unit (0 = 16) */
-    int32_t gr_waves_per_cu; /* global-relabel launch size in wavefronts per compute unit (0 = 28) */
+    int32_t pr_waves_per_cu; /* push/relabel launch size in wavefronts per compute unit (0 = as many as stay resident) */
+    int32_t gr_waves_per_cu; /* global-relabel launch size in wavefronts per compute unit (0 = 48) */
     int32_t lazy_caps;       /* residual capacities of the wavefront kernels: 0 = chosen per build (lazy for
                                 large batches with few steps), 1 = written by the build, 2 = lazy (written
                                 before a tile's first push pass) */
