@@ -644,6 +644,8 @@ def pmc_traffic(args, opts, kernel_prefix, launches_per_solve):
              "--gr-cap", str(opts.gr_cap)]
     if opts.gr_waves_per_cu:
         child += ["--gr-waves", str(opts.gr_waves_per_cu)]
+    if opts.pr_waves_per_cu:
+        child += ["--pr-waves", str(opts.pr_waves_per_cu)]
     got = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="gcfg_pmc_")
@@ -874,7 +876,7 @@ def main():
     assert np.array_equal(flows_p, flows_b), "profiled solve differs"
     pbatch.close()
     if not args.no_pmc:
-        name = {"k_prw": "k_prw", "k_pr": "k_pr(", "k_grb": "k_grb<", "k_gr": "k_gr(", "k_buildw": "k_buildw(",
+        name = {"k_prw": "k_prw<", "k_pr": "k_pr(", "k_grb": "k_grb<", "k_gr": "k_gr(", "k_buildw": "k_buildw<",
                 "k_build": "k_build("}[dom]
         traffic, note = pmc_traffic(args, opts, name, dlaunch)
         if traffic:
