@@ -303,18 +303,25 @@ __device__ __forceinline__ int32_t trunc_sat(double v) {
 
 // Links pass: six planes for four pixels per lane. The weight of each
 // horizontal pair is computed once (left of pixel k = right of pixel k - 1).
+// The kernel-density mode is arithmetic-bound (seeds x pixels): its chunks are
+// split over DT_KDE_SPLIT workgroups for more wavefronts in flight.
+constexpr int DT_KDE_SPLIT = 2;
 template <bool KDE>
 __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
     extern __shared__ int32_t thr[];
     __shared__ uint32_t lut2[KDE ? 1 : DT_LUT / 2];  // u16 costs, two per word
-    __shared__ float g[KDE ? KDE_TAPS : 1];
+    // Kernel-density mode: the kernel value by squared distance,
+    // gs[s] = g[floor(sqrt(s))] for s < 101^2 (no square root per seed and pixel).
+    __shared__ float gs[KDE ? KDE_TAPS * KDE_TAPS : 1];
     DtJob J;
     int c;
-    dt_job(L, blockIdx.x, J, c);
+    constexpr int SPLIT = KDE ? DT_KDE_SPLIT : 1;
+    dt_job(L, blockIdx.x / SPLIT, J, c);
+    const int part = blockIdx.x % SPLIT;  // this workgroup's rounds of the chunk
     const int lambda = J.lambda;
     for (int i = threadIdx.x; i <= lambda; i += DT_NT) thr[i] = J.thr[i];
     if (KDE) {
-        for (int i = threadIdx.x; i < KDE_TAPS; i += DT_NT) g[i] = J.gauss[i];
+        for (int i = threadIdx.x; i < KDE_TAPS * KDE_TAPS; i += DT_NT) gs[i] = J.gauss[isqrt24(i)];
     } else {
         const uint32_t *src = reinterpret_cast<const uint32_t *>(J.tlut);
         for (int i = threadIdx.x; i < DT_LUT / 2; i += DT_NT) lut2[i] = src[i];
@@ -324,8 +331,9 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
     const int64_t total = 3 * (int64_t)n;
     // Loads of two rounds are in flight before either is used (the stores of
     // the six planes then overlap the next pair's loads).
+    static_assert(DT_ROUNDS % (2 * SPLIT) == 0, "whole pairs of rounds per workgroup");
 #pragma unroll
-    for (int r2 = 0; r2 < DT_ROUNDS; r2 += 2) {
+    for (int r2 = part * (DT_ROUNDS / SPLIT); r2 < (part + 1) * (DT_ROUNDS / SPLIT); r2 += 2) {
     uint32_t ww[2][5], aa[2][3], bb[2][3];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -334,6 +342,38 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
             load_words<5>(J.img, 3 * (int64_t)i0 - 4, total, ww[h]);  // p(-1) .. p(4)
             load_px12(J.img, 3 * ((int64_t)i0 - W), total, aa[h]);     // above
             load_px12(J.img, 3 * ((int64_t)i0 + W), total, bb[h]);     // below
+        } else {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) ww[h][k] = 0;  // read by the density pass only
+        }
+    }
+    // Kernel density of the eight pixels in one pass over the seed list (its
+    // entries are uniform: scalar loads), eight independent f64 sums, each in
+    // list order.
+    double dens[2][4];
+    if (KDE) {
+        int px[2][4][3];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const Rgb q = px_at(ww[h], 4 + 3 * k);
+                px[h][k][0] = q.c0;
+                px[h][k][1] = q.c1;
+                px[h][k][2] = q.c2;
+                dens[h][k] = 0.0;
+            }
+        for (int j = 0; j < J.kde_n; ++j) {
+            const uint32_t cj = J.kde[j];
+            const int s0 = (int)(cj >> 16), s1 = (int)((cj >> 8) & 255u), s2 = (int)(cj & 255u);
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int d0 = px[h][k][0] - s0, d1 = px[h][k][1] - s1, d2 = px[h][k][2] - s2;
+                    const int sq2 = d0 * d0 + d1 * d1 + d2 * d2;
+                    if (sq2 < KDE_TAPS * KDE_TAPS) dens[h][k] += (double)gs[sq2];
+                }
         }
     }
 #pragma unroll
@@ -352,7 +392,7 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
         for (int k = 0; k < 4; ++k) {
             const Rgb pk = px_at(w, 4 + 3 * k);
             if (KDE) {
-                const float pf = (float)kde_density(pk, J.kde, J.kde_n, g);
+                const float pf = (float)dens[h][k];
                 vp[k] = trunc_sat(300000.0 * (double)pf);
                 vn[k] = trunc_sat(300000.0 * (double)(1.0f - pf));
             } else {
@@ -393,7 +433,7 @@ void dt_launch(const DtLaunch &L, int total_chunks, hipStream_t s) {
     hipLaunchKernelGGL(k_dt_table, dim3(jobs), dim3(DT_NT), 0, s, L);
     const size_t lds = (size_t)(L.lds_lambda + 1) * sizeof(int32_t);
     if (kde)
-        hipLaunchKernelGGL(k_dt_links<true>, dim3(total_chunks), dim3(DT_NT), lds, s, L);
+        hipLaunchKernelGGL(k_dt_links<true>, dim3(total_chunks * DT_KDE_SPLIT), dim3(DT_NT), lds, s, L);
     else
         hipLaunchKernelGGL(k_dt_links<false>, dim3(total_chunks), dim3(DT_NT), lds, s, L);
 }
