@@ -71,6 +71,18 @@ __device__ __forceinline__ void load_px12(const uint8_t *img, int64_t o, int64_t
     r[2] = __builtin_amdgcn_alignbyte(w[3], w[2], s);
 }
 
+// Work unit of workgroup b among n: units are dealt to the 8 XCDs (one L2
+// each) in contiguous eighths, since workgroup b runs on XCD b % 8. A chunk
+// reads the rows above and below it, which the neighbouring chunks own: kept
+// on one XCD and in flight together, those reads hit its L2 instead of
+// fetching every row twice (the stats pass's loads take half the time).
+// Units past n (the grid is rounded up to whole eighths) return at once.
+__device__ __forceinline__ int xcd_unit(int b, int n) {
+    const int per = (n + NUM_XCD - 1) / NUM_XCD;
+    return (b % NUM_XCD) * per + b / NUM_XCD;
+}
+__host__ __device__ inline int xcd_grid(int n) { return (n + NUM_XCD - 1) / NUM_XCD * NUM_XCD; }
+
 // The job of a chunk: the single job, or the job table through the chunk map.
 __device__ __forceinline__ void dt_job(const DtLaunch &L, int b, DtJob &J, int &c) {
     if (L.njobs == 0) {
@@ -93,14 +105,29 @@ __device__ __forceinline__ int nlink_weight(int s, const int32_t *thr, int lambd
     return lo;
 }
 
+// The same weight from the closed form's f32 estimate k = rint(lambda *
+// exp(-s / (2 beta))), confirmed against two thresholds: k is the weight
+// exactly when thr[k] >= s > thr[k + 1] (thr[0] = +inf; past lambda, -inf).
+// The f32 estimate misses only next to a rounding boundary of the f64
+// formula; those take the binary search. Two independent LDS reads instead of
+// a chain of log2(lambda) dependent ones.
+__device__ __forceinline__ int nlink_weight_est(int s, const int32_t *thr, int lambda, float neg_inv_2beta) {
+    int k = (int)__builtin_rintf((float)lambda * __expf((float)s * neg_inv_2beta));
+    k = min(max(k, 0), lambda);
+    const int a = thr[k], b = k < lambda ? thr[k + 1] : -1;
+    return (a >= s && b < s) ? k : nlink_weight(s, thr, lambda);
+}
+
 // Stats pass: seed histograms and the chunk's sum of squared colour
 // differences over right and down pairs (each pair once).
 __global__ __launch_bounds__(DT_NT) void k_dt_stats(DtLaunch L) {
     __shared__ int hist[DT_HIST];
     __shared__ unsigned long long red[DT_NT / 64];
+    const int unit = xcd_unit(blockIdx.x, L.units);
+    if (unit >= L.units) return;
     DtJob J;
     int c;
-    dt_job(L, blockIdx.x, J, c);
+    dt_job(L, unit, J, c);
     for (int i = threadIdx.x; i < DT_HIST; i += DT_NT) hist[i] = 0;
     __syncthreads();
     const int W = J.W, H = J.H, n = J.W * J.H;
@@ -176,15 +203,22 @@ __device__ __forceinline__ long long nlink_exact(int s, int lambda, double beta)
     return beta > 0.0 ? llround((double)lambda * exp(-(double)s / (2.0 * beta))) : (long long)lambda;
 }
 
-// Table pass (one workgroup per image): reduce the partials, keep the
-// statistics, build the t-link costs of the 16^3 colour bins of both classes
-// and the n-link thresholds thr[k] = largest squared difference s with
-// w(s) >= k (thr[0] = +inf, -1 where no s qualifies).
+// Table pass (DT_TABLE_PARTS workgroups per image): reduce the partials,
+// keep the statistics, build the t-link costs of the 16^3 colour bins of both
+// classes and the n-link thresholds thr[k] = largest squared difference s
+// with w(s) >= k (thr[0] = +inf, -1 where no s qualifies). Every part reduces
+// the partials and histograms itself and evaluates its share of the costs and
+// thresholds (f64 logarithms and exponentials: one image's table would keep a
+// single workgroup busy for about 15 us); part 0 writes the statistics. The
+// histogram accumulator is cleared by the links pass, after every part has
+// read it.
+constexpr int DT_TABLE_PARTS = 8;
 __global__ __launch_bounds__(DT_NT) void k_dt_table(DtLaunch L) {
     __shared__ int hist[DT_HIST];
     __shared__ double rr[2][3][16];
     __shared__ unsigned long long red[DT_NT / 64];
-    const DtJob J = L.njobs ? L.jobs[blockIdx.x] : L.one;
+    const int part = blockIdx.x % DT_TABLE_PARTS;
+    const DtJob J = L.njobs ? L.jobs[blockIdx.x / DT_TABLE_PARTS] : L.one;
     const int nch = dt_chunks(J.W, J.H);
     unsigned long long bs = 0;
     for (int k = threadIdx.x; k < nch; k += DT_NT) bs += J.beta_part[k];
@@ -194,19 +228,18 @@ __global__ __launch_bounds__(DT_NT) void k_dt_table(DtLaunch L) {
         bs += ((unsigned long long)hi << 32) | lo;
     }
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bs;
-    for (int i = threadIdx.x; i < DT_HIST; i += DT_NT) {
-        const int acc = J.hist_acc[i];
-        J.hist_acc[i] = 0;  // ready for the next compute
-        hist[i] = J.frozen ? (i < 96 ? (&J.frozen->hist[0][0][0])[i] : J.frozen->counts[i - 96]) : acc;
-    }
+    for (int i = threadIdx.x; i < DT_HIST; i += DT_NT)
+        hist[i] = J.frozen ? (i < 96 ? (&J.frozen->hist[0][0][0])[i] : J.frozen->counts[i - 96]) : J.hist_acc[i];
     __syncthreads();
     unsigned long long tot = 0;
     for (int k = 0; k < DT_NT / 64; ++k) tot += red[k];
-    for (int i = threadIdx.x; i < DT_HIST; i += DT_NT) {
-        if (i < 96) (&J.stats->hist[0][0][0])[i] = hist[i];
-        else J.stats->counts[i - 96] = hist[i];
+    if (part == 0) {
+        for (int i = threadIdx.x; i < DT_HIST; i += DT_NT) {
+            if (i < 96) (&J.stats->hist[0][0][0])[i] = hist[i];
+            else J.stats->counts[i - 96] = hist[i];
+        }
+        if (threadIdx.x == 0) J.stats->beta_sum = tot;
     }
-    if (threadIdx.x == 0) J.stats->beta_sum = tot;
     if (threadIdx.x < 96) {
         const int k = threadIdx.x / 48;
         rr[k][(threadIdx.x / 16) % 3][threadIdx.x % 16] =
@@ -214,7 +247,8 @@ __global__ __launch_bounds__(DT_NT) void k_dt_table(DtLaunch L) {
     }
     __syncthreads();
     if (J.model == GCFG_DT_MODEL_HISTOGRAM) {
-        for (int e = threadIdx.x; e < DT_LUT; e += DT_NT) {
+        constexpr int PER = DT_LUT / DT_TABLE_PARTS;
+        for (int e = part * PER + threadIdx.x; e < (part + 1) * PER; e += DT_NT) {
             const int k = e >> 12, idx = e & 4095;
             double p = 1.0;
             p *= rr[k][0][(idx >> 8) & 15];
@@ -226,7 +260,7 @@ __global__ __launch_bounds__(DT_NT) void k_dt_table(DtLaunch L) {
     const int64_t pairs = (int64_t)(J.W - 1) * J.H + (int64_t)J.W * (J.H - 1);
     const double beta = pairs > 0 ? (double)tot / (double)pairs : 0.0;
     const int lambda = J.lambda;
-    for (int k = threadIdx.x; k <= lambda; k += DT_NT) {
+    for (int k = part * DT_NT + threadIdx.x; k <= lambda; k += DT_NT * DT_TABLE_PARTS) {
         int t;
         if (k == 0) {
             t = 0x7fffffff;
@@ -316,8 +350,14 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
     DtJob J;
     int c;
     constexpr int SPLIT = KDE ? DT_KDE_SPLIT : 1;
-    dt_job(L, blockIdx.x / SPLIT, J, c);
-    const int part = blockIdx.x % SPLIT;  // this workgroup's rounds of the chunk
+    const int unit = xcd_unit(blockIdx.x, L.units);
+    if (unit >= L.units) return;
+    dt_job(L, unit / SPLIT, J, c);
+    const int part = unit % SPLIT;  // this workgroup's rounds of the chunk
+    // The histogram accumulator is read by every table part; cleared here,
+    // after the table pass and before the next compute's stats pass.
+    if (c == 0 && part == 0)
+        for (int i = threadIdx.x; i < DT_HIST; i += DT_NT) J.hist_acc[i] = 0;
     const int lambda = J.lambda;
     for (int i = threadIdx.x; i <= lambda; i += DT_NT) thr[i] = J.thr[i];
     if (KDE) {
@@ -329,6 +369,11 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
     __syncthreads();
     const int W = J.W, H = J.H, n = J.W * J.H;
     const int64_t total = 3 * (int64_t)n;
+    // beta as the table pass computed it (-1 / (2 beta) for the estimates; 0
+    // when beta is 0: every weight is lambda).
+    const int64_t pairs = (int64_t)(W - 1) * H + (int64_t)W * (H - 1);
+    const double beta = pairs > 0 ? (double)J.stats->beta_sum / (double)pairs : 0.0;
+    const float nib = beta > 0.0 ? (float)(-0.5 / beta) : 0.0f;
     // Loads of two rounds are in flight before either is used (the stores of
     // the six planes then overlap the next pair's loads).
     static_assert(DT_ROUNDS % (2 * SPLIT) == 0, "whole pairs of rounds per workgroup");
@@ -385,7 +430,8 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
         const uint32_t(&b)[3] = bb[h];
         int hw[5];  // weight of the pair (k - 1, k)
 #pragma unroll
-        for (int k = 0; k < 5; ++k) hw[k] = nlink_weight(sq(px_at(w, 1 + 3 * k), px_at(w, 4 + 3 * k)), thr, lambda);
+        for (int k = 0; k < 5; ++k)
+            hw[k] = nlink_weight_est(sq(px_at(w, 1 + 3 * k), px_at(w, 4 + 3 * k)), thr, lambda, nib);
         int vp[4], vn[4], vu[4], vd[4], vl[4], vr[4];
         int x = i0 % W, y = i0 / W;
 #pragma unroll
@@ -403,8 +449,8 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
             }
             vl[k] = x > 0 ? hw[k] : 0;
             vr[k] = x + 1 < W ? hw[k + 1] : 0;
-            vu[k] = y > 0 ? nlink_weight(sq(px_at(a, 3 * k), pk), thr, lambda) : 0;
-            vd[k] = y + 1 < H ? nlink_weight(sq(pk, px_at(b, 3 * k)), thr, lambda) : 0;
+            vu[k] = y > 0 ? nlink_weight_est(sq(px_at(a, 3 * k), pk), thr, lambda, nib) : 0;
+            vd[k] = y + 1 < H ? nlink_weight_est(sq(pk, px_at(b, 3 * k)), thr, lambda, nib) : 0;
             if (++x == W) {
                 x = 0;
                 ++y;
@@ -429,13 +475,17 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
 void dt_launch(const DtLaunch &L, int total_chunks, hipStream_t s) {
     const int jobs = L.njobs ? L.njobs : 1;
     const bool kde = L.njobs == 0 && L.one.model == GCFG_DT_MODEL_KDE;
-    hipLaunchKernelGGL(k_dt_stats, dim3(total_chunks), dim3(DT_NT), 0, s, L);
-    hipLaunchKernelGGL(k_dt_table, dim3(jobs), dim3(DT_NT), 0, s, L);
-    const size_t lds = (size_t)(L.lds_lambda + 1) * sizeof(int32_t);
-    if (kde)
-        hipLaunchKernelGGL(k_dt_links<true>, dim3(total_chunks * DT_KDE_SPLIT), dim3(DT_NT), lds, s, L);
-    else
-        hipLaunchKernelGGL(k_dt_links<false>, dim3(total_chunks), dim3(DT_NT), lds, s, L);
+    DtLaunch P = L;
+    P.units = total_chunks;
+    hipLaunchKernelGGL(k_dt_stats, dim3(xcd_grid(P.units)), dim3(DT_NT), 0, s, P);
+    hipLaunchKernelGGL(k_dt_table, dim3(jobs * DT_TABLE_PARTS), dim3(DT_NT), 0, s, P);
+    const size_t lds = (size_t)(P.lds_lambda + 1) * sizeof(int32_t);
+    if (kde) {
+        P.units = total_chunks * DT_KDE_SPLIT;
+        hipLaunchKernelGGL(k_dt_links<true>, dim3(xcd_grid(P.units)), dim3(DT_NT), lds, s, P);
+    } else {
+        hipLaunchKernelGGL(k_dt_links<false>, dim3(xcd_grid(P.units)), dim3(DT_NT), lds, s, P);
+    }
 }
 
 __device__ __forceinline__ int sqdiff(const uint8_t *a, const uint8_t *b) {

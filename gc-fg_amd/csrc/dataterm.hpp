@@ -88,6 +88,8 @@ struct DtLaunch {
     const int32_t *cmap;
     int32_t njobs;
     int32_t lds_lambda;  // largest lambda of the launch (dynamic LDS of the links pass)
+    int32_t units;       // work units of the launching pass (set by dt_launch)
+    int32_t pad_;
 };
 
 __host__ __device__ inline int dt_chunks(int W, int H) { return (int)(((int64_t)W * H + DT_CHUNK - 1) / DT_CHUNK); }
