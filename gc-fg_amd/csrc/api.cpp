@@ -345,6 +345,8 @@ int gcfg_graph_destroy(gcfg_graph *g) {
 }
 
 // ---------------------------------------------------------------------------
+static int alloc_label_bits(gcfg_batch *b);
+
 int gcfg_batch_create(int32_t count, const gcfg_instance *inst, const gcfg_options *opt, gcfg_batch **out) {
     if (!out || count <= 0 || !inst) {
         set_error("gcfg_batch_create: invalid arguments");
@@ -383,6 +385,7 @@ int gcfg_batch_create(int32_t count, const gcfg_instance *inst, const gcfg_optio
         set_error("gcfg_batch_create: label arena allocation failed");
         rc = GCFG_ERR_NOMEM;
     }
+    if (rc == GCFG_OK) rc = alloc_label_bits(b);
     for (int i = 0; rc == GCFG_OK && i < count; ++i) {
         const gcfg_instance &in = inst[i];
         const int s = b->eng.add_instance(in.width, in.height, in.d_pos, in.d_neg, in.d_up, in.d_down, in.d_left,
@@ -392,6 +395,8 @@ int gcfg_batch_create(int32_t count, const gcfg_instance *inst, const gcfg_optio
     }
     if (rc != GCFG_OK) {
         if (b->d_labels) (void)hipFree(b->d_labels);
+        if (b->d_bits) (void)hipFree(b->d_bits);
+        if (b->h_bits) (void)hipHostFree(b->h_bits);
         delete b;
         return rc;
     }
@@ -491,27 +496,33 @@ int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_sta
     return GCFG_OK;
 }
 
+// Device and pinned host buffers of a batch's packed labels (1/32 of the
+// int32 label arena each), allocated with the batch so that no solve pays
+// for them.
+static int alloc_label_bits(gcfg_batch *b) {
+    size_t words = 0;
+    b->bits_off.clear();
+    for (const gcfg_instance &in : b->inst) {
+        b->bits_off.push_back(words);
+        words += Engine::label_words(in.width, in.height);
+    }
+    if (hipMalloc((void **)&b->d_bits, std::max<size_t>(words, 1) * sizeof(uint64_t)) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_bits, std::max<size_t>(words, 1) * sizeof(uint64_t), hipHostMallocDefault) !=
+            hipSuccess) {
+        if (b->d_bits) (void)hipFree(b->d_bits);
+        b->d_bits = nullptr;
+        b->h_bits = nullptr;
+        set_error("packed label allocation failed");
+        return GCFG_ERR_NOMEM;
+    }
+    b->bits_words = words;
+    return GCFG_OK;
+}
+
 int gcfg_batch_solve_bits(gcfg_batch *b, const uint64_t **h_bits, int64_t *flows, gcfg_stats *stats) {
     if (!b) return GCFG_ERR_INVALID;
     const double t0 = now_ms();
-    if (!b->d_bits) {
-        size_t words = 0;
-        b->bits_off.clear();
-        for (const gcfg_instance &in : b->inst) {
-            b->bits_off.push_back(words);
-            words += Engine::label_words(in.width, in.height);
-        }
-        if (hipMalloc((void **)&b->d_bits, std::max<size_t>(words, 1) * sizeof(uint64_t)) != hipSuccess ||
-            hipHostMalloc((void **)&b->h_bits, std::max<size_t>(words, 1) * sizeof(uint64_t),
-                          hipHostMallocDefault) != hipSuccess) {
-            if (b->d_bits) (void)hipFree(b->d_bits);
-            b->d_bits = nullptr;
-            b->h_bits = nullptr;
-            set_error("gcfg_batch_solve_bits: packed label allocation failed");
-            return GCFG_ERR_NOMEM;
-        }
-        b->bits_words = words;
-    }
+    if (!b->d_bits) return GCFG_ERR_STATE;
     const int rc = batch_run(b, nullptr, true, flows, stats);
     if (rc != GCFG_OK) return rc;
     if (h_bits)

@@ -496,12 +496,14 @@ void Engine::drop_graphs() {
     graphs_.clear();
 }
 
+hipGraphExec_t Engine::cached_graph(int steps, int parity) const {
+    for (const StepGraph &g : graphs_)
+        if (g.steps == steps && g.parity == parity && g.wide == kp_.sched_wide) return g.exec;
+    return nullptr;
+}
+
 int Engine::step_graph(int steps, int parity, hipGraphExec_t *out) {
-    for (StepGraph &g : graphs_)
-        if (g.steps == steps && g.parity == parity && g.wide == kp_.sched_wide) {
-            *out = g.exec;
-            return GCFG_OK;
-        }
+    if ((*out = cached_graph(steps, parity)) != nullptr) return GCFG_OK;
     if (graphs_.size() >= 16) {  // keep a handful of chunk lengths, parities and widths
         (void)hipGraphExecDestroy(graphs_.front().exec);
         (void)hipGraphDestroy(graphs_.front().graph);
@@ -556,10 +558,20 @@ int Engine::launch_steps(int n) {
     }
     if (opt.use_graph) {
         const int steps = std::max(n, 1);
-        hipGraphExec_t exec = nullptr;
-        int rc = step_graph(steps, parity_, &exec);
-        if (rc != GCFG_OK) return rc;
-        GCFG_HIP(hipGraphLaunch(exec, stream_));
+        hipGraphExec_t exec = cached_graph(steps, parity_);
+        if (exec) {
+            GCFG_HIP(hipGraphLaunch(exec, stream_));
+        } else {
+            // A chunk length not captured yet: launch it directly, then
+            // capture, instantiate and upload its graph while the GPU runs
+            // these steps (instantiating and the first launch of a new graph
+            // cost about 0.2 ms each on the host, the whole of a short solve).
+            for (int i = 0; i < steps; ++i) launch_step(kp_, grid_pr_, grid_gr_, (parity_ + i) & 1, stream_);
+            GCFG_HIP(hipGetLastError());
+            int rc = step_graph(steps, parity_, &exec);
+            if (rc != GCFG_OK) return rc;
+            GCFG_HIP(hipGraphUpload(exec, stream_));
+        }
         parity_ ^= steps & 1;
         steps_run += steps;
     } else {

@@ -360,6 +360,7 @@ class Engine {
     std::vector<StepGraph> graphs_;
     void drop_graphs();
     int step_graph(int steps, int parity, hipGraphExec_t *out);
+    hipGraphExec_t cached_graph(int steps, int parity) const;
     int parity_ = 0;  // parity of the next step to launch
     std::vector<void *> allocs_;
     std::vector<hipEvent_t> events_;  // 6 per profiled step: start/stop of sched, pr, gr
