@@ -45,6 +45,44 @@ def test_histograms_and_tlinks_closed_form(ora):
     assert out["pos"][10, 5] > out["neg"][10, 5]
 
 
+def test_model_image_histograms_frame_beta(ora):
+    """A video session's later frame: the seed histograms come from the model
+    image (the session's first frame), beta and the n-links from the frame.
+    Frame and model are built so that every quantity differs between the two
+    readings, and each expected value is a closed form."""
+    W, H = 32, 8
+    A, B, C = (200, 40, 40), (40, 200, 100), (120, 120, 120)  # bins (12,2,2), (2,12,6), (7,7,7)
+    left = np.zeros((H, W), bool)
+    left[:, :16] = True
+    model = two_colour_image(W, H, A, B, left)  # model: A | B
+    frame = two_colour_image(W, H, C, A, left)  # frame: C | A
+    seeds = np.zeros((H, W), np.uint8)
+    seeds[:, 2] = 1  # model colour A (the frame has C there)
+    seeds[:, 20] = 2  # model colour B (the frame has A there)
+    out = ora.dataterm(W, H, frame, seeds, 50, model_rgb=model)
+    n = 8
+    assert list(out["counts"]) == [n, n]
+    hist = out["hist"]
+    for c in range(3):
+        assert hist[0, c, A[c] >> 4] == n and hist[0, c].sum() == n
+        assert hist[1, c, B[c] >> 4] == n and hist[1, c].sum() == n
+    # beta from the frame: only the 8 pairs across x = 15 | 16 differ, by |C - A|^2.
+    pairs = (W - 1) * H + W * (H - 1)
+    sq = sum((C[c] - A[c]) ** 2 for c in range(3))
+    assert out["beta_sum"] == 8 * sq and out["pairs"] == pairs
+    beta = 8 * sq / pairs
+    # C pixels (left half of the frame): no seed colour shares a bin with C.
+    assert out["pos"][3, 5] == round(64 * 3 * math.log(n + 16))
+    assert out["neg"][3, 5] == round(64 * 3 * math.log(n + 16))
+    # A pixels (right half): the foreground model's colour.
+    assert out["neg"][3, 25] == round(-64 * 3 * math.log((n + 1) / (n + 16)))
+    assert out["pos"][3, 25] == round(64 * 3 * math.log(n + 16))
+    # n-links: lambda inside each half, the edge weight from the frame's beta.
+    w_edge = round(50 * math.exp(-sq / (2 * beta)))
+    assert out["right"][3, 15] == w_edge and out["left"][3, 16] == w_edge
+    assert out["right"][3, 5] == 50 and out["down"][3, 25] == 50
+
+
 def test_unseeded_class_is_uniform(ora):
     W, H = 12, 10
     img = np.full((H, W, 3), 77, np.uint8)
