@@ -14,13 +14,18 @@
 //     added to the image's accumulator only by chunks that hold seeds) and the
 //     chunk's exact sum of squared colour differences over right and down
 //     pairs (one partial per chunk, no same-address atomics);
-//   * table: one workgroup per image; reduces the partials, then evaluates the
-//     t-link costs of the 2 x 16^3 colour bins and the lambda n-link
-//     thresholds in f64, the same expressions the oracle evaluates per pixel;
+//   * table: eight workgroups per image; each reduces the partials, then
+//     evaluates its share of the t-link costs of the 2 x 16^3 colour bins and
+//     of the lambda n-link thresholds in f64, the same expressions the oracle
+//     evaluates per pixel;
 //   * links: one workgroup per chunk, four pixels per lane (aligned dword
 //     loads of the interleaved RGB, the rows above and below through a
-//     shifted window), every pair's weight computed once, six int32 planes
-//     written with 16-byte stores.
+//     shifted window), each horizontal pair's weight computed once (vertical
+//     pairs once from each side), n-link weights from an f32 estimate
+//     confirmed against the thresholds, six int32 planes written with 16-byte
+//     stores.
+// Chunks go to the XCDs in contiguous eighths, so the rows a chunk reads from
+// its neighbours are in its L2.
 //
 // The reference's own colour model is kept as a second mode (GCFG_DT_MODEL_KDE,
 // DataTermKernels.cu:19-31,61-83,116-150; DataTerm.cpp:97-137): a Gaussian
