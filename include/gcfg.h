@@ -48,7 +48,7 @@ enum { GCFG_DT_MODEL_HISTOGRAM = 0, GCFG_DT_MODEL_KDE = 1 };
 
 typedef struct gcfg_options {
     int32_t sweeps_per_gr;   /* colour-pass pairs of tile push-relabel between global relabels */
-    int32_t inner_iters;     /* max push/relabel iterations per tile pass (default 8) */
+    int32_t inner_iters;     /* max push/relabel iterations per tile pass (default 8; twice as many, at most 64, in exact rounds) */
     int32_t steps_per_poll;  /* scheduler steps launched between two host polls */
     int32_t use_graph;       /* capture a chunk of steps in a hipGraph and replay it */
     int32_t max_steps;       /* 0 = unbounded; else GCFG_ERR_NOT_CONVERGED past this */
