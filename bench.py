@@ -62,7 +62,6 @@ GR_RELAX_BYTES_PER_NODE = 9  # relaxation sweep: mask 1 + distance 4 read, dista
 LABEL_BYTES_PER_NODE = 4    # label written by every sweep of an exact round and by FIN
 COMPULSORY_BYTES_PER_PX = 28  # 6 int32 inputs read + int32 label written
 DATATERM_BYTES_PER_PX = 30  # RGB read by two passes + six int32 planes written
-CHILD_SOLVES = 3  # solves of the counter child run: --steps 1 --warmup 0, then two event-timed solves
 BUILD_IN_BYTES_PER_PX = 24  # pos, neg and four n-link planes read by the build
 BUILD_OUT_BYTES_PER_NODE = 9  # excess 4 + distance 4 + mask 1 written per arena node (capacities stay lazy)
 BUILD_OUT_BYTES_PER_TILE = 640  # the tile's inflow slots cleared (160 int32)
@@ -624,10 +623,12 @@ def pmc_traffic(args, opts, kernel_prefix, launches_per_solve):
     """HBM bytes per launch of the dominant kernel from two rocprofv3 counter
     runs of this script (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch), each
     scaled by what the same counter reads for a 1 GiB device-to-device copy
-    run in the same process. The child solves the batch CHILD_SOLVES times
-    (one timed solve, two event-timed ones); a first solve may launch a few
-    idle steps whose dispatches exit at once, so bytes per launch are the
-    summed bytes over every dispatch / (solves x launches of one solve).
+    run in the same process. The child solves the batch several times (the
+    timed solves of each label format and the event-timed ones); every solve
+    arms its instances with one k_arm dispatch, so the child's solve count is
+    the number of k_arm dispatches. A first solve may launch a few idle steps
+    whose dispatches exit at once, so bytes per launch are the summed bytes
+    over every dispatch / (solves x launches of one solve).
     Returns (traffic dict or None, note)."""
     import shutil
     import sqlite3
@@ -663,15 +664,16 @@ def pmc_traffic(args, opts, kernel_prefix, launches_per_solve):
         shutil.rmtree(d, ignore_errors=True)
         calib = max((v for n, v in rows if "copyBuffer" in n), default=0.0)
         vals = [v for n, v in rows if kernel_prefix in n]
-        if calib <= 0 or not vals:
-            return None, f"no {counter} samples for the copy or {kernel_prefix}"
+        solves = sum(1 for n, _ in rows if "k_arm" in n)
+        if calib <= 0 or not vals or not solves:
+            return None, f"no {counter} samples for the copy, k_arm or {kernel_prefix}"
         scale = (1 << 20) / calib  # the copy moves 1 GiB = 2^20 KiB each way
-        per_launch = sum(vals) / (CHILD_SOLVES * max(1, launches_per_solve))
-        got[counter] = (per_launch * 1024 * scale, scale, len(vals))
-    fetch, fscale, n = got["FETCH_SIZE"]
-    write, wscale, _ = got["WRITE_SIZE"]
+        per_launch = sum(vals) / (solves * max(1, launches_per_solve))
+        got[counter] = (per_launch * 1024 * scale, scale, len(vals), solves)
+    fetch, fscale, n, solves = got["FETCH_SIZE"]
+    write, wscale, _, _ = got["WRITE_SIZE"]
     return dict(bytes=int(fetch + write), fetch_bytes=int(fetch), write_bytes=int(write),
-                fetch_scale=round(fscale, 4), write_scale=round(wscale, 4), dispatches=n), "ok"
+                fetch_scale=round(fscale, 4), write_scale=round(wscale, 4), dispatches=n, solves=solves), "ok"
 
 
 def unseen_first_solve(args, opts):

@@ -398,7 +398,17 @@ __device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, i
                 wbase[w] = tot;
                 tot += wcount[w];
             }
-            s_qbase = tot ? atomicAdd(qlen, tot) : 0;
+            if (p.n_slots == 1) {
+                // One instance slot: this workgroup is the step's only writer
+                // of the queue, which the previous step's scheduler left
+                // empty, and no queue gets items from two emissions of one
+                // step; the reservation is a plain store, not an atomic
+                // round trip.
+                s_qbase = s_total;
+                if (tot) *qlen = s_total + tot;
+            } else {
+                s_qbase = tot ? atomicAdd(qlen, tot) : 0;
+            }
             s_total += tot;
         }
         __syncthreads();
@@ -473,13 +483,15 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     const int s = blockIdx.x;
     if (s == 0 && threadIdx.x == 0) p.ctl->q[parity ^ 1] = QCtl{0, 0, 0, 0};
     InstState &st = p.state[s];
+    // The descriptor is loaded with the state, not after the phase test, so
+    // the two loads share one round trip.
+    const InstDesc D = p.desc[s];
     const int phase = st.phase;
     if (phase == PH_IDLE || phase == PH_DONE) return;
     const int err = st.err, sweeps = st.sweeps, cur_cap = st.cap, rounds0 = st.gr_rounds;
     int pr_iter = st.pr_iter;
     __syncthreads();  // every thread has read the state before thread 0 writes it
     if (threadIdx.x == 0) st.steps++;
-    const InstDesc D = p.desc[s];
     if ((phase == PH_START && err) || phase == PH_FIN) {
         // flow = sum of pos - sum of max(excess, 0) over the interior, from
         // the per-tile sums of the build and the FIN sweep.
@@ -565,6 +577,18 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
         st.pr_iter = pr_iter + 1;
         st.pr_tiles += n;
     }
+    // The last colour pass before an exact round: that round's REINIT sweep
+    // is queued now and runs right after this push pass, in the same step
+    // (the relabel launch follows the push launch). Its tiles are every block
+    // the previous round found a reachable node in (`live`, which push passes
+    // do not write), and it reads the masks this pass leaves, so it is the
+    // sweep the next step would have queued, one step sooner. A capped round
+    // cannot start early: its tiles depend on the stale flags this pass sets.
+    // Only after an exact round (cur_cap DINF): the push pass reads the
+    // instance's cap, which starting the round sets, so it keeps the cap of
+    // the round it follows.
+    if (reinit && cur_cap >= DINF && cap >= DINF && pr_iter + 1 >= ncol * sweeps)
+        start_global_relabel<SNT>(p, D, st, parity, reinit, cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -1880,7 +1904,9 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
     // Each workgroup takes a contiguous run of the queue: consecutive items
     // are usually neighbouring tiles of one instance, so their queue words,
     // tile records and descriptor hit the scalar cache, and their halo rows
-    // are each other's tile rows.
+    // are each other's tile rows. (Dealing positions out from a counter
+    // instead balanced the runs but serialised thousands of same-address
+    // atomics per launch: twice the time.)
     const int per = (qlen + (int)gridDim.x - 1) / (int)gridDim.x;
     const int q_end = min(qlen, ((int)blockIdx.x + 1) * per);
     for (int q = (int)blockIdx.x * per; q < q_end; ++q) {
