@@ -1586,6 +1586,7 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, Masks &m
                 rch |= 1u << y;
             }
         }
+        uint32_t rrow = 0;  // rows the row scans changed
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
             int hl = shr1(d[y]), hr = shl1(d[y]);
@@ -1605,11 +1606,18 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, Masks &m
             nd = min(A2, dmin_sat(hsr + B2, DINF));
             if (nd < d[y]) {
                 d[y] = nd;
-                rch |= 1u << y;
+                rrow |= 1u << y;
             }
         }
+        rch |= rrow;
         chg |= rch;
-        if (!__any(rch != 0)) return true;
+        // 4-neighbourhood: a column's shortest paths are monotone, so the
+        // downward and upward sweeps leave every column at its own fixed
+        // point; if the row scans then change nothing, every row is at its
+        // fixed point too and the tile has converged -- no confirming round.
+        // (Diagonal arcs let a path go up and down again within the sweeps,
+        // so the 8-neighbourhood waits for a round without any change.)
+        if (!__any((NB == 4 ? rrow : rch) != 0)) return true;
     }
     return false;
 }
