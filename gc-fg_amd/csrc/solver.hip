@@ -2420,22 +2420,32 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             // image with heavy-tailed t-links 7.1 -> 6.6 ms; instances that
             // finish within the capped rounds never run one).
             const int iters = capv >= DINF ? min(2 * p.inner_iters, 64) : p.inner_iters;
-            while (it < iters) {
-                ++it;
+            // One iteration: pushes with the heights of its start, then
+            // relabels. 4-neighbourhood: every other iteration pushes from the
+            // last row up, so flow pushed into the row above is pushed on in
+            // the same iteration, as flow into the row below is on the way
+            // down (either order is a legal sequence of pushes). Returns
+            // whether a node is still active.
+            auto iteration = [&](auto upward) -> bool {
                 if (NB == 8) {
                     pl = hUL;
                     pr = hUR;
                     cl = shl_row(0);
                     cr = shr_row(0);
                 }
+                if (NB == 4 && decltype(upward)::value) {
 #pragma unroll
-                for (int y = 0; y < TH; ++y) {
-                    next_row(y);
-                    push_row(y, cl, cr, pl, pr, nl, nr);
-                    pl = cl;
-                    pr = cr;
-                    cl = nl;
-                    cr = nr;
+                    for (int y = TH - 1; y >= 0; --y) push_row(y, 0, 0, 0, 0, 0, 0);
+                } else {
+#pragma unroll
+                    for (int y = 0; y < TH; ++y) {
+                        next_row(y);
+                        push_row(y, cl, cr, pl, pr, nl, nr);
+                        pl = cl;
+                        pr = cr;
+                        cl = nl;
+                        cr = nr;
+                    }
                 }
                 int act = 0;
                 if (NB == 8) {
@@ -2455,7 +2465,14 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
                     cl = nl;
                     cr = nr;
                 }
-                if (!__any(act)) break;
+                return __any(act);
+            };
+            while (it < iters) {
+                ++it;
+                if (!iteration(std::false_type{})) break;
+                if (NB == 8 || it >= iters) continue;
+                ++it;
+                if (!iteration(std::true_type{})) break;
             }
 #pragma unroll
             for (int y = 0; y < TH; ++y) store_row(y);
