@@ -606,7 +606,7 @@ def single(args, opts):
                device_resident_ms=round(dms, 3), device_resident_mpx_s=round(W * H / dms / 1e3, 2),
                flow=int(st.flow), steps=int(st.steps), gr_rounds=int(st.gr_rounds), pr_tiles=int(st.pr_tiles),
                gr_tiles=int(st.gr_tiles), sweeps_per_gr=opts.sweeps_per_gr, inner_iters=opts.inner_iters,
-               gr_cap=opts.gr_cap)
+               gr_cap=opts.gr_cap, gr_cap_late=opts.gr_cap_late)
     if not args.no_cpu:
         import oracle
         n = W * H
@@ -642,7 +642,7 @@ def pmc_traffic(args, opts, kernel_prefix, launches_per_solve):
              "--no-unseen", "--steps", "1", "--warmup", "0", "--calibrate-copy", "--instances", str(args.instances),
              "--seed", str(args.seed), "--sweeps", str(opts.sweeps_per_gr), "--inner", str(opts.inner_iters),
              "--poll", str(opts.steps_per_poll), "--gr-block", f"{opts.gr_block_w}x{opts.gr_block_h}",
-             "--gr-cap", str(opts.gr_cap)]
+             "--gr-cap", str(opts.gr_cap), "--gr-cap-late", str(opts.gr_cap_late)]
     if opts.gr_waves_per_cu:
         child += ["--gr-waves", str(opts.gr_waves_per_cu)]
     if opts.pr_waves_per_cu:
@@ -732,7 +732,8 @@ def main():
     ap.add_argument("--inner", type=int, default=0)
     ap.add_argument("--poll", type=int, default=0)
     ap.add_argument("--gr-block", default=None, help="global-relabel block WxH in tiles, e.g. 1x4")
-    ap.add_argument("--gr-cap", type=int, default=None, help="distance cap of intermediate global relabels")
+    ap.add_argument("--gr-cap", type=int, default=None, help="distance cap of the first global relabel")
+    ap.add_argument("--gr-cap-late", type=int, default=None, help="distance cap of the next three global relabels")
     ap.add_argument("--gr-waves", type=int, default=None, help="relabel launch wavefronts per CU")
     ap.add_argument("--pr-waves", type=int, default=None, help="push/relabel launch wavefronts per CU")
     ap.add_argument("--no-autotune", action="store_true")
@@ -765,6 +766,8 @@ def main():
         opts.gr_block_w, opts.gr_block_h = (int(v) for v in args.gr_block.split("x"))
     if args.gr_cap is not None:
         opts.gr_cap = args.gr_cap
+    if args.gr_cap_late is not None:
+        opts.gr_cap_late = args.gr_cap_late
     if args.gr_waves:
         opts.gr_waves_per_cu = args.gr_waves
     if args.pr_waves:
@@ -791,7 +794,7 @@ def main():
     batch = gcfg.Batch(batch_items(insts), opts)
     tuned = None
     if not args.no_autotune and not (args.sweeps or args.inner or args.poll or args.gr_block or
-                                     args.gr_cap is not None):
+                                     args.gr_cap is not None or args.gr_cap_late is not None):
         # One cache entry per workload class (instance count), not per seed.
         key = f"batch:ellipse:n{args.instances}"
         cache = args.tune_cache
@@ -906,7 +909,7 @@ def main():
             g.close()
             extra["plain_options"] = {k: getattr(popts, k) for k in ("sweeps_per_gr", "inner_iters", "gr_rounds",
                                                                    "steps_per_poll", "gr_block_w", "gr_block_h",
-                                                                   "gr_cap")}
+                                                                   "gr_cap", "gr_cap_late")}
         graphs, hosts = [], []
         for it in insts:
             g = gcfg.Graph(it.W, it.H, popts)
@@ -969,6 +972,7 @@ def main():
                    "sweeps_per_gr": opts.sweeps_per_gr, "inner_iters": opts.inner_iters,
                    "steps_per_poll": opts.steps_per_poll, "gr_rounds": opts.gr_rounds,
                    "gr_block": f"{opts.gr_block_w}x{opts.gr_block_h}", "gr_cap": opts.gr_cap,
+                   "gr_cap_late": opts.gr_cap_late,
                    "gr_waves_per_cu": opts.gr_waves_per_cu, "autotuned": tuned is not None},
         "roofline": roofline,
         "cpu_baseline": cpu,

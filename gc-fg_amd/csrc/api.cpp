@@ -61,7 +61,8 @@ void gcfg_default_options(gcfg_options *o) {
     o->gr_rounds = 48;
     o->gr_block_w = 1;
     o->gr_block_h = 1;
-    o->gr_cap = 8;  // capped while flow moves locally; exact after 4 rounds (see the scheduler)
+    o->gr_cap = 4;        // the build's relabel: capped short while flow moves locally
+    o->gr_cap_late = 15;  // the next three capped rounds; exact after 4 rounds (see the scheduler)
     o->pr_waves_per_cu = 0;
     o->gr_waves_per_cu = 0;
     o->lazy_caps = 0;

@@ -166,8 +166,8 @@ bool options_valid(const gcfg_options &o) {
         set_error("options: lazy_caps must be 0, 1 or 2");
         return false;
     }
-    if (o.gr_cap < 0) {
-        set_error("options: gr_cap must be >= 0");
+    if (o.gr_cap < 0 || o.gr_cap_late < 0) {
+        set_error("options: gr_cap and gr_cap_late must be >= 0");
         return false;
     }
     if ((o.gr_block_w || o.gr_block_h) && !gr_block_supported(o.gr_block_w, o.gr_block_h)) {
@@ -205,9 +205,16 @@ void Engine::apply_options() {
     // which slows the kernels being timed, so only profile >= 2 turns them on.
     const int prof = opt.profile >= 2 ? 1 : 0;
     const int cap = variant && opt.gr_cap > 0 ? opt.gr_cap : 0;
+    // Later capped rounds never use a smaller cap than the first: a
+    // reachability round reads distances below the last cap as witnesses,
+    // and a block skipped since an earlier round holds distances below that
+    // round's cap.
+    const int cap_late = cap ? std::max(cap, opt.gr_cap_late) : 0;
     if (kp_.inner_iters != opt.inner_iters || kp_.variant != variant || grid_pr_ != gpr || grid_gr_ != ggr ||
-        kp_.gr_rounds != grr || kp_.profile != prof || kp_.gbx != gbx || kp_.gby != gby || kp_.gr_cap != cap) {
+        kp_.gr_rounds != grr || kp_.profile != prof || kp_.gbx != gbx || kp_.gby != gby || kp_.gr_cap != cap ||
+        kp_.gr_cap_late != cap_late) {
         kp_.gr_cap = cap;
+        kp_.gr_cap_late = cap_late;
         kp_.inner_iters = opt.inner_iters;
         kp_.gr_rounds = grr;
         kp_.profile = prof;

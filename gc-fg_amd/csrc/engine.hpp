@@ -89,7 +89,8 @@ struct InstState {
     int32_t gr_rounds;
     int32_t steps;
     int64_t sum_pos;       // change of the interior sum of pos by edits since the build
-    int64_t sum_exc;       // unused
+    int32_t wcap;          // cap of the capped round a reachability round follows (its witnesses)
+    int32_t pad2_;
     int64_t flow;
     int64_t pr_tiles;      // tile push-relabel passes scheduled
     int64_t gr_tiles;      // tile BFS sweeps scheduled (initial and relaxation)
@@ -97,7 +98,7 @@ struct InstState {
     int32_t sweeps;        // colour-pass pairs between global relabels
     int64_t gr_init_tiles; // of which initial sweeps (INIT / REINIT)
     int32_t cap;           // distance cap of the current global relabel (DINF: exact, CAP_FLOOD: reachability)
-    int32_t pad_;
+    int32_t partial;       // 1: the current exact round recomputes only the blocks near pushes
     int64_t gr_exact_tiles;  // relabel items of exact rounds (those write labels)
     int64_t fin_items;       // FIN items (blocks left unreachable)
 };
@@ -155,6 +156,7 @@ struct KParams {
     int32_t profile;    // count inner iterations / relaxation rounds (extra atomics)
     int32_t gbx, gby;   // global-relabel block, in tiles (one wavefront per tile)
     int32_t gr_cap;     // distance cap of intermediate global relabels (0: always exact)
+    int32_t gr_cap_late;  // cap of the capped rounds after the first (>= gr_cap; 0 when gr_cap is)
     // 8-neighbourhood (nbhd == 8; workgroup kernels only): per node the
     // diagonal residual capacities int4 {up-left, up-right, down-left,
     // down-right} and a second mask byte (bits 0-3: those arcs residual); per

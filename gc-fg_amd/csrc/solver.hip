@@ -363,8 +363,13 @@ __device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, i
                     // that may not reach a sink. Capped round: blocks with a
                     // node below the cap that a push pass may have changed
                     // (the others keep their labels).
+                    // Partial exact round (colour 3): every block with a
+                    // reachable node within the stale distance of a push.
                     const int lv = v0[k];
-                    pred = colour == 1 ? lv != 0 : colour == 2 ? (lv == 1 || lv == 2) : (lv >= 2 && v1[k]);
+                    pred = colour == 1   ? lv != 0
+                           : colour == 2 ? (lv == 1 || lv == 2)
+                           : colour == 3 ? (lv != 0 && v1[k])
+                                         : (lv >= 2 && v1[k]);
                     if (v1[k]) p.stale[t] = 0;
                     if (!colour && lv == 1) {
                         // A skipped block's nodes are all frozen: none is active.
@@ -453,11 +458,14 @@ __device__ __forceinline__ bool any_active_tile(const KParams &p, const InstDesc
 // exact one.
 template <int SNT>
 __device__ __forceinline__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st, int parity,
-                                                     bool reinit, int cap, bool all = false) {
+                                                     bool reinit, int cap, bool all = false, bool partial = false,
+                                                     bool grow = false, int wcap = 0) {
     if (threadIdx.x == 0) {
         st.gr_rounds++;
         st.phase = PH_GR;
         st.cap = cap;
+        st.partial = partial;
+        if (cap == CAP_FLOOD) st.wcap = wcap;
     }
     // An exact round (cap DINF) revisits every block with a reachable node. A
     // capped round with cap K recomputes only blocks that have a node below
@@ -469,8 +477,8 @@ __device__ __forceinline__ void start_global_relabel(const KParams &p, const Ins
     const bool whole = cap >= DINF || cap == CAP_FLOOD;  // exact distances or reachability: every live block
     const int n = !reinit ? emit_tiles<SNT>(p, D, EM_ALL, 0, false, OP_GR_INIT, parity)
                   : all   ? emit_tiles<SNT>(p, D, EM_ALL, 0, false, OP_GR_REINIT, parity)
-                          : emit_tiles<SNT>(p, D, EM_LIVE, cap == CAP_FLOOD ? 2 : (whole ? 1 : 0), false, OP_GR_REINIT,
-                                       parity);
+                          : emit_tiles<SNT>(p, D, EM_LIVE, cap == CAP_FLOOD ? 2 : partial ? 3 : ((whole || grow) ? 1 : 0),
+                                       false, OP_GR_REINIT, parity);
     if (threadIdx.x == 0) {
         st.gr_tiles += n;
         st.gr_init_tiles += n;
@@ -488,7 +496,7 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     const InstDesc D = p.desc[s];
     const int phase = st.phase;
     if (phase == PH_IDLE || phase == PH_DONE) return;
-    const int err = st.err, sweeps = st.sweeps, cur_cap = st.cap, rounds0 = st.gr_rounds;
+    const int err = st.err, sweeps = st.sweeps, cur_cap = st.cap, rounds0 = st.gr_rounds, cur_partial = st.partial;
     int pr_iter = st.pr_iter;
     __syncthreads();  // every thread has read the state before thread 0 writes it
     if (threadIdx.x == 0) st.steps++;
@@ -514,8 +522,26 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     // still unfinished after CAP_ROUNDS relabel rounds has flow travelling
     // far, where capped rounds multiply the steps: it relabels exactly from
     // then on.
-    const int cap = (p.variant && p.gr_cap > 0 && rounds0 < CAP_ROUNDS) ? p.gr_cap : DINF;
+    // The first round (the one the build runs) is capped at gr_cap, the
+    // next CAP_ROUNDS - 1 at gr_cap_late (>= gr_cap): a short cap while most
+    // flow still moves locally, where instances that finish early do, and a
+    // longer one for those that go on. A capped round with a larger cap than
+    // the last one recomputes every live block: nodes frozen at the old cap
+    // may lie below the new one.
+    const int cap = (p.variant && p.gr_cap > 0 && rounds0 < CAP_ROUNDS) ? (rounds0 ? p.gr_cap_late : p.gr_cap) : DINF;
+    const bool grow = cap < DINF && cur_cap > 0 && cur_cap < cap;
     const bool reinit = p.variant != 0;
+    // Exact rounds after the first recompute only the blocks with a
+    // reachable node within the stale distance of a push since the last round
+    // (partial rounds). Exact distances never decrease and the labelling
+    // before the round is valid, so a block reset to the sinks and relaxed
+    // from its neighbours' distances gets valid labels no lower than before,
+    // and blocks outside the round keep theirs (a neighbour's new distances
+    // are no lower than its old ones, so no relaxation can change them). The
+    // active nodes all lie in the round (a tile with one was pushed, which
+    // marked it stale). Labels outside the round may be lower than exact,
+    // which costs pushes, not correctness; the last round is a full one.
+    const bool part = reinit && p.gr_cap > 0 && rounds0 > CAP_ROUNDS && cap >= DINF;
     if (phase == PH_START) {
         // A warm start relabels every tile: an edit may have revived tiles
         // the kept flags call unreachable (edits keep the masks current).
@@ -538,7 +564,14 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
                 // Nothing active below the cap: check reachability before
                 // finishing (a flood settles in far fewer tile visits than
                 // exact distances; it also writes the labels).
-                start_global_relabel<SNT>(p, D, st, parity, reinit, CAP_FLOOD);
+                start_global_relabel<SNT>(p, D, st, parity, reinit, CAP_FLOOD, false, false, false, cur_cap);
+                return;
+            }
+            if (cur_partial && !flood) {
+                // A partial round's labels outside its blocks are valid but
+                // may be stale: a full exact round writes the labels (it finds
+                // nothing active either, every excess node being at DINF).
+                start_global_relabel<SNT>(p, D, st, parity, reinit, DINF);
                 return;
             }
             if (threadIdx.x == 0) st.phase = PH_FIN;
@@ -562,7 +595,7 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     // PH_PR: cycle through the tile colours, then relabel globally again.
     const int ncol = p.nbhd == 8 ? 4 : 2;
     if (pr_iter >= ncol * sweeps) {
-        start_global_relabel<SNT>(p, D, st, parity, reinit, cap);
+        start_global_relabel<SNT>(p, D, st, parity, reinit, cap, false, part, grow);
         return;
     }
     const int colour = pr_iter & (ncol - 1);
@@ -570,7 +603,7 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     for (int c = 0; c < ncol && n == 0; ++c)
         n = emit_tiles<SNT>(p, D, EM_COLOUR, (colour + c) & (ncol - 1), true, 0, parity);
     if (n == 0) {
-        start_global_relabel<SNT>(p, D, st, parity, reinit, cap);
+        start_global_relabel<SNT>(p, D, st, parity, reinit, cap, false, part, grow);
         return;
     }
     if (threadIdx.x == 0) {
@@ -587,7 +620,7 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     // Only after an exact round (cur_cap DINF): the push pass reads the
     // instance's cap, which starting the round sets, so it keeps the cap of
     // the round it follows.
-    if (reinit && cur_cap >= DINF && cap >= DINF && pr_iter + 1 >= ncol * sweeps)
+    if (reinit && cur_cap >= DINF && cap >= DINF && !part && pr_iter + 1 >= ncol * sweeps)
         start_global_relabel<SNT>(p, D, st, parity, reinit, cap);
 }
 
@@ -1364,7 +1397,7 @@ __device__ __forceinline__ int side_halo(const TileInfo &ti, int lane, int &idx)
 
 template <int SRC, int NB = 4>
 __device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, int t, int lane, int &hU, int &hD,
-                                          int &hs) {
+                                          int &hs, int wk = 0) {
     const int nu = ti.nbr[2], nd = ti.nbr[3];
     int sidx;
     const int side = side_halo<NB>(ti, lane, sidx);
@@ -1381,9 +1414,9 @@ __device__ __forceinline__ void load_halo(const KParams &p, const TileInfo &ti, 
         vd = (p.mask[gd] & MB_ENEG) ? 0 : DINF;
         vs = (p.mask[gs] & MB_ENEG) ? 0 : DINF;
     } else if (SRC == HALO_WITNESS) {
-        vu = p.dist[gu] < p.gr_cap ? 0 : DINF;
-        vd = p.dist[gd] < p.gr_cap ? 0 : DINF;
-        vs = p.dist[gs] < p.gr_cap ? 0 : DINF;
+        vu = p.dist[gu] < wk ? 0 : DINF;
+        vd = p.dist[gd] < wk ? 0 : DINF;
+        vs = p.dist[gs] < wk ? 0 : DINF;
     } else {
         vu = p.dist[gu];
         vd = p.dist[gd];
@@ -1946,6 +1979,14 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
         // the push kernel's OP_MASK pass and every relabel starts with REINIT.)
         const bool reinit = op == OP_GR_REINIT;
         const int capv = rstate[inst].cap;  // DINF: exact, CAP_FLOOD: reachability, else the cap
+        // A partial exact round resets only some blocks: their REINIT sweep
+        // reads sink halos only, so each block queues itself for a sweep that
+        // reads its neighbours' distances (those outside the round are final).
+        const bool self_dirty = reinit && rstate[inst].partial;
+        const int wcap = rstate[inst].wcap;  // a reachability round's witness threshold
+        // "Near" (live 2: a node below the cap) is the current capped round's
+        // cap; other rounds keep the option's (no capped round uses it after them).
+        const int near_thr = (capv > 0 && capv < DINF) ? capv : p.gr_cap;
         const bool flood = capv == CAP_FLOOD;
         const int vinit = flood ? DINF : capv;  // start value of non-sink nodes
         int d[TH];
@@ -1967,7 +2008,7 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
 #pragma unroll
             for (int y = 0; y < TH; ++y) d[y] = DINF;
         } else if (reinit && flood) {
-            load_halo<HALO_WITNESS, NB>(p, ti, t, lane, hU, hD, hs);
+            load_halo<HALO_WITNESS, NB>(p, ti, t, lane, hU, hD, hs, wcap);
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
                 m.put(y, T.m[y * TW + lane]);
@@ -1975,7 +2016,7 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
                 d[y] = T.d[y * TW + lane];
             }
 #pragma unroll
-            for (int y = 0; y < TH; ++y) d[y] = ((m.get(y) & MB_ENEG) || d[y] < p.gr_cap) ? 0 : DINF;
+            for (int y = 0; y < TH; ++y) d[y] = ((m.get(y) & MB_ENEG) || d[y] < wcap) ? 0 : DINF;
         } else if (reinit) {
             load_halo<HALO_MASK, NB>(p, ti, t, lane, hU, hD, hs);
 #pragma unroll
@@ -1987,7 +2028,7 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
             for (int y = 0; y < TH; ++y) d[y] = (m.get(y) & MB_ENEG) ? 0 : vinit;
         } else if (flood) {
             // Visited blocks hold 0 or DINF by now; skipped ones their capped distances.
-            load_halo<HALO_WITNESS, NB>(p, ti, t, lane, hU, hD, hs);
+            load_halo<HALO_WITNESS, NB>(p, ti, t, lane, hU, hD, hs, wcap);
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
                 m.put(y, T.m[y * TW + lane]);
@@ -2055,7 +2096,7 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
                 if (reinit || ((chg >> y) & 1)) T.d[y * TW + lane] = d[y];
                 act |= (m.get(y) & MB_EPOS) && d[y] < vinit;
                 fin |= d[y] < DINF;
-                nearv |= d[y] < p.gr_cap;
+                nearv |= d[y] < near_thr;
                 const int Y = ty * TH + y;
                 // an interior node not known to reach a sink (at or past the cap, or not at 0 in a flood)
                 open |= colin && Y > 0 && Y < D.H - 1 && d[y] >= (flood ? 1 : capv);
@@ -2096,7 +2137,7 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
         const bool any_near = p.gr_cap <= 0 || (NW == 1 ? __any(nearv) : __syncthreads_or(nearv));
         const bool any_open = NW == 1 ? __any(open) : __syncthreads_or(open);
         if (threadIdx.x == 0) {
-            if (capped) mark_dirty(p, t0);
+            if (capped || self_dirty) mark_dirty(p, t0);
             p.live[t0] = any_fin ? ((capv < DINF && !any_open) ? 3 : (any_near ? 2 : 1)) : 0;
         }
     }
@@ -2578,8 +2619,8 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             // Blocks within cap distance of this tile (and of the tiles it
             // pushed into) need their capped distances recomputed. A path of
             // length K moves at most K columns and K rows, diagonals included.
-            const int rx = (p.gr_cap + TW - 1) / TW + ((anyL || anyR || corners) ? 1 : 0);
-            const int ry = (p.gr_cap + TH - 1) / TH + ((anyU || anyD || corners) ? 1 : 0);
+            const int rx = (p.gr_cap_late + TW - 1) / TW + ((anyL || anyR || corners) ? 1 : 0);
+            const int ry = (p.gr_cap_late + TH - 1) / TH + ((anyU || anyD || corners) ? 1 : 0);
             const int span = 2 * rx + 1, cnt = span * (2 * ry + 1);
             for (int k = lane; k < cnt; k += TW) {
                 const int x = ti.tx + k % span - rx, y = ti.ty + k / span - ry;

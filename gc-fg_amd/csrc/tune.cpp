@@ -47,25 +47,26 @@ int coordinate_search(gcfg_options start, int trials, const std::function<double
         {&gcfg_options::sweeps_per_gr, {1, 2, 4, 8}},
         {&gcfg_options::gr_rounds, {2, 4, 8, 48}},
         {&gcfg_options::steps_per_poll, {4, 8, 16, 32}},
-        {&gcfg_options::gr_cap, {0, 4, 8, 12, 16, 32, 64}},
+        {&gcfg_options::gr_cap, {0, 2, 3, 4, 6, 8}},
+        {&gcfg_options::gr_cap_late, {4, 8, 12, 15, 16, 32}},
         {&gcfg_options::gr_waves_per_cu, {8, 16, 28, 56}},
         {&gcfg_options::pr_waves_per_cu, {8, 16, 32}},
     };
     // Global-relabel block shapes (width, height in tiles) are one knob.
     const std::vector<std::pair<int32_t, int32_t>> shapes = {{1, 1}, {1, 2}, {1, 4}, {1, 8}, {1, 16},
                                                              {2, 2}, {2, 4}, {2, 8}, {4, 4}};
-    // The relabel cap and the push iterations per pass pay off together (a
-    // larger cap lets flow go further per round, more iterations let it get
-    // there): they are tried as pairs before one at a time. On a 4096^2 image
-    // with heavy-tailed t-links cap 12 alone and 12 iterations alone each gain
-    // under 2 %, together 15 %.
-    const std::vector<std::pair<int32_t, int32_t>> cap_iters = {{4, 2}, {8, 8}, {12, 12}, {16, 16}};
+    // The later rounds' relabel cap and the push iterations per pass pay off
+    // together (a larger cap lets flow go further per round, more iterations
+    // let it get there): they are tried as pairs before one at a time. On a
+    // 4096^2 image with heavy-tailed t-links cap 12 alone and 12 iterations
+    // alone each gained under 2 %, together 15 %.
+    const std::vector<std::pair<int32_t, int32_t>> cap_iters = {{4, 2}, {8, 8}, {12, 12}, {15, 12}, {16, 16}};
     for (int pass = 0; pass < 2; ++pass) {
         bool improved = false;
         for (const auto &ci : cap_iters) {
-            if (cur.gr_cap == ci.first && cur.inner_iters == ci.second) continue;
+            if (cur.gr_cap_late == ci.first && cur.inner_iters == ci.second) continue;
             gcfg_options cand = cur;
-            cand.gr_cap = ci.first;
+            cand.gr_cap_late = ci.first;
             cand.inner_iters = ci.second;
             const double ms = measure(cand);
             if (ms < 0) return GCFG_ERR_HIP;
@@ -116,13 +117,13 @@ extern "C" {
 static const char *kFields[] = {"sweeps_per_gr", "inner_iters", "steps_per_poll", "use_graph",
                                 "max_steps",     "verbose",     "profile",        "kernels",
                                 "gr_rounds",     "gr_block_w",  "gr_block_h",     "gr_cap",
-                                "pr_waves_per_cu", "gr_waves_per_cu", "lazy_caps"};
+                                "pr_waves_per_cu", "gr_waves_per_cu", "lazy_caps", "gr_cap_late"};
 
 static std::vector<int32_t *> option_fields(gcfg_options *o) {
     return {&o->sweeps_per_gr, &o->inner_iters, &o->steps_per_poll, &o->use_graph,
             &o->max_steps,     &o->verbose,     &o->profile,        &o->kernels,
             &o->gr_rounds,     &o->gr_block_w,  &o->gr_block_h,     &o->gr_cap,
-            &o->pr_waves_per_cu, &o->gr_waves_per_cu, &o->lazy_caps};
+            &o->pr_waves_per_cu, &o->gr_waves_per_cu, &o->lazy_caps, &o->gr_cap_late};
 }
 
 int gcfg_tune_store(const char *path, const char *key, const gcfg_options *opt) {

@@ -50,7 +50,7 @@ class Options(ctypes.Structure):
                 ("kernels", ctypes.c_int32), ("gr_rounds", ctypes.c_int32),
                 ("gr_block_w", ctypes.c_int32), ("gr_block_h", ctypes.c_int32), ("gr_cap", ctypes.c_int32),
                 ("pr_waves_per_cu", ctypes.c_int32), ("gr_waves_per_cu", ctypes.c_int32),
-                ("lazy_caps", ctypes.c_int32)]
+                ("lazy_caps", ctypes.c_int32), ("gr_cap_late", ctypes.c_int32)]
 
 
 class Stats(ctypes.Structure):

@@ -60,16 +60,20 @@ typedef struct gcfg_options {
                                 default 48) */
     int32_t gr_block_w;      /* global-relabel block in tiles (one workgroup, BFS exchanged through LDS): */
     int32_t gr_block_h;      /*   1x1, 1x2, 1x4, 1x8, 1x16, 2x2, 2x4, 2x8 or 4x4; 0 = default */
-    int32_t gr_cap;          /* >0 (default 8): the first 4 global relabels of an instance stop BFS
-                                fronts at this distance (nodes beyond are frozen until the next
-                                relabel), later ones are exact; an instance finished within capped
-                                rounds ends with a reachability round (and an exact round if that
-                                finds work left); 0: every relabel exact */
+    int32_t gr_cap;          /* >0 (default 4): the first global relabel of an instance (the one the
+                                build runs) stops BFS fronts at this distance (nodes beyond are
+                                frozen until the next relabel), the next three at gr_cap_late, later
+                                ones are exact; exact rounds after the first recompute only the
+                                blocks near the pushes since the last round, and an instance ends
+                                with a round over every block (reachability after capped rounds,
+                                else exact); 0: every relabel exact over every block */
     int32_t pr_waves_per_cu; /* push/relabel launch size in wavefronts per compute unit (0 = as many as stay resident) */
     int32_t gr_waves_per_cu; /* global-relabel launch size in wavefronts per compute unit (0 = 48) */
     int32_t lazy_caps;       /* residual capacities of the wavefront kernels: 0 = chosen per build (lazy for
                                 large batches with few steps), 1 = written by the build, 2 = lazy (written
                                 before a tile's first push pass) */
+    int32_t gr_cap_late;     /* cap of the second to fourth global relabels when gr_cap > 0 (default 15;
+                                0 or anything below gr_cap: gr_cap) */
 } gcfg_options;
 
 typedef struct gcfg_stats {

@@ -424,6 +424,48 @@ def test_capped_relabels(g, ora, cap):
     assert st.flow == ora.maxflow(W, H, pos, neg, edges=E)[0]
 
 
+@pytest.mark.parametrize("cfg", [dict(), dict(gr_cap=8, gr_cap_late=8), dict(gr_cap=6, gr_cap_late=2),
+                                 dict(gr_cap=2, gr_cap_late=31), dict(inner_iters=1),
+                                 dict(gr_block_w=2, gr_block_h=2), dict(gr_cap=0)])
+def test_cap_schedule_and_partial_rounds(g, ora, cfg):
+    """The first relabel capped at gr_cap, the next three at gr_cap_late
+    (a larger late cap recomputes every live block; a smaller one is raised
+    to gr_cap), then exact rounds that recompute only the blocks near the
+    last pushes, and a full exact round at the end: a heavy-tailed instance
+    that needs more than five rounds gives the exact flow and labels, and its
+    final distances are an exact BFS of the returned residual graph."""
+    W, H = 640, 480
+    pos, neg = g.gen_stress_tlinks(W, H, 21)
+    img = g.gen_perlin_image(W, H, 22)
+    dt = g.DataTerm(W, H)
+    dt.set_image(img)
+    dt.set_seeds(np.zeros((H, W), np.uint8))
+    dt.compute()
+    _, _, up, down, left, right = dt.links()
+    E = (up, down, left, right)
+    o = _opts(g, 1, **cfg)
+    rflow, rlabel = ora.maxflow(W, H, pos, neg, edges=E)
+    gr = g.Graph(W, H, o)
+    gr.set_dataterms(pos, neg)
+    gr.set_edges(*E)
+    gr.build()
+    label, st = gr.optimize()
+    caps, exc = gr.read_state()
+    dist = gr.read_distances()
+    gr.close()
+    assert st.flow == rflow
+    assert np.array_equal(label, rlabel), int((label != rlabel).sum())
+    if not cfg:
+        assert st.gr_rounds > 5, st.gr_rounds  # partial exact rounds ran
+    inter = np.zeros((H, W), bool)
+    inter[1:-1, 1:-1] = True
+    want = _bfs_to_sinks(W, H, caps, exc)[inter]
+    reach = want < 0x3FFFFFFF
+    assert np.array_equal(dist[inter] < 0x3FFFFFFF, reach)
+    if not (dist[inter][reach] == 0).all():  # ended with an exact round, not a reachability flood
+        assert np.array_equal(dist[inter].astype(np.int64), want)
+
+
 def test_caller_stream_ordering(g, ora):
     """With a caller stream set, a device copy of the labels queued on that
     stream right after optimize_device sees the final labels; results equal
