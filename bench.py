@@ -607,6 +607,31 @@ def single(args, opts):
                flow=int(st.flow), steps=int(st.steps), gr_rounds=int(st.gr_rounds), pr_tiles=int(st.pr_tiles),
                gr_tiles=int(st.gr_tiles), sweeps_per_gr=opts.sweeps_per_gr, inner_iters=opts.inner_iters,
                gr_cap=opts.gr_cap, gr_cap_late=opts.gr_cap_late)
+    # Event-timed solve (profile mode) for each kernel's time per solve and
+    # its rate by the byte model of the batch roofline (relabel: REINIT and
+    # relaxation visits and labels; push: tile passes).
+    popts = gcfg.Options.from_buffer_copy(opts)
+    popts.profile = 1
+    gp = gcfg.Graph(W, H, popts)
+    gp.set_dataterms(bufs[0], bufs[1])
+    gp.set_edges(*bufs[2:6])
+    for _ in range(2):  # the second solve launches exactly the steps the first needed
+        gp.build()
+        pst = gp.optimize_device(d_label)
+    gp.close()
+    gr_bytes = 1024 * max(1, opts.gr_block_w) * max(1, opts.gr_block_h) * (
+        pst.gr_init_tiles * GR_INIT_BYTES_PER_NODE + (pst.gr_tiles - pst.gr_init_tiles) * GR_RELAX_BYTES_PER_NODE +
+        (pst.gr_exact_tiles + pst.fin_items) * LABEL_BYTES_PER_NODE)
+    pr_bytes = pst.pr_tiles * 1024 * PR_BYTES_PER_NODE
+
+    def rate(ms_k, b):
+        return dict(ms=round(ms_k, 4), bytes=int(b), gb_s=round(b / (ms_k * 1e-3) / 1e9, 1),
+                    frac=round(b / (ms_k * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)) if ms_k > 0 else None
+    out["kernels"] = dict(k_grb=rate(pst.gr_kernel_ms, gr_bytes), k_prw=rate(pst.pr_kernel_ms, pr_bytes),
+                          k_sched_ms=round(pst.sched_kernel_ms, 4), launches=int(pst.launches),
+                          gr_items=int(pst.gr_tiles), gr_init_items=int(pst.gr_init_tiles),
+                          gr_exact_items=int(pst.gr_exact_tiles), pr_items=int(pst.pr_tiles),
+                          note="kernel time per solve includes the launches a step leaves empty")
     if not args.no_cpu:
         import oracle
         n = W * H
