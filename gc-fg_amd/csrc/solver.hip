@@ -2087,6 +2087,29 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
                 break;
             }
         }
+        // A relaxation sweep only lowers distances, so the halo loaded at the
+        // start bounds each neighbour's current values from above: a
+        // neighbour whose boundary nodes v all have d(u) + 1 >= halo(v) for
+        // the changed nodes u next to them cannot improve, and is not queued
+        // again. (In a REINIT sweep neighbours may be reset to higher values
+        // after the halo was read, so every change counts.) Computed before
+        // the stores, while the halos are still in registers.
+        bool impL = true, impR = true, impU = true, impD = true;
+        if (NB == 4 && NW == 1 && !reinit && have) {
+            impU = __any((chg & 1u) && d[0] + 1 < hU);
+            impD = __any(((chg >> (TH - 1)) & 1u) && d[TH - 1] + 1 < hD);
+            // Every lane compares its rows with the side halo's; lanes 0 and
+            // 63 are the boundary columns.
+            bool cl = false, cr = false;
+#pragma unroll
+            for (int y = 0; y < TH; ++y) {
+                const bool c = (chg >> y) & 1u;
+                cl |= c && d[y] + 1 < __builtin_amdgcn_readlane(hs, y);
+                cr |= c && d[y] + 1 < __builtin_amdgcn_readlane(hs, TH + y);
+            }
+            impL = __any(lane == 0 && cl);
+            impR = __any(lane == TW - 1 && cr);
+        }
         int act = 0, fin = 0, nearv = 0, open = 0;
         if (have) {
             const int X = tx * TW + lane;
@@ -2106,10 +2129,14 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
             if (capv >= DINF || flood) write_labels(D, tx, ty, lane, d);
         }
         // Outer boundary changes re-queue the neighbouring blocks.
-        const bool chL = __any(lane == 0 && chg != 0) && !inL && have && ti.nbr[0] >= 0;
-        const bool chR = __any(lane == TW - 1 && chg != 0) && !inR && have && ti.nbr[1] >= 0;
-        const bool chU = __any((chg & 1u) != 0) && !inU && have && ti.nbr[2] >= 0;
-        const bool chD = __any((chg >> (TH - 1)) & 1u) && !inD && have && ti.nbr[3] >= 0;
+        bool chL = __any(lane == 0 && chg != 0) && !inL && have && ti.nbr[0] >= 0;
+        bool chR = __any(lane == TW - 1 && chg != 0) && !inR && have && ti.nbr[1] >= 0;
+        bool chU = __any((chg & 1u) != 0) && !inU && have && ti.nbr[2] >= 0;
+        bool chD = __any((chg >> (TH - 1)) & 1u) && !inD && have && ti.nbr[3] >= 0;
+        chL = chL && impL;
+        chR = chR && impR;
+        chU = chU && impU;
+        chD = chD && impD;
         // 8-neighbourhood (one-tile blocks): a changed corner node is a halo
         // node of the diagonal tile as well.
         bool chUL = false, chUR = false, chDL = false, chDR = false;
