@@ -127,7 +127,10 @@ __global__ __launch_bounds__(DT_NT) void k_dt_stats(DtLaunch L) {
     if (unit >= L.units) return;
     DtJob J;
     int c;
-    dt_job(L, unit, J, c);
+    const int split = L.split;
+    dt_job(L, unit / split, J, c);
+    // This workgroup's share of the chunk's rounds.
+    const int part = unit % split, r0 = part * (DT_ROUNDS / split), r1 = r0 + DT_ROUNDS / split;
     for (int i = threadIdx.x; i < DT_HIST; i += DT_NT) hist[i] = 0;
     __syncthreads();
     const int W = J.W, H = J.H, n = J.W * J.H;
@@ -141,7 +144,7 @@ __global__ __launch_bounds__(DT_NT) void k_dt_stats(DtLaunch L) {
     for (int r = 0; r < DT_ROUNDS; ++r) {
         const int i0 = c * DT_CHUNK + (r * DT_NT + (int)threadIdx.x) * 4;
         sd[r] = 0;
-        if (i0 < n) {
+        if (r >= r0 && r < r1 && i0 < n) {
             load_words<5>(J.img, 3 * (int64_t)i0 - 4, total, w[r]);  // p(-1) .. p(4)
             load_px12(J.img, 3 * ((int64_t)i0 + W), total, b[r]);     // the four pixels below
             if (J.seeds) {
@@ -156,7 +159,8 @@ __global__ __launch_bounds__(DT_NT) void k_dt_stats(DtLaunch L) {
 #pragma unroll
     for (int r = 0; r < DT_ROUNDS; ++r) {
         const int i0 = c * DT_CHUNK + (r * DT_NT + (int)threadIdx.x) * 4;
-        if (i0 >= n) break;
+        if (r < r0) continue;
+        if (r >= r1 || i0 >= n) break;
         int x = i0 % W, y = i0 / W;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -190,7 +194,7 @@ __global__ __launch_bounds__(DT_NT) void k_dt_stats(DtLaunch L) {
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
         for (int k = 0; k < DT_NT / 64; ++k) t += red[k];
-        J.beta_part[c] = t;
+        J.beta_part[c * split + part] = t;
     }
     // Only the few chunks that hold seed pixels touch the image's counters.
     if (any_seed)
@@ -203,7 +207,7 @@ __device__ __forceinline__ long long nlink_exact(int s, int lambda, double beta)
     return beta > 0.0 ? llround((double)lambda * exp(-(double)s / (2.0 * beta))) : (long long)lambda;
 }
 
-// Table pass (DT_TABLE_PARTS workgroups per image): reduce the partials,
+// Table pass (L.tparts workgroups per image): reduce the partials,
 // keep the statistics, build the t-link costs of the 16^3 colour bins of both
 // classes and the n-link thresholds thr[k] = largest squared difference s
 // with w(s) >= k (thr[0] = +inf, -1 where no s qualifies). Every part reduces
@@ -212,14 +216,14 @@ __device__ __forceinline__ long long nlink_exact(int s, int lambda, double beta)
 // single workgroup busy for about 15 us); part 0 writes the statistics. The
 // histogram accumulator is cleared by the links pass, after every part has
 // read it.
-constexpr int DT_TABLE_PARTS = 8;
 __global__ __launch_bounds__(DT_NT) void k_dt_table(DtLaunch L) {
     __shared__ int hist[DT_HIST];
     __shared__ double rr[2][3][16];
     __shared__ unsigned long long red[DT_NT / 64];
-    const int part = blockIdx.x % DT_TABLE_PARTS;
-    const DtJob J = L.njobs ? L.jobs[blockIdx.x / DT_TABLE_PARTS] : L.one;
-    const int nch = dt_chunks(J.W, J.H);
+    const int parts = L.tparts;
+    const int part = blockIdx.x % parts;
+    const DtJob J = L.njobs ? L.jobs[blockIdx.x / parts] : L.one;
+    const int nch = dt_chunks(J.W, J.H) * L.split;  // beta partials
     unsigned long long bs = 0;
     for (int k = threadIdx.x; k < nch; k += DT_NT) bs += J.beta_part[k];
 #pragma unroll
@@ -247,7 +251,7 @@ __global__ __launch_bounds__(DT_NT) void k_dt_table(DtLaunch L) {
     }
     __syncthreads();
     if (J.model == GCFG_DT_MODEL_HISTOGRAM) {
-        constexpr int PER = DT_LUT / DT_TABLE_PARTS;
+        const int PER = DT_LUT / parts;
         for (int e = part * PER + threadIdx.x; e < (part + 1) * PER; e += DT_NT) {
             const int k = e >> 12, idx = e & 4095;
             double p = 1.0;
@@ -260,7 +264,7 @@ __global__ __launch_bounds__(DT_NT) void k_dt_table(DtLaunch L) {
     const int64_t pairs = (int64_t)(J.W - 1) * J.H + (int64_t)J.W * (J.H - 1);
     const double beta = pairs > 0 ? (double)tot / (double)pairs : 0.0;
     const int lambda = J.lambda;
-    for (int k = part * DT_NT + threadIdx.x; k <= lambda; k += DT_NT * DT_TABLE_PARTS) {
+    for (int k = part * DT_NT + threadIdx.x; k <= lambda; k += DT_NT * parts) {
         int t;
         if (k == 0) {
             t = 0x7fffffff;
@@ -349,11 +353,12 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
     __shared__ float gs[KDE ? KDE_TAPS * KDE_TAPS : 1];
     DtJob J;
     int c;
-    constexpr int SPLIT = KDE ? DT_KDE_SPLIT : 1;
+    const int split = KDE ? max(DT_KDE_SPLIT, L.split) : L.split;
     const int unit = xcd_unit(blockIdx.x, L.units);
     if (unit >= L.units) return;
-    dt_job(L, unit / SPLIT, J, c);
-    const int part = unit % SPLIT;  // this workgroup's rounds of the chunk
+    dt_job(L, unit / split, J, c);
+    const int part = unit % split;  // this workgroup's rounds of the chunk
+    const int r_beg = part * (DT_ROUNDS / split), r_end = r_beg + DT_ROUNDS / split;
     // The histogram accumulator is read by every table part; cleared here,
     // after the table pass and before the next compute's stats pass.
     if (c == 0 && part == 0)
@@ -375,15 +380,14 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
     const double beta = pairs > 0 ? (double)J.stats->beta_sum / (double)pairs : 0.0;
     const float nib = beta > 0.0 ? (float)(-0.5 / beta) : 0.0f;
     // Loads of two rounds are in flight before either is used (the stores of
-    // the six planes then overlap the next pair's loads).
-    static_assert(DT_ROUNDS % (2 * SPLIT) == 0, "whole pairs of rounds per workgroup");
-#pragma unroll
-    for (int r2 = part * (DT_ROUNDS / SPLIT); r2 < (part + 1) * (DT_ROUNDS / SPLIT); r2 += 2) {
+    // the six planes then overlap the next pair's loads); a workgroup with a
+    // single round of the chunk runs half a pair.
+    for (int r2 = r_beg; r2 < r_end; r2 += 2) {
     uint32_t ww[2][5], aa[2][3], bb[2][3];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int i0 = c * DT_CHUNK + ((r2 + h) * DT_NT + (int)threadIdx.x) * 4;
-        if (i0 < n) {
+        if (r2 + h < r_end && i0 < n) {
             load_words<5>(J.img, 3 * (int64_t)i0 - 4, total, ww[h]);  // p(-1) .. p(4)
             load_px12(J.img, 3 * ((int64_t)i0 - W), total, aa[h]);     // above
             load_px12(J.img, 3 * ((int64_t)i0 + W), total, bb[h]);     // below
@@ -424,7 +428,7 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
         const int i0 = c * DT_CHUNK + ((r2 + h) * DT_NT + (int)threadIdx.x) * 4;
-        if (i0 >= n) break;
+        if (r2 + h >= r_end || i0 >= n) break;
         const uint32_t(&w)[5] = ww[h];
         const uint32_t(&a)[3] = aa[h];
         const uint32_t(&b)[3] = bb[h];
@@ -476,12 +480,15 @@ void dt_launch(const DtLaunch &L, int total_chunks, hipStream_t s) {
     const int jobs = L.njobs ? L.njobs : 1;
     const bool kde = L.njobs == 0 && L.one.model == GCFG_DT_MODEL_KDE;
     DtLaunch P = L;
-    P.units = total_chunks;
+    P.split = L.njobs == 0 && L.split > 1 ? std::min(L.split, DT_SPLIT_MAX) : 1;
+    P.tparts = L.tparts == 16 || L.tparts == 32 ? L.tparts : 8;
+    P.units = total_chunks * P.split;
     hipLaunchKernelGGL(k_dt_stats, dim3(xcd_grid(P.units)), dim3(DT_NT), 0, s, P);
-    hipLaunchKernelGGL(k_dt_table, dim3(jobs * DT_TABLE_PARTS), dim3(DT_NT), 0, s, P);
+    hipLaunchKernelGGL(k_dt_table, dim3(jobs * P.tparts), dim3(DT_NT), 0, s, P);
     const size_t lds = (size_t)(P.lds_lambda + 1) * sizeof(int32_t);
+    P.units = total_chunks * P.split;
     if (kde) {
-        P.units = total_chunks * DT_KDE_SPLIT;
+        P.units = total_chunks * std::max(DT_KDE_SPLIT, P.split);
         hipLaunchKernelGGL(k_dt_links<true>, dim3(xcd_grid(P.units)), dim3(DT_NT), lds, s, P);
     } else {
         hipLaunchKernelGGL(k_dt_links<false>, dim3(xcd_grid(P.units)), dim3(DT_NT), lds, s, P);
@@ -577,7 +584,7 @@ static size_t carve_sizes(size_t n, size_t *sz) {
     sz[k++] = n;                                             // seeds
     sz[k++] = sizeof(DtStats);                               // stats
     sz[k++] = sizeof(DtStats);                               // frozen model
-    sz[k++] = (n + DT_CHUNK - 1) / DT_CHUNK * sizeof(unsigned long long);
+    sz[k++] = (n + DT_CHUNK - 1) / DT_CHUNK * DT_SPLIT_MAX * sizeof(unsigned long long);
     sz[k++] = DT_HIST * sizeof(int32_t);
     sz[k++] = DT_LUT * sizeof(uint16_t);
     sz[k++] = (DT_MAX_LAMBDA + 1) * sizeof(int32_t);
@@ -651,7 +658,7 @@ int DtContext::init(int w, int h, hipStream_t stream, bool own_outputs, size_t c
     rc |= dalloc((void **)&seeds_, n);
     rc |= dalloc((void **)&d_stats_, sizeof(DtStats));
     rc |= dalloc((void **)&d_frozen_, sizeof(DtStats));
-    rc |= dalloc((void **)&d_beta_part_, (n + DT_CHUNK - 1) / DT_CHUNK * sizeof(unsigned long long));
+    rc |= dalloc((void **)&d_beta_part_, (n + DT_CHUNK - 1) / DT_CHUNK * DT_SPLIT_MAX * sizeof(unsigned long long));
     rc |= dalloc((void **)&d_hist_acc_, DT_HIST * sizeof(int32_t));
     rc |= dalloc((void **)&d_tlut_, DT_LUT * sizeof(uint16_t));
     rc |= dalloc((void **)&d_thr_, (DT_MAX_LAMBDA + 1) * sizeof(int32_t));
@@ -844,7 +851,13 @@ int DtContext::compute(int32_t *pos, int32_t *neg, int32_t *up, int32_t *down, i
     DtLaunch L{};
     L.one = job(outs);
     L.lds_lambda = lambda;
-    dt_launch(L, dt_chunks(W, H), stream_);
+    // One image: a 1024^2 image has 256 chunks, one workgroup per CU; split
+    // its chunks' rounds over more workgroups (up to four) and its table over
+    // 32 so the passes have the wavefronts in flight that a batch has.
+    const int nch = dt_chunks(W, H);
+    L.split = nch <= 512 ? 4 : nch <= 2048 ? 2 : 1;
+    L.tparts = 32;
+    dt_launch(L, nch, stream_);
     GCFG_HIP(hipGetLastError());
     computed = true;
     return GCFG_OK;

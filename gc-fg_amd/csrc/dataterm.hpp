@@ -94,8 +94,14 @@ struct DtLaunch {
     int32_t njobs;
     int32_t lds_lambda;  // largest lambda of the launch (dynamic LDS of the links pass)
     int32_t units;       // work units of the launching pass (set by dt_launch)
-    int32_t pad_;
+    // Workgroups per chunk in the stats and links passes (1, 2 or 4, each a
+    // share of the chunk's rounds; single images only, whose beta partials
+    // hold DT_SPLIT_MAX slots per chunk) and table workgroups per image
+    // (8, 16 or 32); 0: 1 and 8.
+    int32_t split;
+    int32_t tparts;
 };
+constexpr int DT_SPLIT_MAX = 4;
 
 __host__ __device__ inline int dt_chunks(int W, int H) { return (int)(((int64_t)W * H + DT_CHUNK - 1) / DT_CHUNK); }
 // Stats, table and links passes over every chunk of the launch.
