@@ -423,14 +423,27 @@ def host_cores():
 
 
 def cpu_model():
+    """The host's model-name string with its family/model/stepping numbers
+    (a virtual machine may expose only a short name such as "AMD EPYC"; the
+    numbers still identify the generation)."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" not in line:
+                    if fields:
+                        break  # the first processor's block is enough
+                    continue
+                k, v = (t.strip() for t in line.split(":", 1))
+                if k in ("model name", "cpu family", "model", "stepping", "vendor_id") and k not in fields:
+                    fields[k] = v
     except OSError:
-        pass
-    return None
+        return None
+    name = fields.get("model name")
+    if not name:
+        return None
+    ids = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields]
+    return f"{name} ({', '.join(ids)})" if ids else name
 
 
 def cpu_baseline(insts, flows_gpu, labels_gpu_fn, budget_s):
