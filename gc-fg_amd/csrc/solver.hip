@@ -319,8 +319,8 @@ __device__ __forceinline__ int emit_tiles(const KParams &p, const InstDesc &D, i
     // (push items of a lazy instance: whether the tile's capacities are
     // materialised yet).
     const int32_t *f0 = kind == EM_DIRTY ? p.dirty : kind == EM_COLOUR ? p.act_flag : p.live;
-    const int32_t *f1 = kind == EM_LIVE    ? p.stale
-                        : kind == EM_DIRTY ? p.live
+    const int32_t *f1 = kind == EM_LIVE                ? p.stale
+                        : (kind == EM_DIRTY && colour) ? p.live  // reachability rounds skip live-3 blocks
                         : (kind == EM_COLOUR && D.lazy) ? p.mat
                                                         : nullptr;
     __syncthreads();  // earlier readers of s_total are done
@@ -548,6 +548,7 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
         start_global_relabel<SNT>(p, D, st, parity, reinit, cap, true);
         return;
     }
+    bool converged = false;  // a distance round has just converged (PH_GR -> PH_PR in this step)
     if (phase == PH_GR) {
         const int n = emit_tiles<SNT>(p, D, EM_DIRTY, cur_cap == CAP_FLOOD ? 1 : 0, false, OP_GR_RELAX, parity);
         if (n > 0) {
@@ -557,28 +558,16 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
             }
             return;
         }
-        // The BFS has converged: finish, or push from the active tiles.
+        // The BFS has converged: finish, or push from the active tiles. After
+        // a distance round the first push colour is emitted straight away
+        // (below); finding no active tile there is the finishing test.
         const bool flood = cur_cap == CAP_FLOOD;
-        if (!any_active_tile<SNT>(p, D)) {
-            if (cur_cap < DINF && !flood) {
-                // Nothing active below the cap: check reachability before
-                // finishing (a flood settles in far fewer tile visits than
-                // exact distances; it also writes the labels).
-                start_global_relabel<SNT>(p, D, st, parity, reinit, CAP_FLOOD, false, false, false, cur_cap);
-                return;
-            }
-            if (cur_partial && !flood) {
-                // A partial round's labels outside its blocks are valid but
-                // may be stale: a full exact round writes the labels (it finds
-                // nothing active either, every excess node being at DINF).
-                start_global_relabel<SNT>(p, D, st, parity, reinit, DINF);
-                return;
-            }
+        if (flood && !any_active_tile<SNT>(p, D)) {
             if (threadIdx.x == 0) st.phase = PH_FIN;
-            // Wavefront kernels: the exact or reachability round wrote the
-            // labels of every block it visited; FIN labels the blocks it
-            // skipped (unreachable, or reaching a sink everywhere).
-            const int nf = emit_tiles<SNT>(p, D, p.variant ? EM_DEAD : EM_ALL, 0, false, OP_FIN, parity);
+            // The reachability round wrote the labels of every block it
+            // visited; FIN labels the blocks it skipped (unreachable, or
+            // reaching a sink everywhere).
+            const int nf = emit_tiles<SNT>(p, D, EM_DEAD, 0, false, OP_FIN, parity);
             if (threadIdx.x == 0) st.fin_items += nf;
             return;
         }
@@ -591,6 +580,7 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
         }
         if (threadIdx.x == 0) st.phase = PH_PR;
         pr_iter = 0;
+        converged = true;
     }
     // PH_PR: cycle through the tile colours, then relabel globally again.
     const int ncol = p.nbhd == 8 ? 4 : 2;
@@ -602,6 +592,29 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     int n = 0;
     for (int c = 0; c < ncol && n == 0; ++c)
         n = emit_tiles<SNT>(p, D, EM_COLOUR, (colour + c) & (ncol - 1), true, 0, parity);
+    if (n == 0 && converged) {
+        // A distance round converged with no active tile.
+        if (cur_cap < DINF) {
+            // Nothing active below the cap: check reachability before
+            // finishing (a flood settles in far fewer tile visits than exact
+            // distances; it also writes the labels).
+            start_global_relabel<SNT>(p, D, st, parity, reinit, CAP_FLOOD, false, false, false, cur_cap);
+            return;
+        }
+        if (cur_partial) {
+            // A partial round's labels outside its blocks are valid but may
+            // be stale: a full exact round writes the labels (it finds
+            // nothing active either, every excess node being at DINF).
+            start_global_relabel<SNT>(p, D, st, parity, reinit, DINF);
+            return;
+        }
+        if (threadIdx.x == 0) st.phase = PH_FIN;
+        // Wavefront kernels: the exact round wrote the labels of every block
+        // it visited; FIN labels the blocks it skipped (unreachable).
+        const int nf = emit_tiles<SNT>(p, D, p.variant ? EM_DEAD : EM_ALL, 0, false, OP_FIN, parity);
+        if (threadIdx.x == 0) st.fin_items += nf;
+        return;
+    }
     if (n == 0) {
         start_global_relabel<SNT>(p, D, st, parity, reinit, cap, false, part, grow);
         return;
