@@ -246,3 +246,21 @@ def test_compat_driver_neighborhood8(g, ora, tmp_path):
     assert label.sum() > 1000
     f4, _ = ora.maxflow(W, H, pos, neg, penalty=pen)
     assert f4 != flow  # the diagonal arcs change the cut
+
+
+@pytest.mark.parametrize("cfg", [dict(), dict(gr_cap=2, gr_cap_late=31), dict(inner_iters=1)])
+def test_heavy_tailed8_partial_rounds(g, ora, cfg):
+    """An 8-connected instance that needs more than the four capped rounds:
+    growing capped rounds, partial exact rounds (whose blocks queue themselves
+    for a sweep that reads their neighbours' distances) and a full exact round
+    at the end give the exact flow and labels."""
+    W, H = 480, 360
+    pos, neg = g.gen_stress_tlinks(W, H, 17)
+    rng = np.random.default_rng(17)
+    dirs = tuple(rng.integers(0, 40, (H, W)).astype(np.int32) for _ in range(8))
+    o = g.default_options()
+    for k, v in cfg.items():
+        setattr(o, k, v)
+    st = check8(g, ora, W, H, pos, neg, dirs, options=o)
+    if not cfg:
+        assert st.gr_rounds > 5, st.gr_rounds
