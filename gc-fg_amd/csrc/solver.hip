@@ -457,11 +457,12 @@ __device__ __forceinline__ bool any_active_tile(const KParams &p, const InstDesc
 // the solve, so a capped round that leaves no active node is followed by an
 // exact one.
 template <int SNT>
-__device__ __forceinline__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st, int parity,
-                                                     bool reinit, int cap, bool all = false, bool partial = false,
-                                                     bool grow = false, int wcap = 0) {
+__device__ __forceinline__ void start_global_relabel(const KParams &p, const InstDesc &D, InstState &st,
+                                                     const InstState &S0, int parity, bool reinit, int cap,
+                                                     bool all = false, bool partial = false, bool grow = false,
+                                                     int wcap = 0) {
     if (threadIdx.x == 0) {
-        st.gr_rounds++;
+        st.gr_rounds = S0.gr_rounds + 1;
         st.phase = PH_GR;
         st.cap = cap;
         st.partial = partial;
@@ -480,9 +481,9 @@ __device__ __forceinline__ void start_global_relabel(const KParams &p, const Ins
                           : emit_tiles<SNT>(p, D, EM_LIVE, cap == CAP_FLOOD ? 2 : partial ? 3 : ((whole || grow) ? 1 : 0),
                                        false, OP_GR_REINIT, parity);
     if (threadIdx.x == 0) {
-        st.gr_tiles += n;
-        st.gr_init_tiles += n;
-        if (whole) st.gr_exact_tiles += n;
+        st.gr_tiles = S0.gr_tiles + n;
+        st.gr_init_tiles = S0.gr_init_tiles + n;
+        if (whole) st.gr_exact_tiles = S0.gr_exact_tiles + n;
     }
 }
 
@@ -491,15 +492,19 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     const int s = blockIdx.x;
     if (s == 0 && threadIdx.x == 0) p.ctl->q[parity ^ 1] = QCtl{0, 0, 0, 0};
     InstState &st = p.state[s];
-    // The descriptor is loaded with the state, not after the phase test, so
-    // the two loads share one round trip.
+    // The descriptor and the whole state are loaded up front, in one round
+    // trip. This workgroup is the only writer of the state, so its counters
+    // are written as loaded value + change, with no read-modify-write (a
+    // dependent round trip that every wave would wait for at the next
+    // barrier).
     const InstDesc D = p.desc[s];
-    const int phase = st.phase;
+    const InstState S0 = st;
+    const int phase = S0.phase;
     if (phase == PH_IDLE || phase == PH_DONE) return;
-    const int err = st.err, sweeps = st.sweeps, cur_cap = st.cap, rounds0 = st.gr_rounds, cur_partial = st.partial;
-    int pr_iter = st.pr_iter;
+    const int err = S0.err, sweeps = S0.sweeps, cur_cap = S0.cap, rounds0 = S0.gr_rounds, cur_partial = S0.partial;
+    int pr_iter = S0.pr_iter;
     __syncthreads();  // every thread has read the state before thread 0 writes it
-    if (threadIdx.x == 0) st.steps++;
+    if (threadIdx.x == 0) st.steps = S0.steps + 1;
     if ((phase == PH_START && err) || phase == PH_FIN) {
         // flow = sum of pos - sum of max(excess, 0) over the interior, from
         // the per-tile sums of the build and the FIN sweep.
@@ -512,7 +517,7 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
             }
         const int64_t tot = block_sum64<SNT>(v, red);
         if (threadIdx.x == 0) {
-            if (phase == PH_FIN) st.flow = st.sum_pos + tot;
+            if (phase == PH_FIN) st.flow = S0.sum_pos + tot;
             st.phase = PH_DONE;
             atomicSub(&p.ctl->n_live, 1);
         }
@@ -545,7 +550,7 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     if (phase == PH_START) {
         // A warm start relabels every tile: an edit may have revived tiles
         // the kept flags call unreachable (edits keep the masks current).
-        start_global_relabel<SNT>(p, D, st, parity, reinit, cap, true);
+        start_global_relabel<SNT>(p, D, st, S0, parity, reinit, cap, true);
         return;
     }
     bool converged = false;  // a distance round has just converged (PH_GR -> PH_PR in this step)
@@ -553,8 +558,8 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
         const int n = emit_tiles<SNT>(p, D, EM_DIRTY, cur_cap == CAP_FLOOD ? 1 : 0, false, OP_GR_RELAX, parity);
         if (n > 0) {
             if (threadIdx.x == 0) {
-                st.gr_tiles += n;
-                if (cur_cap >= DINF || cur_cap == CAP_FLOOD) st.gr_exact_tiles += n;
+                st.gr_tiles = S0.gr_tiles + n;
+                if (cur_cap >= DINF || cur_cap == CAP_FLOOD) st.gr_exact_tiles = S0.gr_exact_tiles + n;
             }
             return;
         }
@@ -568,14 +573,14 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
             // visited; FIN labels the blocks it skipped (unreachable, or
             // reaching a sink everywhere).
             const int nf = emit_tiles<SNT>(p, D, EM_DEAD, 0, false, OP_FIN, parity);
-            if (threadIdx.x == 0) st.fin_items += nf;
+            if (threadIdx.x == 0) st.fin_items = S0.fin_items + nf;
             return;
         }
         if (flood) {
             // An excess node can still reach a sink. Flood values are not
             // valid heights: relabel exactly before pushing (a capped round
             // could freeze the same nodes again).
-            start_global_relabel<SNT>(p, D, st, parity, reinit, DINF);
+            start_global_relabel<SNT>(p, D, st, S0, parity, reinit, DINF);
             return;
         }
         if (threadIdx.x == 0) st.phase = PH_PR;
@@ -585,7 +590,7 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     // PH_PR: cycle through the tile colours, then relabel globally again.
     const int ncol = p.nbhd == 8 ? 4 : 2;
     if (pr_iter >= ncol * sweeps) {
-        start_global_relabel<SNT>(p, D, st, parity, reinit, cap, false, part, grow);
+        start_global_relabel<SNT>(p, D, st, S0, parity, reinit, cap, false, part, grow);
         return;
     }
     const int colour = pr_iter & (ncol - 1);
@@ -598,30 +603,30 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
             // Nothing active below the cap: check reachability before
             // finishing (a flood settles in far fewer tile visits than exact
             // distances; it also writes the labels).
-            start_global_relabel<SNT>(p, D, st, parity, reinit, CAP_FLOOD, false, false, false, cur_cap);
+            start_global_relabel<SNT>(p, D, st, S0, parity, reinit, CAP_FLOOD, false, false, false, cur_cap);
             return;
         }
         if (cur_partial) {
             // A partial round's labels outside its blocks are valid but may
             // be stale: a full exact round writes the labels (it finds
             // nothing active either, every excess node being at DINF).
-            start_global_relabel<SNT>(p, D, st, parity, reinit, DINF);
+            start_global_relabel<SNT>(p, D, st, S0, parity, reinit, DINF);
             return;
         }
         if (threadIdx.x == 0) st.phase = PH_FIN;
         // Wavefront kernels: the exact round wrote the labels of every block
         // it visited; FIN labels the blocks it skipped (unreachable).
         const int nf = emit_tiles<SNT>(p, D, p.variant ? EM_DEAD : EM_ALL, 0, false, OP_FIN, parity);
-        if (threadIdx.x == 0) st.fin_items += nf;
+        if (threadIdx.x == 0) st.fin_items = S0.fin_items + nf;
         return;
     }
     if (n == 0) {
-        start_global_relabel<SNT>(p, D, st, parity, reinit, cap, false, part, grow);
+        start_global_relabel<SNT>(p, D, st, S0, parity, reinit, cap, false, part, grow);
         return;
     }
     if (threadIdx.x == 0) {
         st.pr_iter = pr_iter + 1;
-        st.pr_tiles += n;
+        st.pr_tiles = S0.pr_tiles + n;
     }
     // The last colour pass before an exact round: that round's REINIT sweep
     // is queued now and runs right after this push pass, in the same step
@@ -634,7 +639,7 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     // instance's cap, which starting the round sets, so it keeps the cap of
     // the round it follows.
     if (reinit && cur_cap >= DINF && cap >= DINF && !part && pr_iter + 1 >= ncol * sweeps)
-        start_global_relabel<SNT>(p, D, st, parity, reinit, cap);
+        start_global_relabel<SNT>(p, D, st, S0, parity, reinit, cap);
 }
 
 // ---------------------------------------------------------------------------
