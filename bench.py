@@ -277,7 +277,7 @@ def serve_point(args, opts, pool, reqs, last, expected, qps):
     gaps = rng.exponential(1000.0 / qps, n)
     arrival = np.cumsum(gaps) - gaps[0]
     srv = gcfg.Server(args.serve_budget_gb << 30, opts, label_format=gcfg.LABELS_BITS)
-    sess_of, rid_of, lat, labels = {}, {}, {}, {}
+    sess_of, rid_of, lat = {}, {}, {}
     matches = True
     t0 = gcfg.now_ms()
     i = done = 0
@@ -285,14 +285,16 @@ def serve_point(args, opts, pool, reqs, last, expected, qps):
         now = gcfg.now_ms() - t0
         while i < n and arrival[i] <= now:
             q = reqs[i]
+            # Labels are lent by the server (pinned memory, valid until the
+            # next poll) instead of copied into a fresh caller array.
             if q["kind"] == "image":
                 W, H, img, seeds = pool[q["pool"]]
-                rid, sess, lab = srv.submit_image(img, seeds)
+                rid, sess, _ = srv.submit_image(img, seeds, borrow=True)
                 sess_of[i] = sess
             else:
-                rid, lab = srv.submit_edit(sess_of[q["target"]], q["strokes"], 1 << 16, shape=(q["H"], q["W"]))
+                rid, _ = srv.submit_edit(sess_of[q["target"]], q["strokes"], 1 << 16, shape=(q["H"], q["W"]),
+                                         borrow=True)
             rid_of[rid] = i
-            labels[i] = lab
             i += 1
         for r in srv.poll(wait=False):
             k = rid_of[r["request_id"]]
@@ -300,10 +302,9 @@ def serve_point(args, opts, pool, reqs, last, expected, qps):
                 raise RuntimeError(f"serving request {k} failed with status {r['status']}")
             lat[k] = r["done_ms"] - (t0 + arrival[k])
             ef, ew = expected[k]
-            if r["flow"] != ef or not np.array_equal(labels[k], ew):
+            if r["flow"] != ef or not np.array_equal(r["labels"], ew):
                 matches = False
                 log(f"serve: request {k} differs from its cold solve (flow {r['flow']} vs {ef})")
-            labels.pop(k)
             done += 1
             sess = k if reqs[k]["kind"] == "image" else reqs[k]["target"]
             if last[sess] == k:

@@ -495,6 +495,18 @@ void dt_launch(const DtLaunch &L, int total_chunks, hipStream_t s) {
     }
 }
 
+__global__ void k_dt_freeze(const DtJob *jobs, int n) {
+    const DtJob &J = jobs[blockIdx.x];
+    if (!J.freeze_to) return;
+    const int32_t *src = reinterpret_cast<const int32_t *>(J.stats);
+    int32_t *dst = reinterpret_cast<int32_t *>(J.freeze_to);
+    for (int i = threadIdx.x; i < (int)(sizeof(DtStats) / sizeof(int32_t)); i += blockDim.x) dst[i] = src[i];
+}
+
+void dt_freeze(const DtJob *d_jobs, int njobs, hipStream_t s) {
+    if (njobs > 0) hipLaunchKernelGGL(k_dt_freeze, dim3(njobs), dim3(64), 0, s, d_jobs, njobs);
+}
+
 __device__ __forceinline__ int sqdiff(const uint8_t *a, const uint8_t *b) {
     const int d0 = (int)a[0] - b[0], d1 = (int)a[1] - b[1], d2 = (int)a[2] - b[2];
     return d0 * d0 + d1 * d1 + d2 * d2;
@@ -861,6 +873,13 @@ int DtContext::compute(int32_t *pos, int32_t *neg, int32_t *up, int32_t *down, i
     GCFG_HIP(hipGetLastError());
     computed = true;
     return GCFG_OK;
+}
+
+DtJob DtContext::batch_job() const {
+    int32_t *outs[6] = {pos_, neg_, up_, down_, left_, right_};
+    DtJob j = job(outs);
+    j.freeze_to = d_frozen_;
+    return j;
 }
 
 int DtContext::freeze_model() {

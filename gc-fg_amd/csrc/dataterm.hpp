@@ -75,6 +75,7 @@ struct DtJob {
     int32_t kde_n;           // seed colours in kde
     int32_t aligned;         // 1: every output plane is 16-byte aligned (vector stores)
     int32_t pad_;
+    DtStats *freeze_to;      // dt_freeze copies the statistics here (a session's kept colour model), or null
     const uint32_t *kde;     // kernel-density seed colours (0x00RRGGBB order c0, c1, c2)
     const float *gauss;      // kernel table (KDE_TAPS)
     unsigned long long *beta_part;  // one partial sum per chunk (stats pass)
@@ -106,6 +107,9 @@ constexpr int DT_SPLIT_MAX = 4;
 __host__ __device__ inline int dt_chunks(int W, int H) { return (int)(((int64_t)W * H + DT_CHUNK - 1) / DT_CHUNK); }
 // Stats, table and links passes over every chunk of the launch.
 void dt_launch(const DtLaunch &L, int total_chunks, hipStream_t s);
+// After a batched launch: each job's statistics copied to its freeze_to
+// (one launch for the table of njobs device jobs).
+void dt_freeze(const DtJob *d_jobs, int njobs, hipStream_t s);
 
 class DtContext {
    public:
@@ -141,6 +145,15 @@ class DtContext {
     // the kernel-density seed colours are kept anyway) for later images, as the
     // reference keeps its density across frames (ComplexApplication.cu:202-207).
     int freeze_model();
+    // A compute into the owned planes as one job of a batched launch over
+    // several contexts (histogram model): the job, whose statistics dt_freeze
+    // keeps as the colour model (freeze_to), and, once the caller has queued
+    // both on this context's stream, batch_queued() to record it.
+    DtJob batch_job() const;
+    void batch_queued() {
+        computed = true;
+        frozen = true;
+    }
     // Compute into the second set of output planes (allocated on first use)
     // and swap it with the first: the previous outputs stay readable through
     // prev_*() until the next swap.

@@ -289,6 +289,10 @@ class Engine {
     int sync();
 
     hipStream_t stream() const { return stream_; }
+    // Host bytes copied to the device in stream order through the engine's
+    // pinned ring (no wait for queued kernels; a wrap of the ring waits for
+    // the stream).
+    int stage(void *d_dst, const void *h_src, size_t bytes);
     const InstDesc &desc(int slot) const { return h_desc_[slot]; }
     size_t tiles_free() const;
     int cap_tiles() const { return cap_tiles_; }
@@ -348,7 +352,6 @@ class Engine {
     // bytes. Uploads therefore never wait for queued kernels.
     uint8_t *ring_ = nullptr;
     size_t ring_cap_ = 0, ring_off_ = 0;
-    int stage(void *d_dst, const void *h_src, size_t bytes);
     // Captured step chunks, one per chunk length and starting queue parity
     // (a chunk of any length continues the parity sequence, so a solve
     // replays exactly the steps it needs); dropped when the options change.

@@ -34,6 +34,11 @@
 #include "engine.hpp"
 
 using gcfg::DtContext;
+using gcfg::DtJob;
+using gcfg::DtLaunch;
+using gcfg::dt_chunks;
+using gcfg::dt_freeze;
+using gcfg::dt_launch;
 using gcfg::Engine;
 using gcfg::InstDesc;
 using gcfg::InstState;
@@ -130,7 +135,15 @@ struct Request {
     int64_t flow = 0;
     Staging stg;
     size_t label_bytes = 0;
-    hipEvent_t ev = nullptr;  // label copy done
+    struct CopyGroup *grp = nullptr;  // the label copies queued with this one (one completion event)
+};
+
+// The label copies queued from one snapshot: one wait on the snapshot, the
+// copies, one event after them (instead of three calls per request).
+struct CopyGroup {
+    hipEvent_t ev = nullptr;
+    int pending = 0;      // requests of the group not finished yet
+    int state = 0;        // 0 not known done, 1 done, -1 failed (as of the current pass)
 };
 
 struct Session {
@@ -180,11 +193,20 @@ struct gcfg_server {
     DevHeap heap;                     // session buffers
     std::vector<Buffers *> pool;      // idle session buffers (counted in used)
     std::vector<Staging> staging;     // idle pinned label staging
+    std::vector<Staging> lent;        // staging whose labels the last poll lent to the caller
     std::vector<hipEvent_t> events;   // idle events
     Snapshot snaps[SNAPSHOTS];
     int64_t snap_seq = 0;             // sequence number of the next snapshot
     int hi_slot = 0;                  // slots [0, hi_slot) may be in use
     std::deque<gcfg_result> ready;
+    // Batched data term of the images admitted in one pass: the job table
+    // (one per slot at most) and the job of every chunk, uploaded through the
+    // engine's staging ring.
+    DtJob *d_jobs = nullptr;
+    int32_t *d_cmap = nullptr;
+    size_t cmap_cap = 0;
+    std::vector<DtJob> h_jobs;
+    std::vector<int32_t> h_cmap;
 };
 
 static size_t tile_bytes_for(const Engine &eng, int W, int H) {
@@ -326,7 +348,11 @@ static int take_buffers(gcfg_server *s, int W, int H, Buffers **out) {
     return GCFG_OK;
 }
 
-static int admit_image(gcfg_server *s, Request *r) {
+// An image request's session: buffers, the image and seed map copied on
+// the upload stream, an engine slot pointing at the buffers' planes. The
+// data term and the build are queued for all images of an admission pass
+// together (admit_finish).
+static int admit_image(gcfg_server *s, Request *r, Session **out) {
     Session *ss = new (std::nothrow) Session();
     if (!ss) return GCFG_ERR_NOMEM;
     ss->id = r->session;
@@ -339,13 +365,6 @@ static int admit_image(gcfg_server *s, Request *r) {
     }
     DtContext &dt = ss->buf->dt;
     rc = dt.upload_on(s->up, r->rgb, r->seeds);
-    if (rc == GCFG_OK && (hipEventRecord(s->up_ev, s->up) != hipSuccess ||
-                          hipStreamWaitEvent(s->eng.stream(), s->up_ev, 0) != hipSuccess)) {
-        set_error("server: ordering the upload failed");
-        rc = GCFG_ERR_HIP;
-    }
-    if (rc == GCFG_OK) rc = dt.compute_owned();
-    if (rc == GCFG_OK) rc = dt.freeze_model();  // later frames of the session reuse this colour model
     if (rc == GCFG_OK) {
         const bool bits = r->format == GCFG_LABELS_BITS;
         ss->slot = s->eng.add_instance(r->W, r->H, dt.pos_, dt.neg_, dt.up_, dt.down_, dt.left_, dt.right_, 0,
@@ -353,10 +372,9 @@ static int admit_image(gcfg_server *s, Request *r) {
         if (ss->slot < 0) rc = ss->slot;
         else s->eng.set_label_bits(ss->slot, bits ? (uint64_t *)ss->buf->d_label : nullptr);
     }
-    if (rc == GCFG_OK) rc = s->eng.build({ss->slot});
     if (rc != GCFG_OK) {
         if (ss->slot >= 0) s->eng.remove_instance(ss->slot);
-        (void)s->eng.sync();  // work queued on the buffers ends before they serve another session
+        (void)hipStreamSynchronize(s->up);  // the upload ends before the buffers serve another session
         free_buffers(s, ss->buf);
         delete ss;
         return rc;
@@ -368,7 +386,54 @@ static int admit_image(gcfg_server *s, Request *r) {
     s->hi_slot = std::max(s->hi_slot, ss->slot + 1);
     ss->busy = true;
     ss->last_used = now_ms();
+    *out = ss;
     return GCFG_OK;
+}
+
+// The GPU work of the images admitted in one pass, queued once for all of
+// them: one wait for their uploads, one batched data term (stats, table and
+// links launches over every chunk of every image; the per-image launch
+// latencies were most of a small request's GPU time), one launch that keeps
+// each image's statistics as its session's colour model, one build.
+static int admit_finish(gcfg_server *s, const std::vector<Session *> &fresh) {
+    if (fresh.empty()) return GCFG_OK;
+    if (hipEventRecord(s->up_ev, s->up) != hipSuccess || hipStreamWaitEvent(s->eng.stream(), s->up_ev, 0) != hipSuccess) {
+        set_error("server: ordering the uploads failed");
+        return GCFG_ERR_HIP;
+    }
+    s->h_jobs.clear();
+    s->h_cmap.clear();
+    int lambda = 0;
+    std::vector<int> slots;
+    for (Session *ss : fresh) {
+        DtJob j = ss->buf->dt.batch_job();
+        j.chunk0 = (int32_t)s->h_cmap.size();
+        const int nc = dt_chunks(j.W, j.H);
+        s->h_cmap.insert(s->h_cmap.end(), (size_t)nc, (int32_t)s->h_jobs.size());
+        lambda = std::max(lambda, (int)j.lambda);
+        s->h_jobs.push_back(j);
+        slots.push_back(ss->slot);
+    }
+    if (s->h_cmap.size() > s->cmap_cap || s->h_jobs.size() > (size_t)s->cap_slots) {
+        set_error("server: admission pass larger than its job tables");
+        return GCFG_ERR_STATE;
+    }
+    int rc = s->eng.stage(s->d_jobs, s->h_jobs.data(), s->h_jobs.size() * sizeof(DtJob));
+    if (rc == GCFG_OK) rc = s->eng.stage(s->d_cmap, s->h_cmap.data(), s->h_cmap.size() * sizeof(int32_t));
+    if (rc != GCFG_OK) return rc;
+    DtLaunch L{};
+    L.jobs = s->d_jobs;
+    L.cmap = s->d_cmap;
+    L.njobs = (int32_t)s->h_jobs.size();
+    L.lds_lambda = lambda;
+    dt_launch(L, (int)s->h_cmap.size(), s->eng.stream());
+    dt_freeze(s->d_jobs, L.njobs, s->eng.stream());
+    if (hipGetLastError() != hipSuccess) {
+        set_error("server: data-term launch failed");
+        return GCFG_ERR_HIP;
+    }
+    for (Session *ss : fresh) ss->buf->dt.batch_queued();  // later frames reuse this colour model
+    return s->eng.build(slots);
 }
 
 // Label destination of a session's next solve in the request's format.
@@ -391,18 +456,30 @@ static hipEvent_t take_event(gcfg_server *s) {
     return e;
 }
 
+// Pinned staging in power-of-two sizes from 4 KiB: a freed buffer serves any
+// later request of its size class, so the steady state allocates none (an
+// allocation of pinned memory costs tens of microseconds on the request
+// path).
+static size_t staging_class(size_t bytes) {
+    size_t c = 4096;
+    while (c < bytes) c <<= 1;
+    return c;
+}
+
 static int take_staging(gcfg_server *s, size_t bytes, Staging *out) {
+    const size_t want = staging_class(bytes);
     int best = -1;
     for (size_t i = 0; i < s->staging.size(); ++i)
-        if (s->staging[i].bytes >= bytes && (best < 0 || s->staging[i].bytes < s->staging[best].bytes))
+        if (s->staging[i].bytes >= want && (best < 0 || s->staging[i].bytes < s->staging[best].bytes))
             best = (int)i;
     if (best >= 0) {
         *out = s->staging[best];
-        s->staging.erase(s->staging.begin() + best);
+        s->staging[best] = s->staging.back();
+        s->staging.pop_back();
         return GCFG_OK;
     }
     Staging st;
-    st.bytes = std::max<size_t>(bytes, 4096);
+    st.bytes = want;
     if (hipHostMalloc(&st.h, st.bytes, hipHostMallocDefault) != hipSuccess) {
         set_error("server: pinned staging allocation failed");
         return GCFG_ERR_NOMEM;
@@ -418,15 +495,28 @@ static void finish(gcfg_server *s, Request *r, int status) {
         it->second->last_used = now_ms();
     }
     push_result(s, r, status, status == GCFG_OK ? r->flow : 0);
-    if (r->stg.h) s->staging.push_back(r->stg);
-    if (r->ev) s->events.push_back(r->ev);
+    if (r->stg.h) {
+        if (status == GCFG_OK && !r->h_label) {
+            // Lent: the result points at the staging until the next poll.
+            s->ready.back().labels = r->stg.h;
+            s->ready.back().label_bytes = r->label_bytes;
+            s->lent.push_back(r->stg);
+        } else {
+            s->staging.push_back(r->stg);
+        }
+    }
+    if (r->grp && --r->grp->pending == 0) {
+        s->events.push_back(r->grp->ev);
+        delete r->grp;
+    }
     delete r;
 }
 
 // Requests whose instance a completed snapshot shows done: queue the label
 // copy on the download stream.
 static int process_snapshot(gcfg_server *s, const Snapshot &sn) {
-    std::vector<Request *> still;
+    std::vector<Request *> still, done;
+    std::vector<const void *> src;  // the device labels of each request in `done`
     for (Request *r : s->running) {
         auto it = s->sessions.find(r->session);
         if (it == s->sessions.end()) {
@@ -447,32 +537,40 @@ static int process_snapshot(gcfg_server *s, const Snapshot &sn) {
             continue;
         }
         r->flow = st.flow;
-        if (!r->h_label) {
-            finish(s, r, GCFG_OK);
-            continue;
-        }
         r->label_bytes = label_bytes_for(ss->W, ss->H, r->format);
-        int rc = take_staging(s, r->label_bytes, &r->stg);
-        r->ev = rc == GCFG_OK ? take_event(s) : nullptr;
-        if (rc == GCFG_OK && !r->ev) {
-            set_error("server: event creation failed");
-            rc = GCFG_ERR_HIP;
-        }
-        if (rc == GCFG_OK &&
-            (hipStreamWaitEvent(s->down, sn.ev, 0) != hipSuccess ||
-             hipMemcpyAsync(r->stg.h, ss->buf->d_label, r->label_bytes, hipMemcpyDeviceToHost, s->down) !=
-                 hipSuccess ||
-             hipEventRecord(r->ev, s->down) != hipSuccess)) {
-            set_error("server: label copy failed");
-            rc = GCFG_ERR_HIP;
-        }
+        const int rc = take_staging(s, r->label_bytes, &r->stg);
         if (rc != GCFG_OK) {
             finish(s, r, rc);
             continue;
         }
-        s->copying.push_back(r);
+        done.push_back(r);
+        src.push_back(ss->buf->d_label);
     }
     s->running.swap(still);
+    if (done.empty()) return GCFG_OK;
+    CopyGroup *g = new (std::nothrow) CopyGroup();
+    if (g) g->ev = take_event(s);
+    bool ok = g && g->ev && hipStreamWaitEvent(s->down, sn.ev, 0) == hipSuccess;
+    for (size_t i = 0; ok && i < done.size(); ++i) {
+        Request *r = done[i];
+        ok = hipMemcpyAsync(r->stg.h, src[i], r->label_bytes, hipMemcpyDeviceToHost, s->down) == hipSuccess;
+    }
+    ok = ok && hipEventRecord(g->ev, s->down) == hipSuccess;
+    if (!ok) {
+        set_error("server: label copy failed");
+        (void)hipStreamSynchronize(s->down);  // nothing queued may still write the staging
+        if (g) {
+            if (g->ev) s->events.push_back(g->ev);
+            delete g;
+        }
+        for (Request *r : done) finish(s, r, GCFG_ERR_HIP);
+        return GCFG_OK;
+    }
+    g->pending = (int)done.size();
+    for (Request *r : done) {
+        r->grp = g;
+        s->copying.push_back(r);
+    }
     return GCFG_OK;
 }
 
@@ -480,20 +578,25 @@ static int process_snapshot(gcfg_server *s, const Snapshot &sn) {
 static bool finish_copies(gcfg_server *s, bool block_first) {
     bool any = false;
     std::vector<Request *> still;
+    for (Request *r : s->copying) r->grp->state = 0;
     for (size_t i = 0; i < s->copying.size(); ++i) {
         Request *r = s->copying[i];
-        hipError_t q = (block_first && i == 0) ? hipEventSynchronize(r->ev) : hipEventQuery(r->ev);
-        if (q == hipErrorNotReady) {
+        CopyGroup *g = r->grp;
+        if (g->state == 0) {
+            const hipError_t q = (block_first && i == 0) ? hipEventSynchronize(g->ev) : hipEventQuery(g->ev);
+            g->state = q == hipSuccess ? 1 : q == hipErrorNotReady ? 2 : -1;
+        }
+        if (g->state == 2) {
             still.push_back(r);
             continue;
         }
-        if (q != hipSuccess) {
+        if (g->state < 0) {
             set_error("server: label copy failed");
             finish(s, r, GCFG_ERR_HIP);
             any = true;
             continue;
         }
-        std::memcpy(r->h_label, r->stg.h, r->label_bytes);
+        if (r->h_label) std::memcpy(r->h_label, r->stg.h, r->label_bytes);
         finish(s, r, GCFG_OK);
         any = true;
     }
@@ -544,6 +647,8 @@ static int feed(gcfg_server *s) {
 
 static int admit(gcfg_server *s) {
     std::deque<Request *> keep;
+    std::vector<Session *> fresh;  // images admitted in this pass (GPU work queued together at the end)
+    std::vector<Request *> fresh_req;
     bool blocked = false;  // an image that does not fit blocks later images (FIFO fairness)
     while (!s->queue.empty()) {
         Request *r = s->queue.front();
@@ -632,7 +737,8 @@ static int admit(gcfg_server *s) {
         }
         r->t_admit = now_ms();
         r->first_snapshot = s->snap_seq;
-        int rc = admit_image(s, r);
+        Session *ss = nullptr;
+        int rc = admit_image(s, r, &ss);
         if (rc == GCFG_ERR_BUDGET) {  // tile range fragmented: wait for frees
             blocked = true;
             r->t_admit = 0;
@@ -644,9 +750,23 @@ static int admit(gcfg_server *s) {
             delete r;
             continue;
         }
-        s->running.push_back(r);
+        fresh.push_back(ss);
+        fresh_req.push_back(r);
     }
     s->queue.swap(keep);
+    const int rc = admit_finish(s, fresh);
+    if (rc != GCFG_OK) {
+        // Nothing of the pass runs: its requests fail, and their buffers
+        // return to the pool once the queued work on them has ended.
+        (void)s->eng.sync();
+        for (size_t i = 0; i < fresh.size(); ++i) {
+            push_result(s, fresh_req[i], rc, 0);
+            delete fresh_req[i];
+            destroy_session(s, fresh[i]);
+        }
+        return rc;
+    }
+    s->running.insert(s->running.end(), fresh_req.begin(), fresh_req.end());
     return GCFG_OK;
 }
 
@@ -683,6 +803,16 @@ int gcfg_server_create(size_t hbm_budget_bytes, const gcfg_options *opt, gcfg_se
                           hipEventCreateWithFlags(&s->up_ev, hipEventDisableTiming) != hipSuccess)) {
         set_error("gcfg_server_create: stream or event creation failed");
         rc = GCFG_ERR_HIP;
+    }
+    if (rc == GCFG_OK) {
+        // An image has at most a quarter as many data-term chunks as tiles,
+        // plus one, and every admitted image fits in the arena.
+        s->cmap_cap = (size_t)tiles / 4 + (size_t)s->cap_slots;
+        if (hipMalloc((void **)&s->d_jobs, (size_t)s->cap_slots * sizeof(DtJob)) != hipSuccess ||
+            hipMalloc((void **)&s->d_cmap, s->cmap_cap * sizeof(int32_t)) != hipSuccess) {
+            set_error("gcfg_server_create: job table allocation failed");
+            rc = GCFG_ERR_NOMEM;
+        }
     }
     for (int i = 0; rc == GCFG_OK && i < SNAPSHOTS; ++i) {
         if (hipHostMalloc((void **)&s->snaps[i].h, (size_t)s->cap_slots * sizeof(InstState),
@@ -779,6 +909,9 @@ int gcfg_server_submit_frame(gcfg_server *s, int32_t session, int32_t width, int
 int gcfg_server_poll(gcfg_server *s, int32_t wait, int32_t max_results, gcfg_result *results,
                      int32_t *n_results) {
     if (!s || (max_results > 0 && !results)) return GCFG_ERR_INVALID;
+    // Labels lent by the previous poll expire now.
+    for (const Staging &st : s->lent) s->staging.push_back(st);
+    s->lent.clear();
     int rc = GCFG_OK;
     for (;;) {
         bool progress = finish_copies(s, false);
@@ -842,9 +975,13 @@ int gcfg_server_destroy(gcfg_server *s) {
     for (Request *r : s->running) delete r;
     for (Request *r : s->copying) {
         if (r->stg.h) (void)hipHostFree(r->stg.h);
-        if (r->ev) (void)hipEventDestroy(r->ev);
+        if (r->grp && --r->grp->pending == 0) {
+            (void)hipEventDestroy(r->grp->ev);
+            delete r->grp;
+        }
         delete r;
     }
+    for (Staging &st : s->lent) (void)hipHostFree(st.h);
     std::vector<Session *> all;
     for (auto &kv : s->sessions) all.push_back(kv.second);
     for (Session *ss : all) destroy_session(s, ss);
@@ -856,6 +993,8 @@ int gcfg_server_destroy(gcfg_server *s) {
         if (sn.ev) (void)hipEventDestroy(sn.ev);
     }
     if (s->up_ev) (void)hipEventDestroy(s->up_ev);
+    if (s->d_jobs) (void)hipFree(s->d_jobs);
+    if (s->d_cmap) (void)hipFree(s->d_cmap);
     if (s->up) (void)hipStreamDestroy(s->up);
     if (s->down) (void)hipStreamDestroy(s->down);
     delete s;

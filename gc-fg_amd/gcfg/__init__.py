@@ -73,7 +73,7 @@ class Instance(ctypes.Structure):
 class Result(ctypes.Structure):
     _fields_ = [("request_id", ctypes.c_int64), ("session", ctypes.c_int32), ("status", ctypes.c_int32),
                 ("flow", ctypes.c_int64), ("latency_ms", ctypes.c_double), ("solve_ms", ctypes.c_double),
-                ("done_ms", ctypes.c_double)]
+                ("done_ms", ctypes.c_double), ("labels", ctypes.c_void_p), ("label_bytes", ctypes.c_size_t)]
 
 
 _lib = None
@@ -791,6 +791,7 @@ class Server:
                                         ctypes.byref(h)), "gcfg_server_create")
         self.h = h.value
         self._pending = {}
+        self._lent = {}  # request id -> (shape, dtype) of labels the server lends (borrow=True)
         self.label_format = LABELS_INT32
         if label_format != LABELS_INT32:
             self.set_label_format(label_format)
@@ -800,39 +801,55 @@ class Server:
         _check(lib().gcfg_server_set_label_format(_VP(self.h), int(fmt)), "gcfg_server_set_label_format")
         self.label_format = fmt
 
-    def _label(self, H, W):
+    def _label_shape(self, H, W):
         if self.label_format == LABELS_BITS:
-            return np.empty((H, (W + 63) // 64), np.uint64)
-        return np.empty((H, W), np.int32)
+            return (H, (W + 63) // 64), np.uint64
+        return (H, W), np.int32
 
-    def submit_image(self, rgb: np.ndarray, seeds: np.ndarray):
+    def _label(self, H, W, borrow):
+        """The caller's label array, or None when the result lends the labels."""
+        if borrow:
+            return None
+        shape, dt = self._label_shape(H, W)
+        return np.empty(shape, dt)
+
+    def _lend(self, rid, H, W, borrow):
+        if borrow:
+            self._lent[rid] = self._label_shape(H, W)
+
+    def submit_image(self, rgb: np.ndarray, seeds: np.ndarray, borrow: bool = False):
+        """borrow=True: no label array; the result's "labels" is a view of the
+        server's pinned memory, valid until the next poll."""
         rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
         seeds = np.ascontiguousarray(seeds, dtype=np.uint8)
         H, W = seeds.shape
-        label = self._label(H, W)
+        label = self._label(H, W, borrow)
         rid, sess = ctypes.c_int64(), ctypes.c_int32()
         _check(lib().gcfg_server_submit_image(_VP(self.h), W, H, _VP(rgb.ctypes.data), _VP(seeds.ctypes.data),
-                                              _VP(label.ctypes.data), ctypes.byref(rid), ctypes.byref(sess)),
-               "gcfg_server_submit_image")
+                                              _VP(label.ctypes.data) if label is not None else None,
+                                              ctypes.byref(rid), ctypes.byref(sess)), "gcfg_server_submit_image")
         self._pending[rid.value] = (rgb, seeds, label)
+        self._lend(rid.value, H, W, borrow)
         return rid.value, sess.value, label
 
-    def submit_edit(self, session: int, strokes, delta: int = 1 << 16, shape=None):
+    def submit_edit(self, session: int, strokes, delta: int = 1 << 16, shape=None, borrow: bool = False):
         st = np.ascontiguousarray(np.asarray(strokes, dtype=np.int32).reshape(-1, 5))
         if shape is None:
             raise ValueError("shape (H, W) of the session image is required for the label buffer")
-        label = self._label(*shape)
+        label = self._label(*shape, borrow)
         rid = ctypes.c_int64()
         _check(lib().gcfg_server_submit_edit(_VP(self.h), int(session), st.shape[0], _VP(st.ctypes.data), int(delta),
-                                             _VP(label.ctypes.data), ctypes.byref(rid)), "gcfg_server_submit_edit")
+                                             _VP(label.ctypes.data) if label is not None else None,
+                                             ctypes.byref(rid)), "gcfg_server_submit_edit")
         self._pending[rid.value] = (st, label)
+        self._lend(rid.value, *shape, borrow)
         return rid.value, label
 
     def submit_frame(self, session: int, rgb: np.ndarray):
         """Next video frame of a session (same W x H): warm start from its residual graph."""
         rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
         H, W = rgb.shape[:2]
-        label = self._label(H, W)
+        label = self._label(H, W, False)
         rid = ctypes.c_int64()
         _check(lib().gcfg_server_submit_frame(_VP(self.h), int(session), W, H, _VP(rgb.ctypes.data),
                                               _VP(label.ctypes.data), ctypes.byref(rid)), "gcfg_server_submit_frame")
@@ -847,7 +864,13 @@ class Server:
         for i in range(n.value):
             r = res[i]
             self._pending.pop(r.request_id, None)
-            out.append({k: getattr(r, k) for k, _ in Result._fields_})
+            d = {k: getattr(r, k) for k, _ in Result._fields_ if k not in ("labels", "label_bytes")}
+            lent = self._lent.pop(r.request_id, None)
+            if lent is not None and r.labels:
+                shape, dt = lent
+                buf = (ctypes.c_uint8 * r.label_bytes).from_address(r.labels)
+                d["labels"] = np.frombuffer(buf, dtype=dt, count=int(np.prod(shape))).reshape(shape)
+            out.append(d)
         return out
 
     def load(self):

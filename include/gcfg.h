@@ -331,6 +331,11 @@ typedef struct gcfg_result {
     double latency_ms;   /* submit -> labels in host memory */
     double solve_ms;     /* admission -> labels in host memory */
     double done_ms;      /* completion time on the gcfg_now_ms() clock */
+    /* Requests submitted with h_label NULL: the labels in the server's pinned
+     * host memory (label_bytes bytes, the request's label format), valid until
+     * the next gcfg_server_poll on this server; NULL otherwise. */
+    const void *labels;
+    size_t label_bytes;
 } gcfg_result;
 /* Monotonic clock (ms) shared with gcfg_result.done_ms. */
 double gcfg_now_ms(void);
@@ -339,9 +344,11 @@ int gcfg_server_create(size_t hbm_budget_bytes, const gcfg_options *opt, gcfg_se
  * (default GCFG_LABELS_INT32). No counterpart in the reference. */
 int gcfg_server_set_label_format(gcfg_server *s, int32_t format);
 /* New session: image + seed map (host), data term and solve on the device.
- * h_label (host, in the server's label format) receives the labels. h_rgb,
- * h_seeds and h_label must stay valid, and the inputs unchanged, until the
- * request's result is returned by gcfg_server_poll (the image is copied to
+ * h_label (host, in the server's label format) receives the labels; with
+ * h_label NULL the result lends them instead (gcfg_result.labels: no copy
+ * into caller memory on the request path). h_rgb, h_seeds and h_label must
+ * stay valid, and the inputs unchanged, until the request's result is
+ * returned by gcfg_server_poll (the image is copied to
  * the device asynchronously; from pinned memory the copy does not hold up
  * the calling thread). */
 int gcfg_server_submit_image(gcfg_server *s, int32_t width, int32_t height, const uint8_t *h_rgb,

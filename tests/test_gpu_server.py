@@ -241,3 +241,44 @@ def test_admission_while_chunks_run(g, ora):
         assert res[rid]["flow"] == rf, (W, H)
         assert np.array_equal(g.unpack_bits(label, W, H), rl), (W, H)
     srv.close()
+
+
+@pytest.mark.parametrize("fmt", [0, 1])
+def test_lent_labels(g, ora, fmt):
+    """Requests submitted without a label buffer get their labels lent in the
+    server's pinned memory (valid until the next poll); mixed in one stream
+    with requests that bring their own buffers, and with edits of both kinds:
+    every result equals a cold solve."""
+    rng = np.random.default_rng(80 + fmt)
+    shapes = [(1024, 768), (640, 480), (300, 300), (900, 1100), (256, 600)]
+    srv = g.Server(4 << 30, label_format=fmt)
+    reqs = []  # (rid, own label array or None, shape index, strokes so far)
+    imgs = []
+    for i, (W, H) in enumerate(shapes):
+        img, seeds = _req(g, W, H, 110 + i)
+        imgs.append((W, H, img, seeds))
+        rid, sess, label = srv.submit_image(img, seeds, borrow=i % 2 == 0)
+        assert (label is None) == (i % 2 == 0)
+        reqs.append((rid, label, i, []))
+        st = _strokes(rng, W, H, 3)
+        rid, label = srv.submit_edit(sess, st, 1 << 16, shape=(H, W), borrow=i % 2 == 1)
+        reqs.append((rid, label, i, st))
+    res, lent = {}, {}
+    while len(res) < len(reqs):
+        for r in srv.poll(wait=True):
+            res[r["request_id"]] = r
+            if "labels" in r:
+                lent[r["request_id"]] = np.array(r["labels"])  # a copy before the next poll
+    for rid, label, i, strokes in reqs:
+        W, H, img, seeds = imgs[i]
+        assert res[rid]["status"] == 0, (i, res[rid])
+        dpos = np.zeros((H, W), np.int64)
+        dneg = np.zeros((H, W), np.int64)
+        _apply(dpos, dneg, strokes, 1 << 16)
+        rf, rl = _cold(g, ora, img, seeds, W, H, dpos, dneg)
+        got = lent[rid] if label is None else label
+        if fmt == 1:
+            got = g.unpack_bits(got, W, H)
+        assert res[rid]["flow"] == rf, (i, len(strokes))
+        assert np.array_equal(got, rl), (i, len(strokes), int((got != rl).sum()))
+    srv.close()
