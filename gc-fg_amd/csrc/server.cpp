@@ -630,6 +630,13 @@ static int drain_snapshots(gcfg_server *s, bool block, bool *progress) {
 
 // Launch a chunk of steps followed by a snapshot, while fewer than
 // SNAPSHOTS chunks are in flight.
+// A snapshot slot is free, so feed() can launch the next chunk.
+static bool can_pump(const gcfg_server *s) {
+    for (int i = 0; i < SNAPSHOTS; ++i)
+        if (!s->snaps[i].pending) return true;
+    return false;
+}
+
 static int feed(gcfg_server *s) {
     if (s->running.empty()) return GCFG_OK;
     for (int i = 0; i < SNAPSHOTS; ++i) {
@@ -917,10 +924,17 @@ int gcfg_server_poll(gcfg_server *s, int32_t wait, int32_t max_results, gcfg_res
         bool progress = finish_copies(s, false);
         rc = drain_snapshots(s, false, &progress);
         if (rc != GCFG_OK) break;
-        rc = admit(s);
-        if (rc != GCFG_OK) break;
-        rc = feed(s);
-        if (rc != GCFG_OK) break;
+        // Admission right before a chunk is pumped: an admitted request's
+        // work is queued on the engine stream behind the chunks in flight and
+        // starts with the next chunk either way, so requests that arrived
+        // since the last chunk are admitted together (one batched data term
+        // and build), and the queue is not rescanned on every poll.
+        if (can_pump(s)) {
+            rc = admit(s);
+            if (rc != GCFG_OK) break;
+            rc = feed(s);
+            if (rc != GCFG_OK) break;
+        }
         const bool idle = s->running.empty() && s->queue.empty() && s->copying.empty();
         if (!wait || !s->ready.empty() || idle) break;
         if (!progress) {
