@@ -38,8 +38,10 @@ struct gcfg_batch {
     int32_t *d_labels = nullptr;  // internal label arena
     std::vector<size_t> label_off;
     // Packed labels (gcfg_batch_solve_bits): device words of every instance
-    // and their pinned host copy, allocated on first use.
-    uint64_t *d_bits = nullptr, *h_bits = nullptr;
+    // and their pinned host copy (coherent, written by the scheduler kernel
+    // as each instance finishes; h_bits_dev is its device-visible address),
+    // allocated with the batch.
+    uint64_t *d_bits = nullptr, *h_bits = nullptr, *h_bits_dev = nullptr;
     std::vector<size_t> bits_off;
     size_t bits_words = 0;
     std::vector<InstState> states;
@@ -407,9 +409,10 @@ int gcfg_batch_create(int32_t count, const gcfg_instance *inst, const gcfg_optio
 }
 
 // Cold solve of every instance. Labels go to d_targets (device int32), or
-// with bits to the packed arena, whose host copy is queued before the final
-// state copy so one synchronisation covers both; otherwise to the internal
-// int32 arena.
+// with bits to the packed arena, from which the scheduler copies each
+// instance's words to the pinned host buffer the step it finishes (so the
+// copies overlap the solve of the others and none is left after it);
+// otherwise to the internal int32 arena.
 static int batch_run(gcfg_batch *b, int32_t **d_targets, bool bits, int64_t *flows, gcfg_stats *stats) {
     const double t0 = now_ms();
     const int n = (int)b->slots.size();
@@ -419,6 +422,7 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, bool bits, int64_t *flo
         b->eng.set_inputs(b->slots[i], in.d_pos, in.d_neg, in.d_up, in.d_down, in.d_left, in.d_right, in.penalty,
                           lab);
         b->eng.set_label_bits(b->slots[i], bits ? b->d_bits + b->bits_off[i] : nullptr);
+        b->eng.set_label_host(b->slots[i], bits ? b->h_bits_dev + b->bits_off[i] : nullptr);
     }
     b->eng.reset_profile();
     const int64_t steps0 = b->eng.steps_run;
@@ -426,12 +430,6 @@ static int batch_run(gcfg_batch *b, int32_t **d_targets, bool bits, int64_t *flo
     if (rc == GCFG_OK) rc = b->eng.build(b->slots);
     if (rc == GCFG_OK) rc = b->eng.run(b->eng.opt.max_steps);
     if (rc == GCFG_OK) rc = b->join.after(b->eng.stream());
-    if (rc == GCFG_OK && bits &&
-        hipMemcpyAsync(b->h_bits, b->d_bits, b->bits_words * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                       b->eng.stream()) != hipSuccess) {
-        set_error("gcfg_batch_solve_bits: label copy failed");
-        rc = GCFG_ERR_HIP;
-    }
     // Instance slots were assigned 0..n-1 in order at creation: one copy.
     // (Copying them with every completion poll instead cost a one-off stall
     // of several ms a few solves into a process.)
@@ -499,23 +497,29 @@ int gcfg_batch_solve(gcfg_batch *b, int32_t **h_labels, int64_t *flows, gcfg_sta
 
 // Device and pinned host buffers of a batch's packed labels (1/32 of the
 // int32 label arena each), allocated with the batch so that no solve pays
-// for them.
+// for them. Each instance starts on a 128-byte boundary, so the completion
+// copies are whole 16-byte lanes and full lines. The host buffer is
+// coherent: the scheduler's stores reach host memory without a cache flush.
 static int alloc_label_bits(gcfg_batch *b) {
     size_t words = 0;
     b->bits_off.clear();
     for (const gcfg_instance &in : b->inst) {
         b->bits_off.push_back(words);
-        words += Engine::label_words(in.width, in.height);
+        words += (Engine::label_words(in.width, in.height) + 15) & ~(size_t)15;
     }
-    if (hipMalloc((void **)&b->d_bits, std::max<size_t>(words, 1) * sizeof(uint64_t)) != hipSuccess ||
-        hipHostMalloc((void **)&b->h_bits, std::max<size_t>(words, 1) * sizeof(uint64_t), hipHostMallocDefault) !=
-            hipSuccess) {
+    const size_t bytes = std::max<size_t>(words, 1) * sizeof(uint64_t);
+    void *dev = nullptr;
+    if (hipMalloc((void **)&b->d_bits, bytes) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_bits, bytes, hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer(&dev, b->h_bits, 0) != hipSuccess || !dev) {
         if (b->d_bits) (void)hipFree(b->d_bits);
+        if (b->h_bits) (void)hipHostFree(b->h_bits);
         b->d_bits = nullptr;
         b->h_bits = nullptr;
         set_error("packed label allocation failed");
         return GCFG_ERR_NOMEM;
     }
+    b->h_bits_dev = static_cast<uint64_t *>(dev);
     b->bits_words = words;
     return GCFG_OK;
 }

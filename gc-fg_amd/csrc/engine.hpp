@@ -81,6 +81,11 @@ struct InstDesc {
     // pixel (64 k + x, Y); bits past W are 0. Written with, or instead of,
     // label_out.
     uint64_t *label_bits;
+    // Pinned host copy of label_bits (device-visible pointer, may be null):
+    // the scheduler copies the instance's packed words there when the
+    // instance finishes, while the other instances are still being solved.
+    // Both pointers 16-byte aligned for 16-byte copies (else 8-byte ones).
+    uint64_t *label_host;
 };
 
 struct InstState {
@@ -237,6 +242,9 @@ class Engine {
     void remove_instance(int slot);
     // Packed label destination of an instance (see InstDesc::label_bits); null: none.
     void set_label_bits(int slot, uint64_t *bits) { h_desc_[slot].label_bits = bits; }
+    // Host destination the packed labels are copied to on completion (see
+    // InstDesc::label_host); null: the caller copies them.
+    void set_label_host(int slot, uint64_t *host) { h_desc_[slot].label_host = host; }
     // Words of an instance's packed label map.
     static size_t label_words(int W, int H) { return (size_t)((W + TW - 1) / TW) * (size_t)H; }
     // diag: the four diagonal planes (up-left, up-right, down-left,

@@ -487,6 +487,33 @@ __device__ __forceinline__ void start_global_relabel(const KParams &p, const Ins
     }
 }
 
+// Packed label words of a finished instance to its pinned host buffer
+// (writes over PCIe from the workgroup: 16 bytes a lane, four loads in
+// flight before their stores).
+template <int SNT>
+__device__ __forceinline__ void copy_words_home(const InstDesc &D) {
+    const size_t nw = (size_t)D.tiles_x * (size_t)D.H;
+    const uint64_t *__restrict__ src = D.label_bits;
+    uint64_t *__restrict__ dst = D.label_host;
+    size_t done = 0;
+    if ((((uintptr_t)src | (uintptr_t)dst) & 15) == 0) {
+        const uint4 *__restrict__ s4 = reinterpret_cast<const uint4 *>(src);
+        uint4 *__restrict__ d4 = reinterpret_cast<uint4 *>(dst);
+        const size_t n4 = nw / 2;
+        size_t i = threadIdx.x;
+        for (; i + 3 * SNT < n4; i += 4 * SNT) {
+            uint4 v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = s4[i + (size_t)k * SNT];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) d4[i + (size_t)k * SNT] = v[k];
+        }
+        for (; i < n4; i += SNT) d4[i] = s4[i];
+        done = 2 * n4;
+    }
+    for (size_t i = done + threadIdx.x; i < nw; i += SNT) dst[i] = src[i];
+}
+
 template <int SNT>
 __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
     const int s = blockIdx.x;
@@ -515,6 +542,9 @@ __global__ __launch_bounds__(SNT) void k_sched(KParams p, int parity) {
                 const size_t t = (size_t)(D.tile0 + lt);
                 v += p.tsum[2 * t] - p.tsum[2 * t + 1];
             }
+        // The labels are final (the FIN sweep ran in the last step): their
+        // host copy goes out now, while other instances are still solved.
+        if (phase == PH_FIN && D.label_host && D.label_bits) copy_words_home<SNT>(D);
         const int64_t tot = block_sum64<SNT>(v, red);
         if (threadIdx.x == 0) {
             if (phase == PH_FIN) st.flow = S0.sum_pos + tot;

@@ -265,7 +265,9 @@ int gcfg_batch_solve_device(gcfg_batch *b, int32_t **d_labels, int64_t *flows, g
  * ceil(W_i/64) words h_bits[i][y * ceil(W_i/64) + k], bit x of word k being
  * the label of pixel (64 k + x, y); bits past W_i are 0. The GC_Optimize
  * result (GraphCut.cu:528-529: int32 labels copied to the host) in 1/32 of
- * the bytes: one device-to-host copy for the whole batch. */
+ * the bytes, written to the host as each instance finishes (by the device,
+ * while the other instances are still being solved), so no transfer is left
+ * after the solve. */
 int gcfg_batch_solve_bits(gcfg_batch *b, const uint64_t **h_bits, int64_t *flows, gcfg_stats *stats);
 int gcfg_batch_destroy(gcfg_batch *b);
 

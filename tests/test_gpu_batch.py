@@ -49,8 +49,13 @@ def test_mixed_batch_matches_oracle(g, ora, cfg):
     assert np.array_equal(flows_d, flows)
     for k, (rf, rl) in enumerate(refs):
         assert np.array_equal(b.device_labels(k), rl)
-    # Packed labels on the host: the same maps, bits past W zero.
+    # Packed labels on the host: the same maps, bits past W zero. Before the
+    # second solve the pinned buffer is overwritten, so every instance's
+    # words must be written again by that solve.
     for rep in range(2):
+        if rep:
+            for w in b.solve_bits(copy=False)[1]:
+                w.fill(~np.uint64(0))
         flows_b, words, _ = b.solve_bits()
         assert np.array_equal(flows_b, flows), rep
         for k, (rf, rl) in enumerate(refs):
