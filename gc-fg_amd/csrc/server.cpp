@@ -15,9 +15,13 @@
 //    host reads the previous snapshot;
 //  * an upload stream copies request images and seed maps; the engine stream
 //    waits for them with an event;
-//  * a download stream copies a finished instance's labels (int32 or packed
-//    one bit per pixel) into pinned staging; the request completes when that
-//    copy's event has passed, and the labels then go to the caller's buffer.
+//  * packed labels (one bit per pixel) reach pinned staging without a copy
+//    of their own: the request's staging is assigned when its solve starts,
+//    and the scheduler kernel writes the words there in the step the
+//    instance finishes, so the snapshot that shows it done completes it;
+//  * a download stream copies a finished instance's int32 labels into pinned
+//    staging; the request completes when that copy's event has passed.
+//  Either way the labels then go to the caller's buffer, or are lent.
 // Small uploads (descriptors, slot lists, strokes) go through the engine's
 // pinned ring. Session buffers (image, seed map, data term, labels) return to
 // a pool when a session closes and serve later sessions of at most as many
@@ -114,9 +118,12 @@ struct Buffers {
     size_t bytes() const { return block_bytes + (dt.device_bytes() - dt_bytes0); }
 };
 
-// Pinned host memory that a finished request's labels are copied into.
+// Pinned host memory that a finished request's labels are copied into
+// (coherent: the scheduler kernel writes packed labels there directly; d is
+// its device-visible address).
 struct Staging {
     void *h = nullptr;
+    void *d = nullptr;
     size_t bytes = 0;
 };
 
@@ -348,6 +355,9 @@ static int take_buffers(gcfg_server *s, int W, int H, Buffers **out) {
     return GCFG_OK;
 }
 
+static int stage_for_device(gcfg_server *s, Request *r, int W, int H);
+static void drop_staging(gcfg_server *s, Request *r);
+
 // An image request's session: buffers, the image and seed map copied on
 // the upload stream, an engine slot pointing at the buffers' planes. The
 // data term and the build are queued for all images of an admission pass
@@ -372,7 +382,10 @@ static int admit_image(gcfg_server *s, Request *r, Session **out) {
         if (ss->slot < 0) rc = ss->slot;
         else s->eng.set_label_bits(ss->slot, bits ? (uint64_t *)ss->buf->d_label : nullptr);
     }
+    if (rc == GCFG_OK) rc = stage_for_device(s, r, r->W, r->H);
+    if (rc == GCFG_OK) s->eng.set_label_host(ss->slot, (uint64_t *)r->stg.d);
     if (rc != GCFG_OK) {
+        drop_staging(s, r);
         if (ss->slot >= 0) s->eng.remove_instance(ss->slot);
         (void)hipStreamSynchronize(s->up);  // the upload ends before the buffers serve another session
         free_buffers(s, ss->buf);
@@ -436,13 +449,15 @@ static int admit_finish(gcfg_server *s, const std::vector<Session *> &fresh) {
     return s->eng.build(slots);
 }
 
-// Label destination of a session's next solve in the request's format.
-static void point_labels(gcfg_server *s, Session *ss, int32_t format) {
-    const bool bits = format == GCFG_LABELS_BITS;
+// Label destination of a session's next solve in the request's format
+// (and the staging the device writes packed labels to).
+static void point_labels(gcfg_server *s, Session *ss, const Request *r) {
+    const bool bits = r->format == GCFG_LABELS_BITS;
     const InstDesc &d = s->eng.desc(ss->slot);
     s->eng.set_inputs(ss->slot, d.pos, d.neg, d.up, d.down, d.left, d.right, d.penalty,
                       bits ? nullptr : (int32_t *)ss->buf->d_label);
     s->eng.set_label_bits(ss->slot, bits ? (uint64_t *)ss->buf->d_label : nullptr);
+    s->eng.set_label_host(ss->slot, (uint64_t *)r->stg.d);
 }
 
 static hipEvent_t take_event(gcfg_server *s) {
@@ -480,12 +495,37 @@ static int take_staging(gcfg_server *s, size_t bytes, Staging *out) {
     }
     Staging st;
     st.bytes = want;
-    if (hipHostMalloc(&st.h, st.bytes, hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc(&st.h, st.bytes, hipHostMallocCoherent) != hipSuccess) {
         set_error("server: pinned staging allocation failed");
         return GCFG_ERR_NOMEM;
     }
+    if (hipHostGetDevicePointer(&st.d, st.h, 0) != hipSuccess || !st.d) {
+        (void)hipHostFree(st.h);
+        set_error("server: pinned staging has no device address");
+        return GCFG_ERR_HIP;
+    }
     *out = st;
     return GCFG_OK;
+}
+
+// Packed labels are written to the request's staging by the device when
+// its instance finishes (Engine::set_label_host); int32 labels are copied
+// after it.
+static bool labels_by_device(const Request *r) { return r->format == GCFG_LABELS_BITS; }
+
+// Staging for a request whose labels the device writes, taken as its solve
+// starts.
+static int stage_for_device(gcfg_server *s, Request *r, int W, int H) {
+    if (!labels_by_device(r)) return GCFG_OK;
+    r->label_bytes = label_bytes_for(W, H, r->format);
+    return take_staging(s, r->label_bytes, &r->stg);
+}
+
+// A request that ends before its solve could write the staging: the
+// staging goes back to the pool.
+static void drop_staging(gcfg_server *s, Request *r) {
+    if (r->stg.h) s->staging.push_back(r->stg);
+    r->stg = Staging{};
 }
 
 static void finish(gcfg_server *s, Request *r, int status) {
@@ -521,6 +561,7 @@ static int process_snapshot(gcfg_server *s, const Snapshot &sn) {
         auto it = s->sessions.find(r->session);
         if (it == s->sessions.end()) {
             set_error("server: session vanished while solving");
+            drop_staging(s, r);
             push_result(s, r, GCFG_ERR_STATE, 0);
             delete r;
             continue;
@@ -537,6 +578,13 @@ static int process_snapshot(gcfg_server *s, const Snapshot &sn) {
             continue;
         }
         r->flow = st.flow;
+        if (labels_by_device(r)) {
+            // The scheduler wrote the packed labels to the staging in the
+            // step the instance finished, before this snapshot.
+            if (r->h_label) std::memcpy(r->h_label, r->stg.h, r->label_bytes);
+            finish(s, r, GCFG_OK);
+            continue;
+        }
         r->label_bytes = label_bytes_for(ss->W, ss->H, r->format);
         const int rc = take_staging(s, r->label_bytes, &r->stg);
         if (rc != GCFG_OK) {
@@ -710,10 +758,12 @@ static int admit(gcfg_server *s) {
                 if (rc == GCFG_OK)
                     rc = s->eng.reframe(ss->slot, dt.pos_, dt.neg_, dt.up_, dt.down_, dt.left_, dt.right_, 0);
             }
-            if (rc == GCFG_OK) point_labels(s, ss, r->format);
+            if (rc == GCFG_OK) rc = stage_for_device(s, r, ss->W, ss->H);
+            if (rc == GCFG_OK) point_labels(s, ss, r);
             if (rc == GCFG_OK) rc = s->eng.upload_desc_slot(ss->slot);
             if (rc == GCFG_OK) rc = s->eng.rearm({ss->slot});
             if (rc != GCFG_OK) {
+                drop_staging(s, r);
                 push_result(s, r, rc, 0);
                 delete r;
                 continue;
@@ -767,6 +817,7 @@ static int admit(gcfg_server *s) {
         // return to the pool once the queued work on them has ended.
         (void)s->eng.sync();
         for (size_t i = 0; i < fresh.size(); ++i) {
+            drop_staging(s, fresh_req[i]);
             push_result(s, fresh_req[i], rc, 0);
             delete fresh_req[i];
             destroy_session(s, fresh[i]);
@@ -986,7 +1037,10 @@ int gcfg_server_destroy(gcfg_server *s) {
     if (s->up) (void)hipStreamSynchronize(s->up);
     if (s->down) (void)hipStreamSynchronize(s->down);
     for (Request *r : s->queue) delete r;
-    for (Request *r : s->running) delete r;
+    for (Request *r : s->running) {
+        if (r->stg.h) (void)hipHostFree(r->stg.h);
+        delete r;
+    }
     for (Request *r : s->copying) {
         if (r->stg.h) (void)hipHostFree(r->stg.h);
         if (r->grp && --r->grp->pending == 0) {
