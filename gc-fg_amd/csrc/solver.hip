@@ -1979,26 +1979,42 @@ constexpr int GR_MAX_EXCHANGES = 64;
 // One-tile blocks fit 96 VGPRs without spilling: five wavefronts per SIMD
 // instead of four (the bigger blocks would spill, so they keep the default;
 // so would the 8-neighbourhood's one-tile blocks at five, they run four).
+// One-tile blocks: GR_PACK independent wavefronts per workgroup, each a
+// block of its own (no barrier joins them): the same wavefronts in a quarter
+// of the workgroups. An empty kernel took 3.7 us to dispatch as 12 k one-wave
+// workgroups and 1.6-1.8 us as 3 k of four waves (in a graph); in the solves
+// the 4096^2 image went from 5.29 to 5.23 ms and 1024^2 from 0.182 to
+// 0.178 ms (an empty launch still waits for its queue length from memory).
+constexpr int GR_PACK = 4;
+template <int BX, int BY>
+constexpr int grb_threads() { return BX * BY == 1 ? 64 * GR_PACK : 64 * BX * BY; }
+
 template <int BX, int BY, int NB = 4>
-__global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX * BY == 1 ? (NB == 8 ? 4 : 5) : 1))) void k_grb(
+__global__ __launch_bounds__((grb_threads<BX, BY>())) __attribute__((amdgpu_waves_per_eu(BX * BY == 1 ? (NB == 8 ? 4 : 5) : 1))) void k_grb(
     KParams p, int parity, GCFG_RO_ARGS) {
     constexpr int NW = BX * BY;
     static_assert(NB == 4 || NW == 1, "8-neighbourhood relabels run one-tile blocks");
     __shared__ int s_top[NW][TW], s_bot[NW][TW], s_side[NW][2 * TH];
     const int qlen = p.ctl->q[parity].qlen_gr;
-    if ((int)blockIdx.x >= qlen) return;
-    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // Block index and count: the workgroup's own for multi-tile blocks, the
+    // wavefront's for packed one-tile blocks.
+    const int blk = NW == 1 ? (int)blockIdx.x * GR_PACK + wave : (int)blockIdx.x;
+    const int nblk = NW == 1 ? (int)gridDim.x * GR_PACK : (int)gridDim.x;
+    if (blk >= qlen) return;
+    const int w = NW == 1 ? 0 : wave;  // the tile's position in its block
+    const bool leader = NW == 1 ? lane == 0 : threadIdx.x == 0;
     const int wi = w % BX, wj = w / BX;
     int64_t n_rounds = 0;
-    // Each workgroup takes a contiguous run of the queue: consecutive items
+    // Each block takes a contiguous run of the queue: consecutive items
     // are usually neighbouring tiles of one instance, so their queue words,
     // tile records and descriptor hit the scalar cache, and their halo rows
     // are each other's tile rows. (Dealing positions out from a counter
     // instead balanced the runs but serialised thousands of same-address
     // atomics per launch: twice the time.)
-    const int per = (qlen + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int q_end = min(qlen, ((int)blockIdx.x + 1) * per);
-    for (int q = (int)blockIdx.x * per; q < q_end; ++q) {
+    const int per = (qlen + nblk - 1) / nblk;
+    const int q_end = min(qlen, (blk + 1) * per);
+    for (int q = blk * per; q < q_end; ++q) {
         const int item = rq[q];
         const int op = item & 7, t0 = item >> 3;
         const int inst = rtiles[t0].inst;
@@ -2106,33 +2122,34 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
             if (need)
                 conv = tile_bfs<NB>(d, m, m8, hU, hD, hs, lane, capv, chg, rounds, p.gr_rounds, reinit && !flood && xr == 0);
             n_rounds += rounds;
-            if (NW == 1) {
+            if constexpr (NW == 1) {
                 capped = !conv;
                 break;
-            }
-            s_top[w][lane] = d[0];
-            s_bot[w][lane] = d[TH - 1];
-            if (lane == 0 || lane == TW - 1) {
-                const int o = lane ? TH : 0;  // column 0 at [0, 16), column 63 at [16, 32)
+            } else {
+                s_top[w][lane] = d[0];
+                s_bot[w][lane] = d[TH - 1];
+                if (lane == 0 || lane == TW - 1) {
+                    const int o = lane ? TH : 0;  // column 0 at [0, 16), column 63 at [16, 32)
 #pragma unroll
-                for (int y = 0; y < TH; ++y) s_side[w][o + y] = d[y];
-            }
-            __syncthreads();
-            int nU = hU, nD = hD, nS = hs;
-            if (inU) nU = s_bot[w - BX][lane];
-            if (inD) nD = s_top[w + BX][lane];
-            if (lane < TH && inL) nS = s_side[w - 1][TH + lane];
-            if (lane >= TH && lane < 2 * TH && inR) nS = s_side[w + 1][lane - TH];
-            const bool hch = have && __any(nU < hU || nD < hD || nS < hs);
-            hU = nU;
-            hD = nD;
-            hs = nS;
-            need = hch || !conv;
-            // Also the barrier before the next round's LDS writes.
-            if (!__syncthreads_or(need)) break;
-            if (xr + 1 >= GR_MAX_EXCHANGES) {
-                capped = true;
-                break;
+                    for (int y = 0; y < TH; ++y) s_side[w][o + y] = d[y];
+                }
+                __syncthreads();
+                int nU = hU, nD = hD, nS = hs;
+                if (inU) nU = s_bot[w - BX][lane];
+                if (inD) nD = s_top[w + BX][lane];
+                if (lane < TH && inL) nS = s_side[w - 1][TH + lane];
+                if (lane >= TH && lane < 2 * TH && inR) nS = s_side[w + 1][lane - TH];
+                const bool hch = have && __any(nU < hU || nD < hD || nS < hs);
+                hU = nU;
+                hD = nD;
+                hs = nS;
+                need = hch || !conv;
+                // Also the barrier before the next round's LDS writes.
+                if (!__syncthreads_or(need)) break;
+                if (xr + 1 >= GR_MAX_EXCHANGES) {
+                    capped = true;
+                    break;
+                }
             }
         }
         // A relaxation sweep only lowers distances, so the halo loaded at the
@@ -2211,7 +2228,7 @@ __global__ __launch_bounds__(64 * BX * BY) __attribute__((amdgpu_waves_per_eu(BX
         const bool any_fin = NW == 1 ? __any(fin) : __syncthreads_or(fin);
         const bool any_near = p.gr_cap <= 0 || (NW == 1 ? __any(nearv) : __syncthreads_or(nearv));
         const bool any_open = NW == 1 ? __any(open) : __syncthreads_or(open);
-        if (threadIdx.x == 0) {
+        if (leader) {
             if (capped || self_dirty) mark_dirty(p, t0);
             p.live[t0] = any_fin ? ((capv < DINF && !any_open) ? 3 : (any_near ? 2 : 1)) : 0;
         }
@@ -3218,10 +3235,13 @@ void launch_pr(const KParams &p, int grid, int parity, hipStream_t s, const Laun
     else
         launch_timed(k_pr, dim3(grid), dim3(NT), s, ev, p, parity);
 }
+// grid: blocks (one-tile blocks: wavefronts, GR_PACK to a workgroup).
 template <int BX, int BY, int NB = 4>
 static void launch_grb(const KParams &p, int grid, int parity, hipStream_t s, const LaunchEvents &ev) {
-    launch_timed(k_grb<BX, BY, NB>, dim3(grid), dim3(64 * BX * BY), s, ev, p, parity, (const int32_t *)p.queue_gr,
-                 (const TileInfo *)p.tiles, (const InstDesc *)p.desc, (const InstState *)p.state);
+    const int wg = BX * BY == 1 ? (grid + GR_PACK - 1) / GR_PACK : grid;
+    launch_timed(k_grb<BX, BY, NB>, dim3(wg), dim3(grb_threads<BX, BY>()), s, ev, p, parity,
+                 (const int32_t *)p.queue_gr, (const TileInfo *)p.tiles, (const InstDesc *)p.desc,
+                 (const InstState *)p.state);
 }
 
 int pr_resident_waves_per_cu(const KParams &p) {
