@@ -1609,17 +1609,19 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, Masks &m
             for (int i = 0; i < TH / 4; ++i) asm volatile("" : "+v"(m8.w[i]));
         }
         uint32_t rch = 0;
+        // 4-neighbourhood sweeps as selects: written as nested ifs they
+        // compiled to two exec-mask branches per row.
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
             const int U = y ? d[y - 1] : hU;
             if (NB == 4) {
-                if (m.get(y) & 4) {
-                    const int cand = U + 1;
-                    if (cand < d[y]) {
-                        d[y] = cand;
-                        rch |= 1u << y;
-                    }
-                }
+                const int cand = (m.get(y) & 4) ? U + 1 : DINF;
+                const bool lower = cand < d[y];
+                rch |= lower ? 1u << y : 0u;
+                d[y] = lower ? cand : d[y];
+                // Row by row: scheduled freely, every row's candidate stayed
+                // live until the change bits were gathered (43 VGPRs spilled).
+                asm volatile("" : "+v"(d[y]), "+v"(rch));
                 continue;
             }
             int cand = (m.get(y) & 4) ? U + 1 : DINF;
@@ -1643,13 +1645,13 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, Masks &m
         for (int y = TH - 1; y >= 0; --y) {
             const int Dn = y < TH - 1 ? d[y + 1] : hD;
             if (NB == 4) {
-                if (m.get(y) & 8) {
-                    const int cand = Dn + 1;
-                    if (cand < d[y]) {
-                        d[y] = cand;
-                        rch |= 1u << y;
-                    }
-                }
+                const int cand = (m.get(y) & 8) ? Dn + 1 : DINF;
+                const bool lower = cand < d[y];
+                rch |= lower ? 1u << y : 0u;
+                d[y] = lower ? cand : d[y];
+                // Row by row: scheduled freely, every row's candidate stayed
+                // live until the change bits were gathered (43 VGPRs spilled).
+                asm volatile("" : "+v"(d[y]), "+v"(rch));
                 continue;
             }
             int cand = (m.get(y) & 8) ? Dn + 1 : DINF;
