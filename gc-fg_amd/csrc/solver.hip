@@ -2363,7 +2363,26 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             int ek = e[y];
             int4 ck = c[y];
             int4 c8k = NB == 8 ? c8[y] : make_int4(0, 0, 0, 0);
-            if (ek > 0 && d[y] < capv) {
+            const bool active = ek > 0 && d[y] < capv;
+            if (NB == 4) {
+                // As selects (nested ifs compiled to exec-mask branches):
+                // residual capacities are never negative, so an admissible
+                // arc of an active node carries min(excess, capacity), which
+                // is 0 once the excess is gone or the arc is saturated.
+                const int tgt = d[y] - 1;
+                fl = (active && hl == tgt) ? min(ek, ck.x) : 0;
+                ek -= fl;
+                ck.x -= fl;
+                fr = (active && hr == tgt) ? min(ek, ck.y) : 0;
+                ek -= fr;
+                ck.y -= fr;
+                fu = (active && hu == tgt) ? min(ek, ck.z) : 0;
+                ek -= fu;
+                ck.z -= fu;
+                fd = (active && hd == tgt) ? min(ek, ck.w) : 0;
+                ek -= fd;
+                ck.w -= fd;
+            } else if (active) {
                 const int tgt = d[y] - 1;
                 if (ck.x > 0 && hl == tgt) {
                     fl = min(ek, ck.x);
@@ -2385,7 +2404,7 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
                     ek -= fd;
                     ck.w -= fd;
                 }
-                if (NB == 8) {
+                {
                     if (ek > 0 && c8k.x > 0 && hul == tgt) {
                         ful = min(ek, c8k.x);
                         ek -= ful;
@@ -2474,7 +2493,9 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             }
             const int hu = y ? d[y - 1] : hU;
             const int hd = y < TH - 1 ? d[y + 1] : hD;
-            if (!(e[y] > 0 && d[y] < capv)) return 0;
+            // Every lane computes; only active nodes take the new label (a
+            // per-lane early return compiled to an exec-mask branch).
+            const bool active = e[y] > 0 && d[y] < capv;
             int mm = DINF;
             if (c[y].x > 0) mm = min(mm, hl);
             if (c[y].y > 0) mm = min(mm, hr);
@@ -2492,8 +2513,8 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             // labelling: cap < nd <= d(v) + 1).
             if (nd >= dthr) nd = DINF;
             else if (nd > capv) nd = capv;
-            if (nd > d[y]) d[y] = nd;
-            return d[y] < capv;
+            if (active && nd > d[y]) d[y] = nd;
+            return active && d[y] < capv;
         };
         int mine = 0;
         int64_t sx = 0;  // sum of max(e, 0) over the tile (the flow identity's stuck excess)
