@@ -1525,33 +1525,42 @@ __device__ __forceinline__ void compose_after(int &A, int &B, int pA, int pB) {
     B = min(pB + B, DINF);
 }
 
+// The composition steps carry Ac = INF - A, whose identity is 0: a lane
+// without a source in the DPP pattern reads 0, so the shifted value and the
+// subtraction fuse into one DPP instruction (with A itself every step first
+// re-materialised INF as the move's old value). min(A, pA + B) becomes
+// max(Ac, pAc - B).
 template <int CTRL, int ROWS>
-__device__ __forceinline__ void dpp_compose(int &A, int &B) {
-    // Lanes without a source in this DPP pattern keep the identity (INF, 0).
-    const int pA = __builtin_amdgcn_update_dpp(DINF, A, CTRL, ROWS, 0xF, false);
+__device__ __forceinline__ void dpp_compose_c(int &Ac, int &B) {
+    const int pAc = __builtin_amdgcn_update_dpp(0, Ac, CTRL, ROWS, 0xF, false);
     const int pB = __builtin_amdgcn_update_dpp(0, B, CTRL, ROWS, 0xF, false);
-    compose_after(A, B, pA, pB);
+    Ac = max(Ac, pAc - B);
+    B = min(pB + B, DINF);
 }
 
 // From lane 0 upwards: Hillis-Steele within each 16-lane row (row_shr 1, 2,
 // 4, 8), then row_bcast:15 (row 0 -> row 1, row 2 -> row 3) and
 // row_bcast:31 (rows 0..1 -> rows 2 and 3).
 __device__ __forceinline__ void scan_from_left(int &A, int &B) {
-    dpp_compose<0x111, 0xF>(A, B);
-    dpp_compose<0x112, 0xF>(A, B);
-    dpp_compose<0x114, 0xF>(A, B);
-    dpp_compose<0x118, 0xF>(A, B);
-    dpp_compose<0x142, 0xA>(A, B);
-    dpp_compose<0x143, 0xC>(A, B);
+    int Ac = DINF - A;
+    dpp_compose_c<0x111, 0xF>(Ac, B);
+    dpp_compose_c<0x112, 0xF>(Ac, B);
+    dpp_compose_c<0x114, 0xF>(Ac, B);
+    dpp_compose_c<0x118, 0xF>(Ac, B);
+    dpp_compose_c<0x142, 0xA>(Ac, B);
+    dpp_compose_c<0x143, 0xC>(Ac, B);
+    A = DINF - Ac;
 }
 
 // From lane 63 downwards: within each row by row_shl 1, 2, 4, 8; across rows
 // from the row totals at lanes 16, 32 and 48 (scalar reads).
 __device__ __forceinline__ void scan_from_right(int &A, int &B, int lane) {
-    dpp_compose<0x101, 0xF>(A, B);
-    dpp_compose<0x102, 0xF>(A, B);
-    dpp_compose<0x104, 0xF>(A, B);
-    dpp_compose<0x108, 0xF>(A, B);
+    int Ac = DINF - A;
+    dpp_compose_c<0x101, 0xF>(Ac, B);
+    dpp_compose_c<0x102, 0xF>(Ac, B);
+    dpp_compose_c<0x104, 0xF>(Ac, B);
+    dpp_compose_c<0x108, 0xF>(Ac, B);
+    A = DINF - Ac;
     const int a1 = __builtin_amdgcn_readlane(A, 16), b1 = __builtin_amdgcn_readlane(B, 16);
     const int a2 = __builtin_amdgcn_readlane(A, 32), b2 = __builtin_amdgcn_readlane(B, 32);
     const int a3 = __builtin_amdgcn_readlane(A, 48), b3 = __builtin_amdgcn_readlane(B, 48);
@@ -1672,10 +1681,11 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, Masks &m
         uint32_t rrow = 0;  // rows the row scans changed
 #pragma unroll
         for (int y = 0; y < TH; ++y) {
-            int hl = shr1(d[y]), hr = shl1(d[y]);
+            // Row y one lane over, lanes 0 and 63 taking the side halo's row
+            // y (the DPP move's old value is kept where a lane has no source).
             const int hsl = __builtin_amdgcn_readlane(hs, y), hsr = __builtin_amdgcn_readlane(hs, TH + y);
-            if (lane == 0) hl = hsl;
-            if (lane == 63) hr = hsr;
+            const int hl = __builtin_amdgcn_update_dpp(hsl, d[y], 0x138, 0xF, 0xF, false);
+            const int hr = __builtin_amdgcn_update_dpp(hsr, d[y], 0x130, 0xF, 0xF, false);
             const int my = m.get(y);
             const bool can = ((my & 1) && hl + 1 < d[y]) || ((my & 2) && hr + 1 < d[y]);
             if (!__any(can)) continue;
@@ -1687,10 +1697,10 @@ __device__ __forceinline__ bool relax_tile_wave(int (&d)[TH], Masks &m, Masks &m
             int A2 = nd, B2 = (my & 2) ? 1 : DINF;
             scan_from_right(A2, B2, lane);
             nd = min(A2, dmin_sat(hsr + B2, DINF));
-            if (nd < d[y]) {
-                d[y] = nd;
-                rrow |= 1u << y;
-            }
+            const bool lower = nd < d[y];
+            rrow |= lower ? 1u << y : 0u;
+            d[y] = lower ? nd : d[y];
+            asm volatile("" : "+v"(d[y]), "+v"(rrow));
         }
         rch |= rrow;
         chg |= rch;
