@@ -71,18 +71,19 @@ __device__ __forceinline__ int64_t fin_rows_wg(const KParams &p, const InstDesc 
 // Residual capacities {left, right, up, down} of pixel (X, Y) as the build
 // defines them from the inputs: zero outside the interior and on arcs into the
 // border (GraphCutKernels.cu:122-146, :179-203).
+// (Loaded from a clamped in-bounds address and masked, not branched around.)
 __device__ __forceinline__ int4 input_caps(const InstDesc &D, int X, int Y) {
-    int4 c = make_int4(0, 0, 0, 0);
-    if (!(X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1)) return c;
-    const size_t i = (size_t)Y * D.W + X;
+    const bool in = (X > 0) & (Y > 0) & (X < D.W - 1) & (Y < D.H - 1);
+    const size_t i = (size_t)(in ? Y * D.W + X : 0);
+    int4 c;
     if (D.up)
         c = make_int4(ldg_in(D.left, i), ldg_in(D.right, i), ldg_in(D.up, i), ldg_in(D.down, i));
     else
         c = make_int4(D.penalty, D.penalty, D.penalty, D.penalty);
-    if (X == 1) c.x = 0;
-    if (X == D.W - 2) c.y = 0;
-    if (Y == 1) c.z = 0;
-    if (Y == D.H - 2) c.w = 0;
+    c.x = (in && X != 1) ? c.x : 0;
+    c.y = (in && X != D.W - 2) ? c.y : 0;
+    c.z = (in && Y != 1) ? c.z : 0;
+    c.w = (in && Y != D.H - 2) ? c.w : 0;
     return c;
 }
 
@@ -2177,12 +2178,13 @@ __global__ __launch_bounds__((grb_threads<BX, BY>())) __attribute__((amdgpu_wave
             impD = __any(((chg >> (TH - 1)) & 1u) && d[TH - 1] + 1 < hD);
             // Every lane compares its rows with the side halo's; lanes 0 and
             // 63 are the boundary columns.
-            bool cl = false, cr = false;
+            // (Bitwise: short-circuit && compiled to a branch per row.)
+            uint32_t cl = 0, cr = 0;
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
-                const bool c = (chg >> y) & 1u;
-                cl |= c && d[y] + 1 < __builtin_amdgcn_readlane(hs, y);
-                cr |= c && d[y] + 1 < __builtin_amdgcn_readlane(hs, TH + y);
+                const uint32_t c = (chg >> y) & 1u;
+                cl |= c & (uint32_t)(d[y] + 1 < __builtin_amdgcn_readlane(hs, y));
+                cr |= c & (uint32_t)(d[y] + 1 < __builtin_amdgcn_readlane(hs, TH + y));
             }
             impL = __any(lane == 0 && cl);
             impR = __any(lane == TW - 1 && cr);
@@ -2320,8 +2322,8 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
 #pragma unroll
             for (int y = 0; y < TH; ++y) {
                 const int l = __builtin_amdgcn_readlane(fs, y), r = __builtin_amdgcn_readlane(fs, TH + y);
-                if (lane == 0) c[y].x += l;
-                if (lane == TW - 1) c[y].y += r;
+                c[y].x += lane == 0 ? l : 0;
+                c[y].y += lane == TW - 1 ? r : 0;
             }
             c[0].z += ft;
             c[TH - 1].w += fb;
@@ -2523,7 +2525,9 @@ __global__ __launch_bounds__(64) void k_prw(KParams p, int parity, GCFG_RO_ARGS)
             // labelling: cap < nd <= d(v) + 1).
             if (nd >= dthr) nd = DINF;
             else if (nd > capv) nd = capv;
-            if (active && nd > d[y]) d[y] = nd;
+            const int raised = max(d[y], nd);
+            d[y] = active ? raised : d[y];
+            asm volatile("" : "+v"(d[y]));  // kept a select (as an if it became a branch)
             return active && d[y] < capv;
         };
         int mine = 0;
