@@ -71,10 +71,12 @@ __device__ __forceinline__ int64_t fin_rows_wg(const KParams &p, const InstDesc 
 // Residual capacities {left, right, up, down} of pixel (X, Y) as the build
 // defines them from the inputs: zero outside the interior and on arcs into the
 // border (GraphCutKernels.cu:122-146, :179-203).
-// (Loaded from a clamped in-bounds address and masked, not branched around.)
+// (Loaded from a clamped in-bounds address and masked, not branched around;
+// an in-bounds index is below 2^30, shapes being at most 2^30 pixels, and a
+// 32-bit product keeps the selection a select.)
 __device__ __forceinline__ int4 input_caps(const InstDesc &D, int X, int Y) {
     const bool in = (X > 0) & (Y > 0) & (X < D.W - 1) & (Y < D.H - 1);
-    const size_t i = (size_t)(in ? Y * D.W + X : 0);
+    const size_t i = in ? (size_t)((uint32_t)Y * (uint32_t)D.W + (uint32_t)X) : 0;
     int4 c;
     if (D.up)
         c = make_int4(ldg_in(D.left, i), ldg_in(D.right, i), ldg_in(D.up, i), ldg_in(D.down, i));
@@ -2201,7 +2203,7 @@ __global__ __launch_bounds__((grb_threads<BX, BY>())) __attribute__((amdgpu_wave
                 nearv |= d[y] < near_thr;
                 const int Y = ty * TH + y;
                 // an interior node not known to reach a sink (at or past the cap, or not at 0 in a flood)
-                open |= colin && Y > 0 && Y < D.H - 1 && d[y] >= (flood ? 1 : capv);
+                open |= (int)colin & (int)(Y > 0) & (int)(Y < D.H - 1) & (int)(d[y] >= (flood ? 1 : capv));
             }
             // An exact or reachability round writes the labels of every tile
             // it visits; the last visit of the last round leaves the final ones.
@@ -2873,8 +2875,8 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
 #pragma unroll
             for (int k = 0; k < TH / 2; ++k) {
                 const int Y = Y0 + h + k;
-                const bool in = X > 0 && Y > 0 && X < D.W - 1 && Y < D.H - 1;
-                const size_t i = in ? (size_t)Y * D.W + X : 0;
+                const bool in = (X > 0) & (Y > 0) & (X < D.W - 1) & (Y < D.H - 1);
+                const size_t i = in ? (size_t)((uint32_t)Y * (uint32_t)D.W + (uint32_t)X) : 0;
                 vin[k] = in;
                 vp[k] = ldg_in(D.pos, i);
                 vn[k] = ldg_in(D.neg, i);
@@ -2953,7 +2955,7 @@ __global__ __launch_bounds__(64) void k_buildw(KParams p, const int32_t *__restr
             act |= (m.get(y) & MB_EPOS) && d[y] < capv;
             fin |= d[y] < DINF;
             nearv |= d[y] < capv;
-            open |= colin && Y0 + y > 0 && Y0 + y < D.H - 1 && d[y] >= capv;
+            open |= (int)colin & (int)(Y0 + y > 0) & (int)(Y0 + y < D.H - 1) & (int)(d[y] >= capv);
         }
         // Uncapped, this sweep belongs to an exact relabel round: labels as
         // of this sweep (rewritten by any later sweep of the tile).
