@@ -504,12 +504,14 @@ __device__ __forceinline__ void copy_words_home(const InstDesc &D) {
         uint4 *__restrict__ d4 = reinterpret_cast<uint4 *>(dst);
         const size_t n4 = nw / 2;
         size_t i = threadIdx.x;
+        // Named values, not an array: a private array here was moved to LDS
+        // (64 bytes a thread).
         for (; i + 3 * SNT < n4; i += 4 * SNT) {
-            uint4 v[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) v[k] = s4[i + (size_t)k * SNT];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) d4[i + (size_t)k * SNT] = v[k];
+            const uint4 v0 = s4[i], v1 = s4[i + SNT], v2 = s4[i + 2 * SNT], v3 = s4[i + 3 * SNT];
+            d4[i] = v0;
+            d4[i + SNT] = v1;
+            d4[i + 2 * SNT] = v2;
+            d4[i + 3 * SNT] = v3;
         }
         for (; i < n4; i += SNT) d4[i] = s4[i];
         done = 2 * n4;
