@@ -367,8 +367,8 @@ int Engine::build(const std::vector<int> &slots) {
         list[n + i] = acc;
         acc += h_desc_[slots[i]].ntiles;
     }
-    for (int s : slots) {
-        const int rc = upload_desc_slot(s);
+    {
+        const int rc = upload_desc_slots(slots);
         if (rc != GCFG_OK) return rc;
     }
     if (dev_list_ != list) {
@@ -439,14 +439,34 @@ int Engine::upload_desc_slot(int slot) {
     return rc;
 }
 
+// Each staged copy is a host call and a small copy on the stream, so a new
+// batch of 256 instances uploaded one descriptor at a time paid 256 of them
+// before its first step; consecutive changed slots go as one block.
+int Engine::upload_desc_slots(const std::vector<int> &slots) {
+    std::vector<int> ch;
+    for (int s : slots)
+        if (s >= 0 && s < cap_slots_ && std::memcmp(&dev_desc_[s], &h_desc_[s], sizeof(InstDesc)) != 0)
+            ch.push_back(s);
+    std::sort(ch.begin(), ch.end());
+    ch.erase(std::unique(ch.begin(), ch.end()), ch.end());
+    for (size_t i = 0; i < ch.size();) {
+        size_t j = i + 1;
+        while (j < ch.size() && ch[j] == ch[j - 1] + 1) ++j;
+        const int a = ch[i], n = ch[j - 1] - a + 1;
+        const int rc = stage(kp_.desc + a, &h_desc_[a], (size_t)n * sizeof(InstDesc));
+        if (rc != GCFG_OK) return rc;
+        for (int k = a; k < a + n; ++k) dev_desc_[k] = h_desc_[k];
+        i = j;
+    }
+    return GCFG_OK;
+}
+
 // Descriptors that differ from the device's, queued after the work already
 // on the stream.
 int Engine::upload_desc() {
-    for (int s = 0; s < cap_slots_; ++s) {
-        const int rc = upload_desc_slot(s);
-        if (rc != GCFG_OK) return rc;
-    }
-    return GCFG_OK;
+    std::vector<int> all(cap_slots_);
+    for (int s = 0; s < cap_slots_; ++s) all[s] = s;
+    return upload_desc_slots(all);
 }
 
 int Engine::snapshot_states(InstState *h_dst, int n, hipEvent_t ev) {

@@ -259,6 +259,9 @@ class Engine {
     // queued after the work already on the stream.
     int upload_desc();
     int upload_desc_slot(int slot);
+    // The changed descriptors of `slots`, one staged copy per run of
+    // consecutive slots.
+    int upload_desc_slots(const std::vector<int> &slots);
     // Re-arm instances from their current residual graph (warm start).
     int rearm(const std::vector<int> &slots);
     // Run steps until every armed instance is done. polled_ctl() then holds
