@@ -342,7 +342,10 @@ __device__ __forceinline__ int32_t trunc_sat(double v) {
 // Links pass: six planes for four pixels per lane. The weight of each
 // horizontal pair is computed once (left of pixel k = right of pixel k - 1).
 // The kernel-density mode is arithmetic-bound (seeds x pixels): its chunks are
-// split over DT_KDE_SPLIT workgroups for more wavefronts in flight.
+// split over DT_KDE_SPLIT workgroups for more wavefronts in flight, whatever
+// split the other passes use (a workgroup with a single round of its chunk
+// runs the pass over the seed list for half a pair of rounds, twice the work
+// per pixel, and each workgroup builds the kernel table).
 constexpr int DT_KDE_SPLIT = 2;
 template <bool KDE>
 __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
@@ -353,7 +356,7 @@ __global__ __launch_bounds__(DT_NT) void k_dt_links(DtLaunch L) {
     __shared__ float gs[KDE ? KDE_TAPS * KDE_TAPS : 1];
     DtJob J;
     int c;
-    const int split = KDE ? max(DT_KDE_SPLIT, L.split) : L.split;
+    const int split = KDE ? DT_KDE_SPLIT : L.split;
     const int unit = xcd_unit(blockIdx.x, L.units);
     if (unit >= L.units) return;
     dt_job(L, unit / split, J, c);
@@ -488,7 +491,7 @@ void dt_launch(const DtLaunch &L, int total_chunks, hipStream_t s) {
     const size_t lds = (size_t)(P.lds_lambda + 1) * sizeof(int32_t);
     P.units = total_chunks * P.split;
     if (kde) {
-        P.units = total_chunks * std::max(DT_KDE_SPLIT, P.split);
+        P.units = total_chunks * DT_KDE_SPLIT;
         hipLaunchKernelGGL(k_dt_links<true>, dim3(xcd_grid(P.units)), dim3(DT_NT), lds, s, P);
     } else {
         hipLaunchKernelGGL(k_dt_links<false>, dim3(xcd_grid(P.units)), dim3(DT_NT), lds, s, P);
