@@ -37,15 +37,17 @@ def _cold(ora, frame, model, seeds, W, H, dpos, dneg):
     return ora.maxflow(W, H, ref["pos"] + dpos, ref["neg"] + dneg, edges=E)
 
 
-@pytest.mark.parametrize("kernels,lazy", [(1, 0), (1, 2), (0, 0)])
-def test_video_frames_warm_equal_cold(g, ora, kernels, lazy):
+@pytest.mark.parametrize("kernels,lazy,fmt", [(1, 0, 0), (1, 2, 0), (0, 0, 0), (1, 0, 1), (0, 0, 1)])
+def test_video_frames_warm_equal_cold(g, ora, kernels, lazy, fmt):
+    """fmt 1: labels packed one bit per pixel, which the scheduler writes to
+    the request's pinned staging as each frame's solve finishes."""
     W, H = 320, 240
     frames = _frames(g, W, H, 3, 5)
     seeds = g.gen_seed_map(W, H)
     o = g.default_options()
     o.kernels = kernels
     o.lazy_caps = lazy
-    srv = g.Server(1 << 30, o)
+    srv = g.Server(1 << 30, o, label_format=fmt)
     rid0, sess, lab0 = srv.submit_image(frames[0], seeds)
     ids = [(rid0, lab0)]
     for f in frames[1:3]:
@@ -62,9 +64,10 @@ def test_video_frames_warm_equal_cold(g, ora, kernels, lazy):
             dneg[30:46, 40:56] += 1 << 16
             dpos[110:126, 150:166] += 1 << 16
         rf, rl = _cold(ora, frames[k], frames[0], seeds, W, H, dpos, dneg)
+        got = g.unpack_bits(label, W, H) if fmt == 1 else label
         assert res[rid]["flow"] == rf, k
-        assert np.array_equal(label, rl), k
-        assert 2000 < label.sum() < W * H // 2
+        assert np.array_equal(got, rl), k
+        assert 2000 < got.sum() < W * H // 2
     srv.close()
 
 
