@@ -282,3 +282,57 @@ def test_lent_labels(g, ora, fmt):
         assert res[rid]["flow"] == rf, (i, len(strokes))
         assert np.array_equal(got, rl), (i, len(strokes), int((got != rl).sum()))
     srv.close()
+
+
+def test_label_formats_mixed_in_flight(g, ora):
+    """Requests of both label formats in flight on one server: int32 images,
+    then packed edits of those sessions and a packed new image, then int32
+    edits again. A session's label destination follows each request's format
+    (packed labels are written by the device into the request's staging,
+    int32 labels copied after the solve), and every result equals a cold
+    solve of the same capacities."""
+    rng = np.random.default_rng(77)
+    shapes = [(300, 200), (257, 130), (512, 384)]
+    srv = g.Server(1 << 30)
+    reqs = []  # (rid, label, format, session index, strokes so far)
+    sess, imgs = [], []
+    for i, (W, H) in enumerate(shapes):
+        img, seeds = _req(g, W, H, 90 + i)
+        imgs.append((W, H, img, seeds))
+        rid, s, label = srv.submit_image(img, seeds)
+        sess.append(s)
+        reqs.append((rid, label, 0, i, []))
+    done = [[] for _ in shapes]
+    srv.set_label_format(g.LABELS_BITS)
+    for i, (W, H) in enumerate(shapes):
+        st = _strokes(rng, W, H, 2)
+        done[i] = done[i] + st
+        rid, label = srv.submit_edit(sess[i], st, 1 << 16, shape=(H, W))
+        reqs.append((rid, label, 1, i, list(done[i])))
+    W, H = 200, 160
+    img, seeds = _req(g, W, H, 99)
+    imgs.append((W, H, img, seeds))
+    shapes.append((W, H))
+    done.append([])
+    rid, s, label = srv.submit_image(img, seeds)
+    sess.append(s)
+    reqs.append((rid, label, 1, 3, []))
+    srv.set_label_format(g.LABELS_INT32)
+    for i, (W, H) in enumerate(shapes):
+        st = _strokes(rng, W, H, 1)
+        done[i] = done[i] + st
+        rid, label = srv.submit_edit(sess[i], st, 1 << 16, shape=(H, W))
+        reqs.append((rid, label, 0, i, list(done[i])))
+    print(f"[server] {len(reqs)} requests, formats mixed", flush=True)
+    res = _drain(srv, len(reqs))
+    for rid, label, fmt, i, strokes in reqs:
+        W, H, img, seeds = imgs[i]
+        assert res[rid]["status"] == 0, (i, fmt, res[rid])
+        dpos = np.zeros((H, W), np.int64)
+        dneg = np.zeros((H, W), np.int64)
+        _apply(dpos, dneg, strokes, 1 << 16)
+        rf, rl = _cold(g, ora, img, seeds, W, H, dpos, dneg)
+        got = g.unpack_bits(label, W, H) if fmt == 1 else label
+        assert res[rid]["flow"] == rf, (i, fmt, len(strokes))
+        assert np.array_equal(got, rl), (i, fmt, len(strokes))
+    srv.close()
